@@ -1,0 +1,312 @@
+"""ctypes binding of the C ABI declared in include/impec_hip.h.
+
+The product path: every call goes to libimpec_hip.so (hand-written HIP for
+gfx950).  There is no CPU fallback; loading fails loudly when the library is
+missing and context creation fails with IMPEC_ENODEV when no GPU is visible.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libimpec_hip.so")
+
+IMPEC_OK = 0
+IMPEC_EINVAL = -1
+IMPEC_EHIP = -2
+IMPEC_ENOCONV = -3
+IMPEC_ENAN = -4
+IMPEC_ENOMEM = -5
+IMPEC_ENODEV = -6
+
+SRC_PARALLEL, SRC_DIAGONAL = 0, 1
+AVG_HARMONIC, AVG_MEANCONC = 0, 1
+SPLIT_ALTERNATING, SPLIT_X_FIRST, SPLIT_Y_FIRST, SPLIT_STRANG, SPLIT_PARALLEL = range(5)
+HYPER_MUSCL, HYPER_UPWIND = 0, 1
+PARA_EXPLICIT, PARA_CN = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+GUESS_ZERO, GUESS_PREVIOUS = 0, 1
+
+_INT_FIELDS = ["N1", "N2", "num_ghostcells", "source_type", "average_procedure", "dim_splitting",
+               "hyper_method", "diffusion", "para_method", "is_test", "alpha_absolute",
+               "max_iterations", "precision", "initial_guess", "mg_pre", "mg_post"]
+_DBL_FIELDS = ["x_lb", "x_ub", "y_lb", "y_ub", "nu", "mu_res", "mu_inj", "porosity", "cfl",
+               "inj_fluid_conc", "alpha_l", "alpha_t", "dt_diff_limiter", "d_m", "eps", "para_eps",
+               "I", "tstart", "inner_tol"]
+
+
+class Params(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in _INT_FIELDS] + [("reserved_i", C.c_int * 4)]
+                + [(n, C.c_double) for n in _DBL_FIELDS] + [("reserved_d", C.c_double * 3)])
+
+
+class Stats(C.Structure):
+    _fields_ = [("last_its", C.c_int), ("last_vcycles", C.c_int), ("last_parabolic_its", C.c_int),
+                ("total_its", C.c_int), ("calls", C.c_int), ("reserved_i", C.c_int * 3),
+                ("last_relres", C.c_double), ("last_parabolic_relres", C.c_double),
+                ("total_mass_outflux", C.c_double), ("reserved_d", C.c_double * 5)]
+
+
+# every entry point declared in include/impec_hip.h: name -> (argtypes, restype)
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+SIGNATURES = {
+    "impec_params_default": ([C.POINTER(Params)], C.c_int),
+    "impec_create": ([C.POINTER(Params), C.POINTER(C.c_void_p)], C.c_int),
+    "impec_destroy": ([_vp], C.c_int),
+    "impec_last_error": ([_vp], C.c_char_p),
+    "impec_strerror": ([C.c_int], C.c_char_p),
+    "impec_synchronize": ([_vp], C.c_int),
+    "impec_device_count": ([], C.c_int),
+    "impec_set_concentration": ([_vp, _dp], C.c_int),
+    "impec_get_concentration": ([_vp, _dp], C.c_int),
+    "impec_set_pressure": ([_vp, _dp], C.c_int),
+    "impec_get_pressure": ([_vp, _dp], C.c_int),
+    "impec_set_velocity": ([_vp, _dp], C.c_int),
+    "impec_get_velocity": ([_vp, _dp], C.c_int),
+    "impec_set_source": ([_vp, _dp], C.c_int),
+    "impec_set_permeability": ([_vp, _dp, _dp], C.c_int),
+    "impec_set_pin_kappa": ([_vp, C.c_double], C.c_int),
+    "impec_fill_boundaries": ([_vp], C.c_int),
+    "impec_pressure_solve": ([_vp, _ip, _dp], C.c_int),
+    "impec_velocities": ([_vp], C.c_int),
+    "impec_max_stable_timestep": ([_vp, _dp], C.c_int),
+    "impec_source_split": ([_vp, C.c_double], C.c_int),
+    "impec_advect_split": ([_vp, C.c_double], C.c_int),
+    "impec_advect_x": ([_vp, C.c_double], C.c_int),
+    "impec_advect_y": ([_vp, C.c_double], C.c_int),
+    "impec_dispersion_cn": ([_vp, C.c_double, _ip], C.c_int),
+    "impec_hyperbolic_solve": ([_vp, C.c_double], C.c_int),
+    "impec_step": ([_vp, C.c_double, C.c_double, _dp], C.c_int),
+    "impec_total_mass": ([_vp, _dp], C.c_int),
+    "impec_mass_error": ([_vp, C.c_double, _dp], C.c_int),
+    "impec_divergence_error": ([_vp, _dp], C.c_int),
+    "impec_velocity_boundary_error": ([_vp, _dp], C.c_int),
+    "impec_minmax_concentration": ([_vp, _dp, _dp], C.c_int),
+    "impec_get_stats": ([_vp, C.POINTER(Stats)], C.c_int),
+    "impec_pressure_operator": ([_vp, _dp], C.c_int),
+    "impec_parabolic_operator": ([_vp, C.c_double, _dp, _dp], C.c_int),
+    "impec_profile_enable": ([_vp, C.c_int], C.c_int),
+    "impec_profile_read": ([_vp, C.c_char_p, C.POINTER(C.c_longlong), _dp], C.c_int),
+    "impec_profile_reset": ([_vp], C.c_int),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libimpec_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libimpec_hip.so is not built; run `python -m impec2d_amd.build`")
+        lib = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+class ImpecError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (load().impec_strerror(code).decode(), code, msg))
+        self.code = code
+
+
+def default_params(**kw):
+    p = Params()
+    load().impec_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k) or k.startswith("reserved"):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """One device context (impec_ctx) with the reference's host layouts."""
+
+    def __init__(self, params):
+        self.lib = load()
+        self.params = params
+        h = C.c_void_p()
+        st = self.lib.impec_create(C.byref(params), C.byref(h))
+        if st != IMPEC_OK:
+            raise ImpecError(st, "impec_create failed")
+        self.h = h
+        self.N1, self.N2, self.b = params.N1, params.N2, params.num_ghostcells
+
+    def close(self):
+        if self.h:
+            self.lib.impec_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != IMPEC_OK:
+            raise ImpecError(st, self.lib.impec_last_error(self.h).decode())
+        return st
+
+    # ---- shapes
+    def scalar_shape(self):
+        return (self.N1 + 2 * self.b, self.N2 + 2 * self.b)
+
+    def velocity_shape(self):
+        return (self.N1 + 1 + 2 * self.b, self.N2 + 1 + 2 * self.b, 2)
+
+    # ---- transfers
+    def set_concentration(self, c):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        assert c.shape == self.scalar_shape()
+        self._check(self.lib.impec_set_concentration(self.h, _ptr(c)))
+
+    def get_concentration(self):
+        c = np.empty(self.scalar_shape())
+        self._check(self.lib.impec_get_concentration(self.h, _ptr(c)))
+        return c
+
+    def set_pressure(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        assert p.shape == self.scalar_shape()
+        self._check(self.lib.impec_set_pressure(self.h, _ptr(p)))
+
+    def get_pressure(self):
+        p = np.empty(self.scalar_shape())
+        self._check(self.lib.impec_get_pressure(self.h, _ptr(p)))
+        return p
+
+    def set_velocity(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.shape == self.velocity_shape()
+        self._check(self.lib.impec_set_velocity(self.h, _ptr(v)))
+
+    def get_velocity(self):
+        v = np.empty(self.velocity_shape())
+        self._check(self.lib.impec_get_velocity(self.h, _ptr(v)))
+        return v
+
+    def set_source(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        assert q.size == self.N1 * self.N2
+        self._check(self.lib.impec_set_source(self.h, _ptr(q)))
+
+    def set_permeability(self, K11=None, K22=None):
+        a = None if K11 is None else np.ascontiguousarray(K11, dtype=np.float64)
+        b = None if K22 is None else np.ascontiguousarray(K22, dtype=np.float64)
+        self._check(self.lib.impec_set_permeability(self.h, None if a is None else _ptr(a),
+                                                    None if b is None else _ptr(b)))
+
+    def set_pin_kappa(self, kappa):
+        self._check(self.lib.impec_set_pin_kappa(self.h, kappa))
+
+    # ---- stages
+    def fill_boundaries(self):
+        self._check(self.lib.impec_fill_boundaries(self.h))
+
+    def pressure_solve(self):
+        its = C.c_int(0)
+        rr = C.c_double(0)
+        self._check(self.lib.impec_pressure_solve(self.h, C.byref(its), C.byref(rr)))
+        return its.value, rr.value
+
+    def velocities(self):
+        self._check(self.lib.impec_velocities(self.h))
+
+    def max_stable_timestep(self):
+        dt = C.c_double(0)
+        self._check(self.lib.impec_max_stable_timestep(self.h, C.byref(dt)))
+        return dt.value
+
+    def source_split(self, dt):
+        self._check(self.lib.impec_source_split(self.h, dt))
+
+    def advect_split(self, dt):
+        self._check(self.lib.impec_advect_split(self.h, dt))
+
+    def advect_x(self, dt):
+        self._check(self.lib.impec_advect_x(self.h, dt))
+
+    def advect_y(self, dt):
+        self._check(self.lib.impec_advect_y(self.h, dt))
+
+    def dispersion_cn(self, dt):
+        its = C.c_int(0)
+        self._check(self.lib.impec_dispersion_cn(self.h, dt, C.byref(its)))
+        return its.value
+
+    def hyperbolic_solve(self, dt):
+        self._check(self.lib.impec_hyperbolic_solve(self.h, dt))
+
+    def step(self, t=0.0, t_next=-1.0):
+        dt = C.c_double(0)
+        self._check(self.lib.impec_step(self.h, t, t_next, C.byref(dt)))
+        return dt.value
+
+    def synchronize(self):
+        self._check(self.lib.impec_synchronize(self.h))
+
+    # ---- diagnostics
+    def _scalar(self, fn, *args):
+        out = C.c_double(0)
+        self._check(fn(self.h, *args, C.byref(out)))
+        return out.value
+
+    def total_mass(self):
+        return self._scalar(self.lib.impec_total_mass)
+
+    def mass_error(self, t):
+        return self._scalar(self.lib.impec_mass_error, C.c_double(t))
+
+    def divergence_error(self):
+        return self._scalar(self.lib.impec_divergence_error)
+
+    def velocity_boundary_error(self):
+        return self._scalar(self.lib.impec_velocity_boundary_error)
+
+    def minmax_concentration(self):
+        a, b = C.c_double(0), C.c_double(0)
+        self._check(self.lib.impec_minmax_concentration(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def stats(self):
+        s = Stats()
+        self._check(self.lib.impec_get_stats(self.h, C.byref(s)))
+        return s
+
+    def pressure_operator(self):
+        A = np.empty(9 * self.N1 * self.N2)
+        self._check(self.lib.impec_pressure_operator(self.h, _ptr(A)))
+        return A
+
+    def parabolic_operator(self, dt):
+        A = np.empty(9 * self.N1 * self.N2)
+        r = np.empty(self.N1 * self.N2)
+        self._check(self.lib.impec_parabolic_operator(self.h, dt, _ptr(A), _ptr(r)))
+        return A, r
+
+    # ---- measurement
+    def profile(self, on=True):
+        self._check(self.lib.impec_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        self._check(self.lib.impec_profile_reset(self.h))
+
+    def profile_read(self, name):
+        n = C.c_longlong(0)
+        ms = C.c_double(0)
+        self.lib.impec_profile_read(self.h, name.encode(), C.byref(n), C.byref(ms))
+        return n.value, ms.value

@@ -1,0 +1,78 @@
+// impec_device.h -- device-side helpers shared by the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include "impec_internal.h"
+
+namespace impec {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ int tid_linear() {
+    return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+}
+__device__ __forceinline__ int block_threads() { return blockDim.x * blockDim.y * blockDim.z; }
+
+enum RedOp { RED_SUM = 0, RED_MAX = 1, RED_MIN = 2 };
+
+template <int OP>
+__device__ __forceinline__ double red_combine(double a, double b) {
+    if (OP == RED_SUM) return a + b;
+    if (OP == RED_MAX) return b > a ? b : a;
+    return b < a ? b : a;
+}
+
+template <int OP>
+__device__ __forceinline__ double red_identity() {
+    if (OP == RED_SUM) return 0.0;
+    if (OP == RED_MAX) return -DBL_MAX;
+    return DBL_MAX;
+}
+
+// Deterministic block reduction of NV values: wavefront shuffle tree, then
+// wave 0 combines the per-wave results in index order.  Thread 0 returns the
+// block result in v[].  Every thread of the block must call it.
+template <int OP, int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV]) {
+    __shared__ double sh[NV][16];
+    const int t = tid_linear(), lane = t & (WAVE - 1), wv = t / WAVE;
+    const int nw = (block_threads() + WAVE - 1) / WAVE;
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+        for (int off = WAVE / 2; off > 0; off >>= 1) v[k] = red_combine<OP>(v[k], __shfl_down(v[k], off, WAVE));
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) sh[k][wv] = v[k];
+    __syncthreads();
+    if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = sh[k][0];
+            for (int w = 1; w < nw; w++) s = red_combine<OP>(s, sh[k][w]);
+            v[k] = s;
+        }
+    }
+}
+
+// mu(c) of EllipticSolver.cpp:692-695
+__device__ __forceinline__ double mu_of(double c, double mob1over4, double mu_res) {
+    double x = (1. - c + mob1over4 * c);
+    return 1. / (x * x * x * x) * mu_res;
+}
+
+// face mobility between two cells from the stored per-cell quantity w
+// (EllipticSolver.cpp:478-524): harmonic 2/(w_l + w_r), mean 1/2 (w_l + w_r)
+__device__ __forceinline__ double face_mob(double wl, double wr, int harmonic) {
+    return harmonic ? 2. / (wl + wr) : 1. / 2 * (wl + wr);
+}
+
+// vertex mobility from the four cells around it, in the reference's
+// summation order (i+1,j), (i,j), (i,j+1), (i+1,j+1) (EllipticSolver.cpp:697-754)
+__device__ __forceinline__ double vertex_mob(double w_ip1j, double w_ij, double w_ijp1, double w_ip1jp1,
+                                             int harmonic) {
+    double s = w_ip1j + w_ij + w_ijp1 + w_ip1jp1;
+    return harmonic ? 4. / s : 1. / 4 * s;
+}
+
+}  // namespace impec

@@ -1,0 +1,160 @@
+// impec_internal.h -- device state of one context and the kernel launchers
+// shared by the .hip translation units of libimpec_hip.so (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include "../../include/impec_hip.h"
+
+namespace impec {
+
+constexpr int GHOST = 2;          // num_ghostcells of every shipped config
+constexpr int MAX_LEVELS = 24;
+constexpr int RED_BLOCKS = 1024;  // max partial sums per reduction pass
+
+// Scalars kept in device memory so that solver loops need no host round trip
+// except the convergence test.
+enum Scal {
+    S_RHO = 0, S_RHO_OLD = 1, S_DQ = 2, S_RR = 3, S_RSUM = 4, S_ZSUM = 5, S_ALPHA = 6, S_BETA = 7,
+    S_OMEGA = 8, S_TS = 9, S_TT = 10, S_R0V = 11, S_XPIN = 12,
+    S_MASS = 16, S_DIV = 17, S_NOFLOW = 18,          // written together by the diagnostics pass
+    S_MAXVEL = 20, S_MINDT = 21, S_MINDT_DIFF = 22,  // CFL pass
+    S_OUTFLUX = 24, S_TMO = 25, S_CMAX = 26, S_CMIN = 27,
+    S_COUNT = 32
+};
+
+// Geometry and constants of the nu-weighted pressure operator on one level.
+// Constant factors are formed on the host with the reference's expressions
+// (MGSolver.cpp:760-804), so device code multiplies by exactly the same
+// doubles the reference computes.
+struct OpGeom {
+    int n1, n2;          // cells of the level
+    int harmonic;        // average_procedure
+    int diag;            // nu < 1: rotated stencil present
+    double nu, one_m_nu; // nu, (1 - nu)
+    double cx, cy;       // 1./(dx*dx), 1./(dy*dy)
+    double mcy;          // -1./(dy*dy)
+    double s2, cosa2, sina2, cost2, sint2, costsina, cosasint;
+    double dxi2, deta2;  // dxi*dxi, deta*deta (divisors of NE/SW/NW/SE)
+    double mcxi, ceta;   // -1./(dxi*dxi), 1./(deta*deta)
+    double cnd, mcnd;    // 1./(deta*dxi), -1./(deta*dxi)
+};
+
+struct Level {
+    int n1, n2, ld;      // cells, row pitch of ring-1 arrays
+    double dx, dy;
+    OpGeom g;
+    double *w;           // inverse-mobility (harmonic) or mobility, ring 1 (level >= 1)
+    double *x, *b, *r;   // ring-1 arrays (n1 + 2) x ld
+};
+
+struct ProfEntry {
+    const char *name;
+    long long launches;
+    double ms;
+};
+
+}  // namespace impec
+
+struct impec_ctx {
+    impec_params P;
+    int N1, N2, B;
+    int lds, ldv;              // row pitch of scalar / velocity device arrays
+    double dx, dy, dA;
+    double alpha, theta, dxi, deta;
+    double secamt2, cosa2, sina2, costsina, cosasint, cost2, sint2, cosa, sint, sina, cost;
+    double mob1over4;
+    double kappa;
+    int device;
+    hipStream_t stream;
+
+    // fields (scalar layout: (N1+2B) rows; velocity: (N1+1+2B) rows)
+    double *c, *p, *u, *v, *q;
+    double *K11, *K22;         // nullptr when K == 1
+    double *w, *w22;           // mobility quantity of the fine level (scalar layout)
+    double *t1, *t2, *t3;      // transport temporaries (scalar layout)
+
+    // pressure solve
+    int nlev;
+    impec::Level lev[impec::MAX_LEVELS];
+    double *pr, *pz, *pd, *pq; // PCG vectors on the level-0 ring-1 layout
+    double *scal;              // device scalars (Scal)
+    double *part;              // reduction partials
+    double *h_scal;            // pinned host mirror of scal
+    int symmetric;             // shift formulation usable
+    int pin_i, pin_j;
+    double bnorm, bsum;        // ||-q||_2 and sum(-q), exact host sums
+    double q_pin;              // q at the pinned cell
+    int have_prev_p;
+
+    // Crank-Nicolson
+    double *d11, *d12, *d22, *d21;   // face tensors (N1+1) x (N2+1), pitch ldv
+    double *cn_rhs, *cn_x, *cn_r, *cn_r0, *cn_p, *cn_v, *cn_s, *cn_t, *cn_y, *cn_z, *cn_dg;
+
+    // sources
+    double *tr_md;             // per-source mass change scratch
+    int nsrc;
+    int *src_idx;              // cell index i*N2+j, row-major order
+    double *src_q;
+
+    int prev_first_sweep;      // 0 X_SWEEP, 1 Y_SWEEP (HyperbolicSolver.cpp:84)
+    impec_stats stats;
+    char err[512];
+
+    int prof_on;
+    impec::ProfEntry prof[32];
+    int nprof;
+    hipEvent_t ev0, ev1;
+};
+
+namespace impec {
+
+int set_error(impec_ctx *ctx, int code, const char *fmt, ...);
+
+#define IMPEC_HIP_CHECK(ctx, call)                                                            \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return ::impec::set_error((ctx), IMPEC_EHIP, "%s:%d %s: %s", __FILE__, __LINE__,  \
+                                      #call, hipGetErrorString(e_));                          \
+    } while (0)
+
+// profiling brackets around launches of one kernel class
+void prof_begin(impec_ctx *ctx);
+void prof_end(impec_ctx *ctx, const char *name);
+
+inline size_t sidx(const impec_ctx *ctx, int i, int j) {
+    return (size_t)(i + ctx->B) * ctx->lds + (j + ctx->B);
+}
+
+// ---- field kernels (k_field.hip) ----
+int launch_fill_scalar(impec_ctx *ctx, double *a, int periodic);
+int launch_fill_velocity(impec_ctx *ctx);
+int launch_mobility(impec_ctx *ctx);
+int launch_velocity(impec_ctx *ctx);
+int launch_cfl(impec_ctx *ctx, double *dt);
+int launch_tr(impec_ctx *ctx, double dt);
+int launch_diagnostics(impec_ctx *ctx, int which);  // results in h_scal after sync
+int launch_minmax(impec_ctx *ctx, double *cmax, double *cmin);
+
+// ---- pressure (k_pressure.hip) ----
+int pressure_setup_levels(impec_ctx *ctx);
+void pressure_free_levels(impec_ctx *ctx);
+int pressure_solve(impec_ctx *ctx, int *its, double *relres);
+int pressure_operator_dump(impec_ctx *ctx, double *A9_host);
+
+// ---- transport (k_transport.hip) ----
+int advect_x(impec_ctx *ctx, const double *cin, double *cout, double dt);
+int advect_y(impec_ctx *ctx, const double *cin, double *cout, double dt);
+int advect_split(impec_ctx *ctx, double dt);
+int copy_scalar(impec_ctx *ctx, const double *src, double *dst);
+
+// ---- dispersion (k_cn.hip) ----
+int cn_solve(impec_ctx *ctx, double dt, int *its);
+int cn_operator_dump(impec_ctx *ctx, double dt, double *A9_host, double *rhs_host);
+
+// ---- reductions (k_field.hip) ----
+// sum of NV per-block partial vectors into scal[dst .. dst+NV)
+int finish_sum(impec_ctx *ctx, int nblocks, int nv, int dst);
+
+}  // namespace impec

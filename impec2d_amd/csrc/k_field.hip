@@ -1,0 +1,482 @@
+// k_field.hip -- ghost fills, mobility, Darcy face velocities, CFL reduction,
+// trapezoidal-rule well sources and diagnostics reductions (FP64).
+#include "impec_device.h"
+#include <stdio.h>
+
+namespace impec {
+
+// ---------------------------------------------------------------------------
+// ghost fills
+// ---------------------------------------------------------------------------
+
+// FillGhostCellsNeumannScalar / FillGhostCellsPeriodicScalar (Util.cpp:215-324):
+// every ghost cell of the ghosted scalar array copies the interior cell its
+// even mirror (or periodic image) maps to; corners compose both maps, which
+// is what the reference's corner loops produce.
+__global__ void k_fill_scalar(double *__restrict__ a, int N1, int N2, int B, int ld, int periodic) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x - B;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y - B;
+    if (i >= N1 + B || j >= N2 + B) return;
+    const bool gi = i < 0 || i >= N1, gj = j < 0 || j >= N2;
+    if (!gi && !gj) return;
+    int si, sj;
+    if (periodic) {
+        si = i < 0 ? i + N1 : (i >= N1 ? i - N1 : i);
+        sj = j < 0 ? j + N2 : (j >= N2 ? j - N2 : j);
+    } else {
+        si = i < 0 ? -1 - i : (i >= N1 ? 2 * N1 - 1 - i : i);
+        sj = j < 0 ? -1 - j : (j >= N2 ? 2 * N2 - 1 - j : j);
+    }
+    a[(size_t)(i + B) * ld + (j + B)] = a[(size_t)(si + B) * ld + (sj + B)];
+}
+
+// FillGhostCellsNeumannVelocity (Util.cpp:331-438).  u (x-faces) is odd in x
+// about the faces i = 0 and i = N1 and even in y; v (y-faces) is odd in y about
+// j = 0 and j = N2 and even in x.  Entries the reference never writes
+// (u at j = N2+B, v at i = N1+B) are left untouched.
+__global__ void k_fill_velocity(double *__restrict__ u, double *__restrict__ v, int N1, int N2, int B,
+                                int ld) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x - B;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y - B;
+    if (i > N1 + B || j > N2 + B) return;
+    const size_t o = (size_t)(i + B) * ld + (j + B);
+    // u component
+    if (!((i >= 0 && i <= N1 && j >= 0 && j < N2) || j == N2 + B)) {
+        const int jm = j < 0 ? -1 - j : (j >= N2 ? 2 * N2 - 1 - j : j);
+        double val;
+        if (i < 0) val = -u[(size_t)(-i + B) * ld + (jm + B)];
+        else if (i > N1) val = -u[(size_t)(2 * N1 - i + B) * ld + (jm + B)];
+        else val = u[(size_t)(i + B) * ld + (jm + B)];
+        u[o] = val;
+    }
+    // v component
+    if (!((i >= 0 && i < N1 && j >= 0 && j <= N2) || i == N1 + B)) {
+        const int im = i < 0 ? -1 - i : (i >= N1 ? 2 * N1 - 1 - i : i);
+        double val;
+        if (j < 0) val = -v[(size_t)(im + B) * ld + (-j + B)];
+        else if (j > N2) val = -v[(size_t)(im + B) * ld + (2 * N2 - j + B)];
+        else val = v[(size_t)(im + B) * ld + (j + B)];
+        v[o] = val;
+    }
+}
+
+int launch_fill_scalar(impec_ctx *ctx, double *a, int periodic) {
+    dim3 blk(64, 4), grd((ctx->N2 + 2 * ctx->B + 63) / 64, (ctx->N1 + 2 * ctx->B + 3) / 4);
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_fill_scalar, grd, blk, 0, ctx->stream, a, ctx->N1, ctx->N2, ctx->B, ctx->lds,
+                       periodic);
+    prof_end(ctx, "fill_scalar");
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+int launch_fill_velocity(impec_ctx *ctx) {
+    dim3 blk(64, 4), grd((ctx->N2 + 1 + 2 * ctx->B + 63) / 64, (ctx->N1 + 1 + 2 * ctx->B + 3) / 4);
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_fill_velocity, grd, blk, 0, ctx->stream, ctx->u, ctx->v, ctx->N1, ctx->N2, ctx->B,
+                       ctx->ldv);
+    prof_end(ctx, "fill_velocity");
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// mobility: w = mu(c)/K (harmonic) or K/mu(c) (mean concentration) on every
+// cell of the ghosted array.  Face and vertex mobilities are formed from w
+// with the reference's expressions wherever they are needed.
+// ---------------------------------------------------------------------------
+__global__ void k_mobility(const double *__restrict__ c, const double *__restrict__ K11,
+                           const double *__restrict__ K22, double *__restrict__ w, double *__restrict__ w22,
+                           int rows, int cols, int ld, double mob1over4, double mu_res, int harmonic) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= rows || j >= cols) return;
+    const size_t o = (size_t)i * ld + j;
+    const double mu = mu_of(c[o], mob1over4, mu_res);
+    const double k11 = K11 ? K11[o] : 1.;
+    w[o] = harmonic ? mu / k11 : k11 / mu;
+    if (w22) {
+        const double k22 = K22[o];
+        w22[o] = harmonic ? mu / k22 : k22 / mu;
+    }
+}
+
+int launch_mobility(impec_ctx *ctx) {
+    const int rows = ctx->N1 + 2 * ctx->B, cols = ctx->N2 + 2 * ctx->B;
+    dim3 blk(64, 4), grd((cols + 63) / 64, (rows + 3) / 4);
+    const int aniso = ctx->K22 != nullptr && ctx->K22 != ctx->K11;
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_mobility, grd, blk, 0, ctx->stream, ctx->c, ctx->K11, ctx->K22, ctx->w,
+                       aniso ? ctx->w22 : nullptr, rows, cols, ctx->lds, ctx->mob1over4, ctx->P.mu_res,
+                       ctx->P.average_procedure == IMPEC_AVG_HARMONIC);
+    prof_end(ctx, "mobility");
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Darcy face velocities (EllipticSolver.cpp:541-683)
+// ---------------------------------------------------------------------------
+struct VelConst {
+    int N1, N2, B, lds, ldv, harmonic;
+    double nu, one_m_nu, dx, dy, idxi, ideta;
+    double s2, cosa2, sina2, cost2, sint2, costsina, cosasint, cosa, sint, sina, cost;
+};
+
+// rotated-frame velocity w at the vertex whose lower-left cell is (a, b)
+// (M_im12j + GetDiagonalW_im12j, EllipticSolver.cpp:554-574)
+__device__ __forceinline__ void vertex_w(const VelConst &k, const double *__restrict__ p,
+                                         const double *__restrict__ w11, const double *__restrict__ w22, int a,
+                                         int b, double &w1, double &w2) {
+    const int ld = k.lds, B = k.B;
+#define P_(i, j) p[(size_t)((i) + B) * ld + ((j) + B)]
+#define W1(i, j) w11[(size_t)((i) + B) * ld + ((j) + B)]
+#define W2(i, j) w22[(size_t)((i) + B) * ld + ((j) + B)]
+    // cells: (a,b) = (i-1,j-1), (a+1,b+1) = (i,j)
+    const double d1dp = k.idxi * (P_(a + 1, b + 1) - P_(a, b));
+    const double d2dp = k.ideta * (P_(a, b + 1) - P_(a + 1, b));
+    const double K11 = vertex_mob(W1(a + 1, b), W1(a, b), W1(a, b + 1), W1(a + 1, b + 1), k.harmonic);
+    const double K22 = vertex_mob(W2(a + 1, b), W2(a, b), W2(a, b + 1), W2(a + 1, b + 1), k.harmonic);
+    const double M11 = k.s2 * (K11 * k.cosa2 + K22 * k.sina2);
+    const double M12 = k.s2 * (K22 * k.costsina - K11 * k.cosasint);
+    const double M22 = k.s2 * (K11 * k.sint2 + K22 * k.cost2);
+    w1 = -d1dp * M11 - d2dp * M12;
+    w2 = -d1dp * M12 - d2dp * M22;
+#undef P_
+#undef W1
+#undef W2
+}
+
+__global__ void k_velocity(VelConst k, const double *__restrict__ p, const double *__restrict__ w11,
+                           const double *__restrict__ w22, double *__restrict__ u, double *__restrict__ v) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > k.N1 || j > k.N2) return;
+    const int ld = k.lds, B = k.B;
+#define P_(i, j) p[(size_t)((i) + B) * ld + ((j) + B)]
+#define W1(i, j) w11[(size_t)((i) + B) * ld + ((j) + B)]
+#define W2(i, j) w22[(size_t)((i) + B) * ld + ((j) + B)]
+    double w1m, w2m;  // vertex (i-1/2, j-1/2), shared by u and v
+    vertex_w(k, p, w11, w22, i - 1, j - 1, w1m, w2m);
+    if (j < k.N2) {
+        // GetParallelU1_im12j + GetDiagonalU1_im12j
+        const double lam = face_mob(W1(i - 1, j), W1(i, j), k.harmonic);
+        const double par = -lam * (P_(i, j) - P_(i - 1, j)) / k.dx;
+        double w1p, w2p;
+        vertex_w(k, p, w11, w22, i - 1, j, w1p, w2p);
+        const double u1m = k.cost * w1m - k.sina * w2m;
+        const double u1p = k.cost * w1p - k.sina * w2p;
+        const double dg = 0.5 * (u1p + u1m);
+        u[(size_t)(i + B) * k.ldv + (j + B)] = k.nu * par + k.one_m_nu * dg;
+    }
+    if (i < k.N1) {
+        // GetParallelU2_ijm12 + GetDiagonalU2_ijm12
+        const double lam = face_mob(W2(i, j - 1), W2(i, j), k.harmonic);
+        const double par = -lam * (P_(i, j) - P_(i, j - 1)) / k.dy;
+        double w1p, w2p;
+        vertex_w(k, p, w11, w22, i, j - 1, w1p, w2p);
+        const double u2m = k.sint * w1m + k.cosa * w2m;
+        const double u2p = k.sint * w1p + k.cosa * w2p;
+        const double dg = 0.5 * (u2m + u2p);
+        v[(size_t)(i + B) * k.ldv + (j + B)] = k.nu * par + k.one_m_nu * dg;
+    }
+#undef P_
+#undef W1
+#undef W2
+}
+
+int launch_velocity(impec_ctx *ctx) {
+    // CalcVelocities fills p's ghosts first (EllipticSolver.cpp:663)
+    int st = launch_fill_scalar(ctx, ctx->p, 0);
+    if (st) return st;
+    VelConst k;
+    k.N1 = ctx->N1; k.N2 = ctx->N2; k.B = ctx->B; k.lds = ctx->lds; k.ldv = ctx->ldv;
+    k.harmonic = ctx->P.average_procedure == IMPEC_AVG_HARMONIC;
+    k.nu = ctx->P.nu; k.one_m_nu = 1. - ctx->P.nu;
+    k.dx = ctx->dx; k.dy = ctx->dy;
+    k.idxi = 1. / ctx->dxi; k.ideta = 1. / ctx->deta;
+    k.s2 = ctx->secamt2; k.cosa2 = ctx->cosa2; k.sina2 = ctx->sina2; k.cost2 = ctx->cost2; k.sint2 = ctx->sint2;
+    k.costsina = ctx->costsina; k.cosasint = ctx->cosasint;
+    k.cosa = ctx->cosa; k.sint = ctx->sint; k.sina = ctx->sina; k.cost = ctx->cost;
+    const double *w22 = ctx->K22 && ctx->K22 != ctx->K11 ? ctx->w22 : ctx->w;
+    dim3 blk(64, 4), grd((ctx->N2 + 1 + 63) / 64, (ctx->N1 + 1 + 3) / 4);
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_velocity, grd, blk, 0, ctx->stream, k, ctx->p, ctx->w, w22, ctx->u, ctx->v);
+    prof_end(ctx, "velocity");
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return launch_fill_velocity(ctx);
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+
+// second pass: scal[dst+k] = combine over nblocks partials of value k
+template <int OP>
+__global__ void k_finish(const double *__restrict__ part, int nblocks, int nv, double *__restrict__ scal,
+                         int dst) {
+    for (int k = 0; k < nv; k++) {
+        double v[1] = {red_identity<OP>()};
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v[0] = red_combine<OP>(v[0], part[k * RED_BLOCKS + b]);
+        block_reduce<OP, 1>(v);
+        if (threadIdx.x == 0) scal[dst + k] = v[0];
+        __syncthreads();
+    }
+}
+
+template <int OP>
+static int finish_op(impec_ctx *ctx, int nblocks, int nv, int dst) {
+    hipLaunchKernelGGL(k_finish<OP>, dim3(1), dim3(256), 0, ctx->stream, ctx->part, nblocks, nv, ctx->scal, dst);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+int finish_sum(impec_ctx *ctx, int nblocks, int nv, int dst) { return finish_op<RED_SUM>(ctx, nblocks, nv, dst); }
+
+static int red_blocks(long long n) {
+    long long b = (n + 1023) / 1024;
+    if (b < 1) b = 1;
+    if (b > RED_BLOCKS) b = RED_BLOCKS;
+    return (int)b;
+}
+
+// CFL (HyperbolicSolver.cpp:100-198): value 0 = max centred speed (combined),
+// value 1 = min advective dt over cells (test mode), value 2 = min diffusive dt
+struct CflConst {
+    int N1, N2, B, ld, mode, diff;
+    double dx, dy, al, at;
+};
+
+__global__ void k_cfl(CflConst k, const double *__restrict__ u, const double *__restrict__ v,
+                      double *__restrict__ part) {
+    const long long n = (long long)k.N1 * k.N2;
+    double mx[1] = {0.0}, mn[2] = {DBL_MAX, 1.e10};
+    for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(id / k.N2), j = (int)(id - (long long)i * k.N2);
+        const size_t o = (size_t)(i + k.B) * k.ld + (j + k.B);
+        const double u0 = u[o], u1 = u[o + k.ld], v0 = v[o], v1 = v[o + 1];
+        double uc = fabs(0.5 * (u0 + u1));
+        double vc = fabs(0.5 * (v0 + v1));
+        if (uc < DBL_MIN) uc = DBL_MIN;
+        if (vc < DBL_MIN) vc = DBL_MIN;
+        if (mx[0] < uc) mx[0] = uc;
+        if (mx[0] < vc) mx[0] = vc;
+        if (k.mode) {
+            const double a = k.dx / uc, b = k.dy / vc;
+            const double dt = b < a ? b : a;
+            if (dt < mn[0]) mn[0] = dt;
+        }
+        if (k.diff) {
+            const double vij = 0.5 * (v0 + v1), uij = 0.5 * (u0 + u1);
+            const double vij2 = vij * vij, uij2 = uij * uij;
+            const double absu = sqrt(vij2 + uij2);
+            const double d11 = k.dx / absu * (k.al * uij2 + k.at * vij2);
+            const double d22 = k.dx / absu * (k.al * vij2 + k.at * uij2);
+            const double dt = 1. / 2 * k.dx * k.dx / (d11 + d22);
+            if (dt < mn[1]) mn[1] = dt;
+        }
+    }
+    block_reduce<RED_MAX, 1>(mx);
+    block_reduce<RED_MIN, 2>(mn);
+    if (tid_linear() == 0) {
+        part[0 * RED_BLOCKS + blockIdx.x] = mx[0];
+        part[1 * RED_BLOCKS + blockIdx.x] = mn[0];
+        part[2 * RED_BLOCKS + blockIdx.x] = mn[1];
+    }
+}
+
+int launch_cfl(impec_ctx *ctx, double *dt_out) {
+    CflConst k;
+    k.N1 = ctx->N1; k.N2 = ctx->N2; k.B = ctx->B; k.ld = ctx->ldv;
+    k.mode = ctx->P.is_test;
+    k.diff = ctx->P.diffusion && ctx->P.para_method != IMPEC_PARA_CN && !ctx->P.is_test;
+    k.dx = ctx->dx; k.dy = ctx->dy; k.al = ctx->P.alpha_l; k.at = ctx->P.alpha_t;
+    const int nb = red_blocks((long long)ctx->N1 * ctx->N2);
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, ctx->stream, k, ctx->u, ctx->v, ctx->part);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    int st = finish_op<RED_MAX>(ctx, nb, 1, S_MAXVEL);
+    if (!st) {
+        // min over the two remaining partial rows: shift them down with an offset
+        hipLaunchKernelGGL(k_finish<RED_MIN>, dim3(1), dim3(256), 0, ctx->stream, ctx->part + RED_BLOCKS, nb, 2,
+                           ctx->scal, (int)S_MINDT);
+        IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    }
+    prof_end(ctx, "cfl");
+    if (st) return st;
+    IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+    IMPEC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const double *S = ctx->h_scal;
+    double dt;
+    if (ctx->P.is_test) {
+        // MaxStableAdvectionTimestep (HyperbolicSolver.cpp:100-121)
+        dt = S[S_MINDT] * ctx->P.cfl;
+    } else {
+        // MaxStableTimestepCombined (HyperbolicSolver.cpp:123-198)
+        const double maxv = S[S_MAXVEL];
+        const double a = ctx->dx / maxv, b = ctx->dy / maxv;
+        double dt_adv = b < a ? b : a;
+        dt_adv *= ctx->P.cfl;
+        double dt_diff = DBL_MAX;
+        if (ctx->P.diffusion) dt_diff = S[S_MINDT_DIFF] * ctx->P.dt_diff_limiter;
+        if (ctx->P.para_method == IMPEC_PARA_CN) dt = dt_adv;
+        else dt = dt_adv < dt_diff ? dt_adv : dt_diff;
+    }
+    *dt_out = dt;
+    return IMPEC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// TRIntegrator::Integrate over the source cells only (TRIntegrator.cpp:49-71);
+// cells with q == 0 are untouched by the reference loop.  The produced-mass
+// sum runs over the sinks in row-major order on one thread when there are few
+// of them (the reference's order), as a block tree otherwise.
+// ---------------------------------------------------------------------------
+__global__ void k_tr(double *__restrict__ c, const int *__restrict__ idx, const double *__restrict__ qv, int nsrc,
+                     int N2, int B, int ld, double dt, double hdt, double inj, double *__restrict__ scal,
+                     double *__restrict__ md) {
+    const int t = threadIdx.x;
+    for (int k = t; k < nsrc; k += blockDim.x) {
+        const int cell = idx[k];
+        const int i = cell / N2, j = cell - i * N2;
+        double *cij = &c[(size_t)(i + B) * ld + (j + B)];
+        const double qij = qv[k];
+        double d = 0.;
+        if (qij < 0) {
+            const double old = *cij;
+            *cij = *cij * (1 + hdt * qij) / (1 - hdt * qij);
+            d = *cij - old;
+        } else if (qij > 0) {
+            *cij += dt * inj * qij;
+        }
+        md[k] = d;
+    }
+    __syncthreads();
+    double tmo[1] = {0.};
+    if (nsrc <= 4096) {
+        if (t == 0)
+            for (int k = 0; k < nsrc; k++)
+                if (qv[k] < 0) tmo[0] += md[k];
+        __syncthreads();
+    } else {
+        for (int k = t; k < nsrc; k += blockDim.x) tmo[0] += md[k];
+        block_reduce<RED_SUM, 1>(tmo);
+    }
+    if (t == 0) {
+        scal[S_TMO] = tmo[0];
+        scal[S_OUTFLUX] += fabs(tmo[0]);
+    }
+}
+
+int launch_tr(impec_ctx *ctx, double dt) {
+    if (ctx->nsrc == 0) return IMPEC_OK;
+    const double hdt = 0.5 * dt;
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_tr, dim3(1), dim3(1024), 0, ctx->stream, ctx->c, ctx->src_idx, ctx->src_q, ctx->nsrc,
+                       ctx->N2, ctx->B, ctx->lds, dt, hdt, ctx->P.inj_fluid_conc, ctx->scal, ctx->tr_md);
+    prof_end(ctx, "tr_source");
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics (Impec2DSolver.cpp:530-620)
+// ---------------------------------------------------------------------------
+__global__ void k_diag(const double *__restrict__ c, const double *__restrict__ u, const double *__restrict__ v,
+                       const double *__restrict__ q, int N1, int N2, int B, int lds, int ldv, double dx, double dy,
+                       double *__restrict__ part) {
+    const long long n = (long long)N1 * N2;
+    double s[3] = {0., 0., 0.};
+    for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(id / N2), j = (int)(id - (long long)i * N2);
+        s[0] += c[(size_t)(i + B) * lds + (j + B)];
+        const size_t o = (size_t)(i + B) * ldv + (j + B);
+        const double div = (v[o + 1] - v[o]) / dy + (u[o + ldv] - u[o]) / dx;
+        s[1] += fabs(div - q[id]);
+        double nf = 0.;
+        if (j == 0) nf += fabs(v[o]);
+        if (j == N2 - 1) nf += fabs(v[o + 1]);
+        if (i == 0) nf += fabs(u[o]);
+        if (i == N1 - 1) nf += fabs(u[o + ldv]);
+        s[2] += nf;
+    }
+    block_reduce<RED_SUM, 3>(s);
+    if (tid_linear() == 0)
+        for (int k = 0; k < 3; k++) part[k * RED_BLOCKS + blockIdx.x] = s[k];
+}
+
+int launch_diagnostics(impec_ctx *ctx, int which) {
+    (void)which;
+    const int nb = red_blocks((long long)ctx->N1 * ctx->N2);
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_diag, dim3(nb), dim3(256), 0, ctx->stream, ctx->c, ctx->u, ctx->v, ctx->q, ctx->N1,
+                       ctx->N2, ctx->B, ctx->lds, ctx->ldv, ctx->dx, ctx->dy, ctx->part);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    int st = finish_sum(ctx, nb, 3, S_MASS);
+    prof_end(ctx, "diagnostics");
+    if (st) return st;
+    // S_MASS, S_MASS+1 (div sum), S_MASS+2 (noflow sum)
+    IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+    IMPEC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return IMPEC_OK;
+}
+
+__global__ void k_minmax(const double *__restrict__ c, long long n, double *__restrict__ part) {
+    double mx[1] = {-DBL_MAX}, mn[1] = {DBL_MAX};
+    for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+         id += (long long)gridDim.x * blockDim.x) {
+        const double x = c[id];
+        mx[0] = x > mx[0] ? x : mx[0];
+        mn[0] = x < mn[0] ? x : mn[0];
+    }
+    block_reduce<RED_MAX, 1>(mx);
+    block_reduce<RED_MIN, 1>(mn);
+    if (tid_linear() == 0) {
+        part[blockIdx.x] = mx[0];
+        part[RED_BLOCKS + blockIdx.x] = mn[0];
+    }
+}
+
+// max/min over the whole ghosted array, like blitz max(c)/min(c); the row
+// padding beyond N2+2B columns is excluded by walking rows of the pitch.
+__global__ void k_minmax_rows(const double *__restrict__ c, int rows, int cols, int ld, double *__restrict__ part) {
+    const long long n = (long long)rows * cols;
+    double mx[1] = {-DBL_MAX}, mn[1] = {DBL_MAX};
+    for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(id / cols), j = (int)(id - (long long)i * cols);
+        const double x = c[(size_t)i * ld + j];
+        mx[0] = x > mx[0] ? x : mx[0];
+        mn[0] = x < mn[0] ? x : mn[0];
+    }
+    block_reduce<RED_MAX, 1>(mx);
+    block_reduce<RED_MIN, 1>(mn);
+    if (tid_linear() == 0) {
+        part[blockIdx.x] = mx[0];
+        part[RED_BLOCKS + blockIdx.x] = mn[0];
+    }
+}
+
+int launch_minmax(impec_ctx *ctx, double *cmax, double *cmin) {
+    const int rows = ctx->N1 + 2 * ctx->B, cols = ctx->N2 + 2 * ctx->B;
+    const int nb = red_blocks((long long)rows * cols);
+    hipLaunchKernelGGL(k_minmax_rows, dim3(nb), dim3(256), 0, ctx->stream, ctx->c, rows, cols, ctx->lds, ctx->part);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    int st = finish_op<RED_MAX>(ctx, nb, 1, S_CMAX);
+    if (st) return st;
+    hipLaunchKernelGGL(k_finish<RED_MIN>, dim3(1), dim3(256), 0, ctx->stream, ctx->part + RED_BLOCKS, nb, 1,
+                       ctx->scal, (int)S_CMIN);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+    IMPEC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    *cmax = ctx->h_scal[S_CMAX];
+    *cmin = ctx->h_scal[S_CMIN];
+    return IMPEC_OK;
+}
+
+}  // namespace impec

@@ -1,0 +1,839 @@
+// k_pressure.hip -- the nu-weighted 9-point pressure operator, a geometric
+// multigrid V-cycle on cell-centred Neumann levels, and the FP64 conjugate
+// gradient loop that replaces HYPRE StructPCG + StructSMG
+// (MGSolver.cpp:268-667).
+//
+// Formulation.  The reference pins row (1,1) to p = -q(1,1) dA / kappa
+// (MGSolver.cpp:517-567).  When the folded Neumann matrix A_N is symmetric
+// (nu == 1, or dx == dy with isotropic K) its column sums vanish, so the pinned
+// solution equals the solution of A_N x = b~ (b~ = b with sum(b) removed from
+// the pinned entry) shifted so that x(1,1) takes the pinned value.  CG then
+// runs on the consistent singular Neumann system with a mean-projected
+// multigrid preconditioner, which converges in a grid-independent number of
+// cycles.  Otherwise the pinned row and column are eliminated and CG runs on
+// the remaining rows.  Either way the returned residual is that of the
+// reference's pinned system.
+#include "impec_device.h"
+#include <math.h>
+#include <stdlib.h>
+#include <vector>
+
+namespace impec {
+
+// ---------------------------------------------------------------------------
+// operator row
+// ---------------------------------------------------------------------------
+enum { ST_C = 0, ST_W, ST_SW, ST_S, ST_SE, ST_E, ST_NE, ST_N, ST_NW };
+
+// Folding of the entries of boundary rows onto in-domain neighbours, in the
+// reference's order of edge and corner overwrites (MGSolver.cpp:305-515).
+__device__ __forceinline__ void fold_row(int i, int j, int n1, int n2, double a[9]) {
+    const bool bi0 = i == 0, bi1 = i == n1 - 1, bj0 = j == 0, bj1 = j == n2 - 1;
+    if (!(bi0 | bi1 | bj0 | bj1)) return;
+    const double C = a[ST_C], W = a[ST_W], SW = a[ST_SW], S = a[ST_S], SE = a[ST_SE], E = a[ST_E], NE = a[ST_NE],
+                 N = a[ST_N], NW = a[ST_NW];
+    double o[9];
+    if (bi0 && bj0) {
+        o[0] = C + W + S + SW; o[1] = 0; o[2] = 0; o[3] = 0; o[4] = 0; o[5] = E + SE; o[6] = NE; o[7] = N + NW; o[8] = 0;
+    } else if (bi0 && bj1) {
+        o[0] = C + W + N + NW; o[1] = 0; o[2] = 0; o[3] = S + SW; o[4] = SE; o[5] = E + NE; o[6] = 0; o[7] = 0; o[8] = 0;
+    } else if (bi1 && bj1) {
+        o[0] = C + N + NE + E; o[1] = W + NW; o[2] = SW; o[3] = S + SE; o[4] = 0; o[5] = 0; o[6] = 0; o[7] = 0; o[8] = 0;
+    } else if (bi1 && bj0) {
+        o[0] = C + S + E + SE; o[1] = W + SW; o[2] = 0; o[3] = 0; o[4] = 0; o[5] = 0; o[6] = 0; o[7] = N + NE; o[8] = NW;
+    } else if (bi1) {
+        o[0] = C + E; o[1] = W; o[2] = SW; o[3] = S + SE; o[4] = 0; o[5] = 0; o[6] = 0; o[7] = N + NE; o[8] = NW;
+    } else if (bi0) {
+        o[0] = C + W; o[1] = 0; o[2] = 0; o[3] = S + SW; o[4] = SE; o[5] = E; o[6] = NE; o[7] = N + NW; o[8] = 0;
+    } else if (bj1) {
+        o[0] = C + N; o[1] = W + NW; o[2] = SW; o[3] = S; o[4] = SE; o[5] = E + NE; o[6] = 0; o[7] = 0; o[8] = 0;
+    } else {
+        o[0] = C + S; o[1] = W + SW; o[2] = 0; o[3] = 0; o[4] = 0; o[5] = E + SE; o[6] = NE; o[7] = N; o[8] = NW;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) a[k] = o[k];
+}
+
+// Folded coefficients of row (i,j) from the per-cell mobility quantity
+// (GetWeightedMatrixCoefficientsForCell, MGSolver.cpp:710-841).  w1/w2 point
+// at cell (0,0) of arrays with pitch ldw and at least one ghost ring.
+template <bool DIAG>
+__device__ __forceinline__ void op_row(const OpGeom &g, const double *__restrict__ w1,
+                                       const double *__restrict__ w2, int ldw, int i, int j, double a[9]) {
+#define W1(a_, b_) w1[(long)(a_) * ldw + (b_)]
+#define W2(a_, b_) w2[(long)(a_) * ldw + (b_)]
+    const double fxm = face_mob(W1(i - 1, j), W1(i, j), g.harmonic);
+    const double fxp = face_mob(W1(i, j), W1(i + 1, j), g.harmonic);
+    const double fym = face_mob(W2(i, j - 1), W2(i, j), g.harmonic);
+    const double fyp = face_mob(W2(i, j), W2(i, j + 1), g.harmonic);
+    // GetParallelCoefficientsForCell (MGSolver.cpp:760-773)
+    const double Wp = g.cx * fxm, Ep = g.cx * fxp, Sp = g.cy * fym, Np = g.cy * fyp;
+    const double Cp = g.mcy * (fym + fyp) - g.cx * (fxm + fxp);
+    if (!DIAG) {
+        a[ST_C] = Cp; a[ST_W] = Wp; a[ST_SW] = 0.; a[ST_S] = Sp; a[ST_SE] = 0.;
+        a[ST_E] = Ep; a[ST_NE] = 0.; a[ST_N] = Np; a[ST_NW] = 0.;
+    } else {
+        // GetTransformedMobilityTensor + GetDiagonalCoefficientsForCell (MGSolver.cpp:777-841)
+        const double K11pp = vertex_mob(W1(i + 1, j), W1(i, j), W1(i, j + 1), W1(i + 1, j + 1), g.harmonic);
+        const double K11mm = vertex_mob(W1(i, j - 1), W1(i - 1, j - 1), W1(i - 1, j), W1(i, j), g.harmonic);
+        const double K11mp = vertex_mob(W1(i, j), W1(i - 1, j), W1(i - 1, j + 1), W1(i, j + 1), g.harmonic);
+        const double K11pm = vertex_mob(W1(i + 1, j - 1), W1(i, j - 1), W1(i, j), W1(i + 1, j), g.harmonic);
+        double K22pp = K11pp, K22mm = K11mm, K22mp = K11mp, K22pm = K11pm;
+        if (w2 != w1) {
+            K22pp = vertex_mob(W2(i + 1, j), W2(i, j), W2(i, j + 1), W2(i + 1, j + 1), g.harmonic);
+            K22mm = vertex_mob(W2(i, j - 1), W2(i - 1, j - 1), W2(i - 1, j), W2(i, j), g.harmonic);
+            K22mp = vertex_mob(W2(i, j), W2(i - 1, j), W2(i - 1, j + 1), W2(i, j + 1), g.harmonic);
+            K22pm = vertex_mob(W2(i + 1, j - 1), W2(i, j - 1), W2(i, j), W2(i + 1, j), g.harmonic);
+        }
+        const double M11p = g.s2 * (K11pp * g.cosa2 + K22pp * g.sina2);
+        const double M11m = g.s2 * (K11mm * g.cosa2 + K22mm * g.sina2);
+        const double M22p = g.s2 * (K22mp * g.cost2 + K11mp * g.sint2);
+        const double M22m = g.s2 * (K22pm * g.cost2 + K11pm * g.sint2);
+        const double M12pp = g.s2 * (K22pp * g.costsina - K11pp * g.cosasint);
+        const double M21mp = g.s2 * (K22mp * g.costsina - K11mp * g.cosasint);
+        const double M12mm = g.s2 * (K22mm * g.costsina - K11mm * g.cosasint);
+        const double M21pm = g.s2 * (K22pm * g.costsina - K11pm * g.cosasint);
+        const double NEd = M11p / g.dxi2, SWd = M11m / g.dxi2, NWd = M22p / g.deta2, SEd = M22m / g.deta2;
+        const double Cd = g.mcxi * (M11p + M11m) - g.ceta * (M22p + M22m);
+        const double Nd = g.cnd * (M12pp + M21mp);
+        const double Wd = g.mcnd * (M21mp + M12mm);
+        const double Ed = g.mcnd * (M12pp + M21pm);
+        const double Sd = g.cnd * (M12mm + M21pm);
+        const double nu = g.nu, om = g.one_m_nu;
+        a[ST_C] = nu * Cp + om * Cd;
+        a[ST_W] = nu * Wp + om * Wd;
+        a[ST_SW] = nu * 0. + om * SWd;
+        a[ST_S] = nu * Sp + om * Sd;
+        a[ST_SE] = nu * 0. + om * SEd;
+        a[ST_E] = nu * Ep + om * Ed;
+        a[ST_NE] = nu * 0. + om * NEd;
+        a[ST_N] = nu * Np + om * Nd;
+        a[ST_NW] = nu * 0. + om * NWd;
+    }
+    fold_row(i, j, g.n1, g.n2, a);
+#undef W1
+#undef W2
+}
+
+// sum_k a_k x_nb(k) in stencil order; x points at cell (0,0), pitch ld;
+// out-of-domain neighbours have zero coefficients after folding and are
+// skipped, so the ring contents never enter.
+template <bool DIAG>
+__device__ __forceinline__ double row_dot(const double a[9], const double *__restrict__ x, int ld, int i, int j,
+                                          int n1, int n2) {
+    const double *xc = x + (long)i * ld + j;
+    double s = a[ST_C] * xc[0];
+    const bool w = i > 0, e = i < n1 - 1, so = j > 0, no = j < n2 - 1;
+    if (w) s += a[ST_W] * xc[-ld];
+    if (DIAG && w && so) s += a[ST_SW] * xc[-ld - 1];
+    if (so) s += a[ST_S] * xc[-1];
+    if (DIAG && e && so) s += a[ST_SE] * xc[ld - 1];
+    if (e) s += a[ST_E] * xc[ld];
+    if (DIAG && e && no) s += a[ST_NE] * xc[ld + 1];
+    if (no) s += a[ST_N] * xc[1];
+    if (DIAG && w && no) s += a[ST_NW] * xc[-ld + 1];
+    return s;
+}
+
+// off-diagonal part for Gauss-Seidel: b - sum_{k != C} a_k x_nb
+template <bool DIAG>
+__device__ __forceinline__ double gs_rhs(const double a[9], double b, const double *__restrict__ x, int ld, int i,
+                                         int j, int n1, int n2) {
+    const double *xc = x + (long)i * ld + j;
+    double s = b;
+    const bool w = i > 0, e = i < n1 - 1, so = j > 0, no = j < n2 - 1;
+    if (w) s -= a[ST_W] * xc[-ld];
+    if (DIAG && w && so) s -= a[ST_SW] * xc[-ld - 1];
+    if (so) s -= a[ST_S] * xc[-1];
+    if (DIAG && e && so) s -= a[ST_SE] * xc[ld - 1];
+    if (e) s -= a[ST_E] * xc[ld];
+    if (DIAG && e && no) s -= a[ST_NE] * xc[ld + 1];
+    if (no) s -= a[ST_N] * xc[1];
+    if (DIAG && w && no) s -= a[ST_NW] * xc[-ld + 1];
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// level data passed by value to the kernels
+// ---------------------------------------------------------------------------
+struct LevArgs {
+    OpGeom g;
+    const double *w1, *w2;  // at cell (0,0)
+    int ldw;
+    int ld;                 // pitch of x/b/r
+};
+
+// ---------------------------------------------------------------------------
+// multigrid kernels
+// ---------------------------------------------------------------------------
+template <bool DIAG>
+__global__ void k_smooth(LevArgs L, double *__restrict__ x, const double *__restrict__ b, int color) {
+    const int tj = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ti = blockIdx.y * blockDim.y + threadIdx.y;
+    int i, j;
+    if (DIAG) {
+        i = 2 * ti + (color >> 1);
+        j = 2 * tj + (color & 1);
+    } else {
+        i = ti;
+        j = 2 * tj + ((i + color) & 1);
+    }
+    if (i >= L.g.n1 || j >= L.g.n2) return;
+    double a[9];
+    op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+    const double s = gs_rhs<DIAG>(a, b[(long)i * L.ld + j], x, L.ld, i, j, L.g.n1, L.g.n2);
+    x[(long)i * L.ld + j] = a[ST_C] != 0. ? s / a[ST_C] : 0.;
+}
+
+template <bool DIAG>
+__global__ void k_residual(LevArgs L, const double *__restrict__ x, const double *__restrict__ b,
+                           double *__restrict__ r) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= L.g.n1 || j >= L.g.n2) return;
+    double a[9];
+    op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+    const long o = (long)i * L.ld + j;
+    r[o] = b[o] - row_dot<DIAG>(a, x, L.ld, i, j, L.g.n1, L.g.n2);
+}
+
+// 1-D weight of fine index f for coarse index I under the cell-centred
+// bilinear prolongation with constant extrapolation at the ends
+__device__ __forceinline__ double pw(int f, int I, int nc) {
+    int I0 = f >> 1;
+    int I1 = (f & 1) ? I0 + 1 : I0 - 1;
+    if (I0 > nc - 1) I0 = nc - 1;
+    if (I1 < 0) I1 = 0;
+    if (I1 > nc - 1) I1 = nc - 1;
+    return (I0 == I ? 0.75 : 0.) + (I1 == I ? 0.25 : 0.);
+}
+
+// b_c = P^T r / 4 (full weighting; the transpose of k_prolong)
+__global__ void k_restrict(const double *__restrict__ r, int n1, int n2, int ldf, double *__restrict__ bc, int m1,
+                           int m2, int ldc) {
+    const int J = blockIdx.x * blockDim.x + threadIdx.x;
+    const int I = blockIdx.y * blockDim.y + threadIdx.y;
+    if (I >= m1 || J >= m2) return;
+    double s = 0.;
+    for (int f = 2 * I - 1; f <= 2 * I + 2; f++) {
+        if (f < 0 || f >= n1) continue;
+        const double wi = pw(f, I, m1);
+        if (wi == 0.) continue;
+        double t = 0.;
+        for (int g = 2 * J - 1; g <= 2 * J + 2; g++) {
+            if (g < 0 || g >= n2) continue;
+            const double wj = pw(g, J, m2);
+            if (wj != 0.) t += wj * r[(long)f * ldf + g];
+        }
+        s += wi * t;
+    }
+    bc[(long)I * ldc + J] = 0.25 * s;
+}
+
+// x_f += P x_c (cell-centred bilinear, constant extrapolation at the ends)
+__global__ void k_prolong(double *__restrict__ x, int n1, int n2, int ldf, const double *__restrict__ xc, int m1,
+                          int m2, int ldc) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= n1 || j >= n2) return;
+    int I0 = i >> 1, I1 = (i & 1) ? I0 + 1 : I0 - 1;
+    int J0 = j >> 1, J1 = (j & 1) ? J0 + 1 : J0 - 1;
+    I0 = I0 > m1 - 1 ? m1 - 1 : I0;
+    J0 = J0 > m2 - 1 ? m2 - 1 : J0;
+    I1 = I1 < 0 ? 0 : (I1 > m1 - 1 ? m1 - 1 : I1);
+    J1 = J1 < 0 ? 0 : (J1 > m2 - 1 ? m2 - 1 : J1);
+    const double e = 0.5625 * xc[(long)I0 * ldc + J0] + 0.1875 * xc[(long)I0 * ldc + J1] +
+                     0.1875 * xc[(long)I1 * ldc + J0] + 0.0625 * xc[(long)I1 * ldc + J1];
+    x[(long)i * ldf + j] += e;
+}
+
+__global__ void k_fill_zero(double *__restrict__ a, long n) {
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) a[k] = 0.;
+}
+
+// coarse mobility quantity: mean over the existing fine cells of each 2x2
+// block, on the interior and (clamped) ghost ring of the coarse level
+__global__ void k_coarsen_w(const double *__restrict__ wf, int ldf, int n1, int n2, double *__restrict__ wc, int ldc,
+                            int m1, int m2) {
+    const int J = blockIdx.x * blockDim.x + threadIdx.x - 1;
+    const int I = blockIdx.y * blockDim.y + threadIdx.y - 1;
+    if (I > m1 || J > m2) return;
+    const int Ic = I < 0 ? 0 : (I >= m1 ? m1 - 1 : I);
+    const int Jc = J < 0 ? 0 : (J >= m2 ? m2 - 1 : J);
+    double s = 0.;
+    int cnt = 0;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            const int f = 2 * Ic + a, g = 2 * Jc + b;
+            if (f < n1 && g < n2) {
+                s += wf[(long)f * ldf + g];
+                cnt++;
+            }
+        }
+    wc[(long)I * ldc + J] = s / cnt;
+}
+
+// coarsest level: symmetric multicolour Gauss-Seidel sweeps from zero on one
+// thread, then mean projection (the level is at most 2 x 2 cells)
+template <bool DIAG>
+__global__ void k_coarsest(LevArgs L, double *__restrict__ x, const double *__restrict__ b, int sweeps) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int n1 = L.g.n1, n2 = L.g.n2, nc = DIAG ? 4 : 2;
+    for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++) x[(long)i * L.ld + j] = 0.;
+    for (int s = 0; s < sweeps; s++)
+        for (int dir = 0; dir < 2; dir++)
+            for (int cc = 0; cc < nc; cc++) {
+                const int col = dir == 0 ? cc : nc - 1 - cc;
+                for (int i = 0; i < n1; i++)
+                    for (int j = 0; j < n2; j++) {
+                        const int mycol = DIAG ? ((i & 1) * 2 + (j & 1)) : ((i + j) & 1);
+                        if (mycol != col) continue;
+                        double a[9];
+                        op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+                        const double r = gs_rhs<DIAG>(a, b[(long)i * L.ld + j], x, L.ld, i, j, n1, n2);
+                        x[(long)i * L.ld + j] = a[ST_C] != 0. ? r / a[ST_C] : 0.;
+                    }
+            }
+    double m = 0.;
+    for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++) m += x[(long)i * L.ld + j];
+    m /= (double)(n1 * n2);
+    for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++) x[(long)i * L.ld + j] -= m;
+}
+
+// ---------------------------------------------------------------------------
+// conjugate-gradient vector kernels (level-0 ring-1 layout; x lives in p)
+// ---------------------------------------------------------------------------
+struct CgArgs {
+    int n1, n2, ld;     // ring-1 layout pitch
+    int ldp, B;         // pressure array pitch and ghost width
+    long n;             // n1 * n2
+    int pin_i, pin_j;
+    int shift;          // consistent-Neumann formulation
+    double ppin;        // pinned value
+    double dpin;        // kappa / dA
+    double bsum;        // sum of b = -q
+};
+
+__device__ __forceinline__ void cg_cell(const CgArgs &c, long id, int &i, int &j) {
+    i = (int)(id / c.n2);
+    j = (int)(id - (long)i * c.n2);
+}
+
+// r = b - A_N x with b = -q; pin handling; partials (r.r, sum r)
+template <bool DIAG>
+__global__ void k_cg_init(LevArgs L, CgArgs c, double *__restrict__ p, const double *__restrict__ q,
+                          double *__restrict__ r, double *__restrict__ part) {
+    double s[2] = {0., 0.};
+    double *x = p + (long)c.B * c.ldp + c.B;
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        double a[9];
+        op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+        double xv[9];
+        (void)xv;
+        double bij = -q[id];
+        const bool pin = i == c.pin_i && j == c.pin_j;
+        double rv;
+        if (!c.shift) {
+            // x_pin = ppin for every neighbour read
+            const double *xc = x + (long)i * c.ldp + j;
+            double acc = 0.;
+            const int dI[9] = {0, -1, -1, 0, 1, 1, 1, 0, -1}, dJ[9] = {0, 0, -1, -1, -1, 0, 1, 1, 1};
+            for (int k = 0; k < 9; k++) {
+                const int ii = i + dI[k], jj = j + dJ[k];
+                if (ii < 0 || jj < 0 || ii >= c.n1 || jj >= c.n2) continue;
+                if (!DIAG && (k == ST_SW || k == ST_SE || k == ST_NE || k == ST_NW)) continue;
+                const double xn = (ii == c.pin_i && jj == c.pin_j) ? c.ppin : xc[dI[k] * c.ldp + dJ[k]];
+                acc += a[k] * xn;
+            }
+            rv = pin ? 0. : bij - acc;
+        } else {
+            rv = bij - row_dot<DIAG>(a, x, c.ldp, i, j, c.n1, c.n2);
+            if (pin) rv -= c.bsum;
+        }
+        r[(long)(i + 1) * c.ld + (j + 1)] = rv;
+        s[0] += rv * rv;
+        s[1] += rv;
+    }
+    block_reduce<RED_SUM, 2>(s);
+    if (tid_linear() == 0) {
+        part[blockIdx.x] = s[0];
+        part[RED_BLOCKS + blockIdx.x] = s[1];
+    }
+}
+
+// mg.b = r - mean(r) (the mean comes from S_RSUM)
+__global__ void k_cg_proj_in(CgArgs c, const double *__restrict__ r, double *__restrict__ b0,
+                             const double *__restrict__ S) {
+    const double mean = S[S_RSUM] / (double)c.n;
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        const long o = (long)(i + 1) * c.ld + (j + 1);
+        b0[o] = r[o] - mean;
+    }
+}
+
+// partial sum of the V-cycle output
+__global__ void k_sum(CgArgs c, const double *__restrict__ x0, double *__restrict__ part) {
+    double s[1] = {0.};
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        s[0] += x0[(long)(i + 1) * c.ld + (j + 1)];
+    }
+    block_reduce<RED_SUM, 1>(s);
+    if (tid_linear() == 0) part[blockIdx.x] = s[0];
+}
+
+// z = x0 - mean(x0) (pin entry zeroed in the eliminated formulation); r.z partials
+__global__ void k_cg_z(CgArgs c, const double *__restrict__ x0, const double *__restrict__ r, double *__restrict__ z,
+                       const double *__restrict__ S, double *__restrict__ part) {
+    const double mean = S[S_ZSUM] / (double)c.n;
+    double s[1] = {0.};
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        const long o = (long)(i + 1) * c.ld + (j + 1);
+        double zv = x0[o] - mean;
+        if (!c.shift && i == c.pin_i && j == c.pin_j) zv = 0.;
+        z[o] = zv;
+        s[0] += r[o] * zv;
+    }
+    block_reduce<RED_SUM, 1>(s);
+    if (tid_linear() == 0) part[blockIdx.x] = s[0];
+}
+
+// d = z + beta d (beta = 0 on the first iteration)
+__global__ void k_cg_dir(CgArgs c, const double *__restrict__ z, double *__restrict__ d, const double *__restrict__ S,
+                         int first) {
+    const double beta = first ? 0. : S[S_RHO] / S[S_RHO_OLD];
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        const long o = (long)(i + 1) * c.ld + (j + 1);
+        d[o] = first ? z[o] : z[o] + beta * d[o];
+    }
+}
+
+// q = A_N d (pin row zeroed in the eliminated formulation); d.q partials
+template <bool DIAG>
+__global__ void k_cg_matvec(LevArgs L, CgArgs c, const double *__restrict__ d, double *__restrict__ qv,
+                            double *__restrict__ part) {
+    double s[1] = {0.};
+    const double *d0 = d + c.ld + 1;
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        double a[9];
+        op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+        double y = row_dot<DIAG>(a, d0, c.ld, i, j, c.n1, c.n2);
+        if (!c.shift && i == c.pin_i && j == c.pin_j) y = 0.;
+        const long o = (long)(i + 1) * c.ld + (j + 1);
+        qv[o] = y;
+        s[0] += d0[(long)i * c.ld + j] * y;
+    }
+    block_reduce<RED_SUM, 1>(s);
+    if (tid_linear() == 0) part[blockIdx.x] = s[0];
+}
+
+// x += alpha d, r -= alpha q; partials (r.r, sum r)
+__global__ void k_cg_update(CgArgs c, double *__restrict__ p, const double *__restrict__ d, double *__restrict__ r,
+                            const double *__restrict__ qv, const double *__restrict__ S, double *__restrict__ part) {
+    const double alpha = S[S_RHO] / S[S_DQ];
+    double s[2] = {0., 0.};
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        const long o = (long)(i + 1) * c.ld + (j + 1);
+        double *xp = p + (long)(i + c.B) * c.ldp + (j + c.B);
+        *xp = *xp + alpha * d[o];
+        const double rv = r[o] - alpha * qv[o];
+        r[o] = rv;
+        s[0] += rv * rv;
+        s[1] += rv;
+    }
+    block_reduce<RED_SUM, 2>(s);
+    if (tid_linear() == 0) {
+        part[blockIdx.x] = s[0];
+        part[RED_BLOCKS + blockIdx.x] = s[1];
+    }
+}
+
+__global__ void k_copy_scal(double *__restrict__ S, int dst, int src) { S[dst] = S[src]; }
+
+__global__ void k_read_pin(CgArgs c, const double *__restrict__ p, double *__restrict__ S) {
+    S[S_XPIN] = p[(long)(c.pin_i + c.B) * c.ldp + (c.pin_j + c.B)];
+}
+
+// x += ppin - x_pin on the interior, x_pin = ppin exactly
+__global__ void k_shift(CgArgs c, double *__restrict__ p, const double *__restrict__ S) {
+    const double sh = c.ppin - S[S_XPIN];
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        double *xp = p + (long)(i + c.B) * c.ldp + (j + c.B);
+        *xp = (i == c.pin_i && j == c.pin_j) ? c.ppin : *xp + sh;
+    }
+}
+
+// ||b - A x||^2 of the reference's pinned system
+template <bool DIAG>
+__global__ void k_true_residual(LevArgs L, CgArgs c, const double *__restrict__ p, const double *__restrict__ q,
+                                double *__restrict__ part) {
+    double s[1] = {0.};
+    const double *x = p + (long)c.B * c.ldp + c.B;
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        double rv;
+        if (i == c.pin_i && j == c.pin_j) {
+            rv = -q[id] - c.dpin * x[(long)i * c.ldp + j];
+        } else {
+            double a[9];
+            op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+            rv = -q[id] - row_dot<DIAG>(a, x, c.ldp, i, j, c.n1, c.n2);
+        }
+        s[0] += rv * rv;
+    }
+    block_reduce<RED_SUM, 1>(s);
+    if (tid_linear() == 0) part[blockIdx.x] = s[0];
+}
+
+// operator dump: nine N1*N2 arrays including the pinned row (MGSolver.cpp:517-567)
+template <bool DIAG>
+__global__ void k_op_dump(LevArgs L, int pin_i, int pin_j, double dpin, double *__restrict__ A) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int n1 = L.g.n1, n2 = L.g.n2;
+    if (i >= n1 || j >= n2) return;
+    double a[9];
+    op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+    if (i == pin_i && j == pin_j) {
+        for (int k = 0; k < 9; k++) a[k] = 0.;
+        a[ST_C] = dpin;
+    }
+    const long M = (long)n1 * n2;
+    for (int k = 0; k < 9; k++) A[k * M + (long)i * n2 + j] = a[k];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static OpGeom make_geom(const impec_ctx *ctx, int n1, int n2, double dx, double dy) {
+    OpGeom g;
+    g.n1 = n1; g.n2 = n2;
+    g.harmonic = ctx->P.average_procedure == IMPEC_AVG_HARMONIC;
+    g.nu = ctx->P.nu;
+    g.one_m_nu = 1 - ctx->P.nu;
+    g.diag = ctx->P.nu != 1.;
+    const double dxi = dx / cos(ctx->theta), deta = dy / cos(ctx->alpha);
+    g.cx = 1. / (dx * dx);
+    g.cy = 1. / (dy * dy);
+    g.mcy = -1. / (dy * dy);
+    g.s2 = ctx->secamt2; g.cosa2 = ctx->cosa2; g.sina2 = ctx->sina2; g.cost2 = ctx->cost2; g.sint2 = ctx->sint2;
+    g.costsina = ctx->costsina; g.cosasint = ctx->cosasint;
+    g.dxi2 = dxi * dxi;
+    g.deta2 = deta * deta;
+    g.mcxi = -1. / (dxi * dxi);
+    g.ceta = 1. / (deta * deta);
+    g.cnd = 1. / (deta * dxi);
+    g.mcnd = -1. / (deta * dxi);
+    return g;
+}
+
+static LevArgs lev_args(const impec_ctx *ctx, int l) {
+    LevArgs a;
+    const Level &L = ctx->lev[l];
+    a.g = L.g;
+    a.ld = L.ld;
+    if (l == 0) {
+        a.ldw = ctx->lds;
+        a.w1 = ctx->w + (long)ctx->B * ctx->lds + ctx->B;
+        const bool aniso = ctx->K22 && ctx->K22 != ctx->K11;
+        a.w2 = aniso ? ctx->w22 + (long)ctx->B * ctx->lds + ctx->B : a.w1;
+    } else {
+        a.ldw = L.ld;
+        a.w1 = L.w + L.ld + 1;
+        a.w2 = a.w1;
+    }
+    return a;
+}
+
+static int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+int pressure_setup_levels(impec_ctx *ctx) {
+    int n1 = ctx->N1, n2 = ctx->N2;
+    double dx = ctx->dx, dy = ctx->dy;
+    int l = 0;
+    for (;;) {
+        Level &L = ctx->lev[l];
+        memset(&L, 0, sizeof(L));
+        L.n1 = n1; L.n2 = n2; L.dx = dx; L.dy = dy;
+        L.ld = round_up(n2 + 2, 16);
+        L.g = make_geom(ctx, n1, n2, dx, dy);
+        const size_t bytes = sizeof(double) * (size_t)(n1 + 2) * L.ld;
+        if (hipMalloc(&L.x, bytes) != hipSuccess || hipMalloc(&L.b, bytes) != hipSuccess ||
+            hipMalloc(&L.r, bytes) != hipSuccess)
+            return set_error(ctx, IMPEC_ENOMEM, "multigrid level %d allocation failed", l);
+        hipMemset(L.x, 0, bytes);
+        hipMemset(L.b, 0, bytes);
+        hipMemset(L.r, 0, bytes);
+        if (l > 0) {
+            if (hipMalloc(&L.w, bytes) != hipSuccess)
+                return set_error(ctx, IMPEC_ENOMEM, "multigrid level %d allocation failed", l);
+            hipMemset(L.w, 0, bytes);
+        }
+        ctx->nlev = ++l;
+        if ((n1 <= 2 && n2 <= 2) || l >= MAX_LEVELS) break;
+        n1 = (n1 + 1) / 2;
+        n2 = (n2 + 1) / 2;
+        dx *= 2;
+        dy *= 2;
+    }
+    // CG vectors on the level-0 layout
+    const size_t bytes0 = sizeof(double) * (size_t)(ctx->N1 + 2) * ctx->lev[0].ld;
+    double **vecs[4] = {&ctx->pr, &ctx->pz, &ctx->pd, &ctx->pq};
+    for (int k = 0; k < 4; k++) {
+        if (hipMalloc(vecs[k], bytes0) != hipSuccess) return set_error(ctx, IMPEC_ENOMEM, "CG vector allocation failed");
+        hipMemset(*vecs[k], 0, bytes0);
+    }
+    return IMPEC_OK;
+}
+
+void pressure_free_levels(impec_ctx *ctx) {
+    for (int l = 0; l < ctx->nlev; l++) {
+        Level &L = ctx->lev[l];
+        hipFree(L.x); hipFree(L.b); hipFree(L.r);
+        if (L.w) hipFree(L.w);
+    }
+    ctx->nlev = 0;
+    hipFree(ctx->pr); hipFree(ctx->pz); hipFree(ctx->pd); hipFree(ctx->pq);
+}
+
+static dim3 grid2(int n1, int n2, dim3 blk) { return dim3((n2 + blk.x - 1) / blk.x, (n1 + blk.y - 1) / blk.y); }
+
+static int red_grid(long n) {
+    long b = (n + 1023) / 1024;
+    if (b < 1) b = 1;
+    if (b > RED_BLOCKS) b = RED_BLOCKS;
+    return (int)b;
+}
+
+// coarse-level mobility quantities for the current fine w
+static int build_hierarchy(impec_ctx *ctx) {
+    dim3 blk(64, 4);
+    for (int l = 1; l < ctx->nlev; l++) {
+        const Level &F = ctx->lev[l - 1];
+        Level &Cl = ctx->lev[l];
+        const double *wf;
+        int ldf;
+        if (l == 1) {
+            wf = ctx->w + (long)ctx->B * ctx->lds + ctx->B;
+            ldf = ctx->lds;
+        } else {
+            wf = F.w + F.ld + 1;
+            ldf = F.ld;
+        }
+        hipLaunchKernelGGL(k_coarsen_w, grid2(Cl.n1 + 2, Cl.n2 + 2, blk), blk, 0, ctx->stream, wf, ldf, F.n1, F.n2,
+                           Cl.w + Cl.ld + 1, Cl.ld, Cl.n1, Cl.n2);
+    }
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+template <bool DIAG>
+static void smooth(impec_ctx *ctx, int l, int color) {
+    const Level &L = ctx->lev[l];
+    LevArgs a = lev_args(ctx, l);
+    dim3 blk(64, 4), grd;
+    if (DIAG) grd = grid2((L.n1 + 1) / 2, (L.n2 + 1) / 2, blk);
+    else grd = grid2(L.n1, (L.n2 + 1) / 2, blk);
+    hipLaunchKernelGGL(k_smooth<DIAG>, grd, blk, 0, ctx->stream, a, L.x + L.ld + 1, L.b + L.ld + 1, color);
+}
+
+template <bool DIAG>
+static int vcycle(impec_ctx *ctx) {
+    const int nc = DIAG ? 4 : 2;
+    const int npre = ctx->P.mg_pre > 0 ? ctx->P.mg_pre : 1;
+    const int npost = ctx->P.mg_post > 0 ? ctx->P.mg_post : 1;
+    dim3 blk(64, 4);
+    const int L = ctx->nlev;
+    prof_begin(ctx);
+    for (int l = 0; l < L - 1; l++) {
+        Level &F = ctx->lev[l];
+        Level &Cl = ctx->lev[l + 1];
+        const long nx = (long)(F.n1 + 2) * F.ld;
+        hipLaunchKernelGGL(k_fill_zero, dim3((int)((nx + 255) / 256 < 4096 ? (nx + 255) / 256 : 4096)), dim3(256), 0,
+                           ctx->stream, F.x, nx);
+        for (int s = 0; s < npre; s++)
+            for (int col = 0; col < nc; col++) smooth<DIAG>(ctx, l, col);
+        LevArgs a = lev_args(ctx, l);
+        hipLaunchKernelGGL(k_residual<DIAG>, grid2(F.n1, F.n2, blk), blk, 0, ctx->stream, a, F.x + F.ld + 1,
+                           F.b + F.ld + 1, F.r + F.ld + 1);
+        hipLaunchKernelGGL(k_restrict, grid2(Cl.n1, Cl.n2, blk), blk, 0, ctx->stream, F.r + F.ld + 1, F.n1, F.n2,
+                           F.ld, Cl.b + Cl.ld + 1, Cl.n1, Cl.n2, Cl.ld);
+    }
+    {
+        Level &Cl = ctx->lev[L - 1];
+        LevArgs a = lev_args(ctx, L - 1);
+        hipLaunchKernelGGL(k_coarsest<DIAG>, dim3(1), dim3(64), 0, ctx->stream, a, Cl.x + Cl.ld + 1, Cl.b + Cl.ld + 1,
+                           50);
+    }
+    for (int l = L - 2; l >= 0; l--) {
+        Level &F = ctx->lev[l];
+        Level &Cl = ctx->lev[l + 1];
+        hipLaunchKernelGGL(k_prolong, grid2(F.n1, F.n2, blk), blk, 0, ctx->stream, F.x + F.ld + 1, F.n1, F.n2, F.ld,
+                           Cl.x + Cl.ld + 1, Cl.n1, Cl.n2, Cl.ld);
+        for (int s = 0; s < npost; s++)
+            for (int col = nc - 1; col >= 0; col--) smooth<DIAG>(ctx, l, col);
+    }
+    prof_end(ctx, "vcycle");
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+static int fetch_scal(impec_ctx *ctx) {
+    IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+    IMPEC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return IMPEC_OK;
+}
+
+__global__ void k_zero_interior(CgArgs c, double *__restrict__ p) {
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        p[(long)(i + c.B) * c.ldp + (j + c.B)] = 0.;
+    }
+}
+
+__global__ void k_set_pin(CgArgs c, double *__restrict__ p) {
+    p[(long)(c.pin_i + c.B) * c.ldp + (c.pin_j + c.B)] = c.ppin;
+}
+
+template <bool DIAG>
+static int solve_impl(impec_ctx *ctx, int *its_out, double *relres_out) {
+    int st = launch_mobility(ctx);
+    if (st) return st;
+    st = build_hierarchy(ctx);
+    if (st) return st;
+    const Level &L0 = ctx->lev[0];
+    LevArgs a0 = lev_args(ctx, 0);
+    CgArgs c;
+    c.n1 = ctx->N1; c.n2 = ctx->N2; c.ld = L0.ld; c.ldp = ctx->lds; c.B = ctx->B;
+    c.n = (long)ctx->N1 * ctx->N2;
+    c.pin_i = ctx->pin_i; c.pin_j = ctx->pin_j;
+    c.shift = ctx->symmetric;
+    c.dpin = ctx->kappa / ctx->dA;           // pinned diagonal kappa/dA (MGSolver.cpp:555)
+    c.ppin = -ctx->q_pin / c.dpin;           // its solution for b = -q
+    c.bsum = ctx->bsum;
+    const int nb = red_grid(c.n);
+    hipStream_t s = ctx->stream;
+
+    if (ctx->P.initial_guess != IMPEC_GUESS_PREVIOUS || !ctx->have_prev_p)
+        hipLaunchKernelGGL(k_zero_interior, dim3(nb), dim3(256), 0, s, c, ctx->p);
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_cg_init<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q, ctx->pr, ctx->part);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    if ((st = finish_sum(ctx, nb, 2, S_RR))) return st;
+    prof_end(ctx, "cg_init");
+    if ((st = fetch_scal(ctx))) return st;
+    const double tol = ctx->P.eps;
+    const double bb = ctx->bnorm * ctx->bnorm;
+    const int maxit = ctx->P.max_iterations > 0 ? ctx->P.max_iterations : 10000;
+    double rr = ctx->h_scal[S_RR];
+    int its = 0, status = IMPEC_OK;
+    while (!(rr <= tol * tol * bb)) {
+        if (its >= maxit) {
+            status = IMPEC_ENOCONV;
+            break;
+        }
+        if (!(rr == rr)) {
+            status = IMPEC_ENAN;
+            break;
+        }
+        prof_begin(ctx);
+        hipLaunchKernelGGL(k_cg_proj_in, dim3(nb), dim3(256), 0, s, c, ctx->pr, L0.b, ctx->scal);
+        prof_end(ctx, "cg_vector");
+        if ((st = vcycle<DIAG>(ctx))) return st;
+        prof_begin(ctx);
+        hipLaunchKernelGGL(k_sum, dim3(nb), dim3(256), 0, s, c, L0.x, ctx->part);
+        if ((st = finish_sum(ctx, nb, 1, S_ZSUM))) return st;
+        hipLaunchKernelGGL(k_cg_z, dim3(nb), dim3(256), 0, s, c, L0.x, ctx->pr, ctx->pz, ctx->scal, ctx->part);
+        if ((st = finish_sum(ctx, nb, 1, S_RHO))) return st;
+        hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(256), 0, s, c, ctx->pz, ctx->pd, ctx->scal, its == 0);
+        prof_end(ctx, "cg_vector");
+        prof_begin(ctx);
+        hipLaunchKernelGGL(k_cg_matvec<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->pd, ctx->pq, ctx->part);
+        if ((st = finish_sum(ctx, nb, 1, S_DQ))) return st;
+        prof_end(ctx, "cg_matvec");
+        prof_begin(ctx);
+        hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(256), 0, s, c, ctx->p, ctx->pd, ctx->pr, ctx->pq, ctx->scal,
+                           ctx->part);
+        if ((st = finish_sum(ctx, nb, 2, S_RR))) return st;
+        hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, s, ctx->scal, (int)S_RHO_OLD, (int)S_RHO);
+        prof_end(ctx, "cg_vector");
+        IMPEC_HIP_CHECK(ctx, hipGetLastError());
+        if ((st = fetch_scal(ctx))) return st;
+        rr = ctx->h_scal[S_RR];
+        its++;
+    }
+    if (c.shift) {
+        hipLaunchKernelGGL(k_read_pin, dim3(1), dim3(1), 0, s, c, ctx->p, ctx->scal);
+        hipLaunchKernelGGL(k_shift, dim3(nb), dim3(256), 0, s, c, ctx->p, ctx->scal);
+    } else {
+        hipLaunchKernelGGL(k_set_pin, dim3(1), dim3(1), 0, s, c, ctx->p);
+    }
+    hipLaunchKernelGGL(k_true_residual<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q, ctx->part);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    if ((st = finish_sum(ctx, nb, 1, S_TT))) return st;
+    if ((st = fetch_scal(ctx))) return st;
+    const double relres = ctx->bnorm > 0 ? sqrt(ctx->h_scal[S_TT]) / ctx->bnorm : 0.;
+    ctx->have_prev_p = 1;
+    ctx->stats.last_its = its;
+    ctx->stats.last_vcycles = its;
+    ctx->stats.last_relres = relres;
+    ctx->stats.total_its += its;
+    ctx->stats.calls += 1;
+    if (its_out) *its_out = its;
+    if (relres_out) *relres_out = relres;
+    if (status == IMPEC_ENOCONV)
+        return set_error(ctx, status, "pressure solve: no convergence after %d iterations (relres %.3e)", its, relres);
+    if (status == IMPEC_ENAN) return set_error(ctx, status, "pressure solve: non-finite residual");
+    return IMPEC_OK;
+}
+
+int pressure_solve(impec_ctx *ctx, int *its, double *relres) {
+    if (ctx->P.nu != 1.) return solve_impl<true>(ctx, its, relres);
+    return solve_impl<false>(ctx, its, relres);
+}
+
+int pressure_operator_dump(impec_ctx *ctx, double *A9_host) {
+    int st = launch_mobility(ctx);
+    if (st) return st;
+    LevArgs a0 = lev_args(ctx, 0);
+    const size_t bytes = sizeof(double) * 9 * (size_t)ctx->N1 * ctx->N2;
+    double *dA = nullptr;
+    IMPEC_HIP_CHECK(ctx, hipMalloc(&dA, bytes));
+    dim3 blk(64, 4);
+    const double dpin = ctx->kappa / ctx->dA;
+    if (ctx->P.nu != 1.)
+        hipLaunchKernelGGL(k_op_dump<true>, grid2(ctx->N1, ctx->N2, blk), blk, 0, ctx->stream, a0, ctx->pin_i,
+                           ctx->pin_j, dpin, dA);
+    else
+        hipLaunchKernelGGL(k_op_dump<false>, grid2(ctx->N1, ctx->N2, blk), blk, 0, ctx->stream, a0, ctx->pin_i,
+                           ctx->pin_j, dpin, dA);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(A9_host, dA, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(dA);
+    if (e != hipSuccess) return set_error(ctx, IMPEC_EHIP, "operator dump: %s", hipGetErrorString(e));
+    return IMPEC_OK;
+}
+
+}  // namespace impec

@@ -1,0 +1,243 @@
+// k_transport.hip -- dimension-split transport sweeps: MUSCL-Hancock with the
+// ratio-form limiter and upwind Riemann flux (MUSCLSplitSolver.cpp:54-189),
+// first-order upwind (UpwindSplitSolver.cpp:43-65), conservative updates
+// (HyperbolicSplitSolver.cpp:113-159) and the splitting modes
+// (HyperbolicSplitSolver.cpp:51-106).  FP64.
+//
+// Sweeps are out of place: cells a sweep updates are written from the input
+// state, every other cell of the ghosted array is copied, which equals the
+// reference's in-place update because its fluxes are all formed before the
+// update loop.
+#include "impec_device.h"
+
+namespace impec {
+
+struct SweepConst {
+    int N1, N2, B, lds, ldv;
+    double hdtdx;   // 0.5 * dt / dx   (MUSCLSplitSolver.cpp:118)
+    double dtdx;    // dt / dx         (HyperbolicSplitSolver.cpp:131)
+    double phi;     // porosity
+};
+
+template <bool PHI1>
+__device__ __forceinline__ double flux(double vel, double conc, double phi) {
+    // HyperbolicSolver::Flux (HyperbolicSolver.cpp:208-210); x / 1.0 == x
+    return PHI1 ? vel * conc : vel * conc / phi;
+}
+
+template <bool PHI1>
+__device__ __forceinline__ double upwind(double speed, double UL, double UR, double phi) {
+    return (speed > 0) ? flux<PHI1>(speed, UL, phi) : flux<PHI1>(speed, UR, phi);
+}
+
+// Reconstruct + GetSlope + SlopeLimiter (MUSCLSplitSolver.cpp:54-93).  The
+// limiter min(2r/(1+r), 2/(1+r)) is evaluated with one division: both
+// candidates share the rounded denominator 1+r and correctly rounded division
+// is monotone in the numerator, so the smaller quotient is the one whose
+// numerator min(2r, 2) is smaller -- bit-identical to evaluating both.
+__device__ __forceinline__ void reconstruct(double cm, double c0, double cp, double &UL, double &UR) {
+    const double dprev = c0 - cm, dnext = cp - c0;
+    double lim = 0.;
+    if (dprev * dnext > 0) {
+        double r = dprev / dnext;
+        if (r > DBL_MAX) r = DBL_MAX;
+        const double num = r < 1. ? 2. * r : 2.;
+        lim = num / (1. + r);
+    }
+    const double d = lim * 0.5 * (dprev + dnext);
+    UR = c0 + 0.5 * d;
+    UL = c0 - 0.5 * d;
+}
+
+// half-step evolved interface states of cell k: left state of face k+1 and
+// right state of face k
+template <bool PHI1>
+__device__ __forceinline__ void hancock(double cm, double c0, double cp, double vl, double vr, const SweepConst &k,
+                                        double &left_of_next, double &right_of_this) {
+    double UL, UR;
+    reconstruct(cm, c0, cp, UL, UR);
+    const double df = k.hdtdx * (flux<PHI1>(vl, UL, k.phi) - flux<PHI1>(vr, UR, k.phi));
+    left_of_next = UR + df;
+    right_of_this = UL + df;
+}
+
+// X sweep (Advect_X): cells i in [0, N1), j in [-B, N2+B) are updated.
+template <bool MUSCL, bool PHI1>
+__global__ void k_sweep_x(SweepConst k, const double *__restrict__ cin, double *__restrict__ cout,
+                          const double *__restrict__ u) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x - k.B;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y - k.B;
+    if (i >= k.N1 + k.B || j >= k.N2 + k.B) return;
+    const long o = (long)(i + k.B) * k.lds + (j + k.B);
+    if (i < 0 || i >= k.N1) {
+        cout[o] = cin[o];
+        return;
+    }
+    const long ov = (long)(i + k.B) * k.ldv + (j + k.B);
+    const long L = k.lds, LV = k.ldv;
+    double Fi, Fip1;
+    if (MUSCL) {
+        const double cm2 = cin[o - 2 * L], cm1 = cin[o - L], c0 = cin[o], cp1 = cin[o + L], cp2 = cin[o + 2 * L];
+        const double um1 = u[ov - LV], u0 = u[ov], up1 = u[ov + LV], up2 = u[ov + 2 * LV];
+        double lA, rA, lB, rB, lC, rC;
+        hancock<PHI1>(cm2, cm1, c0, um1, u0, k, lA, rA);   // cell i-1
+        hancock<PHI1>(cm1, c0, cp1, u0, up1, k, lB, rB);   // cell i
+        hancock<PHI1>(c0, cp1, cp2, up1, up2, k, lC, rC);  // cell i+1
+        (void)rA; (void)lC;
+        Fi = upwind<PHI1>(u0, lA, rB, k.phi);
+        Fip1 = upwind<PHI1>(up1, lB, rC, k.phi);
+    } else {
+        Fi = upwind<PHI1>(u[ov], cin[o - L], cin[o], k.phi);
+        Fip1 = upwind<PHI1>(u[ov + LV], cin[o], cin[o + L], k.phi);
+    }
+    cout[o] = cin[o] + k.dtdx * (Fi - Fip1);
+}
+
+// Y sweep (Advect_Y): cells i in [-B, N1+B), j in [0, N2) are updated.
+template <bool MUSCL, bool PHI1>
+__global__ void k_sweep_y(SweepConst k, const double *__restrict__ cin, double *__restrict__ cout,
+                          const double *__restrict__ v) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x - k.B;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y - k.B;
+    if (i >= k.N1 + k.B || j >= k.N2 + k.B) return;
+    const long o = (long)(i + k.B) * k.lds + (j + k.B);
+    if (j < 0 || j >= k.N2) {
+        cout[o] = cin[o];
+        return;
+    }
+    const long ov = (long)(i + k.B) * k.ldv + (j + k.B);
+    double Gj, Gjp1;
+    if (MUSCL) {
+        const double cm2 = cin[o - 2], cm1 = cin[o - 1], c0 = cin[o], cp1 = cin[o + 1], cp2 = cin[o + 2];
+        const double vm1 = v[ov - 1], v0 = v[ov], vp1 = v[ov + 1], vp2 = v[ov + 2];
+        double lA, rA, lB, rB, lC, rC;
+        hancock<PHI1>(cm2, cm1, c0, vm1, v0, k, lA, rA);
+        hancock<PHI1>(cm1, c0, cp1, v0, vp1, k, lB, rB);
+        hancock<PHI1>(c0, cp1, cp2, vp1, vp2, k, lC, rC);
+        (void)rA; (void)lC;
+        Gj = upwind<PHI1>(v0, lA, rB, k.phi);
+        Gjp1 = upwind<PHI1>(vp1, lB, rC, k.phi);
+    } else {
+        Gj = upwind<PHI1>(v[ov], cin[o - 1], cin[o], k.phi);
+        Gjp1 = upwind<PHI1>(v[ov + 1], cin[o], cin[o + 1], k.phi);
+    }
+    cout[o] = cin[o] + k.dtdx * (Gj - Gjp1);
+}
+
+// c = 0.5 * (a + b) over the whole ghosted array (HyperbolicSplitSolver.cpp:97-101)
+__global__ void k_average(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c, int rows,
+                          int cols, int ld) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= rows || j >= cols) return;
+    const long o = (long)i * ld + j;
+    c[o] = 0.5 * (a[o] + b[o]);
+}
+
+__global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int rows, int cols, int ld) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= rows || j >= cols) return;
+    const long o = (long)i * ld + j;
+    b[o] = a[o];
+}
+
+static SweepConst sweep_const(const impec_ctx *ctx, double dt, double h) {
+    SweepConst k;
+    k.N1 = ctx->N1; k.N2 = ctx->N2; k.B = ctx->B; k.lds = ctx->lds; k.ldv = ctx->ldv;
+    k.hdtdx = 0.5 * dt / h;
+    k.dtdx = dt / h;
+    k.phi = ctx->P.porosity;
+    return k;
+}
+
+template <bool MUSCL, bool PHI1>
+static void launch_x(impec_ctx *ctx, const SweepConst &k, const double *cin, double *cout) {
+    dim3 blk(64, 4), grd((k.N2 + 2 * k.B + 63) / 64, (k.N1 + 2 * k.B + 3) / 4);
+    hipLaunchKernelGGL((k_sweep_x<MUSCL, PHI1>), grd, blk, 0, ctx->stream, k, cin, cout, ctx->u);
+}
+template <bool MUSCL, bool PHI1>
+static void launch_y(impec_ctx *ctx, const SweepConst &k, const double *cin, double *cout) {
+    dim3 blk(64, 4), grd((k.N2 + 2 * k.B + 63) / 64, (k.N1 + 2 * k.B + 3) / 4);
+    hipLaunchKernelGGL((k_sweep_y<MUSCL, PHI1>), grd, blk, 0, ctx->stream, k, cin, cout, ctx->v);
+}
+
+int advect_x(impec_ctx *ctx, const double *cin, double *cout, double dt) {
+    SweepConst k = sweep_const(ctx, dt, ctx->dx);
+    const bool muscl = ctx->P.hyper_method == IMPEC_HYPER_MUSCL, phi1 = ctx->P.porosity == 1.;
+    prof_begin(ctx);
+    if (muscl && phi1) launch_x<true, true>(ctx, k, cin, cout);
+    else if (muscl) launch_x<true, false>(ctx, k, cin, cout);
+    else if (phi1) launch_x<false, true>(ctx, k, cin, cout);
+    else launch_x<false, false>(ctx, k, cin, cout);
+    prof_end(ctx, "sweep_x");
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+int advect_y(impec_ctx *ctx, const double *cin, double *cout, double dt) {
+    SweepConst k = sweep_const(ctx, dt, ctx->dy);
+    const bool muscl = ctx->P.hyper_method == IMPEC_HYPER_MUSCL, phi1 = ctx->P.porosity == 1.;
+    prof_begin(ctx);
+    if (muscl && phi1) launch_y<true, true>(ctx, k, cin, cout);
+    else if (muscl) launch_y<true, false>(ctx, k, cin, cout);
+    else if (phi1) launch_y<false, true>(ctx, k, cin, cout);
+    else launch_y<false, false>(ctx, k, cin, cout);
+    prof_end(ctx, "sweep_y");
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+int copy_scalar(impec_ctx *ctx, const double *src, double *dst) {
+    const int rows = ctx->N1 + 2 * ctx->B, cols = ctx->N2 + 2 * ctx->B;
+    dim3 blk(64, 4), grd((cols + 63) / 64, (rows + 3) / 4);
+    hipLaunchKernelGGL(k_copy, grd, blk, 0, ctx->stream, src, dst, rows, cols, ctx->lds);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+// AdvectSplit (HyperbolicSplitSolver.cpp:51-106) after its FillBoundaries
+int advect_split(impec_ctx *ctx, double dt) {
+    int st = launch_fill_scalar(ctx, ctx->c, ctx->P.is_test);
+    if (st) return st;
+    double *c = ctx->c, *t1 = ctx->t1, *t2 = ctx->t2, *t3 = ctx->t3;
+    switch (ctx->P.dim_splitting) {
+    case IMPEC_SPLIT_ALTERNATING:
+        if (ctx->prev_first_sweep == 1) {
+            if ((st = advect_x(ctx, c, t1, dt)) || (st = advect_y(ctx, t1, c, dt))) return st;
+            ctx->prev_first_sweep = 0;
+        } else {
+            if ((st = advect_y(ctx, c, t1, dt)) || (st = advect_x(ctx, t1, c, dt))) return st;
+            ctx->prev_first_sweep = 1;
+        }
+        break;
+    case IMPEC_SPLIT_X_FIRST:
+        if ((st = advect_x(ctx, c, t1, dt)) || (st = advect_y(ctx, t1, c, dt))) return st;
+        ctx->prev_first_sweep = 0;
+        break;
+    case IMPEC_SPLIT_Y_FIRST:
+        if ((st = advect_y(ctx, c, t1, dt)) || (st = advect_x(ctx, t1, c, dt))) return st;
+        ctx->prev_first_sweep = 1;
+        break;
+    case IMPEC_SPLIT_STRANG:
+        if ((st = advect_x(ctx, c, t1, 0.5 * dt)) || (st = advect_y(ctx, t1, t2, dt)) ||
+            (st = advect_x(ctx, t2, c, 0.5 * dt)))
+            return st;
+        ctx->prev_first_sweep = 0;
+        break;
+    default: {
+        if ((st = advect_x(ctx, c, t1, dt)) || (st = advect_y(ctx, t1, t2, dt))) return st;
+        if ((st = advect_y(ctx, c, t1, dt)) || (st = advect_x(ctx, t1, t3, dt))) return st;
+        const int rows = ctx->N1 + 2 * ctx->B, cols = ctx->N2 + 2 * ctx->B;
+        dim3 blk(64, 4), grd((cols + 63) / 64, (rows + 3) / 4);
+        prof_begin(ctx);
+        hipLaunchKernelGGL(k_average, grd, blk, 0, ctx->stream, t2, t3, c, rows, cols, ctx->lds);
+        prof_end(ctx, "average");
+        IMPEC_HIP_CHECK(ctx, hipGetLastError());
+        ctx->prev_first_sweep = 0;
+    }
+    }
+    return IMPEC_OK;
+}
+
+}  // namespace impec

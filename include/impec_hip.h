@@ -1,0 +1,194 @@
+/*
+ * impec_hip.h -- C ABI of libimpec_hip.so, the gfx950 IMPEC hot path.
+ *
+ * Plain C: opaque handle, plain pointers and sizes, int status returns.
+ * Host arrays passed across this boundary use the reference's host layouts
+ * (pw359/Impec2D src/GenSimDefs.h:22-31, src/Impec2DSolver.cpp:151-182):
+ *
+ *   scalar field  double[(N1+2b)*(N2+2b)]        row-major, j fastest, (i,j) at
+ *                                                 ((i+b)*(N2+2b) + (j+b)), i,j >= -b
+ *   velocity      double[(N1+1+2b)*(N2+1+2b)*2]  component last: u = comp 0 on
+ *                                                 x-faces, v = comp 1 on y-faces
+ *   source q      double[N1*N2]                  index i*N2 + j
+ *
+ * with b = num_ghostcells (must be 2, the value every shipped config uses).
+ * Device state is authoritative: transfers happen only in impec_set_* /
+ * impec_get_*.  One host thread per context; a context owns one HIP stream.
+ * Every function returns IMPEC_OK (0) or a negative IMPEC_E* code and leaves
+ * a message for impec_last_error().  No function calls exit().
+ *
+ * Each stage entry point names the reference function it replaces.
+ */
+#ifndef IMPEC_HIP_H
+#define IMPEC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMPEC_OK 0
+#define IMPEC_EINVAL (-1)    /* bad argument or unsupported parameter        */
+#define IMPEC_EHIP (-2)      /* HIP runtime error                            */
+#define IMPEC_ENOCONV (-3)   /* linear solve hit max_iterations              */
+#define IMPEC_ENAN (-4)      /* non-finite value detected                    */
+#define IMPEC_ENOMEM (-5)    /* device allocation failed                     */
+#define IMPEC_ENODEV (-6)    /* no usable GPU                                */
+
+/* SimParams enums (pw359/Impec2D src/SimParams.h:14-27) */
+#define IMPEC_SRC_PARALLEL 0
+#define IMPEC_SRC_DIAGONAL 1
+#define IMPEC_AVG_HARMONIC 0
+#define IMPEC_AVG_MEANCONC 1
+#define IMPEC_SPLIT_ALTERNATING 0
+#define IMPEC_SPLIT_X_FIRST 1
+#define IMPEC_SPLIT_Y_FIRST 2
+#define IMPEC_SPLIT_STRANG 3
+#define IMPEC_SPLIT_PARALLEL 4
+#define IMPEC_HYPER_MUSCL 0
+#define IMPEC_HYPER_UPWIND 1
+#define IMPEC_PARA_EXPLICIT 0
+#define IMPEC_PARA_CN 1
+#define IMPEC_PREC_FP64 0    /* plain FP64 multigrid-preconditioned CG       */
+#define IMPEC_PREC_MIXED 1   /* FP32 V-cycles inside FP64 defect correction  */
+#define IMPEC_GUESS_ZERO 0   /* reference semantics (MGSolver.cpp:573-590)   */
+#define IMPEC_GUESS_PREVIOUS 1
+
+/* Parameter block.  Field meanings follow SimParams (SimParams.h:29-164);
+ * fields marked "ext" extend the reference. */
+typedef struct impec_params {
+    int N1, N2;                /* x_cells, y_cells                             */
+    int num_ghostcells;        /* must be 2                                    */
+    int source_type;           /* IMPEC_SRC_*                                  */
+    int average_procedure;     /* IMPEC_AVG_*                                  */
+    int dim_splitting;         /* IMPEC_SPLIT_*                                */
+    int hyper_method;          /* IMPEC_HYPER_*                                */
+    int diffusion;             /* hyperbolic.diffusion                         */
+    int para_method;           /* IMPEC_PARA_*                                 */
+    int is_test;               /* periodic BC, advection-only CFL              */
+    int alpha_absolute;        /* ext: alpha_l/t are absolute dispersivities   */
+    int max_iterations;        /* linear solver cap (reference: 10000)         */
+    int precision;             /* ext: IMPEC_PREC_*                            */
+    int initial_guess;         /* ext: IMPEC_GUESS_*                           */
+    int mg_pre, mg_post;       /* smoothing sweeps per V-cycle leg             */
+    int reserved_i[4];
+    double x_lb, x_ub, y_lb, y_ub;
+    double nu;                 /* hyperbolic.nu, 5-pt/rotated stencil weight   */
+    double mu_res, mu_inj;
+    double porosity, cfl, inj_fluid_conc;
+    double alpha_l, alpha_t, dt_diff_limiter;
+    double d_m;                /* ext: molecular diffusion                     */
+    double eps;                /* elliptic.eps (relative residual, 2-norm)     */
+    double para_eps;           /* parabolic.eps                                */
+    double I;                  /* general.I, injection rate                    */
+    double tstart;             /* general.starttime                            */
+    double inner_tol;          /* ext: FP32 inner tolerance (mixed precision)  */
+    double reserved_d[3];
+} impec_params;
+
+typedef struct impec_stats {
+    int last_its;              /* outer iterations of the last pressure solve  */
+    int last_vcycles;          /* V-cycles in the last pressure solve          */
+    int last_parabolic_its;    /* BiCGStab iterations, last CN solve           */
+    int total_its, calls;      /* EllipticSolver::GetAvgIts bookkeeping        */
+    int reserved_i[3];
+    double last_relres;        /* true ||b - A p|| / ||b|| of the pinned system */
+    double last_parabolic_relres;
+    double total_mass_outflux; /* HyperbolicSolver::GetTotalMassOutflux        */
+    double reserved_d[5];
+} impec_stats;
+
+typedef struct impec_ctx impec_ctx;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* Fills p with the defaults of the shipped example configs. */
+int impec_params_default(impec_params *p);
+/* Replaces the constructors of Impec2DSolver/MGSolver/MUSCLSplitSolver
+ * (Impec2DSolver.cpp:125-182, MGSolver.cpp:52-143): allocates device state,
+ * c = p = v = 0, q = 0. */
+int impec_create(const impec_params *p, impec_ctx **out);
+int impec_destroy(impec_ctx *ctx);
+const char *impec_last_error(const impec_ctx *ctx);
+const char *impec_strerror(int code);
+int impec_synchronize(impec_ctx *ctx);
+/* Number of HIP devices visible (0 when none). */
+int impec_device_count(void);
+
+/* ---- host <-> device (layouts above) ----------------------------------- */
+int impec_set_concentration(impec_ctx *ctx, const double *c);
+int impec_get_concentration(impec_ctx *ctx, double *c);
+int impec_set_pressure(impec_ctx *ctx, const double *p);
+int impec_get_pressure(impec_ctx *ctx, double *p);
+int impec_set_velocity(impec_ctx *ctx, const double *v);
+int impec_get_velocity(impec_ctx *ctx, double *v);
+/* Impec2DSolver::InitializeSource result (Impec2DSolver.cpp:241-280). */
+int impec_set_source(impec_ctx *ctx, const double *q);
+/* ext: per-cell K11, K22 on the scalar layout including ghosts (replaces the
+ * analytic K_11/K_22 of EllipticSolver.cpp:418-470); NULL restores K = 1;
+ * K22 == NULL with K11 != NULL means isotropic. */
+int impec_set_permeability(impec_ctx *ctx, const double *K11, const double *K22);
+/* Pin coefficient kappa of row (1,1) (MGSolver.cpp:528-548; 1 outside tests). */
+int impec_set_pin_kappa(impec_ctx *ctx, double kappa);
+
+/* ---- stages of one IMPEC step -------------------------------------------- */
+/* HyperbolicSolver::FillBoundaries -> FillGhostCellsNeumannScalar /
+ * FillGhostCellsPeriodicScalar (HyperbolicSolver.cpp:56-61, Util.cpp:215-324). */
+int impec_fill_boundaries(impec_ctx *ctx);
+/* EllipticSolver::Solve -> MGSolver::Solve_Elliptic_cc_to_c
+ * (EllipticSolver.cpp:133-142, MGSolver.cpp:223-248): assembles the
+ * nu-weighted 9-point operator of the ghost-filled c, solves for p (interior)
+ * to ||b - A p||_2 <= eps ||b||_2.  its/relres may be NULL. */
+int impec_pressure_solve(impec_ctx *ctx, int *its, double *relres);
+/* EllipticSolver::CalcVelocities (EllipticSolver.cpp:660-683). */
+int impec_velocities(impec_ctx *ctx);
+/* HyperbolicSolver::MaxStableTimestep (HyperbolicSolver.cpp:93-198). */
+int impec_max_stable_timestep(impec_ctx *ctx, double *dt);
+/* HyperbolicSolver::SourceSplit -> TRIntegrator::Integrate
+ * (HyperbolicSolver.cpp:201-205, TRIntegrator.cpp:49-71). */
+int impec_source_split(impec_ctx *ctx, double dt);
+/* HyperbolicSplitSolver::AdvectSplit (HyperbolicSplitSolver.cpp:51-106). */
+int impec_advect_split(impec_ctx *ctx, double dt);
+/* HyperbolicSplitSolver::Advect_X / Advect_Y (HyperbolicSplitSolver.cpp:113-159)
+ * with MUSCLSplitSolver or UpwindSplitSolver fluxes. */
+int impec_advect_x(impec_ctx *ctx, double dt);
+int impec_advect_y(impec_ctx *ctx, double dt);
+/* HyperbolicSolver::DiffusionSplitCN -> MGSolver::Solve_Parabolic_cc_to_c
+ * (HyperbolicSolver.cpp:217-223, MGSolver.cpp:952-976); dt is the split step
+ * (the caller passes dt/2 as HyperbolicSplitSolver::Solve does). */
+int impec_dispersion_cn(impec_ctx *ctx, double dt, int *its);
+/* HyperbolicSplitSolver::Solve (HyperbolicSplitSolver.cpp:162-198). */
+int impec_hyperbolic_solve(impec_ctx *ctx, double dt);
+/* One pass of the Impec2DSolver::Solve loop (Impec2DSolver.cpp:339-388):
+ * fill, pressure, velocities, dt (clipped to t_next - t when t + dt >= t_next
+ * and t_next > t), transport.  Writes the step size to *dt_out. */
+int impec_step(impec_ctx *ctx, double t, double t_next, double *dt_out);
+
+/* ---- diagnostics (Impec2DSolver.cpp:530-620) ------------------------------ */
+int impec_total_mass(impec_ctx *ctx, double *mass);
+int impec_mass_error(impec_ctx *ctx, double t, double *err);
+int impec_divergence_error(impec_ctx *ctx, double *err);
+int impec_velocity_boundary_error(impec_ctx *ctx, double *err);
+/* max(c), min(c) over the whole ghosted array (Impec2DSolver.cpp:430-435). */
+int impec_minmax_concentration(impec_ctx *ctx, double *cmax, double *cmin);
+int impec_get_stats(impec_ctx *ctx, impec_stats *st);
+
+/* ---- operator access for parity checks ----------------------------------- */
+/* SetMatrixCoefficients_Elliptic (MGSolver.cpp:268-570) for the current c:
+ * nine arrays of N1*N2 doubles, stencil order {C,W,SW,S,SE,E,NE,N,NW},
+ * including the pinned row (1,1). */
+int impec_pressure_operator(impec_ctx *ctx, double *A9);
+/* SetMatrixCoefficients_Parabolic + SetRHS_Parabolic (MGSolver.cpp:997-1370)
+ * for the current c, v and split step dt. */
+int impec_parabolic_operator(impec_ctx *ctx, double dt, double *A9, double *rhs);
+
+/* ---- measurement --------------------------------------------------------- */
+/* Per-kernel-class HIP event timing on the context stream (off by default). */
+int impec_profile_enable(impec_ctx *ctx, int on);
+/* Accumulated launches and milliseconds of one kernel class by name; returns
+ * IMPEC_EINVAL for an unknown name. */
+int impec_profile_read(impec_ctx *ctx, const char *name, long long *launches, double *ms);
+int impec_profile_reset(impec_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMPEC_HIP_H */

@@ -1,0 +1,239 @@
+"""Parity of the HIP path against the CPU oracle on the same inputs.
+
+Per-cell arithmetic (operator assembly, velocities, CFL, sources, transport
+sweeps, dispersion operator) is compared bit for bit; linear solves are
+compared to the solver tolerance (relative to max|field|).
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import pair, params, random_c, rel_err, smooth_c
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+OP_CASES = [
+    dict(nu=1.0),
+    dict(nu=2 / 3.0),
+    dict(nu=0.0),
+    dict(nu=2 / 3.0, avgproc=1),
+    dict(nu=2 / 3.0, stype=0, x_ub=2 ** 0.5, y_ub=2 ** 0.5, mu_res=41.0, mu_inj=1.0),
+    dict(nu=2 / 3.0, N1=24, N2=12),          # dx != dy
+    dict(nu=1.0, N1=17, N2=9),               # odd sizes
+]
+
+
+@pytest.mark.parametrize("kw", OP_CASES)
+def test_pressure_operator_bitexact(gpu_lib, kw):
+    p = params(**dict(dict(N1=16, N2=16), **kw))
+    _say("operator", kw)
+    o, g = pair(p)
+    c = random_c(o, seed=1)
+    g.set_concentration(c)
+    A_gpu = g.pressure_operator()
+    A_ref = o.assemble(c)
+    assert np.array_equal(A_gpu, A_ref)
+
+
+def test_pressure_operator_bitexact_anisotropic_K(gpu_lib):
+    p = params(N1=12, N2=12, nu=2 / 3.0)
+    o0 = __import__("oracle.pyoracle", fromlist=["x"]).Oracle(p)
+    rng = np.random.default_rng(5)
+    K11 = np.exp(rng.standard_normal(o0.scalar().shape))
+    K22 = np.exp(rng.standard_normal(o0.scalar().shape))
+    _say("operator anisotropic K")
+    o, g = pair(p, K11, K22)
+    c = random_c(o, seed=2)
+    g.set_concentration(c)
+    assert np.array_equal(g.pressure_operator(), o.assemble(c))
+
+
+@pytest.mark.parametrize("kw", [dict(nu=1.0), dict(nu=2 / 3.0), dict(nu=0.3, avgproc=1),
+                                dict(nu=2 / 3.0, N1=20, N2=13)])
+def test_velocities_bitexact(gpu_lib, kw):
+    p = params(**dict(dict(N1=16, N2=16), **kw))
+    _say("velocities", kw)
+    o, g = pair(p)
+    c = random_c(o, seed=3)
+    rng = np.random.default_rng(4)
+    pr = o.scalar()
+    o.interior(pr)[:] = rng.standard_normal((o.N1, o.N2))
+    g.set_concentration(c)
+    g.set_pressure(pr)
+    g.velocities()
+    v_ref = o.velocity()
+    pr_ref = pr.copy()
+    o.velocities(pr_ref, v_ref, c)
+    v_gpu = g.get_velocity()
+    b = o.b
+    # every entry the reference writes (the two never-written rows stay 0 on both sides)
+    assert np.array_equal(v_gpu, v_ref)
+    assert np.array_equal(g.get_pressure(), pr_ref)
+    # and the time step derived from them
+    assert g.max_stable_timestep() == o.timestep(v_ref)
+    del b
+
+
+def test_timestep_modes_bitexact(gpu_lib):
+    rng = np.random.default_rng(7)
+    for kw in [dict(), dict(is_test=1), dict(diffusion=1, para_cn=0, alpha_l=0.3, alpha_t=0.1)]:
+        p = params(N1=18, N2=14, **kw)
+        _say("timestep", kw)
+        o, g = pair(p)
+        v = rng.standard_normal(o.velocity().shape)
+        g.set_velocity(v)
+        assert g.max_stable_timestep() == o.timestep(v)
+
+
+@pytest.mark.parametrize("profile,stype", [("one_cell", 1), ("one_cell", 0), ("circle", 1), ("circle", 0)])
+def test_source_split_bitexact(gpu_lib, profile, stype):
+    p = params(N1=32, N2=32, stype=stype, well_rc=0.2)
+    _say("source", profile, stype)
+    o, g = pair(p)
+    q = o.source(profile)
+    c = random_c(o, seed=8)
+    g.set_source(q)
+    g.set_concentration(c)
+    dt = 1.7e-4
+    for _ in range(3):
+        g.source_split(dt)
+        o.L.or_source_split(o.ctx, c, q, dt)
+    cg = g.get_concentration()
+    assert np.array_equal(o.interior(cg), o.interior(c))
+    out_gpu, out_ref = g.stats().total_mass_outflux, o.total_mass_outflux
+    if profile == "one_cell":
+        assert out_gpu == out_ref          # one or two sinks: same summation order
+    else:
+        assert abs(out_gpu - out_ref) <= 1e-14 * abs(out_ref)
+
+
+@pytest.mark.parametrize("kw", [dict(dim_split=4), dict(dim_split=0), dict(dim_split=1), dict(dim_split=2),
+                                dict(dim_split=3), dict(dim_split=4, hyper=1), dict(dim_split=4, porosity=0.7),
+                                dict(dim_split=4, is_test=1), dict(dim_split=4, N1=21, N2=11)])
+def test_advect_split_bitexact(gpu_lib, kw):
+    p = params(**dict(dict(N1=24, N2=24), **kw))
+    _say("advect", kw)
+    o, g = pair(p)
+    c = smooth_c(o, seed=9)
+    rng = np.random.default_rng(10)
+    v = rng.standard_normal(o.velocity().shape)
+    g.set_concentration(c)
+    g.set_velocity(v)
+    dt = 0.2 / max(o.N1, o.N2) / np.abs(v).max()
+    for _ in range(2):
+        g.advect_split(dt)
+        o.advect_split(c, v, dt)
+    cg = g.get_concentration()
+    assert np.array_equal(o.interior(cg), o.interior(c))
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(alpha_abs=1, alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5),
+                                dict(N1=18, N2=10, porosity=0.8)])
+def test_parabolic_operator_bitexact(gpu_lib, kw):
+    p = params(**dict(dict(N1=16, N2=16, diffusion=1, alpha_l=0.5, alpha_t=0.2), **kw))
+    _say("parabolic operator", kw)
+    o, g = pair(p)
+    c = smooth_c(o, seed=11)
+    rng = np.random.default_rng(12)
+    v = rng.standard_normal(o.velocity().shape) + 0.5
+    o.fill_velocity(v)
+    g.set_concentration(c)
+    g.set_velocity(v)
+    dt = 3e-3
+    A, r = g.parabolic_operator(dt)
+    assert np.array_equal(A, o.parabolic_matrix(v, dt))
+    assert np.array_equal(r, o.parabolic_rhs(c, v, dt))
+
+
+def test_parabolic_solve_matches_dense(gpu_lib):
+    from oracle import pyoracle as po
+    p = params(N1=16, N2=16, diffusion=1, alpha_l=0.5, alpha_t=0.2, para_eps=1e-14)
+    _say("parabolic solve")
+    o, g = pair(p)
+    c = smooth_c(o, seed=13)
+    rng = np.random.default_rng(14)
+    v = rng.standard_normal(o.velocity().shape) + 0.5
+    o.fill_velocity(v)
+    g.set_concentration(c)
+    g.set_velocity(v)
+    dt = 3e-3
+    its = g.dispersion_cn(dt)
+    x = po.dense_solve9(o.N1, o.N2, o.parabolic_matrix(v, dt), o.parabolic_rhs(c, v, dt))
+    cg = o.interior(g.get_concentration())
+    assert its > 0
+    assert rel_err(cg.reshape(-1), x) < 1e-12
+
+
+@pytest.mark.parametrize("kw", [dict(nu=1.0), dict(nu=2 / 3.0), dict(nu=2 / 3.0, stype=0, x_ub=2 ** 0.5,
+                                                                      y_ub=2 ** 0.5),
+                                dict(nu=2 / 3.0, N1=24, N2=12), dict(nu=1.0, avgproc=1)])
+def test_pressure_solve_matches_dense(gpu_lib, kw):
+    p = params(**dict(dict(N1=24, N2=24, eps=1e-13, mu_inj=0.1), **kw))
+    _say("pressure solve", kw)
+    o, g = pair(p)
+    c = smooth_c(o, seed=15)
+    q = o.source("one_cell")
+    g.set_concentration(c)
+    g.set_source(q)
+    its, relres = g.pressure_solve()
+    p_ref = o.scalar()
+    assert o.pressure_solve_dense(c, p_ref, q) == 0
+    pg = g.get_pressure()
+    assert relres <= 1e-13
+    assert rel_err(o.interior(pg), o.interior(p_ref)) < 1e-11
+    assert its < 60
+
+
+def test_pressure_solve_circle_wells(gpu_lib):
+    p = params(N1=32, N2=32, eps=1e-13, mu_inj=0.1, well_rc=0.1, nu=2 / 3.0)
+    _say("pressure solve circle")
+    o, g = pair(p)
+    c = smooth_c(o, seed=16)
+    q = o.source("circle")
+    g.set_concentration(c)
+    g.set_source(q)
+    its, relres = g.pressure_solve()
+    p_ref = o.scalar()
+    assert o.pressure_solve_dense(c, p_ref, q) == 0
+    assert relres <= 1e-13
+    assert rel_err(o.interior(g.get_pressure()), o.interior(p_ref)) < 1e-11
+
+
+STEP_CASES = [
+    dict(nu=1.0),
+    dict(nu=2 / 3.0),
+    dict(nu=1.0, stype=0, x_ub=2 ** 0.5, y_ub=2 ** 0.5),
+    dict(nu=2 / 3.0, diffusion=1, alpha_abs=1, alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5),
+    dict(nu=2 / 3.0, diffusion=1, alpha_l=1.0, alpha_t=1.0, mu_res=41.0, mu_inj=1.0, stype=0,
+         x_ub=2 ** 0.5, y_ub=2 ** 0.5),
+]
+
+
+@pytest.mark.parametrize("kw", STEP_CASES)
+def test_steps_1_and_10(gpu_lib, kw):
+    """Pressure, velocity and concentration after 1 and 10 IMPEC steps agree
+    with the oracle to 1e-10 relative (FP64, both solves at 1e-13)."""
+    p = params(**dict(dict(N1=32, N2=32, eps=1e-13, para_eps=1e-13, mu_inj=0.1), **kw))
+    _say("steps", kw)
+    o, g = pair(p)
+    q = o.source("one_cell")
+    c, pr, v = o.scalar(), o.scalar(), o.velocity()
+    g.set_source(q)
+    g.set_concentration(c)
+    t_ref = t_gpu = 0.0
+    for k in range(1, 11):
+        st, dt_ref = o.step(c, pr, v, q, t_ref)
+        assert st == 0
+        t_ref += dt_ref
+        dt_gpu = g.step(t_gpu)
+        t_gpu += dt_gpu
+        if k in (1, 10):
+            cg, pg, vg = g.get_concentration(), g.get_pressure(), g.get_velocity()
+            assert rel_err(o.interior(pg), o.interior(pr)) < 1e-10, k
+            assert rel_err(vg, v) < 1e-10, k
+            assert rel_err(o.interior(cg), o.interior(c)) < 1e-10, k
+            assert abs(dt_gpu - dt_ref) <= 1e-10 * dt_ref
