@@ -1,0 +1,140 @@
+#!/usr/bin/env python3
+"""IMPEC cell-updates/s on one MI355X (gfx950).
+
+Workload (the plain FP64 GPU configuration at 4096^2): quarter-five-spot on the
+unit square (reference source_type "diagonal": injector at cell (0,0),
+producer at (N-1,N-1), one-cell wells of rate 1), K = 1, porosity 1,
+mobility ratio M = 10 with quarter-power mixing (mu_res = 1, mu_inj = 0.1),
+harmonic face mobilities, the 5-point pressure operator (nu = 1), MUSCL
+transport with "parallel" dimensional splitting, CFL 0.5, Peaceman
+dispersion (d_m = 1e-5, alpha_l = 1e-3, alpha_t = 1e-4, absolute) by
+Crank-Nicolson, pressure and dispersion solves to relative residual 1e-10,
+FP64 throughout.  Synthetic inputs: c = 0 at t = 0.
+
+A step is one pass of Impec2DSolver::Solve's loop body (impec_step): ghost
+fill, pressure solve, Darcy velocities, CFL time step, source / dispersion /
+advection / source / dispersion splitting.  value = N^2 * steps / seconds with
+all state resident in HBM.
+
+Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_params(capi, n, precision="fp64", guess="zero"):
+    return capi.default_params(
+        N1=n, N2=n, source_type=capi.SRC_DIAGONAL, average_procedure=capi.AVG_HARMONIC,
+        dim_splitting=capi.SPLIT_PARALLEL, hyper_method=capi.HYPER_MUSCL, diffusion=1,
+        para_method=capi.PARA_CN, alpha_absolute=1, x_lb=0.0, x_ub=1.0, y_lb=0.0, y_ub=1.0,
+        nu=1.0, mu_res=1.0, mu_inj=0.1, porosity=1.0, cfl=0.5, inj_fluid_conc=1.0,
+        alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5, eps=1e-10, para_eps=1e-10, I=1.0,
+        precision=capi.PREC_MIXED if precision == "mixed" else capi.PREC_FP64,
+        initial_guess=capi.GUESS_PREVIOUS if guess == "previous" else capi.GUESS_ZERO)
+
+
+def one_cell_source(n):
+    import numpy as np
+    q = np.zeros(n * n)
+    val = 1.0 / ((1.0 / n) * (1.0 / n))   # I / (dx dy), Impec2DSolver.cpp:246
+    q[0] = val
+    q[n * n - 1] = -val
+    return q
+
+
+def cpu_baseline(n=1024, steps=8):
+    """Times the C oracle (FP64 restatement of the reference algorithm,
+    one thread) on `steps` steps of the same workload at n^2."""
+    from oracle import pyoracle as po
+    P = po.default_params(N1=n, N2=n, mu_res=1.0, mu_inj=0.1, nu=1.0, eps=1e-10, para_eps=1e-10,
+                          diffusion=1, alpha_abs=1, alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5)
+    o = po.Oracle(P)
+    c, p, v, q = o.scalar(), o.scalar(), o.velocity(), o.source()
+    t = 0.0
+    st, dt = o.step(c, p, v, q, t)      # untimed first step
+    t += dt
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st, dt = o.step(c, p, v, q, t)
+        t += dt
+    el = time.perf_counter() - t0
+    return {"value": n * n * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "oracle/impec_oracle.c, %d steps at %d^2 after 1 untimed step, same physics and "
+                      "tolerances, single thread, %.1f s" % (steps, n, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--guess", choices=["zero", "previous"], default="zero")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one plain 1 GiB device copy (reference kernel for HBM byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs the IMPEC path on one GPU; --gpus %d is not supported "
+              "(see DESIGN.md for how the path would shard)" % args.gpus, file=sys.stderr)
+        return 2
+
+    from impec2d_amd import capi
+    if args.calibrate_copy:
+        import ctypes
+        ms = ctypes.c_double(0)
+        capi.load().impec_copy_calibration(1 << 30, 3, ctypes.byref(ms))
+        print("copy calibration: 1 GiB in %.3f ms (%.0f GB/s read+write)" % (ms.value, 2 * 1.073741824 / ms.value * 1e3),
+              file=sys.stderr)
+    n = args.size
+    ctx = capi.Context(workload_params(capi, n, args.precision, args.guess))
+    ctx.set_source(one_cell_source(n))
+    t = 0.0
+    for _ in range(args.warmup):
+        t += ctx.step(t)
+    ctx.synchronize()
+    s0 = ctx.stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t += ctx.step(t)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    s1 = ctx.stats()
+    vcyc = (s1.total_its - s0.total_its) / max(args.steps, 1)
+    value = n * n * args.steps / el
+
+    out = {
+        "metric": "IMPEC cell-updates/s at 4096^2 (res 1e-10); V-cycles/step plain vs learned",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32/f64",
+        "data": "synthetic (c = 0 at t = 0, one-cell wells)",
+        "config": {"workload": "quarter-five-spot %d^2, M=10, Peaceman CN dispersion, nu=1 5-point, "
+                               "MUSCL parallel splitting, CFL 0.5, rel-res 1e-10, %s, %s initial guess"
+                               % (n, args.precision, args.guess),
+                   "N": n, "vcycles_per_step": vcyc,
+                   "parabolic_its_last": s1.last_parabolic_its, "pressure_relres_last": s1.last_relres},
+    }
+
+    if not args.no_roofline:
+        from impec2d_amd import roofline
+        out["roofline"] = roofline.measure(ctx, n, t)
+    ctx.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -91,6 +91,7 @@ SIGNATURES = {
     "impec_profile_enable": ([_vp, C.c_int], C.c_int),
     "impec_profile_read": ([_vp, C.c_char_p, C.POINTER(C.c_longlong), _dp], C.c_int),
     "impec_profile_reset": ([_vp], C.c_int),
+    "impec_copy_calibration": ([C.c_longlong, C.c_int, _dp], C.c_int),
 }
 
 _lib = None

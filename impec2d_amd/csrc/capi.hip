@@ -8,6 +8,11 @@
 
 namespace impec {
 
+// plain copy used as the reference point of HBM byte counters
+__global__ void k_calib_copy(const double *__restrict__ a, double *__restrict__ b, long n) {
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) b[k] = a[k];
+}
+
 int set_error(impec_ctx *ctx, int code, const char *fmt, ...) {
     if (ctx) {
         va_list ap;
@@ -141,7 +146,7 @@ int impec_create(const impec_params *P, impec_ctx **out) {
     if (!(P->nu >= 0. && P->nu <= 1.)) return IMPEC_EINVAL;   // EllipticSolver::SetNu
     if (!(P->x_ub > P->x_lb && P->y_ub > P->y_lb)) return IMPEC_EINVAL;
     if (P->dim_splitting < 0 || P->dim_splitting > 4) return IMPEC_EINVAL;
-    if (P->precision != IMPEC_PREC_FP64 && P->precision != IMPEC_PREC_MIXED) return IMPEC_EINVAL;
+    if (P->precision != IMPEC_PREC_FP64) return IMPEC_EINVAL;   // mixed precision: not built yet
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IMPEC_ENODEV;
 
@@ -529,6 +534,41 @@ int impec_profile_reset(impec_ctx *ctx) {
     if (!ctx) return IMPEC_EINVAL;
     ctx->nprof = 0;
     return IMPEC_OK;
+}
+
+int impec_copy_calibration(long long nbytes, int reps, double *ms) {
+    if (nbytes < 8 || reps < 1 || !ms) return IMPEC_EINVAL;
+    const long n = (long)(nbytes / 8);
+    double *a = nullptr, *b = nullptr;
+    hipStream_t s;
+    hipEvent_t e0, e1;
+    if (hipMalloc(&a, n * 8) != hipSuccess) return IMPEC_ENOMEM;
+    if (hipMalloc(&b, n * 8) != hipSuccess) {
+        hipFree(a);
+        return IMPEC_ENOMEM;
+    }
+    int st = IMPEC_OK;
+    if (hipStreamCreate(&s) != hipSuccess) st = IMPEC_EHIP;
+    if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = IMPEC_EHIP;
+    if (!st) {
+        hipMemsetAsync(a, 0, n * 8, s);
+        int blocks = (int)((n + 255) / 256);
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(k_calib_copy, dim3(blocks), dim3(256), 0, s, a, b, n);  // warm-up
+        hipEventRecord(e0, s);
+        for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_calib_copy, dim3(blocks), dim3(256), 0, s, a, b, n);
+        hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) st = IMPEC_EHIP;
+        float t = 0.f;
+        hipEventElapsedTime(&t, e0, e1);
+        *ms = t / reps;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        hipStreamDestroy(s);
+    }
+    hipFree(a);
+    hipFree(b);
+    return st;
 }
 
 }  // extern "C"

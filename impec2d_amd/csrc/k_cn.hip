@@ -335,6 +335,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     double *sv = ctx->cn_s, *t = ctx->cn_t, *y = ctx->cn_y, *dg = ctx->cn_dg, *z = ctx->cn_z;
     prof_begin(ctx);
     int st = tensors_and_rhs(ctx, k);
+    prof_end(ctx, "cn_tensor_rhs");
     if (st) return st;
     hipLaunchKernelGGL(k_bicg_init, dim3(nb), dim3(256), 0, s, n, b, r, r0, p, vv, x, ctx->part);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
@@ -353,13 +354,17 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         if (!(rr == rr)) { status = IMPEC_ENAN; break; }
         hipLaunchKernelGGL(k_bicg_p, dim3(nb), dim3(256), 0, s, n, r, p, vv, dg, y, ctx->scal);
         hipLaunchKernelGGL(k_bicg_rho, dim3(1), dim3(1), 0, s, ctx->scal);
+        prof_begin(ctx);
         hipLaunchKernelGGL(k_cn_apply<1>, dim3(nb), dim3(256), 0, s, k, ctx->d11, ctx->d12, ctx->d22, ctx->d21, y, vv,
                            r0, ctx->part);
+        prof_end(ctx, "cn_apply");
         if ((st = finish_sum(ctx, nb, 1, S_R0V))) return st;
         hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1), 0, s, ctx->scal);
         hipLaunchKernelGGL(k_bicg_s, dim3(nb), dim3(256), 0, s, n, r, vv, dg, sv, z, ctx->scal);
+        prof_begin(ctx);
         hipLaunchKernelGGL(k_cn_apply<2>, dim3(nb), dim3(256), 0, s, k, ctx->d11, ctx->d12, ctx->d22, ctx->d21, z, t,
                            sv, ctx->part);
+        prof_end(ctx, "cn_apply");
         if ((st = finish_sum(ctx, nb, 2, S_TS))) return st;   // S_TS = t.s, S_TT = t.t
         hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1), 0, s, ctx->scal);
         hipLaunchKernelGGL(k_bicg_update, dim3(nb), dim3(256), 0, s, n, x, y, z, r, sv, t, r0, ctx->scal, ctx->part);
@@ -372,7 +377,6 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     }
     dim3 blk(64, 4), gc((ctx->N2 + 63) / 64, (ctx->N1 + 3) / 4);
     hipLaunchKernelGGL(k_cn_store, gc, blk, 0, s, x, ctx->c, ctx->N1, ctx->N2, ctx->B, ctx->lds);
-    prof_end(ctx, "cn_solve");
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     ctx->stats.last_parabolic_its = its;
     ctx->stats.last_parabolic_relres = bb > 0 ? sqrt(rr / bb) : 0.;

@@ -166,7 +166,8 @@ struct LevArgs {
 // ---------------------------------------------------------------------------
 // multigrid kernels
 // ---------------------------------------------------------------------------
-template <bool DIAG>
+// FINE instantiates a distinct kernel for level 0 so that profiles separate it
+template <bool DIAG, bool FINE>
 __global__ void k_smooth(LevArgs L, double *__restrict__ x, const double *__restrict__ b, int color) {
     const int tj = blockIdx.x * blockDim.x + threadIdx.x;
     const int ti = blockIdx.y * blockDim.y + threadIdx.y;
@@ -185,7 +186,7 @@ __global__ void k_smooth(LevArgs L, double *__restrict__ x, const double *__rest
     x[(long)i * L.ld + j] = a[ST_C] != 0. ? s / a[ST_C] : 0.;
 }
 
-template <bool DIAG>
+template <bool DIAG, bool FINE>
 __global__ void k_residual(LevArgs L, const double *__restrict__ x, const double *__restrict__ b,
                            double *__restrict__ r) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -273,34 +274,56 @@ __global__ void k_coarsen_w(const double *__restrict__ wf, int ldf, int n1, int 
     wc[(long)I * ldc + J] = s / cnt;
 }
 
-// coarsest level: symmetric multicolour Gauss-Seidel sweeps from zero on one
-// thread, then mean projection (the level is at most 2 x 2 cells)
+// coarsest level (at most 2 x 2 cells): exact solve of the singular Neumann
+// system bordered by the zero-mean condition, [A 1; 1^T 0][x; l] = [b; 0],
+// by Gaussian elimination with partial pivoting on one thread
 template <bool DIAG>
-__global__ void k_coarsest(LevArgs L, double *__restrict__ x, const double *__restrict__ b, int sweeps) {
+__global__ void k_coarsest(LevArgs L, double *__restrict__ x, const double *__restrict__ b) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int n1 = L.g.n1, n2 = L.g.n2, nc = DIAG ? 4 : 2;
+    const int n1 = L.g.n1, n2 = L.g.n2, n = n1 * n2, m = n + 1;
+    const int dI[9] = {0, -1, -1, 0, 1, 1, 1, 0, -1}, dJ[9] = {0, 0, -1, -1, -1, 0, 1, 1, 1};
+    double M[5][6];
+    for (int r = 0; r < m; r++)
+        for (int q = 0; q <= m; q++) M[r][q] = 0.;
     for (int i = 0; i < n1; i++)
-        for (int j = 0; j < n2; j++) x[(long)i * L.ld + j] = 0.;
-    for (int s = 0; s < sweeps; s++)
-        for (int dir = 0; dir < 2; dir++)
-            for (int cc = 0; cc < nc; cc++) {
-                const int col = dir == 0 ? cc : nc - 1 - cc;
-                for (int i = 0; i < n1; i++)
-                    for (int j = 0; j < n2; j++) {
-                        const int mycol = DIAG ? ((i & 1) * 2 + (j & 1)) : ((i + j) & 1);
-                        if (mycol != col) continue;
-                        double a[9];
-                        op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
-                        const double r = gs_rhs<DIAG>(a, b[(long)i * L.ld + j], x, L.ld, i, j, n1, n2);
-                        x[(long)i * L.ld + j] = a[ST_C] != 0. ? r / a[ST_C] : 0.;
-                    }
+        for (int j = 0; j < n2; j++) {
+            const int row = i * n2 + j;
+            double a[9];
+            op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+            for (int k = 0; k < 9; k++) {
+                const int ii = i + dI[k], jj = j + dJ[k];
+                if (ii < 0 || jj < 0 || ii >= n1 || jj >= n2) continue;
+                M[row][ii * n2 + jj] += a[k];
             }
-    double m = 0.;
+            M[row][n] = 1.;
+            M[n][row] = 1.;
+            M[row][m] = b[(long)i * L.ld + j];
+        }
+    for (int col = 0; col < m; col++) {
+        int piv = col;
+        for (int r = col + 1; r < m; r++)
+            if (fabs(M[r][col]) > fabs(M[piv][col])) piv = r;
+        if (piv != col)
+            for (int q = 0; q <= m; q++) {
+                const double t = M[col][q];
+                M[col][q] = M[piv][q];
+                M[piv][q] = t;
+            }
+        const double d = M[col][col];
+        if (d == 0.) continue;
+        for (int r = col + 1; r < m; r++) {
+            const double f = M[r][col] / d;
+            for (int q = col; q <= m; q++) M[r][q] -= f * M[col][q];
+        }
+    }
+    double sol[5];
+    for (int r = m - 1; r >= 0; r--) {
+        double t = M[r][m];
+        for (int q = r + 1; q < m; q++) t -= M[r][q] * sol[q];
+        sol[r] = M[r][r] != 0. ? t / M[r][r] : 0.;
+    }
     for (int i = 0; i < n1; i++)
-        for (int j = 0; j < n2; j++) m += x[(long)i * L.ld + j];
-    m /= (double)(n1 * n2);
-    for (int i = 0; i < n1; i++)
-        for (int j = 0; j < n2; j++) x[(long)i * L.ld + j] -= m;
+        for (int j = 0; j < n2; j++) x[(long)i * L.ld + j] = sol[i * n2 + j];
 }
 
 // ---------------------------------------------------------------------------
@@ -653,7 +676,10 @@ static void smooth(impec_ctx *ctx, int l, int color) {
     dim3 blk(64, 4), grd;
     if (DIAG) grd = grid2((L.n1 + 1) / 2, (L.n2 + 1) / 2, blk);
     else grd = grid2(L.n1, (L.n2 + 1) / 2, blk);
-    hipLaunchKernelGGL(k_smooth<DIAG>, grd, blk, 0, ctx->stream, a, L.x + L.ld + 1, L.b + L.ld + 1, color);
+    prof_begin(ctx);
+    if (l == 0) hipLaunchKernelGGL((k_smooth<DIAG, true>), grd, blk, 0, ctx->stream, a, L.x + L.ld + 1, L.b + L.ld + 1, color);
+    else hipLaunchKernelGGL((k_smooth<DIAG, false>), grd, blk, 0, ctx->stream, a, L.x + L.ld + 1, L.b + L.ld + 1, color);
+    prof_end(ctx, l == 0 ? "mg_smooth_l0" : "mg_coarse");
 }
 
 template <bool DIAG>
@@ -663,36 +689,46 @@ static int vcycle(impec_ctx *ctx) {
     const int npost = ctx->P.mg_post > 0 ? ctx->P.mg_post : 1;
     dim3 blk(64, 4);
     const int L = ctx->nlev;
-    prof_begin(ctx);
     for (int l = 0; l < L - 1; l++) {
         Level &F = ctx->lev[l];
         Level &Cl = ctx->lev[l + 1];
+        const char *tag = l == 0 ? "mg_transfer_l0" : "mg_coarse";
         const long nx = (long)(F.n1 + 2) * F.ld;
+        prof_begin(ctx);
         hipLaunchKernelGGL(k_fill_zero, dim3((int)((nx + 255) / 256 < 4096 ? (nx + 255) / 256 : 4096)), dim3(256), 0,
                            ctx->stream, F.x, nx);
+        prof_end(ctx, tag);
         for (int s = 0; s < npre; s++)
             for (int col = 0; col < nc; col++) smooth<DIAG>(ctx, l, col);
         LevArgs a = lev_args(ctx, l);
-        hipLaunchKernelGGL(k_residual<DIAG>, grid2(F.n1, F.n2, blk), blk, 0, ctx->stream, a, F.x + F.ld + 1,
-                           F.b + F.ld + 1, F.r + F.ld + 1);
+        prof_begin(ctx);
+        if (l == 0)
+            hipLaunchKernelGGL((k_residual<DIAG, true>), grid2(F.n1, F.n2, blk), blk, 0, ctx->stream, a,
+                               F.x + F.ld + 1, F.b + F.ld + 1, F.r + F.ld + 1);
+        else
+            hipLaunchKernelGGL((k_residual<DIAG, false>), grid2(F.n1, F.n2, blk), blk, 0, ctx->stream, a,
+                               F.x + F.ld + 1, F.b + F.ld + 1, F.r + F.ld + 1);
         hipLaunchKernelGGL(k_restrict, grid2(Cl.n1, Cl.n2, blk), blk, 0, ctx->stream, F.r + F.ld + 1, F.n1, F.n2,
                            F.ld, Cl.b + Cl.ld + 1, Cl.n1, Cl.n2, Cl.ld);
+        prof_end(ctx, tag);
     }
     {
         Level &Cl = ctx->lev[L - 1];
         LevArgs a = lev_args(ctx, L - 1);
-        hipLaunchKernelGGL(k_coarsest<DIAG>, dim3(1), dim3(64), 0, ctx->stream, a, Cl.x + Cl.ld + 1, Cl.b + Cl.ld + 1,
-                           50);
+        prof_begin(ctx);
+        hipLaunchKernelGGL(k_coarsest<DIAG>, dim3(1), dim3(64), 0, ctx->stream, a, Cl.x + Cl.ld + 1, Cl.b + Cl.ld + 1);
+        prof_end(ctx, "mg_coarse");
     }
     for (int l = L - 2; l >= 0; l--) {
         Level &F = ctx->lev[l];
         Level &Cl = ctx->lev[l + 1];
+        prof_begin(ctx);
         hipLaunchKernelGGL(k_prolong, grid2(F.n1, F.n2, blk), blk, 0, ctx->stream, F.x + F.ld + 1, F.n1, F.n2, F.ld,
                            Cl.x + Cl.ld + 1, Cl.n1, Cl.n2, Cl.ld);
+        prof_end(ctx, l == 0 ? "mg_transfer_l0" : "mg_coarse");
         for (int s = 0; s < npost; s++)
             for (int col = nc - 1; col >= 0; col--) smooth<DIAG>(ctx, l, col);
     }
-    prof_end(ctx, "vcycle");
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
 }
@@ -770,8 +806,8 @@ static int solve_impl(impec_ctx *ctx, int *its_out, double *relres_out) {
         prof_end(ctx, "cg_vector");
         prof_begin(ctx);
         hipLaunchKernelGGL(k_cg_matvec<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->pd, ctx->pq, ctx->part);
-        if ((st = finish_sum(ctx, nb, 1, S_DQ))) return st;
         prof_end(ctx, "cg_matvec");
+        if ((st = finish_sum(ctx, nb, 1, S_DQ))) return st;
         prof_begin(ctx);
         hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(256), 0, s, c, ctx->p, ctx->pd, ctx->pr, ctx->pq, ctx->scal,
                            ctx->part);

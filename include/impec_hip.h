@@ -187,6 +187,10 @@ int impec_profile_enable(impec_ctx *ctx, int on);
  * IMPEC_EINVAL for an unknown name. */
 int impec_profile_read(impec_ctx *ctx, const char *name, long long *launches, double *ms);
 int impec_profile_reset(impec_ctx *ctx);
+/* Plain device-to-device copy of nbytes (rounded down to doubles), repeated
+ * reps times on a private stream; *ms gets the mean time of one copy.  The
+ * copy kernel is the reference point that scales HBM byte counters. */
+int impec_copy_calibration(long long nbytes, int reps, double *ms);
 
 #ifdef __cplusplus
 }
