@@ -178,6 +178,7 @@ int impec_create(const impec_params *P, impec_ctx **out) {
     ctx->mob1over4 = pow(P->mu_res / P->mu_inj, 0.25);
     ctx->kappa = 1.;
     ctx->pin_i = 1; ctx->pin_j = 1;   // MGSolver.cpp:109-118
+    ctx->force_unfused = P->mg_variant == 1;
     ctx->prev_first_sweep = 1;        // HyperbolicSolver.cpp:84
     ctx->lds = round_up(ctx->N2 + 2 * ctx->B, 16);
     ctx->ldv = round_up(ctx->N2 + 1 + 2 * ctx->B, 16);

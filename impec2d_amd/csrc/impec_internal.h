@@ -10,13 +10,13 @@ namespace impec {
 
 constexpr int GHOST = 2;          // num_ghostcells of every shipped config
 constexpr int MAX_LEVELS = 24;
-constexpr int RED_BLOCKS = 1024;  // max partial sums per reduction pass
+constexpr int RED_BLOCKS = 65536; // max partial sums per reduction pass
 
 // Scalars kept in device memory so that solver loops need no host round trip
 // except the convergence test.
 enum Scal {
     S_RHO = 0, S_RHO_OLD = 1, S_DQ = 2, S_RR = 3, S_RSUM = 4, S_ZSUM = 5, S_ALPHA = 6, S_BETA = 7,
-    S_OMEGA = 8, S_TS = 9, S_TT = 10, S_R0V = 11, S_XPIN = 12,
+    S_OMEGA = 8, S_TS = 9, S_TT = 10, S_R0V = 11, S_XPIN = 12, S_RX = 14,
     S_MASS = 16, S_DIV = 17, S_NOFLOW = 18,          // written together by the diagnostics pass
     S_MAXVEL = 20, S_MINDT = 21, S_MINDT_DIFF = 22,  // CFL pass
     S_OUTFLUX = 24, S_TMO = 25, S_CMAX = 26, S_CMIN = 27,
@@ -82,6 +82,7 @@ struct impec_ctx {
     double *part;              // reduction partials
     double *h_scal;            // pinned host mirror of scal
     int symmetric;             // shift formulation usable
+    int force_unfused;         // use the unfused multigrid kernels (testing)
     int pin_i, pin_j;
     double bnorm, bsum;        // ||-q||_2 and sum(-q), exact host sums
     double q_pin;              // q at the pinned cell

@@ -226,7 +226,7 @@ __global__ void k_finish(const double *__restrict__ part, int nblocks, int nv, d
 
 template <int OP>
 static int finish_op(impec_ctx *ctx, int nblocks, int nv, int dst) {
-    hipLaunchKernelGGL(k_finish<OP>, dim3(1), dim3(256), 0, ctx->stream, ctx->part, nblocks, nv, ctx->scal, dst);
+    hipLaunchKernelGGL(k_finish<OP>, dim3(1), dim3(1024), 0, ctx->stream, ctx->part, nblocks, nv, ctx->scal, dst);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
 }

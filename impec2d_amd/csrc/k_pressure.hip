@@ -752,8 +752,525 @@ __global__ void k_set_pin(CgArgs c, double *__restrict__ p) {
     p[(long)(c.pin_i + c.B) * c.ldp + (c.pin_j + c.B)] = c.ppin;
 }
 
+// ===========================================================================
+// Fused V-cycle for the 5-point operator (nu = 1): one kernel per leg and
+// level, temporally blocked in LDS.  A workgroup owns a TI x TJ tile of the
+// level and recomputes a halo, so each leg reads its inputs from HBM once:
+//   down: red-black Gauss-Seidel from a zero guess, residual, full weighting
+//   up:   bilinear prolongation + correction, black-red Gauss-Seidel
+// Colour 0 is (i + j) even.  Face mobilities are formed once per tile.
+// ===========================================================================
+constexpr int TI = 16, TJ = 64, NT = 256;
+
+struct FusedArgs {
+    OpGeom g;
+    const double *w;    // mobility quantity at cell (0,0), pitch ldw, ring >= 1
+    int ldw;
+    int ld;             // pitch of this level's x/b (ring-1 arrays at cell (0,0))
+    const double *b;    // level rhs (levels >= 1)
+    const double *r;    // level 0: CG residual, b = r - S[S_RSUM]/n
+    const double *S;
+    double inv_n;
+    double *x;          // this level's x (down: written; up: read)
+    double *xo;         // up: corrected, post-smoothed x (a different array,
+                        // since neighbouring tiles read x in their halos)
+    const double *xc;   // up: coarse-level solution at (0,0)
+    double *bc;         // down: coarse-level b at (0,0)
+    int m1, m2, ldc;    // coarse level
+    double *part;       // up at level 0: partials (sum x, sum r x)
+};
+
+__device__ __forceinline__ bool in_dom(int i, int j, int n1, int n2) {
+    return i >= 0 && j >= 0 && i < n1 && j < n2;
+}
+
+// folded 5-point row from the four face mobilities (same arithmetic as op_row<false>)
+__device__ __forceinline__ void row5(const OpGeom &g, double fxm, double fxp, double fym, double fyp, int i, int j,
+                                     double a[9]) {
+    a[ST_C] = g.mcy * (fym + fyp) - g.cx * (fxm + fxp);
+    a[ST_W] = g.cx * fxm; a[ST_SW] = 0.; a[ST_S] = g.cy * fym; a[ST_SE] = 0.;
+    a[ST_E] = g.cx * fxp; a[ST_NE] = 0.; a[ST_N] = g.cy * fyp; a[ST_NW] = 0.;
+    fold_row(i, j, g.n1, g.n2, a);
+}
+
+__device__ __forceinline__ double level_b(const FusedArgs &A, bool fine0, int i, int j) {
+    if (fine0) return A.r[(long)i * A.ld + j] - A.S[S_RSUM] * A.inv_n;
+    return A.b[(long)i * A.ld + j];
+}
+
+// Tile-local arrays: region [i0-H, i0+TI+H) x [j0-H, j0+TJ+H) stored with
+// pitch (TJ + 2H).  fx(i,j) is the face between rows i-1 and i, fy(i,j) the
+// face between columns j-1 and j.
+template <bool FINE0>
+__global__ void __launch_bounds__(NT) k_mg_down(FusedArgs A) {
+    constexpr int HW = 4, HB = 3, HR = 1;
+    constexpr int WW = TJ + 2 * HW, WB = TJ + 2 * HB, WR = TJ + 2 * HR;
+    constexpr int RWW = TI + 2 * HW, RB = TI + 2 * HB, RR = TI + 2 * HR;
+    constexpr int FXR = TI + 2 * HB + 1, FXC = WB;       // fx rows [i0-3, i0+TI+4)
+    constexpr int FYR = RB, FYC = WB + 1;                // fy cols [j0-3, j0+TJ+4)
+    __shared__ double sw[RWW * WW];                      // w, later the residual
+    __shared__ double sfx[FXR * FXC], sfy[FYR * FYC];
+    __shared__ double sb[RB * WB], sx[RB * WB];
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
+    const int n1 = A.g.n1, n2 = A.g.n2;
+    {
+        // every global load of the staging phase is issued before the first
+        // use, so a thread has all of them in flight at once
+        constexpr int NW = (RWW * WW + NT - 1) / NT, NB = (RB * WB + NT - 1) / NT;
+        const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
+        const double *src = FINE0 ? A.r : A.b;
+        double vw[NW], vb[NB];
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            const int k = t + q * NT;
+            const int ii = i0 - HW + k / WW, jj = j0 - HW + k % WW;
+            const bool ok = k < RWW * WW && ii >= -1 && jj >= -1 && ii <= n1 && jj <= n2;
+            vw[q] = ok ? A.w[(long)ii * A.ldw + jj] : 1.;
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int k = t + q * NT;
+            const int ii = i0 - HB + k / WB, jj = j0 - HB + k % WB;
+            vb[q] = (k < RB * WB && in_dom(ii, jj, n1, n2)) ? src[(long)ii * A.ld + jj] - mean : 0.;
+        }
+#pragma unroll
+        for (int q = 0; q < NW; q++)
+            if (t + q * NT < RWW * WW) sw[t + q * NT] = vw[q];
+#pragma unroll
+        for (int q = 0; q < NB; q++)
+            if (t + q * NT < RB * WB) {
+                sb[t + q * NT] = vb[q];
+                sx[t + q * NT] = 0.;
+            }
+    }
+    __syncthreads();
+    for (int k = t; k < FXR * FXC; k += NT) {
+        const int r = k / FXC, c = k % FXC;   // face between rows (i0-4+r) and (i0-3+r), column j0-3+c
+        sfx[k] = face_mob(sw[r * WW + c + 1], sw[(r + 1) * WW + c + 1], A.g.harmonic);
+    }
+    for (int k = t; k < FYR * FYC; k += NT) {
+        const int r = k / FYC, c = k % FYC;   // row i0-3+r, face between columns j0-4+c and j0-3+c
+        sfy[k] = face_mob(sw[(r + 1) * WW + c], sw[(r + 1) * WW + c + 1], A.g.harmonic);
+    }
+    __syncthreads();
+#define FX(r, c) sfx[(r) * FXC + (c)]
+#define FY(r, c) sfy[(r) * FYC + (c)]
+    // red from zero on [i0-3, i0+TI+3) x [j0-3, j0+TJ+3)
+    for (int k = t; k < RB * WB; k += NT) {
+        const int r = k / WB, c = k % WB, ii = i0 - HB + r, jj = j0 - HB + c;
+        if (((ii + jj) & 1) || !in_dom(ii, jj, n1, n2)) continue;
+        double a[9];
+        row5(A.g, FX(r, c), FX(r + 1, c), FY(r, c), FY(r, c + 1), ii, jj, a);
+        sx[k] = a[ST_C] != 0. ? sb[k] / a[ST_C] : 0.;
+    }
+    __syncthreads();
+    // black on [i0-2, i0+TI+2) x [j0-2, j0+TJ+2)
+    for (int k = t; k < (TI + 4) * (TJ + 4); k += NT) {
+        const int r = k / (TJ + 4) + 1, c = k % (TJ + 4) + 1, ii = i0 - HB + r, jj = j0 - HB + c;
+        if (!((ii + jj) & 1) || !in_dom(ii, jj, n1, n2)) continue;
+        double a[9];
+        row5(A.g, FX(r, c), FX(r + 1, c), FY(r, c), FY(r, c + 1), ii, jj, a);
+        const int o = r * WB + c;
+        double s = sb[o];
+        if (ii > 0) s -= a[ST_W] * sx[o - WB];
+        if (jj > 0) s -= a[ST_S] * sx[o - 1];
+        if (ii < n1 - 1) s -= a[ST_E] * sx[o + WB];
+        if (jj < n2 - 1) s -= a[ST_N] * sx[o + 1];
+        sx[o] = a[ST_C] != 0. ? s / a[ST_C] : 0.;
+    }
+    __syncthreads();
+    // residual on [i0-1, i0+TI+1) x [j0-1, j0+TJ+1) into sw
+    for (int k = t; k < RR * WR; k += NT) {
+        const int rr = k / WR, cr = k % WR, r = rr + 2, c = cr + 2, ii = i0 - HB + r, jj = j0 - HB + c;
+        double res = 0.;
+        if (in_dom(ii, jj, n1, n2)) {
+            double a[9];
+            row5(A.g, FX(r, c), FX(r + 1, c), FY(r, c), FY(r, c + 1), ii, jj, a);
+            const int o = r * WB + c;
+            double s = a[ST_C] * sx[o];
+            if (ii > 0) s += a[ST_W] * sx[o - WB];
+            if (jj > 0) s += a[ST_S] * sx[o - 1];
+            if (ii < n1 - 1) s += a[ST_E] * sx[o + WB];
+            if (jj < n2 - 1) s += a[ST_N] * sx[o + 1];
+            res = sb[o] - s;
+        }
+        sw[k] = res;
+    }
+    // smoothed x of the tile
+    for (int k = t; k < TI * TJ; k += NT) {
+        const int r = k / TJ + HB, c = k % TJ + HB, ii = i0 - HB + r, jj = j0 - HB + c;
+        if (in_dom(ii, jj, n1, n2)) A.x[(long)ii * A.ld + jj] = sx[r * WB + c];
+    }
+    __syncthreads();
+    // full weighting onto the coarse cells of this tile (one per thread)
+    {
+        const int I = i0 / 2 + t / (TJ / 2), J = j0 / 2 + t % (TJ / 2);
+        if (I < A.m1 && J < A.m2) {
+            double s = 0.;
+            for (int f = 2 * I - 1; f <= 2 * I + 2; f++) {
+                if (f < 0 || f >= n1) continue;
+                const double wi = pw(f, I, A.m1);
+                if (wi == 0.) continue;
+                double tt = 0.;
+                for (int gg = 2 * J - 1; gg <= 2 * J + 2; gg++) {
+                    if (gg < 0 || gg >= n2) continue;
+                    const double wj = pw(gg, J, A.m2);
+                    if (wj != 0.) tt += wj * sw[(f - (i0 - 1)) * WR + (gg - (j0 - 1))];
+                }
+                s += wi * tt;
+            }
+            A.bc[(long)I * A.ldc + J] = 0.25 * s;
+        }
+    }
+#undef FX
+#undef FY
+}
+
+template <bool FINE0>
+__global__ void __launch_bounds__(NT) k_mg_up(FusedArgs A) {
+    constexpr int HX = 2;
+    constexpr int WX = TJ + 2 * HX, RX = TI + 2 * HX;        // x and w region T +- 2
+    constexpr int WB = TJ + 2, RB = TI + 2;                  // black region T +- 1
+    constexpr int FXR = RB + 1, FXC = WB, FYR = RB, FYC = WB + 1;
+    __shared__ double sw[RX * WX], sx[RX * WX];
+    __shared__ double sfx[FXR * FXC], sfy[FYR * FYC], sb[RB * WB];
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
+    const int n1 = A.g.n1, n2 = A.g.n2, m1 = A.m1, m2 = A.m2;
+    {
+        constexpr int NX = (RX * WX + NT - 1) / NT, NB = (RB * WB + NT - 1) / NT;
+        const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
+        const double *src = FINE0 ? A.r : A.b;
+        double vw[NX], vx[NX], vb[NB], e00[NX], e01[NX], e10[NX], e11[NX];
+#pragma unroll
+        for (int q = 0; q < NX; q++) {
+            const int k = t + q * NT;
+            const int ii = i0 - HX + k / WX, jj = j0 - HX + k % WX;
+            const bool ok = k < RX * WX && ii >= -1 && jj >= -1 && ii <= n1 && jj <= n2;
+            vw[q] = ok ? A.w[(long)ii * A.ldw + jj] : 1.;
+            const bool in = k < RX * WX && in_dom(ii, jj, n1, n2);
+            int I0 = ii >> 1, I1 = (ii & 1) ? I0 + 1 : I0 - 1;
+            int J0 = jj >> 1, J1 = (jj & 1) ? J0 + 1 : J0 - 1;
+            I0 = I0 > m1 - 1 ? m1 - 1 : (I0 < 0 ? 0 : I0);
+            J0 = J0 > m2 - 1 ? m2 - 1 : (J0 < 0 ? 0 : J0);
+            I1 = I1 < 0 ? 0 : (I1 > m1 - 1 ? m1 - 1 : I1);
+            J1 = J1 < 0 ? 0 : (J1 > m2 - 1 ? m2 - 1 : J1);
+            vx[q] = in ? A.x[(long)ii * A.ld + jj] : 0.;
+            e00[q] = in ? A.xc[(long)I0 * A.ldc + J0] : 0.;
+            e01[q] = in ? A.xc[(long)I0 * A.ldc + J1] : 0.;
+            e10[q] = in ? A.xc[(long)I1 * A.ldc + J0] : 0.;
+            e11[q] = in ? A.xc[(long)I1 * A.ldc + J1] : 0.;
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int k = t + q * NT;
+            const int ii = i0 - 1 + k / WB, jj = j0 - 1 + k % WB;
+            vb[q] = (k < RB * WB && in_dom(ii, jj, n1, n2)) ? src[(long)ii * A.ld + jj] - mean : 0.;
+        }
+#pragma unroll
+        for (int q = 0; q < NX; q++) {
+            const int k = t + q * NT;
+            if (k >= RX * WX) continue;
+            const int ii = i0 - HX + k / WX, jj = j0 - HX + k % WX;
+            sw[k] = vw[q];
+            const double e = 0.5625 * e00[q] + 0.1875 * e01[q] + 0.1875 * e10[q] + 0.0625 * e11[q];
+            sx[k] = in_dom(ii, jj, n1, n2) ? vx[q] + e : 0.;
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++)
+            if (t + q * NT < RB * WB) sb[t + q * NT] = vb[q];
+    }
+    __syncthreads();
+    for (int k = t; k < FXR * FXC; k += NT) {
+        const int r = k / FXC, c = k % FXC;   // face between rows i0-2+r and i0-1+r, column j0-1+c
+        sfx[k] = face_mob(sw[r * WX + c + 1], sw[(r + 1) * WX + c + 1], A.g.harmonic);
+    }
+    for (int k = t; k < FYR * FYC; k += NT) {
+        const int r = k / FYC, c = k % FYC;   // row i0-1+r, face between columns j0-2+c and j0-1+c
+        sfy[k] = face_mob(sw[(r + 1) * WX + c], sw[(r + 1) * WX + c + 1], A.g.harmonic);
+    }
+    __syncthreads();
+#define FX(r, c) sfx[(r) * FXC + (c)]
+#define FY(r, c) sfy[(r) * FYC + (c)]
+    for (int pass = 0; pass < 2; pass++) {
+        // pass 0: black on T +- 1; pass 1: red on T
+        const int lo = pass == 0 ? 0 : 1, nr = pass == 0 ? RB : TI, nc = pass == 0 ? WB : TJ;
+        for (int k = t; k < nr * nc; k += NT) {
+            const int r = k / nc + lo, c = k % nc + lo, ii = i0 - 1 + r, jj = j0 - 1 + c;
+            const int col = (ii + jj) & 1;
+            if ((pass == 0 && !col) || (pass == 1 && col) || !in_dom(ii, jj, n1, n2)) continue;
+            double a[9];
+            row5(A.g, FX(r, c), FX(r + 1, c), FY(r, c), FY(r, c + 1), ii, jj, a);
+            const int o = (r + 1) * WX + (c + 1);
+            double s = sb[r * WB + c];
+            if (ii > 0) s -= a[ST_W] * sx[o - WX];
+            if (jj > 0) s -= a[ST_S] * sx[o - 1];
+            if (ii < n1 - 1) s -= a[ST_E] * sx[o + WX];
+            if (jj < n2 - 1) s -= a[ST_N] * sx[o + 1];
+            sx[o] = a[ST_C] != 0. ? s / a[ST_C] : 0.;
+        }
+        __syncthreads();
+    }
+#undef FX
+#undef FY
+    double acc[2] = {0., 0.};
+    for (int k = t; k < TI * TJ; k += NT) {
+        const int r = k / TJ, c = k % TJ, ii = i0 + r, jj = j0 + c;
+        if (!in_dom(ii, jj, n1, n2)) continue;
+        const double xv = sx[(r + HX) * WX + (c + HX)];
+        A.xo[(long)ii * A.ld + jj] = xv;
+        if (FINE0) {
+            acc[0] += xv;
+            acc[1] += A.r[(long)ii * A.ld + jj] * xv;
+        }
+    }
+    if (FINE0) {
+        block_reduce<RED_SUM, 2>(acc);
+        if (t == 0) {
+            const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+            A.part[blk] = acc[0];
+            A.part[RED_BLOCKS + blk] = acc[1];
+        }
+    }
+}
+
+// sums of the V-cycle output and rho = r.z with z = x0 - mean(x0)
+__global__ void k_finish_rho(const double *__restrict__ part, int nblocks, double inv_n, double *__restrict__ S) {
+    double v[2] = {0., 0.};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        v[0] += part[b];
+        v[1] += part[RED_BLOCKS + b];
+    }
+    block_reduce<RED_SUM, 2>(v);
+    if (threadIdx.x == 0) {
+        S[S_ZSUM] = v[0];
+        S[S_RX] = v[1];
+        S[S_RHO_OLD] = S[S_RHO];
+        S[S_RHO] = v[1] - v[0] * inv_n * S[S_RSUM];
+    }
+}
+
+// tiled 2-D launch shape for the level-0 vector kernels: 64 x 4 threads,
+// rows strided so that the grid has at most RED_BLOCKS workgroups
+struct Grid2 {
+    dim3 grid, block;
+    int nblocks;
+};
+static Grid2 vec_grid(int n1, int n2) {
+    Grid2 g;
+    g.block = dim3(64, 4);
+    const int bx = (n2 + 63) / 64;
+    int by = (n1 + 3) / 4;
+    if (by * bx > RED_BLOCKS) by = RED_BLOCKS / bx > 0 ? RED_BLOCKS / bx : 1;
+    g.grid = dim3(bx, by);
+    g.nblocks = bx * by;
+    return g;
+}
+
+// d_new = (x0 - mean) + beta d_old on the cell and its four neighbours,
+// q = A_N d_new (pinned row zeroed in the eliminated formulation), d.q partials
+__global__ void k_cg_dir_matvec(LevArgs L, CgArgs c, const double *__restrict__ x0, const double *__restrict__ dold,
+                                double *__restrict__ dnew, double *__restrict__ qv, const double *__restrict__ S,
+                                int first, double *__restrict__ part) {
+    const double mean = S[S_ZSUM] / (double)c.n;
+    const double beta = first ? 0. : S[S_RHO] / S[S_RHO_OLD];
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    double s[1] = {0.};
+    const long ld = c.ld;
+    for (int i = blockIdx.y * 4 + threadIdx.y; i < c.n1; i += gridDim.y * 4) {
+        if (j >= c.n2) break;
+        const long o = (long)(i + 1) * ld + (j + 1);
+#define DN(off_, ii_, jj_)                                                                                  \
+    ((!c.shift && (ii_) == c.pin_i && (jj_) == c.pin_j) ? 0.                                                \
+                                                         : (first ? x0[o + (off_)] - mean                   \
+                                                                  : (x0[o + (off_)] - mean) + beta * dold[o + (off_)]))
+        const double dc = DN(0, i, j);
+        double a[9];
+        op_row<false>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+        double y = a[ST_C] * dc;
+        if (i > 0) y += a[ST_W] * DN(-ld, i - 1, j);
+        if (j > 0) y += a[ST_S] * DN(-1, i, j - 1);
+        if (i < c.n1 - 1) y += a[ST_E] * DN(ld, i + 1, j);
+        if (j < c.n2 - 1) y += a[ST_N] * DN(1, i, j + 1);
+#undef DN
+        if (!c.shift && i == c.pin_i && j == c.pin_j) y = 0.;
+        dnew[o] = dc;
+        qv[o] = y;
+        s[0] += dc * y;
+    }
+    block_reduce<RED_SUM, 1>(s);
+    if (tid_linear() == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = s[0];
+}
+
+// x += alpha d, r -= alpha q; partials (r.r, sum r)
+__global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__restrict__ d, double *__restrict__ r,
+                             const double *__restrict__ qv, const double *__restrict__ S, double *__restrict__ part) {
+    const double alpha = S[S_RHO] / S[S_DQ];
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    double s[2] = {0., 0.};
+    for (int i = blockIdx.y * 4 + threadIdx.y; i < c.n1; i += gridDim.y * 4) {
+        if (j >= c.n2) break;
+        const long o = (long)(i + 1) * c.ld + (j + 1);
+        double *xp = p + (long)(i + c.B) * c.ldp + (j + c.B);
+        *xp = *xp + alpha * d[o];
+        const double rv = r[o] - alpha * qv[o];
+        r[o] = rv;
+        s[0] += rv * rv;
+        s[1] += rv;
+    }
+    block_reduce<RED_SUM, 2>(s);
+    if (tid_linear() == 0) {
+        const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+        part[blk] = s[0];
+        part[RED_BLOCKS + blk] = s[1];
+    }
+}
+
+static FusedArgs fused_args(impec_ctx *ctx, int l) {
+    FusedArgs A;
+    memset(&A, 0, sizeof(A));
+    const Level &L = ctx->lev[l];
+    LevArgs la = lev_args(ctx, l);
+    A.g = L.g;
+    A.w = la.w1;
+    A.ldw = la.ldw;
+    A.ld = L.ld;
+    A.b = L.b + L.ld + 1;
+    A.r = ctx->pr + L.ld + 1;
+    A.S = ctx->scal;
+    A.inv_n = 1.0 / ((double)ctx->N1 * ctx->N2);
+    A.x = L.x + L.ld + 1;
+    A.xo = L.r + L.ld + 1;   // the fused cycle leaves each level's result in r
+    if (l + 1 < ctx->nlev) {
+        const Level &Cl = ctx->lev[l + 1];
+        // the coarsest level is solved in place into x, the others end in r
+        A.xc = (l + 2 == ctx->nlev ? Cl.x : Cl.r) + Cl.ld + 1;
+        A.bc = Cl.b + Cl.ld + 1;
+        A.m1 = Cl.n1;
+        A.m2 = Cl.n2;
+        A.ldc = Cl.ld;
+    }
+    A.part = ctx->part;
+    return A;
+}
+
+static dim3 tile_grid(int n1, int n2) { return dim3((n2 + TJ - 1) / TJ, (n1 + TI - 1) / TI); }
+
+// one V(1,1) cycle on the level-0 rhs r - mean(r); the result x0 is left in
+// lev[0].r, with the partials of sum(x0) and r.x0 for k_finish_rho
+static int vcycle_fused(impec_ctx *ctx) {
+    const int L = ctx->nlev;
+    for (int l = 0; l < L - 1; l++) {
+        FusedArgs A = fused_args(ctx, l);
+        prof_begin(ctx);
+        if (l == 0) hipLaunchKernelGGL(k_mg_down<true>, tile_grid(A.g.n1, A.g.n2), dim3(NT), 0, ctx->stream, A);
+        else hipLaunchKernelGGL(k_mg_down<false>, tile_grid(A.g.n1, A.g.n2), dim3(NT), 0, ctx->stream, A);
+        prof_end(ctx, l == 0 ? "mg_down_l0" : "mg_coarse");
+    }
+    {
+        Level &Cl = ctx->lev[L - 1];
+        LevArgs a = lev_args(ctx, L - 1);
+        prof_begin(ctx);
+        hipLaunchKernelGGL(k_coarsest<false>, dim3(1), dim3(64), 0, ctx->stream, a, Cl.x + Cl.ld + 1, Cl.b + Cl.ld + 1);
+        prof_end(ctx, "mg_coarse");
+    }
+    for (int l = L - 2; l >= 0; l--) {
+        FusedArgs A = fused_args(ctx, l);
+        prof_begin(ctx);
+        if (l == 0) hipLaunchKernelGGL(k_mg_up<true>, tile_grid(A.g.n1, A.g.n2), dim3(NT), 0, ctx->stream, A);
+        else hipLaunchKernelGGL(k_mg_up<false>, tile_grid(A.g.n1, A.g.n2), dim3(NT), 0, ctx->stream, A);
+        prof_end(ctx, l == 0 ? "mg_up_l0" : "mg_coarse");
+    }
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+// r = b~ - A_N x recomputed from the current iterate (the CG start-up
+// kernel), with ||r||^2 returned; also (re)starts the recursion
 template <bool DIAG>
-static int solve_impl(impec_ctx *ctx, int *its_out, double *relres_out) {
+static int cg_residual(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, int nb, double *rr) {
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_cg_init<DIAG>, dim3(nb), dim3(256), 0, ctx->stream, a0, c, ctx->p, ctx->q, ctx->pr, ctx->part);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    int st = finish_sum(ctx, nb, 2, S_RR);
+    prof_end(ctx, "cg_init");
+    if (st) return st;
+    if ((st = fetch_scal(ctx))) return st;
+    *rr = ctx->h_scal[S_RR];
+    return IMPEC_OK;
+}
+
+// one CG iteration with the one-kernel-per-pass V-cycle
+template <bool DIAG>
+static int cg_iteration_unfused(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, int nb, bool first) {
+    const Level &L0 = ctx->lev[0];
+    hipStream_t s = ctx->stream;
+    int st;
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_cg_proj_in, dim3(nb), dim3(256), 0, s, c, ctx->pr, L0.b, ctx->scal);
+    prof_end(ctx, "cg_vector");
+    if ((st = vcycle<DIAG>(ctx))) return st;
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_sum, dim3(nb), dim3(256), 0, s, c, L0.x, ctx->part);
+    if ((st = finish_sum(ctx, nb, 1, S_ZSUM))) return st;
+    hipLaunchKernelGGL(k_cg_z, dim3(nb), dim3(256), 0, s, c, L0.x, ctx->pr, ctx->pz, ctx->scal, ctx->part);
+    if ((st = finish_sum(ctx, nb, 1, S_RHO))) return st;
+    hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(256), 0, s, c, ctx->pz, ctx->pd, ctx->scal, first);
+    prof_end(ctx, "cg_vector");
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_cg_matvec<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->pd, ctx->pq, ctx->part);
+    prof_end(ctx, "cg_matvec");
+    if ((st = finish_sum(ctx, nb, 1, S_DQ))) return st;
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(256), 0, s, c, ctx->p, ctx->pd, ctx->pr, ctx->pq, ctx->scal,
+                       ctx->part);
+    if ((st = finish_sum(ctx, nb, 2, S_RR))) return st;
+    hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, s, ctx->scal, (int)S_RHO_OLD, (int)S_RHO);
+    prof_end(ctx, "cg_vector");
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+// one CG iteration with the LDS-fused V-cycle (nu = 1); d ping-pongs
+static int cg_iteration_fused(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, bool first, double **dold,
+                              double **dnew) {
+    const Level &L0 = ctx->lev[0];
+    hipStream_t s = ctx->stream;
+    const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
+    const dim3 tg0 = tile_grid(ctx->N1, ctx->N2);
+    const double inv_n = 1.0 / (double)c.n;
+    int st;
+    if ((st = vcycle_fused(ctx))) return st;
+    hipLaunchKernelGGL(k_finish_rho, dim3(1), dim3(1024), 0, s, ctx->part, (int)(tg0.x * tg0.y), inv_n, ctx->scal);
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_cg_dir_matvec, vg.grid, vg.block, 0, s, a0, c, L0.r, *dold, *dnew, ctx->pq, ctx->scal,
+                       first, ctx->part);
+    prof_end(ctx, "cg_dir_matvec");
+    if ((st = finish_sum(ctx, vg.nblocks, 1, S_DQ))) return st;
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_cg_update2, vg.grid, vg.block, 0, s, c, ctx->p, *dnew, ctx->pr, ctx->pq, ctx->scal,
+                       ctx->part);
+    prof_end(ctx, "cg_update");
+    if ((st = finish_sum(ctx, vg.nblocks, 2, S_RR))) return st;
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    double *t = *dold;
+    *dold = *dnew;
+    *dnew = t;
+    return IMPEC_OK;
+}
+
+// Multigrid-preconditioned CG for the reference's pinned system.
+//  phase 0: consistent Neumann formulation (when A_N is symmetric), then the
+//           shift to the pinned value;
+//  phase 1: the pinned row and column eliminated, from the current iterate;
+//           runs only when phase 0 was not applicable or the shifted iterate's
+//           pinned residual exceeds the tolerance (the assembled rows sum to
+//           zero only to rounding, so a large shift leaves a small residual).
+// Each phase confirms convergence on a recomputed residual and restarts from
+// it when the recursion has drifted.
+template <bool DIAG, bool FUSED>
+static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     int st = launch_mobility(ctx);
     if (st) return st;
     st = build_hierarchy(ctx);
@@ -764,72 +1281,55 @@ static int solve_impl(impec_ctx *ctx, int *its_out, double *relres_out) {
     c.n1 = ctx->N1; c.n2 = ctx->N2; c.ld = L0.ld; c.ldp = ctx->lds; c.B = ctx->B;
     c.n = (long)ctx->N1 * ctx->N2;
     c.pin_i = ctx->pin_i; c.pin_j = ctx->pin_j;
-    c.shift = ctx->symmetric;
     c.dpin = ctx->kappa / ctx->dA;           // pinned diagonal kappa/dA (MGSolver.cpp:555)
     c.ppin = -ctx->q_pin / c.dpin;           // its solution for b = -q
     c.bsum = ctx->bsum;
     const int nb = red_grid(c.n);
     hipStream_t s = ctx->stream;
-
-    if (ctx->P.initial_guess != IMPEC_GUESS_PREVIOUS || !ctx->have_prev_p)
-        hipLaunchKernelGGL(k_zero_interior, dim3(nb), dim3(256), 0, s, c, ctx->p);
-    prof_begin(ctx);
-    hipLaunchKernelGGL(k_cg_init<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q, ctx->pr, ctx->part);
-    IMPEC_HIP_CHECK(ctx, hipGetLastError());
-    if ((st = finish_sum(ctx, nb, 2, S_RR))) return st;
-    prof_end(ctx, "cg_init");
-    if ((st = fetch_scal(ctx))) return st;
     const double tol = ctx->P.eps;
     const double bb = ctx->bnorm * ctx->bnorm;
     const int maxit = ctx->P.max_iterations > 0 ? ctx->P.max_iterations : 10000;
-    double rr = ctx->h_scal[S_RR];
+
+    if (ctx->P.initial_guess != IMPEC_GUESS_PREVIOUS || !ctx->have_prev_p)
+        hipLaunchKernelGGL(k_zero_interior, dim3(nb), dim3(256), 0, s, c, ctx->p);
     int its = 0, status = IMPEC_OK;
-    while (!(rr <= tol * tol * bb)) {
-        if (its >= maxit) {
-            status = IMPEC_ENOCONV;
-            break;
+    double relres = 0., rr = 0.;
+    double *dold = ctx->pd, *dnew = ctx->pz;
+    for (int phase = ctx->symmetric ? 0 : 1; phase < 2; phase++) {
+        c.shift = phase == 0;
+        if ((st = cg_residual<DIAG>(ctx, a0, c, nb, &rr))) return st;
+        int since = 0, restarts = 0;
+        for (;;) {
+            while (!(rr <= tol * tol * bb)) {
+                if (its >= maxit) { status = IMPEC_ENOCONV; break; }
+                if (!(rr == rr)) { status = IMPEC_ENAN; break; }
+                if (FUSED && !DIAG) st = cg_iteration_fused(ctx, a0, c, since == 0, &dold, &dnew);
+                else st = cg_iteration_unfused<DIAG>(ctx, a0, c, nb, since == 0);
+                if (st) return st;
+                if ((st = fetch_scal(ctx))) return st;
+                rr = ctx->h_scal[S_RR];
+                its++;
+                since++;
+            }
+            if (status != IMPEC_OK) break;
+            if ((st = cg_residual<DIAG>(ctx, a0, c, nb, &rr))) return st;
+            if (rr <= tol * tol * bb || restarts >= 3) break;
+            restarts++;
+            since = 0;
         }
-        if (!(rr == rr)) {
-            status = IMPEC_ENAN;
-            break;
+        if (c.shift) {
+            hipLaunchKernelGGL(k_read_pin, dim3(1), dim3(1), 0, s, c, ctx->p, ctx->scal);
+            hipLaunchKernelGGL(k_shift, dim3(nb), dim3(256), 0, s, c, ctx->p, ctx->scal);
+        } else {
+            hipLaunchKernelGGL(k_set_pin, dim3(1), dim3(1), 0, s, c, ctx->p);
         }
-        prof_begin(ctx);
-        hipLaunchKernelGGL(k_cg_proj_in, dim3(nb), dim3(256), 0, s, c, ctx->pr, L0.b, ctx->scal);
-        prof_end(ctx, "cg_vector");
-        if ((st = vcycle<DIAG>(ctx))) return st;
-        prof_begin(ctx);
-        hipLaunchKernelGGL(k_sum, dim3(nb), dim3(256), 0, s, c, L0.x, ctx->part);
-        if ((st = finish_sum(ctx, nb, 1, S_ZSUM))) return st;
-        hipLaunchKernelGGL(k_cg_z, dim3(nb), dim3(256), 0, s, c, L0.x, ctx->pr, ctx->pz, ctx->scal, ctx->part);
-        if ((st = finish_sum(ctx, nb, 1, S_RHO))) return st;
-        hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(256), 0, s, c, ctx->pz, ctx->pd, ctx->scal, its == 0);
-        prof_end(ctx, "cg_vector");
-        prof_begin(ctx);
-        hipLaunchKernelGGL(k_cg_matvec<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->pd, ctx->pq, ctx->part);
-        prof_end(ctx, "cg_matvec");
-        if ((st = finish_sum(ctx, nb, 1, S_DQ))) return st;
-        prof_begin(ctx);
-        hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(256), 0, s, c, ctx->p, ctx->pd, ctx->pr, ctx->pq, ctx->scal,
-                           ctx->part);
-        if ((st = finish_sum(ctx, nb, 2, S_RR))) return st;
-        hipLaunchKernelGGL(k_copy_scal, dim3(1), dim3(1), 0, s, ctx->scal, (int)S_RHO_OLD, (int)S_RHO);
-        prof_end(ctx, "cg_vector");
+        hipLaunchKernelGGL(k_true_residual<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q, ctx->part);
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
+        if ((st = finish_sum(ctx, nb, 1, S_TT))) return st;
         if ((st = fetch_scal(ctx))) return st;
-        rr = ctx->h_scal[S_RR];
-        its++;
+        relres = ctx->bnorm > 0 ? sqrt(ctx->h_scal[S_TT]) / ctx->bnorm : 0.;
+        if (status != IMPEC_OK || relres <= tol) break;
     }
-    if (c.shift) {
-        hipLaunchKernelGGL(k_read_pin, dim3(1), dim3(1), 0, s, c, ctx->p, ctx->scal);
-        hipLaunchKernelGGL(k_shift, dim3(nb), dim3(256), 0, s, c, ctx->p, ctx->scal);
-    } else {
-        hipLaunchKernelGGL(k_set_pin, dim3(1), dim3(1), 0, s, c, ctx->p);
-    }
-    hipLaunchKernelGGL(k_true_residual<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q, ctx->part);
-    IMPEC_HIP_CHECK(ctx, hipGetLastError());
-    if ((st = finish_sum(ctx, nb, 1, S_TT))) return st;
-    if ((st = fetch_scal(ctx))) return st;
-    const double relres = ctx->bnorm > 0 ? sqrt(ctx->h_scal[S_TT]) / ctx->bnorm : 0.;
     ctx->have_prev_p = 1;
     ctx->stats.last_its = its;
     ctx->stats.last_vcycles = its;
@@ -845,8 +1345,12 @@ static int solve_impl(impec_ctx *ctx, int *its_out, double *relres_out) {
 }
 
 int pressure_solve(impec_ctx *ctx, int *its, double *relres) {
-    if (ctx->P.nu != 1.) return solve_impl<true>(ctx, its, relres);
-    return solve_impl<false>(ctx, its, relres);
+    if (ctx->P.nu != 1.) return solve_cg<true, false>(ctx, its, relres);
+    const dim3 tg = tile_grid(ctx->N1, ctx->N2);
+    const bool fused_ok = ctx->P.mg_pre <= 1 && ctx->P.mg_post <= 1 && (long)tg.x * tg.y <= RED_BLOCKS &&
+                          !ctx->force_unfused;
+    if (fused_ok) return solve_cg<false, true>(ctx, its, relres);
+    return solve_cg<false, false>(ctx, its, relres);
 }
 
 int pressure_operator_dump(impec_ctx *ctx, double *A9_host) {

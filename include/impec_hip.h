@@ -70,7 +70,9 @@ typedef struct impec_params {
     int precision;             /* ext: IMPEC_PREC_*                            */
     int initial_guess;         /* ext: IMPEC_GUESS_*                           */
     int mg_pre, mg_post;       /* smoothing sweeps per V-cycle leg             */
-    int reserved_i[4];
+    int mg_variant;            /* 0: fused LDS-tiled V-cycle where available,  */
+                               /* 1: one kernel per smoothing/transfer pass    */
+    int reserved_i[3];
     double x_lb, x_ub, y_lb, y_ub;
     double nu;                 /* hyperbolic.nu, 5-pt/rotated stencil weight   */
     double mu_res, mu_inj;

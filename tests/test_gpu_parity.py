@@ -237,3 +237,27 @@ def test_steps_1_and_10(gpu_lib, kw):
             assert rel_err(vg, v) < 1e-10, k
             assert rel_err(o.interior(cg), o.interior(c)) < 1e-10, k
             assert abs(dt_gpu - dt_ref) <= 1e-10 * dt_ref
+
+
+@pytest.mark.parametrize("n1,n2", [(256, 256), (200, 120), (77, 45), (2048, 2048)])
+def test_fused_and_unfused_multigrid_agree(gpu_lib, n1, n2):
+    """The LDS-fused V-cycle and the one-kernel-per-pass V-cycle are different
+    preconditioners of the same CG; both reach the tolerance and agree on p."""
+    from impec2d_amd import capi
+    # 1e-11: the FP64 rounding floor of ||b - A p|| / ||b|| at 2048^2 is ~1e-12
+    p = params(N1=n1, N2=n2, nu=1.0, eps=1e-11, mu_inj=0.1)
+    _say("fused vs unfused", n1, n2)
+    out = []
+    for variant in (0, 1):
+        o, g = pair(p, mg_variant=variant)
+        c = smooth_c(o, seed=17)
+        q = o.source("one_cell")
+        g.set_concentration(c)
+        g.set_source(q)
+        its, rr = g.pressure_solve()
+        assert rr <= 1e-11
+        out.append((its, o.interior(g.get_pressure())))
+        del capi
+        from impec2d_amd import capi  # noqa: F401
+    assert abs(out[0][0] - out[1][0]) <= 3, (out[0][0], out[1][0])
+    assert rel_err(out[0][1], out[1][1]) < 1e-9
