@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def workload_params(capi, n, precision="fp64", guess="zero"):
+def workload_params(capi, n, precision="fp64", guess="zero", mg_rows=0):
     return capi.default_params(
         N1=n, N2=n, source_type=capi.SRC_DIAGONAL, average_procedure=capi.AVG_HARMONIC,
         dim_splitting=capi.SPLIT_PARALLEL, hyper_method=capi.HYPER_MUSCL, diffusion=1,
@@ -38,7 +38,7 @@ def workload_params(capi, n, precision="fp64", guess="zero"):
         nu=1.0, mu_res=1.0, mu_inj=0.1, porosity=1.0, cfl=0.5, inj_fluid_conc=1.0,
         alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5, eps=1e-10, para_eps=1e-10, I=1.0,
         precision=capi.PREC_MIXED if precision == "mixed" else capi.PREC_FP64,
-        initial_guess=capi.GUESS_PREVIOUS if guess == "previous" else capi.GUESS_ZERO)
+        initial_guess=capi.GUESS_PREVIOUS if guess == "previous" else capi.GUESS_ZERO, mg_rows=mg_rows)
 
 
 def one_cell_source(n):
@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--guess", choices=["zero", "previous"], default="zero")
+    ap.add_argument("--mg-rows", type=int, default=0,
+                    help="rows per wavefront chunk of the finest multigrid level (0: automatic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -97,7 +99,7 @@ def main():
         print("copy calibration: 1 GiB in %.3f ms (%.0f GB/s read+write)" % (ms.value, 2 * 1.073741824 / ms.value * 1e3),
               file=sys.stderr)
     n = args.size
-    ctx = capi.Context(workload_params(capi, n, args.precision, args.guess))
+    ctx = capi.Context(workload_params(capi, n, args.precision, args.guess, args.mg_rows))
     ctx.set_source(one_cell_source(n))
     t = 0.0
     for _ in range(args.warmup):

@@ -753,284 +753,380 @@ __global__ void k_set_pin(CgArgs c, double *__restrict__ p) {
 }
 
 // ===========================================================================
-// Fused V-cycle for the 5-point operator (nu = 1): one kernel per leg and
-// level, temporally blocked in LDS.  A workgroup owns a TI x TJ tile of the
-// level and recomputes a halo, so each leg reads its inputs from HBM once:
-//   down: red-black Gauss-Seidel from a zero guess, residual, full weighting
-//   up:   bilinear prolongation + correction, black-red Gauss-Seidel
-// Colour 0 is (i + j) even.  Face mobilities are formed once per tile.
+// Streaming V-cycle legs for the 5-point operator (nu = 1), one kernel per leg
+// and level.  A wavefront owns a strip of 128 columns, two adjacent columns
+// per lane, and a chunk of R rows, and marches down the rows keeping a window
+// of four rows in registers; values of the neighbouring columns come from the
+// adjacent lanes by DPP wave shifts.  In every row the two columns of a lane
+// hold one red and one black cell, at the same offset in every lane, so the
+// colour updates of a step have no lane divergence.  Each strip recomputes 4
+// columns on either side and writes the middle 120; each chunk recomputes a
+// few rows above and below.  Colour 0 (red) is (i + j) even.
+//   down, step k: red from a zero guess on row k, black on row k-1, residual
+//                 on row k-2, full weighting into the coarse rows completed
+//                 by that residual row;
+//   up,   step k: x + P e on row k, black on row k-1, red on row k-2.
+// The coefficients are formed from the mobility quantity w as in op_row<false>
+// with the boundary faces dropped (the Neumann folding), and the diagonal is
+// minus the sum of the four faces.
 // ===========================================================================
-constexpr int TI = 16, TJ = 64, NT = 256;
+constexpr int SW_OUT = 120, SW_HALO = 4, SW_WAVES = 4;
 
 struct FusedArgs {
     OpGeom g;
-    const double *w;    // mobility quantity at cell (0,0), pitch ldw, ring >= 1
+    const double *w, *w2;   // mobility quantity (x faces, y faces) at cell (0,0), pitch ldw
     int ldw;
-    int ld;             // pitch of this level's x/b (ring-1 arrays at cell (0,0))
-    const double *b;    // level rhs (levels >= 1)
-    const double *r;    // level 0: CG residual, b = r - S[S_RSUM]/n
+    int ld;                 // pitch of this level's x/b (ring-1 arrays at cell (0,0))
+    const double *b;        // level rhs (levels >= 1)
+    const double *r;        // level 0: CG residual, b = r - S[S_RSUM]/n
     const double *S;
     double inv_n;
-    double *x;          // this level's x (down: written; up: read)
-    double *xo;         // up: corrected, post-smoothed x (a different array,
-                        // since neighbouring tiles read x in their halos)
-    const double *xc;   // up: coarse-level solution at (0,0)
-    double *bc;         // down: coarse-level b at (0,0)
-    int m1, m2, ldc;    // coarse level
-    double *part;       // up at level 0: partials (sum x, sum r x)
+    double *x;              // this level's x (down: written; up: read)
+    double *xo;             // up: corrected, post-smoothed x (a different array:
+                            // neighbouring strips and chunks read x)
+    const double *xc;       // up: coarse-level solution at (0,0)
+    double *bc;             // down: coarse-level b at (0,0)
+    int m1, m2, ldc;        // coarse level
+    double *part;           // up at level 0: per-wave partials (sum x, sum r x)
+    int R, nstrip, nwave;   // rows per chunk, strips, wavefronts
 };
 
-__device__ __forceinline__ bool in_dom(int i, int j, int n1, int n2) {
-    return i >= 0 && j >= 0 && i < n1 && j < n2;
+// lane l receives v of lane l-1 (lane 0 receives 0): DPP wave_shr:1
+__device__ __forceinline__ double lane_prev(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+// lane l receives v of lane l+1 (lane 63 receives 0): DPP wave_shl:1
+__device__ __forceinline__ double lane_next(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
 }
 
-// folded 5-point row from the four face mobilities (same arithmetic as op_row<false>)
-__device__ __forceinline__ void row5(const OpGeom &g, double fxm, double fxp, double fym, double fyp, int i, int j,
-                                     double a[9]) {
-    a[ST_C] = g.mcy * (fym + fyp) - g.cx * (fxm + fxp);
-    a[ST_W] = g.cx * fxm; a[ST_SW] = 0.; a[ST_S] = g.cy * fym; a[ST_SE] = 0.;
-    a[ST_E] = g.cx * fxp; a[ST_NE] = 0.; a[ST_N] = g.cy * fyp; a[ST_NW] = 0.;
-    fold_row(i, j, g.n1, g.n2, a);
+// scaled face coefficients of one row for a lane's two columns g0, g0+1:
+// E = face to the next row, S0 = face left of g0, M = between g0 and g0+1,
+// N1 = face right of g0+1 (zero where the face is a domain boundary)
+struct FRow {
+    double E0, E1, S0, M, N1;
+};
+
+struct Lane {
+    int g0;
+    bool v0, v1, vR;   // columns g0, g0+1 and the pairs (g0,g0+1), (g0+1,g0+2) inside
+};
+
+__device__ __forceinline__ FRow row_faces(const OpGeom &g, const Lane &L, int m, double a0, double a1, double an0,
+                                          double an1, double y0, double y1) {
+    const bool rin = m >= 0 && m < g.n1, rnx = m >= 0 && m + 1 < g.n1;
+    FRow f;
+    f.E0 = (rnx && L.v0) ? g.cx * face_mob(a0, an0, g.harmonic) : 0.;
+    f.E1 = (rnx && L.v1) ? g.cx * face_mob(a1, an1, g.harmonic) : 0.;
+    f.M = (rin && L.v1) ? g.cy * face_mob(y0, y1, g.harmonic) : 0.;
+    const double yR = lane_next(y0);
+    f.N1 = (rin && L.vR) ? g.cy * face_mob(y1, yR, g.harmonic) : 0.;
+    f.S0 = lane_prev(f.N1);
+    return f;
 }
 
-__device__ __forceinline__ double level_b(const FusedArgs &A, bool fine0, int i, int j) {
-    if (fine0) return A.r[(long)i * A.ld + j] - A.S[S_RSUM] * A.inv_n;
-    return A.b[(long)i * A.ld + j];
+// one Gauss-Seidel update of the cell at column offset c of a row with faces
+// F, face above Wf (E of the previous row), neighbours above/below xa/xb and
+// the row's two values xr[2]
+template <int C>
+__device__ __forceinline__ double gs_cell(const FRow &F, const double Wf[2], double b, double xa, double xb,
+                                          const double xr[2]) {
+    double Wc, Ec, Sc, Nc, xS, xN;
+    if (C == 0) {
+        Wc = Wf[0]; Ec = F.E0; Sc = F.S0; Nc = F.M;
+        xS = lane_prev(xr[1]); xN = xr[1];
+    } else {
+        Wc = Wf[1]; Ec = F.E1; Sc = F.M; Nc = F.N1;
+        xS = xr[0]; xN = lane_next(xr[0]);
+    }
+    const double Cc = -((Wc + Ec) + (Sc + Nc));
+    double s = b;
+    s -= Wc * xa;
+    s -= Sc * xS;
+    s -= Ec * xb;
+    s -= Nc * xN;
+    return Cc != 0. ? s / Cc : 0.;
 }
 
-// Tile-local arrays: region [i0-H, i0+TI+H) x [j0-H, j0+TJ+H) stored with
-// pitch (TJ + 2H).  fx(i,j) is the face between rows i-1 and i, fy(i,j) the
-// face between columns j-1 and j.
+struct Chunk {
+    int i0, i1;
+    bool out;   // the lane writes (lanes 2..61)
+};
+
+__device__ __forceinline__ bool wave_setup(const FusedArgs &A, Lane &L, Chunk &K) {
+    const int lane = threadIdx.x & 63;
+    const int wid = blockIdx.x * SW_WAVES + (threadIdx.x >> 6);
+    if (wid >= A.nwave) return false;
+    const int strip = wid % A.nstrip, chunk = wid / A.nstrip;
+    const int n2 = A.g.n2;
+    L.g0 = strip * SW_OUT - SW_HALO + 2 * lane;
+    L.v0 = L.g0 >= 0 && L.g0 < n2;
+    L.v1 = L.g0 >= 0 && L.g0 + 1 < n2;
+    L.vR = L.g0 >= 0 && L.g0 + 2 < n2;
+    K.i0 = chunk * A.R;
+    K.i1 = K.i0 + A.R < A.g.n1 ? K.i0 + A.R : A.g.n1;
+    K.out = lane >= 2 && lane < 62;
+    return true;
+}
+
+// the lane's two values of row m of p (fill outside the level)
+__device__ __forceinline__ void load_pair(const double *__restrict__ p, int ldp, const Lane &L, int m, int n1,
+                                          double fill, double &a, double &b) {
+    const bool rin = m >= 0 && m < n1;
+    const double *q = p + (long)m * ldp + L.g0;
+    a = (rin && L.v0) ? q[0] : fill;
+    b = (rin && L.v1) ? q[1] : fill;
+}
+
+constexpr int SW_PF = 4;   // prefetch distance in rows (one slot per unrolled step)
+
 template <bool FINE0>
-__global__ void __launch_bounds__(NT) k_mg_down(FusedArgs A) {
-    constexpr int HW = 4, HB = 3, HR = 1;
-    constexpr int WW = TJ + 2 * HW, WB = TJ + 2 * HB, WR = TJ + 2 * HR;
-    constexpr int RWW = TI + 2 * HW, RB = TI + 2 * HB, RR = TI + 2 * HR;
-    constexpr int FXR = TI + 2 * HB + 1, FXC = WB;       // fx rows [i0-3, i0+TI+4)
-    constexpr int FYR = RB, FYC = WB + 1;                // fy cols [j0-3, j0+TJ+4)
-    __shared__ double sw[RWW * WW];                      // w, later the residual
-    __shared__ double sfx[FXR * FXC], sfy[FYR * FYC];
-    __shared__ double sb[RB * WB], sx[RB * WB];
-    const int t = threadIdx.x;
-    const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
+__global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
+    Lane L;
+    Chunk K;
+    if (!wave_setup(A, L, K)) return;
     const int n1 = A.g.n1, n2 = A.g.n2;
-    {
-        // every global load of the staging phase is issued before the first
-        // use, so a thread has all of them in flight at once
-        constexpr int NW = (RWW * WW + NT - 1) / NT, NB = (RB * WB + NT - 1) / NT;
-        const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
-        const double *src = FINE0 ? A.r : A.b;
-        double vw[NW], vb[NB];
+    const bool aniso = A.w2 != A.w;
+    const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
+    const double *src = FINE0 ? A.r : A.b;
+    // full weighting: this lane's coarse column and the weights of its four fine columns
+    const int J = L.g0 >> 1;
+    const bool wcol = K.out && L.g0 >= 0 && J < A.m2;
+    const double wjL = (L.g0 >= 1 && L.g0 - 1 < n2) ? pw(L.g0 - 1, J, A.m2) : 0.;
+    const double wj0 = L.v0 ? pw(L.g0, J, A.m2) : 0.;
+    const double wj1 = L.v1 ? pw(L.g0 + 1, J, A.m2) : 0.;
+    const double wjR = L.vR ? pw(L.g0 + 2, J, A.m2) : 0.;
+    const int Ilo = K.i0 >> 1, Ihi = (K.i1 + 1) >> 1;
+    const int kstart = K.i0 - 4, kend = 2 * Ihi + 2;
+
+    double PW[SW_PF][2], PY[SW_PF][2], PB[SW_PF][2];
+    auto fetch = [&](int u, int k) {   // inputs of step k: w(k+1), w2(k), b(k)
+        load_pair(A.w, A.ldw, L, k + 1, n1, 1., PW[u][0], PW[u][1]);
+        if (aniso) load_pair(A.w2, A.ldw, L, k, n1, 1., PY[u][0], PY[u][1]);
+        load_pair(src, A.ld, L, k, n1, 0., PB[u][0], PB[u][1]);
+    };
 #pragma unroll
-        for (int q = 0; q < NW; q++) {
-            const int k = t + q * NT;
-            const int ii = i0 - HW + k / WW, jj = j0 - HW + k % WW;
-            const bool ok = k < RWW * WW && ii >= -1 && jj >= -1 && ii <= n1 && jj <= n2;
-            vw[q] = ok ? A.w[(long)ii * A.ldw + jj] : 1.;
-        }
+    for (int u = 0; u < SW_PF; u++) fetch(u, kstart + u);
+    double wc0, wc1;
+    load_pair(A.w, A.ldw, L, kstart, n1, 1., wc0, wc1);
+
+    double X[4][2] = {{0., 0.}, {0., 0.}, {0., 0.}, {0., 0.}};   // rows k-3 .. k
+    double B[3][2] = {{0., 0.}, {0., 0.}, {0., 0.}};             // rows k-2 .. k
+    FRow F[3];                                                   // rows k-2 .. k
+    double E[4][2] = {{0., 0.}, {0., 0.}, {0., 0.}, {0., 0.}};   // E of rows k-3 .. k
+    for (int q = 0; q < 3; q++) F[q] = FRow{0., 0., 0., 0., 0.};
+    double acc_c = 0., acc_n = 0.;
+
+    for (int kb = kstart; kb <= kend; kb += SW_PF) {
 #pragma unroll
-        for (int q = 0; q < NB; q++) {
-            const int k = t + q * NT;
-            const int ii = i0 - HB + k / WB, jj = j0 - HB + k % WB;
-            vb[q] = (k < RB * WB && in_dom(ii, jj, n1, n2)) ? src[(long)ii * A.ld + jj] - mean : 0.;
-        }
+        for (int u = 0; u < SW_PF; u++) {
+            const int k = kb + u;
+            if (k > kend) break;
+            const double wn0 = PW[u][0], wn1 = PW[u][1];
+            const double y0 = aniso ? PY[u][0] : wc0, y1 = aniso ? PY[u][1] : wc1;
+            double bk[2];
+            bk[0] = L.v0 && k >= 0 && k < n1 ? PB[u][0] - mean : 0.;
+            bk[1] = L.v1 && k >= 0 && k < n1 ? PB[u][1] - mean : 0.;
+            fetch(u, k + SW_PF);
+            const FRow fk = row_faces(A.g, L, k, wc0, wc1, wn0, wn1, y0, y1);
+            wc0 = wn0;
+            wc1 = wn1;
 #pragma unroll
-        for (int q = 0; q < NW; q++)
-            if (t + q * NT < RWW * WW) sw[t + q * NT] = vw[q];
-#pragma unroll
-        for (int q = 0; q < NB; q++)
-            if (t + q * NT < RB * WB) {
-                sb[t + q * NT] = vb[q];
-                sx[t + q * NT] = 0.;
+            for (int q = 0; q < 3; q++) {
+                X[q][0] = X[q + 1][0]; X[q][1] = X[q + 1][1];
+                E[q][0] = E[q + 1][0]; E[q][1] = E[q + 1][1];
             }
-    }
-    __syncthreads();
-    for (int k = t; k < FXR * FXC; k += NT) {
-        const int r = k / FXC, c = k % FXC;   // face between rows (i0-4+r) and (i0-3+r), column j0-3+c
-        sfx[k] = face_mob(sw[r * WW + c + 1], sw[(r + 1) * WW + c + 1], A.g.harmonic);
-    }
-    for (int k = t; k < FYR * FYC; k += NT) {
-        const int r = k / FYC, c = k % FYC;   // row i0-3+r, face between columns j0-4+c and j0-3+c
-        sfy[k] = face_mob(sw[(r + 1) * WW + c], sw[(r + 1) * WW + c + 1], A.g.harmonic);
-    }
-    __syncthreads();
-#define FX(r, c) sfx[(r) * FXC + (c)]
-#define FY(r, c) sfy[(r) * FYC + (c)]
-    // red from zero on [i0-3, i0+TI+3) x [j0-3, j0+TJ+3)
-    for (int k = t; k < RB * WB; k += NT) {
-        const int r = k / WB, c = k % WB, ii = i0 - HB + r, jj = j0 - HB + c;
-        if (((ii + jj) & 1) || !in_dom(ii, jj, n1, n2)) continue;
-        double a[9];
-        row5(A.g, FX(r, c), FX(r + 1, c), FY(r, c), FY(r, c + 1), ii, jj, a);
-        sx[k] = a[ST_C] != 0. ? sb[k] / a[ST_C] : 0.;
-    }
-    __syncthreads();
-    // black on [i0-2, i0+TI+2) x [j0-2, j0+TJ+2)
-    for (int k = t; k < (TI + 4) * (TJ + 4); k += NT) {
-        const int r = k / (TJ + 4) + 1, c = k % (TJ + 4) + 1, ii = i0 - HB + r, jj = j0 - HB + c;
-        if (!((ii + jj) & 1) || !in_dom(ii, jj, n1, n2)) continue;
-        double a[9];
-        row5(A.g, FX(r, c), FX(r + 1, c), FY(r, c), FY(r, c + 1), ii, jj, a);
-        const int o = r * WB + c;
-        double s = sb[o];
-        if (ii > 0) s -= a[ST_W] * sx[o - WB];
-        if (jj > 0) s -= a[ST_S] * sx[o - 1];
-        if (ii < n1 - 1) s -= a[ST_E] * sx[o + WB];
-        if (jj < n2 - 1) s -= a[ST_N] * sx[o + 1];
-        sx[o] = a[ST_C] != 0. ? s / a[ST_C] : 0.;
-    }
-    __syncthreads();
-    // residual on [i0-1, i0+TI+1) x [j0-1, j0+TJ+1) into sw
-    for (int k = t; k < RR * WR; k += NT) {
-        const int rr = k / WR, cr = k % WR, r = rr + 2, c = cr + 2, ii = i0 - HB + r, jj = j0 - HB + c;
-        double res = 0.;
-        if (in_dom(ii, jj, n1, n2)) {
-            double a[9];
-            row5(A.g, FX(r, c), FX(r + 1, c), FY(r, c), FY(r, c + 1), ii, jj, a);
-            const int o = r * WB + c;
-            double s = a[ST_C] * sx[o];
-            if (ii > 0) s += a[ST_W] * sx[o - WB];
-            if (jj > 0) s += a[ST_S] * sx[o - 1];
-            if (ii < n1 - 1) s += a[ST_E] * sx[o + WB];
-            if (jj < n2 - 1) s += a[ST_N] * sx[o + 1];
-            res = sb[o] - s;
-        }
-        sw[k] = res;
-    }
-    // smoothed x of the tile
-    for (int k = t; k < TI * TJ; k += NT) {
-        const int r = k / TJ + HB, c = k % TJ + HB, ii = i0 - HB + r, jj = j0 - HB + c;
-        if (in_dom(ii, jj, n1, n2)) A.x[(long)ii * A.ld + jj] = sx[r * WB + c];
-    }
-    __syncthreads();
-    // full weighting onto the coarse cells of this tile (one per thread)
-    {
-        const int I = i0 / 2 + t / (TJ / 2), J = j0 / 2 + t % (TJ / 2);
-        if (I < A.m1 && J < A.m2) {
-            double s = 0.;
-            for (int f = 2 * I - 1; f <= 2 * I + 2; f++) {
-                if (f < 0 || f >= n1) continue;
-                const double wi = pw(f, I, A.m1);
-                if (wi == 0.) continue;
-                double tt = 0.;
-                for (int gg = 2 * J - 1; gg <= 2 * J + 2; gg++) {
-                    if (gg < 0 || gg >= n2) continue;
-                    const double wj = pw(gg, J, A.m2);
-                    if (wj != 0.) tt += wj * sw[(f - (i0 - 1)) * WR + (gg - (j0 - 1))];
+            F[0] = F[1]; F[1] = F[2]; F[2] = fk;
+            B[0][0] = B[1][0]; B[0][1] = B[1][1]; B[1][0] = B[2][0]; B[1][1] = B[2][1];
+            B[2][0] = bk[0]; B[2][1] = bk[1];
+            E[3][0] = fk.E0; E[3][1] = fk.E1;
+            // red from zero on row k (its black cell stays zero)
+            const int c = u & 1;   // k & 1 (kstart is even)
+            {
+                const double C0 = -((E[2][0] + fk.E0) + (fk.S0 + fk.M));
+                const double C1 = -((E[2][1] + fk.E1) + (fk.M + fk.N1));
+                if (c == 0) {
+                    X[3][0] = C0 != 0. ? bk[0] / C0 : 0.;
+                    X[3][1] = 0.;
+                } else {
+                    X[3][0] = 0.;
+                    X[3][1] = C1 != 0. ? bk[1] / C1 : 0.;
                 }
-                s += wi * tt;
             }
-            A.bc[(long)I * A.ldc + J] = 0.25 * s;
+            // black on row k-1
+            if (c == 0) X[2][0] = gs_cell<0>(F[1], E[1], B[1][0], X[1][0], X[3][0], X[2]);
+            else X[2][1] = gs_cell<1>(F[1], E[1], B[1][1], X[1][1], X[3][1], X[2]);
+            // residual on row k-2
+            double r0, r1;
+            {
+                const FRow &fr = F[0];
+                const double xl = lane_prev(X[1][1]), xr = lane_next(X[1][0]);
+                double W = E[0][0], Ee = fr.E0, S = fr.S0, N = fr.M;
+                double Cc = -((W + Ee) + (S + N));
+                double s = Cc * X[1][0];
+                s += W * X[0][0];
+                s += S * xl;
+                s += Ee * X[2][0];
+                s += N * X[1][1];
+                r0 = B[0][0] - s;
+                W = E[0][1]; Ee = fr.E1; S = fr.M; N = fr.N1;
+                Cc = -((W + Ee) + (S + N));
+                s = Cc * X[1][1];
+                s += W * X[0][1];
+                s += S * X[1][0];
+                s += Ee * X[2][1];
+                s += N * xr;
+                r1 = B[0][1] - s;
+            }
+            const int f = k - 2;
+            if (K.out && f >= K.i0 && f < K.i1) {
+                double *xp = A.x + (long)f * A.ld + L.g0;
+                if (L.v0) xp[0] = X[1][0];
+                if (L.v1) xp[1] = X[1][1];
+            }
+            // full weighting (P^T / 4, the transpose of the prolongation)
+            {
+                const double rL = lane_prev(r1), rR = lane_next(r0);
+                const double h = ((wjL * rL + wj0 * r0) + wj1 * r1) + wjR * rR;
+                const bool fin = f >= 0 && f < n1;
+                const int cur = (f - 1) >> 1;
+                const double wc = fin ? pw(f, cur, A.m1) : 0., wn = fin ? pw(f, cur + 1, A.m1) : 0.;
+                if (wc != 0.) acc_c += wc * h;
+                if (wn != 0.) acc_n += wn * h;
+                if (c == 0) {   // f even: coarse row cur is complete
+                    if (wcol && cur >= Ilo && cur < Ihi) A.bc[(long)cur * A.ldc + J] = 0.25 * acc_c;
+                    acc_c = acc_n;
+                    acc_n = 0.;
+                }
+            }
         }
     }
-#undef FX
-#undef FY
 }
 
 template <bool FINE0>
-__global__ void __launch_bounds__(NT) k_mg_up(FusedArgs A) {
-    constexpr int HX = 2;
-    constexpr int WX = TJ + 2 * HX, RX = TI + 2 * HX;        // x and w region T +- 2
-    constexpr int WB = TJ + 2, RB = TI + 2;                  // black region T +- 1
-    constexpr int FXR = RB + 1, FXC = WB, FYR = RB, FYC = WB + 1;
-    __shared__ double sw[RX * WX], sx[RX * WX];
-    __shared__ double sfx[FXR * FXC], sfy[FYR * FYC], sb[RB * WB];
-    const int t = threadIdx.x;
-    const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
-    const int n1 = A.g.n1, n2 = A.g.n2, m1 = A.m1, m2 = A.m2;
+__global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
+    Lane L;
+    Chunk K;
+    if (!wave_setup(A, L, K)) return;
+    const int n1 = A.g.n1, m1 = A.m1, m2 = A.m2;
+    const bool aniso = A.w2 != A.w;
+    const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
+    const double *src = FINE0 ? A.r : A.b;
+    // prolongation: this lane's coarse column J and its clamped neighbours
+    const int J = L.g0 >> 1;
+    const bool jok = L.g0 >= 0 && J < m2;
+    const bool jfirst = J == 0, jlast = J == m2 - 1;
+    auto coarse = [&](int I) -> double {
+        I = I < 0 ? 0 : (I > m1 - 1 ? m1 - 1 : I);
+        return jok ? A.xc[(long)I * A.ldc + J] : 0.;
+    };
+    const int kstart = K.i0 - 2, kend = K.i1 + 1;
+
+    double PX[SW_PF][2], PW[SW_PF][2], PY[SW_PF][2], PB[SW_PF][2], PC[SW_PF] = {0., 0., 0., 0.};
+    auto fetch = [&](int u, int k) {   // inputs of step k: x(k), w(k+1), w2(k), b(k), coarse row (k+3)/2
+        load_pair(A.x, A.ld, L, k, n1, 0., PX[u][0], PX[u][1]);
+        load_pair(A.w, A.ldw, L, k + 1, n1, 1., PW[u][0], PW[u][1]);
+        if (aniso) load_pair(A.w2, A.ldw, L, k, n1, 1., PY[u][0], PY[u][1]);
+        load_pair(src, A.ld, L, k, n1, 0., PB[u][0], PB[u][1]);
+        if (u & 1) PC[u] = coarse((k + 3) >> 1);
+    };
+#pragma unroll
+    for (int u = 0; u < SW_PF; u++) fetch(u, kstart + u);
+    double wc0, wc1;
+    load_pair(A.w, A.ldw, L, kstart, n1, 1., wc0, wc1);
+    // coarse rows (k>>1) - 1, k>>1, (k>>1) + 1 as (left, own, right)
+    double CL[3], CO[3], CR[3];
     {
-        constexpr int NX = (RX * WX + NT - 1) / NT, NB = (RB * WB + NT - 1) / NT;
-        const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
-        const double *src = FINE0 ? A.r : A.b;
-        double vw[NX], vx[NX], vb[NB], e00[NX], e01[NX], e10[NX], e11[NX];
+        const int Im = kstart >> 1;
 #pragma unroll
-        for (int q = 0; q < NX; q++) {
-            const int k = t + q * NT;
-            const int ii = i0 - HX + k / WX, jj = j0 - HX + k % WX;
-            const bool ok = k < RX * WX && ii >= -1 && jj >= -1 && ii <= n1 && jj <= n2;
-            vw[q] = ok ? A.w[(long)ii * A.ldw + jj] : 1.;
-            const bool in = k < RX * WX && in_dom(ii, jj, n1, n2);
-            int I0 = ii >> 1, I1 = (ii & 1) ? I0 + 1 : I0 - 1;
-            int J0 = jj >> 1, J1 = (jj & 1) ? J0 + 1 : J0 - 1;
-            I0 = I0 > m1 - 1 ? m1 - 1 : (I0 < 0 ? 0 : I0);
-            J0 = J0 > m2 - 1 ? m2 - 1 : (J0 < 0 ? 0 : J0);
-            I1 = I1 < 0 ? 0 : (I1 > m1 - 1 ? m1 - 1 : I1);
-            J1 = J1 < 0 ? 0 : (J1 > m2 - 1 ? m2 - 1 : J1);
-            vx[q] = in ? A.x[(long)ii * A.ld + jj] : 0.;
-            e00[q] = in ? A.xc[(long)I0 * A.ldc + J0] : 0.;
-            e01[q] = in ? A.xc[(long)I0 * A.ldc + J1] : 0.;
-            e10[q] = in ? A.xc[(long)I1 * A.ldc + J0] : 0.;
-            e11[q] = in ? A.xc[(long)I1 * A.ldc + J1] : 0.;
+        for (int q = 0; q < 3; q++) {
+            CO[q] = coarse(Im - 1 + q);
+            CL[q] = lane_prev(CO[q]);
+            CR[q] = lane_next(CO[q]);
         }
+    }
+
+    double X[4][2] = {{0., 0.}, {0., 0.}, {0., 0.}, {0., 0.}};   // rows k-3 .. k
+    double B[3][2] = {{0., 0.}, {0., 0.}, {0., 0.}};             // raw rhs rows k-2 .. k
+    FRow F[3];
+    double E[4][2] = {{0., 0.}, {0., 0.}, {0., 0.}, {0., 0.}};
+    for (int q = 0; q < 3; q++) F[q] = FRow{0., 0., 0., 0., 0.};
+    double acc0 = 0., acc1 = 0.;
+
+    for (int kb = kstart; kb <= kend; kb += SW_PF) {
 #pragma unroll
-        for (int q = 0; q < NB; q++) {
-            const int k = t + q * NT;
-            const int ii = i0 - 1 + k / WB, jj = j0 - 1 + k % WB;
-            vb[q] = (k < RB * WB && in_dom(ii, jj, n1, n2)) ? src[(long)ii * A.ld + jj] - mean : 0.;
-        }
+        for (int u = 0; u < SW_PF; u++) {
+            const int k = kb + u;
+            if (k > kend) break;
+            const int c = u & 1;   // k & 1
+            const double xk0 = PX[u][0], xk1 = PX[u][1], wn0 = PW[u][0], wn1 = PW[u][1];
+            const double y0 = aniso ? PY[u][0] : wc0, y1 = aniso ? PY[u][1] : wc1;
+            const double bk0 = PB[u][0], bk1 = PB[u][1];
+            const double cnew = PC[u];
+            fetch(u, k + SW_PF);
+            const FRow fk = row_faces(A.g, L, k, wc0, wc1, wn0, wn1, y0, y1);
+            wc0 = wn0;
+            wc1 = wn1;
 #pragma unroll
-        for (int q = 0; q < NX; q++) {
-            const int k = t + q * NT;
-            if (k >= RX * WX) continue;
-            const int ii = i0 - HX + k / WX, jj = j0 - HX + k % WX;
-            sw[k] = vw[q];
-            const double e = 0.5625 * e00[q] + 0.1875 * e01[q] + 0.1875 * e10[q] + 0.0625 * e11[q];
-            sx[k] = in_dom(ii, jj, n1, n2) ? vx[q] + e : 0.;
-        }
+            for (int q = 0; q < 3; q++) {
+                X[q][0] = X[q + 1][0]; X[q][1] = X[q + 1][1];
+                E[q][0] = E[q + 1][0]; E[q][1] = E[q + 1][1];
+            }
+            F[0] = F[1]; F[1] = F[2]; F[2] = fk;
+            B[0][0] = B[1][0]; B[0][1] = B[1][1]; B[1][0] = B[2][0]; B[1][1] = B[2][1];
+            B[2][0] = bk0; B[2][1] = bk1;
+            E[3][0] = fk.E0; E[3][1] = fk.E1;
+            // x + P e on row k: rows I0 = k>>1 (CO[1]) and I1 = I0 -+ 1
+            {
+                const int i1 = c == 0 ? 0 : 2;
+                const double o0 = CO[1], o1 = CO[i1];
+                const double l0 = jfirst ? o0 : CL[1], l1 = jfirst ? o1 : CL[i1];
+                const double r0 = jlast ? o0 : CR[1], r1 = jlast ? o1 : CR[i1];
+                const double e0 = 0.5625 * o0 + 0.1875 * l0 + 0.1875 * o1 + 0.0625 * l1;
+                const double e1 = 0.5625 * o0 + 0.1875 * r0 + 0.1875 * o1 + 0.0625 * r1;
+                const bool rin = k >= 0 && k < n1;
+                X[3][0] = (rin && L.v0) ? xk0 + e0 : 0.;
+                X[3][1] = (rin && L.v1) ? xk1 + e1 : 0.;
+            }
+            // black on row k-1
+            if (c == 0) X[2][0] = gs_cell<0>(F[1], E[1], B[1][0] - mean, X[1][0], X[3][0], X[2]);
+            else X[2][1] = gs_cell<1>(F[1], E[1], B[1][1] - mean, X[1][1], X[3][1], X[2]);
+            // red on row k-2
+            if (c == 0) X[1][0] = gs_cell<0>(F[0], E[0], B[0][0] - mean, X[0][0], X[2][0], X[1]);
+            else X[1][1] = gs_cell<1>(F[0], E[0], B[0][1] - mean, X[0][1], X[2][1], X[1]);
+            const int f = k - 2;
+            if (K.out && f >= K.i0 && f < K.i1) {
+                double *xp = A.xo + (long)f * A.ld + L.g0;
+                if (L.v0) xp[0] = X[1][0];
+                if (L.v1) xp[1] = X[1][1];
+                if (FINE0) {
+                    if (L.v0) { acc0 += X[1][0]; acc1 += B[0][0] * X[1][0]; }
+                    if (L.v1) { acc0 += X[1][1]; acc1 += B[0][1] * X[1][1]; }
+                }
+            }
+            if (c == 1) {   // next step starts coarse row (k+1)>>1
 #pragma unroll
-        for (int q = 0; q < NB; q++)
-            if (t + q * NT < RB * WB) sb[t + q * NT] = vb[q];
-    }
-    __syncthreads();
-    for (int k = t; k < FXR * FXC; k += NT) {
-        const int r = k / FXC, c = k % FXC;   // face between rows i0-2+r and i0-1+r, column j0-1+c
-        sfx[k] = face_mob(sw[r * WX + c + 1], sw[(r + 1) * WX + c + 1], A.g.harmonic);
-    }
-    for (int k = t; k < FYR * FYC; k += NT) {
-        const int r = k / FYC, c = k % FYC;   // row i0-1+r, face between columns j0-2+c and j0-1+c
-        sfy[k] = face_mob(sw[(r + 1) * WX + c], sw[(r + 1) * WX + c + 1], A.g.harmonic);
-    }
-    __syncthreads();
-#define FX(r, c) sfx[(r) * FXC + (c)]
-#define FY(r, c) sfy[(r) * FYC + (c)]
-    for (int pass = 0; pass < 2; pass++) {
-        // pass 0: black on T +- 1; pass 1: red on T
-        const int lo = pass == 0 ? 0 : 1, nr = pass == 0 ? RB : TI, nc = pass == 0 ? WB : TJ;
-        for (int k = t; k < nr * nc; k += NT) {
-            const int r = k / nc + lo, c = k % nc + lo, ii = i0 - 1 + r, jj = j0 - 1 + c;
-            const int col = (ii + jj) & 1;
-            if ((pass == 0 && !col) || (pass == 1 && col) || !in_dom(ii, jj, n1, n2)) continue;
-            double a[9];
-            row5(A.g, FX(r, c), FX(r + 1, c), FY(r, c), FY(r, c + 1), ii, jj, a);
-            const int o = (r + 1) * WX + (c + 1);
-            double s = sb[r * WB + c];
-            if (ii > 0) s -= a[ST_W] * sx[o - WX];
-            if (jj > 0) s -= a[ST_S] * sx[o - 1];
-            if (ii < n1 - 1) s -= a[ST_E] * sx[o + WX];
-            if (jj < n2 - 1) s -= a[ST_N] * sx[o + 1];
-            sx[o] = a[ST_C] != 0. ? s / a[ST_C] : 0.;
-        }
-        __syncthreads();
-    }
-#undef FX
-#undef FY
-    double acc[2] = {0., 0.};
-    for (int k = t; k < TI * TJ; k += NT) {
-        const int r = k / TJ, c = k % TJ, ii = i0 + r, jj = j0 + c;
-        if (!in_dom(ii, jj, n1, n2)) continue;
-        const double xv = sx[(r + HX) * WX + (c + HX)];
-        A.xo[(long)ii * A.ld + jj] = xv;
-        if (FINE0) {
-            acc[0] += xv;
-            acc[1] += A.r[(long)ii * A.ld + jj] * xv;
+                for (int q = 0; q < 2; q++) { CO[q] = CO[q + 1]; CL[q] = CL[q + 1]; CR[q] = CR[q + 1]; }
+                CO[2] = cnew;
+                CL[2] = lane_prev(cnew);
+                CR[2] = lane_next(cnew);
+            }
         }
     }
     if (FINE0) {
-        block_reduce<RED_SUM, 2>(acc);
-        if (t == 0) {
-            const int blk = blockIdx.y * gridDim.x + blockIdx.x;
-            A.part[blk] = acc[0];
-            A.part[RED_BLOCKS + blk] = acc[1];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            acc0 += __shfl_xor(acc0, o);
+            acc1 += __shfl_xor(acc1, o);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            const int wid = blockIdx.x * SW_WAVES + (threadIdx.x >> 6);
+            A.part[wid] = acc0;
+            A.part[RED_BLOCKS + wid] = acc1;
         }
     }
 }
@@ -1127,6 +1223,18 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
     }
 }
 
+// rows per chunk of the streaming legs: about 2048 wavefronts on large
+// levels, short chunks on small ones; even, since the march starts on an
+// even row
+static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip) {
+    if (l == 0 && ctx->P.mg_rows > 0) return (ctx->P.mg_rows + 1) & ~1;
+    long r = ((long)n1 * nstrip + 2047) / 2048;
+    r = (r + 1) & ~1L;
+    if (r < 8) r = 8;
+    if (r > 64) r = 64;
+    return (int)r;
+}
+
 static FusedArgs fused_args(impec_ctx *ctx, int l) {
     FusedArgs A;
     memset(&A, 0, sizeof(A));
@@ -1134,6 +1242,7 @@ static FusedArgs fused_args(impec_ctx *ctx, int l) {
     LevArgs la = lev_args(ctx, l);
     A.g = L.g;
     A.w = la.w1;
+    A.w2 = la.w2;
     A.ldw = la.ldw;
     A.ld = L.ld;
     A.b = L.b + L.ld + 1;
@@ -1152,20 +1261,24 @@ static FusedArgs fused_args(impec_ctx *ctx, int l) {
         A.ldc = Cl.ld;
     }
     A.part = ctx->part;
+    A.nstrip = (L.n2 + SW_OUT - 1) / SW_OUT;
+    A.R = stream_rows(ctx, l, L.n1, A.nstrip);
+    A.nwave = A.nstrip * ((L.n1 + A.R - 1) / A.R);
     return A;
 }
 
-static dim3 tile_grid(int n1, int n2) { return dim3((n2 + TJ - 1) / TJ, (n1 + TI - 1) / TI); }
+static dim3 stream_grid(const FusedArgs &A) { return dim3((A.nwave + SW_WAVES - 1) / SW_WAVES); }
 
 // one V(1,1) cycle on the level-0 rhs r - mean(r); the result x0 is left in
-// lev[0].r, with the partials of sum(x0) and r.x0 for k_finish_rho
+// lev[0].r, with the per-wave partials of sum(x0) and r.x0 for k_finish_rho
 static int vcycle_fused(impec_ctx *ctx) {
     const int L = ctx->nlev;
+    const dim3 blk(64 * SW_WAVES);
     for (int l = 0; l < L - 1; l++) {
         FusedArgs A = fused_args(ctx, l);
         prof_begin(ctx);
-        if (l == 0) hipLaunchKernelGGL(k_mg_down<true>, tile_grid(A.g.n1, A.g.n2), dim3(NT), 0, ctx->stream, A);
-        else hipLaunchKernelGGL(k_mg_down<false>, tile_grid(A.g.n1, A.g.n2), dim3(NT), 0, ctx->stream, A);
+        if (l == 0) hipLaunchKernelGGL(k_mg_down<true>, stream_grid(A), blk, 0, ctx->stream, A);
+        else hipLaunchKernelGGL(k_mg_down<false>, stream_grid(A), blk, 0, ctx->stream, A);
         prof_end(ctx, l == 0 ? "mg_down_l0" : "mg_coarse");
     }
     {
@@ -1178,8 +1291,8 @@ static int vcycle_fused(impec_ctx *ctx) {
     for (int l = L - 2; l >= 0; l--) {
         FusedArgs A = fused_args(ctx, l);
         prof_begin(ctx);
-        if (l == 0) hipLaunchKernelGGL(k_mg_up<true>, tile_grid(A.g.n1, A.g.n2), dim3(NT), 0, ctx->stream, A);
-        else hipLaunchKernelGGL(k_mg_up<false>, tile_grid(A.g.n1, A.g.n2), dim3(NT), 0, ctx->stream, A);
+        if (l == 0) hipLaunchKernelGGL(k_mg_up<true>, stream_grid(A), blk, 0, ctx->stream, A);
+        else hipLaunchKernelGGL(k_mg_up<false>, stream_grid(A), blk, 0, ctx->stream, A);
         prof_end(ctx, l == 0 ? "mg_up_l0" : "mg_coarse");
     }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
@@ -1232,17 +1345,17 @@ static int cg_iteration_unfused(impec_ctx *ctx, const LevArgs &a0, const CgArgs 
     return IMPEC_OK;
 }
 
-// one CG iteration with the LDS-fused V-cycle (nu = 1); d ping-pongs
+// one CG iteration with the streaming V-cycle (nu = 1); d ping-pongs
 static int cg_iteration_fused(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, bool first, double **dold,
                               double **dnew) {
     const Level &L0 = ctx->lev[0];
     hipStream_t s = ctx->stream;
     const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
-    const dim3 tg0 = tile_grid(ctx->N1, ctx->N2);
+    const int nwave0 = fused_args(ctx, 0).nwave;
     const double inv_n = 1.0 / (double)c.n;
     int st;
     if ((st = vcycle_fused(ctx))) return st;
-    hipLaunchKernelGGL(k_finish_rho, dim3(1), dim3(1024), 0, s, ctx->part, (int)(tg0.x * tg0.y), inv_n, ctx->scal);
+    hipLaunchKernelGGL(k_finish_rho, dim3(1), dim3(1024), 0, s, ctx->part, nwave0, inv_n, ctx->scal);
     prof_begin(ctx);
     hipLaunchKernelGGL(k_cg_dir_matvec, vg.grid, vg.block, 0, s, a0, c, L0.r, *dold, *dnew, ctx->pq, ctx->scal,
                        first, ctx->part);
@@ -1346,8 +1459,7 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
 
 int pressure_solve(impec_ctx *ctx, int *its, double *relres) {
     if (ctx->P.nu != 1.) return solve_cg<true, false>(ctx, its, relres);
-    const dim3 tg = tile_grid(ctx->N1, ctx->N2);
-    const bool fused_ok = ctx->P.mg_pre <= 1 && ctx->P.mg_post <= 1 && (long)tg.x * tg.y <= RED_BLOCKS &&
+    const bool fused_ok = ctx->P.mg_pre <= 1 && ctx->P.mg_post <= 1 && fused_args(ctx, 0).nwave <= RED_BLOCKS &&
                           !ctx->force_unfused;
     if (fused_ok) return solve_cg<false, true>(ctx, its, relres);
     return solve_cg<false, false>(ctx, its, relres);

@@ -70,9 +70,11 @@ typedef struct impec_params {
     int precision;             /* ext: IMPEC_PREC_*                            */
     int initial_guess;         /* ext: IMPEC_GUESS_*                           */
     int mg_pre, mg_post;       /* smoothing sweeps per V-cycle leg             */
-    int mg_variant;            /* 0: fused LDS-tiled V-cycle where available,  */
+    int mg_variant;            /* 0: streaming fused V-cycle where available,  */
                                /* 1: one kernel per smoothing/transfer pass    */
-    int reserved_i[3];
+    int mg_rows;               /* rows per wavefront chunk of the fused finest */
+                               /* level (0: automatic; rounded up to even)     */
+    int reserved_i[2];
     double x_lb, x_ub, y_lb, y_ub;
     double nu;                 /* hyperbolic.nu, 5-pt/rotated stencil weight   */
     double mu_res, mu_inj;
