@@ -804,6 +804,21 @@ __device__ __forceinline__ double lane_next(double v) {
     return __hiloint2double(hi, lo);
 }
 
+// 1/d from the hardware reciprocal estimate refined by one Newton-Raphson
+// step, r' = r + r (1 - d r).  Only the multigrid preconditioner uses it: its
+// rounding differs from IEEE division but is the same every time for the same
+// d, so the preconditioner stays a fixed linear operator.
+__device__ __forceinline__ double recip(double d) {
+    const double r = __builtin_amdgcn_rcp(d);
+    const double e = fma(-d, r, 1.);
+    return fma(r, e, r);
+}
+
+// face mobility as face_mob, with recip in place of the division
+__device__ __forceinline__ double face_mob_pc(double wl, double wr, int harmonic) {
+    return harmonic ? 2. * recip(wl + wr) : 0.5 * (wl + wr);
+}
+
 // scaled face coefficients of one row for a lane's two columns g0, g0+1:
 // E = face to the next row, S0 = face left of g0, M = between g0 and g0+1,
 // N1 = face right of g0+1 (zero where the face is a domain boundary)
@@ -813,18 +828,22 @@ struct FRow {
 
 struct Lane {
     int g0;
+    int c0, c1;        // g0, g0+1 clamped into the level (every load is issued)
     bool v0, v1, vR;   // columns g0, g0+1 and the pairs (g0,g0+1), (g0+1,g0+2) inside
 };
 
 __device__ __forceinline__ FRow row_faces(const OpGeom &g, const Lane &L, int m, double a0, double a1, double an0,
                                           double an1, double y0, double y1) {
+#pragma clang fp contract(fast)
     const bool rin = m >= 0 && m < g.n1, rnx = m >= 0 && m + 1 < g.n1;
     FRow f;
-    f.E0 = (rnx && L.v0) ? g.cx * face_mob(a0, an0, g.harmonic) : 0.;
-    f.E1 = (rnx && L.v1) ? g.cx * face_mob(a1, an1, g.harmonic) : 0.;
-    f.M = (rin && L.v1) ? g.cy * face_mob(y0, y1, g.harmonic) : 0.;
+    const double e0 = g.cx * face_mob_pc(a0, an0, g.harmonic), e1 = g.cx * face_mob_pc(a1, an1, g.harmonic);
     const double yR = lane_next(y0);
-    f.N1 = (rin && L.vR) ? g.cy * face_mob(y1, yR, g.harmonic) : 0.;
+    const double fm = g.cy * face_mob_pc(y0, y1, g.harmonic), fn = g.cy * face_mob_pc(y1, yR, g.harmonic);
+    f.E0 = (rnx && L.v0) ? e0 : 0.;
+    f.E1 = (rnx && L.v1) ? e1 : 0.;
+    f.M = (rin && L.v1) ? fm : 0.;
+    f.N1 = (rin && L.vR) ? fn : 0.;
     f.S0 = lane_prev(f.N1);
     return f;
 }
@@ -835,6 +854,7 @@ __device__ __forceinline__ FRow row_faces(const OpGeom &g, const Lane &L, int m,
 template <int C>
 __device__ __forceinline__ double gs_cell(const FRow &F, const double Wf[2], double b, double xa, double xb,
                                           const double xr[2]) {
+#pragma clang fp contract(fast)
     double Wc, Ec, Sc, Nc, xS, xN;
     if (C == 0) {
         Wc = Wf[0]; Ec = F.E0; Sc = F.S0; Nc = F.M;
@@ -849,7 +869,7 @@ __device__ __forceinline__ double gs_cell(const FRow &F, const double Wf[2], dou
     s -= Sc * xS;
     s -= Ec * xb;
     s -= Nc * xN;
-    return Cc != 0. ? s / Cc : 0.;
+    return Cc != 0. ? s * recip(Cc) : 0.;
 }
 
 struct Chunk {
@@ -859,7 +879,9 @@ struct Chunk {
 
 __device__ __forceinline__ bool wave_setup(const FusedArgs &A, Lane &L, Chunk &K) {
     const int lane = threadIdx.x & 63;
-    const int wid = blockIdx.x * SW_WAVES + (threadIdx.x >> 6);
+    // wave-uniform by construction; readfirstlane lets the compiler know, so
+    // the march loop and the row masks are scalar
+    const int wid = blockIdx.x * SW_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wid >= A.nwave) return false;
     const int strip = wid % A.nstrip, chunk = wid / A.nstrip;
     const int n2 = A.g.n2;
@@ -867,30 +889,41 @@ __device__ __forceinline__ bool wave_setup(const FusedArgs &A, Lane &L, Chunk &K
     L.v0 = L.g0 >= 0 && L.g0 < n2;
     L.v1 = L.g0 >= 0 && L.g0 + 1 < n2;
     L.vR = L.g0 >= 0 && L.g0 + 2 < n2;
+    L.c0 = L.g0 < 0 ? 0 : (L.g0 > n2 - 1 ? n2 - 1 : L.g0);
+    L.c1 = L.g0 + 1 < 0 ? 0 : (L.g0 + 1 > n2 - 1 ? n2 - 1 : L.g0 + 1);
     K.i0 = chunk * A.R;
     K.i1 = K.i0 + A.R < A.g.n1 ? K.i0 + A.R : A.g.n1;
     K.out = lane >= 2 && lane < 62;
     return true;
 }
 
-// the lane's two values of row m of p (fill outside the level)
+// the lane's two values of row m of p, loaded unconditionally from a clamped
+// address, so that no lane-dependent branch separates a prefetch from its
+// use; mask_pair replaces the values outside the level when they are used
+// (a select at load time would wait for the load there).
 __device__ __forceinline__ void load_pair(const double *__restrict__ p, int ldp, const Lane &L, int m, int n1,
-                                          double fill, double &a, double &b) {
+                                          double &a, double &b) {
+    const int mc = m < 0 ? 0 : (m > n1 - 1 ? n1 - 1 : m);
+    const double *q = p + (long)mc * ldp;
+    a = q[L.c0];
+    b = q[L.c1];
+}
+__device__ __forceinline__ void mask_pair(const Lane &L, int m, int n1, double fill, double &a, double &b) {
     const bool rin = m >= 0 && m < n1;
-    const double *q = p + (long)m * ldp + L.g0;
-    a = (rin && L.v0) ? q[0] : fill;
-    b = (rin && L.v1) ? q[1] : fill;
+    a = (rin && L.v0) ? a : fill;
+    b = (rin && L.v1) ? b : fill;
 }
 
 constexpr int SW_PF = 4;   // prefetch distance in rows (one slot per unrolled step)
 
-template <bool FINE0>
+template <bool FINE0, bool ANISO>
 __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
+#pragma clang fp contract(fast)
     Lane L;
     Chunk K;
     if (!wave_setup(A, L, K)) return;
     const int n1 = A.g.n1, n2 = A.g.n2;
-    const bool aniso = A.w2 != A.w;
+    constexpr bool aniso = ANISO;
     const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
     const double *src = FINE0 ? A.r : A.b;
     // full weighting: this lane's coarse column and the weights of its four fine columns
@@ -905,14 +938,15 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
 
     double PW[SW_PF][2], PY[SW_PF][2], PB[SW_PF][2];
     auto fetch = [&](int u, int k) {   // inputs of step k: w(k+1), w2(k), b(k)
-        load_pair(A.w, A.ldw, L, k + 1, n1, 1., PW[u][0], PW[u][1]);
-        if (aniso) load_pair(A.w2, A.ldw, L, k, n1, 1., PY[u][0], PY[u][1]);
-        load_pair(src, A.ld, L, k, n1, 0., PB[u][0], PB[u][1]);
+        load_pair(A.w, A.ldw, L, k + 1, n1, PW[u][0], PW[u][1]);
+        if (aniso) load_pair(A.w2, A.ldw, L, k, n1, PY[u][0], PY[u][1]);
+        load_pair(src, A.ld, L, k, n1, PB[u][0], PB[u][1]);
     };
 #pragma unroll
     for (int u = 0; u < SW_PF; u++) fetch(u, kstart + u);
     double wc0, wc1;
-    load_pair(A.w, A.ldw, L, kstart, n1, 1., wc0, wc1);
+    load_pair(A.w, A.ldw, L, kstart, n1, wc0, wc1);
+    mask_pair(L, kstart, n1, 1., wc0, wc1);
 
     double X[4][2] = {{0., 0.}, {0., 0.}, {0., 0.}, {0., 0.}};   // rows k-3 .. k
     double B[3][2] = {{0., 0.}, {0., 0.}, {0., 0.}};             // rows k-2 .. k
@@ -921,13 +955,21 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
     for (int q = 0; q < 3; q++) F[q] = FRow{0., 0., 0., 0., 0.};
     double acc_c = 0., acc_n = 0.;
 
-    for (int kb = kstart; kb <= kend; kb += SW_PF) {
+    // whole groups of SW_PF steps (no exit inside the unrolled group, so the
+    // prefetch ring stays in place); steps past kend write nothing
+    const int kstop = kstart + (kend - kstart + SW_PF) / SW_PF * SW_PF;
+    for (int kb = kstart; kb < kstop; kb += SW_PF) {
 #pragma unroll
         for (int u = 0; u < SW_PF; u++) {
             const int k = kb + u;
-            if (k > kend) break;
-            const double wn0 = PW[u][0], wn1 = PW[u][1];
-            const double y0 = aniso ? PY[u][0] : wc0, y1 = aniso ? PY[u][1] : wc1;
+            double wn0 = PW[u][0], wn1 = PW[u][1];
+            mask_pair(L, k + 1, n1, 1., wn0, wn1);
+            double y0 = wc0, y1 = wc1;
+            if (aniso) {
+                y0 = PY[u][0];
+                y1 = PY[u][1];
+                mask_pair(L, k, n1, 1., y0, y1);
+            }
             double bk[2];
             bk[0] = L.v0 && k >= 0 && k < n1 ? PB[u][0] - mean : 0.;
             bk[1] = L.v1 && k >= 0 && k < n1 ? PB[u][1] - mean : 0.;
@@ -950,11 +992,11 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
                 const double C0 = -((E[2][0] + fk.E0) + (fk.S0 + fk.M));
                 const double C1 = -((E[2][1] + fk.E1) + (fk.M + fk.N1));
                 if (c == 0) {
-                    X[3][0] = C0 != 0. ? bk[0] / C0 : 0.;
+                    X[3][0] = C0 != 0. ? bk[0] * recip(C0) : 0.;
                     X[3][1] = 0.;
                 } else {
                     X[3][0] = 0.;
-                    X[3][1] = C1 != 0. ? bk[1] / C1 : 0.;
+                    X[3][1] = C1 != 0. ? bk[1] * recip(C1) : 0.;
                 }
             }
             // black on row k-1
@@ -1007,44 +1049,48 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
     }
 }
 
-template <bool FINE0>
+template <bool FINE0, bool ANISO>
 __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
+#pragma clang fp contract(fast)
     Lane L;
     Chunk K;
     if (!wave_setup(A, L, K)) return;
     const int n1 = A.g.n1, m1 = A.m1, m2 = A.m2;
-    const bool aniso = A.w2 != A.w;
+    constexpr bool aniso = ANISO;
     const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
     const double *src = FINE0 ? A.r : A.b;
     // prolongation: this lane's coarse column J and its clamped neighbours
     const int J = L.g0 >> 1;
     const bool jok = L.g0 >= 0 && J < m2;
     const bool jfirst = J == 0, jlast = J == m2 - 1;
-    auto coarse = [&](int I) -> double {
+    const int Jc = J < 0 ? 0 : (J > m2 - 1 ? m2 - 1 : J);
+    auto coarse = [&](int I) -> double {   // raw; select with jok at use
         I = I < 0 ? 0 : (I > m1 - 1 ? m1 - 1 : I);
-        return jok ? A.xc[(long)I * A.ldc + J] : 0.;
+        return A.xc[(long)I * A.ldc + Jc];
     };
     const int kstart = K.i0 - 2, kend = K.i1 + 1;
 
     double PX[SW_PF][2], PW[SW_PF][2], PY[SW_PF][2], PB[SW_PF][2], PC[SW_PF] = {0., 0., 0., 0.};
     auto fetch = [&](int u, int k) {   // inputs of step k: x(k), w(k+1), w2(k), b(k), coarse row (k+3)/2
-        load_pair(A.x, A.ld, L, k, n1, 0., PX[u][0], PX[u][1]);
-        load_pair(A.w, A.ldw, L, k + 1, n1, 1., PW[u][0], PW[u][1]);
-        if (aniso) load_pair(A.w2, A.ldw, L, k, n1, 1., PY[u][0], PY[u][1]);
-        load_pair(src, A.ld, L, k, n1, 0., PB[u][0], PB[u][1]);
+        load_pair(A.x, A.ld, L, k, n1, PX[u][0], PX[u][1]);
+        load_pair(A.w, A.ldw, L, k + 1, n1, PW[u][0], PW[u][1]);
+        if (aniso) load_pair(A.w2, A.ldw, L, k, n1, PY[u][0], PY[u][1]);
+        load_pair(src, A.ld, L, k, n1, PB[u][0], PB[u][1]);
         if (u & 1) PC[u] = coarse((k + 3) >> 1);
     };
 #pragma unroll
     for (int u = 0; u < SW_PF; u++) fetch(u, kstart + u);
     double wc0, wc1;
-    load_pair(A.w, A.ldw, L, kstart, n1, 1., wc0, wc1);
+    load_pair(A.w, A.ldw, L, kstart, n1, wc0, wc1);
+    mask_pair(L, kstart, n1, 1., wc0, wc1);
     // coarse rows (k>>1) - 1, k>>1, (k>>1) + 1 as (left, own, right)
     double CL[3], CO[3], CR[3];
     {
         const int Im = kstart >> 1;
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            CO[q] = coarse(Im - 1 + q);
+            const double v = coarse(Im - 1 + q);
+            CO[q] = jok ? v : 0.;
             CL[q] = lane_prev(CO[q]);
             CR[q] = lane_next(CO[q]);
         }
@@ -1057,16 +1103,26 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
     for (int q = 0; q < 3; q++) F[q] = FRow{0., 0., 0., 0., 0.};
     double acc0 = 0., acc1 = 0.;
 
-    for (int kb = kstart; kb <= kend; kb += SW_PF) {
+    // whole groups of SW_PF steps (no exit inside the unrolled group, so the
+    // prefetch ring stays in place); steps past kend write nothing
+    const int kstop = kstart + (kend - kstart + SW_PF) / SW_PF * SW_PF;
+    for (int kb = kstart; kb < kstop; kb += SW_PF) {
 #pragma unroll
         for (int u = 0; u < SW_PF; u++) {
             const int k = kb + u;
-            if (k > kend) break;
             const int c = u & 1;   // k & 1
-            const double xk0 = PX[u][0], xk1 = PX[u][1], wn0 = PW[u][0], wn1 = PW[u][1];
-            const double y0 = aniso ? PY[u][0] : wc0, y1 = aniso ? PY[u][1] : wc1;
-            const double bk0 = PB[u][0], bk1 = PB[u][1];
-            const double cnew = PC[u];
+            const double xk0 = PX[u][0], xk1 = PX[u][1];
+            double wn0 = PW[u][0], wn1 = PW[u][1];
+            mask_pair(L, k + 1, n1, 1., wn0, wn1);
+            double y0 = wc0, y1 = wc1;
+            if (aniso) {
+                y0 = PY[u][0];
+                y1 = PY[u][1];
+                mask_pair(L, k, n1, 1., y0, y1);
+            }
+            double bk0 = PB[u][0], bk1 = PB[u][1];
+            mask_pair(L, k, n1, 0., bk0, bk1);
+            const double cnew = jok ? PC[u] : 0.;
             fetch(u, k + SW_PF);
             const FRow fk = row_faces(A.g, L, k, wc0, wc1, wn0, wn1, y0, y1);
             wc0 = wn0;
@@ -1124,7 +1180,7 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
             acc1 += __shfl_xor(acc1, o);
         }
         if ((threadIdx.x & 63) == 0) {
-            const int wid = blockIdx.x * SW_WAVES + (threadIdx.x >> 6);
+            const int wid = blockIdx.x * SW_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
             A.part[wid] = acc0;
             A.part[RED_BLOCKS + wid] = acc1;
         }
@@ -1277,8 +1333,9 @@ static int vcycle_fused(impec_ctx *ctx) {
     for (int l = 0; l < L - 1; l++) {
         FusedArgs A = fused_args(ctx, l);
         prof_begin(ctx);
-        if (l == 0) hipLaunchKernelGGL(k_mg_down<true>, stream_grid(A), blk, 0, ctx->stream, A);
-        else hipLaunchKernelGGL(k_mg_down<false>, stream_grid(A), blk, 0, ctx->stream, A);
+        if (l == 0 && A.w2 != A.w) hipLaunchKernelGGL((k_mg_down<true, true>), stream_grid(A), blk, 0, ctx->stream, A);
+        else if (l == 0) hipLaunchKernelGGL((k_mg_down<true, false>), stream_grid(A), blk, 0, ctx->stream, A);
+        else hipLaunchKernelGGL((k_mg_down<false, false>), stream_grid(A), blk, 0, ctx->stream, A);
         prof_end(ctx, l == 0 ? "mg_down_l0" : "mg_coarse");
     }
     {
@@ -1291,8 +1348,9 @@ static int vcycle_fused(impec_ctx *ctx) {
     for (int l = L - 2; l >= 0; l--) {
         FusedArgs A = fused_args(ctx, l);
         prof_begin(ctx);
-        if (l == 0) hipLaunchKernelGGL(k_mg_up<true>, stream_grid(A), blk, 0, ctx->stream, A);
-        else hipLaunchKernelGGL(k_mg_up<false>, stream_grid(A), blk, 0, ctx->stream, A);
+        if (l == 0 && A.w2 != A.w) hipLaunchKernelGGL((k_mg_up<true, true>), stream_grid(A), blk, 0, ctx->stream, A);
+        else if (l == 0) hipLaunchKernelGGL((k_mg_up<true, false>), stream_grid(A), blk, 0, ctx->stream, A);
+        else hipLaunchKernelGGL((k_mg_up<false, false>), stream_grid(A), blk, 0, ctx->stream, A);
         prof_end(ctx, l == 0 ? "mg_up_l0" : "mg_coarse");
     }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
