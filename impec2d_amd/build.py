@@ -29,17 +29,22 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False, jobs=8):
-    """Compiles every source to an object (in parallel) and links the .so."""
-    objdir = os.path.join(HERE, "_build")
+def build(verbose=False, jobs=8, out=LIB, defines=()):
+    """Compiles every source to an object (in parallel) and links the .so.
+
+    `defines` are extra -D options (the kernels' tuning constants, e.g.
+    "IMPEC_SW_PF=2"); `out` is the library path (objects go beside it)."""
+    objdir = os.path.join(os.path.dirname(os.path.abspath(out)), "_build")
+    if out != LIB:
+        objdir = os.path.join(os.path.dirname(os.path.abspath(out)), "_build_" + os.path.basename(out))
     os.makedirs(objdir, exist_ok=True)
     procs = []
     objs = []
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-c",
-                                         os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc()] + HIPCC_FLAGS + ["-D" + d for d in defines] + ["-I", os.path.join(ROOT, "include"), "-c",
+                                                                      os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -47,8 +52,8 @@ def build(verbose=False, jobs=8):
             procs[0][1].wait()
     failed = []
     for src, p in procs:
-        out, _ = p.communicate()
-        text = out.decode(errors="replace")
+        log, _ = p.communicate()
+        text = log.decode(errors="replace")
         # hipcc's host pass does not know the packed-fp32 feature: harmless
         lines = [l for l in text.splitlines() if "packed-fp32-ops" not in l and l.strip()]
         if p.returncode != 0:
@@ -58,11 +63,11 @@ def build(verbose=False, jobs=8):
             print("\n".join(lines))
     if failed:
         raise RuntimeError("hipcc failed for: " + ", ".join(failed))
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
     subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

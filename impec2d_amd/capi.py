@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libimpec_hip.so")
+# IMPEC_HIP_LIB selects another build of the library (e.g. one with other tuning constants)
+LIB_PATH = os.environ.get("IMPEC_HIP_LIB") or os.path.join(HERE, "libimpec_hip.so")
 
 IMPEC_OK = 0
 IMPEC_EINVAL = -1

@@ -55,6 +55,29 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
     }
 }
 
+// this thread's share of a combine over n partials p[0..n) read with stride
+// blockDim.x, eight independent loads in flight (a fixed order: the result
+// does not depend on timing)
+template <int OP>
+__device__ __forceinline__ double strided_combine(const double *__restrict__ p, int n) {
+    const int T = blockDim.x;
+    double acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc[u] = red_identity<OP>();
+    int b = threadIdx.x;
+    for (; b + 7 * T < n; b += 8 * T) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = p[b + u * T];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc[u] = red_combine<OP>(acc[u], v[u]);
+    }
+    for (; b < n; b += T) acc[0] = red_combine<OP>(acc[0], p[b]);
+#pragma unroll
+    for (int u = 1; u < 8; u++) acc[0] = red_combine<OP>(acc[0], acc[u]);
+    return acc[0];
+}
+
 // mu(c) of EllipticSolver.cpp:692-695
 __device__ __forceinline__ double mu_of(double c, double mob1over4, double mu_res) {
     double x = (1. - c + mob1over4 * c);
@@ -73,6 +96,20 @@ __device__ __forceinline__ double vertex_mob(double w_ip1j, double w_ij, double 
                                              int harmonic) {
     double s = w_ip1j + w_ij + w_ijp1 + w_ip1jp1;
     return harmonic ? 4. / s : 1. / 4 * s;
+}
+
+// lane l receives v of lane l-1 (lane 0 receives 0): DPP wave_shr:1 (a register move,
+// no memory traffic)
+__device__ __forceinline__ double lane_prev(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+// lane l receives v of lane l+1 (lane 63 receives 0): DPP wave_shl:1
+__device__ __forceinline__ double lane_next(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
 }
 
 }  // namespace impec

@@ -11,6 +11,12 @@ namespace impec {
 constexpr int GHOST = 2;          // num_ghostcells of every shipped config
 constexpr int MAX_LEVELS = 24;
 constexpr int RED_BLOCKS = 65536; // max partial sums per reduction pass
+#ifndef IMPEC_VEC_BLOCKS
+#define IMPEC_VEC_BLOCKS 65536
+#endif
+// workgroups of the grid-stride reduction kernels (their partials are summed
+// by one workgroup); build-time tunable
+constexpr int VEC_BLOCKS = IMPEC_VEC_BLOCKS;
 
 // Scalars kept in device memory so that solver loops need no host round trip
 // except the convergence test.
@@ -78,6 +84,7 @@ struct impec_ctx {
     int nlev;
     impec::Level lev[impec::MAX_LEVELS];
     double *pr, *pz, *pd, *pq; // PCG vectors on the level-0 ring-1 layout
+    double *fx, *fy;           // level-0 face mobilities of the 5-point operator (per solve)
     double *scal;              // device scalars (Scal)
     double *part;              // reduction partials
     double *h_scal;            // pinned host mirror of scal

@@ -134,120 +134,260 @@ __global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const
     rhs[(long)i * k.N2 + j] = r;
     double a[9];
     cn_row(k, d11, d12, d22, d21, i, j, a);
-    diag[(long)i * k.N2 + j] = a[ST_C];
+    diag[(long)i * k.N2 + j] = 1. / a[ST_C];   // Jacobi preconditioner
 }
 
-// y = A x on the flat N1*N2 layout, with optional partial dot products
-// (y.s and y.y) for BiCGStab
+// ---------------------------------------------------------------------------
+// Jacobi-preconditioned BiCGStab, streamed.  The two operator applications of
+// an iteration carry the vector update that produces their input:
+//   k_cn_stream<0>: p = r + beta (p - omega v),  v = A (p / D),  partial r0.v
+//   k_cn_stream<1>: s = r - alpha v,             t = A (s / D),  partials t.s, t.t
+// and k_bicg_xr finishes the iteration (x += alpha p/D + omega s/D,
+// r = s - omega t, partials r.r and r0.r).  A wavefront owns a strip of 64
+// columns (one per lane, 62 written) and a chunk of rows and marches down the
+// rows with the input vector of three rows in registers; the column
+// neighbours come from the adjacent lanes.  The scalars live in device memory
+// (rho_new = r0.r, rho, alpha, omega), so an iteration needs one host
+// synchronisation, for the convergence test.
+// ---------------------------------------------------------------------------
+constexpr int S_RHO_NEW = 13;
+constexpr int CN_OUT = 62, CN_WAVES = 4;
+// build-time tuning: rows in flight per wavefront, wavefronts per launch
+#ifndef IMPEC_CN_PF
+#define IMPEC_CN_PF 4
+#endif
+#ifndef IMPEC_CN_TARGET_WAVES
+#define IMPEC_CN_TARGET_WAVES 4096
+#endif
+
+struct CnArgs {
+    CnConst k;
+    const double *d11, *d12, *d22, *d21;   // face tensors, pitch ldv
+    const double *dinv;                    // inverse of the folded diagonal (flat N1 x N2)
+    const double *r, *pin, *vin, *r0;      // flat N1 x N2
+    double *pout;                          // <0>: new p; <1>: s
+    double *out;                           // <0>: v; <1>: t
+    const double *S;
+    double *part;                          // per-wave partials
+    int R, nstrip, nwave;
+};
+
 template <int MODE>
-__global__ void k_cn_apply(CnConst k, const double *__restrict__ d11, const double *__restrict__ d12,
-                           const double *__restrict__ d22, const double *__restrict__ d21, const double *__restrict__ x,
-                           double *__restrict__ y, const double *__restrict__ w, double *__restrict__ part) {
-    const long n = (long)k.N1 * k.N2;
-    double s[2] = {0., 0.};
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(id / k.N2), j = (int)(id - (long)i * k.N2);
-        double a[9];
-        cn_row(k, d11, d12, d22, d21, i, j, a);
-        const double *xc = x + id;
-        const long ld = k.N2;
-        double acc = a[ST_C] * xc[0];
-        const bool we = i > 0, ea = i < k.N1 - 1, so = j > 0, no = j < k.N2 - 1;
-        if (we) acc += a[ST_W] * xc[-ld];
-        if (we && so) acc += a[ST_SW] * xc[-ld - 1];
-        if (so) acc += a[ST_S] * xc[-1];
-        if (ea && so) acc += a[ST_SE] * xc[ld - 1];
-        if (ea) acc += a[ST_E] * xc[ld];
-        if (ea && no) acc += a[ST_NE] * xc[ld + 1];
-        if (no) acc += a[ST_N] * xc[1];
-        if (we && no) acc += a[ST_NW] * xc[-ld + 1];
-        y[id] = acc;
-        if (MODE == 1) s[0] += w[id] * acc;            // r0 . v
-        if (MODE == 2) { s[0] += acc * w[id]; s[1] += acc * acc; }  // t . s, t . t
+__global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
+    const int lane = threadIdx.x & 63;
+    const int wid = blockIdx.x * CN_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wid >= A.nwave) return;
+    const CnConst &k = A.k;
+    const int N1 = k.N1, N2 = k.N2;
+    const long L = k.ldv;
+    const int strip = wid % A.nstrip, chunk = wid / A.nstrip;
+    const int j = strip * CN_OUT - 1 + lane;
+    const bool jv = j >= 0 && j < N2;
+    const int jc = j < 0 ? 0 : (j > N2 - 1 ? N2 - 1 : j);   // cell column for loads
+    const int jf = j < 0 ? 0 : (j > N2 ? N2 : j);           // face column for loads (faces run to N2)
+    const bool outl = lane >= 1 && lane <= CN_OUT && jv;
+    const int i0 = chunk * A.R, i1 = i0 + A.R < N1 ? i0 + A.R : N1;
+    double beta = 0., om = 0., alpha = 0.;
+    if (MODE == 0) {
+        beta = (A.S[S_RHO_NEW] / A.S[S_RHO]) * (A.S[S_ALPHA] / A.S[S_OMEGA]);
+        om = A.S[S_OMEGA];
+    } else {
+        alpha = A.S[S_ALPHA];
     }
-    if (MODE) {
-        block_reduce<RED_SUM, 2>(s);
-        if (tid_linear() == 0) {
-            part[blockIdx.x] = s[0];
-            part[RED_BLOCKS + blockIdx.x] = s[1];
+    // raw row loads from clamped addresses (no lane-dependent branches)
+    auto cell = [&](const double *p, int m) -> double {
+        const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
+        return p[(long)mc * N2 + jc];
+    };
+    auto xface = [&](const double *p, int m) -> double {   // x-faces: rows 0..N1
+        const int mc = m < 0 ? 0 : (m > N1 ? N1 : m);
+        return p[(long)mc * L + jc];
+    };
+    auto yface = [&](const double *p, int m) -> double {   // y-faces: columns 0..N2
+        const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
+        return p[(long)mc * L + jf];
+    };
+    // ring of two prefetched rows: r, p, v, D^-1, r0 of row m; x-faces of row m
+    // (between m-1 and m); y-faces of row m
+    constexpr int PF = IMPEC_CN_PF;   // rows in flight; a slot is refilled at the end of the step that used it
+    double Pr[PF] = {}, Pp[PF] = {}, Pv[PF] = {}, Pd[PF] = {}, P0[PF] = {}, P11[PF] = {}, P12[PF] = {},
+           P22[PF] = {}, P21[PF] = {};
+    auto fetch = [&](int u, int m) {
+        Pr[u] = cell(A.r, m);
+        if (MODE == 0) Pp[u] = cell(A.pin, m);
+        Pv[u] = cell(A.vin, m);
+        Pd[u] = cell(A.dinv, m);
+        if (MODE == 0) P0[u] = cell(A.r0, m);
+        P11[u] = xface(A.d11, m);
+        P12[u] = xface(A.d12, m);
+        P22[u] = yface(A.d22, m);
+        P21[u] = yface(A.d21, m);
+    };
+    const int mstart = i0 - 1, mend = i1;
+    const int mstop = mstart + (mend - mstart + PF) / PF * PF;
+#pragma unroll
+    for (int u = 0; u < PF; u++) fetch(u, mstart + u);
+    // windows: input vector y of rows m-2, m-1, m; x-faces of rows m-1, m;
+    // y-faces of row m-1; the written vector (p or s) and r0 of row m-1
+    double Y0 = 0., Y1 = 0., Y2 = 0.;
+    double f11a = 0., f12a = 0., f11b = 0., f12b = 0., f22a = 0., f21a = 0.;
+    double wprev = 0., r0prev = 0.;
+    double s0 = 0., s1 = 0.;
+    for (int mb = mstart; mb < mstop; mb += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int m = mb + u;
+            const double rv = Pr[u], pv = Pp[u], vv = Pv[u], dv = Pd[u], r0v = P0[u];
+            const double e11 = P11[u], e12 = P12[u], e22 = P22[u], e21 = P21[u];
+            const bool rin = m >= 0 && m < N1;
+            // the input vector on row m (and the written vector)
+            double wv;
+            if (MODE == 0) wv = rv + beta * (pv - om * vv);
+            else wv = rv - alpha * vv;
+            const double yv = (rin && jv) ? wv * dv : 0.;
+            if (rin && m >= i0 && m < i1 && outl) A.pout[(long)m * N2 + j] = wv;
+            Y0 = Y1; Y1 = Y2; Y2 = yv;
+            f11a = f11b; f12a = f12b; f11b = e11; f12b = e12;
+            // row q = m - 1: GetCoefficients_Parabolic with the faces in registers
+            const int q = m - 1;
+            const double d22p = lane_next(f22a), d21p = lane_next(f21a);
+            if (q >= i0 && q < i1) {
+                double a[9];
+                const double d11m = f11a, d11p = f11b, d12m = f12a, d12p = f12b, d22m = f22a, d21m = f21a;
+                a[ST_NW] = k.cA * (d12m + d21p);
+                a[ST_NE] = k.mcA * (d12p + d21p);
+                a[ST_N] = k.mdt8 * (k.c4dy2 * d22p + k.c1dxdy * (-d12m + d12p));
+                a[ST_SW] = k.mcA * (d12m + d21m);
+                a[ST_SE] = k.cA * (d12p + d21m);
+                a[ST_S] = k.mdt8 * (k.c4dy2 * d22m + k.c1dxdy * (-d12p + d12m));
+                a[ST_W] = k.mdt8 * (k.c4dx2 * d11m + k.c1dxdy * (-d21p + d21m));
+                a[ST_C] = 1. + k.cx2 * (d11p + d11m) + k.cy2 * (d22p + d22m);
+                a[ST_E] = k.mdt8 * (k.c4dx2 * d11p + k.c1dxdy * (-d21m + d21p));
+                cn_fold(q, j, N1, N2, a);
+                const double yl0 = lane_prev(Y0), yl1 = lane_prev(Y1), yl2 = lane_prev(Y2);
+                const double yr0 = lane_next(Y0), yr1 = lane_next(Y1), yr2 = lane_next(Y2);
+                const bool we = q > 0, ea = q < N1 - 1, so = j > 0, no = j < N2 - 1;
+                double acc = a[ST_C] * Y1;
+                if (we) acc += a[ST_W] * Y0;
+                if (we && so) acc += a[ST_SW] * yl0;
+                if (so) acc += a[ST_S] * yl1;
+                if (ea && so) acc += a[ST_SE] * yl2;
+                if (ea) acc += a[ST_E] * Y2;
+                if (ea && no) acc += a[ST_NE] * yr2;
+                if (no) acc += a[ST_N] * yr1;
+                if (we && no) acc += a[ST_NW] * yr0;
+                if (outl) {
+                    A.out[(long)q * N2 + j] = acc;
+                    if (MODE == 0) s0 += r0prev * acc;
+                    else {
+                        s0 += acc * wprev;
+                        s1 += acc * acc;
+                    }
+                }
+            }
+            f22a = e22; f21a = e21;
+            wprev = wv;
+            r0prev = r0v;
+            fetch(u, m + PF);
         }
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+    }
+    if (lane == 0) {
+        A.part[wid] = s0;
+        A.part[RED_BLOCKS + wid] = s1;
+    }
 }
 
-// BiCGStab vector steps; the scalars rho, r0.r of the next step (rho_new),
-// alpha and omega live in device memory
-constexpr int S_RHO_NEW = 13;
+// launch shape of the element-wise BiCGStab kernels: 64 x 4 threads over
+// (column, row), rows strided so that there are at most CN_VBLOCKS blocks
+constexpr int CN_VBLOCKS = 2048;
+static dim3 cn_vgrid(int N1, int N2) {
+    const int bx = (N2 + 63) / 64;
+    int by = (N1 + 3) / 4;
+    if ((long)bx * by > CN_VBLOCKS) by = CN_VBLOCKS / bx > 0 ? CN_VBLOCKS / bx : 1;
+    return dim3(bx, by);
+}
 
-__global__ void k_bicg_init(long n, const double *__restrict__ b, double *__restrict__ r, double *__restrict__ r0,
-                            double *__restrict__ p, double *__restrict__ vv, double *__restrict__ x,
-                            double *__restrict__ part) {
+// r = r0 = b, p = v = x = 0; partials b.b (twice: ||b||^2 and r0.r)
+__global__ void k_bicg_init(int N1, int N2, const double *__restrict__ b, double *__restrict__ r,
+                            double *__restrict__ r0, double *__restrict__ p, double *__restrict__ vv,
+                            double *__restrict__ x, double *__restrict__ part) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
     double s[2] = {0., 0.};
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (long)gridDim.x * blockDim.x) {
-        const double bv = b[id];
-        r[id] = bv;
-        r0[id] = bv;
-        p[id] = 0.;
-        vv[id] = 0.;
-        x[id] = 0.;
-        s[0] += bv * bv;
-        s[1] += bv * bv;
-    }
+    if (j < N2)
+        for (int i = blockIdx.y * 4 + threadIdx.y; i < N1; i += gridDim.y * 4) {
+            const long id = (long)i * N2 + j;
+            const double bv = b[id];
+            r[id] = bv;
+            r0[id] = bv;
+            p[id] = 0.;
+            vv[id] = 0.;
+            x[id] = 0.;
+            s[0] += bv * bv;
+            s[1] += bv * bv;
+        }
     block_reduce<RED_SUM, 2>(s);
     if (tid_linear() == 0) {
-        part[blockIdx.x] = s[0];
-        part[RED_BLOCKS + blockIdx.x] = s[1];
+        const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+        part[blk] = s[0];
+        part[RED_BLOCKS + blk] = s[1];
     }
 }
 
-__global__ void k_bicg_scal_init(double *__restrict__ S) {
-    S[S_RHO] = 1.; S[S_ALPHA] = 1.; S[S_OMEGA] = 1.;
-}
-
-// p = r + beta (p - omega v), y = p / diag; then rho <- rho_new
-__global__ void k_bicg_p(long n, const double *__restrict__ r, double *__restrict__ p, const double *__restrict__ vv,
-                         const double *__restrict__ diag, double *__restrict__ y, const double *__restrict__ S) {
-    const double beta = (S[S_RHO_NEW] / S[S_RHO]) * (S[S_ALPHA] / S[S_OMEGA]);
-    const double om = S[S_OMEGA];
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (long)gridDim.x * blockDim.x) {
-        const double pv = r[id] + beta * (p[id] - om * vv[id]);
-        p[id] = pv;
-        y[id] = pv / diag[id];
-    }
-}
-
-__global__ void k_bicg_rho(double *__restrict__ S) { S[S_RHO] = S[S_RHO_NEW]; }
-__global__ void k_bicg_alpha(double *__restrict__ S) { S[S_ALPHA] = S[S_RHO] / S[S_R0V]; }
-__global__ void k_bicg_omega(double *__restrict__ S) { S[S_OMEGA] = S[S_TT] > 0 ? S[S_TS] / S[S_TT] : 0.; }
-
-// s = r - alpha v, z = s / diag
-__global__ void k_bicg_s(long n, const double *__restrict__ r, const double *__restrict__ vv,
-                         const double *__restrict__ diag, double *__restrict__ sv, double *__restrict__ z,
-                         const double *__restrict__ S) {
-    const double alpha = S[S_ALPHA];
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (long)gridDim.x * blockDim.x) {
-        const double s = r[id] - alpha * vv[id];
-        sv[id] = s;
-        z[id] = s / diag[id];
-    }
-}
-
-// x += alpha y + omega z, r = s - omega t; partials r.r, r0.r
-__global__ void k_bicg_update(long n, double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
-                              double *__restrict__ r, const double *__restrict__ sv, const double *__restrict__ t,
-                              const double *__restrict__ r0, const double *__restrict__ S, double *__restrict__ part) {
+// x += alpha p/D + omega s/D, r = s - omega t; partials r.r, r0.r
+__global__ void k_bicg_xr(int N1, int N2, double *__restrict__ x, const double *__restrict__ p,
+                          const double *__restrict__ sv, const double *__restrict__ dinv, double *__restrict__ r,
+                          const double *__restrict__ t, const double *__restrict__ r0, const double *__restrict__ S,
+                          double *__restrict__ part) {
     const double alpha = S[S_ALPHA], om = S[S_OMEGA];
+    const int j = blockIdx.x * 64 + threadIdx.x;
     double s[2] = {0., 0.};
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (long)gridDim.x * blockDim.x) {
-        x[id] += alpha * y[id] + om * z[id];
-        const double rv = sv[id] - om * t[id];
-        r[id] = rv;
-        s[0] += rv * rv;
-        s[1] += r0[id] * rv;
-    }
+    if (j < N2)
+        for (int i = blockIdx.y * 4 + threadIdx.y; i < N1; i += gridDim.y * 4) {
+            const long id = (long)i * N2 + j;
+            const double d = dinv[id], sval = sv[id];
+            x[id] += alpha * (p[id] * d) + om * (sval * d);
+            const double rv = sval - om * t[id];
+            r[id] = rv;
+            s[0] += rv * rv;
+            s[1] += r0[id] * rv;
+        }
     block_reduce<RED_SUM, 2>(s);
     if (tid_linear() == 0) {
-        part[blockIdx.x] = s[0];
-        part[RED_BLOCKS + blockIdx.x] = s[1];
+        const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+        part[blk] = s[0];
+        part[RED_BLOCKS + blk] = s[1];
+    }
+}
+
+// second pass of the BiCGStab reductions and the scalar recurrences:
+//  stage 0 (init):    rr = b.b, rho = alpha = omega = 1, rho_new = b.b
+//  stage 1 (after <0>): r0.v; rho = rho_new; alpha = rho / r0.v
+//  stage 2 (after <1>): t.s, t.t; omega = t.s / t.t (0 when t = 0)
+//  stage 3 (after xr):  rr = r.r, rho_new = r0.r
+__global__ void k_bicg_finish(const double *__restrict__ part, int nb, int stage, double *__restrict__ S) {
+    double v[2] = {strided_combine<RED_SUM>(part, nb), strided_combine<RED_SUM>(part + RED_BLOCKS, nb)};
+    block_reduce<RED_SUM, 2>(v);
+    if (threadIdx.x != 0) return;
+    if (stage == 0) {
+        S[S_RR] = v[0];
+        S[S_RHO] = 1.; S[S_ALPHA] = 1.; S[S_OMEGA] = 1.;
+        S[S_RHO_NEW] = v[1];
+    } else if (stage == 1) {
+        S[S_R0V] = v[0];
+        S[S_RHO] = S[S_RHO_NEW];
+        S[S_ALPHA] = S[S_RHO] / v[0];
+    } else if (stage == 2) {
+        S[S_TS] = v[0];
+        S[S_TT] = v[1];
+        S[S_OMEGA] = v[1] > 0 ? v[0] / v[1] : 0.;
+    } else {
+        S[S_RR] = v[0];
+        S[S_RHO_NEW] = v[1];
     }
 }
 
@@ -295,16 +435,7 @@ static CnConst cn_const(const impec_ctx *ctx, double dt) {
     return k;
 }
 
-static int red_grid(long n) {
-    long b = (n + 1023) / 1024;
-    if (b < 1) b = 1;
-    if (b > RED_BLOCKS) b = RED_BLOCKS;
-    return (int)b;
-}
-
-__global__ void k_copy_scal_cn(double *__restrict__ S, int dst, int src) { S[dst] = S[src]; }
-
-// face tensors, right-hand side (into cn_rhs) and folded diagonal (into cn_dg)
+// face tensors, right-hand side (into cn_rhs) and inverse folded diagonal (into cn_dg)
 static int tensors_and_rhs(impec_ctx *ctx, const CnConst &k) {
     dim3 blk(64, 4);
     dim3 gt((ctx->N2 + 1 + 63) / 64, (ctx->N1 + 1 + 3) / 4);
@@ -325,58 +456,76 @@ static int fetch(impec_ctx *ctx) {
 }
 
 // Solve_Parabolic_cc_to_c: x0 = 0 (AssignRHS_Parabolic, MGSolver.cpp:1375-1392),
-// stop on ||rhs - A x||_2 <= para_eps ||rhs||_2, solution into c's interior.
+// stop on ||rhs - A x||_2 <= para_eps ||rhs||_2 (recursive residual), solution
+// into c's interior.
 int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     const CnConst k = cn_const(ctx, dt);
-    const long n = (long)ctx->N1 * ctx->N2;
-    const int nb = red_grid(n);
+    const int N1 = ctx->N1, N2 = ctx->N2;
     hipStream_t s = ctx->stream;
-    double *b = ctx->cn_rhs, *x = ctx->cn_x, *r = ctx->cn_r, *r0 = ctx->cn_r0, *p = ctx->cn_p, *vv = ctx->cn_v;
-    double *sv = ctx->cn_s, *t = ctx->cn_t, *y = ctx->cn_y, *dg = ctx->cn_dg, *z = ctx->cn_z;
+    double *b = ctx->cn_rhs, *x = ctx->cn_x, *r = ctx->cn_r, *r0 = ctx->cn_r0, *sv = ctx->cn_s, *t = ctx->cn_t;
+    double *pb[2] = {ctx->cn_p, ctx->cn_y}, *vb[2] = {ctx->cn_v, ctx->cn_z};
     prof_begin(ctx);
     int st = tensors_and_rhs(ctx, k);
     prof_end(ctx, "cn_tensor_rhs");
     if (st) return st;
-    hipLaunchKernelGGL(k_bicg_init, dim3(nb), dim3(256), 0, s, n, b, r, r0, p, vv, x, ctx->part);
-    IMPEC_HIP_CHECK(ctx, hipGetLastError());
-    if ((st = finish_sum(ctx, nb, 2, S_RR))) return st;   // S_RR = b.b, S_RSUM = r0.r = b.b
-    hipLaunchKernelGGL(k_bicg_scal_init, dim3(1), dim3(1), 0, s, ctx->scal);
-    hipLaunchKernelGGL(k_copy_scal_cn, dim3(1), dim3(1), 0, s, ctx->scal, S_RHO_NEW, (int)S_RSUM);
+    const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
+    const int nvb = (int)(vg.x * vg.y);
+    hipLaunchKernelGGL(k_bicg_init, vg, vblk, 0, s, N1, N2, b, r, r0, pb[0], vb[0], x, ctx->part);
+    hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 0, ctx->scal);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     if ((st = fetch(ctx))) return st;
     const double bb = ctx->h_scal[S_RR];
     double rr = bb;
     const double tol = ctx->P.para_eps;
     const int maxit = ctx->P.max_iterations > 0 ? ctx->P.max_iterations : 10000;
-    int its = 0, status = IMPEC_OK;
+
+    CnArgs a;
+    a.k = k;
+    a.d11 = ctx->d11; a.d12 = ctx->d12; a.d22 = ctx->d22; a.d21 = ctx->d21;
+    a.dinv = ctx->cn_dg;
+    a.r = r; a.r0 = r0;
+    a.S = ctx->scal;
+    a.part = ctx->part;
+    a.nstrip = (N2 + CN_OUT - 1) / CN_OUT;
+    {
+        // about IMPEC_CN_TARGET_WAVES wavefronts; chunks of at least 8 rows
+        long R = ((long)N1 * a.nstrip + IMPEC_CN_TARGET_WAVES - 1) / IMPEC_CN_TARGET_WAVES;
+        if (R < 8) R = 8;
+        if (R > N1) R = N1;
+        a.R = (int)R;
+    }
+    a.nwave = a.nstrip * ((N1 + a.R - 1) / a.R);
+    if (a.nwave > RED_BLOCKS) return set_error(ctx, IMPEC_EINVAL, "dispersion solve: grid too large");
+    const dim3 sg((a.nwave + CN_WAVES - 1) / CN_WAVES), sblk(64 * CN_WAVES);
+    int cur = 0, its = 0, status = IMPEC_OK;
     while (!(rr <= tol * tol * bb)) {
         if (its >= maxit) { status = IMPEC_ENOCONV; break; }
         if (!(rr == rr)) { status = IMPEC_ENAN; break; }
-        hipLaunchKernelGGL(k_bicg_p, dim3(nb), dim3(256), 0, s, n, r, p, vv, dg, y, ctx->scal);
-        hipLaunchKernelGGL(k_bicg_rho, dim3(1), dim3(1), 0, s, ctx->scal);
+        // p, v ping-pong: a strip reads its neighbours' old values at the halo
+        a.pin = pb[cur]; a.vin = vb[cur]; a.pout = pb[cur ^ 1]; a.out = vb[cur ^ 1];
         prof_begin(ctx);
-        hipLaunchKernelGGL(k_cn_apply<1>, dim3(nb), dim3(256), 0, s, k, ctx->d11, ctx->d12, ctx->d22, ctx->d21, y, vv,
-                           r0, ctx->part);
+        hipLaunchKernelGGL(k_cn_stream<0>, sg, sblk, 0, s, a);
         prof_end(ctx, "cn_apply");
-        if ((st = finish_sum(ctx, nb, 1, S_R0V))) return st;
-        hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(1), 0, s, ctx->scal);
-        hipLaunchKernelGGL(k_bicg_s, dim3(nb), dim3(256), 0, s, n, r, vv, dg, sv, z, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 1, ctx->scal);
+        CnArgs a2 = a;
+        a2.pin = nullptr; a2.vin = vb[cur ^ 1]; a2.pout = sv; a2.out = t;
         prof_begin(ctx);
-        hipLaunchKernelGGL(k_cn_apply<2>, dim3(nb), dim3(256), 0, s, k, ctx->d11, ctx->d12, ctx->d22, ctx->d21, z, t,
-                           sv, ctx->part);
+        hipLaunchKernelGGL(k_cn_stream<1>, sg, sblk, 0, s, a2);
         prof_end(ctx, "cn_apply");
-        if ((st = finish_sum(ctx, nb, 2, S_TS))) return st;   // S_TS = t.s, S_TT = t.t
-        hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(1), 0, s, ctx->scal);
-        hipLaunchKernelGGL(k_bicg_update, dim3(nb), dim3(256), 0, s, n, x, y, z, r, sv, t, r0, ctx->scal, ctx->part);
-        if ((st = finish_sum(ctx, nb, 2, S_RR))) return st;   // S_RR = r.r, S_RSUM = r0.r
-        hipLaunchKernelGGL(k_copy_scal_cn, dim3(1), dim3(1), 0, s, ctx->scal, S_RHO_NEW, (int)S_RSUM);
+        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 2, ctx->scal);
+        prof_begin(ctx);
+        hipLaunchKernelGGL(k_bicg_xr, vg, vblk, 0, s, N1, N2, x, pb[cur ^ 1], sv, ctx->cn_dg, r, t, r0, ctx->scal,
+                           ctx->part);
+        prof_end(ctx, "cn_vector");
+        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 3, ctx->scal);
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = fetch(ctx))) return st;
         rr = ctx->h_scal[S_RR];
+        cur ^= 1;
         its++;
     }
-    dim3 blk(64, 4), gc((ctx->N2 + 63) / 64, (ctx->N1 + 3) / 4);
-    hipLaunchKernelGGL(k_cn_store, gc, blk, 0, s, x, ctx->c, ctx->N1, ctx->N2, ctx->B, ctx->lds);
+    dim3 blk(64, 4), gc((N2 + 63) / 64, (N1 + 3) / 4);
+    hipLaunchKernelGGL(k_cn_store, gc, blk, 0, s, x, ctx->c, N1, N2, ctx->B, ctx->lds);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     ctx->stats.last_parabolic_its = its;
     ctx->stats.last_parabolic_relres = bb > 0 ? sqrt(rr / bb) : 0.;

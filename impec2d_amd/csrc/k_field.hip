@@ -216,8 +216,7 @@ template <int OP>
 __global__ void k_finish(const double *__restrict__ part, int nblocks, int nv, double *__restrict__ scal,
                          int dst) {
     for (int k = 0; k < nv; k++) {
-        double v[1] = {red_identity<OP>()};
-        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v[0] = red_combine<OP>(v[0], part[k * RED_BLOCKS + b]);
+        double v[1] = {strided_combine<OP>(part + k * RED_BLOCKS, nblocks)};
         block_reduce<OP, 1>(v);
         if (threadIdx.x == 0) scal[dst + k] = v[0];
         __syncthreads();
@@ -236,7 +235,7 @@ int finish_sum(impec_ctx *ctx, int nblocks, int nv, int dst) { return finish_op<
 static int red_blocks(long long n) {
     long long b = (n + 1023) / 1024;
     if (b < 1) b = 1;
-    if (b > RED_BLOCKS) b = RED_BLOCKS;
+    if (b > VEC_BLOCKS) b = VEC_BLOCKS;
     return (int)b;
 }
 

@@ -161,7 +161,50 @@ struct LevArgs {
     const double *w1, *w2;  // at cell (0,0)
     int ldw;
     int ld;                 // pitch of x/b/r
+    // level 0 of a 5-point solve: the face mobilities of op_row, formed once
+    // per solve (fx(i,j) between rows i-1 and i, fy(i,j) between columns j-1
+    // and j; pitch ld); null elsewhere
+    const double *fx, *fy;
 };
+
+// op_row<false> from the stored face mobilities when present (the same
+// values and arithmetic, without the four divisions), else op_row<false>
+__device__ __forceinline__ void op_row5(const LevArgs &L, int i, int j, double a[9]) {
+    if (!L.fx) {
+        op_row<false>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+        return;
+    }
+    const OpGeom &g = L.g;
+    const long o = (long)i * L.ld + j;
+    const double fxm = L.fx[o], fxp = L.fx[o + L.ld], fym = L.fy[o], fyp = L.fy[o + 1];
+    const double Wp = g.cx * fxm, Ep = g.cx * fxp, Sp = g.cy * fym, Np = g.cy * fyp;
+    const double Cp = g.mcy * (fym + fyp) - g.cx * (fxm + fxp);
+    a[ST_C] = Cp; a[ST_W] = Wp; a[ST_SW] = 0.; a[ST_S] = Sp; a[ST_SE] = 0.;
+    a[ST_E] = Ep; a[ST_NE] = 0.; a[ST_N] = Np; a[ST_NW] = 0.;
+    fold_row(i, j, g.n1, g.n2, a);
+}
+
+template <bool DIAG>
+__device__ __forceinline__ void op_row_L(const LevArgs &L, int i, int j, double a[9]) {
+    if (DIAG) op_row<true>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+    else op_row5(L, i, j, a);
+}
+
+#ifndef IMPEC_NO_FACES
+#define IMPEC_NO_FACES 0
+#endif
+// face mobilities of the level-0 5-point operator (see LevArgs)
+__global__ void k_faces(LevArgs L, double *__restrict__ fx, double *__restrict__ fy) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int n1 = L.g.n1, n2 = L.g.n2;
+#define W1(a_, b_) L.w1[(long)(a_) * L.ldw + (b_)]
+#define W2(a_, b_) L.w2[(long)(a_) * L.ldw + (b_)]
+    if (i <= n1 && j < n2) fx[(long)i * L.ld + j] = face_mob(W1(i - 1, j), W1(i, j), L.g.harmonic);
+    if (i < n1 && j <= n2) fy[(long)i * L.ld + j] = face_mob(W2(i, j - 1), W2(i, j), L.g.harmonic);
+#undef W1
+#undef W2
+}
 
 // ---------------------------------------------------------------------------
 // multigrid kernels
@@ -355,7 +398,7 @@ __global__ void k_cg_init(LevArgs L, CgArgs c, double *__restrict__ p, const dou
         int i, j;
         cg_cell(c, id, i, j);
         double a[9];
-        op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+        op_row_L<DIAG>(L, i, j, a);
         double xv[9];
         (void)xv;
         double bij = -q[id];
@@ -453,7 +496,7 @@ __global__ void k_cg_matvec(LevArgs L, CgArgs c, const double *__restrict__ d, d
         int i, j;
         cg_cell(c, id, i, j);
         double a[9];
-        op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+        op_row_L<DIAG>(L, i, j, a);
         double y = row_dot<DIAG>(a, d0, c.ld, i, j, c.n1, c.n2);
         if (!c.shift && i == c.pin_i && j == c.pin_j) y = 0.;
         const long o = (long)(i + 1) * c.ld + (j + 1);
@@ -518,7 +561,7 @@ __global__ void k_true_residual(LevArgs L, CgArgs c, const double *__restrict__ 
             rv = -q[id] - c.dpin * x[(long)i * c.ldp + j];
         } else {
             double a[9];
-            op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+            op_row_L<DIAG>(L, i, j, a);
             rv = -q[id] - row_dot<DIAG>(a, x, c.ldp, i, j, c.n1, c.n2);
         }
         s[0] += rv * rv;
@@ -574,6 +617,7 @@ static LevArgs lev_args(const impec_ctx *ctx, int l) {
     const Level &L = ctx->lev[l];
     a.g = L.g;
     a.ld = L.ld;
+    a.fx = a.fy = nullptr;
     if (l == 0) {
         a.ldw = ctx->lds;
         a.w1 = ctx->w + (long)ctx->B * ctx->lds + ctx->B;
@@ -618,10 +662,10 @@ int pressure_setup_levels(impec_ctx *ctx) {
         dx *= 2;
         dy *= 2;
     }
-    // CG vectors on the level-0 layout
+    // CG vectors and the face mobilities on the level-0 layout
     const size_t bytes0 = sizeof(double) * (size_t)(ctx->N1 + 2) * ctx->lev[0].ld;
-    double **vecs[4] = {&ctx->pr, &ctx->pz, &ctx->pd, &ctx->pq};
-    for (int k = 0; k < 4; k++) {
+    double **vecs[6] = {&ctx->pr, &ctx->pz, &ctx->pd, &ctx->pq, &ctx->fx, &ctx->fy};
+    for (int k = 0; k < 6; k++) {
         if (hipMalloc(vecs[k], bytes0) != hipSuccess) return set_error(ctx, IMPEC_ENOMEM, "CG vector allocation failed");
         hipMemset(*vecs[k], 0, bytes0);
     }
@@ -636,6 +680,7 @@ void pressure_free_levels(impec_ctx *ctx) {
     }
     ctx->nlev = 0;
     hipFree(ctx->pr); hipFree(ctx->pz); hipFree(ctx->pd); hipFree(ctx->pq);
+    hipFree(ctx->fx); hipFree(ctx->fy);
 }
 
 static dim3 grid2(int n1, int n2, dim3 blk) { return dim3((n2 + blk.x - 1) / blk.x, (n1 + blk.y - 1) / blk.y); }
@@ -643,7 +688,7 @@ static dim3 grid2(int n1, int n2, dim3 blk) { return dim3((n2 + blk.x - 1) / blk
 static int red_grid(long n) {
     long b = (n + 1023) / 1024;
     if (b < 1) b = 1;
-    if (b > RED_BLOCKS) b = RED_BLOCKS;
+    if (b > VEC_BLOCKS) b = VEC_BLOCKS;
     return (int)b;
 }
 
@@ -791,19 +836,6 @@ struct FusedArgs {
     int R, nstrip, nwave;   // rows per chunk, strips, wavefronts
 };
 
-// lane l receives v of lane l-1 (lane 0 receives 0): DPP wave_shr:1
-__device__ __forceinline__ double lane_prev(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-// lane l receives v of lane l+1 (lane 63 receives 0): DPP wave_shl:1
-__device__ __forceinline__ double lane_next(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-
 // 1/d from the hardware reciprocal estimate refined by one Newton-Raphson
 // step, r' = r + r (1 - d r).  Only the multigrid preconditioner uses it: its
 // rounding differs from IEEE division but is the same every time for the same
@@ -914,7 +946,11 @@ __device__ __forceinline__ void mask_pair(const Lane &L, int m, int n1, double f
     b = (rin && L.v1) ? b : fill;
 }
 
-constexpr int SW_PF = 4;   // prefetch distance in rows (one slot per unrolled step)
+// prefetch distance in rows (one slot per unrolled step); build-time tunable
+#ifndef IMPEC_SW_PF
+#define IMPEC_SW_PF 4
+#endif
+constexpr int SW_PF = IMPEC_SW_PF;
 
 template <bool FINE0, bool ANISO>
 __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
@@ -973,7 +1009,6 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
             double bk[2];
             bk[0] = L.v0 && k >= 0 && k < n1 ? PB[u][0] - mean : 0.;
             bk[1] = L.v1 && k >= 0 && k < n1 ? PB[u][1] - mean : 0.;
-            fetch(u, k + SW_PF);
             const FRow fk = row_faces(A.g, L, k, wc0, wc1, wn0, wn1, y0, y1);
             wc0 = wn0;
             wc1 = wn1;
@@ -1045,6 +1080,9 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
                     acc_n = 0.;
                 }
             }
+            // refill the slot once its registers are dead (an earlier refill
+            // would copy them out and wait for the load that filled them)
+            fetch(u, k + SW_PF);
         }
     }
 }
@@ -1070,31 +1108,22 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
     };
     const int kstart = K.i0 - 2, kend = K.i1 + 1;
 
-    double PX[SW_PF][2], PW[SW_PF][2], PY[SW_PF][2], PB[SW_PF][2], PC[SW_PF] = {0., 0., 0., 0.};
-    auto fetch = [&](int u, int k) {   // inputs of step k: x(k), w(k+1), w2(k), b(k), coarse row (k+3)/2
+    double PX[SW_PF][2], PW[SW_PF][2], PY[SW_PF][2], PB[SW_PF][2], PC[SW_PF][2];
+    // inputs of step k: x(k), w(k+1), w2(k), b(k) and the coarse rows I0 = k>>1,
+    // I1 = I0 -+ 1 of the prolongation (re-read for every fine row; cached)
+    auto fetch = [&](int u, int k) {
         load_pair(A.x, A.ld, L, k, n1, PX[u][0], PX[u][1]);
         load_pair(A.w, A.ldw, L, k + 1, n1, PW[u][0], PW[u][1]);
         if (aniso) load_pair(A.w2, A.ldw, L, k, n1, PY[u][0], PY[u][1]);
         load_pair(src, A.ld, L, k, n1, PB[u][0], PB[u][1]);
-        if (u & 1) PC[u] = coarse((k + 3) >> 1);
+        PC[u][0] = coarse(k >> 1);
+        PC[u][1] = coarse((u & 1) ? (k >> 1) + 1 : (k >> 1) - 1);
     };
 #pragma unroll
     for (int u = 0; u < SW_PF; u++) fetch(u, kstart + u);
     double wc0, wc1;
     load_pair(A.w, A.ldw, L, kstart, n1, wc0, wc1);
     mask_pair(L, kstart, n1, 1., wc0, wc1);
-    // coarse rows (k>>1) - 1, k>>1, (k>>1) + 1 as (left, own, right)
-    double CL[3], CO[3], CR[3];
-    {
-        const int Im = kstart >> 1;
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const double v = coarse(Im - 1 + q);
-            CO[q] = jok ? v : 0.;
-            CL[q] = lane_prev(CO[q]);
-            CR[q] = lane_next(CO[q]);
-        }
-    }
 
     double X[4][2] = {{0., 0.}, {0., 0.}, {0., 0.}, {0., 0.}};   // rows k-3 .. k
     double B[3][2] = {{0., 0.}, {0., 0.}, {0., 0.}};             // raw rhs rows k-2 .. k
@@ -1122,8 +1151,7 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
             }
             double bk0 = PB[u][0], bk1 = PB[u][1];
             mask_pair(L, k, n1, 0., bk0, bk1);
-            const double cnew = jok ? PC[u] : 0.;
-            fetch(u, k + SW_PF);
+            const double o0 = jok ? PC[u][0] : 0., o1 = jok ? PC[u][1] : 0.;
             const FRow fk = row_faces(A.g, L, k, wc0, wc1, wn0, wn1, y0, y1);
             wc0 = wn0;
             wc1 = wn1;
@@ -1136,12 +1164,11 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
             B[0][0] = B[1][0]; B[0][1] = B[1][1]; B[1][0] = B[2][0]; B[1][1] = B[2][1];
             B[2][0] = bk0; B[2][1] = bk1;
             E[3][0] = fk.E0; E[3][1] = fk.E1;
-            // x + P e on row k: rows I0 = k>>1 (CO[1]) and I1 = I0 -+ 1
+            // x + P e on row k from coarse rows I0 (o0) and I1 (o1)
             {
-                const int i1 = c == 0 ? 0 : 2;
-                const double o0 = CO[1], o1 = CO[i1];
-                const double l0 = jfirst ? o0 : CL[1], l1 = jfirst ? o1 : CL[i1];
-                const double r0 = jlast ? o0 : CR[1], r1 = jlast ? o1 : CR[i1];
+                const double pl0 = lane_prev(o0), pl1 = lane_prev(o1), pr0 = lane_next(o0), pr1 = lane_next(o1);
+                const double l0 = jfirst ? o0 : pl0, l1 = jfirst ? o1 : pl1;
+                const double r0 = jlast ? o0 : pr0, r1 = jlast ? o1 : pr1;
                 const double e0 = 0.5625 * o0 + 0.1875 * l0 + 0.1875 * o1 + 0.0625 * l1;
                 const double e1 = 0.5625 * o0 + 0.1875 * r0 + 0.1875 * o1 + 0.0625 * r1;
                 const bool rin = k >= 0 && k < n1;
@@ -1164,13 +1191,7 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
                     if (L.v1) { acc0 += X[1][1]; acc1 += B[0][1] * X[1][1]; }
                 }
             }
-            if (c == 1) {   // next step starts coarse row (k+1)>>1
-#pragma unroll
-                for (int q = 0; q < 2; q++) { CO[q] = CO[q + 1]; CL[q] = CL[q + 1]; CR[q] = CR[q + 1]; }
-                CO[2] = cnew;
-                CL[2] = lane_prev(cnew);
-                CR[2] = lane_next(cnew);
-            }
+            fetch(u, k + SW_PF);   // the slot's registers are dead here
         }
     }
     if (FINE0) {
@@ -1189,11 +1210,7 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
 
 // sums of the V-cycle output and rho = r.z with z = x0 - mean(x0)
 __global__ void k_finish_rho(const double *__restrict__ part, int nblocks, double inv_n, double *__restrict__ S) {
-    double v[2] = {0., 0.};
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-        v[0] += part[b];
-        v[1] += part[RED_BLOCKS + b];
-    }
+    double v[2] = {strided_combine<RED_SUM>(part, nblocks), strided_combine<RED_SUM>(part + RED_BLOCKS, nblocks)};
     block_reduce<RED_SUM, 2>(v);
     if (threadIdx.x == 0) {
         S[S_ZSUM] = v[0];
@@ -1214,7 +1231,7 @@ static Grid2 vec_grid(int n1, int n2) {
     g.block = dim3(64, 4);
     const int bx = (n2 + 63) / 64;
     int by = (n1 + 3) / 4;
-    if (by * bx > RED_BLOCKS) by = RED_BLOCKS / bx > 0 ? RED_BLOCKS / bx : 1;
+    if (by * bx > VEC_BLOCKS) by = VEC_BLOCKS / bx > 0 ? VEC_BLOCKS / bx : 1;
     g.grid = dim3(bx, by);
     g.nblocks = bx * by;
     return g;
@@ -1239,7 +1256,7 @@ __global__ void k_cg_dir_matvec(LevArgs L, CgArgs c, const double *__restrict__ 
                                                                   : (x0[o + (off_)] - mean) + beta * dold[o + (off_)]))
         const double dc = DN(0, i, j);
         double a[9];
-        op_row<false>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+        op_row5(L, i, j, a);
         double y = a[ST_C] * dc;
         if (i > 0) y += a[ST_W] * DN(-ld, i - 1, j);
         if (j > 0) y += a[ST_S] * DN(-1, i, j - 1);
@@ -1279,12 +1296,15 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
     }
 }
 
-// rows per chunk of the streaming legs: about 2048 wavefronts on large
+#ifndef IMPEC_MG_WAVES
+#define IMPEC_MG_WAVES 2048
+#endif
+// rows per chunk of the streaming legs: about IMPEC_MG_WAVES wavefronts on large
 // levels, short chunks on small ones; even, since the march starts on an
 // even row
 static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip) {
     if (l == 0 && ctx->P.mg_rows > 0) return (ctx->P.mg_rows + 1) & ~1;
-    long r = ((long)n1 * nstrip + 2047) / 2048;
+    long r = ((long)n1 * nstrip + IMPEC_MG_WAVES - 1) / IMPEC_MG_WAVES;
     r = (r + 1) & ~1L;
     if (r < 8) r = 8;
     if (r > 64) r = 64;
@@ -1448,6 +1468,13 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     if (st) return st;
     const Level &L0 = ctx->lev[0];
     LevArgs a0 = lev_args(ctx, 0);
+    if (!DIAG && !IMPEC_NO_FACES) {
+        dim3 blk(64, 4);
+        hipLaunchKernelGGL(k_faces, grid2(ctx->N1 + 1, ctx->N2 + 1, blk), blk, 0, ctx->stream, a0, ctx->fx, ctx->fy);
+        IMPEC_HIP_CHECK(ctx, hipGetLastError());
+        a0.fx = ctx->fx;
+        a0.fy = ctx->fy;
+    }
     CgArgs c;
     c.n1 = ctx->N1; c.n2 = ctx->N2; c.ld = L0.ld; c.ldp = ctx->lds; c.B = ctx->B;
     c.n = (long)ctx->N1 * ctx->N2;
