@@ -70,5 +70,34 @@ def build(verbose=False, jobs=8, out=LIB, defines=()):
     return out
 
 
+HOST_LIB = os.path.join(HERE, "libimpec_host.so")
+HOST_RUN = os.path.join(HERE, "impec2d_run")
+GXX_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+
+
+def build_host(verbose=False):
+    """The C++ solver classes (host/impec_solvers.cpp) as libimpec_host.so and
+    the impec2d_run driver, both linked against libimpec_hip.so (g++ only: no
+    device code on this side)."""
+    gxx = os.environ.get("CXX", "g++")
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host")]
+    link = ["-L", HERE, "-limpec_hip", "-Wl,-rpath,$ORIGIN"]
+    steps = [
+        [gxx] + GXX_FLAGS + ["-shared"] + inc + [os.path.join(HERE, "host", "impec_solvers.cpp"), "-o",
+                                                 HOST_LIB + ".tmp"] + link,
+        [gxx] + GXX_FLAGS + inc + [os.path.join(HERE, "host", "impec2d_run.cpp"), "-o", HOST_RUN + ".tmp",
+                                   "-L", HERE, "-limpec_host"] + link,
+    ]
+    for i, cmd in enumerate(steps):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        if i == 0:
+            os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    os.replace(HOST_RUN + ".tmp", HOST_RUN)
+    return HOST_LIB, HOST_RUN
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_host(verbose="-v" in sys.argv))

@@ -78,6 +78,7 @@ SIGNATURES = {
     "impec_advect_split": ([_vp, C.c_double], C.c_int),
     "impec_advect_x": ([_vp, C.c_double], C.c_int),
     "impec_advect_y": ([_vp, C.c_double], C.c_int),
+    "impec_intercell_flux": ([_vp, C.c_int, C.c_double, _dp], C.c_int),
     "impec_dispersion_cn": ([_vp, C.c_double, _ip], C.c_int),
     "impec_hyperbolic_solve": ([_vp, C.c_double], C.c_int),
     "impec_step": ([_vp, C.c_double, C.c_double, _dp], C.c_int),
@@ -244,6 +245,13 @@ class Context:
 
     def advect_y(self, dt):
         self._check(self.lib.impec_advect_y(self.h, dt))
+
+    def intercell_flux(self, direction, dt):
+        """F (direction 0, shape (N1+1, N2+4)) or G (direction 1, shape (N1+4, N2+1))."""
+        shape = (self.N1 + 1, self.N2 + 4) if direction == 0 else (self.N1 + 4, self.N2 + 1)
+        out = np.zeros(shape)
+        self._check(self.lib.impec_intercell_flux(self.h, int(direction), dt, out.ctypes.data_as(_dp)))
+        return out
 
     def dispersion_cn(self, dt):
         its = C.c_int(0)

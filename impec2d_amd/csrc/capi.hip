@@ -410,6 +410,22 @@ int impec_advect_y(impec_ctx *ctx, double dt) {
     return st;
 }
 
+int impec_intercell_flux(impec_ctx *ctx, int dir, double dt, double *F) {
+    if (!ctx || !F || (dir != 0 && dir != 1)) return IMPEC_EINVAL;
+    const size_t n = dir == 0 ? (size_t)(ctx->N1 + 1) * (ctx->N2 + 2 * ctx->B)
+                              : (size_t)(ctx->N1 + 2 * ctx->B) * (ctx->N2 + 1);
+    double *d = nullptr;
+    IMPEC_HIP_CHECK(ctx, hipMalloc(&d, n * sizeof(double)));
+    int st = intercell_flux(ctx, dir, dt, d);
+    hipError_t e = hipSuccess;
+    if (!st) e = hipMemcpyAsync(F, d, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (!st && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(d);
+    if (st) return st;
+    if (e != hipSuccess) return set_error(ctx, IMPEC_EHIP, "intercell flux: %s", hipGetErrorString(e));
+    return IMPEC_OK;
+}
+
 int impec_dispersion_cn(impec_ctx *ctx, double dt, int *its) {
     if (!ctx) return IMPEC_EINVAL;
     int st = ensure_cn(ctx);

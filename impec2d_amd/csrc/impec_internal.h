@@ -155,6 +155,7 @@ int pressure_operator_dump(impec_ctx *ctx, double *A9_host);
 int advect_x(impec_ctx *ctx, const double *cin, double *cout, double dt);
 int advect_y(impec_ctx *ctx, const double *cin, double *cout, double dt);
 int advect_split(impec_ctx *ctx, double dt);
+int intercell_flux(impec_ctx *ctx, int dir, double dt, double *F);
 int copy_scalar(impec_ctx *ctx, const double *src, double *dst);
 
 // ---- dispersion (k_cn.hip) ----

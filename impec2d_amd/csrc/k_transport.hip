@@ -124,6 +124,54 @@ __global__ void k_sweep_y(SweepConst k, const double *__restrict__ cin, double *
     cout[o] = cin[o] + k.dtdx * (Gj - Gjp1);
 }
 
+// The intercell fluxes alone, with the sweeps' arithmetic
+// (CalculateIntercellFlux_F, MUSCLSplitSolver.cpp:102-145 / UpwindSplitSolver.cpp:43-51):
+// F(i,j) for faces i in [0, N1], j in [-B, N2+B) into F[i * (N2+2B) + j + B].
+template <bool MUSCL, bool PHI1>
+__global__ void k_flux_x(SweepConst k, const double *__restrict__ c, const double *__restrict__ u,
+                         double *__restrict__ F) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x - k.B;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > k.N1 || j >= k.N2 + k.B) return;
+    const long o = (long)(i + k.B) * k.lds + (j + k.B);
+    const long ov = (long)(i + k.B) * k.ldv + (j + k.B);
+    const long L = k.lds, LV = k.ldv;
+    double Fi;
+    if (MUSCL) {
+        double lA, rA, lB, rB;
+        hancock<PHI1>(c[o - 2 * L], c[o - L], c[o], u[ov - LV], u[ov], k, lA, rA);   // cell i-1
+        hancock<PHI1>(c[o - L], c[o], c[o + L], u[ov], u[ov + LV], k, lB, rB);       // cell i
+        (void)rA; (void)lB;
+        Fi = upwind<PHI1>(u[ov], lA, rB, k.phi);
+    } else {
+        Fi = upwind<PHI1>(u[ov], c[o - L], c[o], k.phi);
+    }
+    F[(long)i * (k.N2 + 2 * k.B) + (j + k.B)] = Fi;
+}
+
+// G(i,j) for i in [-B, N1+B), faces j in [0, N2] into G[(i+B) * (N2+1) + j]
+// (CalculateIntercellFlux_G, MUSCLSplitSolver.cpp:147-189 / UpwindSplitSolver.cpp:53-61)
+template <bool MUSCL, bool PHI1>
+__global__ void k_flux_y(SweepConst k, const double *__restrict__ c, const double *__restrict__ v,
+                         double *__restrict__ G) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y - k.B;
+    if (i >= k.N1 + k.B || j > k.N2) return;
+    const long o = (long)(i + k.B) * k.lds + (j + k.B);
+    const long ov = (long)(i + k.B) * k.ldv + (j + k.B);
+    double Gj;
+    if (MUSCL) {
+        double lA, rA, lB, rB;
+        hancock<PHI1>(c[o - 2], c[o - 1], c[o], v[ov - 1], v[ov], k, lA, rA);
+        hancock<PHI1>(c[o - 1], c[o], c[o + 1], v[ov], v[ov + 1], k, lB, rB);
+        (void)rA; (void)lB;
+        Gj = upwind<PHI1>(v[ov], lA, rB, k.phi);
+    } else {
+        Gj = upwind<PHI1>(v[ov], c[o - 1], c[o], k.phi);
+    }
+    G[(long)(i + k.B) * (k.N2 + 1) + j] = Gj;
+}
+
 // c = 0.5 * (a + b) over the whole ghosted array (HyperbolicSplitSolver.cpp:97-101)
 __global__ void k_average(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c, int rows,
                           int cols, int ld) {
@@ -184,6 +232,29 @@ int advect_y(impec_ctx *ctx, const double *cin, double *cout, double dt) {
     else if (phi1) launch_y<false, true>(ctx, k, cin, cout);
     else launch_y<false, false>(ctx, k, cin, cout);
     prof_end(ctx, "sweep_y");
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+// fluxes of the current c and face velocities into F (device, layouts of
+// k_flux_x / k_flux_y)
+int intercell_flux(impec_ctx *ctx, int dir, double dt, double *F) {
+    SweepConst k = sweep_const(ctx, dt, dir == 0 ? ctx->dx : ctx->dy);
+    const bool muscl = ctx->P.hyper_method == IMPEC_HYPER_MUSCL, phi1 = ctx->P.porosity == 1.;
+    dim3 blk(64, 4);
+    if (dir == 0) {
+        dim3 grd((k.N2 + 2 * k.B + 63) / 64, (k.N1 + 1 + 3) / 4);
+        if (muscl && phi1) hipLaunchKernelGGL((k_flux_x<true, true>), grd, blk, 0, ctx->stream, k, ctx->c, ctx->u, F);
+        else if (muscl) hipLaunchKernelGGL((k_flux_x<true, false>), grd, blk, 0, ctx->stream, k, ctx->c, ctx->u, F);
+        else if (phi1) hipLaunchKernelGGL((k_flux_x<false, true>), grd, blk, 0, ctx->stream, k, ctx->c, ctx->u, F);
+        else hipLaunchKernelGGL((k_flux_x<false, false>), grd, blk, 0, ctx->stream, k, ctx->c, ctx->u, F);
+    } else {
+        dim3 grd((k.N2 + 1 + 63) / 64, (k.N1 + 2 * k.B + 3) / 4);
+        if (muscl && phi1) hipLaunchKernelGGL((k_flux_y<true, true>), grd, blk, 0, ctx->stream, k, ctx->c, ctx->v, F);
+        else if (muscl) hipLaunchKernelGGL((k_flux_y<true, false>), grd, blk, 0, ctx->stream, k, ctx->c, ctx->v, F);
+        else if (phi1) hipLaunchKernelGGL((k_flux_y<false, true>), grd, blk, 0, ctx->stream, k, ctx->c, ctx->v, F);
+        else hipLaunchKernelGGL((k_flux_y<false, false>), grd, blk, 0, ctx->stream, k, ctx->c, ctx->v, F);
+    }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
 }

@@ -1,0 +1,357 @@
+// impec_solvers.hpp -- the reference's solver class interfaces over the C ABI
+// of impec_hip.h, so that an Impec2DSolver-style driver runs unchanged on one
+// MI355X.  Class names, virtual methods, argument meaning and bool/int return
+// conventions follow
+//   EllipticSolver.h:15-110, MGSolver.h:11-80, HyperbolicSolver.h:14-43,
+//   HyperbolicSplitSolver.h:10-25, MUSCLSplitSolver.h:10-23,
+//   UpwindSplitSolver.h:8-17, TimeIntegrator.h:6-16, TRIntegrator.h:8-13,
+//   Impec2DSolver.h (DiagState, Solve loop).
+//
+// Device state is authoritative.  Host arrays (Array2d, Array3d) keep the
+// reference's Blitz layout (row-major, last index contiguous, base -b) and are
+// bound to one device field each; a solver call uploads an argument only when
+// its host copy was written since the last transfer, and host reads download
+// lazily.  The per-step loop therefore moves no data over PCIe unless the
+// caller reads or writes host arrays.
+//
+// Errors: methods return false / a negative status and print the library's
+// message to std::cerr, as the reference's solvers do; nothing calls exit().
+#pragma once
+
+#include "impec_hip.h"
+
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace impec_host {
+
+enum SourceType { PARALLEL = IMPEC_SRC_PARALLEL, DIAGONAL = IMPEC_SRC_DIAGONAL };   // SimParams.h stype
+enum AvgProcedure { HARMONIC = IMPEC_AVG_HARMONIC, MEANCONC = IMPEC_AVG_MEANCONC };
+enum DimSplitting {   // SimParams.h:21, same order
+    ALTERNATING_SWEEPS = IMPEC_SPLIT_ALTERNATING,
+    X_SWEEP_FIRST = IMPEC_SPLIT_X_FIRST,
+    Y_SWEEP_FIRST = IMPEC_SPLIT_Y_FIRST,
+    STRANG = IMPEC_SPLIT_STRANG,
+    SYMMETRIC_PARALLEL = IMPEC_SPLIT_PARALLEL
+};
+enum HyperbolicMethod { MUSCL = IMPEC_HYPER_MUSCL, UPWIND = IMPEC_HYPER_UPWIND };
+enum ParabolicMethod { EXPLICIT = IMPEC_PARA_EXPLICIT, CRANKNICOLSON = IMPEC_PARA_CN };
+
+// The SimParams fields the hot path reads (SimParams.h:29-164), with the
+// reference's names.
+struct SimParams {
+    int N1 = 64, N2 = 64, c_bounds = 2;
+    double x_lb = 0., x_ub = 1., y_lb = 0., y_ub = 1.;
+    double dx = 0., dy = 0.;                     // set by Finalize()
+    double tstart = 0., tfinal = 1.;
+    std::vector<double> times;                   // output times (the loop clips dt to them)
+    int max_steps = 0;                           // 0: run to tfinal
+    double cfl = 0.5, nu = 1., mu_res = 1., mu_inj = 0.1, porosity = 1., I = 1., inj_fluid_conc = 1.;
+    double alpha_l = 0., alpha_t = 0., dt_diff_limit = 1., d_m = 0.;
+    double eps = 1e-10, para_eps = 1e-10;
+    SourceType stype = DIAGONAL;
+    AvgProcedure avgproc = HARMONIC;
+    DimSplitting dim_splitting = SYMMETRIC_PARALLEL;
+    HyperbolicMethod hmethod = MUSCL;
+    ParabolicMethod ptype = CRANKNICOLSON;
+    bool diffusion = false, is_test = false, alpha_absolute = false;
+    bool check_mass_conservation = false, check_noflow_boundary = false, check_divergence = false;
+    bool verbose = true;                         // console line per step (Impec2DSolver::OutputState)
+    int precision = IMPEC_PREC_FP64, initial_guess = IMPEC_GUESS_ZERO, max_iterations = 10000;
+
+    void Finalize();                             // dx, dy from the domain and N1, N2
+    impec_params ToImpec() const;
+};
+
+class Device;
+
+// Blitz-layout scalar field: extent (N1+2b) x (N2+2b), element (i,j) for
+// i in [-b, N1+b), j in [-b, N2+b).
+class Array2d {
+  public:
+    Array2d() = default;
+    Array2d(int n1, int n2, int b) { resize(n1, n2, b); }
+    Array2d(const Array2d &) = delete;
+    Array2d &operator=(const Array2d &) = delete;
+    ~Array2d();
+    void resize(int n1, int n2, int b);
+    double &operator()(int i, int j) {
+        for_write();
+        return buf_[(size_t)(i + b_) * ld_ + (j + b_)];
+    }
+    double operator()(int i, int j) const {
+        for_read();
+        return buf_[(size_t)(i + b_) * ld_ + (j + b_)];
+    }
+    double *data() {            // element (-b,-b); marks the host copy as written
+        for_write();
+        return buf_.data();
+    }
+    const double *data() const {
+        for_read();
+        return buf_.data();
+    }
+    int n1() const { return n1_; }
+    int n2() const { return n2_; }
+    int bounds() const { return b_; }
+
+  private:
+    friend class Device;
+    void for_read() const;
+    void for_write();
+    std::vector<double> buf_;
+    int n1_ = 0, n2_ = 0, b_ = 0, ld_ = 0;
+    Device *dev_ = nullptr;
+    int field_ = -1;
+    mutable bool host_stale_ = false;   // the device copy is newer
+    bool host_dirty_ = true;            // the host copy was written since the last upload
+};
+
+// Blitz-layout face velocities: extent (N1+1+2b) x (N2+1+2b) x 2, component
+// last (Array3d v of Impec2DSolver.cpp:151-182).
+class Array3d {
+  public:
+    Array3d() = default;
+    Array3d(const Array3d &) = delete;
+    Array3d &operator=(const Array3d &) = delete;
+    ~Array3d();
+    void resize(int n1, int n2, int b);
+    double &operator()(int i, int j, int k) {
+        for_write();
+        return buf_[((size_t)(i + b_) * ld_ + (j + b_)) * 2 + k];
+    }
+    double operator()(int i, int j, int k) const {
+        for_read();
+        return buf_[((size_t)(i + b_) * ld_ + (j + b_)) * 2 + k];
+    }
+    double *data() {
+        for_write();
+        return buf_.data();
+    }
+    const double *data() const {
+        for_read();
+        return buf_.data();
+    }
+
+  private:
+    friend class Device;
+    void for_read() const;
+    void for_write();
+    std::vector<double> buf_;
+    int n1_ = 0, n2_ = 0, b_ = 0, ld_ = 0;
+    Device *dev_ = nullptr;
+    mutable bool host_stale_ = false;
+    bool host_dirty_ = true;
+};
+
+// One impec_ctx (one HIP stream, all fields resident in HBM) shared by the
+// solver objects of one simulation.
+class Device {
+  public:
+    enum Field { C = 0, P = 1 };
+    explicit Device(const SimParams &sp);   // throws std::runtime_error when creation fails
+    ~Device();
+    Device(const Device &) = delete;
+    Device &operator=(const Device &) = delete;
+    impec_ctx *ctx() const { return ctx_; }
+    const impec_params &params() const { return params_; }
+    // true on IMPEC_OK; otherwise prints "<where>: <message>" to std::cerr
+    bool ok(int status, const char *where) const;
+    // argument protocol of the solver shims
+    bool use(Array2d &a, Field f);           // bind; upload if the host copy is newer
+    bool use(const Array3d &v);
+    bool use_source(double const *q);        // upload when the pointer changes
+    void written(Array2d &a) { a.host_stale_ = true; }
+    void written(const Array3d &v) { const_cast<Array3d &>(v).host_stale_ = true; }
+    void produced(Array3d &v);               // bind v as the device's output (no upload)
+    bool download(const Array2d &a) const;
+    bool download(const Array3d &v) const;
+    void unbind(Array2d *a);
+    void unbind(Array3d *v);
+    impec_stats stats() const;
+
+  private:
+    impec_ctx *ctx_ = nullptr;
+    impec_params params_;
+    Array2d *bound_[2] = {nullptr, nullptr};
+    Array3d *bound_v_ = nullptr;
+    double const *q_ = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// EllipticSolver / MGSolver
+// ---------------------------------------------------------------------------
+class EllipticSolver {
+  public:
+    EllipticSolver(SimParams *sp, Device *dev);
+    virtual ~EllipticSolver();
+    virtual bool SolveLinearMultigrid(Array2d &c_c, int c_bounds, Array2d &p_c, int p_bounds,
+                                      double const *const q_v) = 0;
+    bool Solve(Array2d &p, Array2d &c, double *q);            // EllipticSolver.cpp:96-150
+    void CalcVelocities(Array2d &p, Array3d &v, Array2d &c);  // EllipticSolver.cpp:660-683
+    void SetNu(double nu);                                    // EllipticSolver.cpp:526-534
+    int GetLastIts();
+    int GetLastParabolicIts();
+    double GetAvgIts();
+
+  protected:
+    SimParams *sp;
+    Device *dev;
+    int last_its = 0;
+    int last_parabolic_its = 0;
+    int total_its = 0;
+    int calls = 0;
+};
+
+class MGSolver : public EllipticSolver {
+  public:
+    MGSolver(SimParams *sp, Device *dev);
+    virtual ~MGSolver();
+    // pressure: multigrid-preconditioned CG to sp->eps (MGSolver.cpp:268-667)
+    virtual bool SolveLinearMultigrid(Array2d &c_c, int c_bounds, Array2d &p_c, int p_bounds,
+                                      double const *const q_v);
+    // Crank-Nicolson dispersion over dt into c (MGSolver.cpp:952-976); ls must be null
+    bool Solve_Parabolic_cc_to_c(Array2d &c_c, int c_bounds, const Array3d &v, int v_bounds,
+                                 Array2d const *const ls, int ls_bounds, const double dt);
+    double GetLastRelres() const { return last_relres; }
+
+  private:
+    double last_relres = 0.;
+};
+
+// ---------------------------------------------------------------------------
+// TimeIntegrator / TRIntegrator
+// ---------------------------------------------------------------------------
+class TimeIntegrator {
+  public:
+    TimeIntegrator(SimParams *sp, Device *dev);
+    virtual ~TimeIntegrator();
+    double GetTotalMassOutflux();
+    virtual int Integrate(Array2d &c, double const *const q, const double dt) = 0;
+
+  protected:
+    SimParams *sp;
+    Device *dev;
+};
+
+class TRIntegrator : public TimeIntegrator {
+  public:
+    TRIntegrator(SimParams *sp, Device *dev);
+    virtual ~TRIntegrator();
+    // trapezoidal source step over dt (TRIntegrator.cpp:49-71); 0 on success
+    virtual int Integrate(Array2d &c, double const *const q, const double dt);
+};
+
+// ---------------------------------------------------------------------------
+// HyperbolicSolver / HyperbolicSplitSolver / MUSCLSplitSolver / UpwindSplitSolver
+// ---------------------------------------------------------------------------
+class HyperbolicSolver {
+  public:
+    HyperbolicSolver(SimParams *sp, Device *dev);
+    virtual ~HyperbolicSolver();
+    virtual bool Solve(Array2d &c, const Array3d &v, double const *const q, const double dt) = 0;
+    double GetTotalMassOutflux();
+    void FillBoundaries(Array2d &c);                                   // HyperbolicSolver.cpp:56-61
+    double MaxStableTimestepCombined(const Array2d &c, const Array3d &v);   // :122-198
+    double MaxStableAdvectionTimestep(const Array2d &c, const Array3d &v);  // :100-121
+    double MaxStableTimestep(const Array2d &c, const Array3d &v);           // :93-98
+    bool DiffusionSplitCN(Array2d &c, const Array3d &v, const double dt);   // :217-223
+    void AssignMGSolver(MGSolver *mg);
+
+  protected:
+    bool SourceSplit(Array2d &c, double const *const q, const double dt);  // :201-205
+    SimParams *sp;
+    Device *dev;
+
+  private:
+    TimeIntegrator *ti;
+    MGSolver *mg = nullptr;
+};
+
+class HyperbolicSplitSolver : public HyperbolicSolver {
+  public:
+    HyperbolicSplitSolver(SimParams *sp, Device *dev);
+    virtual ~HyperbolicSplitSolver();
+    // F(i,j): faces i in [0,N1], j in [-b,N2+b), F (N1+1) x (N2+2b) row-major;
+    // G(i,j): i in [-b,N1+b), faces j in [0,N2], G (N1+2b) x (N2+1)
+    virtual void CalculateIntercellFlux_F(std::vector<double> &F, const Array2d &c, const Array3d &v,
+                                          const double dt) = 0;
+    virtual void CalculateIntercellFlux_G(std::vector<double> &G, const Array2d &c, const Array3d &v,
+                                          const double dt) = 0;
+    bool Advect_X(Array2d &c, const Array3d &v, const double dt);   // HyperbolicSplitSolver.cpp:113-134
+    bool Advect_Y(Array2d &c, const Array3d &v, const double dt);   // :140-159
+    // Source(dt/2), fill, [CN(dt/2)], AdvectSplit(dt), Source(dt/2), fill, [CN(dt/2)] (:162-198)
+    virtual bool Solve(Array2d &c, const Array3d &v, double const *const q, const double dt);
+
+  protected:
+    bool AdvectSplit(Array2d &c, const Array3d &v, double const *const q, const double dt);   // :51-106
+    bool Flux(int dir, std::vector<double> &F, const Array2d &c, const Array3d &v, double dt);
+};
+
+class MUSCLSplitSolver : public HyperbolicSplitSolver {
+  public:
+    MUSCLSplitSolver(SimParams *sp, Device *dev);   // the device must run the MUSCL scheme
+    virtual ~MUSCLSplitSolver();
+    virtual void CalculateIntercellFlux_F(std::vector<double> &F, const Array2d &c, const Array3d &v,
+                                          const double dt);
+    virtual void CalculateIntercellFlux_G(std::vector<double> &G, const Array2d &c, const Array3d &v,
+                                          const double dt);
+};
+
+class UpwindSplitSolver : public HyperbolicSplitSolver {
+  public:
+    UpwindSplitSolver(SimParams *sp, Device *dev);  // the device must run the upwind scheme
+    virtual ~UpwindSplitSolver();
+    virtual void CalculateIntercellFlux_F(std::vector<double> &F, const Array2d &c, const Array3d &v,
+                                          const double dt);
+    virtual void CalculateIntercellFlux_G(std::vector<double> &G, const Array2d &c, const Array3d &v,
+                                          const double dt);
+};
+
+// ---------------------------------------------------------------------------
+// Impec2DSolver: the step loop of Impec2DSolver.cpp:293-466
+// ---------------------------------------------------------------------------
+struct DiagState {   // Impec2DSolver.h
+    int ctr;
+    int its;
+    double t;
+    double dt;
+    double mass_err;
+    double diff_err;
+    double vel_err;
+    double fillperdt;
+};
+
+class Impec2DSolver {
+  public:
+    explicit Impec2DSolver(const SimParams &params);   // throws std::runtime_error without a device
+    virtual ~Impec2DSolver();
+    bool Run();                                        // Solve() from the initial state
+    bool Solve();
+    const std::vector<DiagState> &Diagnostics() const { return diagstates; }
+    void WriteDiagnostics(const std::string &fname) const;   // diagnostics.dat columns
+    Array2d &Concentration() { return c; }
+    Array2d &Pressure() { return p; }
+    Array3d &Velocity() { return v; }
+    std::vector<double> &Source() { return q; }
+    SimParams sp;
+
+  private:
+    void Initialize();
+    void InitializeSource();          // one-cell wells (Impec2DSolver.cpp:241-280, Util.cpp:565-586)
+    void InitializeConcentration();   // c = 0 (Impec2DSolver.cpp:191-233 without an init file)
+    double CalculateMassError(double t);
+    double CalculateVelocityBoundaryError();
+    double CalculateDivergenceError();
+    void OutputState(const DiagState &ds);
+    std::unique_ptr<Device> dev;
+    Array2d p, c;
+    Array3d v;
+    std::vector<double> q;
+    EllipticSolver *es = nullptr;
+    HyperbolicSolver *hs = nullptr;
+    std::vector<DiagState> diagstates;
+};
+
+}  // namespace impec_host

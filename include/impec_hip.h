@@ -155,6 +155,12 @@ int impec_advect_split(impec_ctx *ctx, double dt);
  * with MUSCLSplitSolver or UpwindSplitSolver fluxes. */
 int impec_advect_x(impec_ctx *ctx, double dt);
 int impec_advect_y(impec_ctx *ctx, double dt);
+/* HyperbolicSplitSolver::CalculateIntercellFlux_F / _G of the selected scheme
+ * (MUSCLSplitSolver.cpp:102-189, UpwindSplitSolver.cpp:43-61) for the current c
+ * (ghosts as filled) and face velocities.  dir 0: F(i,j), i in [0,N1],
+ * j in [-2,N2+2), host array (N1+1) x (N2+4), F[i*(N2+4) + j+2]; dir 1: G(i,j),
+ * i in [-2,N1+2), j in [0,N2], host array (N1+4) x (N2+1), G[(i+2)*(N2+1) + j]. */
+int impec_intercell_flux(impec_ctx *ctx, int dir, double dt, double *F);
 /* HyperbolicSolver::DiffusionSplitCN -> MGSolver::Solve_Parabolic_cc_to_c
  * (HyperbolicSolver.cpp:217-223, MGSolver.cpp:952-976); dt is the split step
  * (the caller passes dt/2 as HyperbolicSplitSolver::Solve does). */

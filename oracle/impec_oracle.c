@@ -1180,6 +1180,10 @@ static void flux_G(const or_ctx *ctx, double *G, const double *c, const double *
 #undef G_
 }
 
+/* Exported fluxes (CalculateIntercellFlux_F / _G of the selected scheme). */
+void or_flux_F(const or_ctx *ctx, double *F, const double *c, const double *v, double dt) { flux_F(ctx, F, c, v, dt); }
+void or_flux_G(const or_ctx *ctx, double *G, const double *c, const double *v, double dt) { flux_G(ctx, G, c, v, dt); }
+
 /* Advect_X (HyperbolicSplitSolver.cpp:113-134). */
 void or_advect_x(const or_ctx *ctx, double *c, const double *v, double dt)
 {

@@ -63,6 +63,8 @@ def lib():
         L.or_tr_integrate.argtypes = [vp, dp, dp, C.c_double]
         L.or_tr_integrate.restype = C.c_double
         L.or_source_split.argtypes = [vp, dp, dp, C.c_double]
+        L.or_flux_F.argtypes = [vp, dp, dp, dp, C.c_double]
+        L.or_flux_G.argtypes = [vp, dp, dp, dp, C.c_double]
         L.or_advect_x.argtypes = [vp, dp, dp, C.c_double]
         L.or_advect_y.argtypes = [vp, dp, dp, C.c_double]
         L.or_advect_split.argtypes = [vp, dp, dp, C.c_double]
@@ -182,6 +184,14 @@ class Oracle:
 
     def tr(self, c, q, dt):
         return self.L.or_tr_integrate(self.ctx, c, q, dt)
+
+    def flux(self, direction, c, v, dt):
+        """F (direction 0, (N1+1) x (N2+2b)) or G (direction 1, (N1+2b) x (N2+1))."""
+        b = self.b
+        shape = (self.N1 + 1, self.N2 + 2 * b) if direction == 0 else (self.N1 + 2 * b, self.N2 + 1)
+        out = np.zeros(shape)
+        (self.L.or_flux_F if direction == 0 else self.L.or_flux_G)(self.ctx, out, c, v, dt)
+        return out
 
     def advect_x(self, c, v, dt):
         self.L.or_advect_x(self.ctx, c, v, dt)

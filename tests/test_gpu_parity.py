@@ -131,6 +131,24 @@ def test_advect_split_bitexact(gpu_lib, kw):
     assert np.array_equal(o.interior(cg), o.interior(c))
 
 
+@pytest.mark.parametrize("kw", [dict(), dict(hyper=1), dict(N1=13, N2=21, porosity=0.7),
+                                dict(N1=9, N2=5, hyper=1, porosity=0.5)])
+def test_intercell_flux_bitexact(gpu_lib, kw):
+    """CalculateIntercellFlux_F/G (MUSCL and upwind) on the device equal the
+    oracle's fluxes bit for bit, ghost rows and columns included."""
+    p = params(**dict(dict(N1=24, N2=24), **kw))
+    _say("intercell flux", kw)
+    o, g = pair(p)
+    rng = np.random.default_rng(21)
+    c = rng.random(o.scalar().shape)
+    v = rng.standard_normal(o.velocity().shape)
+    g.set_concentration(c)
+    g.set_velocity(v)
+    dt = 0.3 / max(o.N1, o.N2) / np.abs(v).max()
+    for d in (0, 1):
+        assert np.array_equal(g.intercell_flux(d, dt), o.flux(d, c, v, dt)), d
+
+
 @pytest.mark.parametrize("kw", [dict(), dict(alpha_abs=1, alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5),
                                 dict(N1=18, N2=10, porosity=0.8)])
 def test_parabolic_operator_bitexact(gpu_lib, kw):

@@ -1,0 +1,105 @@
+"""The C++ solver classes (impec2d_amd/host) and the impec2d_run driver: the
+reference's class interfaces (EllipticSolver.h, MGSolver.h, HyperbolicSolver.h,
+HyperbolicSplitSolver.h, TimeIntegrator.h, Impec2DSolver.h) over the C ABI."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import pair, params, rel_err
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "impec2d_amd", "impec2d_run")
+HPP = os.path.join(ROOT, "impec2d_amd", "host", "impec_solvers.hpp")
+
+
+def _have_driver():
+    if not os.path.exists(RUN):
+        pytest.skip("impec2d_run is not built (run __graft_entry__.build())")
+
+
+def test_class_interfaces_declared():
+    """Every reference class and virtual entry point the drop-in keeps."""
+    text = open(HPP).read()
+    for decl in ["class EllipticSolver", "class MGSolver : public EllipticSolver", "class HyperbolicSolver",
+                 "class HyperbolicSplitSolver : public HyperbolicSolver",
+                 "class MUSCLSplitSolver : public HyperbolicSplitSolver",
+                 "class UpwindSplitSolver : public HyperbolicSplitSolver", "class TimeIntegrator",
+                 "class TRIntegrator : public TimeIntegrator", "class Impec2DSolver", "struct DiagState",
+                 "virtual bool SolveLinearMultigrid(Array2d &c_c, int c_bounds, Array2d &p_c, int p_bounds",
+                 "bool Solve(Array2d &p, Array2d &c, double *q)",
+                 "void CalcVelocities(Array2d &p, Array3d &v, Array2d &c)",
+                 "bool Solve_Parabolic_cc_to_c(Array2d &c_c, int c_bounds, const Array3d &v, int v_bounds",
+                 "virtual bool Solve(Array2d &c, const Array3d &v, double const *const q, const double dt) = 0",
+                 "double MaxStableTimestep(const Array2d &c, const Array3d &v)",
+                 "bool DiffusionSplitCN(Array2d &c, const Array3d &v, const double dt)",
+                 "void AssignMGSolver(MGSolver *mg)", "bool Advect_X(Array2d &c, const Array3d &v, const double dt)",
+                 "virtual int Integrate(Array2d &c, double const *const q, const double dt) = 0",
+                 "double GetTotalMassOutflux()", "int GetLastIts()", "int GetLastParabolicIts()",
+                 "double GetAvgIts()"]:
+        assert decl in text, decl
+
+
+def test_driver_usage_and_no_device_status():
+    """Bad arguments exit 2; without a usable GPU the driver exits 3 with the
+    library's message (never a silent CPU run)."""
+    _have_driver()
+    r = subprocess.run([RUN, "--bogus"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    if os.environ.get("HIP_VISIBLE_DEVICES", None) == "" or not os.path.exists("/dev/kfd"):
+        r = subprocess.run([RUN, "--N", "8", "--steps", "1", "--quiet"], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 3, (r.returncode, r.stderr)
+        assert "impec_create" in r.stderr
+
+
+def _dump(path, n1, n2):
+    raw = np.fromfile(path, dtype=np.float64)
+    ns, nv = (n1 + 4) * (n2 + 4), (n1 + 5) * (n2 + 5) * 2
+    assert raw.size == 2 * ns + nv
+    return (raw[:ns].reshape(n1 + 4, n2 + 4), raw[ns:2 * ns].reshape(n1 + 4, n2 + 4),
+            raw[2 * ns:].reshape(n1 + 5, n2 + 5, 2))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kw,args", [
+    (dict(), []),
+    (dict(diffusion=1, alpha_abs=1, alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5), ["--diffusion", "--alpha-absolute",
+                                                                          "--alpha-l", "1e-3", "--alpha-t", "1e-4",
+                                                                          "--d-m", "1e-5"]),
+    (dict(stype=0, hyper=1, nu=2.0 / 3.0), ["--stype", "parallel", "--upwind", "--nu", repr(2.0 / 3.0)]),
+])
+def test_cpp_solver_classes_match_capi_and_oracle(gpu_lib, tmp_path, kw, args):
+    """Ten steps through Impec2DSolver -> MGSolver / MUSCLSplitSolver /
+    TRIntegrator (C++) equal ten impec_step calls bit for bit and the oracle to
+    1e-10 relative."""
+    _have_driver()
+    n = 32
+    p = params(**dict(dict(N1=n, N2=n, eps=1e-13, para_eps=1e-13, mu_inj=0.1), **kw))
+    print("cpp driver", kw, flush=True)
+    out = tmp_path / "fields.bin"
+    cmd = [RUN, "--N", str(n), "--steps", "10", "--eps", "1e-13", "--para-eps", "1e-13", "--mu-inj", "0.1",
+           "--quiet", "--checks", "--dump", str(out), "--diagnostics", str(tmp_path / "diag.dat")] + args
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    c_cpp, p_cpp, v_cpp = _dump(out, n, n)
+    rows = np.loadtxt(tmp_path / "diag.dat")
+    assert rows.shape[0] == 10 and np.all(rows[:, 1] > 0)
+
+    o, g = pair(p)
+    q = o.source("one_cell")
+    c, pr, v = o.scalar(), o.scalar(), o.velocity()
+    g.set_source(q)
+    g.set_concentration(c)
+    t_ref = t_gpu = 0.0
+    for _ in range(10):
+        st, dt = o.step(c, pr, v, q, t_ref)
+        assert st == 0
+        t_ref += dt
+        t_gpu += g.step(t_gpu)
+    assert np.array_equal(o.interior(c_cpp), o.interior(g.get_concentration()))
+    assert np.array_equal(o.interior(p_cpp), o.interior(g.get_pressure()))
+    assert np.array_equal(v_cpp, g.get_velocity())
+    assert abs(rows[-1, 2] - t_gpu) <= 1e-14 * t_gpu
+    assert rel_err(o.interior(c_cpp), o.interior(c)) < 1e-10
+    assert rel_err(o.interior(p_cpp), o.interior(pr)) < 1e-10
