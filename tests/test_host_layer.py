@@ -100,6 +100,6 @@ def test_cpp_solver_classes_match_capi_and_oracle(gpu_lib, tmp_path, kw, args):
     assert np.array_equal(o.interior(c_cpp), o.interior(g.get_concentration()))
     assert np.array_equal(o.interior(p_cpp), o.interior(g.get_pressure()))
     assert np.array_equal(v_cpp, g.get_velocity())
-    assert abs(rows[-1, 2] - t_gpu) <= 1e-14 * t_gpu
+    assert abs(rows[-1, 2] - t_gpu) <= 1e-5 * t_gpu   # diagnostics.dat keeps the stream's 6 digits
     assert rel_err(o.interior(c_cpp), o.interior(c)) < 1e-10
     assert rel_err(o.interior(p_cpp), o.interior(pr)) < 1e-10
