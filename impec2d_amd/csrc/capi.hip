@@ -146,7 +146,7 @@ int impec_create(const impec_params *P, impec_ctx **out) {
     if (!(P->nu >= 0. && P->nu <= 1.)) return IMPEC_EINVAL;   // EllipticSolver::SetNu
     if (!(P->x_ub > P->x_lb && P->y_ub > P->y_lb)) return IMPEC_EINVAL;
     if (P->dim_splitting < 0 || P->dim_splitting > 4) return IMPEC_EINVAL;
-    if (P->precision != IMPEC_PREC_FP64) return IMPEC_EINVAL;   // mixed precision: not built yet
+    if (P->precision != IMPEC_PREC_FP64 && P->precision != IMPEC_PREC_MIXED) return IMPEC_EINVAL;
     if (P->mg_rows < 0 || P->mg_rows > 4096) return IMPEC_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IMPEC_ENODEV;

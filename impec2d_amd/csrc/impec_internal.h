@@ -52,6 +52,9 @@ struct Level {
     OpGeom g;
     double *w;           // inverse-mobility (harmonic) or mobility, ring 1 (level >= 1)
     double *x, *b, *r;   // ring-1 arrays (n1 + 2) x ld
+    // FP32 copies for the mixed-precision V-cycle (allocated on first use):
+    // w (level 0: of ctx->w; wf2 of ctx->w22 when K is anisotropic), x, b, r
+    float *wf, *wf2, *xf, *bf, *rf;
 };
 
 struct ProfEntry {

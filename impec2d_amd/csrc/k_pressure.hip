@@ -16,6 +16,7 @@
 #include "impec_device.h"
 #include <math.h>
 #include <stdlib.h>
+#include <type_traits>
 #include <vector>
 
 namespace impec {
@@ -320,8 +321,9 @@ __global__ void k_coarsen_w(const double *__restrict__ wf, int ldf, int n1, int 
 // coarsest level (at most 2 x 2 cells): exact solve of the singular Neumann
 // system bordered by the zero-mean condition, [A 1; 1^T 0][x; l] = [b; 0],
 // by Gaussian elimination with partial pivoting on one thread
-template <bool DIAG>
-__global__ void k_coarsest(LevArgs L, double *__restrict__ x, const double *__restrict__ b) {
+// (x, b stored as T; the elimination is FP64 in both precisions)
+template <bool DIAG, typename T = double>
+__global__ void k_coarsest(LevArgs L, T *__restrict__ x, const T *__restrict__ b) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int n1 = L.g.n1, n2 = L.g.n2, n = n1 * n2, m = n + 1;
     const int dI[9] = {0, -1, -1, 0, 1, 1, 1, 0, -1}, dJ[9] = {0, 0, -1, -1, -1, 0, 1, 1, 1};
@@ -340,7 +342,7 @@ __global__ void k_coarsest(LevArgs L, double *__restrict__ x, const double *__re
             }
             M[row][n] = 1.;
             M[n][row] = 1.;
-            M[row][m] = b[(long)i * L.ld + j];
+            M[row][m] = (double)b[(long)i * L.ld + j];
         }
     for (int col = 0; col < m; col++) {
         int piv = col;
@@ -366,7 +368,7 @@ __global__ void k_coarsest(LevArgs L, double *__restrict__ x, const double *__re
         sol[r] = M[r][r] != 0. ? t / M[r][r] : 0.;
     }
     for (int i = 0; i < n1; i++)
-        for (int j = 0; j < n2; j++) x[(long)i * L.ld + j] = sol[i * n2 + j];
+        for (int j = 0; j < n2; j++) x[(long)i * L.ld + j] = (T)sol[i * n2 + j];
 }
 
 // ---------------------------------------------------------------------------
@@ -676,6 +678,7 @@ void pressure_free_levels(impec_ctx *ctx) {
     for (int l = 0; l < ctx->nlev; l++) {
         Level &L = ctx->lev[l];
         hipFree(L.x); hipFree(L.b); hipFree(L.r);
+        hipFree(L.wf); hipFree(L.wf2); hipFree(L.xf); hipFree(L.bf); hipFree(L.rf);
         if (L.w) hipFree(L.w);
     }
     ctx->nlev = 0;
@@ -817,24 +820,28 @@ __global__ void k_set_pin(CgArgs c, double *__restrict__ p) {
 // ===========================================================================
 constexpr int SW_OUT = 120, SW_HALO = 4, SW_WAVES = 4;
 
-struct FusedArgs {
+// T is the storage and arithmetic type of the cycle: double (FP64) or float
+// (the FP32 V-cycles of the mixed-precision solve, inside the FP64 CG)
+template <typename T>
+struct FusedArgsT {
     OpGeom g;
-    const double *w, *w2;   // mobility quantity (x faces, y faces) at cell (0,0), pitch ldw
+    const T *w, *w2;        // mobility quantity (x faces, y faces) at cell (0,0), pitch ldw
     int ldw;
     int ld;                 // pitch of this level's x/b (ring-1 arrays at cell (0,0))
-    const double *b;        // level rhs (levels >= 1)
-    const double *r;        // level 0: CG residual, b = r - S[S_RSUM]/n
+    const T *b;             // level rhs (levels >= 1)
+    const double *r;        // level 0: CG residual (FP64), b = r - S[S_RSUM]/n
     const double *S;
     double inv_n;
-    double *x;              // this level's x (down: written; up: read)
-    double *xo;             // up: corrected, post-smoothed x (a different array:
+    T *x;                   // this level's x (down: written; up: read)
+    T *xo;                  // up: corrected, post-smoothed x (a different array:
                             // neighbouring strips and chunks read x)
-    const double *xc;       // up: coarse-level solution at (0,0)
-    double *bc;             // down: coarse-level b at (0,0)
+    const T *xc;            // up: coarse-level solution at (0,0)
+    T *bc;                  // down: coarse-level b at (0,0)
     int m1, m2, ldc;        // coarse level
-    double *part;           // up at level 0: per-wave partials (sum x, sum r x)
+    double *part;           // up at level 0: per-wave partials (sum x, sum r x), FP64
     int R, nstrip, nwave;   // rows per chunk, strips, wavefronts
 };
+using FusedArgs = FusedArgsT<double>;
 
 // 1/d from the hardware reciprocal estimate refined by one Newton-Raphson
 // step, r' = r + r (1 - d r).  Only the multigrid preconditioner uses it: its
@@ -845,17 +852,24 @@ __device__ __forceinline__ double recip(double d) {
     const double e = fma(-d, r, 1.);
     return fma(r, e, r);
 }
+__device__ __forceinline__ float recip(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float e = fmaf(-d, r, 1.f);
+    return fmaf(r, e, r);
+}
 
 // face mobility as face_mob, with recip in place of the division
-__device__ __forceinline__ double face_mob_pc(double wl, double wr, int harmonic) {
-    return harmonic ? 2. * recip(wl + wr) : 0.5 * (wl + wr);
+template <typename T>
+__device__ __forceinline__ T face_mob_pc(T wl, T wr, int harmonic) {
+    return harmonic ? T(2) * recip(wl + wr) : T(0.5) * (wl + wr);
 }
 
 // scaled face coefficients of one row for a lane's two columns g0, g0+1:
 // E = face to the next row, S0 = face left of g0, M = between g0 and g0+1,
 // N1 = face right of g0+1 (zero where the face is a domain boundary)
-struct FRow {
-    double E0, E1, S0, M, N1;
+template <typename T>
+struct FRowT {
+    T E0, E1, S0, M, N1;
 };
 
 struct Lane {
@@ -864,18 +878,20 @@ struct Lane {
     bool v0, v1, vR;   // columns g0, g0+1 and the pairs (g0,g0+1), (g0+1,g0+2) inside
 };
 
-__device__ __forceinline__ FRow row_faces(const OpGeom &g, const Lane &L, int m, double a0, double a1, double an0,
-                                          double an1, double y0, double y1) {
+template <typename T>
+__device__ __forceinline__ FRowT<T> row_faces(const OpGeom &g, const Lane &L, int m, T a0, T a1, T an0, T an1, T y0,
+                                              T y1) {
 #pragma clang fp contract(fast)
     const bool rin = m >= 0 && m < g.n1, rnx = m >= 0 && m + 1 < g.n1;
-    FRow f;
-    const double e0 = g.cx * face_mob_pc(a0, an0, g.harmonic), e1 = g.cx * face_mob_pc(a1, an1, g.harmonic);
-    const double yR = lane_next(y0);
-    const double fm = g.cy * face_mob_pc(y0, y1, g.harmonic), fn = g.cy * face_mob_pc(y1, yR, g.harmonic);
-    f.E0 = (rnx && L.v0) ? e0 : 0.;
-    f.E1 = (rnx && L.v1) ? e1 : 0.;
-    f.M = (rin && L.v1) ? fm : 0.;
-    f.N1 = (rin && L.vR) ? fn : 0.;
+    const T cx = T(g.cx), cy = T(g.cy);
+    FRowT<T> f;
+    const T e0 = cx * face_mob_pc(a0, an0, g.harmonic), e1 = cx * face_mob_pc(a1, an1, g.harmonic);
+    const T yR = lane_next(y0);
+    const T fm = cy * face_mob_pc(y0, y1, g.harmonic), fn = cy * face_mob_pc(y1, yR, g.harmonic);
+    f.E0 = (rnx && L.v0) ? e0 : T(0);
+    f.E1 = (rnx && L.v1) ? e1 : T(0);
+    f.M = (rin && L.v1) ? fm : T(0);
+    f.N1 = (rin && L.vR) ? fn : T(0);
     f.S0 = lane_prev(f.N1);
     return f;
 }
@@ -883,11 +899,10 @@ __device__ __forceinline__ FRow row_faces(const OpGeom &g, const Lane &L, int m,
 // one Gauss-Seidel update of the cell at column offset c of a row with faces
 // F, face above Wf (E of the previous row), neighbours above/below xa/xb and
 // the row's two values xr[2]
-template <int C>
-__device__ __forceinline__ double gs_cell(const FRow &F, const double Wf[2], double b, double xa, double xb,
-                                          const double xr[2]) {
+template <int C, typename T>
+__device__ __forceinline__ T gs_cell(const FRowT<T> &F, const T Wf[2], T b, T xa, T xb, const T xr[2]) {
 #pragma clang fp contract(fast)
-    double Wc, Ec, Sc, Nc, xS, xN;
+    T Wc, Ec, Sc, Nc, xS, xN;
     if (C == 0) {
         Wc = Wf[0]; Ec = F.E0; Sc = F.S0; Nc = F.M;
         xS = lane_prev(xr[1]); xN = xr[1];
@@ -895,13 +910,13 @@ __device__ __forceinline__ double gs_cell(const FRow &F, const double Wf[2], dou
         Wc = Wf[1]; Ec = F.E1; Sc = F.M; Nc = F.N1;
         xS = xr[0]; xN = lane_next(xr[0]);
     }
-    const double Cc = -((Wc + Ec) + (Sc + Nc));
-    double s = b;
+    const T Cc = -((Wc + Ec) + (Sc + Nc));
+    T s = b;
     s -= Wc * xa;
     s -= Sc * xS;
     s -= Ec * xb;
     s -= Nc * xN;
-    return Cc != 0. ? s * recip(Cc) : 0.;
+    return Cc != T(0) ? s * recip(Cc) : T(0);
 }
 
 struct Chunk {
@@ -909,7 +924,8 @@ struct Chunk {
     bool out;   // the lane writes (lanes 2..61)
 };
 
-__device__ __forceinline__ bool wave_setup(const FusedArgs &A, Lane &L, Chunk &K) {
+template <typename Args>
+__device__ __forceinline__ bool wave_setup(const Args &A, Lane &L, Chunk &K) {
     const int lane = threadIdx.x & 63;
     // wave-uniform by construction; readfirstlane lets the compiler know, so
     // the march loop and the row masks are scalar
@@ -933,14 +949,16 @@ __device__ __forceinline__ bool wave_setup(const FusedArgs &A, Lane &L, Chunk &K
 // address, so that no lane-dependent branch separates a prefetch from its
 // use; mask_pair replaces the values outside the level when they are used
 // (a select at load time would wait for the load there).
-__device__ __forceinline__ void load_pair(const double *__restrict__ p, int ldp, const Lane &L, int m, int n1,
-                                          double &a, double &b) {
+template <typename V>
+__device__ __forceinline__ void load_pair(const V *__restrict__ p, int ldp, const Lane &L, int m, int n1, V &a,
+                                          V &b) {
     const int mc = m < 0 ? 0 : (m > n1 - 1 ? n1 - 1 : m);
-    const double *q = p + (long)mc * ldp;
+    const V *q = p + (long)mc * ldp;
     a = q[L.c0];
     b = q[L.c1];
 }
-__device__ __forceinline__ void mask_pair(const Lane &L, int m, int n1, double fill, double &a, double &b) {
+template <typename V>
+__device__ __forceinline__ void mask_pair(const Lane &L, int m, int n1, V fill, V &a, V &b) {
     const bool rin = m >= 0 && m < n1;
     a = (rin && L.v0) ? a : fill;
     b = (rin && L.v1) ? b : fill;
@@ -952,8 +970,8 @@ __device__ __forceinline__ void mask_pair(const Lane &L, int m, int n1, double f
 #endif
 constexpr int SW_PF = IMPEC_SW_PF;
 
-template <bool FINE0, bool ANISO>
-__global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
+template <bool FINE0, bool ANISO, typename T>
+__global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgsT<T> A) {
 #pragma clang fp contract(fast)
     Lane L;
     Chunk K;
@@ -961,18 +979,22 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
     const int n1 = A.g.n1, n2 = A.g.n2;
     constexpr bool aniso = ANISO;
     const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
-    const double *src = FINE0 ? A.r : A.b;
+    using Src = typename std::conditional<FINE0, double, T>::type;   // level 0 reads the FP64 CG residual
+    const Src *src;
+    if constexpr (FINE0) src = A.r;
+    else src = A.b;
     // full weighting: this lane's coarse column and the weights of its four fine columns
     const int J = L.g0 >> 1;
     const bool wcol = K.out && L.g0 >= 0 && J < A.m2;
-    const double wjL = (L.g0 >= 1 && L.g0 - 1 < n2) ? pw(L.g0 - 1, J, A.m2) : 0.;
-    const double wj0 = L.v0 ? pw(L.g0, J, A.m2) : 0.;
-    const double wj1 = L.v1 ? pw(L.g0 + 1, J, A.m2) : 0.;
-    const double wjR = L.vR ? pw(L.g0 + 2, J, A.m2) : 0.;
+    const T wjL = T((L.g0 >= 1 && L.g0 - 1 < n2) ? pw(L.g0 - 1, J, A.m2) : 0.);
+    const T wj0 = T(L.v0 ? pw(L.g0, J, A.m2) : 0.);
+    const T wj1 = T(L.v1 ? pw(L.g0 + 1, J, A.m2) : 0.);
+    const T wjR = T(L.vR ? pw(L.g0 + 2, J, A.m2) : 0.);
     const int Ilo = K.i0 >> 1, Ihi = (K.i1 + 1) >> 1;
     const int kstart = K.i0 - 4, kend = 2 * Ihi + 2;
 
-    double PW[SW_PF][2], PY[SW_PF][2], PB[SW_PF][2];
+    T PW[SW_PF][2], PY[SW_PF][2];
+    Src PB[SW_PF][2];
     auto fetch = [&](int u, int k) {   // inputs of step k: w(k+1), w2(k), b(k)
         load_pair(A.w, A.ldw, L, k + 1, n1, PW[u][0], PW[u][1]);
         if (aniso) load_pair(A.w2, A.ldw, L, k, n1, PY[u][0], PY[u][1]);
@@ -980,16 +1002,16 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
     };
 #pragma unroll
     for (int u = 0; u < SW_PF; u++) fetch(u, kstart + u);
-    double wc0, wc1;
+    T wc0, wc1;
     load_pair(A.w, A.ldw, L, kstart, n1, wc0, wc1);
-    mask_pair(L, kstart, n1, 1., wc0, wc1);
+    mask_pair(L, kstart, n1, T(1), wc0, wc1);
 
-    double X[4][2] = {{0., 0.}, {0., 0.}, {0., 0.}, {0., 0.}};   // rows k-3 .. k
-    double B[3][2] = {{0., 0.}, {0., 0.}, {0., 0.}};             // rows k-2 .. k
-    FRow F[3];                                                   // rows k-2 .. k
-    double E[4][2] = {{0., 0.}, {0., 0.}, {0., 0.}, {0., 0.}};   // E of rows k-3 .. k
-    for (int q = 0; q < 3; q++) F[q] = FRow{0., 0., 0., 0., 0.};
-    double acc_c = 0., acc_n = 0.;
+    T X[4][2] = {};   // rows k-3 .. k
+    T B[3][2] = {};   // rows k-2 .. k
+    FRowT<T> F[3];    // rows k-2 .. k
+    T E[4][2] = {};   // E of rows k-3 .. k
+    for (int q = 0; q < 3; q++) F[q] = FRowT<T>{T(0), T(0), T(0), T(0), T(0)};
+    T acc_c = T(0), acc_n = T(0);
 
     // whole groups of SW_PF steps (no exit inside the unrolled group, so the
     // prefetch ring stays in place); steps past kend write nothing
@@ -998,18 +1020,18 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
 #pragma unroll
         for (int u = 0; u < SW_PF; u++) {
             const int k = kb + u;
-            double wn0 = PW[u][0], wn1 = PW[u][1];
-            mask_pair(L, k + 1, n1, 1., wn0, wn1);
-            double y0 = wc0, y1 = wc1;
+            T wn0 = PW[u][0], wn1 = PW[u][1];
+            mask_pair(L, k + 1, n1, T(1), wn0, wn1);
+            T y0 = wc0, y1 = wc1;
             if (aniso) {
                 y0 = PY[u][0];
                 y1 = PY[u][1];
-                mask_pair(L, k, n1, 1., y0, y1);
+                mask_pair(L, k, n1, T(1), y0, y1);
             }
-            double bk[2];
-            bk[0] = L.v0 && k >= 0 && k < n1 ? PB[u][0] - mean : 0.;
-            bk[1] = L.v1 && k >= 0 && k < n1 ? PB[u][1] - mean : 0.;
-            const FRow fk = row_faces(A.g, L, k, wc0, wc1, wn0, wn1, y0, y1);
+            T bk[2];
+            bk[0] = L.v0 && k >= 0 && k < n1 ? T(PB[u][0] - mean) : T(0);
+            bk[1] = L.v1 && k >= 0 && k < n1 ? T(PB[u][1] - mean) : T(0);
+            const FRowT<T> fk = row_faces(A.g, L, k, wc0, wc1, wn0, wn1, y0, y1);
             wc0 = wn0;
             wc1 = wn1;
 #pragma unroll
@@ -1024,27 +1046,27 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
             // red from zero on row k (its black cell stays zero)
             const int c = u & 1;   // k & 1 (kstart is even)
             {
-                const double C0 = -((E[2][0] + fk.E0) + (fk.S0 + fk.M));
-                const double C1 = -((E[2][1] + fk.E1) + (fk.M + fk.N1));
+                const T C0 = -((E[2][0] + fk.E0) + (fk.S0 + fk.M));
+                const T C1 = -((E[2][1] + fk.E1) + (fk.M + fk.N1));
                 if (c == 0) {
-                    X[3][0] = C0 != 0. ? bk[0] * recip(C0) : 0.;
-                    X[3][1] = 0.;
+                    X[3][0] = C0 != T(0) ? bk[0] * recip(C0) : T(0);
+                    X[3][1] = T(0);
                 } else {
-                    X[3][0] = 0.;
-                    X[3][1] = C1 != 0. ? bk[1] * recip(C1) : 0.;
+                    X[3][0] = T(0);
+                    X[3][1] = C1 != T(0) ? bk[1] * recip(C1) : T(0);
                 }
             }
             // black on row k-1
             if (c == 0) X[2][0] = gs_cell<0>(F[1], E[1], B[1][0], X[1][0], X[3][0], X[2]);
             else X[2][1] = gs_cell<1>(F[1], E[1], B[1][1], X[1][1], X[3][1], X[2]);
             // residual on row k-2
-            double r0, r1;
+            T r0, r1;
             {
-                const FRow &fr = F[0];
-                const double xl = lane_prev(X[1][1]), xr = lane_next(X[1][0]);
-                double W = E[0][0], Ee = fr.E0, S = fr.S0, N = fr.M;
-                double Cc = -((W + Ee) + (S + N));
-                double s = Cc * X[1][0];
+                const FRowT<T> &fr = F[0];
+                const T xl = lane_prev(X[1][1]), xr = lane_next(X[1][0]);
+                T W = E[0][0], Ee = fr.E0, S = fr.S0, N = fr.M;
+                T Cc = -((W + Ee) + (S + N));
+                T s = Cc * X[1][0];
                 s += W * X[0][0];
                 s += S * xl;
                 s += Ee * X[2][0];
@@ -1061,23 +1083,23 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
             }
             const int f = k - 2;
             if (K.out && f >= K.i0 && f < K.i1) {
-                double *xp = A.x + (long)f * A.ld + L.g0;
+                T *xp = A.x + (long)f * A.ld + L.g0;
                 if (L.v0) xp[0] = X[1][0];
                 if (L.v1) xp[1] = X[1][1];
             }
             // full weighting (P^T / 4, the transpose of the prolongation)
             {
-                const double rL = lane_prev(r1), rR = lane_next(r0);
-                const double h = ((wjL * rL + wj0 * r0) + wj1 * r1) + wjR * rR;
+                const T rL = lane_prev(r1), rR = lane_next(r0);
+                const T h = ((wjL * rL + wj0 * r0) + wj1 * r1) + wjR * rR;
                 const bool fin = f >= 0 && f < n1;
                 const int cur = (f - 1) >> 1;
-                const double wc = fin ? pw(f, cur, A.m1) : 0., wn = fin ? pw(f, cur + 1, A.m1) : 0.;
-                if (wc != 0.) acc_c += wc * h;
-                if (wn != 0.) acc_n += wn * h;
+                const T wc = T(fin ? pw(f, cur, A.m1) : 0.), wn = T(fin ? pw(f, cur + 1, A.m1) : 0.);
+                if (wc != T(0)) acc_c += wc * h;
+                if (wn != T(0)) acc_n += wn * h;
                 if (c == 0) {   // f even: coarse row cur is complete
-                    if (wcol && cur >= Ilo && cur < Ihi) A.bc[(long)cur * A.ldc + J] = 0.25 * acc_c;
+                    if (wcol && cur >= Ilo && cur < Ihi) A.bc[(long)cur * A.ldc + J] = T(0.25) * acc_c;
                     acc_c = acc_n;
-                    acc_n = 0.;
+                    acc_n = T(0);
                 }
             }
             // refill the slot once its registers are dead (an earlier refill
@@ -1087,8 +1109,8 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgs A) {
     }
 }
 
-template <bool FINE0, bool ANISO>
-__global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
+template <bool FINE0, bool ANISO, typename T>
+__global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgsT<T> A) {
 #pragma clang fp contract(fast)
     Lane L;
     Chunk K;
@@ -1096,19 +1118,23 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
     const int n1 = A.g.n1, m1 = A.m1, m2 = A.m2;
     constexpr bool aniso = ANISO;
     const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
-    const double *src = FINE0 ? A.r : A.b;
+    using Src = typename std::conditional<FINE0, double, T>::type;   // level 0 reads the FP64 CG residual
+    const Src *src;
+    if constexpr (FINE0) src = A.r;
+    else src = A.b;
     // prolongation: this lane's coarse column J and its clamped neighbours
     const int J = L.g0 >> 1;
     const bool jok = L.g0 >= 0 && J < m2;
     const bool jfirst = J == 0, jlast = J == m2 - 1;
     const int Jc = J < 0 ? 0 : (J > m2 - 1 ? m2 - 1 : J);
-    auto coarse = [&](int I) -> double {   // raw; select with jok at use
+    auto coarse = [&](int I) -> T {   // raw; select with jok at use
         I = I < 0 ? 0 : (I > m1 - 1 ? m1 - 1 : I);
         return A.xc[(long)I * A.ldc + Jc];
     };
     const int kstart = K.i0 - 2, kend = K.i1 + 1;
 
-    double PX[SW_PF][2], PW[SW_PF][2], PY[SW_PF][2], PB[SW_PF][2], PC[SW_PF][2];
+    T PX[SW_PF][2], PW[SW_PF][2], PY[SW_PF][2], PC[SW_PF][2];
+    Src PB[SW_PF][2];
     // inputs of step k: x(k), w(k+1), w2(k), b(k) and the coarse rows I0 = k>>1,
     // I1 = I0 -+ 1 of the prolongation (re-read for every fine row; cached)
     auto fetch = [&](int u, int k) {
@@ -1121,15 +1147,15 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
     };
 #pragma unroll
     for (int u = 0; u < SW_PF; u++) fetch(u, kstart + u);
-    double wc0, wc1;
+    T wc0, wc1;
     load_pair(A.w, A.ldw, L, kstart, n1, wc0, wc1);
-    mask_pair(L, kstart, n1, 1., wc0, wc1);
+    mask_pair(L, kstart, n1, T(1), wc0, wc1);
 
-    double X[4][2] = {{0., 0.}, {0., 0.}, {0., 0.}, {0., 0.}};   // rows k-3 .. k
-    double B[3][2] = {{0., 0.}, {0., 0.}, {0., 0.}};             // raw rhs rows k-2 .. k
-    FRow F[3];
-    double E[4][2] = {{0., 0.}, {0., 0.}, {0., 0.}, {0., 0.}};
-    for (int q = 0; q < 3; q++) F[q] = FRow{0., 0., 0., 0., 0.};
+    T X[4][2] = {};     // rows k-3 .. k
+    Src B[3][2] = {};   // raw rhs rows k-2 .. k
+    FRowT<T> F[3];
+    T E[4][2] = {};
+    for (int q = 0; q < 3; q++) F[q] = FRowT<T>{T(0), T(0), T(0), T(0), T(0)};
     double acc0 = 0., acc1 = 0.;
 
     // whole groups of SW_PF steps (no exit inside the unrolled group, so the
@@ -1140,19 +1166,19 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
         for (int u = 0; u < SW_PF; u++) {
             const int k = kb + u;
             const int c = u & 1;   // k & 1
-            const double xk0 = PX[u][0], xk1 = PX[u][1];
-            double wn0 = PW[u][0], wn1 = PW[u][1];
-            mask_pair(L, k + 1, n1, 1., wn0, wn1);
-            double y0 = wc0, y1 = wc1;
+            const T xk0 = PX[u][0], xk1 = PX[u][1];
+            T wn0 = PW[u][0], wn1 = PW[u][1];
+            mask_pair(L, k + 1, n1, T(1), wn0, wn1);
+            T y0 = wc0, y1 = wc1;
             if (aniso) {
                 y0 = PY[u][0];
                 y1 = PY[u][1];
-                mask_pair(L, k, n1, 1., y0, y1);
+                mask_pair(L, k, n1, T(1), y0, y1);
             }
-            double bk0 = PB[u][0], bk1 = PB[u][1];
-            mask_pair(L, k, n1, 0., bk0, bk1);
-            const double o0 = jok ? PC[u][0] : 0., o1 = jok ? PC[u][1] : 0.;
-            const FRow fk = row_faces(A.g, L, k, wc0, wc1, wn0, wn1, y0, y1);
+            Src bk0 = PB[u][0], bk1 = PB[u][1];
+            mask_pair(L, k, n1, Src(0), bk0, bk1);
+            const T o0 = jok ? PC[u][0] : T(0), o1 = jok ? PC[u][1] : T(0);
+            const FRowT<T> fk = row_faces(A.g, L, k, wc0, wc1, wn0, wn1, y0, y1);
             wc0 = wn0;
             wc1 = wn1;
 #pragma unroll
@@ -1166,29 +1192,29 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgs A) {
             E[3][0] = fk.E0; E[3][1] = fk.E1;
             // x + P e on row k from coarse rows I0 (o0) and I1 (o1)
             {
-                const double pl0 = lane_prev(o0), pl1 = lane_prev(o1), pr0 = lane_next(o0), pr1 = lane_next(o1);
-                const double l0 = jfirst ? o0 : pl0, l1 = jfirst ? o1 : pl1;
-                const double r0 = jlast ? o0 : pr0, r1 = jlast ? o1 : pr1;
-                const double e0 = 0.5625 * o0 + 0.1875 * l0 + 0.1875 * o1 + 0.0625 * l1;
-                const double e1 = 0.5625 * o0 + 0.1875 * r0 + 0.1875 * o1 + 0.0625 * r1;
+                const T pl0 = lane_prev(o0), pl1 = lane_prev(o1), pr0 = lane_next(o0), pr1 = lane_next(o1);
+                const T l0 = jfirst ? o0 : pl0, l1 = jfirst ? o1 : pl1;
+                const T r0 = jlast ? o0 : pr0, r1 = jlast ? o1 : pr1;
+                const T e0 = T(0.5625) * o0 + T(0.1875) * l0 + T(0.1875) * o1 + T(0.0625) * l1;
+                const T e1 = T(0.5625) * o0 + T(0.1875) * r0 + T(0.1875) * o1 + T(0.0625) * r1;
                 const bool rin = k >= 0 && k < n1;
-                X[3][0] = (rin && L.v0) ? xk0 + e0 : 0.;
-                X[3][1] = (rin && L.v1) ? xk1 + e1 : 0.;
+                X[3][0] = (rin && L.v0) ? xk0 + e0 : T(0);
+                X[3][1] = (rin && L.v1) ? xk1 + e1 : T(0);
             }
             // black on row k-1
-            if (c == 0) X[2][0] = gs_cell<0>(F[1], E[1], B[1][0] - mean, X[1][0], X[3][0], X[2]);
-            else X[2][1] = gs_cell<1>(F[1], E[1], B[1][1] - mean, X[1][1], X[3][1], X[2]);
+            if (c == 0) X[2][0] = gs_cell<0>(F[1], E[1], T(B[1][0] - mean), X[1][0], X[3][0], X[2]);
+            else X[2][1] = gs_cell<1>(F[1], E[1], T(B[1][1] - mean), X[1][1], X[3][1], X[2]);
             // red on row k-2
-            if (c == 0) X[1][0] = gs_cell<0>(F[0], E[0], B[0][0] - mean, X[0][0], X[2][0], X[1]);
-            else X[1][1] = gs_cell<1>(F[0], E[0], B[0][1] - mean, X[0][1], X[2][1], X[1]);
+            if (c == 0) X[1][0] = gs_cell<0>(F[0], E[0], T(B[0][0] - mean), X[0][0], X[2][0], X[1]);
+            else X[1][1] = gs_cell<1>(F[0], E[0], T(B[0][1] - mean), X[0][1], X[2][1], X[1]);
             const int f = k - 2;
             if (K.out && f >= K.i0 && f < K.i1) {
-                double *xp = A.xo + (long)f * A.ld + L.g0;
+                T *xp = A.xo + (long)f * A.ld + L.g0;
                 if (L.v0) xp[0] = X[1][0];
                 if (L.v1) xp[1] = X[1][1];
-                if (FINE0) {
-                    if (L.v0) { acc0 += X[1][0]; acc1 += B[0][0] * X[1][0]; }
-                    if (L.v1) { acc0 += X[1][1]; acc1 += B[0][1] * X[1][1]; }
+                if (FINE0) {   // FP64 partials of the cycle output
+                    if (L.v0) { acc0 += double(X[1][0]); acc1 += double(B[0][0]) * double(X[1][0]); }
+                    if (L.v1) { acc0 += double(X[1][1]); acc1 += double(B[0][1]) * double(X[1][1]); }
                 }
             }
             fetch(u, k + SW_PF);   // the slot's registers are dead here
@@ -1239,7 +1265,8 @@ static Grid2 vec_grid(int n1, int n2) {
 
 // d_new = (x0 - mean) + beta d_old on the cell and its four neighbours,
 // q = A_N d_new (pinned row zeroed in the eliminated formulation), d.q partials
-__global__ void k_cg_dir_matvec(LevArgs L, CgArgs c, const double *__restrict__ x0, const double *__restrict__ dold,
+template <typename T>
+__global__ void k_cg_dir_matvec(LevArgs L, CgArgs c, const T *__restrict__ x0, const double *__restrict__ dold,
                                 double *__restrict__ dnew, double *__restrict__ qv, const double *__restrict__ S,
                                 int first, double *__restrict__ part) {
     const double mean = S[S_ZSUM] / (double)c.n;
@@ -1252,8 +1279,8 @@ __global__ void k_cg_dir_matvec(LevArgs L, CgArgs c, const double *__restrict__ 
         const long o = (long)(i + 1) * ld + (j + 1);
 #define DN(off_, ii_, jj_)                                                                                  \
     ((!c.shift && (ii_) == c.pin_i && (jj_) == c.pin_j) ? 0.                                                \
-                                                         : (first ? x0[o + (off_)] - mean                   \
-                                                                  : (x0[o + (off_)] - mean) + beta * dold[o + (off_)]))
+                                                         : (first ? (double)x0[o + (off_)] - mean           \
+                                                                  : ((double)x0[o + (off_)] - mean) + beta * dold[o + (off_)]))
         const double dc = DN(0, i, j);
         double a[9];
         op_row5(L, i, j, a);
@@ -1311,27 +1338,48 @@ static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip) {
     return (int)r;
 }
 
-static FusedArgs fused_args(impec_ctx *ctx, int l) {
-    FusedArgs A;
+// the level arrays a V-cycle of storage type T uses
+template <typename T> struct LevelArrays;
+template <> struct LevelArrays<double> {
+    static double *x(const Level &L) { return L.x; }
+    static double *b(const Level &L) { return L.b; }
+    static double *r(const Level &L) { return L.r; }
+};
+template <> struct LevelArrays<float> {
+    static float *x(const Level &L) { return L.xf; }
+    static float *b(const Level &L) { return L.bf; }
+    static float *r(const Level &L) { return L.rf; }
+};
+
+template <typename T>
+static FusedArgsT<T> fused_args(impec_ctx *ctx, int l) {
+    using LA = LevelArrays<T>;
+    FusedArgsT<T> A;
     memset(&A, 0, sizeof(A));
     const Level &L = ctx->lev[l];
     LevArgs la = lev_args(ctx, l);
     A.g = L.g;
-    A.w = la.w1;
-    A.w2 = la.w2;
-    A.ldw = la.ldw;
+    if constexpr (std::is_same<T, double>::value) {
+        A.w = la.w1;
+        A.w2 = la.w2;
+        A.ldw = la.ldw;
+    } else {   // FP32 copies, ring-1 layout of this level
+        A.w = L.wf + L.ld + 1;
+        A.w2 = (l == 0 && L.wf2) ? L.wf2 + L.ld + 1 : A.w;
+        A.ldw = L.ld;
+    }
     A.ld = L.ld;
-    A.b = L.b + L.ld + 1;
+    A.b = LA::b(L) + L.ld + 1;
     A.r = ctx->pr + L.ld + 1;
     A.S = ctx->scal;
     A.inv_n = 1.0 / ((double)ctx->N1 * ctx->N2);
-    A.x = L.x + L.ld + 1;
-    A.xo = L.r + L.ld + 1;   // the fused cycle leaves each level's result in r
+    A.x = LA::x(L) + L.ld + 1;
+    A.xo = LA::r(L) + L.ld + 1;   // the fused cycle leaves each level's result in r
     if (l + 1 < ctx->nlev) {
         const Level &Cl = ctx->lev[l + 1];
         // the coarsest level is solved in place into x, the others end in r
-        A.xc = (l + 2 == ctx->nlev ? Cl.x : Cl.r) + Cl.ld + 1;
-        A.bc = Cl.b + Cl.ld + 1;
+        A.xc = (l + 2 == ctx->nlev ? LA::x(Cl) : LA::r(Cl)) + Cl.ld + 1;
+        A.bc = LA::b(Cl) + Cl.ld + 1;
         A.m1 = Cl.n1;
         A.m2 = Cl.n2;
         A.ldc = Cl.ld;
@@ -1343,35 +1391,80 @@ static FusedArgs fused_args(impec_ctx *ctx, int l) {
     return A;
 }
 
-static dim3 stream_grid(const FusedArgs &A) { return dim3((A.nwave + SW_WAVES - 1) / SW_WAVES); }
+template <typename T>
+static dim3 stream_grid(const FusedArgsT<T> &A) { return dim3((A.nwave + SW_WAVES - 1) / SW_WAVES); }
 
-// one V(1,1) cycle on the level-0 rhs r - mean(r); the result x0 is left in
-// lev[0].r, with the per-wave partials of sum(x0) and r.x0 for k_finish_rho
+// one V(1,1) cycle on the level-0 rhs r - mean(r) in storage type T; the result
+// x0 is left in lev[0]'s r array of that type, with the per-wave FP64 partials
+// of sum(x0) and r.x0 for k_finish_rho
+template <typename T>
 static int vcycle_fused(impec_ctx *ctx) {
+    using LA = LevelArrays<T>;
     const int L = ctx->nlev;
     const dim3 blk(64 * SW_WAVES);
     for (int l = 0; l < L - 1; l++) {
-        FusedArgs A = fused_args(ctx, l);
+        FusedArgsT<T> A = fused_args<T>(ctx, l);
         prof_begin(ctx);
-        if (l == 0 && A.w2 != A.w) hipLaunchKernelGGL((k_mg_down<true, true>), stream_grid(A), blk, 0, ctx->stream, A);
-        else if (l == 0) hipLaunchKernelGGL((k_mg_down<true, false>), stream_grid(A), blk, 0, ctx->stream, A);
-        else hipLaunchKernelGGL((k_mg_down<false, false>), stream_grid(A), blk, 0, ctx->stream, A);
+        if (l == 0 && A.w2 != A.w)
+            hipLaunchKernelGGL((k_mg_down<true, true, T>), stream_grid(A), blk, 0, ctx->stream, A);
+        else if (l == 0) hipLaunchKernelGGL((k_mg_down<true, false, T>), stream_grid(A), blk, 0, ctx->stream, A);
+        else hipLaunchKernelGGL((k_mg_down<false, false, T>), stream_grid(A), blk, 0, ctx->stream, A);
         prof_end(ctx, l == 0 ? "mg_down_l0" : "mg_coarse");
     }
     {
         Level &Cl = ctx->lev[L - 1];
         LevArgs a = lev_args(ctx, L - 1);
         prof_begin(ctx);
-        hipLaunchKernelGGL(k_coarsest<false>, dim3(1), dim3(64), 0, ctx->stream, a, Cl.x + Cl.ld + 1, Cl.b + Cl.ld + 1);
+        hipLaunchKernelGGL((k_coarsest<false, T>), dim3(1), dim3(64), 0, ctx->stream, a, LA::x(Cl) + Cl.ld + 1,
+                           LA::b(Cl) + Cl.ld + 1);
         prof_end(ctx, "mg_coarse");
     }
     for (int l = L - 2; l >= 0; l--) {
-        FusedArgs A = fused_args(ctx, l);
+        FusedArgsT<T> A = fused_args<T>(ctx, l);
         prof_begin(ctx);
-        if (l == 0 && A.w2 != A.w) hipLaunchKernelGGL((k_mg_up<true, true>), stream_grid(A), blk, 0, ctx->stream, A);
-        else if (l == 0) hipLaunchKernelGGL((k_mg_up<true, false>), stream_grid(A), blk, 0, ctx->stream, A);
-        else hipLaunchKernelGGL((k_mg_up<false, false>), stream_grid(A), blk, 0, ctx->stream, A);
+        if (l == 0 && A.w2 != A.w)
+            hipLaunchKernelGGL((k_mg_up<true, true, T>), stream_grid(A), blk, 0, ctx->stream, A);
+        else if (l == 0) hipLaunchKernelGGL((k_mg_up<true, false, T>), stream_grid(A), blk, 0, ctx->stream, A);
+        else hipLaunchKernelGGL((k_mg_up<false, false, T>), stream_grid(A), blk, 0, ctx->stream, A);
         prof_end(ctx, l == 0 ? "mg_up_l0" : "mg_coarse");
+    }
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+// FP32 copy of a mobility array (rows x cols cells from (0,0) of each)
+__global__ void k_to_f32(const double *__restrict__ src, int lds, float *__restrict__ dst, int ldd, int rows,
+                         int cols) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i < rows && j < cols) dst[(long)i * ldd + j] = (float)src[(long)i * lds + j];
+}
+
+// FP32 level arrays of the mixed-precision cycle (first use) and the FP32
+// copies of this solve's mobility hierarchy
+static int mixed_setup(impec_ctx *ctx) {
+    const bool aniso = ctx->K22 && ctx->K22 != ctx->K11;
+    for (int l = 0; l < ctx->nlev; l++) {
+        Level &L = ctx->lev[l];
+        const size_t bytes = sizeof(float) * (size_t)(L.n1 + 2) * L.ld;
+        float **arrs[5] = {&L.wf, &L.xf, &L.bf, &L.rf, &L.wf2};
+        const int na = (l == 0 && aniso) ? 5 : 4;
+        for (int k = 0; k < na; k++) {
+            if (*arrs[k]) continue;
+            if (hipMalloc(arrs[k], bytes) != hipSuccess)
+                return set_error(ctx, IMPEC_ENOMEM, "mixed-precision level %d allocation failed", l);
+            hipMemset(*arrs[k], 0, bytes);
+        }
+    }
+    dim3 blk(64, 4);
+    for (int l = 0; l < ctx->nlev; l++) {
+        Level &L = ctx->lev[l];
+        LevArgs la = lev_args(ctx, l);
+        hipLaunchKernelGGL(k_to_f32, grid2(L.n1, L.n2, blk), blk, 0, ctx->stream, la.w1, la.ldw, L.wf + L.ld + 1,
+                           L.ld, L.n1, L.n2);
+        if (l == 0 && L.wf2 && aniso)
+            hipLaunchKernelGGL(k_to_f32, grid2(L.n1, L.n2, blk), blk, 0, ctx->stream, la.w2, la.ldw,
+                               L.wf2 + L.ld + 1, L.ld, L.n1, L.n2);
     }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
@@ -1423,20 +1516,23 @@ static int cg_iteration_unfused(impec_ctx *ctx, const LevArgs &a0, const CgArgs 
     return IMPEC_OK;
 }
 
-// one CG iteration with the streaming V-cycle (nu = 1); d ping-pongs
+// one CG iteration with the streaming V-cycle (nu = 1) in storage type T
+// (double: plain FP64; float: the FP32 cycle of the mixed-precision solve);
+// the CG vectors, the residual and every reduction stay FP64; d ping-pongs
+template <typename T>
 static int cg_iteration_fused(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, bool first, double **dold,
                               double **dnew) {
     const Level &L0 = ctx->lev[0];
     hipStream_t s = ctx->stream;
     const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
-    const int nwave0 = fused_args(ctx, 0).nwave;
+    const int nwave0 = fused_args<T>(ctx, 0).nwave;
     const double inv_n = 1.0 / (double)c.n;
     int st;
-    if ((st = vcycle_fused(ctx))) return st;
+    if ((st = vcycle_fused<T>(ctx))) return st;
     hipLaunchKernelGGL(k_finish_rho, dim3(1), dim3(1024), 0, s, ctx->part, nwave0, inv_n, ctx->scal);
     prof_begin(ctx);
-    hipLaunchKernelGGL(k_cg_dir_matvec, vg.grid, vg.block, 0, s, a0, c, L0.r, *dold, *dnew, ctx->pq, ctx->scal,
-                       first, ctx->part);
+    hipLaunchKernelGGL((k_cg_dir_matvec<T>), vg.grid, vg.block, 0, s, a0, c, LevelArrays<T>::r(L0), *dold, *dnew,
+                       ctx->pq, ctx->scal, first, ctx->part);
     prof_end(ctx, "cg_dir_matvec");
     if ((st = finish_sum(ctx, vg.nblocks, 1, S_DQ))) return st;
     prof_begin(ctx);
@@ -1460,12 +1556,13 @@ static int cg_iteration_fused(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c
 //           zero only to rounding, so a large shift leaves a small residual).
 // Each phase confirms convergence on a recomputed residual and restarts from
 // it when the recursion has drifted.
-template <bool DIAG, bool FUSED>
+template <bool DIAG, bool FUSED, typename T = double>
 static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     int st = launch_mobility(ctx);
     if (st) return st;
     st = build_hierarchy(ctx);
     if (st) return st;
+    if (std::is_same<T, float>::value && (st = mixed_setup(ctx))) return st;
     const Level &L0 = ctx->lev[0];
     LevArgs a0 = lev_args(ctx, 0);
     if (!DIAG && !IMPEC_NO_FACES) {
@@ -1501,7 +1598,7 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
             while (!(rr <= tol * tol * bb)) {
                 if (its >= maxit) { status = IMPEC_ENOCONV; break; }
                 if (!(rr == rr)) { status = IMPEC_ENAN; break; }
-                if (FUSED && !DIAG) st = cg_iteration_fused(ctx, a0, c, since == 0, &dold, &dnew);
+                if (FUSED && !DIAG) st = cg_iteration_fused<T>(ctx, a0, c, since == 0, &dold, &dnew);
                 else st = cg_iteration_unfused<DIAG>(ctx, a0, c, nb, since == 0);
                 if (st) return st;
                 if ((st = fetch_scal(ctx))) return st;
@@ -1544,8 +1641,12 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
 
 int pressure_solve(impec_ctx *ctx, int *its, double *relres) {
     if (ctx->P.nu != 1.) return solve_cg<true, false>(ctx, its, relres);
-    const bool fused_ok = ctx->P.mg_pre <= 1 && ctx->P.mg_post <= 1 && fused_args(ctx, 0).nwave <= RED_BLOCKS &&
-                          !ctx->force_unfused;
+    const bool fused_ok = ctx->P.mg_pre <= 1 && ctx->P.mg_post <= 1 &&
+                          fused_args<double>(ctx, 0).nwave <= RED_BLOCKS && !ctx->force_unfused;
+    // mixed precision: FP32 V-cycles as the preconditioner of the FP64 CG (a
+    // Krylov-accelerated defect correction: residual, reductions, update and
+    // the coarsest solve in FP64); on the streaming 5-point path only
+    if (fused_ok && ctx->P.precision == IMPEC_PREC_MIXED) return solve_cg<false, true, float>(ctx, its, relres);
     if (fused_ok) return solve_cg<false, true>(ctx, its, relres);
     return solve_cg<false, false>(ctx, its, relres);
 }
