@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def workload_params(capi, n, precision="fp64", guess="zero", mg_rows=0):
+def workload_params(capi, n, precision="fp64", guess="zero", mg_rows=0, cn_guess=0):
     return capi.default_params(
         N1=n, N2=n, source_type=capi.SRC_DIAGONAL, average_procedure=capi.AVG_HARMONIC,
         dim_splitting=capi.SPLIT_PARALLEL, hyper_method=capi.HYPER_MUSCL, diffusion=1,
@@ -38,7 +38,8 @@ def workload_params(capi, n, precision="fp64", guess="zero", mg_rows=0):
         nu=1.0, mu_res=1.0, mu_inj=0.1, porosity=1.0, cfl=0.5, inj_fluid_conc=1.0,
         alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5, eps=1e-10, para_eps=1e-10, I=1.0,
         precision=capi.PREC_MIXED if precision == "mixed" else capi.PREC_FP64,
-        initial_guess=capi.GUESS_PREVIOUS if guess == "previous" else capi.GUESS_ZERO, mg_rows=mg_rows)
+        initial_guess=capi.GUESS_PREVIOUS if guess == "previous" else capi.GUESS_ZERO, mg_rows=mg_rows,
+        cn_guess=cn_guess)
 
 
 def one_cell_source(n):

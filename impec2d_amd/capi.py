@@ -31,14 +31,14 @@ GUESS_ZERO, GUESS_PREVIOUS = 0, 1
 
 _INT_FIELDS = ["N1", "N2", "num_ghostcells", "source_type", "average_procedure", "dim_splitting",
                "hyper_method", "diffusion", "para_method", "is_test", "alpha_absolute",
-               "max_iterations", "precision", "initial_guess", "mg_pre", "mg_post", "mg_variant", "mg_rows"]
+               "max_iterations", "precision", "initial_guess", "mg_pre", "mg_post", "mg_variant", "mg_rows", "cn_guess"]
 _DBL_FIELDS = ["x_lb", "x_ub", "y_lb", "y_ub", "nu", "mu_res", "mu_inj", "porosity", "cfl",
                "inj_fluid_conc", "alpha_l", "alpha_t", "dt_diff_limiter", "d_m", "eps", "para_eps",
                "I", "tstart", "inner_tol"]
 
 
 class Params(C.Structure):
-    _fields_ = ([(n, C.c_int) for n in _INT_FIELDS] + [("reserved_i", C.c_int * 2)]
+    _fields_ = ([(n, C.c_int) for n in _INT_FIELDS] + [("reserved_i", C.c_int * 1)]
                 + [(n, C.c_double) for n in _DBL_FIELDS] + [("reserved_d", C.c_double * 3)])
 
 

@@ -148,6 +148,7 @@ int impec_create(const impec_params *P, impec_ctx **out) {
     if (P->dim_splitting < 0 || P->dim_splitting > 4) return IMPEC_EINVAL;
     if (P->precision != IMPEC_PREC_FP64 && P->precision != IMPEC_PREC_MIXED) return IMPEC_EINVAL;
     if (P->mg_rows < 0 || P->mg_rows > 4096) return IMPEC_EINVAL;
+    if (P->cn_guess != 0 && P->cn_guess != 1) return IMPEC_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IMPEC_ENODEV;
 

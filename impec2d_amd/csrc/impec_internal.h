@@ -22,7 +22,7 @@ constexpr int VEC_BLOCKS = IMPEC_VEC_BLOCKS;
 // except the convergence test.
 enum Scal {
     S_RHO = 0, S_RHO_OLD = 1, S_DQ = 2, S_RR = 3, S_RSUM = 4, S_ZSUM = 5, S_ALPHA = 6, S_BETA = 7,
-    S_OMEGA = 8, S_TS = 9, S_TT = 10, S_R0V = 11, S_XPIN = 12, S_RX = 14,
+    S_OMEGA = 8, S_TS = 9, S_TT = 10, S_R0V = 11, S_XPIN = 12, S_RX = 14, S_BB = 15,
     S_MASS = 16, S_DIV = 17, S_NOFLOW = 18,          // written together by the diagnostics pass
     S_MAXVEL = 20, S_MINDT = 21, S_MINDT_DIFF = 22,  // CFL pass
     S_OUTFLUX = 24, S_TMO = 25, S_CMAX = 26, S_CMIN = 27,

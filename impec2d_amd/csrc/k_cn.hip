@@ -338,6 +338,46 @@ __global__ void k_bicg_init(int N1, int N2, const double *__restrict__ b, double
     }
 }
 
+// initial guess x0 = c (interior): x = x0, r = r0 = b - A x0; p, v were zeroed
+// by k_bicg_init; partials r.r (twice)
+__global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ d11,
+                             const double *__restrict__ d12, const double *__restrict__ d22,
+                             const double *__restrict__ d21, const double *__restrict__ b, double *__restrict__ x,
+                             double *__restrict__ r, double *__restrict__ r0, double *__restrict__ part) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    double s[2] = {0., 0.};
+    if (j < k.N2)
+        for (int i = blockIdx.y * 4 + threadIdx.y; i < k.N1; i += gridDim.y * 4) {
+            double a[9];
+            cn_row(k, d11, d12, d22, d21, i, j, a);
+            const double *cc = c + (long)(i + k.B) * lds + (j + k.B);
+            const long L = lds;
+            const bool we = i > 0, ea = i < k.N1 - 1, so = j > 0, no = j < k.N2 - 1;
+            double acc = a[ST_C] * cc[0];
+            if (we) acc += a[ST_W] * cc[-L];
+            if (we && so) acc += a[ST_SW] * cc[-L - 1];
+            if (so) acc += a[ST_S] * cc[-1];
+            if (ea && so) acc += a[ST_SE] * cc[L - 1];
+            if (ea) acc += a[ST_E] * cc[L];
+            if (ea && no) acc += a[ST_NE] * cc[L + 1];
+            if (no) acc += a[ST_N] * cc[1];
+            if (we && no) acc += a[ST_NW] * cc[-L + 1];
+            const long id = (long)i * k.N2 + j;
+            const double rv = b[id] - acc;
+            x[id] = cc[0];
+            r[id] = rv;
+            r0[id] = rv;
+            s[0] += rv * rv;
+            s[1] += rv * rv;
+        }
+    block_reduce<RED_SUM, 2>(s);
+    if (tid_linear() == 0) {
+        const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+        part[blk] = s[0];
+        part[RED_BLOCKS + blk] = s[1];
+    }
+}
+
 // x += alpha p/D + omega s/D, r = s - omega t; partials r.r, r0.r
 __global__ void k_bicg_xr(int N1, int N2, double *__restrict__ x, const double *__restrict__ p,
                           const double *__restrict__ sv, const double *__restrict__ dinv, double *__restrict__ r,
@@ -365,7 +405,8 @@ __global__ void k_bicg_xr(int N1, int N2, double *__restrict__ x, const double *
 }
 
 // second pass of the BiCGStab reductions and the scalar recurrences:
-//  stage 0 (init):    rr = b.b, rho = alpha = omega = 1, rho_new = b.b
+//  stage 0 (init):    rr = bb = b.b, rho = alpha = omega = 1, rho_new = b.b
+//  stage 4 (guess):   rr = rho_new = r0.r0 of the initial residual
 //  stage 1 (after <0>): r0.v; rho = rho_new; alpha = rho / r0.v
 //  stage 2 (after <1>): t.s, t.t; omega = t.s / t.t (0 when t = 0)
 //  stage 3 (after xr):  rr = r.r, rho_new = r0.r
@@ -375,7 +416,11 @@ __global__ void k_bicg_finish(const double *__restrict__ part, int nb, int stage
     if (threadIdx.x != 0) return;
     if (stage == 0) {
         S[S_RR] = v[0];
+        S[S_BB] = v[0];
         S[S_RHO] = 1.; S[S_ALPHA] = 1.; S[S_OMEGA] = 1.;
+        S[S_RHO_NEW] = v[1];
+    } else if (stage == 4) {
+        S[S_RR] = v[0];
         S[S_RHO_NEW] = v[1];
     } else if (stage == 1) {
         S[S_R0V] = v[0];
@@ -455,9 +500,10 @@ static int fetch(impec_ctx *ctx) {
     return IMPEC_OK;
 }
 
-// Solve_Parabolic_cc_to_c: x0 = 0 (AssignRHS_Parabolic, MGSolver.cpp:1375-1392),
-// stop on ||rhs - A x||_2 <= para_eps ||rhs||_2 (recursive residual), solution
-// into c's interior.
+// Solve_Parabolic_cc_to_c: x0 = 0 (AssignRHS_Parabolic, MGSolver.cpp:1375-1392)
+// or, with cn_guess = 1, x0 = the current c; stop on
+// ||rhs - A x||_2 <= para_eps ||rhs||_2 (recursive residual); solution into
+// c's interior.
 int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     const CnConst k = cn_const(ctx, dt);
     const int N1 = ctx->N1, N2 = ctx->N2;
@@ -472,10 +518,15 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     const int nvb = (int)(vg.x * vg.y);
     hipLaunchKernelGGL(k_bicg_init, vg, vblk, 0, s, N1, N2, b, r, r0, pb[0], vb[0], x, ctx->part);
     hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 0, ctx->scal);
+    if (ctx->P.cn_guess == 1) {   // start from the current concentration
+        hipLaunchKernelGGL(k_bicg_guess, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->d11, ctx->d12, ctx->d22, ctx->d21,
+                           b, x, r, r0, ctx->part);
+        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 4, ctx->scal);
+    }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     if ((st = fetch(ctx))) return st;
-    const double bb = ctx->h_scal[S_RR];
-    double rr = bb;
+    const double bb = ctx->h_scal[S_BB];
+    double rr = ctx->h_scal[S_RR];
     const double tol = ctx->P.para_eps;
     const int maxit = ctx->P.max_iterations > 0 ? ctx->P.max_iterations : 10000;
 

@@ -74,7 +74,9 @@ typedef struct impec_params {
                                /* 1: one kernel per smoothing/transfer pass    */
     int mg_rows;               /* rows per wavefront chunk of the fused finest */
                                /* level (0: automatic; rounded up to even)     */
-    int reserved_i[2];
+    int cn_guess;              /* ext: initial guess of the dispersion solve:  */
+                               /* 0 zero (AssignRHS_Parabolic), 1 current c    */
+    int reserved_i[1];
     double x_lb, x_ub, y_lb, y_ub;
     double nu;                 /* hyperbolic.nu, 5-pt/rotated stencil weight   */
     double mu_res, mu_inj;
