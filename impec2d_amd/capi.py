@@ -27,7 +27,7 @@ SPLIT_ALTERNATING, SPLIT_X_FIRST, SPLIT_Y_FIRST, SPLIT_STRANG, SPLIT_PARALLEL = 
 HYPER_MUSCL, HYPER_UPWIND = 0, 1
 PARA_EXPLICIT, PARA_CN = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-GUESS_ZERO, GUESS_PREVIOUS = 0, 1
+GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED = 0, 1, 2
 
 _INT_FIELDS = ["N1", "N2", "num_ghostcells", "source_type", "average_procedure", "dim_splitting",
                "hyper_method", "diffusion", "para_method", "is_test", "alpha_absolute",
@@ -52,6 +52,7 @@ class Stats(C.Structure):
 # every entry point declared in include/impec_hip.h: name -> (argtypes, restype)
 _vp = C.c_void_p
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
 SIGNATURES = {
     "impec_params_default": ([C.POINTER(Params)], C.c_int),
@@ -79,6 +80,21 @@ SIGNATURES = {
     "impec_advect_x": ([_vp, C.c_double], C.c_int),
     "impec_advect_y": ([_vp, C.c_double], C.c_int),
     "impec_intercell_flux": ([_vp, C.c_int, C.c_double, _dp], C.c_int),
+    "impec_cnn_config_default": ([_vp], C.c_int),
+    "impec_cnn_create": ([_vp, C.POINTER(C.c_void_p)], C.c_int),
+    "impec_cnn_destroy": ([_vp], C.c_int),
+    "impec_cnn_last_error": ([_vp], C.c_char_p),
+    "impec_cnn_num_params": ([_vp, C.POINTER(C.c_longlong)], C.c_int),
+    "impec_cnn_get_params": ([_vp, _fp], C.c_int),
+    "impec_cnn_set_params": ([_vp, _fp], C.c_int),
+    "impec_cnn_get_scale": ([_vp, C.POINTER(C.c_float)], C.c_int),
+    "impec_cnn_set_scale": ([_vp, C.c_float], C.c_int),
+    "impec_cnn_forward": ([_vp, _fp, C.c_int, C.c_int, C.c_int, _fp], C.c_int),
+    "impec_cnn_gradients": ([_vp, _fp, _fp, C.c_int, C.c_int, C.c_int, _fp, C.POINTER(C.c_double)], C.c_int),
+    "impec_cnn_collect": ([_vp, _vp, C.c_int], C.c_int),
+    "impec_cnn_pairs": ([_vp, C.POINTER(C.c_int)], C.c_int),
+    "impec_cnn_train": ([_vp, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_double)], C.c_int),
+    "impec_set_warmstart": ([_vp, _vp], C.c_int),
     "impec_dispersion_cn": ([_vp, C.c_double, _ip], C.c_int),
     "impec_hyperbolic_solve": ([_vp, C.c_double], C.c_int),
     "impec_step": ([_vp, C.c_double, C.c_double, _dp], C.c_int),
@@ -112,6 +128,93 @@ def load():
             fn.restype = res
         _lib = lib
     return _lib
+
+
+class CnnConfig(C.Structure):
+    _fields_ = [("layers", C.c_int), ("channels", C.c_int), ("capacity", C.c_int), ("crop", C.c_int),
+                ("seed", C.c_ulonglong), ("reserved", C.c_int * 4)]
+
+
+class Network:
+    """The learned-initial-guess CNN (impec_cnn_* entry points)."""
+
+    def __init__(self, layers=8, capacity=4096, crop=64, seed=1234):
+        self.lib = load()
+        cfg = CnnConfig()
+        self.lib.impec_cnn_config_default(C.byref(cfg))
+        cfg.layers, cfg.capacity, cfg.crop, cfg.seed = layers, capacity, crop, seed
+        h = C.c_void_p()
+        st = self.lib.impec_cnn_create(C.byref(cfg), C.byref(h))
+        if st != IMPEC_OK:
+            raise ImpecError(st, "impec_cnn_create")
+        self.h = h
+        self.layers = layers
+        n = C.c_longlong(0)
+        self.lib.impec_cnn_num_params(self.h, C.byref(n))
+        self.nparam = n.value
+
+    def _check(self, st):
+        if st != IMPEC_OK:
+            raise ImpecError(st, self.lib.impec_cnn_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.impec_cnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.nparam, dtype=np.float32)
+        self._check(self.lib.impec_cnn_get_params(self.h, out.ctypes.data_as(_fp)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        self._check(self.lib.impec_cnn_set_params(self.h, p.ctypes.data_as(_fp)))
+
+    def scale(self):
+        v = C.c_float(0)
+        self._check(self.lib.impec_cnn_get_scale(self.h, C.byref(v)))
+        return v.value
+
+    def set_scale(self, v):
+        self._check(self.lib.impec_cnn_set_scale(self.h, float(v)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n, hh, ww, _ = x.shape
+        y = np.zeros((n, hh, ww), dtype=np.float32)
+        self._check(self.lib.impec_cnn_forward(self.h, x.ctypes.data_as(_fp), n, hh, ww, y.ctypes.data_as(_fp)))
+        return y
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        n, hh, ww, _ = x.shape
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = C.c_double(0)
+        self._check(self.lib.impec_cnn_gradients(self.h, x.ctypes.data_as(_fp), t.ctypes.data_as(_fp), n, hh, ww,
+                                                 g.ctypes.data_as(_fp), C.byref(loss)))
+        return g, loss.value
+
+    def collect(self, ctx, crops):
+        self._check(self.lib.impec_cnn_collect(self.h, ctx.h, int(crops)))
+
+    def pairs(self):
+        v = C.c_int(0)
+        self._check(self.lib.impec_cnn_pairs(self.h, C.byref(v)))
+        return v.value
+
+    def train(self, iters, batch=16, lr=1e-3):
+        loss = C.c_double(0)
+        self._check(self.lib.impec_cnn_train(self.h, int(iters), int(batch), float(lr), C.byref(loss)))
+        return loss.value
 
 
 class ImpecError(RuntimeError):
@@ -212,6 +315,11 @@ class Context:
         b = None if K22 is None else np.ascontiguousarray(K22, dtype=np.float64)
         self._check(self.lib.impec_set_permeability(self.h, None if a is None else _ptr(a),
                                                     None if b is None else _ptr(b)))
+
+    def set_warmstart(self, net):
+        """Attach the network used by initial_guess = GUESS_LEARNED (None detaches)."""
+        self._check(self.lib.impec_set_warmstart(self.h, None if net is None else net.h))
+        self._net = net
 
     def set_pin_kappa(self, kappa):
         self._check(self.lib.impec_set_pin_kappa(self.h, kappa))

@@ -149,6 +149,7 @@ int impec_create(const impec_params *P, impec_ctx **out) {
     if (P->precision != IMPEC_PREC_FP64 && P->precision != IMPEC_PREC_MIXED) return IMPEC_EINVAL;
     if (P->mg_rows < 0 || P->mg_rows > 4096) return IMPEC_EINVAL;
     if (P->cn_guess != 0 && P->cn_guess != 1) return IMPEC_EINVAL;
+    if (P->initial_guess < IMPEC_GUESS_ZERO || P->initial_guess > IMPEC_GUESS_LEARNED) return IMPEC_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IMPEC_ENODEV;
 
@@ -221,7 +222,7 @@ int impec_destroy(impec_ctx *ctx) {
     double *bufs[] = {ctx->c, ctx->p, ctx->u, ctx->v, ctx->q, ctx->K11, ctx->K22 != ctx->K11 ? ctx->K22 : nullptr,
                       ctx->w, ctx->w22, ctx->t1, ctx->t2, ctx->t3, ctx->scal, ctx->part, ctx->d11, ctx->d12,
                       ctx->d22, ctx->d21, ctx->cn_rhs, ctx->cn_x, ctx->cn_r, ctx->cn_r0, ctx->cn_p, ctx->cn_v,
-                      ctx->cn_s, ctx->cn_t, ctx->cn_y, ctx->cn_z, ctx->cn_dg, ctx->src_q, ctx->tr_md};
+                      ctx->cn_s, ctx->cn_t, ctx->cn_y, ctx->cn_z, ctx->cn_dg, ctx->src_q, ctx->tr_md, ctx->p_prev};
     for (double *b : bufs)
         if (b) hipFree(b);
     if (ctx->src_idx) hipFree(ctx->src_idx);

@@ -112,6 +112,10 @@ struct impec_ctx {
     impec_stats stats;
     char err[512];
 
+    // learned initial guess of the pressure solve
+    impec_cnn *net;            // not owned
+    double *p_prev;            // pressure before the latest solve (scalar layout)
+
     int prof_on;
     impec::ProfEntry prof[32];
     int nprof;
@@ -142,6 +146,7 @@ inline size_t sidx(const impec_ctx *ctx, int i, int j) {
 int launch_fill_scalar(impec_ctx *ctx, double *a, int periodic);
 int launch_fill_velocity(impec_ctx *ctx);
 int launch_mobility(impec_ctx *ctx);
+int cnn_warm_start(impec_ctx *ctx);
 int launch_velocity(impec_ctx *ctx);
 int launch_cfl(impec_ctx *ctx, double *dt);
 int launch_tr(impec_ctx *ctx, double dt);

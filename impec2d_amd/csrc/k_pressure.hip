@@ -1585,8 +1585,15 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     const double bb = ctx->bnorm * ctx->bnorm;
     const int maxit = ctx->P.max_iterations > 0 ? ctx->P.max_iterations : 10000;
 
-    if (ctx->P.initial_guess != IMPEC_GUESS_PREVIOUS || !ctx->have_prev_p)
-        hipLaunchKernelGGL(k_zero_interior, dim3(nb), dim3(256), 0, s, c, ctx->p);
+    const bool warm = (ctx->P.initial_guess == IMPEC_GUESS_PREVIOUS || ctx->P.initial_guess == IMPEC_GUESS_LEARNED) &&
+                      ctx->have_prev_p;
+    if (!warm) hipLaunchKernelGGL(k_zero_interior, dim3(nb), dim3(256), 0, s, c, ctx->p);
+    else if (ctx->P.initial_guess == IMPEC_GUESS_LEARNED && ctx->net) {
+        prof_begin(ctx);
+        st = cnn_warm_start(ctx);
+        prof_end(ctx, "cnn_guess");
+        if (st) return st;
+    }
     int its = 0, status = IMPEC_OK;
     double relres = 0., rr = 0.;
     double *dold = ctx->pd, *dnew = ctx->pz;
@@ -1640,6 +1647,11 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
 }
 
 int pressure_solve(impec_ctx *ctx, int *its, double *relres) {
+    // the pressure before this solve (training pairs and the learned guess)
+    if (ctx->p_prev && ctx->have_prev_p)
+        IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(ctx->p_prev, ctx->p, sizeof(double) * (size_t)(ctx->N1 + 2 * ctx->B) *
+                                                                     ctx->lds,
+                                            hipMemcpyDeviceToDevice, ctx->stream));
     if (ctx->P.nu != 1.) return solve_cg<true, false>(ctx, its, relres);
     const bool fused_ok = ctx->P.mg_pre <= 1 && ctx->P.mg_post <= 1 &&
                           fused_args<double>(ctx, 0).nwave <= RED_BLOCKS && !ctx->force_unfused;

@@ -52,6 +52,7 @@ extern "C" {
 #define IMPEC_PREC_MIXED 1   /* FP32 V-cycles inside FP64 defect correction  */
 #define IMPEC_GUESS_ZERO 0   /* reference semantics (MGSolver.cpp:573-590)   */
 #define IMPEC_GUESS_PREVIOUS 1
+#define IMPEC_GUESS_LEARNED 2   /* previous p plus the network's increment   */
 
 /* Parameter block.  Field meanings follow SimParams (SimParams.h:29-164);
  * fields marked "ext" extend the reference. */
@@ -68,7 +69,7 @@ typedef struct impec_params {
     int alpha_absolute;        /* ext: alpha_l/t are absolute dispersivities   */
     int max_iterations;        /* linear solver cap (reference: 10000)         */
     int precision;             /* ext: IMPEC_PREC_*                            */
-    int initial_guess;         /* ext: IMPEC_GUESS_*                           */
+    int initial_guess;         /* ext: IMPEC_GUESS_* of the pressure solve     */
     int mg_pre, mg_post;       /* smoothing sweeps per V-cycle leg             */
     int mg_variant;            /* 0: streaming fused V-cycle where available,  */
                                /* 1: one kernel per smoothing/transfer pass    */
@@ -203,6 +204,48 @@ int impec_profile_reset(impec_ctx *ctx);
  * reps times on a private stream; *ms gets the mean time of one copy.  The
  * copy kernel is the reference point that scales HBM byte counters. */
 int impec_copy_calibration(long long nbytes, int reps, double *ms);
+
+/* ---- learned initial guess of the pressure solve ------------------------- */
+/* No counterpart in the reference (its SetRHS_Elliptic always starts from
+ * zero, MGSolver.cpp:573-590): a CNN of 3x3 convolutions maps (log mobility,
+ * previous pressure, source) to the pressure increment; with
+ * initial_guess = IMPEC_GUESS_LEARNED the MG-CG solve starts from
+ * p_prev + increment and still runs to the same tolerance.  The network is
+ * trained on the GPU from pairs the solver generates. */
+typedef struct impec_cnn impec_cnn;
+typedef struct impec_cnn_config {
+    int layers;                /* convolutions: 3->32, (layers-2) x 32->32, 32->1 */
+    int channels;              /* hidden channels (32)                         */
+    int capacity;              /* training pairs kept in device memory         */
+    int crop;                  /* edge of a training crop (multiple of 32)     */
+    unsigned long long seed;   /* weight initialisation and sampling           */
+    int reserved[4];
+} impec_cnn_config;
+int impec_cnn_config_default(impec_cnn_config *c);
+int impec_cnn_create(const impec_cnn_config *c, impec_cnn **out);
+int impec_cnn_destroy(impec_cnn *net);
+const char *impec_cnn_last_error(const impec_cnn *net);
+/* parameters: per layer W[tap][ci][co] (tap = 3*(dy+1) + (dx+1)) then b[co] */
+int impec_cnn_num_params(const impec_cnn *net, long long *count);
+int impec_cnn_get_params(impec_cnn *net, float *params);
+int impec_cnn_set_params(impec_cnn *net, const float *params);
+/* output scale: increment = output * std(p_prev) * scale */
+int impec_cnn_get_scale(impec_cnn *net, float *scale);
+int impec_cnn_set_scale(impec_cnn *net, float scale);
+/* host batches, NHWC: x (n,h,w,3) -> y (n,h,w); gradients of the mean squared
+ * error against t (n,h,w), in the parameter layout, and the loss */
+int impec_cnn_forward(impec_cnn *net, const float *x, int n, int h, int w, float *y);
+int impec_cnn_gradients(impec_cnn *net, const float *x, const float *t, int n, int h, int w, float *grads,
+                        double *loss);
+/* training pairs from a context's latest pressure solve (features of the
+ * previous pressure and the mobility of that solve; target: the increment),
+ * `crops` random crops; then `iters` Adam steps on batches of `batch` pairs */
+int impec_cnn_collect(impec_cnn *net, impec_ctx *ctx, int crops);
+int impec_cnn_pairs(const impec_cnn *net, int *count);
+int impec_cnn_train(impec_cnn *net, int iters, int batch, float lr, double *loss);
+/* attach (or detach with NULL) the network used by IMPEC_GUESS_LEARNED; the
+ * context keeps the previous pressure from then on */
+int impec_set_warmstart(impec_ctx *ctx, impec_cnn *net);
 
 #ifdef __cplusplus
 }
