@@ -46,7 +46,8 @@ class Stats(C.Structure):
     _fields_ = [("last_its", C.c_int), ("last_vcycles", C.c_int), ("last_parabolic_its", C.c_int),
                 ("total_its", C.c_int), ("calls", C.c_int), ("reserved_i", C.c_int * 3),
                 ("last_relres", C.c_double), ("last_parabolic_relres", C.c_double),
-                ("total_mass_outflux", C.c_double), ("reserved_d", C.c_double * 5)]
+                ("total_mass_outflux", C.c_double), ("last_guess_alpha", C.c_double),
+                ("reserved_d", C.c_double * 4)]
 
 
 # every entry point declared in include/impec_hip.h: name -> (argtypes, restype)
@@ -132,17 +133,18 @@ def load():
 
 class CnnConfig(C.Structure):
     _fields_ = [("layers", C.c_int), ("channels", C.c_int), ("capacity", C.c_int), ("crop", C.c_int),
-                ("seed", C.c_ulonglong), ("reserved", C.c_int * 4)]
+                ("seed", C.c_ulonglong), ("mse_loss", C.c_int), ("reserved", C.c_int * 3)]
 
 
 class Network:
     """The learned-initial-guess CNN (impec_cnn_* entry points)."""
 
-    def __init__(self, layers=8, capacity=4096, crop=64, seed=1234):
+    def __init__(self, layers=8, capacity=4096, crop=64, seed=1234, mse_loss=False):
         self.lib = load()
         cfg = CnnConfig()
         self.lib.impec_cnn_config_default(C.byref(cfg))
         cfg.layers, cfg.capacity, cfg.crop, cfg.seed = layers, capacity, crop, seed
+        cfg.mse_loss = 1 if mse_loss else 0
         h = C.c_void_p()
         st = self.lib.impec_cnn_create(C.byref(cfg), C.byref(h))
         if st != IMPEC_OK:

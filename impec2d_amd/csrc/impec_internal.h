@@ -26,6 +26,7 @@ enum Scal {
     S_MASS = 16, S_DIV = 17, S_NOFLOW = 18,          // written together by the diagnostics pass
     S_MAXVEL = 20, S_MINDT = 21, S_MINDT_DIFF = 22,  // CFL pass
     S_OUTFLUX = 24, S_TMO = 25, S_CMAX = 26, S_CMIN = 27,
+    S_GUESS_ALPHA = 28,                              // step length of the learned increment
     S_COUNT = 32
 };
 
@@ -146,7 +147,7 @@ inline size_t sidx(const impec_ctx *ctx, int i, int j) {
 int launch_fill_scalar(impec_ctx *ctx, double *a, int periodic);
 int launch_fill_velocity(impec_ctx *ctx);
 int launch_mobility(impec_ctx *ctx);
-int cnn_warm_start(impec_ctx *ctx);
+int cnn_increment(impec_ctx *ctx, double *d, int ldd);
 int launch_velocity(impec_ctx *ctx);
 int launch_cfl(impec_ctx *ctx, double *dt);
 int launch_tr(impec_ctx *ctx, double dt);

@@ -13,6 +13,7 @@
 //   off(tap) = (tap / 3 - 1, tap % 3 - 1), zero outside the image.
 #include "impec_device.h"
 #include <math.h>
+#include <algorithm>
 #include <random>
 #include <vector>
 
@@ -37,6 +38,7 @@ struct impec_cnn {
     float *pairs = nullptr;     // training pairs: capacity x crop x crop x 4 (3 features + target)
     int count = 0, next = 0;
     float tscale = 0.f;         // target scale (set by the first collected field)
+    int loss_kind = 1;          // 0: mean squared error, 1: H1 seminorm + small L2 (see k_loss)
     // work buffers, grown on demand
     long long act_pix = 0;      // pixels per activation buffer
     int act_nbuf = 0;           // activation buffers: L-1 when training, 2 for inference
@@ -192,14 +194,34 @@ __global__ void __launch_bounds__(256) k_conv_last(const __bf16 *__restrict__ x,
 // ---------------------------------------------------------------------------
 // loss and backward
 // ---------------------------------------------------------------------------
-// dy = 2 (y - t) / npix; per-block partials of sum (y - t)^2
-__global__ void k_loss(const float *__restrict__ y, const float *__restrict__ t, float *__restrict__ dy, long long npix,
-                       double *__restrict__ part) {
+// Losses over e = y - t, normalised by the pixel count:
+//   MSE (kind 0):  sum e^2
+//   H1  (kind 1):  sum over the image's inner faces (e_p - e_q)^2 + H1_L2 sum e^2
+//                  (the gradient mismatch, which is what an initial guess of an
+//                  elliptic solve is judged by, plus a small L2 term that
+//                  fixes the constant)
+// dy = dLoss/dy; per-block partials of the loss
+constexpr float H1_L2 = 0.01f;
+__global__ void k_loss(const float *__restrict__ y, const float *__restrict__ t, float *__restrict__ dy, int N, int H,
+                       int Wd, int kind, double *__restrict__ part) {
+    const long long npix = (long long)N * H * Wd;
     double s[1] = {0.};
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += (long long)gridDim.x * blockDim.x) {
-        const float d = y[p] - t[p];
-        dy[p] = 2.f * d / (float)npix;
-        s[0] += (double)d * d;
+        const float e = y[p] - t[p];
+        if (kind == 0) {
+            dy[p] = 2.f * e / (float)npix;
+            s[0] += (double)e * e;
+            continue;
+        }
+        const int rem = (int)(p % ((long long)H * Wd)), r = rem / Wd, c = rem % Wd;
+        float g = 2.f * H1_L2 * e;
+        double en = (double)H1_L2 * e * e;
+        if (c + 1 < Wd) { const float d = e - (y[p + 1] - t[p + 1]); g += 2.f * d; en += (double)d * d; }
+        if (c > 0) { const float d = e - (y[p - 1] - t[p - 1]); g += 2.f * d; }
+        if (r + 1 < H) { const float d = e - (y[p + Wd] - t[p + Wd]); g += 2.f * d; en += (double)d * d; }
+        if (r > 0) { const float d = e - (y[p - Wd] - t[p - Wd]); g += 2.f * d; }
+        dy[p] = g / (float)npix;
+        s[0] += en;
     }
     block_reduce<RED_SUM, 1>(s);
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
@@ -370,6 +392,17 @@ __global__ void k_features(FeatArgs f, const double *__restrict__ stat, float *_
     o[0] = v[0]; o[1] = v[1]; o[2] = v[2];
 }
 
+// front score for importance-sampled crops: |grad log(lambda)|^2 per cell
+__global__ void k_front_score(FeatArgs f, float *__restrict__ score) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= f.N1 || j >= f.N2) return;
+    const double l = log_mob(f, i, j);
+    double s = 0.;
+    if (i + 1 < f.N1) { const double d = log_mob(f, i + 1, j) - l; s += d * d; }
+    if (j + 1 < f.N2) { const double d = log_mob(f, i, j + 1) - l; s += d * d; }
+    score[(long long)i * f.N2 + j] = (float)s;
+}
+
 // training pairs: crops of (features of p_prev, (p_new - p_prev) / (std p_prev * tscale))
 __global__ void k_collect(FeatArgs f, const double *__restrict__ stat, const double *__restrict__ pnew, int ldp,
                           const int *__restrict__ org, int ncrop, int crop, float tscale, float *__restrict__ pairs,
@@ -402,12 +435,12 @@ __global__ void k_gather(const float *__restrict__ pairs, const int *__restrict_
     t[k] = src[3];
 }
 
-// p += scale * y on the interior (the learned increment)
-__global__ void k_apply_guess(double *__restrict__ p, int ldp, const float *__restrict__ y, int N1, int N2,
-                              const double *__restrict__ stat, double tscale) {
+// d = scale * y on the interior: the learned increment in pressure units
+__global__ void k_increment(double *__restrict__ d, int ldd, const float *__restrict__ y, int N1, int N2,
+                            const double *__restrict__ stat, double tscale) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= N1 || j >= N2) return;
-    p[(long long)i * ldp + j] += (double)y[(long long)i * N2 + j] * stat[3] * tscale;
+    d[(long long)i * ldd + j] = (double)y[(long long)i * N2 + j] * stat[3] * tscale;
 }
 
 // ---------------------------------------------------------------------------
@@ -503,7 +536,7 @@ static int backward(impec_cnn *n, int N, int H, int Wd, double *loss) {
     const dim3 gp((unsigned)((pix + 255) / 256)), b256(256);
     int nb = (int)((pix + 255) / 256);
     if (nb > 1024) nb = 1024;
-    hipLaunchKernelGGL(k_loss, dim3(nb), b256, 0, s, n->y, n->t, n->dy, pix, n->stat_part);
+    hipLaunchKernelGGL(k_loss, dim3(nb), b256, 0, s, n->y, n->t, n->dy, N, H, Wd, n->loss_kind, n->stat_part);
     const long long wb = wgrad_blocks(N, H, Wd);
     int st = ensure_part(n, wb * (TAPS * CH * CH + CH));
     if (st) return st;
@@ -569,8 +602,9 @@ static int field_stats(impec_cnn *n, const FeatArgs &f, hipStream_t s) {
     return IMPEC_OK;
 }
 
-// the learned initial guess: p += network(features of the current mobility and p)
-int cnn_warm_start(impec_ctx *ctx) {
+// the network's pressure increment for the current mobility and p, into d
+// (cell (0,0) at d, pitch ldd)
+int cnn_increment(impec_ctx *ctx, double *d, int ldd) {
     impec_cnn *n = ctx->net;
     if (!n) return IMPEC_OK;
     const int N1 = ctx->N1, N2 = ctx->N2;
@@ -581,8 +615,7 @@ int cnn_warm_start(impec_ctx *ctx) {
     dim3 blk(64, 4), grd((N2 + 63) / 64, (N1 + 3) / 4);
     hipLaunchKernelGGL(k_features, grd, blk, 0, ctx->stream, f, n->stat, n->x0);
     if ((st = forward(n, 1, N1, N2, false, ctx->stream))) return set_error(ctx, st, "learned guess: %s", n->err);
-    hipLaunchKernelGGL(k_apply_guess, grd, blk, 0, ctx->stream, ctx->p + (long long)ctx->B * ctx->lds + ctx->B,
-                       ctx->lds, n->y, N1, N2, n->stat, (double)n->tscale);
+    hipLaunchKernelGGL(k_increment, grd, blk, 0, ctx->stream, d, ldd, n->y, N1, N2, n->stat, (double)n->tscale);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
 }
@@ -617,6 +650,7 @@ int impec_cnn_create(const impec_cnn_config *c, impec_cnn **out) {
     n->crop = c->crop;
     n->capacity = c->capacity;
     n->seed = c->seed;
+    n->loss_kind = c->mse_loss == 1 ? 0 : 1;
     long long off = 0;
     for (int l = 0; l < n->L; l++) {
         n->woff.push_back(off);
@@ -758,11 +792,42 @@ int impec_cnn_collect(impec_cnn *n, impec_ctx *ctx, int crops) {
         const double rms = sqrt(ss / ((double)ctx->N1 * ctx->N2));
         n->tscale = rms > 0 ? (float)(rms / stat[3]) : 1.f;
     }
+    // crop origins: half uniform, half centred on cells drawn with probability
+    // proportional to |grad log(lambda)|^2 (the displacement front, where the
+    // pressure increment concentrates)
+    const long long cells = (long long)ctx->N1 * ctx->N2;
+    std::vector<float> score(cells);
+    {
+        float *dscore = nullptr;
+        CNN_CHECK(n, hipMalloc(&dscore, sizeof(float) * cells));
+        dim3 b2(64, 4), g2((ctx->N2 + 63) / 64, (ctx->N1 + 3) / 4);
+        hipLaunchKernelGGL(k_front_score, g2, b2, 0, s, f, dscore);
+        hipError_t e = hipMemcpyAsync(score.data(), dscore, sizeof(float) * cells, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        hipFree(dscore);
+        CNN_CHECK(n, e);
+    }
+    std::vector<double> cdf(cells);
+    double acc = 0.;
+    for (long long k = 0; k < cells; k++) cdf[k] = (acc += score[k]);
     std::vector<int> org(2 * crops);
     std::mt19937_64 rng(n->seed + 7919ull * (unsigned long long)(n->next + 1));
+    std::uniform_real_distribution<double> uni(0., 1.);
     for (int k = 0; k < crops; k++) {
-        org[2 * k] = (int)(rng() % (unsigned long long)(ctx->N1 - n->crop + 1));
-        org[2 * k + 1] = (int)(rng() % (unsigned long long)(ctx->N2 - n->crop + 1));
+        int oi, oj;
+        if ((k & 1) && acc > 0.) {
+            const long long cell = std::lower_bound(cdf.begin(), cdf.end(), uni(rng) * acc) - cdf.begin();
+            const int ci = (int)(cell / ctx->N2), cj = (int)(cell % ctx->N2);
+            oi = ci - n->crop / 2;
+            oj = cj - n->crop / 2;
+            oi = oi < 0 ? 0 : (oi > ctx->N1 - n->crop ? ctx->N1 - n->crop : oi);
+            oj = oj < 0 ? 0 : (oj > ctx->N2 - n->crop ? ctx->N2 - n->crop : oj);
+        } else {
+            oi = (int)(rng() % (unsigned long long)(ctx->N1 - n->crop + 1));
+            oj = (int)(rng() % (unsigned long long)(ctx->N2 - n->crop + 1));
+        }
+        org[2 * k] = oi;
+        org[2 * k + 1] = oj;
     }
     int *dorg = nullptr;
     CNN_CHECK(n, hipMalloc(&dorg, sizeof(int) * 2 * crops));

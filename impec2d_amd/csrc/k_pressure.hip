@@ -434,6 +434,48 @@ __global__ void k_cg_init(LevArgs L, CgArgs c, double *__restrict__ p, const dou
     }
 }
 
+// Learned initial guess: the network's increment d is used as a search
+// direction from the previous pressure, p = p_prev + alpha d with
+// alpha = d.r0 / d.A_N d, r0 = b~ - A_N p_prev (b~ the consistent right-hand
+// side), the step that minimises the A_N-norm of the error along d -- so the
+// guess is never worse than p_prev in that norm.  Partials (d.r0, d.A_N d).
+__global__ void k_guess_dots(LevArgs L, CgArgs c, const double *__restrict__ p, const double *__restrict__ q,
+                             const double *__restrict__ d, double *__restrict__ part) {
+    double s[2] = {0., 0.};
+    const double *x = p + (long)c.B * c.ldp + c.B;
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        double a[9];
+        op_row5(L, i, j, a);
+        double r0 = -q[id] - row_dot<false>(a, x, c.ldp, i, j, c.n1, c.n2);
+        if (i == c.pin_i && j == c.pin_j) r0 -= c.bsum;
+        const double dc = d[(long)i * c.ld + j];
+        s[0] += dc * r0;
+        s[1] += dc * row_dot<false>(a, d, c.ld, i, j, c.n1, c.n2);
+    }
+    block_reduce<RED_SUM, 2>(s);
+    if (tid_linear() == 0) {
+        part[blockIdx.x] = s[0];
+        part[RED_BLOCKS + blockIdx.x] = s[1];
+    }
+}
+
+// p += alpha d (alpha = 1 without the projection)
+__global__ void k_guess_apply(CgArgs c, double *__restrict__ p, const double *__restrict__ d, double *__restrict__ S,
+                              int project) {
+    double alpha = 1.;
+    // A_N is negative definite (C < 0) on mean-free vectors: the ratio is the
+    // minimiser of the (-A_N)-norm of the error either way
+    if (project) alpha = S[S_TT] != 0. ? S[S_TS] / S[S_TT] : 0.;
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[S_GUESS_ALPHA] = alpha;
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        p[(long)(i + c.B) * c.ldp + (j + c.B)] += alpha * d[(long)i * c.ld + j];
+    }
+}
+
 // mg.b = r - mean(r) (the mean comes from S_RSUM)
 __global__ void k_cg_proj_in(CgArgs c, const double *__restrict__ r, double *__restrict__ b0,
                              const double *__restrict__ S) {
@@ -1590,9 +1632,17 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     if (!warm) hipLaunchKernelGGL(k_zero_interior, dim3(nb), dim3(256), 0, s, c, ctx->p);
     else if (ctx->P.initial_guess == IMPEC_GUESS_LEARNED && ctx->net) {
         prof_begin(ctx);
-        st = cnn_warm_start(ctx);
-        prof_end(ctx, "cnn_guess");
+        double *d = ctx->pd + L0.ld + 1;   // free until the CG starts
+        st = cnn_increment(ctx, d, L0.ld);
         if (st) return st;
+        const bool project = !DIAG && ctx->symmetric;
+        if (project) {
+            hipLaunchKernelGGL(k_guess_dots, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q, d, ctx->part);
+            if ((st = finish_sum(ctx, nb, 2, S_TS))) return st;
+        }
+        hipLaunchKernelGGL(k_guess_apply, dim3(nb), dim3(256), 0, s, c, ctx->p, d, ctx->scal, project ? 1 : 0);
+        IMPEC_HIP_CHECK(ctx, hipGetLastError());
+        prof_end(ctx, "cnn_guess");
     }
     int its = 0, status = IMPEC_OK;
     double relres = 0., rr = 0.;
@@ -1632,6 +1682,8 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
         relres = ctx->bnorm > 0 ? sqrt(ctx->h_scal[S_TT]) / ctx->bnorm : 0.;
         if (status != IMPEC_OK || relres <= tol) break;
     }
+    ctx->stats.last_guess_alpha =
+        (warm && ctx->P.initial_guess == IMPEC_GUESS_LEARNED && ctx->net) ? ctx->h_scal[S_GUESS_ALPHA] : 0.;
     ctx->have_prev_p = 1;
     ctx->stats.last_its = its;
     ctx->stats.last_vcycles = its;

@@ -101,7 +101,9 @@ typedef struct impec_stats {
     double last_relres;        /* true ||b - A p|| / ||b|| of the pinned system */
     double last_parabolic_relres;
     double total_mass_outflux; /* HyperbolicSolver::GetTotalMassOutflux        */
-    double reserved_d[5];
+    double last_guess_alpha;   /* ext: step length given to the learned        */
+                               /* increment in the last solve (0: not used)    */
+    double reserved_d[4];
 } impec_stats;
 
 typedef struct impec_ctx impec_ctx;
@@ -219,7 +221,10 @@ typedef struct impec_cnn_config {
     int capacity;              /* training pairs kept in device memory         */
     int crop;                  /* edge of a training crop (multiple of 32)     */
     unsigned long long seed;   /* weight initialisation and sampling           */
-    int reserved[4];
+    int mse_loss;              /* 1: train on the mean squared error of the    */
+                               /* increment; 0 (default): on its gradient      */
+                               /* mismatch (H1 seminorm) + 0.01 x L2           */
+    int reserved[3];
 } impec_cnn_config;
 int impec_cnn_config_default(impec_cnn_config *c);
 int impec_cnn_create(const impec_cnn_config *c, impec_cnn **out);

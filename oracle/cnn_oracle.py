@@ -9,7 +9,8 @@ GPU gradients can be checked:
   layer 0:        a0 = relu(conv3x3(x, W0) + b0)            3 -> 32
   layers 1..L-2:  a_l = relu(conv3x3(a_{l-1}, W_l) + b_l)    32 -> 32
   layer L-1:      y = conv3x3(a_{L-2}, W_{L-1}) + b_{L-1}    32 -> 1
-  loss = mean((y - t)^2)
+  loss = mean((y - t)^2) ("mse"), or the gradient mismatch of y - t
+         (squared jumps across inner faces) + 0.01 x its L2 norm ("h1")
 
 conv3x3: y[p][co] = sum_{tap, ci} W[tap][ci][co] x[p + off(tap)][ci],
 off(tap) = (tap // 3 - 1, tap % 3 - 1), zero padding; NHWC tensors.
@@ -113,15 +114,35 @@ def forward(params, x, L, bf16_mode=True):
     return y, acts
 
 
-def gradients(params, x, t, L, bf16_mode=True):
-    """Gradient of mean((y - t)^2) in the parameter layout, and the loss."""
+H1_L2 = 0.01
+
+
+def loss_and_dy(y, t, kind):
+    """kind "mse": mean (y - t)^2; kind "h1": (sum over inner faces of the
+    squared jump of e = y - t, plus H1_L2 sum e^2) / npix.  Returns the loss
+    and dLoss/dy."""
+    e = y - t
+    npix = e.size
+    if kind == "mse":
+        return float(np.mean(e * e)), 2.0 * e / npix
+    jx = e[:, :, 1:] - e[:, :, :-1]
+    jy = e[:, 1:, :] - e[:, :-1, :]
+    loss = (np.sum(jx * jx) + np.sum(jy * jy) + H1_L2 * np.sum(e * e)) / npix
+    g = 2.0 * H1_L2 * e
+    g[:, :, :-1] -= 2.0 * jx
+    g[:, :, 1:] += 2.0 * jx
+    g[:, :-1, :] -= 2.0 * jy
+    g[:, 1:, :] += 2.0 * jy
+    return float(loss), g / npix
+
+
+def gradients(params, x, t, L, bf16_mode=True, kind="h1"):
+    """Gradient of the loss (loss_and_dy) in the parameter layout, and the loss."""
     x = np.asarray(x, np.float64)
     Ws, bs = unpack(params, L)
     y, acts = forward(params, x, L, bf16_mode)
-    npix = y.size
-    d = y - t
-    loss = float(np.mean(d * d))
-    dy = (2.0 * d / npix)[..., None]
+    loss, g = loss_and_dy(y, t, kind)
+    dy = g[..., None]
     grads = [None] * L
     grads[L - 1] = (conv_back_weights(acts[L - 2], dy), dy.sum(axis=(0, 1, 2)))
     dz = conv_back_data(dy, Ws[L - 1]) * (acts[L - 2] > 0)
@@ -135,6 +156,6 @@ def gradients(params, x, t, L, bf16_mode=True):
     return flat, loss
 
 
-def loss_only(params, x, t, L):
+def loss_only(params, x, t, L, kind="h1"):
     y, _ = forward(params, x, L, bf16_mode=False)
-    return float(np.mean((y - t) ** 2))
+    return loss_and_dy(y, t, kind)[0]

@@ -29,20 +29,24 @@ def test_conv_against_direct_loop():
         assert np.allclose(y[n, i, j], s, rtol=1e-13, atol=1e-13)
 
 
-def test_backprop_against_finite_differences():
+import pytest
+
+
+@pytest.mark.parametrize("kind", ["mse", "h1"])
+def test_backprop_against_finite_differences(kind):
     L = 4
     rng = np.random.default_rng(1)
     _, _, npar = co.layout(L)
     params = rng.standard_normal(npar) * 0.3
     x = rng.standard_normal((2, 6, 5, 3))
     t = rng.standard_normal((2, 6, 5))
-    g, loss = co.gradients(params, x, t, L, bf16_mode=False)
-    assert abs(loss - co.loss_only(params, x, t, L)) < 1e-12
+    g, loss = co.gradients(params, x, t, L, bf16_mode=False, kind=kind)
+    assert abs(loss - co.loss_only(params, x, t, L, kind)) < 1e-12
     idx = rng.choice(npar, 40, replace=False)
     h = 1e-6
     for k in idx:
         pp, pm = params.copy(), params.copy()
         pp[k] += h
         pm[k] -= h
-        fd = (co.loss_only(pp, x, t, L) - co.loss_only(pm, x, t, L)) / (2 * h)
+        fd = (co.loss_only(pp, x, t, L, kind) - co.loss_only(pm, x, t, L, kind)) / (2 * h)
         assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(fd)), (k, fd, g[k])

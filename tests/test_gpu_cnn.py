@@ -19,11 +19,12 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("L,n,h,w", [(8, 2, 32, 64), (4, 1, 40, 33), (3, 3, 8, 32)])
-def test_forward_and_gradients_match_oracle(gpu_lib, L, n, h, w):
+@pytest.mark.parametrize("L,n,h,w,kind", [(8, 2, 32, 64, "h1"), (4, 1, 40, 33, "mse"), (3, 3, 8, 32, "h1"),
+                                          (8, 1, 32, 32, "mse")])
+def test_forward_and_gradients_match_oracle(gpu_lib, L, n, h, w, kind):
     from impec2d_amd import capi
-    _say("cnn", L, n, h, w)
-    net = capi.Network(layers=L, capacity=4, crop=32, seed=5)
+    _say("cnn", L, n, h, w, kind)
+    net = capi.Network(layers=L, capacity=4, crop=32, seed=5, mse_loss=(kind == "mse"))
     rng = np.random.default_rng(9)
     p = (rng.standard_normal(net.nparam) * 0.2).astype(np.float32)
     p = co.bf16(p)   # weights representable in BF16: the layers only differ by rounding of activations
@@ -34,7 +35,7 @@ def test_forward_and_gradients_match_oracle(gpu_lib, L, n, h, w):
     yo, _ = co.forward(p.astype(np.float64), x, L, bf16_mode=True)
     assert _rel(y, yo) < 1e-2, _rel(y, yo)
     g, loss = net.gradients(x, t)
-    go, losso = co.gradients(p.astype(np.float64), x, t, L, bf16_mode=True)
+    go, losso = co.gradients(p.astype(np.float64), x, t, L, bf16_mode=True, kind=kind)
     assert abs(loss - losso) <= 1e-2 * losso
     _, offs, _ = co.layout(L)
     for l, (wo, bo) in enumerate(offs):
@@ -74,12 +75,18 @@ def test_learned_guess_trains_and_keeps_tolerance(gpu_lib):
         g2.set_warmstart(net)
         g2.set_source(o2.source("one_cell"))
         g2.set_concentration(state)
-        tt, its = t, []
+        tt, its, alphas = t, [], []
         for _ in range(10):
             tt += g2.step(tt)
             st = g2.stats()
             assert st.last_relres <= 1e-10
             its.append(st.last_its)
+            alphas.append(round(st.last_guess_alpha, 3))
         res[guess] = its
+        if guess == capi.GUESS_LEARNED:
+            _say("step lengths of the learned increment", alphas)
         g2.close()
     _say("V-cycles previous-p", res[capi.GUESS_PREVIOUS], "learned", res[capi.GUESS_LEARNED])
+    # projected onto the previous pressure in the energy norm, the learned
+    # guess is never a worse starting point than p_prev
+    assert sum(res[capi.GUESS_LEARNED]) <= sum(res[capi.GUESS_PREVIOUS]) + 2
