@@ -51,6 +51,77 @@ def one_cell_source(n):
     return q
 
 
+def lognormal_params(capi, n, precision, guess):
+    """The mixed-precision workload: ln K Gaussian (sigma 1, squared-exponential
+    correlation length 0.05, seed 1), M = 20, no dispersion, otherwise as the
+    headline workload."""
+    p = workload_params(capi, n, precision, guess)
+    p.diffusion = 0
+    p.mu_inj = 1.0 / 20
+    return p
+
+
+def train_guess_network(capi, n=512, steps=120, crops=32, iters=3000):
+    """The learned initial guess, trained on pairs from a lognormal-K run at n^2
+    (the network is resolution-agnostic: features and target are normalised
+    per field and the increment's step length is fitted in every solve)."""
+    from impec2d_amd import fields
+    net = capi.Network(layers=8, capacity=8192, crop=64, seed=3)
+    ctx = capi.Context(lognormal_params(capi, n, "fp64", "previous"))
+    ctx.set_permeability(fields.lognormal_permeability(n, n, sigma=1.0, corr_len=0.05, seed=11), None)
+    ctx.set_source(one_cell_source(n))
+    ctx.set_warmstart(net)
+    t = 0.0
+    for k in range(steps):
+        t += ctx.step(t)
+        if k >= 1:
+            net.collect(ctx, crops)
+    ctx.close()
+    losses = [net.train(iters // 2, batch=32, lr=1e-3), net.train(iters - iters // 2, batch=32, lr=3e-4)]
+    return net, {"train_size": n, "pairs": net.pairs(), "iterations": iters, "final_loss": losses[-1]}
+
+
+def modes(capi, n, warmup=3, steps=8):
+    """Plain FP64 against mixed precision and the warm starts on the lognormal-K
+    workload at n^2: ms/step and V-cycles/step of each."""
+    from impec2d_amd import fields
+    K = fields.lognormal_permeability(n, n, sigma=1.0, corr_len=0.05, seed=1)
+    net, train_info = train_guess_network(capi)
+    res = {"workload": "lognormal K (sigma 1, corr. length 0.05), M=20, no dispersion, %d^2, rel-res 1e-10" % n,
+           "learned_training": train_info}
+    for name, prec, guess in [("fp64_zero", "fp64", "zero"), ("mixed_zero", "mixed", "zero"),
+                              ("mixed_previous", "mixed", "previous"), ("mixed_learned", "mixed", "learned")]:
+        p = lognormal_params(capi, n, prec, guess if guess != "learned" else "zero")
+        if guess == "learned":
+            p.initial_guess = capi.GUESS_LEARNED
+        ctx = capi.Context(p)
+        ctx.set_permeability(K, None)
+        ctx.set_source(one_cell_source(n))
+        if guess == "learned":
+            ctx.set_warmstart(net)
+        t = 0.0
+        for _ in range(warmup):
+            t += ctx.step(t)
+        ctx.synchronize()
+        s0 = ctx.stats()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t += ctx.step(t)
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        s1 = ctx.stats()
+        res[name] = {"ms_per_step": 1e3 * el / steps, "cell_updates_per_s": n * n * steps / el,
+                     "vcycles_per_step": (s1.total_its - s0.total_its) / steps,
+                     "last_relres": s1.last_relres}
+        if guess == "learned":
+            res[name]["last_step_length"] = s1.last_guess_alpha
+        ctx.close()
+    net.close()
+    res["speedup_mixed_learned_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_learned"]["ms_per_step"]
+    res["speedup_mixed_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_zero"]["ms_per_step"]
+    return res
+
+
 def cpu_baseline(n=1024, steps=8):
     """Times the C oracle (FP64 restatement of the reference algorithm,
     one thread) on `steps` steps of the same workload at n^2."""
@@ -83,6 +154,8 @@ def main():
     ap.add_argument("--mg-rows", type=int, default=0,
                     help="rows per wavefront chunk of the finest multigrid level (0: automatic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-modes", action="store_true",
+                    help="skip the plain / mixed / warm-start comparison on the lognormal-K workload")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (reference kernel for HBM byte counters)")
@@ -131,8 +204,11 @@ def main():
 
     if not args.no_roofline:
         from impec2d_amd import roofline
-        out["roofline"] = roofline.measure(ctx, n, t)
+        out["roofline"] = roofline.measure(ctx, n, t, fp32_cycle=args.precision == "mixed")
+        out["roofline_by_kernel"] = roofline.measure.last_table
     ctx.close()
+    if not args.no_modes:
+        out["modes"] = modes(capi, n)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

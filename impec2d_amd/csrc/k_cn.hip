@@ -556,13 +556,13 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         a.pin = pb[cur]; a.vin = vb[cur]; a.pout = pb[cur ^ 1]; a.out = vb[cur ^ 1];
         prof_begin(ctx);
         hipLaunchKernelGGL(k_cn_stream<0>, sg, sblk, 0, s, a);
-        prof_end(ctx, "cn_apply");
+        prof_end(ctx, "cn_apply_p");
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 1, ctx->scal);
         CnArgs a2 = a;
         a2.pin = nullptr; a2.vin = vb[cur ^ 1]; a2.pout = sv; a2.out = t;
         prof_begin(ctx);
         hipLaunchKernelGGL(k_cn_stream<1>, sg, sblk, 0, s, a2);
-        prof_end(ctx, "cn_apply");
+        prof_end(ctx, "cn_apply_s");
         hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 2, ctx->scal);
         prof_begin(ctx);
         hipLaunchKernelGGL(k_bicg_xr, vg, vblk, 0, s, N1, N2, x, pb[cur ^ 1], sv, ctx->cn_dg, r, t, r0, ctx->scal,

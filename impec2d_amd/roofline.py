@@ -12,18 +12,28 @@ import re
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 
 KERNELS = {
-    # y = A x for the Crank-Nicolson operator, coefficients rebuilt from four
-    # face-tensor arrays, fused with a BiCGStab dot product:
-    # read x, d11, d12, d22, d21, dot operand (6 x 8) + write y (8)
-    "cn_apply": dict(rocprof="k_cn_apply", bytes_per_cell=56),
-    # one colour of red-black Gauss-Seidel on the fine level: read w (8), the
-    # other colour's x (4), this colour's b (4), write this colour's x (4)
-    "mg_smooth_l0": dict(rocprof="k_smooth<false, true>", bytes_per_cell=20),
-    # x-sweep / y-sweep of MUSCL-Hancock: read c and one velocity component, write c
+    # BiCGStab step 1 (k_cn_stream<0>): p = r + beta (p - omega v), v = A (p / D),
+    # r0.v; read r, p, v, 1/D, r0 and the four face tensors (9 x 8), write p, v (2 x 8)
+    "cn_apply_p": dict(rocprof="k_cn_stream<0>", bytes_per_cell=88),
+    # BiCGStab step 2 (k_cn_stream<1>): s = r - alpha v, t = A (s / D), t.s, t.t;
+    # read r, v, 1/D and the four face tensors (7 x 8), write s, t (2 x 8)
+    "cn_apply_s": dict(rocprof="k_cn_stream<1>", bytes_per_cell=72),
+    # BiCGStab update (k_bicg_xr): read x, p, s, 1/D, t, r0 (6 x 8), write x, r (2 x 8)
+    "cn_vector": dict(rocprof="k_bicg_xr", bytes_per_cell=64),
+    # streaming V-cycle, finest level, down leg: read w, r (2 x 8), write x (8)
+    # and the coarse right-hand side (8 / 4); FP32 cycle: w 4, r 8, x 4, b_c 1
+    "mg_down_l0": dict(rocprof="k_mg_down<true, false", bytes_per_cell=34, bytes_per_cell_fp32=17),
+    # up leg: read x, w, r (3 x 8) and the coarse solution (8 / 4), write x (8);
+    # FP32 cycle: x 4, w 4, r 8, x_c 1, x 4
+    "mg_up_l0": dict(rocprof="k_mg_up<true, false", bytes_per_cell=34, bytes_per_cell_fp32=21),
+    # CG direction + matvec: read z, d_old, two face-mobility arrays (4 x 8),
+    # write d, q (2 x 8); the FP32 cycle's z is 4 B
+    "cg_dir_matvec": dict(rocprof="k_cg_dir_matvec", bytes_per_cell=48, bytes_per_cell_fp32=44),
+    # CG update: read p, d, r, q (4 x 8), write p, r (2 x 8)
+    "cg_update": dict(rocprof="k_cg_update2", bytes_per_cell=48),
+    # MUSCL-Hancock x / y sweep: read c and one velocity component, write c
     "sweep_x": dict(rocprof="k_sweep_x", bytes_per_cell=24),
     "sweep_y": dict(rocprof="k_sweep_y", bytes_per_cell=24),
-    # CG matrix-vector product with dot product: read d, w, write q
-    "cg_matvec": dict(rocprof="k_cg_matvec", bytes_per_cell=24),
 }
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -41,9 +51,10 @@ def committed_traffic(name, n):
     return float(m.group(1)) if m else None
 
 
-def measure(ctx, n, t, steps=2):
+def measure(ctx, n, t, steps=2, fp32_cycle=False):
     """Runs `steps` profiled steps on ctx and returns the roofline object of
-    the kernel class with the largest total time."""
+    the kernel class with the largest total time (fp32_cycle: the
+    mixed-precision V-cycle's byte counts)."""
     ctx.profile_reset()
     ctx.profile(True)
     for _ in range(steps):
@@ -51,18 +62,30 @@ def measure(ctx, n, t, steps=2):
     ctx.synchronize()
     ctx.profile(False)
     best = None
+    table = {}
     for name in KERNELS:
         launches, ms = ctx.profile_read(name)
-        if launches and (best is None or ms > best[2]):
+        if not launches:
+            continue
+        k = KERNELS[name]
+        per_cell = k.get("bytes_per_cell_fp32", k["bytes_per_cell"]) if fp32_cycle else k["bytes_per_cell"]
+        avg = ms * 1e-3 / launches
+        gbs = per_cell * float(n) * n / avg / 1e9
+        table[name] = {"launches": launches, "avg_us": round(avg * 1e6, 2), "GB/s": round(gbs, 1),
+                       "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_cell": per_cell}
+        if best is None or ms > best[2]:
             best = (name, launches, ms)
+    measure.last_table = table
     if best is None:
         return None
     name, launches, ms = best
     avg_s = ms * 1e-3 / launches
-    bytes_per_launch = KERNELS[name]["bytes_per_cell"] * float(n) * n
+    k = KERNELS[name]
+    per_cell = k.get("bytes_per_cell_fp32", k["bytes_per_cell"]) if fp32_cycle else k["bytes_per_cell"]
+    bytes_per_launch = per_cell * float(n) * n
     achieved = bytes_per_launch / avg_s / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": committed_traffic(name, n),
-            "kernel": name, "rocprof_name": KERNELS[name]["rocprof"],
+            "kernel": name, "rocprof_name": k["rocprof"],
             "avg_launch_us": avg_s * 1e6, "launches": launches,
             "algorithmic_bytes_per_launch": bytes_per_launch}
