@@ -237,8 +237,9 @@ int impec_cnn_set_params(impec_cnn *net, const float *params);
 /* output scale: increment = output * std(p_prev) * scale */
 int impec_cnn_get_scale(impec_cnn *net, float *scale);
 int impec_cnn_set_scale(impec_cnn *net, float scale);
-/* host batches, NHWC: x (n,h,w,3) -> y (n,h,w); gradients of the mean squared
- * error against t (n,h,w), in the parameter layout, and the loss */
+/* host batches, NHWC: x (n,h,w,3) -> y (n,h,w); gradients of the training
+ * loss (mse_loss of the config) against t (n,h,w), in the parameter layout,
+ * and the loss */
 int impec_cnn_forward(impec_cnn *net, const float *x, int n, int h, int w, float *y);
 int impec_cnn_gradients(impec_cnn *net, const float *x, const float *t, int n, int h, int w, float *grads,
                         double *loss);
