@@ -111,6 +111,13 @@ SIGNATURES = {
     "impec_profile_read": ([_vp, C.c_char_p, C.POINTER(C.c_longlong), _dp], C.c_int),
     "impec_profile_reset": ([_vp], C.c_int),
     "impec_copy_calibration": ([C.c_longlong, C.c_int, _dp], C.c_int),
+    "impec_ensemble_create": ([C.POINTER(Params), C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "impec_ensemble_destroy": ([_vp], C.c_int),
+    "impec_ensemble_last_error": ([_vp], C.c_char_p),
+    "impec_ensemble_member": ([_vp, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "impec_ensemble_set_graphs": ([_vp, C.c_int, C.c_int], C.c_int),
+    "impec_ensemble_autotune": ([_vp, C.c_int, _ip], C.c_int),
+    "impec_ensemble_step": ([_vp, _dp, _ip], C.c_int),
 }
 
 _lib = None
@@ -430,3 +437,73 @@ class Context:
         ms = C.c_double(0)
         self.lib.impec_profile_read(self.h, name.encode(), C.byref(n), C.byref(ms))
         return n.value, ms.value
+
+
+class _Member(Context):
+    """A member context lent by an ensemble (not destroyed by close())."""
+
+    def __init__(self, ens, handle, params):
+        self.lib = ens.lib
+        self.params = params
+        self.h = handle
+        self.N1, self.N2, self.b = params.N1, params.N2, params.num_ghostcells
+        self._ens = ens
+
+    def close(self):
+        self.h = None
+
+
+class Ensemble:
+    """Independent problems of different sizes advanced together
+    (impec_ensemble): one launch per pass over all members, graph replays per
+    step.  ``members[k]`` is member k's Context (source, permeability,
+    concentration, fields, diagnostics)."""
+
+    def __init__(self, params_list):
+        self.lib = load()
+        n = len(params_list)
+        arr = (Params * n)(*params_list)
+        h = C.c_void_p()
+        st = self.lib.impec_ensemble_create(arr, n, C.byref(h))
+        if st != IMPEC_OK:
+            raise ImpecError(st, "impec_ensemble_create failed")
+        self.h = h
+        self.n = n
+        self.members = []
+        for k in range(n):
+            mh = C.c_void_p()
+            self._check(self.lib.impec_ensemble_member(self.h, k, C.byref(mh)))
+            self.members.append(_Member(self, mh, params_list[k]))
+
+    def _check(self, st):
+        if st != IMPEC_OK:
+            raise ImpecError(st, self.lib.impec_ensemble_last_error(self.h).decode())
+        return st
+
+    def close(self):
+        if self.h:
+            for m in self.members:
+                m.close()
+            self.lib.impec_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_graphs(self, on, slots=0):
+        self._check(self.lib.impec_ensemble_set_graphs(self.h, int(bool(on)), int(slots)))
+
+    def autotune(self, reps=3):
+        rows = np.zeros(self.n, dtype=np.int32)
+        self._check(self.lib.impec_ensemble_autotune(self.h, int(reps), rows.ctypes.data_as(_ip)))
+        return rows
+
+    def step(self):
+        """One IMPEC step of every member; returns (dt, its) arrays."""
+        dt = np.zeros(self.n)
+        its = np.zeros(self.n, dtype=np.int32)
+        self._check(self.lib.impec_ensemble_step(self.h, dt.ctypes.data_as(_dp), its.ctypes.data_as(_ip)))
+        return dt, its

@@ -137,7 +137,12 @@ int impec_params_default(impec_params *p) {
     return IMPEC_OK;
 }
 
-int impec_create(const impec_params *P, impec_ctx **out) {
+int impec_create(const impec_params *P, impec_ctx **out) { return impec::ctx_create(P, nullptr, out); }
+
+}  // extern "C"
+
+// a context on its own stream, or on `ext` (not owned) for ensemble members
+int impec::ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out) {
     if (!P || !out) return IMPEC_EINVAL;
     *out = nullptr;
     if (P->N1 < 2 || P->N2 < 2) return IMPEC_EINVAL;
@@ -187,7 +192,9 @@ int impec_create(const impec_params *P, impec_ctx **out) {
     ctx->ldv = round_up(ctx->N2 + 1 + 2 * ctx->B, 16);
 
     int st = IMPEC_OK;
-    if (hipStreamCreate(&ctx->stream) != hipSuccess) st = set_error(ctx, IMPEC_EHIP, "stream");
+    if (ext) ctx->stream = ext;
+    else if (hipStreamCreate(&ctx->stream) != hipSuccess) st = set_error(ctx, IMPEC_EHIP, "stream");
+    else ctx->own_stream = 1;
     if (!st && (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess))
         st = set_error(ctx, IMPEC_EHIP, "events");
     const size_t ns = scalar_elems(ctx), nv = vel_elems(ctx), nq = (size_t)ctx->N1 * ctx->N2;
@@ -216,6 +223,8 @@ int impec_create(const impec_params *P, impec_ctx **out) {
     return IMPEC_OK;
 }
 
+extern "C" {
+
 int impec_destroy(impec_ctx *ctx) {
     if (!ctx) return IMPEC_OK;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
@@ -230,7 +239,7 @@ int impec_destroy(impec_ctx *ctx) {
     if (ctx->h_scal) hipHostFree(ctx->h_scal);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->stream && ctx->own_stream) hipStreamDestroy(ctx->stream);
     free(ctx);
     return IMPEC_OK;
 }

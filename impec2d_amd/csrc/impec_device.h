@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include "impec_internal.h"
+#include "impec_batch.h"
 
 namespace impec {
 

@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <string.h>
 #include "../../include/impec_hip.h"
+#include "impec_batch.h"
 
 namespace impec {
 
@@ -27,8 +28,20 @@ enum Scal {
     S_MAXVEL = 20, S_MINDT = 21, S_MINDT_DIFF = 22,  // CFL pass
     S_OUTFLUX = 24, S_TMO = 25, S_CMAX = 26, S_CMIN = 27,
     S_GUESS_ALPHA = 28,                              // step length of the learned increment
-    S_COUNT = 32
+    // per-member control of an ensemble's pressure solve (device-side loop state)
+    S_EST = 32,        // 0 done, 1 iterating, 2 residual recheck due
+    S_EITS = 33,       // iterations of this solve
+    S_ESINCE = 34,     // iterations since the last (re)start of the recursion
+    S_ERST = 35,       // restarts
+    S_EPAR = 36,       // which of the two direction arrays is the current one
+    S_ETOL = 37,       // tol^2 ||b||^2
+    S_EMAXIT = 38,     // iteration limit
+    S_ESTATUS = 39,    // 0 ok, 1 limit reached, 2 non-finite residual
+    S_DT = 40,         // time step of the ensemble step
+    S_ZEROP = 41,      // start the next solve from zero
+    S_COUNT = 48
 };
+enum { EST_DONE = 0, EST_ITER = 1, EST_CHECK = 2 };
 
 // Geometry and constants of the nu-weighted pressure operator on one level.
 // Constant factors are formed on the host with the reference's expressions
@@ -121,11 +134,43 @@ struct impec_ctx {
     impec::ProfEntry prof[32];
     int nprof;
     hipEvent_t ev0, ev1;
+    int own_stream;            // 0 when the stream belongs to an ensemble
+};
+
+namespace impec {
+struct EnsFieldTabs;           // batched-launch tables of k_field.hip
+struct EnsPressTabs;           // of k_pressure.hip
+struct EnsTransTabs;           // of k_transport.hip
+struct EnsSnap;                // member inputs the tables were built for
+}
+
+// An ensemble: member contexts on one stream, advanced together by batched
+// launches (one launch per pass over all members) with the pressure solve's
+// loop state kept per member on the device.
+struct impec_ensemble {
+    int n;
+    impec_ctx **m;             // member contexts (owned), sharing `stream`
+    hipStream_t stream;
+    int ready;                 // tables built for the current member state
+    int use_graphs;            // replay captured graphs (else launch eagerly)
+    int slots;                 // CG iterations per replay of the loop graph
+    int have_graphs;
+    hipGraphExec_t g_pre, g_loop, g_end, g_post;
+    impec::EnsFieldTabs *field;
+    impec::EnsPressTabs *pres;
+    impec::EnsTransTabs *trans;
+    double **d_S;              // device array of the members' scalar arrays
+    double *d_out, *h_out;     // per-member gathered scalars (device / pinned host)
+    int *d_active, *h_active;  // members still iterating
+    impec::EnsSnap *snap;      // per member
+    long long max_loop;        // replays of the loop graph before giving up
+    char err[512];
 };
 
 namespace impec {
 
 int set_error(impec_ctx *ctx, int code, const char *fmt, ...);
+int ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out);
 
 #define IMPEC_HIP_CHECK(ctx, call)                                                            \
     do {                                                                                      \
@@ -174,5 +219,31 @@ int cn_operator_dump(impec_ctx *ctx, double dt, double *A9_host, double *rhs_hos
 // ---- reductions (k_field.hip) ----
 // sum of NV per-block partial vectors into scal[dst .. dst+NV)
 int finish_sum(impec_ctx *ctx, int nblocks, int nv, int dst);
+
+// ---- ensembles: batched launches (k_field.hip, k_pressure.hip, k_transport.hip) ----
+int ens_set_error(impec_ensemble *e, int code, const char *fmt, ...);
+int red_blocks_of(long long n);   // workgroups of a grid-stride reduction over n cells
+// one reduction finish per member (workgroup m sums member m's partials)
+struct FinishRec {
+    const double *part;
+    double *S;
+    int nblocks, nv, dst, op;
+};
+hipError_t ens_finish(hipStream_t s, const Batch<FinishRec> &b, int need);
+int ens_field_build(impec_ensemble *e);
+void ens_field_free(impec_ensemble *e);
+int ens_fill_c(impec_ensemble *e);
+int ens_mobility(impec_ensemble *e);
+int ens_velocity_dt(impec_ensemble *e);   // fill p, velocities, fill v, CFL time step into S_DT
+int ens_tr(impec_ensemble *e);            // trapezoidal source step over S_DT / 2
+int ens_press_build(impec_ensemble *e);
+void ens_press_free(impec_ensemble *e);
+int ens_solve_begin(impec_ensemble *e);   // hierarchy, faces, guess, first residual
+int ens_cg_slot(impec_ensemble *e);       // one CG iteration (members still iterating)
+int ens_solve_end(impec_ensemble *e);     // shift to the pinned value, true residual
+int ens_vcycle_time(impec_ensemble *e, int reps, float *ms);   // level-0..coarsest legs, all members
+int ens_trans_build(impec_ensemble *e);
+void ens_trans_free(impec_ensemble *e);
+int ens_advect_parallel(impec_ensemble *e);   // the four sweeps and the average
 
 }  // namespace impec

@@ -2,6 +2,8 @@
 // trapezoidal-rule well sources and diagnostics reductions (FP64).
 #include "impec_device.h"
 #include <stdio.h>
+#include <stdarg.h>
+#include <vector>
 
 namespace impec {
 
@@ -13,9 +15,10 @@ namespace impec {
 // every ghost cell of the ghosted scalar array copies the interior cell its
 // even mirror (or periodic image) maps to; corners compose both maps, which
 // is what the reference's corner loops produce.
-__global__ void k_fill_scalar(double *__restrict__ a, int N1, int N2, int B, int ld, int periodic) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x - B;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y - B;
+template <class G>
+__device__ __forceinline__ void fill_scalar_body(const G &gr_, double *__restrict__ a, int N1, int N2, int B, int ld, int periodic) {
+    const int j = gr_.bx() * blockDim.x + threadIdx.x - B;
+    const int i = gr_.by() * blockDim.y + threadIdx.y - B;
     if (i >= N1 + B || j >= N2 + B) return;
     const bool gi = i < 0 || i >= N1, gj = j < 0 || j >= N2;
     if (!gi && !gj) return;
@@ -29,15 +32,19 @@ __global__ void k_fill_scalar(double *__restrict__ a, int N1, int N2, int B, int
     }
     a[(size_t)(i + B) * ld + (j + B)] = a[(size_t)(si + B) * ld + (sj + B)];
 }
+__global__ void k_fill_scalar(double *__restrict__ a, int N1, int N2, int B, int ld, int periodic) {
+    fill_scalar_body(HwGrid{}, a, N1, N2, B, ld, periodic);
+}
 
 // FillGhostCellsNeumannVelocity (Util.cpp:331-438).  u (x-faces) is odd in x
 // about the faces i = 0 and i = N1 and even in y; v (y-faces) is odd in y about
 // j = 0 and j = N2 and even in x.  Entries the reference never writes
 // (u at j = N2+B, v at i = N1+B) are left untouched.
-__global__ void k_fill_velocity(double *__restrict__ u, double *__restrict__ v, int N1, int N2, int B,
+template <class G>
+__device__ __forceinline__ void fill_velocity_body(const G &gr_, double *__restrict__ u, double *__restrict__ v, int N1, int N2, int B,
                                 int ld) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x - B;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y - B;
+    const int j = gr_.bx() * blockDim.x + threadIdx.x - B;
+    const int i = gr_.by() * blockDim.y + threadIdx.y - B;
     if (i > N1 + B || j > N2 + B) return;
     const size_t o = (size_t)(i + B) * ld + (j + B);
     // u component
@@ -58,6 +65,10 @@ __global__ void k_fill_velocity(double *__restrict__ u, double *__restrict__ v, 
         else val = v[(size_t)(im + B) * ld + (j + B)];
         v[o] = val;
     }
+}
+__global__ void k_fill_velocity(double *__restrict__ u, double *__restrict__ v, int N1, int N2, int B,
+                                int ld) {
+    fill_velocity_body(HwGrid{}, u, v, N1, N2, B, ld);
 }
 
 int launch_fill_scalar(impec_ctx *ctx, double *a, int periodic) {
@@ -85,11 +96,12 @@ int launch_fill_velocity(impec_ctx *ctx) {
 // cell of the ghosted array.  Face and vertex mobilities are formed from w
 // with the reference's expressions wherever they are needed.
 // ---------------------------------------------------------------------------
-__global__ void k_mobility(const double *__restrict__ c, const double *__restrict__ K11,
+template <class G>
+__device__ __forceinline__ void mobility_body(const G &gr_, const double *__restrict__ c, const double *__restrict__ K11,
                            const double *__restrict__ K22, double *__restrict__ w, double *__restrict__ w22,
                            int rows, int cols, int ld, double mob1over4, double mu_res, int harmonic) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int j = gr_.bx() * blockDim.x + threadIdx.x;
+    const int i = gr_.by() * blockDim.y + threadIdx.y;
     if (i >= rows || j >= cols) return;
     const size_t o = (size_t)i * ld + j;
     const double mu = mu_of(c[o], mob1over4, mu_res);
@@ -99,6 +111,11 @@ __global__ void k_mobility(const double *__restrict__ c, const double *__restric
         const double k22 = K22[o];
         w22[o] = harmonic ? mu / k22 : k22 / mu;
     }
+}
+__global__ void k_mobility(const double *__restrict__ c, const double *__restrict__ K11,
+                           const double *__restrict__ K22, double *__restrict__ w, double *__restrict__ w22,
+                           int rows, int cols, int ld, double mob1over4, double mu_res, int harmonic) {
+    mobility_body(HwGrid{}, c, K11, K22, w, w22, rows, cols, ld, mob1over4, mu_res, harmonic);
 }
 
 int launch_mobility(impec_ctx *ctx) {
@@ -147,10 +164,11 @@ __device__ __forceinline__ void vertex_w(const VelConst &k, const double *__rest
 #undef W2
 }
 
-__global__ void k_velocity(VelConst k, const double *__restrict__ p, const double *__restrict__ w11,
+template <class G>
+__device__ __forceinline__ void velocity_body(const G &gr_, VelConst k, const double *__restrict__ p, const double *__restrict__ w11,
                            const double *__restrict__ w22, double *__restrict__ u, double *__restrict__ v) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int j = gr_.bx() * blockDim.x + threadIdx.x;
+    const int i = gr_.by() * blockDim.y + threadIdx.y;
     if (i > k.N1 || j > k.N2) return;
     const int ld = k.lds, B = k.B;
 #define P_(i, j) p[(size_t)((i) + B) * ld + ((j) + B)]
@@ -184,6 +202,10 @@ __global__ void k_velocity(VelConst k, const double *__restrict__ p, const doubl
 #undef W1
 #undef W2
 }
+__global__ void k_velocity(VelConst k, const double *__restrict__ p, const double *__restrict__ w11,
+                           const double *__restrict__ w22, double *__restrict__ u, double *__restrict__ v) {
+    velocity_body(HwGrid{}, k, p, w11, w22, u, v);
+}
 
 int launch_velocity(impec_ctx *ctx) {
     // CalcVelocities fills p's ghosts first (EllipticSolver.cpp:663)
@@ -212,8 +234,8 @@ int launch_velocity(impec_ctx *ctx) {
 // ---------------------------------------------------------------------------
 
 // second pass: scal[dst+k] = combine over nblocks partials of value k
-template <int OP>
-__global__ void k_finish(const double *__restrict__ part, int nblocks, int nv, double *__restrict__ scal,
+template <int OP, class G>
+__device__ __forceinline__ void finish_body(const G &gr_, const double *__restrict__ part, int nblocks, int nv, double *__restrict__ scal,
                          int dst) {
     for (int k = 0; k < nv; k++) {
         double v[1] = {strided_combine<OP>(part + k * RED_BLOCKS, nblocks)};
@@ -221,6 +243,11 @@ __global__ void k_finish(const double *__restrict__ part, int nblocks, int nv, d
         if (threadIdx.x == 0) scal[dst + k] = v[0];
         __syncthreads();
     }
+}
+template <int OP>
+__global__ void k_finish(const double *__restrict__ part, int nblocks, int nv, double *__restrict__ scal,
+                         int dst) {
+    finish_body<OP>(HwGrid{}, part, nblocks, nv, scal, dst);
 }
 
 template <int OP>
@@ -246,12 +273,13 @@ struct CflConst {
     double dx, dy, al, at;
 };
 
-__global__ void k_cfl(CflConst k, const double *__restrict__ u, const double *__restrict__ v,
+template <class G>
+__device__ __forceinline__ void cfl_body(const G &gr_, CflConst k, const double *__restrict__ u, const double *__restrict__ v,
                       double *__restrict__ part) {
     const long long n = (long long)k.N1 * k.N2;
     double mx[1] = {0.0}, mn[2] = {DBL_MAX, 1.e10};
-    for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n;
-         id += (long long)gridDim.x * blockDim.x) {
+    for (long long id = (long long)gr_.bx() * blockDim.x + threadIdx.x; id < n;
+         id += (long long)gr_.gx() * blockDim.x) {
         const int i = (int)(id / k.N2), j = (int)(id - (long long)i * k.N2);
         const size_t o = (size_t)(i + k.B) * k.ld + (j + k.B);
         const double u0 = u[o], u1 = u[o + k.ld], v0 = v[o], v1 = v[o + 1];
@@ -279,10 +307,14 @@ __global__ void k_cfl(CflConst k, const double *__restrict__ u, const double *__
     block_reduce<RED_MAX, 1>(mx);
     block_reduce<RED_MIN, 2>(mn);
     if (tid_linear() == 0) {
-        part[0 * RED_BLOCKS + blockIdx.x] = mx[0];
-        part[1 * RED_BLOCKS + blockIdx.x] = mn[0];
-        part[2 * RED_BLOCKS + blockIdx.x] = mn[1];
+        part[0 * RED_BLOCKS + gr_.bx()] = mx[0];
+        part[1 * RED_BLOCKS + gr_.bx()] = mn[0];
+        part[2 * RED_BLOCKS + gr_.bx()] = mn[1];
     }
+}
+__global__ void k_cfl(CflConst k, const double *__restrict__ u, const double *__restrict__ v,
+                      double *__restrict__ part) {
+    cfl_body(HwGrid{}, k, u, v, part);
 }
 
 int launch_cfl(impec_ctx *ctx, double *dt_out) {
@@ -333,7 +365,8 @@ int launch_cfl(impec_ctx *ctx, double *dt_out) {
 // sum runs over the sinks in row-major order on one thread when there are few
 // of them (the reference's order), as a block tree otherwise.
 // ---------------------------------------------------------------------------
-__global__ void k_tr(double *__restrict__ c, const int *__restrict__ idx, const double *__restrict__ qv, int nsrc,
+template <class G>
+__device__ __forceinline__ void tr_body(const G &gr_, double *__restrict__ c, const int *__restrict__ idx, const double *__restrict__ qv, int nsrc,
                      int N2, int B, int ld, double dt, double hdt, double inj, double *__restrict__ scal,
                      double *__restrict__ md) {
     const int t = threadIdx.x;
@@ -367,6 +400,11 @@ __global__ void k_tr(double *__restrict__ c, const int *__restrict__ idx, const 
         scal[S_TMO] = tmo[0];
         scal[S_OUTFLUX] += fabs(tmo[0]);
     }
+}
+__global__ void k_tr(double *__restrict__ c, const int *__restrict__ idx, const double *__restrict__ qv, int nsrc,
+                     int N2, int B, int ld, double dt, double hdt, double inj, double *__restrict__ scal,
+                     double *__restrict__ md) {
+    tr_body(HwGrid{}, c, idx, qv, nsrc, N2, B, ld, dt, hdt, inj, scal, md);
 }
 
 int launch_tr(impec_ctx *ctx, double dt) {
@@ -475,6 +513,256 @@ int launch_minmax(impec_ctx *ctx, double *cmax, double *cmin) {
     IMPEC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     *cmax = ctx->h_scal[S_CMAX];
     *cmin = ctx->h_scal[S_CMIN];
+    return IMPEC_OK;
+}
+
+
+// ===========================================================================
+// ensembles: the kernels above over every member in one launch each
+// ===========================================================================
+int red_blocks_of(long long n) { return red_blocks(n); }
+
+int ens_set_error(impec_ensemble *e, int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(e->err, sizeof(e->err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define ENS_CHECK(e, call)                                                                     \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return ens_set_error((e), IMPEC_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,   \
+                                 hipGetErrorString(e_));                                       \
+    } while (0)
+
+__global__ void __launch_bounds__(1024) kb_finish(const FinishRec *__restrict__ tab, int need) {
+    const FinishRec r = tab[blockIdx.x];
+    if (need >= 0 && r.S[S_EST] != (double)need) return;
+    const VGrid g{0, 0, 1, 1};
+    if (r.op == RED_MAX) finish_body<RED_MAX>(g, r.part, r.nblocks, r.nv, r.S, r.dst);
+    else finish_body<RED_SUM>(g, r.part, r.nblocks, r.nv, r.S, r.dst);
+}
+
+hipError_t ens_finish(hipStream_t s, const Batch<FinishRec> &b, int need) {
+    if (b.nmem == 0) return hipSuccess;
+    hipLaunchKernelGGL(kb_finish, dim3(b.nmem), dim3(1024), 0, s, b.tab, need);
+    return hipGetLastError();
+}
+
+struct FillRec {
+    double *a;
+    int N1, N2, B, ld, gx, gy;
+};
+struct MobRec {
+    const double *c, *K11, *K22;
+    double *w, *w22;
+    int rows, cols, ld, harmonic;
+    double mob1over4, mu_res;
+    int gx, gy;
+};
+struct VelRec {
+    VelConst k;
+    const double *p, *w11, *w22;
+    double *u, *v;
+    int gx, gy;
+};
+struct FillVRec {
+    double *u, *v;
+    int N1, N2, B, ld, gx, gy;
+};
+struct CflRec {
+    CflConst k;
+    const double *u, *v;
+    double *part;
+    int gx, gy;
+};
+struct DtRec {
+    double *S;
+    double dx, dy, cfl;
+};
+struct TrRec {
+    double *c;
+    const int *idx;
+    const double *qv;
+    int nsrc, N2, B, ld;
+    double inj;
+    double *S, *md;
+    int gx, gy;
+};
+
+struct EnsFieldTabs {
+    Batch<FillRec> fill_c, fill_p;
+    Batch<MobRec> mob;
+    Batch<VelRec> vel;
+    Batch<FillVRec> fillv;
+    Batch<CflRec> cfl;
+    Batch<FinishRec> cfl_fin;
+    Batch<DtRec> dt;
+    Batch<TrRec> tr;
+};
+
+__global__ void kb_fill_scalar(const FillRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    fill_scalar_body(G, R.a, R.N1, R.N2, R.B, R.ld, 0);
+}
+__global__ void kb_mobility(const MobRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    mobility_body(G, R.c, R.K11, R.K22, R.w, R.w22, R.rows, R.cols, R.ld, R.mob1over4, R.mu_res, R.harmonic);
+}
+__global__ void kb_velocity(const VelRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    velocity_body(G, R.k, R.p, R.w11, R.w22, R.u, R.v);
+}
+__global__ void kb_fill_velocity(const FillVRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    fill_velocity_body(G, R.u, R.v, R.N1, R.N2, R.B, R.ld);
+}
+__global__ void kb_cfl(const CflRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    cfl_body(G, R.k, R.u, R.v, R.part);
+}
+// MaxStableTimestepCombined without dispersion limit (HyperbolicSolver.cpp:123-198),
+// the expressions of launch_cfl
+__global__ void kb_dt(const DtRec *__restrict__ tab, int n) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const DtRec r = tab[m];
+    const double maxv = r.S[S_MAXVEL];
+    const double a = r.dx / maxv, b = r.dy / maxv;
+    double dt = b < a ? b : a;
+    dt *= r.cfl;
+    r.S[S_DT] = dt;
+}
+// TRIntegrator over half the member's step (HyperbolicSplitSolver.cpp:162-198)
+__global__ void kb_tr(const TrRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    const double dt = 0.5 * R.S[S_DT];
+    const double hdt = 0.5 * dt;
+    tr_body(G, R.c, R.idx, R.qv, R.nsrc, R.N2, R.B, R.ld, dt, hdt, R.inj, R.S, R.md);
+}
+
+template <class E>
+static hipError_t launch_batch(void (*k)(const E *, const int *, int), const Batch<E> &b, dim3 blk, hipStream_t s) {
+    if (b.nmem == 0) return hipSuccess;
+    hipLaunchKernelGGL(k, dim3(b.total), blk, 0, s, b.tab, b.start, b.nmem);
+    return hipGetLastError();
+}
+
+void ens_field_free(impec_ensemble *e) {
+    if (!e->field) return;
+    EnsFieldTabs *t = e->field;
+    batch_free(t->fill_c); batch_free(t->fill_p); batch_free(t->mob); batch_free(t->vel); batch_free(t->fillv);
+    batch_free(t->cfl); batch_free(t->cfl_fin); batch_free(t->dt); batch_free(t->tr);
+    delete t;
+    e->field = nullptr;
+}
+
+int ens_field_build(impec_ensemble *e) {
+    ens_field_free(e);
+    EnsFieldTabs *t = new EnsFieldTabs();
+    e->field = t;
+    const int n = e->n;
+    std::vector<FillRec> fc(n), fp(n);
+    std::vector<MobRec> mb(n);
+    std::vector<VelRec> vl(n);
+    std::vector<FillVRec> fv(n);
+    std::vector<CflRec> cf(n);
+    std::vector<FinishRec> ff(n);
+    std::vector<DtRec> dt(n);
+    std::vector<TrRec> tr(n);
+    std::vector<int> u_fill(n), u_mob(n), u_vel(n), u_fv(n), u_cfl(n), u_one(n, 1), u_tr(n);
+    for (int k = 0; k < n; k++) {
+        impec_ctx *ctx = e->m[k];
+        const int N1 = ctx->N1, N2 = ctx->N2, B = ctx->B;
+        // launch_fill_scalar
+        fc[k] = FillRec{ctx->c, N1, N2, B, ctx->lds, (N2 + 2 * B + 63) / 64, (N1 + 2 * B + 3) / 4};
+        fp[k] = fc[k];
+        fp[k].a = ctx->p;
+        u_fill[k] = fc[k].gx * fc[k].gy;
+        // launch_mobility
+        const int rows = N1 + 2 * B, cols = N2 + 2 * B;
+        const int aniso = ctx->K22 != nullptr && ctx->K22 != ctx->K11;
+        mb[k] = MobRec{ctx->c, ctx->K11, ctx->K22, ctx->w, aniso ? ctx->w22 : nullptr, rows, cols, ctx->lds,
+                       ctx->P.average_procedure == IMPEC_AVG_HARMONIC, ctx->mob1over4, ctx->P.mu_res,
+                       (cols + 63) / 64, (rows + 3) / 4};
+        u_mob[k] = mb[k].gx * mb[k].gy;
+        // launch_velocity
+        VelConst &vk = vl[k].k;
+        vk.N1 = N1; vk.N2 = N2; vk.B = B; vk.lds = ctx->lds; vk.ldv = ctx->ldv;
+        vk.harmonic = ctx->P.average_procedure == IMPEC_AVG_HARMONIC;
+        vk.nu = ctx->P.nu; vk.one_m_nu = 1. - ctx->P.nu;
+        vk.dx = ctx->dx; vk.dy = ctx->dy;
+        vk.idxi = 1. / ctx->dxi; vk.ideta = 1. / ctx->deta;
+        vk.s2 = ctx->secamt2; vk.cosa2 = ctx->cosa2; vk.sina2 = ctx->sina2; vk.cost2 = ctx->cost2;
+        vk.sint2 = ctx->sint2; vk.costsina = ctx->costsina; vk.cosasint = ctx->cosasint;
+        vk.cosa = ctx->cosa; vk.sint = ctx->sint; vk.sina = ctx->sina; vk.cost = ctx->cost;
+        vl[k].p = ctx->p;
+        vl[k].w11 = ctx->w;
+        vl[k].w22 = ctx->K22 && ctx->K22 != ctx->K11 ? ctx->w22 : ctx->w;
+        vl[k].u = ctx->u;
+        vl[k].v = ctx->v;
+        vl[k].gx = (N2 + 1 + 63) / 64;
+        vl[k].gy = (N1 + 1 + 3) / 4;
+        u_vel[k] = vl[k].gx * vl[k].gy;
+        // launch_fill_velocity
+        fv[k] = FillVRec{ctx->u, ctx->v, N1, N2, B, ctx->ldv, (N2 + 1 + 2 * B + 63) / 64, (N1 + 1 + 2 * B + 3) / 4};
+        u_fv[k] = fv[k].gx * fv[k].gy;
+        // launch_cfl (combined mode, no dispersion limit)
+        CflConst &ck = cf[k].k;
+        ck.N1 = N1; ck.N2 = N2; ck.B = B; ck.ld = ctx->ldv; ck.mode = 0; ck.diff = 0;
+        ck.dx = ctx->dx; ck.dy = ctx->dy; ck.al = ctx->P.alpha_l; ck.at = ctx->P.alpha_t;
+        cf[k].u = ctx->u;
+        cf[k].v = ctx->v;
+        cf[k].part = ctx->part;
+        cf[k].gx = red_blocks((long long)N1 * N2);
+        cf[k].gy = 1;
+        u_cfl[k] = cf[k].gx;
+        ff[k] = FinishRec{ctx->part, ctx->scal, cf[k].gx, 1, (int)S_MAXVEL, (int)RED_MAX};
+        dt[k] = DtRec{ctx->scal, ctx->dx, ctx->dy, ctx->P.cfl};
+        // launch_tr
+        tr[k] = TrRec{ctx->c, ctx->src_idx, ctx->src_q, ctx->nsrc, N2, B, ctx->lds, ctx->P.inj_fluid_conc, ctx->scal,
+                      ctx->tr_md, 1, 1};
+        u_tr[k] = ctx->nsrc > 0 ? 1 : 0;
+    }
+    ENS_CHECK(e, batch_upload(t->fill_c, fc, u_fill));
+    ENS_CHECK(e, batch_upload(t->fill_p, fp, u_fill));
+    ENS_CHECK(e, batch_upload(t->mob, mb, u_mob));
+    ENS_CHECK(e, batch_upload(t->vel, vl, u_vel));
+    ENS_CHECK(e, batch_upload(t->fillv, fv, u_fv));
+    ENS_CHECK(e, batch_upload(t->cfl, cf, u_cfl));
+    ENS_CHECK(e, batch_upload(t->cfl_fin, ff, u_one));
+    ENS_CHECK(e, batch_upload(t->dt, dt, u_one));
+    ENS_CHECK(e, batch_upload(t->tr, tr, u_tr));
+    return IMPEC_OK;
+}
+
+int ens_fill_c(impec_ensemble *e) {
+    ENS_CHECK(e, launch_batch(kb_fill_scalar, e->field->fill_c, dim3(64, 4), e->stream));
+    return IMPEC_OK;
+}
+
+int ens_mobility(impec_ensemble *e) {
+    ENS_CHECK(e, launch_batch(kb_mobility, e->field->mob, dim3(64, 4), e->stream));
+    return IMPEC_OK;
+}
+
+int ens_velocity_dt(impec_ensemble *e) {
+    EnsFieldTabs *t = e->field;
+    ENS_CHECK(e, launch_batch(kb_fill_scalar, t->fill_p, dim3(64, 4), e->stream));
+    ENS_CHECK(e, launch_batch(kb_velocity, t->vel, dim3(64, 4), e->stream));
+    ENS_CHECK(e, launch_batch(kb_fill_velocity, t->fillv, dim3(64, 4), e->stream));
+    ENS_CHECK(e, launch_batch(kb_cfl, t->cfl, dim3(256), e->stream));
+    ENS_CHECK(e, ens_finish(e->stream, t->cfl_fin, -1));
+    hipLaunchKernelGGL(kb_dt, dim3((e->n + 63) / 64), dim3(64), 0, e->stream, t->dt.tab, t->dt.nmem);
+    ENS_CHECK(e, hipGetLastError());
+    return IMPEC_OK;
+}
+
+int ens_tr(impec_ensemble *e) {
+    ENS_CHECK(e, launch_batch(kb_tr, e->field->tr, dim3(1024), e->stream));
     return IMPEC_OK;
 }
 

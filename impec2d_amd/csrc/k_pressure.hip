@@ -195,9 +195,10 @@ __device__ __forceinline__ void op_row_L(const LevArgs &L, int i, int j, double 
 #define IMPEC_NO_FACES 0
 #endif
 // face mobilities of the level-0 5-point operator (see LevArgs)
-__global__ void k_faces(LevArgs L, double *__restrict__ fx, double *__restrict__ fy) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+template <class G>
+__device__ __forceinline__ void faces_body(const G &gr_, LevArgs L, double *__restrict__ fx, double *__restrict__ fy) {
+    const int j = gr_.bx() * blockDim.x + threadIdx.x;
+    const int i = gr_.by() * blockDim.y + threadIdx.y;
     const int n1 = L.g.n1, n2 = L.g.n2;
 #define W1(a_, b_) L.w1[(long)(a_) * L.ldw + (b_)]
 #define W2(a_, b_) L.w2[(long)(a_) * L.ldw + (b_)]
@@ -205,6 +206,9 @@ __global__ void k_faces(LevArgs L, double *__restrict__ fx, double *__restrict__
     if (i < n1 && j <= n2) fy[(long)i * L.ld + j] = face_mob(W2(i, j - 1), W2(i, j), L.g.harmonic);
 #undef W1
 #undef W2
+}
+__global__ void k_faces(LevArgs L, double *__restrict__ fx, double *__restrict__ fy) {
+    faces_body(HwGrid{}, L, fx, fy);
 }
 
 // ---------------------------------------------------------------------------
@@ -298,10 +302,11 @@ __global__ void k_fill_zero(double *__restrict__ a, long n) {
 
 // coarse mobility quantity: mean over the existing fine cells of each 2x2
 // block, on the interior and (clamped) ghost ring of the coarse level
-__global__ void k_coarsen_w(const double *__restrict__ wf, int ldf, int n1, int n2, double *__restrict__ wc, int ldc,
+template <class G>
+__device__ __forceinline__ void coarsen_w_body(const G &gr_, const double *__restrict__ wf, int ldf, int n1, int n2, double *__restrict__ wc, int ldc,
                             int m1, int m2) {
-    const int J = blockIdx.x * blockDim.x + threadIdx.x - 1;
-    const int I = blockIdx.y * blockDim.y + threadIdx.y - 1;
+    const int J = gr_.bx() * blockDim.x + threadIdx.x - 1;
+    const int I = gr_.by() * blockDim.y + threadIdx.y - 1;
     if (I > m1 || J > m2) return;
     const int Ic = I < 0 ? 0 : (I >= m1 ? m1 - 1 : I);
     const int Jc = J < 0 ? 0 : (J >= m2 ? m2 - 1 : J);
@@ -317,14 +322,18 @@ __global__ void k_coarsen_w(const double *__restrict__ wf, int ldf, int n1, int 
         }
     wc[(long)I * ldc + J] = s / cnt;
 }
+__global__ void k_coarsen_w(const double *__restrict__ wf, int ldf, int n1, int n2, double *__restrict__ wc, int ldc,
+                            int m1, int m2) {
+    coarsen_w_body(HwGrid{}, wf, ldf, n1, n2, wc, ldc, m1, m2);
+}
 
 // coarsest level (at most 2 x 2 cells): exact solve of the singular Neumann
 // system bordered by the zero-mean condition, [A 1; 1^T 0][x; l] = [b; 0],
 // by Gaussian elimination with partial pivoting on one thread
 // (x, b stored as T; the elimination is FP64 in both precisions)
-template <bool DIAG, typename T = double>
-__global__ void k_coarsest(LevArgs L, T *__restrict__ x, const T *__restrict__ b) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+template <bool DIAG, typename T = double, class G>
+__device__ __forceinline__ void coarsest_body(const G &gr_, LevArgs L, T *__restrict__ x, const T *__restrict__ b) {
+    if (threadIdx.x != 0 || gr_.bx() != 0) return;
     const int n1 = L.g.n1, n2 = L.g.n2, n = n1 * n2, m = n + 1;
     const int dI[9] = {0, -1, -1, 0, 1, 1, 1, 0, -1}, dJ[9] = {0, 0, -1, -1, -1, 0, 1, 1, 1};
     double M[5][6];
@@ -370,6 +379,10 @@ __global__ void k_coarsest(LevArgs L, T *__restrict__ x, const T *__restrict__ b
     for (int i = 0; i < n1; i++)
         for (int j = 0; j < n2; j++) x[(long)i * L.ld + j] = (T)sol[i * n2 + j];
 }
+template <bool DIAG, typename T = double>
+__global__ void k_coarsest(LevArgs L, T *__restrict__ x, const T *__restrict__ b) {
+    coarsest_body<DIAG, T>(HwGrid{}, L, x, b);
+}
 
 // ---------------------------------------------------------------------------
 // conjugate-gradient vector kernels (level-0 ring-1 layout; x lives in p)
@@ -391,12 +404,12 @@ __device__ __forceinline__ void cg_cell(const CgArgs &c, long id, int &i, int &j
 }
 
 // r = b - A_N x with b = -q; pin handling; partials (r.r, sum r)
-template <bool DIAG>
-__global__ void k_cg_init(LevArgs L, CgArgs c, double *__restrict__ p, const double *__restrict__ q,
+template <bool DIAG, class G>
+__device__ __forceinline__ void cg_init_body(const G &gr_, LevArgs L, CgArgs c, double *__restrict__ p, const double *__restrict__ q,
                           double *__restrict__ r, double *__restrict__ part) {
     double s[2] = {0., 0.};
     double *x = p + (long)c.B * c.ldp + c.B;
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+    for (long id = (long)gr_.bx() * blockDim.x + threadIdx.x; id < c.n; id += (long)gr_.gx() * blockDim.x) {
         int i, j;
         cg_cell(c, id, i, j);
         double a[9];
@@ -429,9 +442,14 @@ __global__ void k_cg_init(LevArgs L, CgArgs c, double *__restrict__ p, const dou
     }
     block_reduce<RED_SUM, 2>(s);
     if (tid_linear() == 0) {
-        part[blockIdx.x] = s[0];
-        part[RED_BLOCKS + blockIdx.x] = s[1];
+        part[gr_.bx()] = s[0];
+        part[RED_BLOCKS + gr_.bx()] = s[1];
     }
+}
+template <bool DIAG>
+__global__ void k_cg_init(LevArgs L, CgArgs c, double *__restrict__ p, const double *__restrict__ q,
+                          double *__restrict__ r, double *__restrict__ part) {
+    cg_init_body<DIAG>(HwGrid{}, L, c, p, q, r, part);
 }
 
 // Learned initial guess: the network's increment d is used as a search
@@ -576,28 +594,36 @@ __global__ void k_cg_update(CgArgs c, double *__restrict__ p, const double *__re
 
 __global__ void k_copy_scal(double *__restrict__ S, int dst, int src) { S[dst] = S[src]; }
 
-__global__ void k_read_pin(CgArgs c, const double *__restrict__ p, double *__restrict__ S) {
+template <class G>
+__device__ __forceinline__ void read_pin_body(const G &gr_, CgArgs c, const double *__restrict__ p, double *__restrict__ S) {
     S[S_XPIN] = p[(long)(c.pin_i + c.B) * c.ldp + (c.pin_j + c.B)];
+}
+__global__ void k_read_pin(CgArgs c, const double *__restrict__ p, double *__restrict__ S) {
+    read_pin_body(HwGrid{}, c, p, S);
 }
 
 // x += ppin - x_pin on the interior, x_pin = ppin exactly
-__global__ void k_shift(CgArgs c, double *__restrict__ p, const double *__restrict__ S) {
+template <class G>
+__device__ __forceinline__ void shift_body(const G &gr_, CgArgs c, double *__restrict__ p, const double *__restrict__ S) {
     const double sh = c.ppin - S[S_XPIN];
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+    for (long id = (long)gr_.bx() * blockDim.x + threadIdx.x; id < c.n; id += (long)gr_.gx() * blockDim.x) {
         int i, j;
         cg_cell(c, id, i, j);
         double *xp = p + (long)(i + c.B) * c.ldp + (j + c.B);
         *xp = (i == c.pin_i && j == c.pin_j) ? c.ppin : *xp + sh;
     }
 }
+__global__ void k_shift(CgArgs c, double *__restrict__ p, const double *__restrict__ S) {
+    shift_body(HwGrid{}, c, p, S);
+}
 
 // ||b - A x||^2 of the reference's pinned system
-template <bool DIAG>
-__global__ void k_true_residual(LevArgs L, CgArgs c, const double *__restrict__ p, const double *__restrict__ q,
+template <bool DIAG, class G>
+__device__ __forceinline__ void true_residual_body(const G &gr_, LevArgs L, CgArgs c, const double *__restrict__ p, const double *__restrict__ q,
                                 double *__restrict__ part) {
     double s[1] = {0.};
     const double *x = p + (long)c.B * c.ldp + c.B;
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+    for (long id = (long)gr_.bx() * blockDim.x + threadIdx.x; id < c.n; id += (long)gr_.gx() * blockDim.x) {
         int i, j;
         cg_cell(c, id, i, j);
         double rv;
@@ -611,7 +637,12 @@ __global__ void k_true_residual(LevArgs L, CgArgs c, const double *__restrict__ 
         s[0] += rv * rv;
     }
     block_reduce<RED_SUM, 1>(s);
-    if (tid_linear() == 0) part[blockIdx.x] = s[0];
+    if (tid_linear() == 0) part[gr_.bx()] = s[0];
+}
+template <bool DIAG>
+__global__ void k_true_residual(LevArgs L, CgArgs c, const double *__restrict__ p, const double *__restrict__ q,
+                                double *__restrict__ part) {
+    true_residual_body<DIAG>(HwGrid{}, L, c, p, q, part);
 }
 
 // operator dump: nine N1*N2 arrays including the pinned row (MGSolver.cpp:517-567)
@@ -830,12 +861,16 @@ static int fetch_scal(impec_ctx *ctx) {
     return IMPEC_OK;
 }
 
-__global__ void k_zero_interior(CgArgs c, double *__restrict__ p) {
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+template <class G>
+__device__ __forceinline__ void zero_interior_body(const G &gr_, CgArgs c, double *__restrict__ p) {
+    for (long id = (long)gr_.bx() * blockDim.x + threadIdx.x; id < c.n; id += (long)gr_.gx() * blockDim.x) {
         int i, j;
         cg_cell(c, id, i, j);
         p[(long)(i + c.B) * c.ldp + (j + c.B)] = 0.;
     }
+}
+__global__ void k_zero_interior(CgArgs c, double *__restrict__ p) {
+    zero_interior_body(HwGrid{}, c, p);
 }
 
 __global__ void k_set_pin(CgArgs c, double *__restrict__ p) {
@@ -966,12 +1001,17 @@ struct Chunk {
     bool out;   // the lane writes (lanes 2..61)
 };
 
+// wave index of the calling wave in a launch of SW_WAVES-wave workgroups;
+// wave-uniform by construction, and readfirstlane lets the compiler know, so
+// the march loop and the row masks are scalar
+__device__ __forceinline__ int hw_wave() {
+    return blockIdx.x * SW_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
+// strip and chunk of wave `wid` of a leg's launch over one level
 template <typename Args>
-__device__ __forceinline__ bool wave_setup(const Args &A, Lane &L, Chunk &K) {
+__device__ __forceinline__ bool wave_setup(const Args &A, int wid, Lane &L, Chunk &K) {
     const int lane = threadIdx.x & 63;
-    // wave-uniform by construction; readfirstlane lets the compiler know, so
-    // the march loop and the row masks are scalar
-    const int wid = blockIdx.x * SW_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wid >= A.nwave) return false;
     const int strip = wid % A.nstrip, chunk = wid / A.nstrip;
     const int n2 = A.g.n2;
@@ -1013,11 +1053,11 @@ __device__ __forceinline__ void mask_pair(const Lane &L, int m, int n1, V fill, 
 constexpr int SW_PF = IMPEC_SW_PF;
 
 template <bool FINE0, bool ANISO, typename T>
-__global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgsT<T> A) {
+__device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int wid) {
 #pragma clang fp contract(fast)
     Lane L;
     Chunk K;
-    if (!wave_setup(A, L, K)) return;
+    if (!wave_setup(A, wid, L, K)) return;
     const int n1 = A.g.n1, n2 = A.g.n2;
     constexpr bool aniso = ANISO;
     const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
@@ -1152,11 +1192,16 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgsT<T> A) {
 }
 
 template <bool FINE0, bool ANISO, typename T>
-__global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgsT<T> A) {
+__global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgsT<T> A) {
+    mg_down_body<FINE0, ANISO, T>(A, hw_wave());
+}
+
+template <bool FINE0, bool ANISO, typename T>
+__device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid) {
 #pragma clang fp contract(fast)
     Lane L;
     Chunk K;
-    if (!wave_setup(A, L, K)) return;
+    if (!wave_setup(A, wid, L, K)) return;
     const int n1 = A.g.n1, m1 = A.m1, m2 = A.m2;
     constexpr bool aniso = ANISO;
     const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
@@ -1269,15 +1314,20 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgsT<T> A) {
             acc1 += __shfl_xor(acc1, o);
         }
         if ((threadIdx.x & 63) == 0) {
-            const int wid = blockIdx.x * SW_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
             A.part[wid] = acc0;
             A.part[RED_BLOCKS + wid] = acc1;
         }
     }
 }
 
+template <bool FINE0, bool ANISO, typename T>
+__global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgsT<T> A) {
+    mg_up_body<FINE0, ANISO, T>(A, hw_wave());
+}
+
 // sums of the V-cycle output and rho = r.z with z = x0 - mean(x0)
-__global__ void k_finish_rho(const double *__restrict__ part, int nblocks, double inv_n, double *__restrict__ S) {
+template <class G>
+__device__ __forceinline__ void finish_rho_body(const G &gr_, const double *__restrict__ part, int nblocks, double inv_n, double *__restrict__ S) {
     double v[2] = {strided_combine<RED_SUM>(part, nblocks), strided_combine<RED_SUM>(part + RED_BLOCKS, nblocks)};
     block_reduce<RED_SUM, 2>(v);
     if (threadIdx.x == 0) {
@@ -1286,6 +1336,9 @@ __global__ void k_finish_rho(const double *__restrict__ part, int nblocks, doubl
         S[S_RHO_OLD] = S[S_RHO];
         S[S_RHO] = v[1] - v[0] * inv_n * S[S_RSUM];
     }
+}
+__global__ void k_finish_rho(const double *__restrict__ part, int nblocks, double inv_n, double *__restrict__ S) {
+    finish_rho_body(HwGrid{}, part, nblocks, inv_n, S);
 }
 
 // tiled 2-D launch shape for the level-0 vector kernels: 64 x 4 threads,
@@ -1307,16 +1360,16 @@ static Grid2 vec_grid(int n1, int n2) {
 
 // d_new = (x0 - mean) + beta d_old on the cell and its four neighbours,
 // q = A_N d_new (pinned row zeroed in the eliminated formulation), d.q partials
-template <typename T>
-__global__ void k_cg_dir_matvec(LevArgs L, CgArgs c, const T *__restrict__ x0, const double *__restrict__ dold,
+template <typename T, class G>
+__device__ __forceinline__ void cg_dir_matvec_body(const G &gr_, LevArgs L, CgArgs c, const T *__restrict__ x0, const double *__restrict__ dold,
                                 double *__restrict__ dnew, double *__restrict__ qv, const double *__restrict__ S,
                                 int first, double *__restrict__ part) {
     const double mean = S[S_ZSUM] / (double)c.n;
     const double beta = first ? 0. : S[S_RHO] / S[S_RHO_OLD];
-    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int j = gr_.bx() * 64 + threadIdx.x;
     double s[1] = {0.};
     const long ld = c.ld;
-    for (int i = blockIdx.y * 4 + threadIdx.y; i < c.n1; i += gridDim.y * 4) {
+    for (int i = gr_.by() * 4 + threadIdx.y; i < c.n1; i += gr_.gy() * 4) {
         if (j >= c.n2) break;
         const long o = (long)(i + 1) * ld + (j + 1);
 #define DN(off_, ii_, jj_)                                                                                  \
@@ -1338,16 +1391,23 @@ __global__ void k_cg_dir_matvec(LevArgs L, CgArgs c, const T *__restrict__ x0, c
         s[0] += dc * y;
     }
     block_reduce<RED_SUM, 1>(s);
-    if (tid_linear() == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = s[0];
+    if (tid_linear() == 0) part[gr_.by() * gr_.gx() + gr_.bx()] = s[0];
+}
+template <typename T>
+__global__ void k_cg_dir_matvec(LevArgs L, CgArgs c, const T *__restrict__ x0, const double *__restrict__ dold,
+                                double *__restrict__ dnew, double *__restrict__ qv, const double *__restrict__ S,
+                                int first, double *__restrict__ part) {
+    cg_dir_matvec_body<T>(HwGrid{}, L, c, x0, dold, dnew, qv, S, first, part);
 }
 
 // x += alpha d, r -= alpha q; partials (r.r, sum r)
-__global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__restrict__ d, double *__restrict__ r,
+template <class G>
+__device__ __forceinline__ void cg_update2_body(const G &gr_, CgArgs c, double *__restrict__ p, const double *__restrict__ d, double *__restrict__ r,
                              const double *__restrict__ qv, const double *__restrict__ S, double *__restrict__ part) {
     const double alpha = S[S_RHO] / S[S_DQ];
-    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int j = gr_.bx() * 64 + threadIdx.x;
     double s[2] = {0., 0.};
-    for (int i = blockIdx.y * 4 + threadIdx.y; i < c.n1; i += gridDim.y * 4) {
+    for (int i = gr_.by() * 4 + threadIdx.y; i < c.n1; i += gr_.gy() * 4) {
         if (j >= c.n2) break;
         const long o = (long)(i + 1) * c.ld + (j + 1);
         double *xp = p + (long)(i + c.B) * c.ldp + (j + c.B);
@@ -1359,10 +1419,14 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
     }
     block_reduce<RED_SUM, 2>(s);
     if (tid_linear() == 0) {
-        const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+        const int blk = gr_.by() * gr_.gx() + gr_.bx();
         part[blk] = s[0];
         part[RED_BLOCKS + blk] = s[1];
     }
+}
+__global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__restrict__ d, double *__restrict__ r,
+                             const double *__restrict__ qv, const double *__restrict__ S, double *__restrict__ part) {
+    cg_update2_body(HwGrid{}, c, p, d, r, qv, S, part);
 }
 
 #ifndef IMPEC_MG_WAVES
@@ -1475,11 +1539,16 @@ static int vcycle_fused(impec_ctx *ctx) {
 }
 
 // FP32 copy of a mobility array (rows x cols cells from (0,0) of each)
+template <class G>
+__device__ __forceinline__ void to_f32_body(const G &gr_, const double *__restrict__ src, int lds, float *__restrict__ dst, int ldd, int rows,
+                         int cols) {
+    const int j = gr_.bx() * blockDim.x + threadIdx.x;
+    const int i = gr_.by() * blockDim.y + threadIdx.y;
+    if (i < rows && j < cols) dst[(long)i * ldd + j] = (float)src[(long)i * lds + j];
+}
 __global__ void k_to_f32(const double *__restrict__ src, int lds, float *__restrict__ dst, int ldd, int rows,
                          int cols) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i < rows && j < cols) dst[(long)i * ldd + j] = (float)src[(long)i * lds + j];
+    to_f32_body(HwGrid{}, src, lds, dst, ldd, rows, cols);
 }
 
 // FP32 level arrays of the mixed-precision cycle (first use) and the FP32
@@ -1735,6 +1804,571 @@ int pressure_operator_dump(impec_ctx *ctx, double *A9_host) {
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     hipFree(dA);
     if (e != hipSuccess) return set_error(ctx, IMPEC_EHIP, "operator dump: %s", hipGetErrorString(e));
+    return IMPEC_OK;
+}
+
+
+// ===========================================================================
+// ensembles: the 5-point streaming solve of every member in batched launches.
+// Each member runs the CG of solve_cg (consistent Neumann formulation) with
+// its loop state on the device: kb_ctl_* make, per member, the decisions the
+// host loop of solve_cg makes (restart on drift, iteration limit, non-finite
+// residual), so a member's iterates, residuals and iteration count are those
+// of its own solve_cg.
+// ===========================================================================
+#define ENS_CHECK(e, call)                                                                     \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return ens_set_error((e), IMPEC_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,   \
+                                 hipGetErrorString(e_));                                       \
+    } while (0)
+
+struct FacesRec {
+    LevArgs L;
+    double *fx, *fy;
+    int gx, gy;
+};
+struct CoarsenRec {
+    const double *wf;
+    int ldf, n1, n2;
+    double *wc;
+    int ldc, m1, m2, gx, gy;
+};
+struct ToF32Rec {
+    const double *src;
+    int lds;
+    float *dst;
+    int ldd, rows, cols, gx, gy;
+};
+struct ZeroRec {
+    CgArgs c;
+    double *p;
+    const double *S;
+    int gx, gy;
+};
+struct CgInitRec {
+    LevArgs L;
+    CgArgs c;
+    double *p;
+    const double *q;
+    double *r, *part;
+    const double *S;
+    int gx, gy;
+};
+template <typename T>
+struct CoarsestRec {
+    LevArgs L;
+    T *x;
+    const T *b;
+    const double *S;
+};
+struct RhoRec {
+    const double *part;
+    int nblocks;
+    double inv_n;
+    double *S;
+};
+template <typename T>
+struct DirRec {
+    LevArgs L;
+    CgArgs c;
+    const T *x0;
+    double *d0, *d1, *q;
+    const double *S;
+    double *part;
+    int gx, gy;
+};
+struct Upd2Rec {
+    CgArgs c;
+    double *p, *d0, *d1, *r;
+    const double *q, *S;
+    double *part;
+    int gx, gy;
+};
+struct CtlRec {
+    double *S;
+    int always_zero;
+};
+struct PinRec {
+    CgArgs c;
+    double *p, *S;
+};
+struct ShiftRec {
+    CgArgs c;
+    double *p;
+    const double *S;
+    int gx, gy;
+};
+struct TrueRec {
+    LevArgs L;
+    CgArgs c;
+    const double *p, *q;
+    double *part;
+    int gx, gy;
+};
+
+template <typename T>
+struct LegTabs {
+    Batch<FusedArgsT<T>> down[MAX_LEVELS], up[MAX_LEVELS];
+    Batch<CoarsestRec<T>> coarsest;
+    Batch<DirRec<T>> dir;
+    void free_all() {
+        for (int l = 0; l < MAX_LEVELS; l++) {
+            batch_free(down[l]);
+            batch_free(up[l]);
+        }
+        batch_free(coarsest);
+        batch_free(dir);
+    }
+};
+
+struct EnsPressTabs {
+    int mixed;
+    int lmax;                           // most levels of any member
+    Batch<FacesRec> faces;
+    Batch<CoarsenRec> coarsen[MAX_LEVELS];
+    Batch<ToF32Rec> tof32[MAX_LEVELS];
+    Batch<ZeroRec> zero;
+    Batch<CgInitRec> init;
+    Batch<FinishRec> fin_init, fin_dq, fin_rr, fin_tt;
+    LegTabs<double> l64;
+    LegTabs<float> l32;
+    Batch<RhoRec> rho;
+    Batch<Upd2Rec> upd;
+    Batch<CtlRec> ctl;
+    Batch<PinRec> pin;
+    Batch<ShiftRec> shift;
+    Batch<TrueRec> tres;
+};
+
+__device__ __forceinline__ bool in_state(const double *S, int need) { return need < 0 || S[S_EST] == (double)need; }
+
+__global__ void kb_faces(const FacesRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    faces_body(G, R.L, R.fx, R.fy);
+}
+__global__ void kb_coarsen_w(const CoarsenRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    coarsen_w_body(G, R.wf, R.ldf, R.n1, R.n2, R.wc, R.ldc, R.m1, R.m2);
+}
+__global__ void kb_to_f32(const ToF32Rec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    to_f32_body(G, R.src, R.lds, R.dst, R.ldd, R.rows, R.cols);
+}
+__global__ void kb_zero_p(const ZeroRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    if (R.S[S_ZEROP] == 0.) return;
+    zero_interior_body(G, R.c, R.p);
+}
+__global__ void kb_cg_init(const CgInitRec *__restrict__ tab, const int *__restrict__ start, int nmem, int need) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    if (!in_state(R.S, need)) return;
+    cg_init_body<false>(G, R.L, R.c, R.p, R.q, R.r, R.part);
+}
+template <bool FINE0, typename T>
+__global__ void __launch_bounds__(64 * SW_WAVES)
+    kb_mg_down(const FusedArgsT<T> *__restrict__ tab, const int *__restrict__ start, int nmem, int total, int need) {
+    const int gw = hw_wave();
+    if (gw >= total) return;
+    const int m = member_of(start, nmem, gw);
+    const FusedArgsT<T> A = tab[m];
+    if (!in_state(A.S, need)) return;
+    mg_down_body<FINE0, false, T>(A, gw - start[m]);
+}
+template <bool FINE0, typename T>
+__global__ void __launch_bounds__(64 * SW_WAVES)
+    kb_mg_up(const FusedArgsT<T> *__restrict__ tab, const int *__restrict__ start, int nmem, int total, int need) {
+    const int gw = hw_wave();
+    if (gw >= total) return;
+    const int m = member_of(start, nmem, gw);
+    const FusedArgsT<T> A = tab[m];
+    if (!in_state(A.S, need)) return;
+    mg_up_body<FINE0, false, T>(A, gw - start[m]);
+}
+template <typename T>
+__global__ void kb_coarsest(const CoarsestRec<T> *__restrict__ tab, int need) {
+    const CoarsestRec<T> R = tab[blockIdx.x];
+    if (!in_state(R.S, need)) return;
+    coarsest_body<false, T>(VGrid{0, 0, 1, 1}, R.L, R.x, R.b);
+}
+__global__ void __launch_bounds__(1024) kb_finish_rho(const RhoRec *__restrict__ tab, int need) {
+    const RhoRec R = tab[blockIdx.x];
+    if (!in_state(R.S, need)) return;
+    finish_rho_body(VGrid{0, 0, 1, 1}, R.part, R.nblocks, R.inv_n, R.S);
+}
+// the direction arrays alternate every iteration of a solve (d_old, d_new) =
+// (d0, d1) on even iterations and (d1, d0) on odd ones, as in solve_cg
+template <typename T>
+__global__ void kb_dir_matvec(const DirRec<T> *__restrict__ tab, const int *__restrict__ start, int nmem, int need) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    if (!in_state(R.S, need)) return;
+    const bool odd = ((long long)R.S[S_EITS]) & 1;
+    const int first = R.S[S_ESINCE] == 0.;
+    cg_dir_matvec_body<T>(G, R.L, R.c, R.x0, odd ? R.d1 : R.d0, odd ? R.d0 : R.d1, R.q, R.S, first, R.part);
+}
+__global__ void kb_update2(const Upd2Rec *__restrict__ tab, const int *__restrict__ start, int nmem, int need) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    if (!in_state(R.S, need)) return;
+    const bool odd = ((long long)R.S[S_EITS]) & 1;
+    cg_update2_body(G, R.c, R.p, odd ? R.d0 : R.d1, R.r, R.q, R.S, R.part);
+}
+// after the first residual: iterate unless converged (solve_cg's loop entry)
+__global__ void kb_ctl_first(const CtlRec *__restrict__ tab, int n) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    double *S = tab[m].S;
+    const double rr = S[S_RR];
+    S[S_EITS] = 0.;
+    S[S_ESINCE] = 0.;
+    S[S_ERST] = 0.;
+    S[S_ESTATUS] = 0.;
+    double est = EST_ITER;
+    if (rr <= S[S_ETOL]) est = EST_DONE;
+    else if (S[S_EMAXIT] <= 0.) { est = EST_DONE; S[S_ESTATUS] = 1.; }
+    else if (!(rr == rr)) { est = EST_DONE; S[S_ESTATUS] = 2.; }
+    S[S_EST] = est;
+    if (!tab[m].always_zero) S[S_ZEROP] = 0.;
+}
+// after an iteration: count it; converged recursions get their residual recomputed
+__global__ void kb_ctl_iter(const CtlRec *__restrict__ tab, int n) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    double *S = tab[m].S;
+    if (S[S_EST] != (double)EST_ITER) return;
+    const double its = S[S_EITS] + 1.;
+    S[S_EITS] = its;
+    S[S_ESINCE] = S[S_ESINCE] + 1.;
+    const double rr = S[S_RR];
+    if (rr <= S[S_ETOL]) S[S_EST] = EST_CHECK;
+    else if (its >= S[S_EMAXIT]) { S[S_EST] = EST_DONE; S[S_ESTATUS] = 1.; }
+    else if (!(rr == rr)) { S[S_EST] = EST_DONE; S[S_ESTATUS] = 2.; }
+}
+// after a recomputed residual: done, or restart the recursion (at most 3 times)
+__global__ void kb_ctl_check(const CtlRec *__restrict__ tab, int n) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    double *S = tab[m].S;
+    if (S[S_EST] != (double)EST_CHECK) return;
+    const double rr = S[S_RR];
+    if (rr <= S[S_ETOL] || S[S_ERST] >= 3.) { S[S_EST] = EST_DONE; return; }
+    S[S_ERST] = S[S_ERST] + 1.;
+    S[S_ESINCE] = 0.;
+    if (S[S_EITS] >= S[S_EMAXIT]) { S[S_EST] = EST_DONE; S[S_ESTATUS] = 1.; }
+    else if (!(rr == rr)) { S[S_EST] = EST_DONE; S[S_ESTATUS] = 2.; }
+    else S[S_EST] = EST_ITER;
+}
+__global__ void kb_read_pin(const PinRec *__restrict__ tab, int n) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const PinRec R = tab[m];
+    read_pin_body(VGrid{0, 0, 1, 1}, R.c, R.p, R.S);
+}
+__global__ void kb_shift(const ShiftRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    shift_body(G, R.c, R.p, R.S);
+}
+__global__ void kb_true_residual(const TrueRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    true_residual_body<false>(G, R.L, R.c, R.p, R.q, R.part);
+}
+
+template <typename T>
+static void free_legs(LegTabs<T> &t) { t.free_all(); }
+
+void ens_press_free(impec_ensemble *e) {
+    if (!e->pres) return;
+    EnsPressTabs *t = e->pres;
+    batch_free(t->faces);
+    for (int l = 0; l < MAX_LEVELS; l++) {
+        batch_free(t->coarsen[l]);
+        batch_free(t->tof32[l]);
+    }
+    batch_free(t->zero); batch_free(t->init);
+    batch_free(t->fin_init); batch_free(t->fin_dq); batch_free(t->fin_rr); batch_free(t->fin_tt);
+    free_legs(t->l64);
+    free_legs(t->l32);
+    batch_free(t->rho); batch_free(t->upd); batch_free(t->ctl); batch_free(t->pin); batch_free(t->shift);
+    batch_free(t->tres);
+    delete t;
+    e->pres = nullptr;
+}
+
+// CgArgs of solve_cg's consistent-Neumann phase
+static CgArgs cg_args(const impec_ctx *ctx) {
+    const Level &L0 = ctx->lev[0];
+    CgArgs c;
+    c.n1 = ctx->N1; c.n2 = ctx->N2; c.ld = L0.ld; c.ldp = ctx->lds; c.B = ctx->B;
+    c.n = (long)ctx->N1 * ctx->N2;
+    c.pin_i = ctx->pin_i; c.pin_j = ctx->pin_j;
+    c.dpin = ctx->kappa / ctx->dA;
+    c.ppin = -ctx->q_pin / c.dpin;
+    c.bsum = ctx->bsum;
+    c.shift = 1;
+    return c;
+}
+
+template <typename T>
+static int build_legs(impec_ensemble *e, LegTabs<T> &t) {
+    using LA = LevelArrays<T>;
+    const int n = e->n;
+    t.free_all();
+    const int lmax = e->pres->lmax;
+    for (int l = 0; l + 1 < lmax; l++) {
+        std::vector<FusedArgsT<T>> recs(n);
+        std::vector<int> units(n, 0);
+        for (int k = 0; k < n; k++) {
+            impec_ctx *ctx = e->m[k];
+            if (l + 1 >= ctx->nlev) continue;
+            recs[k] = fused_args<T>(ctx, l);
+            units[k] = recs[k].nwave;
+        }
+        ENS_CHECK(e, batch_upload(t.down[l], recs, units));
+        ENS_CHECK(e, batch_upload(t.up[l], recs, units));
+    }
+    std::vector<CoarsestRec<T>> cr(n);
+    std::vector<DirRec<T>> dr(n);
+    std::vector<int> one(n, 1), u_vec(n);
+    for (int k = 0; k < n; k++) {
+        impec_ctx *ctx = e->m[k];
+        Level &Cl = ctx->lev[ctx->nlev - 1];
+        cr[k] = CoarsestRec<T>{lev_args(ctx, ctx->nlev - 1), LA::x(Cl) + Cl.ld + 1, LA::b(Cl) + Cl.ld + 1, ctx->scal};
+        LevArgs a0 = lev_args(ctx, 0);
+        if (!IMPEC_NO_FACES) {
+            a0.fx = ctx->fx;
+            a0.fy = ctx->fy;
+        }
+        const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
+        dr[k] = DirRec<T>{a0, cg_args(ctx), LA::r(ctx->lev[0]), ctx->pd, ctx->pz, ctx->pq, ctx->scal, ctx->part,
+                          (int)vg.grid.x, (int)vg.grid.y};
+        u_vec[k] = vg.nblocks;
+    }
+    ENS_CHECK(e, batch_upload(t.coarsest, cr, one));
+    ENS_CHECK(e, batch_upload(t.dir, dr, u_vec));
+    return IMPEC_OK;
+}
+
+int ens_press_build(impec_ensemble *e) {
+    ens_press_free(e);
+    EnsPressTabs *t = new EnsPressTabs();
+    e->pres = t;
+    const int n = e->n;
+    t->mixed = e->m[0]->P.precision == IMPEC_PREC_MIXED;
+    t->lmax = 0;
+    for (int k = 0; k < n; k++) {
+        impec_ctx *ctx = e->m[k];
+        if (ctx->nlev > t->lmax) t->lmax = ctx->nlev;
+        if (t->mixed) {
+            int st = mixed_setup(ctx);   // FP32 level arrays
+            if (st) return ens_set_error(e, st, "member %d: %s", k, ctx->err);
+        }
+    }
+    dim3 blk(64, 4);
+    // hierarchy and FP32 copies, level by level
+    for (int l = 0; l < t->lmax; l++) {
+        std::vector<CoarsenRec> cr(n);
+        std::vector<ToF32Rec> fr(n);
+        std::vector<int> uc(n, 0), uf(n, 0);
+        for (int k = 0; k < n; k++) {
+            impec_ctx *ctx = e->m[k];
+            if (l >= ctx->nlev) continue;
+            Level &L = ctx->lev[l];
+            if (l >= 1) {   // build_hierarchy
+                const Level &F = ctx->lev[l - 1];
+                const double *wf = l == 1 ? ctx->w + (long)ctx->B * ctx->lds + ctx->B : F.w + F.ld + 1;
+                const int ldf = l == 1 ? ctx->lds : F.ld;
+                const dim3 g = grid2(L.n1 + 2, L.n2 + 2, blk);
+                cr[k] = CoarsenRec{wf, ldf, F.n1, F.n2, L.w + L.ld + 1, L.ld, L.n1, L.n2, (int)g.x, (int)g.y};
+                uc[k] = g.x * g.y;
+            }
+            if (t->mixed) {   // mixed_setup's copies
+                LevArgs la = lev_args(ctx, l);
+                const dim3 g = grid2(L.n1, L.n2, blk);
+                fr[k] = ToF32Rec{la.w1, la.ldw, L.wf + L.ld + 1, L.ld, L.n1, L.n2, (int)g.x, (int)g.y};
+                uf[k] = g.x * g.y;
+            }
+        }
+        ENS_CHECK(e, batch_upload(t->coarsen[l], cr, uc));
+        ENS_CHECK(e, batch_upload(t->tof32[l], fr, uf));
+    }
+    std::vector<FacesRec> fa(n);
+    std::vector<ZeroRec> zr(n);
+    std::vector<CgInitRec> ir(n);
+    std::vector<FinishRec> fi(n), fd(n), frr(n), ft(n);
+    std::vector<RhoRec> rh(n);
+    std::vector<Upd2Rec> up(n);
+    std::vector<CtlRec> ct(n);
+    std::vector<PinRec> pr(n);
+    std::vector<ShiftRec> sh(n);
+    std::vector<TrueRec> tr(n);
+    std::vector<int> u_fa(n), u_red(n), u_vec(n), one(n, 1);
+    std::vector<double> init(S_COUNT);
+    for (int k = 0; k < n; k++) {
+        impec_ctx *ctx = e->m[k];
+        const Level &L0 = ctx->lev[0];
+        LevArgs a0 = lev_args(ctx, 0);
+        const dim3 gf = grid2(ctx->N1 + 1, ctx->N2 + 1, blk);
+        fa[k] = FacesRec{a0, ctx->fx, ctx->fy, (int)gf.x, (int)gf.y};
+        u_fa[k] = IMPEC_NO_FACES ? 0 : gf.x * gf.y;
+        if (!IMPEC_NO_FACES) {
+            a0.fx = ctx->fx;
+            a0.fy = ctx->fy;
+        }
+        const CgArgs c = cg_args(ctx);
+        const int nb = red_grid(c.n);
+        const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
+        zr[k] = ZeroRec{c, ctx->p, ctx->scal, nb, 1};
+        ir[k] = CgInitRec{a0, c, ctx->p, ctx->q, ctx->pr, ctx->part, ctx->scal, nb, 1};
+        u_red[k] = nb;
+        u_vec[k] = vg.nblocks;
+        fi[k] = FinishRec{ctx->part, ctx->scal, nb, 2, (int)S_RR, (int)RED_SUM};
+        fd[k] = FinishRec{ctx->part, ctx->scal, vg.nblocks, 1, (int)S_DQ, (int)RED_SUM};
+        frr[k] = FinishRec{ctx->part, ctx->scal, vg.nblocks, 2, (int)S_RR, (int)RED_SUM};
+        ft[k] = FinishRec{ctx->part, ctx->scal, nb, 1, (int)S_TT, (int)RED_SUM};
+        const int nwave0 = t->mixed ? fused_args<float>(ctx, 0).nwave : fused_args<double>(ctx, 0).nwave;
+        rh[k] = RhoRec{ctx->part, nwave0, 1.0 / (double)c.n, ctx->scal};
+        up[k] = Upd2Rec{c, ctx->p, ctx->pd, ctx->pz, ctx->pr, ctx->pq, ctx->scal, ctx->part, (int)vg.grid.x,
+                        (int)vg.grid.y};
+        ct[k] = CtlRec{ctx->scal, ctx->P.initial_guess == IMPEC_GUESS_ZERO};
+        pr[k] = PinRec{c, ctx->p, ctx->scal};
+        sh[k] = ShiftRec{c, ctx->p, ctx->scal, nb, 1};
+        tr[k] = TrueRec{a0, c, ctx->p, ctx->q, ctx->part, nb, 1};
+        // loop constants of solve_cg
+        const double tol = ctx->P.eps, bb = ctx->bnorm * ctx->bnorm;
+        double v[4];
+        v[0] = tol * tol * bb;   // S_ETOL
+        v[1] = ctx->P.max_iterations > 0 ? ctx->P.max_iterations : 10000;
+        ENS_CHECK(e, hipMemcpy(ctx->scal + S_ETOL, v, 2 * sizeof(double), hipMemcpyHostToDevice));
+        const double zp = (ctx->P.initial_guess == IMPEC_GUESS_PREVIOUS && ctx->have_prev_p) ? 0. : 1.;
+        ENS_CHECK(e, hipMemcpy(ctx->scal + S_ZEROP, &zp, sizeof(double), hipMemcpyHostToDevice));
+    }
+    ENS_CHECK(e, batch_upload(t->faces, fa, u_fa));
+    ENS_CHECK(e, batch_upload(t->zero, zr, u_red));
+    ENS_CHECK(e, batch_upload(t->init, ir, u_red));
+    ENS_CHECK(e, batch_upload(t->fin_init, fi, one));
+    ENS_CHECK(e, batch_upload(t->fin_dq, fd, one));
+    ENS_CHECK(e, batch_upload(t->fin_rr, frr, one));
+    ENS_CHECK(e, batch_upload(t->fin_tt, ft, one));
+    ENS_CHECK(e, batch_upload(t->rho, rh, one));
+    ENS_CHECK(e, batch_upload(t->upd, up, u_vec));
+    ENS_CHECK(e, batch_upload(t->ctl, ct, one));
+    ENS_CHECK(e, batch_upload(t->pin, pr, one));
+    ENS_CHECK(e, batch_upload(t->shift, sh, u_red));
+    ENS_CHECK(e, batch_upload(t->tres, tr, u_red));
+    int st = t->mixed ? build_legs<float>(e, t->l32) : build_legs<double>(e, t->l64);
+    if (st) return st;
+    ENS_CHECK(e, hipDeviceSynchronize());
+    return IMPEC_OK;
+}
+
+template <class E, class... A>
+static hipError_t launch_b(void (*k)(const E *, const int *, int, A...), const Batch<E> &b, dim3 blk,
+                           hipStream_t s, A... a) {
+    if (b.nmem == 0) return hipSuccess;
+    hipLaunchKernelGGL(k, dim3(b.total), blk, 0, s, b.tab, b.start, b.nmem, a...);
+    return hipGetLastError();
+}
+template <class E>
+static hipError_t launch_m(void (*k)(const E *, int), const Batch<E> &b, dim3 blk, hipStream_t s, int a) {
+    if (b.nmem == 0) return hipSuccess;
+    hipLaunchKernelGGL(k, dim3(b.nmem), blk, 0, s, b.tab, a);
+    return hipGetLastError();
+}
+template <class E>
+static hipError_t launch_t(void (*k)(const E *, int), const Batch<E> &b, hipStream_t s) {
+    if (b.nmem == 0) return hipSuccess;
+    hipLaunchKernelGGL(k, dim3((b.nmem + 63) / 64), dim3(64), 0, s, b.tab, b.nmem);
+    return hipGetLastError();
+}
+
+// the legs of one V-cycle of every member in state `need` (-1: all)
+template <typename T>
+static int ens_vcycle(impec_ensemble *e, const LegTabs<T> &t, int need) {
+    const dim3 blk(64 * SW_WAVES);
+    const int lmax = e->pres->lmax;
+    hipStream_t s = e->stream;
+    for (int l = 0; l + 1 < lmax; l++) {
+        const Batch<FusedArgsT<T>> &b = t.down[l];
+        if (b.nmem == 0) continue;
+        const dim3 g((b.total + SW_WAVES - 1) / SW_WAVES);
+        if (l == 0) hipLaunchKernelGGL((kb_mg_down<true, T>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
+        else hipLaunchKernelGGL((kb_mg_down<false, T>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
+    }
+    ENS_CHECK(e, launch_m(kb_coarsest<T>, t.coarsest, dim3(64), s, need));
+    for (int l = lmax - 2; l >= 0; l--) {
+        const Batch<FusedArgsT<T>> &b = t.up[l];
+        if (b.nmem == 0) continue;
+        const dim3 g((b.total + SW_WAVES - 1) / SW_WAVES);
+        if (l == 0) hipLaunchKernelGGL((kb_mg_up<true, T>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
+        else hipLaunchKernelGGL((kb_mg_up<false, T>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
+    }
+    ENS_CHECK(e, hipGetLastError());
+    return IMPEC_OK;
+}
+
+int ens_solve_begin(impec_ensemble *e) {
+    EnsPressTabs *t = e->pres;
+    hipStream_t s = e->stream;
+    const dim3 blk(64, 4);
+    for (int l = 1; l < t->lmax; l++) ENS_CHECK(e, launch_b(kb_coarsen_w, t->coarsen[l], blk, s));
+    if (t->mixed)
+        for (int l = 0; l < t->lmax; l++) ENS_CHECK(e, launch_b(kb_to_f32, t->tof32[l], blk, s));
+    ENS_CHECK(e, launch_b(kb_faces, t->faces, blk, s));
+    ENS_CHECK(e, launch_b(kb_zero_p, t->zero, dim3(256), s));
+    ENS_CHECK(e, launch_b(kb_cg_init, t->init, dim3(256), s, -1));
+    ENS_CHECK(e, ens_finish(s, t->fin_init, -1));
+    ENS_CHECK(e, launch_t(kb_ctl_first, t->ctl, s));
+    return IMPEC_OK;
+}
+
+template <typename T>
+static int ens_cg_slot_t(impec_ensemble *e, const LegTabs<T> &lt) {
+    EnsPressTabs *t = e->pres;
+    hipStream_t s = e->stream;
+    int st = ens_vcycle<T>(e, lt, EST_ITER);
+    if (st) return st;
+    ENS_CHECK(e, launch_m(kb_finish_rho, t->rho, dim3(1024), s, (int)EST_ITER));
+    ENS_CHECK(e, launch_b(kb_dir_matvec<T>, lt.dir, dim3(64, 4), s, (int)EST_ITER));
+    ENS_CHECK(e, ens_finish(s, t->fin_dq, EST_ITER));
+    ENS_CHECK(e, launch_b(kb_update2, t->upd, dim3(64, 4), s, (int)EST_ITER));
+    ENS_CHECK(e, ens_finish(s, t->fin_rr, EST_ITER));
+    ENS_CHECK(e, launch_t(kb_ctl_iter, t->ctl, s));
+    ENS_CHECK(e, launch_b(kb_cg_init, t->init, dim3(256), s, (int)EST_CHECK));
+    ENS_CHECK(e, ens_finish(s, t->fin_init, EST_CHECK));
+    ENS_CHECK(e, launch_t(kb_ctl_check, t->ctl, s));
+    return IMPEC_OK;
+}
+
+int ens_cg_slot(impec_ensemble *e) {
+    return e->pres->mixed ? ens_cg_slot_t<float>(e, e->pres->l32) : ens_cg_slot_t<double>(e, e->pres->l64);
+}
+
+int ens_solve_end(impec_ensemble *e) {
+    EnsPressTabs *t = e->pres;
+    hipStream_t s = e->stream;
+    ENS_CHECK(e, launch_t(kb_read_pin, t->pin, s));
+    ENS_CHECK(e, launch_b(kb_shift, t->shift, dim3(256), s));
+    ENS_CHECK(e, launch_b(kb_true_residual, t->tres, dim3(256), s));
+    ENS_CHECK(e, ens_finish(s, t->fin_tt, -1));
+    return IMPEC_OK;
+}
+
+int ens_vcycle_time(impec_ensemble *e, int reps, float *ms) {
+    hipEvent_t a, b;
+    ENS_CHECK(e, hipEventCreate(&a));
+    ENS_CHECK(e, hipEventCreate(&b));
+    int st = IMPEC_OK;
+    for (int r = 0; r < reps + 1 && !st; r++) {
+        if (r == 1) (void)hipEventRecord(a, e->stream);
+        st = e->pres->mixed ? ens_vcycle<float>(e, e->pres->l32, -1) : ens_vcycle<double>(e, e->pres->l64, -1);
+    }
+    (void)hipEventRecord(b, e->stream);
+    hipError_t he = hipEventSynchronize(b);
+    if (!st && he == hipSuccess) he = hipEventElapsedTime(ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (st) return st;
+    ENS_CHECK(e, he);
     return IMPEC_OK;
 }
 

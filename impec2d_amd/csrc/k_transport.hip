@@ -9,6 +9,7 @@
 // reference's in-place update because its fluxes are all formed before the
 // update loop.
 #include "impec_device.h"
+#include <vector>
 
 namespace impec {
 
@@ -62,11 +63,11 @@ __device__ __forceinline__ void hancock(double cm, double c0, double cp, double 
 }
 
 // X sweep (Advect_X): cells i in [0, N1), j in [-B, N2+B) are updated.
-template <bool MUSCL, bool PHI1>
-__global__ void k_sweep_x(SweepConst k, const double *__restrict__ cin, double *__restrict__ cout,
+template <bool MUSCL, bool PHI1, class G>
+__device__ __forceinline__ void sweep_x_body(const G &gr_, SweepConst k, const double *__restrict__ cin, double *__restrict__ cout,
                           const double *__restrict__ u) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x - k.B;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y - k.B;
+    const int j = gr_.bx() * blockDim.x + threadIdx.x - k.B;
+    const int i = gr_.by() * blockDim.y + threadIdx.y - k.B;
     if (i >= k.N1 + k.B || j >= k.N2 + k.B) return;
     const long o = (long)(i + k.B) * k.lds + (j + k.B);
     if (i < 0 || i >= k.N1) {
@@ -92,13 +93,18 @@ __global__ void k_sweep_x(SweepConst k, const double *__restrict__ cin, double *
     }
     cout[o] = cin[o] + k.dtdx * (Fi - Fip1);
 }
+template <bool MUSCL, bool PHI1>
+__global__ void k_sweep_x(SweepConst k, const double *__restrict__ cin, double *__restrict__ cout,
+                          const double *__restrict__ u) {
+    sweep_x_body<MUSCL, PHI1>(HwGrid{}, k, cin, cout, u);
+}
 
 // Y sweep (Advect_Y): cells i in [-B, N1+B), j in [0, N2) are updated.
-template <bool MUSCL, bool PHI1>
-__global__ void k_sweep_y(SweepConst k, const double *__restrict__ cin, double *__restrict__ cout,
+template <bool MUSCL, bool PHI1, class G>
+__device__ __forceinline__ void sweep_y_body(const G &gr_, SweepConst k, const double *__restrict__ cin, double *__restrict__ cout,
                           const double *__restrict__ v) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x - k.B;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y - k.B;
+    const int j = gr_.bx() * blockDim.x + threadIdx.x - k.B;
+    const int i = gr_.by() * blockDim.y + threadIdx.y - k.B;
     if (i >= k.N1 + k.B || j >= k.N2 + k.B) return;
     const long o = (long)(i + k.B) * k.lds + (j + k.B);
     if (j < 0 || j >= k.N2) {
@@ -122,6 +128,11 @@ __global__ void k_sweep_y(SweepConst k, const double *__restrict__ cin, double *
         Gjp1 = upwind<PHI1>(v[ov + 1], cin[o], cin[o + 1], k.phi);
     }
     cout[o] = cin[o] + k.dtdx * (Gj - Gjp1);
+}
+template <bool MUSCL, bool PHI1>
+__global__ void k_sweep_y(SweepConst k, const double *__restrict__ cin, double *__restrict__ cout,
+                          const double *__restrict__ v) {
+    sweep_y_body<MUSCL, PHI1>(HwGrid{}, k, cin, cout, v);
 }
 
 // The intercell fluxes alone, with the sweeps' arithmetic
@@ -173,13 +184,18 @@ __global__ void k_flux_y(SweepConst k, const double *__restrict__ c, const doubl
 }
 
 // c = 0.5 * (a + b) over the whole ghosted array (HyperbolicSplitSolver.cpp:97-101)
-__global__ void k_average(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c, int rows,
+template <class G>
+__device__ __forceinline__ void average_body(const G &gr_, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c, int rows,
                           int cols, int ld) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int j = gr_.bx() * blockDim.x + threadIdx.x;
+    const int i = gr_.by() * blockDim.y + threadIdx.y;
     if (i >= rows || j >= cols) return;
     const long o = (long)i * ld + j;
     c[o] = 0.5 * (a[o] + b[o]);
+}
+__global__ void k_average(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c, int rows,
+                          int cols, int ld) {
+    average_body(HwGrid{}, a, b, c, rows, cols, ld);
 }
 
 __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int rows, int cols, int ld) {
@@ -309,6 +325,128 @@ int advect_split(impec_ctx *ctx, double dt) {
     }
     }
     return IMPEC_OK;
+}
+
+
+// ===========================================================================
+// ensembles: the symmetric parallel splitting of AdvectSplit
+// (HyperbolicSplitSolver.cpp:51-106) for every member in one launch per
+// sweep, with each member's time step read from its S_DT
+// ===========================================================================
+#define ENS_CHECK(e, call)                                                                     \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return ens_set_error((e), IMPEC_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,   \
+                                 hipGetErrorString(e_));                                       \
+    } while (0)
+
+struct SweepRec {
+    SweepConst k;           // dt-dependent factors formed in the kernel
+    double h;               // dx (x sweeps) or dy (y sweeps)
+    const double *cin;
+    double *cout;
+    const double *vel;
+    const double *S;
+    int gx, gy;
+};
+struct AvgRec {
+    const double *a, *b;
+    double *c;
+    int rows, cols, ld, gx, gy;
+};
+
+struct EnsTransTabs {
+    int muscl, phi1;
+    Batch<SweepRec> x1, y1, y2, x2;   // c->t1 (x), t1->t2 (y), c->t1 (y), t1->t3 (x)
+    Batch<AvgRec> avg;
+};
+
+// sweep_const's factors from the member's step
+__device__ __forceinline__ SweepConst sweep_factors(const SweepRec &R) {
+    SweepConst k = R.k;
+    const double dt = R.S[S_DT];
+    k.hdtdx = 0.5 * dt / R.h;
+    k.dtdx = dt / R.h;
+    return k;
+}
+
+template <bool MUSCL, bool PHI1>
+__global__ void kb_sweep_x(const SweepRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    sweep_x_body<MUSCL, PHI1>(G, sweep_factors(R), R.cin, R.cout, R.vel);
+}
+template <bool MUSCL, bool PHI1>
+__global__ void kb_sweep_y(const SweepRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    sweep_y_body<MUSCL, PHI1>(G, sweep_factors(R), R.cin, R.cout, R.vel);
+}
+__global__ void kb_average(const AvgRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
+    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
+    average_body(G, R.a, R.b, R.c, R.rows, R.cols, R.ld);
+}
+
+void ens_trans_free(impec_ensemble *e) {
+    if (!e->trans) return;
+    EnsTransTabs *t = e->trans;
+    batch_free(t->x1); batch_free(t->y1); batch_free(t->y2); batch_free(t->x2); batch_free(t->avg);
+    delete t;
+    e->trans = nullptr;
+}
+
+int ens_trans_build(impec_ensemble *e) {
+    ens_trans_free(e);
+    EnsTransTabs *t = new EnsTransTabs();
+    e->trans = t;
+    const int n = e->n;
+    t->muscl = e->m[0]->P.hyper_method == IMPEC_HYPER_MUSCL;
+    t->phi1 = e->m[0]->P.porosity == 1.;
+    std::vector<SweepRec> x1(n), y1(n), y2(n), x2(n);
+    std::vector<AvgRec> av(n);
+    std::vector<int> u(n);
+    for (int k = 0; k < n; k++) {
+        impec_ctx *ctx = e->m[k];
+        const SweepConst base = sweep_const(ctx, 0., 1.);
+        const int gx = (ctx->N2 + 2 * ctx->B + 63) / 64, gy = (ctx->N1 + 2 * ctx->B + 3) / 4;
+        x1[k] = SweepRec{base, ctx->dx, ctx->c, ctx->t1, ctx->u, ctx->scal, gx, gy};
+        y1[k] = SweepRec{base, ctx->dy, ctx->t1, ctx->t2, ctx->v, ctx->scal, gx, gy};
+        y2[k] = SweepRec{base, ctx->dy, ctx->c, ctx->t1, ctx->v, ctx->scal, gx, gy};
+        x2[k] = SweepRec{base, ctx->dx, ctx->t1, ctx->t3, ctx->u, ctx->scal, gx, gy};
+        av[k] = AvgRec{ctx->t2, ctx->t3, ctx->c, ctx->N1 + 2 * ctx->B, ctx->N2 + 2 * ctx->B, ctx->lds, gx, gy};
+        u[k] = gx * gy;
+    }
+    ENS_CHECK(e, batch_upload(t->x1, x1, u));
+    ENS_CHECK(e, batch_upload(t->y1, y1, u));
+    ENS_CHECK(e, batch_upload(t->y2, y2, u));
+    ENS_CHECK(e, batch_upload(t->x2, x2, u));
+    ENS_CHECK(e, batch_upload(t->avg, av, u));
+    return IMPEC_OK;
+}
+
+template <bool MUSCL, bool PHI1>
+static int ens_sweeps(impec_ensemble *e) {
+    EnsTransTabs *t = e->trans;
+    const dim3 blk(64, 4);
+    hipStream_t s = e->stream;
+    const Batch<SweepRec> *order[4] = {&t->x1, &t->y1, &t->y2, &t->x2};
+    for (int q = 0; q < 4; q++) {
+        const Batch<SweepRec> &b = *order[q];
+        if (b.nmem == 0) continue;
+        if (q == 0 || q == 3)
+            hipLaunchKernelGGL((kb_sweep_x<MUSCL, PHI1>), dim3(b.total), blk, 0, s, b.tab, b.start, b.nmem);
+        else hipLaunchKernelGGL((kb_sweep_y<MUSCL, PHI1>), dim3(b.total), blk, 0, s, b.tab, b.start, b.nmem);
+    }
+    if (t->avg.nmem) hipLaunchKernelGGL(kb_average, dim3(t->avg.total), blk, 0, s, t->avg.tab, t->avg.start, t->avg.nmem);
+    ENS_CHECK(e, hipGetLastError());
+    return IMPEC_OK;
+}
+
+int ens_advect_parallel(impec_ensemble *e) {
+    const EnsTransTabs *t = e->trans;
+    if (t->muscl && t->phi1) return ens_sweeps<true, true>(e);
+    if (t->muscl) return ens_sweeps<true, false>(e);
+    if (t->phi1) return ens_sweeps<false, true>(e);
+    return ens_sweeps<false, false>(e);
 }
 
 }  // namespace impec

@@ -253,6 +253,34 @@ int impec_cnn_train(impec_cnn *net, int iters, int batch, float lr, double *loss
  * context keeps the previous pressure from then on */
 int impec_set_warmstart(impec_ctx *ctx, impec_cnn *net);
 
+/* ---- ensembles ------------------------------------------------------------ */
+/* No counterpart in the reference (one Impec2DSolver runs one problem,
+ * Impec2DSolver.cpp:293-466): `count` independent problems, each with its own
+ * parameters and size, advanced together -- every pass of an IMPEC step is one
+ * launch over all members, and a step replays captured HIP graphs.  Members
+ * need nu = 1, no dispersion, parallel splitting, no test mode, initial_guess
+ * zero or previous, isotropic K, and share precision, scheme and whether the
+ * porosity is 1.  impec_ensemble_member lends member k's context: set its
+ * source, permeability and concentration, and read its fields and
+ * diagnostics, with the usual calls (do not destroy it).  A member's fields
+ * after impec_ensemble_step equal, bit for bit, those impec_step computes on
+ * a context of the same parameters (mg_rows included); dt[k] is member k's
+ * time step (no output-time clipping) and its[k] its CG iterations. */
+typedef struct impec_ensemble impec_ensemble;
+int impec_ensemble_create(const impec_params *members, int count, impec_ensemble **out);
+int impec_ensemble_destroy(impec_ensemble *e);
+const char *impec_ensemble_last_error(const impec_ensemble *e);
+int impec_ensemble_member(impec_ensemble *e, int k, impec_ctx **ctx);
+/* graphs on (default) or off (eager launches); CG iterations per replay of
+ * the iteration graph (1..64, 0 keeps the current value, default 4) */
+int impec_ensemble_set_graphs(impec_ensemble *e, int on, int slots);
+/* picks the row height of the finest-level multigrid chunks (8, 16, 32 or 64)
+ * per member grid size by timing the ensemble's V-cycle `reps` times per
+ * candidate; writes each member's choice to rows[k] (may be NULL) and to its
+ * mg_rows */
+int impec_ensemble_autotune(impec_ensemble *e, int reps, int *rows);
+int impec_ensemble_step(impec_ensemble *e, double *dt, int *its);
+
 #ifdef __cplusplus
 }
 #endif

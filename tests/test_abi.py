@@ -84,3 +84,28 @@ def test_invalid_params_rejected():
     for kw in [dict(num_ghostcells=3), dict(N1=1), dict(nu=1.5), dict(source_type=2), dict(dim_splitting=7)]:
         p = capi.default_params(**kw)
         assert lib.impec_create(C.byref(p), C.byref(h)) == capi.IMPEC_EINVAL
+
+
+def test_ensemble_member_validation():
+    """impec_ensemble_create checks every member before touching a device:
+    members outside the batched path (nu < 1, dispersion, other splittings,
+    test mode, learned guess, mixed precisions or schemes) are refused, an
+    acceptable set without a device fails with IMPEC_ENODEV."""
+    import ctypes as C
+    lib = capi.load()
+    h = C.c_void_p()
+    good = dict(nu=1.0, dim_splitting=4)
+
+    def create(*members):
+        arr = (capi.Params * len(members))(*members)
+        return lib.impec_ensemble_create(arr, len(members), C.byref(h))
+
+    base = capi.default_params(**good)
+    for kw in [dict(nu=2 / 3.0), dict(diffusion=1), dict(dim_splitting=0), dict(is_test=1),
+               dict(initial_guess=capi.GUESS_LEARNED), dict(mg_variant=1)]:
+        assert create(base, capi.default_params(**dict(good, **kw))) == capi.IMPEC_EINVAL, kw
+    assert create(base, capi.default_params(**dict(good, precision=capi.PREC_MIXED))) == capi.IMPEC_EINVAL
+    assert create(base, capi.default_params(**dict(good, hyper_method=1))) == capi.IMPEC_EINVAL
+    assert lib.impec_ensemble_create(None, 0, C.byref(h)) == capi.IMPEC_EINVAL
+    if lib.impec_device_count() == 0:
+        assert create(base, capi.default_params(**dict(good, N1=128, N2=96))) == capi.IMPEC_ENODEV
