@@ -122,6 +122,91 @@ def modes(capi, n, warmup=3, steps=8):
     return res
 
 
+def corner_wells_source(n, I=1.0, extent=2 ** 0.5):
+    """Reference source_type "parallel" on the rotated square: injectors at
+    cells (0,0) and (N-1,N-1), producers at (0,N-1) and (N-1,0), rate I each
+    (Util.cpp:565-573)."""
+    import numpy as np
+    h = extent / n
+    val = I / (h * h)
+    q = np.zeros(n * n)
+    q[0] = val
+    q[n * n - 1] = val
+    q[n - 1] = -val
+    q[(n - 1) * n] = -val
+    return q
+
+
+def ensemble(capi, members=256, warmup=2, steps=10, loop_steps=1, seed=2024):
+    """Batched ensemble: `members` realisations with sizes drawn uniformly from
+    {128, 256, 512, 1024}^2, each with its own lognormal K (sigma uniform in
+    [0.5, 2], correlation length 0.05, own seed) and mobility ratio M uniform in
+    [2, 50], the two grids mixed, no dispersion, FP32 V-cycles in the FP64 CG,
+    previous-pressure guess, rel-res 1e-10.  The finest-level chunk heights are
+    autotuned first.  Reports the ensemble's step time with batched launches
+    and graph replays, and the time of the same members stepped one after
+    another on their own contexts."""
+    import numpy as np
+    from impec2d_amd import fields
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([128, 256, 512, 1024], members)
+    sigmas = rng.uniform(0.5, 2.0, members)
+    Ms = rng.uniform(2.0, 50.0, members)
+    grids = rng.integers(0, 2, members)   # 0: unit square, injector/producer; 1: rotated, corner wells
+    plist = []
+    for k in range(members):
+        n, ext = int(sizes[k]), (1.0 if grids[k] == 0 else 2 ** 0.5)
+        plist.append(capi.default_params(
+            N1=n, N2=n, source_type=capi.SRC_DIAGONAL if grids[k] == 0 else capi.SRC_PARALLEL,
+            average_procedure=capi.AVG_HARMONIC, dim_splitting=capi.SPLIT_PARALLEL, hyper_method=capi.HYPER_MUSCL,
+            diffusion=0, x_lb=0.0, x_ub=ext, y_lb=0.0, y_ub=ext, nu=1.0, mu_res=1.0, mu_inj=1.0 / Ms[k],
+            porosity=1.0, cfl=0.5, inj_fluid_conc=1.0, eps=1e-10, I=1.0, precision=capi.PREC_MIXED,
+            initial_guess=capi.GUESS_PREVIOUS))
+    t0 = time.perf_counter()
+    ens = capi.Ensemble(plist)
+    for k in range(members):
+        n, ext = int(sizes[k]), (1.0 if grids[k] == 0 else 2 ** 0.5)
+        m = ens.members[k]
+        m.set_permeability(fields.lognormal_permeability(n, n, sigma=sigmas[k], corr_len=0.05, seed=1000 + k,
+                                                         x_extent=ext, y_extent=ext), None)
+        m.set_source(one_cell_source(n) if grids[k] == 0 else corner_wells_source(n))
+    setup_s = time.perf_counter() - t0
+    rows = ens.autotune(reps=2)
+    cells = int(np.sum(sizes.astype(np.int64) ** 2))
+    for _ in range(warmup):
+        ens.step()
+    its_total = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, its = ens.step()
+        its_total += int(its.sum())
+    el = time.perf_counter() - t0
+    # the same members, one context after another (no batching, no graphs)
+    tl = [0.0] * members
+    t1 = time.perf_counter()
+    for _ in range(loop_steps):
+        for k, m in enumerate(ens.members):
+            tl[k] += m.step(tl[k])
+    ens.members[0].synchronize()
+    el_loop = time.perf_counter() - t1
+    ens.close()
+    tuned = {}
+    for k in range(members):
+        tuned[str(int(sizes[k]))] = int(rows[k])
+    return {
+        "workload": "%d realisations, sizes uniform from {128,256,512,1024}^2, lognormal K (sigma U[0.5,2], "
+                    "corr. length 0.05), M U[2,50], both grids, no dispersion, mixed precision, previous-p "
+                    "guess, rel-res 1e-10; %d timed steps after %d warm-up" % (members, steps, warmup),
+        "members": members, "cells": cells,
+        "members_by_size": {str(s): int(np.sum(sizes == s)) for s in (128, 256, 512, 1024)},
+        "ms_per_step": 1e3 * el / steps, "cell_updates_per_s": cells * steps / el,
+        "cg_iterations_per_member_step": its_total / (steps * members),
+        "per_member_loop_ms_per_step": 1e3 * el_loop / loop_steps,
+        "speedup_vs_per_member_loop": (el_loop / loop_steps) / (el / steps),
+        "autotuned_rows_by_size": tuned, "setup_s": setup_s,
+    }
+
+
 def cpu_baseline(n=1024, steps=8):
     """Times the C oracle (FP64 restatement of the reference algorithm,
     one thread) on `steps` steps of the same workload at n^2."""
@@ -157,6 +242,11 @@ def main():
     ap.add_argument("--no-modes", action="store_true",
                     help="skip the plain / mixed / warm-start comparison on the lognormal-K workload")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the batched-ensemble workload (256 realisations of mixed sizes)")
+    ap.add_argument("--ensemble-members", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--ensemble-warmup", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -209,6 +299,9 @@ def main():
     ctx.close()
     if not args.no_modes:
         out["modes"] = modes(capi, n)
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble(capi, members=args.ensemble_members, warmup=args.ensemble_warmup,
+                                   steps=args.ensemble_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

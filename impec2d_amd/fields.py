@@ -11,23 +11,43 @@ Gaussian draw scaled by their square roots yields two independent fields, of
 which the real part is used.  Host-side set-up only; numpy's PCG64 generator
 with the given seed makes the field reproducible.
 """
+import functools
+
 import numpy as np
+
+try:
+    import scipy.fft as _fft
+    _FFT_KW = {"workers": -1}
+except ImportError:   # numpy's FFT gives the same field to rounding
+    _fft = np.fft
+    _FFT_KW = {}
+
+
+@functools.lru_cache(maxsize=16)
+def _embedding_scale(m1, m2, dx, dy, corr_len):
+    """sqrt(eigenvalues / (m1 m2)) of the circulant embedding of the covariance
+    on the periodic m1 x m2 grid (negative eigenvalues clipped to zero)."""
+    i = np.arange(m1)
+    j = np.arange(m2)
+    hx = np.minimum(i, m1 - i) * dx
+    hy = np.minimum(j, m2 - j) * dy
+    cov = np.exp(-((hx[:, None] ** 2 + hy[None, :] ** 2) / corr_len ** 2))
+    lam = _fft.fft2(cov, **_FFT_KW).real
+    lam = np.maximum(lam, 0.0)
+    out = np.sqrt(lam / (m1 * m2))
+    out.setflags(write=False)
+    return out
 
 
 def gaussian_random_field(n1, n2, dx, dy, corr_len, seed):
     """Zero-mean, unit-variance GRF with covariance exp(-(h/corr_len)^2) on an
     n1 x n2 grid of spacing (dx, dy)."""
     m1, m2 = 2 * n1, 2 * n2
-    i = np.arange(m1)
-    j = np.arange(m2)
-    hx = np.minimum(i, m1 - i) * dx
-    hy = np.minimum(j, m2 - j) * dy
-    cov = np.exp(-((hx[:, None] ** 2 + hy[None, :] ** 2) / corr_len ** 2))
-    lam = np.fft.fft2(cov).real
-    lam = np.maximum(lam, 0.0)
+    scale = _embedding_scale(m1, m2, float(dx), float(dy), float(corr_len))
     rng = np.random.default_rng(seed)
     xi = rng.standard_normal((m1, m2)) + 1j * rng.standard_normal((m1, m2))
-    f = np.fft.fft2(np.sqrt(lam / (m1 * m2)) * xi)
+    xi *= scale
+    f = _fft.fft2(xi, overwrite_x=True, **_FFT_KW) if _FFT_KW else _fft.fft2(xi)
     return np.ascontiguousarray(f.real[:n1, :n2])
 
 
