@@ -137,7 +137,7 @@ def corner_wells_source(n, I=1.0, extent=2 ** 0.5):
     return q
 
 
-def ensemble(capi, members=256, warmup=2, steps=10, loop_steps=1, seed=2024):
+def ensemble(capi, members=256, warmup=2, steps=10, loop_steps=1, seed=2024, graphs=True):
     """Batched ensemble: `members` realisations with sizes drawn uniformly from
     {128, 256, 512, 1024}^2, each with its own lognormal K (sigma uniform in
     [0.5, 2], correlation length 0.05, own seed) and mobility ratio M uniform in
@@ -171,6 +171,8 @@ def ensemble(capi, members=256, warmup=2, steps=10, loop_steps=1, seed=2024):
                                                          x_extent=ext, y_extent=ext), None)
         m.set_source(one_cell_source(n) if grids[k] == 0 else corner_wells_source(n))
     setup_s = time.perf_counter() - t0
+    if not graphs:
+        ens.set_graphs(False)
     rows = ens.autotune(reps=2)
     cells = int(np.sum(sizes.astype(np.int64) ** 2))
     for _ in range(warmup):
@@ -203,7 +205,7 @@ def ensemble(capi, members=256, warmup=2, steps=10, loop_steps=1, seed=2024):
         "cg_iterations_per_member_step": its_total / (steps * members),
         "per_member_loop_ms_per_step": 1e3 * el_loop / loop_steps,
         "speedup_vs_per_member_loop": (el_loop / loop_steps) / (el / steps),
-        "autotuned_rows_by_size": tuned, "setup_s": setup_s,
+        "autotuned_rows_by_size": tuned, "setup_s": setup_s, "graphs": bool(graphs),
     }
 
 
@@ -247,6 +249,8 @@ def main():
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--ensemble-warmup", type=int, default=2)
+    ap.add_argument("--ensemble-eager", action="store_true",
+                    help="launch the ensemble's passes directly instead of replaying captured graphs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -301,7 +305,7 @@ def main():
         out["modes"] = modes(capi, n)
     if not args.no_ensemble:
         out["ensemble"] = ensemble(capi, members=args.ensemble_members, warmup=args.ensemble_warmup,
-                                   steps=args.ensemble_steps)
+                                   steps=args.ensemble_steps, graphs=not args.ensemble_eager)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

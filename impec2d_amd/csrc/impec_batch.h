@@ -84,11 +84,69 @@ inline hipError_t batch_upload(Batch<E> &b, const std::vector<E> &recs, const st
 
 }  // namespace impec
 
-// Prologue of a batched kernel over workgroups: member m_ of flat workgroup
-// blockIdx.x, its record R (which carries the member's own grid gx x gy) and
-// the workgroup's position G in that grid.
-#define IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)                                  \
-    const int m_ = ::impec::member_of((start), (nmem), (int)blockIdx.x);           \
-    const auto R = (tab)[m_];                                                      \
-    const int loc_ = (int)blockIdx.x - (start)[m_];                                \
-    const ::impec::VGrid G{loc_ % R.gx, loc_ / R.gx, R.gx, R.gy};
+namespace impec {
+
+// Workgroup loop of a batched kernel launched with at most as many
+// workgroups as units: workgroup g takes the contiguous units
+// [g*per, (g+1)*per), finds its first member once and steps to the next member
+// when it crosses one (every listed member has at least one unit), so the
+// member search is not repeated per unit.  f(R, G) runs the body for record R
+// at position G of that member's own grid; a barrier separates the units, so
+// bodies may use LDS.
+template <class E, class F>
+__device__ __forceinline__ void batch_for(const E *__restrict__ tab, const int *__restrict__ start, int nmem, F &&f) {
+    const int total = start[nmem];
+    const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    int vb = (int)blockIdx.x * per;
+    const int end = vb + per < total ? vb + per : total;
+    if (vb >= end) return;
+    int k = member_of(start, nmem, vb);
+    int kend = start[k + 1];
+    for (; vb < end; vb++) {
+        if (vb >= kend) {
+            k++;
+            kend = start[k + 1];
+        }
+        const E R = tab[k];
+        const int loc = vb - start[k];
+        f(R, VGrid{loc % R.gx, loc / R.gx, R.gx, R.gy});
+        __syncthreads();
+    }
+}
+
+// The members of a batch that are in a given loop state, compacted on the
+// device: the same loop over the listed members only.
+struct ActList {
+    int *member;     // batch-local member ids, ascending
+    int *start;      // first unit of each listed member; start[n] = all listed units
+    int *n;          // number of listed members
+};
+
+template <class E, class F>
+__device__ __forceinline__ void act_for(const E *__restrict__ tab, const ActList &A, F &&f) {
+    const int nact = A.n[0];
+    if (nact == 0) return;
+    const int total = A.start[nact];
+    const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    int vb = (int)blockIdx.x * per;
+    const int end = vb + per < total ? vb + per : total;
+    if (vb >= end) return;
+    int k = member_of(A.start, nact, vb);
+    int kend = A.start[k + 1];
+    for (; vb < end; vb++) {
+        if (vb >= kend) {
+            k++;
+            kend = A.start[k + 1];
+        }
+        const E R = tab[A.member[k]];
+        const int loc = vb - A.start[k];
+        f(R, VGrid{loc % R.gx, loc / R.gx, R.gx, R.gy});
+        __syncthreads();
+    }
+}
+
+// workgroups of a batched launch over `units` units
+constexpr int BATCH_WG = 4096;
+inline int batch_grid(int units) { return units < BATCH_WG ? units : BATCH_WG; }
+
+}  // namespace impec

@@ -605,24 +605,29 @@ struct EnsFieldTabs {
 };
 
 __global__ void kb_fill_scalar(const FillRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    fill_scalar_body(G, R.a, R.N1, R.N2, R.B, R.ld, 0);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        fill_scalar_body(G, R.a, R.N1, R.N2, R.B, R.ld, 0);
+    });
 }
 __global__ void kb_mobility(const MobRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    mobility_body(G, R.c, R.K11, R.K22, R.w, R.w22, R.rows, R.cols, R.ld, R.mob1over4, R.mu_res, R.harmonic);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        mobility_body(G, R.c, R.K11, R.K22, R.w, R.w22, R.rows, R.cols, R.ld, R.mob1over4, R.mu_res, R.harmonic);
+    });
 }
 __global__ void kb_velocity(const VelRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    velocity_body(G, R.k, R.p, R.w11, R.w22, R.u, R.v);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        velocity_body(G, R.k, R.p, R.w11, R.w22, R.u, R.v);
+    });
 }
 __global__ void kb_fill_velocity(const FillVRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    fill_velocity_body(G, R.u, R.v, R.N1, R.N2, R.B, R.ld);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        fill_velocity_body(G, R.u, R.v, R.N1, R.N2, R.B, R.ld);
+    });
 }
 __global__ void kb_cfl(const CflRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    cfl_body(G, R.k, R.u, R.v, R.part);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        cfl_body(G, R.k, R.u, R.v, R.part);
+    });
 }
 // MaxStableTimestepCombined without dispersion limit (HyperbolicSolver.cpp:123-198),
 // the expressions of launch_cfl
@@ -638,16 +643,17 @@ __global__ void kb_dt(const DtRec *__restrict__ tab, int n) {
 }
 // TRIntegrator over half the member's step (HyperbolicSplitSolver.cpp:162-198)
 __global__ void kb_tr(const TrRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    const double dt = 0.5 * R.S[S_DT];
-    const double hdt = 0.5 * dt;
-    tr_body(G, R.c, R.idx, R.qv, R.nsrc, R.N2, R.B, R.ld, dt, hdt, R.inj, R.S, R.md);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        const double dt = 0.5 * R.S[S_DT];
+        const double hdt = 0.5 * dt;
+        tr_body(G, R.c, R.idx, R.qv, R.nsrc, R.N2, R.B, R.ld, dt, hdt, R.inj, R.S, R.md);
+    });
 }
 
 template <class E>
 static hipError_t launch_batch(void (*k)(const E *, const int *, int), const Batch<E> &b, dim3 blk, hipStream_t s) {
     if (b.nmem == 0) return hipSuccess;
-    hipLaunchKernelGGL(k, dim3(b.total), blk, 0, s, b.tab, b.start, b.nmem);
+    hipLaunchKernelGGL(k, dim3(batch_grid(b.total)), blk, 0, s, b.tab, b.start, b.nmem);
     return hipGetLastError();
 }
 

@@ -1923,9 +1923,25 @@ struct LegTabs {
     }
 };
 
+// The members of a batch that are in a given loop state (ActList), compacted
+// on the device after every state change, so that the iteration kernels visit
+// only the workgroups (waves) of members still iterating.
+struct ListDesc {
+    double *const *S;    // scalar arrays of the batch's members
+    const int *bstart;   // the batch's unit offsets
+    int nmem;
+    ActList act;
+};
+
 struct EnsPressTabs {
     int mixed;
     int lmax;                           // most levels of any member
+    // compacted lists: iter[0] the level-0 vector kernels, iter[1 + l] the legs
+    // of level l; check: the residual recomputation
+    ListDesc *d_iter = nullptr, *d_check = nullptr;
+    int n_iter = 0;
+    std::vector<ListDesc> h_iter, h_check;
+    std::vector<void *> list_mem;
     Batch<FacesRec> faces;
     Batch<CoarsenRec> coarsen[MAX_LEVELS];
     Batch<ToF32Rec> tof32[MAX_LEVELS];
@@ -1945,26 +1961,31 @@ struct EnsPressTabs {
 __device__ __forceinline__ bool in_state(const double *S, int need) { return need < 0 || S[S_EST] == (double)need; }
 
 __global__ void kb_faces(const FacesRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    faces_body(G, R.L, R.fx, R.fy);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        faces_body(G, R.L, R.fx, R.fy);
+    });
 }
 __global__ void kb_coarsen_w(const CoarsenRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    coarsen_w_body(G, R.wf, R.ldf, R.n1, R.n2, R.wc, R.ldc, R.m1, R.m2);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        coarsen_w_body(G, R.wf, R.ldf, R.n1, R.n2, R.wc, R.ldc, R.m1, R.m2);
+    });
 }
 __global__ void kb_to_f32(const ToF32Rec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    to_f32_body(G, R.src, R.lds, R.dst, R.ldd, R.rows, R.cols);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        to_f32_body(G, R.src, R.lds, R.dst, R.ldd, R.rows, R.cols);
+    });
 }
 __global__ void kb_zero_p(const ZeroRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    if (R.S[S_ZEROP] == 0.) return;
-    zero_interior_body(G, R.c, R.p);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        if (R.S[S_ZEROP] == 0.) return;
+        zero_interior_body(G, R.c, R.p);
+    });
 }
 __global__ void kb_cg_init(const CgInitRec *__restrict__ tab, const int *__restrict__ start, int nmem, int need) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    if (!in_state(R.S, need)) return;
-    cg_init_body<false>(G, R.L, R.c, R.p, R.q, R.r, R.part);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        if (!in_state(R.S, need)) return;
+        cg_init_body<false>(G, R.L, R.c, R.p, R.q, R.r, R.part);
+    });
 }
 template <bool FINE0, typename T>
 __global__ void __launch_bounds__(64 * SW_WAVES)
@@ -2001,17 +2022,19 @@ __global__ void __launch_bounds__(1024) kb_finish_rho(const RhoRec *__restrict__
 // (d0, d1) on even iterations and (d1, d0) on odd ones, as in solve_cg
 template <typename T>
 __global__ void kb_dir_matvec(const DirRec<T> *__restrict__ tab, const int *__restrict__ start, int nmem, int need) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    if (!in_state(R.S, need)) return;
-    const bool odd = ((long long)R.S[S_EITS]) & 1;
-    const int first = R.S[S_ESINCE] == 0.;
-    cg_dir_matvec_body<T>(G, R.L, R.c, R.x0, odd ? R.d1 : R.d0, odd ? R.d0 : R.d1, R.q, R.S, first, R.part);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        if (!in_state(R.S, need)) return;
+        const bool odd = ((long long)R.S[S_EITS]) & 1;
+        const int first = R.S[S_ESINCE] == 0.;
+        cg_dir_matvec_body<T>(G, R.L, R.c, R.x0, odd ? R.d1 : R.d0, odd ? R.d0 : R.d1, R.q, R.S, first, R.part);
+    });
 }
 __global__ void kb_update2(const Upd2Rec *__restrict__ tab, const int *__restrict__ start, int nmem, int need) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    if (!in_state(R.S, need)) return;
-    const bool odd = ((long long)R.S[S_EITS]) & 1;
-    cg_update2_body(G, R.c, R.p, odd ? R.d0 : R.d1, R.r, R.q, R.S, R.part);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        if (!in_state(R.S, need)) return;
+        const bool odd = ((long long)R.S[S_EITS]) & 1;
+        cg_update2_body(G, R.c, R.p, odd ? R.d0 : R.d1, R.r, R.q, R.S, R.part);
+    });
 }
 // after the first residual: iterate unless converged (solve_cg's loop entry)
 __global__ void kb_ctl_first(const CtlRec *__restrict__ tab, int n) {
@@ -2065,13 +2088,122 @@ __global__ void kb_read_pin(const PinRec *__restrict__ tab, int n) {
     read_pin_body(VGrid{0, 0, 1, 1}, R.c, R.p, R.S);
 }
 __global__ void kb_shift(const ShiftRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    shift_body(G, R.c, R.p, R.S);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        shift_body(G, R.c, R.p, R.S);
+    });
 }
 __global__ void kb_true_residual(const TrueRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    true_residual_body<false>(G, R.L, R.c, R.p, R.q, R.part);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        true_residual_body<false>(G, R.L, R.c, R.p, R.q, R.part);
+    });
 }
+
+// one list per workgroup: flags of the members in state `need`, their units,
+// block-wide inclusive scans, compacted output
+__global__ void __launch_bounds__(256) kb_compact(const ListDesc *__restrict__ lists, int need) {
+    const ListDesc d = lists[blockIdx.x];
+    if (d.nmem == 0) return;
+    const int t = threadIdx.x;
+    __shared__ int sf[256], su[256];
+    __shared__ int base_n, base_u;
+    if (t == 0) {
+        base_n = 0;
+        base_u = 0;
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < d.nmem; c0 += 256) {
+        const int m = c0 + t;
+        int f = 0, u = 0;
+        if (m < d.nmem && d.S[m][S_EST] == (double)need) {
+            f = 1;
+            u = d.bstart[m + 1] - d.bstart[m];
+        }
+        sf[t] = f;
+        su[t] = u;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int a = t >= off ? sf[t - off] : 0, b = t >= off ? su[t - off] : 0;
+            __syncthreads();
+            sf[t] += a;
+            su[t] += b;
+            __syncthreads();
+        }
+        const int bn = base_n, bu = base_u;
+        if (f) {
+            const int pos = bn + sf[t] - 1;
+            d.act.member[pos] = m;
+            d.act.start[pos] = bu + su[t] - u;
+        }
+        __syncthreads();
+        if (t == 255) {
+            base_n = bn + sf[255];
+            base_u = bu + su[255];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        d.act.start[base_n] = base_u;
+        d.act.n[0] = base_n;
+    }
+}
+
+
+// per-wave version of act_for for the streaming legs: wave w of the launch
+// takes a contiguous range of the listed members' waves
+template <class F>
+__device__ __forceinline__ void act_wave_for(const ActList &A, F &&f) {
+    const int nact = A.n[0];
+    if (nact == 0) return;
+    const int total = A.start[nact], parts = (int)gridDim.x * SW_WAVES;
+    const int per = (total + parts - 1) / parts;
+    int vw = hw_wave() * per;
+    const int end = vw + per < total ? vw + per : total;
+    if (vw >= end) return;
+    int k = member_of(A.start, nact, vw);
+    int kend = A.start[k + 1];
+    for (; vw < end; vw++) {
+        if (vw >= kend) {
+            k++;
+            kend = A.start[k + 1];
+        }
+        f(A.member[k], vw - A.start[k]);
+    }
+}
+
+template <bool FINE0, typename T>
+__global__ void __launch_bounds__(64 * SW_WAVES) kp_mg_down(const FusedArgsT<T> *__restrict__ tab, ActList act) {
+    act_wave_for(act, [&](int m, int loc) {
+        const FusedArgsT<T> A = tab[m];
+        mg_down_body<FINE0, false, T>(A, loc);
+    });
+}
+template <bool FINE0, typename T>
+__global__ void __launch_bounds__(64 * SW_WAVES) kp_mg_up(const FusedArgsT<T> *__restrict__ tab, ActList act) {
+    act_wave_for(act, [&](int m, int loc) {
+        const FusedArgsT<T> A = tab[m];
+        mg_up_body<FINE0, false, T>(A, loc);
+    });
+}
+template <typename T>
+__global__ void kp_dir_matvec(const DirRec<T> *__restrict__ tab, ActList act) {
+    act_for(tab, act, [&](const DirRec<T> &R, const VGrid &G) {
+        const bool odd = ((long long)R.S[S_EITS]) & 1;
+        const int first = R.S[S_ESINCE] == 0.;
+        cg_dir_matvec_body<T>(G, R.L, R.c, R.x0, odd ? R.d1 : R.d0, odd ? R.d0 : R.d1, R.q, R.S, first, R.part);
+    });
+}
+__global__ void kp_update2(const Upd2Rec *__restrict__ tab, ActList act) {
+    act_for(tab, act, [&](const Upd2Rec &R, const VGrid &G) {
+        const bool odd = ((long long)R.S[S_EITS]) & 1;
+        cg_update2_body(G, R.c, R.p, odd ? R.d0 : R.d1, R.r, R.q, R.S, R.part);
+    });
+}
+__global__ void kp_cg_init(const CgInitRec *__restrict__ tab, ActList act) {
+    act_for(tab, act, [&](const CgInitRec &R, const VGrid &G) { cg_init_body<false>(G, R.L, R.c, R.p, R.q, R.r, R.part); });
+}
+
+// persistent grid sizes of the compacted launches (workgroups)
+constexpr int PERSIST_LEG_WG = 2048, PERSIST_VEC_WG = 4096;
 
 template <typename T>
 static void free_legs(LegTabs<T> &t) { t.free_all(); }
@@ -2090,6 +2222,7 @@ void ens_press_free(impec_ensemble *e) {
     free_legs(t->l32);
     batch_free(t->rho); batch_free(t->upd); batch_free(t->ctl); batch_free(t->pin); batch_free(t->shift);
     batch_free(t->tres);
+    for (void *p : t->list_mem) (void)hipFree(p);
     delete t;
     e->pres = nullptr;
 }
@@ -2145,6 +2278,65 @@ static int build_legs(impec_ensemble *e, LegTabs<T> &t) {
     }
     ENS_CHECK(e, batch_upload(t.coarsest, cr, one));
     ENS_CHECK(e, batch_upload(t.dir, dr, u_vec));
+    return IMPEC_OK;
+}
+
+// a compacted list over the members of a batch (those with units > 0, in
+// member order, as batch_upload keeps them)
+static int make_list(impec_ensemble *e, const int *bstart, const std::vector<int> &units, ListDesc &d) {
+    EnsPressTabs *t = e->pres;
+    std::vector<double *> S;
+    for (int k = 0; k < e->n; k++)
+        if (units[k] > 0) S.push_back(e->m[k]->scal);
+    d = ListDesc();
+    d.nmem = (int)S.size();
+    d.bstart = bstart;
+    if (d.nmem == 0) return IMPEC_OK;
+    void *pS, *pm, *ps, *pn;
+    ENS_CHECK(e, hipMalloc(&pS, sizeof(double *) * S.size()));
+    t->list_mem.push_back(pS);
+    ENS_CHECK(e, hipMalloc(&pm, sizeof(int) * S.size()));
+    t->list_mem.push_back(pm);
+    ENS_CHECK(e, hipMalloc(&ps, sizeof(int) * (S.size() + 1)));
+    t->list_mem.push_back(ps);
+    ENS_CHECK(e, hipMalloc(&pn, sizeof(int)));
+    t->list_mem.push_back(pn);
+    ENS_CHECK(e, hipMemcpy(pS, S.data(), sizeof(double *) * S.size(), hipMemcpyHostToDevice));
+    ENS_CHECK(e, hipMemset(pn, 0, sizeof(int)));
+    ENS_CHECK(e, hipMemset(ps, 0, sizeof(int)));
+    d.S = (double *const *)pS;
+    d.act.member = (int *)pm;
+    d.act.start = (int *)ps;
+    d.act.n = (int *)pn;
+    return IMPEC_OK;
+}
+
+static int build_lists(impec_ensemble *e) {
+    EnsPressTabs *t = e->pres;
+    const int n = e->n;
+    t->h_iter.assign(t->lmax, ListDesc());
+    t->h_check.assign(1, ListDesc());
+    std::vector<int> all(n, 1);
+    int st;
+    const int *vstart = t->mixed ? t->l32.dir.start : t->l64.dir.start;
+    if ((st = make_list(e, vstart, all, t->h_iter[0]))) return st;
+    for (int l = 0; l + 1 < t->lmax; l++) {
+        std::vector<int> u(n, 0);
+        for (int k = 0; k < n; k++) u[k] = l + 1 < e->m[k]->nlev;
+        const int *ls = t->mixed ? t->l32.down[l].start : t->l64.down[l].start;
+        if ((st = make_list(e, ls, u, t->h_iter[1 + l]))) return st;
+    }
+    if ((st = make_list(e, t->init.start, all, t->h_check[0]))) return st;
+    t->n_iter = t->lmax;
+    void *pi, *pc;
+    ENS_CHECK(e, hipMalloc(&pi, sizeof(ListDesc) * t->h_iter.size()));
+    t->list_mem.push_back(pi);
+    ENS_CHECK(e, hipMalloc(&pc, sizeof(ListDesc)));
+    t->list_mem.push_back(pc);
+    ENS_CHECK(e, hipMemcpy(pi, t->h_iter.data(), sizeof(ListDesc) * t->h_iter.size(), hipMemcpyHostToDevice));
+    ENS_CHECK(e, hipMemcpy(pc, t->h_check.data(), sizeof(ListDesc), hipMemcpyHostToDevice));
+    t->d_iter = (ListDesc *)pi;
+    t->d_check = (ListDesc *)pc;
     return IMPEC_OK;
 }
 
@@ -2257,6 +2449,7 @@ int ens_press_build(impec_ensemble *e) {
     ENS_CHECK(e, batch_upload(t->tres, tr, u_red));
     int st = t->mixed ? build_legs<float>(e, t->l32) : build_legs<double>(e, t->l64);
     if (st) return st;
+    if ((st = build_lists(e))) return st;
     ENS_CHECK(e, hipDeviceSynchronize());
     return IMPEC_OK;
 }
@@ -2265,7 +2458,7 @@ template <class E, class... A>
 static hipError_t launch_b(void (*k)(const E *, const int *, int, A...), const Batch<E> &b, dim3 blk,
                            hipStream_t s, A... a) {
     if (b.nmem == 0) return hipSuccess;
-    hipLaunchKernelGGL(k, dim3(b.total), blk, 0, s, b.tab, b.start, b.nmem, a...);
+    hipLaunchKernelGGL(k, dim3(batch_grid(b.total)), blk, 0, s, b.tab, b.start, b.nmem, a...);
     return hipGetLastError();
 }
 template <class E>
@@ -2318,24 +2511,55 @@ int ens_solve_begin(impec_ensemble *e) {
     ENS_CHECK(e, launch_b(kb_cg_init, t->init, dim3(256), s, -1));
     ENS_CHECK(e, ens_finish(s, t->fin_init, -1));
     ENS_CHECK(e, launch_t(kb_ctl_first, t->ctl, s));
+    hipLaunchKernelGGL(kb_compact, dim3(t->n_iter), dim3(256), 0, s, t->d_iter, (int)EST_ITER);
+    ENS_CHECK(e, hipGetLastError());
     return IMPEC_OK;
 }
 
+static int persist_grid(int units, int cap) { return units < cap ? units : cap; }
+
+// one CG iteration of the members still iterating (compacted launches)
 template <typename T>
 static int ens_cg_slot_t(impec_ensemble *e, const LegTabs<T> &lt) {
     EnsPressTabs *t = e->pres;
     hipStream_t s = e->stream;
-    int st = ens_vcycle<T>(e, lt, EST_ITER);
-    if (st) return st;
+    const dim3 lblk(64 * SW_WAVES), vblk(64, 4);
+    const int lmax = t->lmax;
+    for (int l = 0; l + 1 < lmax; l++) {
+        const Batch<FusedArgsT<T>> &b = lt.down[l];
+        if (b.nmem == 0) continue;
+        const dim3 g(persist_grid((b.total + SW_WAVES - 1) / SW_WAVES, PERSIST_LEG_WG));
+        const ActList &a = t->h_iter[1 + l].act;
+        if (l == 0) hipLaunchKernelGGL((kp_mg_down<true, T>), g, lblk, 0, s, b.tab, a);
+        else hipLaunchKernelGGL((kp_mg_down<false, T>), g, lblk, 0, s, b.tab, a);
+    }
+    ENS_CHECK(e, launch_m(kb_coarsest<T>, lt.coarsest, dim3(64), s, (int)EST_ITER));
+    for (int l = lmax - 2; l >= 0; l--) {
+        const Batch<FusedArgsT<T>> &b = lt.up[l];
+        if (b.nmem == 0) continue;
+        const dim3 g(persist_grid((b.total + SW_WAVES - 1) / SW_WAVES, PERSIST_LEG_WG));
+        const ActList &a = t->h_iter[1 + l].act;
+        if (l == 0) hipLaunchKernelGGL((kp_mg_up<true, T>), g, lblk, 0, s, b.tab, a);
+        else hipLaunchKernelGGL((kp_mg_up<false, T>), g, lblk, 0, s, b.tab, a);
+    }
+    ENS_CHECK(e, hipGetLastError());
     ENS_CHECK(e, launch_m(kb_finish_rho, t->rho, dim3(1024), s, (int)EST_ITER));
-    ENS_CHECK(e, launch_b(kb_dir_matvec<T>, lt.dir, dim3(64, 4), s, (int)EST_ITER));
+    const ActList &av = t->h_iter[0].act;
+    hipLaunchKernelGGL(kp_dir_matvec<T>, dim3(persist_grid(lt.dir.total, PERSIST_VEC_WG)), vblk, 0, s, lt.dir.tab, av);
+    ENS_CHECK(e, hipGetLastError());
     ENS_CHECK(e, ens_finish(s, t->fin_dq, EST_ITER));
-    ENS_CHECK(e, launch_b(kb_update2, t->upd, dim3(64, 4), s, (int)EST_ITER));
+    hipLaunchKernelGGL(kp_update2, dim3(persist_grid(t->upd.total, PERSIST_VEC_WG)), vblk, 0, s, t->upd.tab, av);
+    ENS_CHECK(e, hipGetLastError());
     ENS_CHECK(e, ens_finish(s, t->fin_rr, EST_ITER));
     ENS_CHECK(e, launch_t(kb_ctl_iter, t->ctl, s));
-    ENS_CHECK(e, launch_b(kb_cg_init, t->init, dim3(256), s, (int)EST_CHECK));
+    hipLaunchKernelGGL(kb_compact, dim3(1), dim3(256), 0, s, t->d_check, (int)EST_CHECK);
+    hipLaunchKernelGGL(kp_cg_init, dim3(persist_grid(t->init.total, PERSIST_VEC_WG)), dim3(256), 0, s, t->init.tab,
+                       t->h_check[0].act);
+    ENS_CHECK(e, hipGetLastError());
     ENS_CHECK(e, ens_finish(s, t->fin_init, EST_CHECK));
     ENS_CHECK(e, launch_t(kb_ctl_check, t->ctl, s));
+    hipLaunchKernelGGL(kb_compact, dim3(t->n_iter), dim3(256), 0, s, t->d_iter, (int)EST_ITER);
+    ENS_CHECK(e, hipGetLastError());
     return IMPEC_OK;
 }
 

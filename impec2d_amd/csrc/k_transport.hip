@@ -373,17 +373,20 @@ __device__ __forceinline__ SweepConst sweep_factors(const SweepRec &R) {
 
 template <bool MUSCL, bool PHI1>
 __global__ void kb_sweep_x(const SweepRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    sweep_x_body<MUSCL, PHI1>(G, sweep_factors(R), R.cin, R.cout, R.vel);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        sweep_x_body<MUSCL, PHI1>(G, sweep_factors(R), R.cin, R.cout, R.vel);
+    });
 }
 template <bool MUSCL, bool PHI1>
 __global__ void kb_sweep_y(const SweepRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    sweep_y_body<MUSCL, PHI1>(G, sweep_factors(R), R.cin, R.cout, R.vel);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        sweep_y_body<MUSCL, PHI1>(G, sweep_factors(R), R.cin, R.cout, R.vel);
+    });
 }
 __global__ void kb_average(const AvgRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    IMPEC_BATCH_BLOCK(R, G, tab, start, nmem)
-    average_body(G, R.a, R.b, R.c, R.rows, R.cols, R.ld);
+    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
+        average_body(G, R.a, R.b, R.c, R.rows, R.cols, R.ld);
+    });
 }
 
 void ens_trans_free(impec_ensemble *e) {
@@ -433,10 +436,11 @@ static int ens_sweeps(impec_ensemble *e) {
         const Batch<SweepRec> &b = *order[q];
         if (b.nmem == 0) continue;
         if (q == 0 || q == 3)
-            hipLaunchKernelGGL((kb_sweep_x<MUSCL, PHI1>), dim3(b.total), blk, 0, s, b.tab, b.start, b.nmem);
-        else hipLaunchKernelGGL((kb_sweep_y<MUSCL, PHI1>), dim3(b.total), blk, 0, s, b.tab, b.start, b.nmem);
+            hipLaunchKernelGGL((kb_sweep_x<MUSCL, PHI1>), dim3(batch_grid(b.total)), blk, 0, s, b.tab, b.start, b.nmem);
+        else hipLaunchKernelGGL((kb_sweep_y<MUSCL, PHI1>), dim3(batch_grid(b.total)), blk, 0, s, b.tab, b.start, b.nmem);
     }
-    if (t->avg.nmem) hipLaunchKernelGGL(kb_average, dim3(t->avg.total), blk, 0, s, t->avg.tab, t->avg.start, t->avg.nmem);
+    if (t->avg.nmem)
+        hipLaunchKernelGGL(kb_average, dim3(batch_grid(t->avg.total)), blk, 0, s, t->avg.tab, t->avg.start, t->avg.nmem);
     ENS_CHECK(e, hipGetLastError());
     return IMPEC_OK;
 }
