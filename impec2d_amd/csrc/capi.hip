@@ -153,6 +153,7 @@ int impec::ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out) {
     if (P->dim_splitting < 0 || P->dim_splitting > 4) return IMPEC_EINVAL;
     if (P->precision != IMPEC_PREC_FP64 && P->precision != IMPEC_PREC_MIXED) return IMPEC_EINVAL;
     if (P->mg_rows < 0 || P->mg_rows > 4096) return IMPEC_EINVAL;
+    if (P->mg_variant < 0 || P->mg_variant > 2) return IMPEC_EINVAL;
     if (P->cn_guess != 0 && P->cn_guess != 1) return IMPEC_EINVAL;
     if (P->initial_guess < IMPEC_GUESS_ZERO || P->initial_guess > IMPEC_GUESS_LEARNED) return IMPEC_EINVAL;
     int ndev = 0;

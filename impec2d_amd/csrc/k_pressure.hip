@@ -1325,6 +1325,261 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgsT<T> A) {
     mg_up_body<FINE0, ANISO, T>(A, hw_wave());
 }
 
+// ===========================================================================
+// The bottom of the streaming V-cycle: from the first level of at most
+// BOTTOM_N x BOTTOM_N cells down to the coarsest and back up in one
+// workgroup, with every level's x, b, r and w in LDS.  Per cell it does what
+// the streaming legs do (faces from w by the same expressions, red then black
+// from a zero guess, residual, full weighting; bilinear prolongation, black,
+// red), and on the coarsest level k_coarsest's bordered solve of the legs'
+// operator (faces from w, boundary faces dropped) assembled from LDS;
+// barriers separate the colour and transfer phases.  It replaces two launches
+// per small level, each a latency chain, and the one-thread coarsest launch.
+// ===========================================================================
+constexpr int BOTTOM_N = 32, BOTTOM_LEVELS = 8, BOTTOM_THREADS = 1024;
+constexpr int BOTTOM_CELLS = 1024 + 256 + 64 + 16 + 4;   // levels of at most 32, 16, 8, 4, 2 cells a side
+
+template <typename T>
+struct BottomLev {
+    const T *w;      // mobility quantity at cell (0,0), pitch ldw
+    int ldw, n1, n2, harmonic;
+    double cx, cy;
+};
+template <typename T>
+struct BottomArgs {
+    int nl;                        // levels, the last one the coarsest
+    BottomLev<T> lev[BOTTOM_LEVELS];
+    const T *b;                    // right-hand side of the first level at (0,0), pitch ldb
+    int ldb;
+    T *out;                        // its corrected, smoothed x at (0,0), pitch ldo
+    int ldo;
+    const double *S;               // loop state (ensembles)
+};
+
+// the four faces of cell (i,j) of an n1 x n2 level (zero on the boundary)
+template <typename T>
+__device__ __forceinline__ void bottom_faces(const T *w, int n1, int n2, int i, int j, T cx, T cy, int harmonic,
+                                             T &W, T &E, T &Sf, T &N) {
+#pragma clang fp contract(fast)
+    const T wc = w[i * n2 + j];
+    W = i > 0 ? cx * face_mob_pc(w[(i - 1) * n2 + j], wc, harmonic) : T(0);
+    E = i < n1 - 1 ? cx * face_mob_pc(wc, w[(i + 1) * n2 + j], harmonic) : T(0);
+    Sf = j > 0 ? cy * face_mob_pc(w[i * n2 + j - 1], wc, harmonic) : T(0);
+    N = j < n2 - 1 ? cy * face_mob_pc(wc, w[i * n2 + j + 1], harmonic) : T(0);
+}
+
+// Gauss-Seidel update of cell (i,j) of colour (i + j) & 1, as gs_cell
+template <typename T>
+__device__ __forceinline__ T bottom_gs(const T *x, const T *w, T bv, int n1, int n2, int i, int j, T cx, T cy,
+                                       int harmonic) {
+#pragma clang fp contract(fast)
+    T W, E, Sf, N;
+    bottom_faces(w, n1, n2, i, j, cx, cy, harmonic, W, E, Sf, N);
+    const T xa = i > 0 ? x[(i - 1) * n2 + j] : T(0), xb = i < n1 - 1 ? x[(i + 1) * n2 + j] : T(0);
+    const T xS = j > 0 ? x[i * n2 + j - 1] : T(0), xN = j < n2 - 1 ? x[i * n2 + j + 1] : T(0);
+    const T Cc = -((W + E) + (Sf + N));
+    T s = bv;
+    s -= W * xa;
+    s -= Sf * xS;
+    s -= E * xb;
+    s -= N * xN;
+    return Cc != T(0) ? s * recip(Cc) : T(0);
+}
+
+template <typename T>
+__device__ void mg_bottom_body(const BottomArgs<T> &A) {
+#pragma clang fp contract(fast)
+    __shared__ T sx[BOTTOM_CELLS], sb[BOTTOM_CELLS], sr[BOTTOM_CELLS], sw[BOTTOM_CELLS];
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int nl = A.nl;
+    int off[BOTTOM_LEVELS];
+    {
+        int o = 0;
+        for (int l = 0; l < nl; l++) {
+            off[l] = o;
+            o += A.lev[l].n1 * A.lev[l].n2;
+        }
+    }
+    for (int l = 0; l < nl; l++) {
+        const int n1 = A.lev[l].n1, n2 = A.lev[l].n2, ldw = A.lev[l].ldw;
+        const T *w = A.lev[l].w;
+        for (int id = t; id < n1 * n2; id += nt) {
+            const int i = id / n2, j = id - i * n2;
+            sw[off[l] + id] = w[(long)i * ldw + j];
+            if (l == 0) sb[id] = A.b[(long)i * A.ldb + j];
+        }
+    }
+    __syncthreads();
+    // down: red from zero, black, residual, full weighting
+    for (int l = 0; l + 1 < nl; l++) {
+        const int n1 = A.lev[l].n1, n2 = A.lev[l].n2, n = n1 * n2, hm = A.lev[l].harmonic;
+        const T cx = T(A.lev[l].cx), cy = T(A.lev[l].cy);
+        T *x = sx + off[l], *r = sr + off[l];
+        const T *b = sb + off[l], *w = sw + off[l];
+        for (int id = t; id < n; id += nt) {
+            const int i = id / n2, j = id - i * n2;
+            T v = T(0);
+            if (((i + j) & 1) == 0) {
+                T W, E, Sf, N;
+                bottom_faces(w, n1, n2, i, j, cx, cy, hm, W, E, Sf, N);
+                const T C = -((W + E) + (Sf + N));
+                v = C != T(0) ? b[id] * recip(C) : T(0);
+            }
+            x[id] = v;
+        }
+        __syncthreads();
+        for (int id = t; id < n; id += nt) {
+            const int i = id / n2, j = id - i * n2;
+            if ((i + j) & 1) x[id] = bottom_gs(x, w, b[id], n1, n2, i, j, cx, cy, hm);
+        }
+        __syncthreads();
+        for (int id = t; id < n; id += nt) {
+            const int i = id / n2, j = id - i * n2;
+            T W, E, Sf, N;
+            bottom_faces(w, n1, n2, i, j, cx, cy, hm, W, E, Sf, N);
+            const T Cc = -((W + E) + (Sf + N));
+            T s = Cc * x[id];
+            s += W * (i > 0 ? x[id - n2] : T(0));
+            s += Sf * (j > 0 ? x[id - 1] : T(0));
+            s += E * (i < n1 - 1 ? x[id + n2] : T(0));
+            s += N * (j < n2 - 1 ? x[id + 1] : T(0));
+            r[id] = b[id] - s;
+        }
+        __syncthreads();
+        const int m1 = A.lev[l + 1].n1, m2 = A.lev[l + 1].n2;
+        T *bc = sb + off[l + 1];
+        for (int id = t; id < m1 * m2; id += nt) {
+            const int I = id / m2, J = id - I * m2;
+            const T wL = T((2 * J - 1 >= 0 && 2 * J - 1 < n2) ? pw(2 * J - 1, J, m2) : 0.);
+            const T w0 = T(2 * J < n2 ? pw(2 * J, J, m2) : 0.);
+            const T w1 = T(2 * J + 1 < n2 ? pw(2 * J + 1, J, m2) : 0.);
+            const T wR = T(2 * J + 2 < n2 ? pw(2 * J + 2, J, m2) : 0.);
+            T acc = T(0);
+            for (int f = 2 * I - 1; f <= 2 * I + 2; f++) {
+                if (f < 0 || f >= n1) continue;
+                const T wi = T(pw(f, I, m1));
+                if (wi == T(0)) continue;
+                const T *rf = r + f * n2;
+                const T rL = wL != T(0) ? rf[2 * J - 1] : T(0), r0 = w0 != T(0) ? rf[2 * J] : T(0);
+                const T r1 = w1 != T(0) ? rf[2 * J + 1] : T(0), rR = wR != T(0) ? rf[2 * J + 2] : T(0);
+                const T h = ((wL * rL + w0 * r0) + w1 * r1) + wR * rR;
+                acc += wi * h;
+            }
+            bc[id] = T(0.25) * acc;
+        }
+        __syncthreads();
+    }
+    // coarsest: k_coarsest's bordered solve (FP64 elimination) of the operator
+    // the legs use on that level, assembled from the faces in LDS
+    if (t == 0) {
+        const int l = nl - 1, n1 = A.lev[l].n1, n2 = A.lev[l].n2, n = n1 * n2, m = n + 1, hm = A.lev[l].harmonic;
+        const T cx = T(A.lev[l].cx), cy = T(A.lev[l].cy);
+        const T *b = sb + off[l], *w = sw + off[l];
+        T *x = sx + off[l];
+        double M[5][6];
+        for (int rr = 0; rr < m; rr++)
+            for (int q = 0; q <= m; q++) M[rr][q] = 0.;
+        for (int i = 0; i < n1; i++)
+            for (int j = 0; j < n2; j++) {
+                const int row = i * n2 + j;
+                T W, E, Sf, N;
+                bottom_faces(w, n1, n2, i, j, cx, cy, hm, W, E, Sf, N);
+                M[row][row] += (double)(-((W + E) + (Sf + N)));
+                if (i > 0) M[row][row - n2] += (double)W;
+                if (i < n1 - 1) M[row][row + n2] += (double)E;
+                if (j > 0) M[row][row - 1] += (double)Sf;
+                if (j < n2 - 1) M[row][row + 1] += (double)N;
+                M[row][n] = 1.;
+                M[n][row] = 1.;
+                M[row][m] = (double)b[row];
+            }
+        for (int col = 0; col < m; col++) {
+            int piv = col;
+            for (int rr = col + 1; rr < m; rr++)
+                if (fabs(M[rr][col]) > fabs(M[piv][col])) piv = rr;
+            if (piv != col)
+                for (int q = 0; q <= m; q++) {
+                    const double tt = M[col][q];
+                    M[col][q] = M[piv][q];
+                    M[piv][q] = tt;
+                }
+            const double d = M[col][col];
+            if (d == 0.) continue;
+            for (int rr = col + 1; rr < m; rr++) {
+                const double f = M[rr][col] / d;
+                for (int q = col; q <= m; q++) M[rr][q] -= f * M[col][q];
+            }
+        }
+        double sol[5];
+        for (int rr = m - 1; rr >= 0; rr--) {
+            double tt = M[rr][m];
+            for (int q = rr + 1; q < m; q++) tt -= M[rr][q] * sol[q];
+            sol[rr] = M[rr][rr] != 0. ? tt / M[rr][rr] : 0.;
+        }
+        for (int k = 0; k < n; k++) x[k] = (T)sol[k];
+    }
+    __syncthreads();
+    // up: x + P e, black, red
+    for (int l = nl - 2; l >= 0; l--) {
+        const int n1 = A.lev[l].n1, n2 = A.lev[l].n2, n = n1 * n2, hm = A.lev[l].harmonic;
+        const int m1 = A.lev[l + 1].n1, m2 = A.lev[l + 1].n2;
+        const T cx = T(A.lev[l].cx), cy = T(A.lev[l].cy);
+        T *x = sx + off[l];
+        const T *xc = sx + off[l + 1], *b = sb + off[l], *w = sw + off[l];
+        for (int id = t; id < n; id += nt) {
+            const int i = id / n2, j = id - i * n2;
+            int I0 = i >> 1, I1 = (i & 1) ? I0 + 1 : I0 - 1;
+            I0 = I0 > m1 - 1 ? m1 - 1 : I0;
+            I1 = I1 < 0 ? 0 : (I1 > m1 - 1 ? m1 - 1 : I1);
+            const int J = j >> 1;
+            const int Jn = (j & 1) ? (J == m2 - 1 ? J : J + 1) : (J == 0 ? J : J - 1);
+            const T o0 = xc[I0 * m2 + J], o1 = xc[I1 * m2 + J], l0 = xc[I0 * m2 + Jn], l1 = xc[I1 * m2 + Jn];
+            const T e = T(0.5625) * o0 + T(0.1875) * l0 + T(0.1875) * o1 + T(0.0625) * l1;
+            x[id] = x[id] + e;
+        }
+        __syncthreads();
+        for (int c = 1; c >= 0; c--) {
+            for (int id = t; id < n; id += nt) {
+                const int i = id / n2, j = id - i * n2;
+                if (((i + j) & 1) == c) x[id] = bottom_gs(x, w, b[id], n1, n2, i, j, cx, cy, hm);
+            }
+            __syncthreads();
+        }
+    }
+    {
+        const int n1 = A.lev[0].n1, n2 = A.lev[0].n2;
+        for (int id = t; id < n1 * n2; id += nt) {
+            const int i = id / n2, j = id - i * n2;
+            A.out[(long)i * A.ldo + j] = sx[id];
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BOTTOM_THREADS) k_mg_bottom(BottomArgs<T> A) {
+    mg_bottom_body<T>(A);
+}
+
+// first level handled by k_mg_bottom, or -1 when the hierarchy has no level of
+// at most BOTTOM_N x BOTTOM_N cells above its coarsest
+static int bottom_level(const impec_ctx *ctx) {
+    if (ctx->P.mg_variant == 2) return -1;
+    for (int l = 1; l + 1 < ctx->nlev; l++)
+        if (ctx->lev[l].n1 <= BOTTOM_N && ctx->lev[l].n2 <= BOTTOM_N) {
+            if (ctx->nlev - l > BOTTOM_LEVELS) return -1;
+            int cells = 0;
+            for (int q = l; q < ctx->nlev; q++) cells += ctx->lev[q].n1 * ctx->lev[q].n2;
+            return cells <= BOTTOM_CELLS ? l : -1;
+        }
+    return -1;
+}
+
+// levels l < top_level run the streaming legs
+static int top_level(const impec_ctx *ctx) {
+    const int lb = bottom_level(ctx);
+    return lb > 0 ? lb : ctx->nlev - 1;
+}
+
 // sums of the V-cycle output and rho = r.z with z = x0 - mean(x0)
 template <class G>
 __device__ __forceinline__ void finish_rho_body(const G &gr_, const double *__restrict__ part, int nblocks, double inv_n, double *__restrict__ S) {
@@ -1503,12 +1758,42 @@ static dim3 stream_grid(const FusedArgsT<T> &A) { return dim3((A.nwave + SW_WAVE
 // one V(1,1) cycle on the level-0 rhs r - mean(r) in storage type T; the result
 // x0 is left in lev[0]'s r array of that type, with the per-wave FP64 partials
 // of sum(x0) and r.x0 for k_finish_rho
+// arguments of k_mg_bottom for levels lb .. nlev-1 in storage type T
+template <typename T>
+static BottomArgs<T> bottom_args(impec_ctx *ctx, int lb) {
+    using LA = LevelArrays<T>;
+    BottomArgs<T> A;
+    memset(&A, 0, sizeof(A));
+    A.nl = ctx->nlev - lb;
+    for (int l = lb; l < ctx->nlev; l++) {
+        const Level &L = ctx->lev[l];
+        const FusedArgsT<T> f = fused_args<T>(ctx, l);
+        BottomLev<T> &b = A.lev[l - lb];
+        b.w = f.w;
+        b.ldw = f.ldw;
+        b.n1 = L.n1;
+        b.n2 = L.n2;
+        b.harmonic = L.g.harmonic;
+        b.cx = L.g.cx;
+        b.cy = L.g.cy;
+    }
+    const Level &B = ctx->lev[lb];
+    A.b = LA::b(B) + B.ld + 1;
+    A.ldb = B.ld;
+    A.out = LA::r(B) + B.ld + 1;
+    A.ldo = B.ld;
+    A.S = ctx->scal;
+    return A;
+}
+
 template <typename T>
 static int vcycle_fused(impec_ctx *ctx) {
     using LA = LevelArrays<T>;
     const int L = ctx->nlev;
     const dim3 blk(64 * SW_WAVES);
-    for (int l = 0; l < L - 1; l++) {
+    const int lb = bottom_level(ctx);
+    const int ltop = lb > 0 ? lb : L - 1;
+    for (int l = 0; l < ltop; l++) {
         FusedArgsT<T> A = fused_args<T>(ctx, l);
         prof_begin(ctx);
         if (l == 0 && A.w2 != A.w)
@@ -1517,7 +1802,11 @@ static int vcycle_fused(impec_ctx *ctx) {
         else hipLaunchKernelGGL((k_mg_down<false, false, T>), stream_grid(A), blk, 0, ctx->stream, A);
         prof_end(ctx, l == 0 ? "mg_down_l0" : "mg_coarse");
     }
-    {
+    if (lb > 0) {
+        prof_begin(ctx);
+        hipLaunchKernelGGL((k_mg_bottom<T>), dim3(1), dim3(BOTTOM_THREADS), 0, ctx->stream, bottom_args<T>(ctx, lb));
+        prof_end(ctx, "mg_coarse");
+    } else {
         Level &Cl = ctx->lev[L - 1];
         LevArgs a = lev_args(ctx, L - 1);
         prof_begin(ctx);
@@ -1525,7 +1814,7 @@ static int vcycle_fused(impec_ctx *ctx) {
                            LA::b(Cl) + Cl.ld + 1);
         prof_end(ctx, "mg_coarse");
     }
-    for (int l = L - 2; l >= 0; l--) {
+    for (int l = ltop - 1; l >= 0; l--) {
         FusedArgsT<T> A = fused_args<T>(ctx, l);
         prof_begin(ctx);
         if (l == 0 && A.w2 != A.w)
@@ -1911,7 +2200,8 @@ struct TrueRec {
 template <typename T>
 struct LegTabs {
     Batch<FusedArgsT<T>> down[MAX_LEVELS], up[MAX_LEVELS];
-    Batch<CoarsestRec<T>> coarsest;
+    Batch<CoarsestRec<T>> coarsest;    // members without a bottom kernel
+    Batch<BottomArgs<T>> bottom;       // members with one
     Batch<DirRec<T>> dir;
     void free_all() {
         for (int l = 0; l < MAX_LEVELS; l++) {
@@ -1919,6 +2209,7 @@ struct LegTabs {
             batch_free(up[l]);
         }
         batch_free(coarsest);
+        batch_free(bottom);
         batch_free(dir);
     }
 };
@@ -2006,6 +2297,12 @@ __global__ void __launch_bounds__(64 * SW_WAVES)
     const FusedArgsT<T> A = tab[m];
     if (!in_state(A.S, need)) return;
     mg_up_body<FINE0, false, T>(A, gw - start[m]);
+}
+template <typename T>
+__global__ void __launch_bounds__(BOTTOM_THREADS) kb_mg_bottom(const BottomArgs<T> *__restrict__ tab, int need) {
+    const BottomArgs<T> &A = tab[blockIdx.x];
+    if (!in_state(A.S, need)) return;
+    mg_bottom_body<T>(A);
 }
 template <typename T>
 __global__ void kb_coarsest(const CoarsestRec<T> *__restrict__ tab, int need) {
@@ -2252,7 +2549,7 @@ static int build_legs(impec_ensemble *e, LegTabs<T> &t) {
         std::vector<int> units(n, 0);
         for (int k = 0; k < n; k++) {
             impec_ctx *ctx = e->m[k];
-            if (l + 1 >= ctx->nlev) continue;
+            if (l >= top_level(ctx)) continue;
             recs[k] = fused_args<T>(ctx, l);
             units[k] = recs[k].nwave;
         }
@@ -2260,12 +2557,17 @@ static int build_legs(impec_ensemble *e, LegTabs<T> &t) {
         ENS_CHECK(e, batch_upload(t.up[l], recs, units));
     }
     std::vector<CoarsestRec<T>> cr(n);
+    std::vector<BottomArgs<T>> br(n);
     std::vector<DirRec<T>> dr(n);
-    std::vector<int> one(n, 1), u_vec(n);
+    std::vector<int> u_co(n), u_bo(n), u_vec(n);
     for (int k = 0; k < n; k++) {
         impec_ctx *ctx = e->m[k];
         Level &Cl = ctx->lev[ctx->nlev - 1];
         cr[k] = CoarsestRec<T>{lev_args(ctx, ctx->nlev - 1), LA::x(Cl) + Cl.ld + 1, LA::b(Cl) + Cl.ld + 1, ctx->scal};
+        const int lb = bottom_level(ctx);
+        if (lb > 0) br[k] = bottom_args<T>(ctx, lb);
+        u_bo[k] = lb > 0;
+        u_co[k] = lb <= 0;
         LevArgs a0 = lev_args(ctx, 0);
         if (!IMPEC_NO_FACES) {
             a0.fx = ctx->fx;
@@ -2276,7 +2578,8 @@ static int build_legs(impec_ensemble *e, LegTabs<T> &t) {
                           (int)vg.grid.x, (int)vg.grid.y};
         u_vec[k] = vg.nblocks;
     }
-    ENS_CHECK(e, batch_upload(t.coarsest, cr, one));
+    ENS_CHECK(e, batch_upload(t.coarsest, cr, u_co));
+    ENS_CHECK(e, batch_upload(t.bottom, br, u_bo));
     ENS_CHECK(e, batch_upload(t.dir, dr, u_vec));
     return IMPEC_OK;
 }
@@ -2322,7 +2625,7 @@ static int build_lists(impec_ensemble *e) {
     if ((st = make_list(e, vstart, all, t->h_iter[0]))) return st;
     for (int l = 0; l + 1 < t->lmax; l++) {
         std::vector<int> u(n, 0);
-        for (int k = 0; k < n; k++) u[k] = l + 1 < e->m[k]->nlev;
+        for (int k = 0; k < n; k++) u[k] = l < top_level(e->m[k]);
         const int *ls = t->mixed ? t->l32.down[l].start : t->l64.down[l].start;
         if ((st = make_list(e, ls, u, t->h_iter[1 + l]))) return st;
     }
@@ -2488,6 +2791,7 @@ static int ens_vcycle(impec_ensemble *e, const LegTabs<T> &t, int need) {
         else hipLaunchKernelGGL((kb_mg_down<false, T>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
     }
     ENS_CHECK(e, launch_m(kb_coarsest<T>, t.coarsest, dim3(64), s, need));
+    ENS_CHECK(e, launch_m(kb_mg_bottom<T>, t.bottom, dim3(BOTTOM_THREADS), s, need));
     for (int l = lmax - 2; l >= 0; l--) {
         const Batch<FusedArgsT<T>> &b = t.up[l];
         if (b.nmem == 0) continue;
@@ -2534,6 +2838,7 @@ static int ens_cg_slot_t(impec_ensemble *e, const LegTabs<T> &lt) {
         else hipLaunchKernelGGL((kp_mg_down<false, T>), g, lblk, 0, s, b.tab, a);
     }
     ENS_CHECK(e, launch_m(kb_coarsest<T>, lt.coarsest, dim3(64), s, (int)EST_ITER));
+    ENS_CHECK(e, launch_m(kb_mg_bottom<T>, lt.bottom, dim3(BOTTOM_THREADS), s, (int)EST_ITER));
     for (int l = lmax - 2; l >= 0; l--) {
         const Batch<FusedArgsT<T>> &b = lt.up[l];
         if (b.nmem == 0) continue;

@@ -72,7 +72,9 @@ typedef struct impec_params {
     int initial_guess;         /* ext: IMPEC_GUESS_* of the pressure solve     */
     int mg_pre, mg_post;       /* smoothing sweeps per V-cycle leg             */
     int mg_variant;            /* 0: streaming fused V-cycle where available,  */
-                               /* 1: one kernel per smoothing/transfer pass    */
+                               /* 1: one kernel per smoothing/transfer pass,   */
+                               /* 2: streaming legs down to the coarsest level */
+                               /* (no single-workgroup bottom kernel)          */
     int mg_rows;               /* rows per wavefront chunk of the fused finest */
                                /* level (0: automatic; rounded up to even)     */
     int cn_guess;              /* ext: initial guess of the dispersion solve:  */

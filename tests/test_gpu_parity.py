@@ -259,14 +259,16 @@ def test_steps_1_and_10(gpu_lib, kw):
 
 @pytest.mark.parametrize("n1,n2", [(256, 256), (200, 120), (77, 45), (2048, 2048)])
 def test_fused_and_unfused_multigrid_agree(gpu_lib, n1, n2):
-    """The LDS-fused V-cycle and the one-kernel-per-pass V-cycle are different
-    preconditioners of the same CG; both reach the tolerance and agree on p."""
+    """The streaming V-cycle (with its single-workgroup bottom kernel), the same
+    legs run down to the coarsest level, and the one-kernel-per-pass V-cycle are
+    preconditioners of the same CG; all reach the tolerance and agree on p, and
+    the bottom kernel does what the small-level legs do (same iteration count)."""
     from impec2d_amd import capi
     # 1e-11: the FP64 rounding floor of ||b - A p|| / ||b|| at 2048^2 is ~1e-12
     p = params(N1=n1, N2=n2, nu=1.0, eps=1e-11, mu_inj=0.1)
     _say("fused vs unfused", n1, n2)
     out = []
-    for variant in (0, 1):
+    for variant in (0, 1, 2):
         o, g = pair(p, mg_variant=variant)
         c = smooth_c(o, seed=17)
         q = o.source("one_cell")
@@ -277,5 +279,9 @@ def test_fused_and_unfused_multigrid_agree(gpu_lib, n1, n2):
         out.append((its, o.interior(g.get_pressure())))
         del capi
         from impec2d_amd import capi  # noqa: F401
+    _say("  iterations: streaming %d, one kernel per pass %d, legs to the coarsest %d"
+         % (out[0][0], out[1][0], out[2][0]))
     assert abs(out[0][0] - out[1][0]) <= 3, (out[0][0], out[1][0])
+    assert abs(out[0][0] - out[2][0]) <= 1, (out[0][0], out[2][0])
     assert rel_err(out[0][1], out[1][1]) < 1e-9
+    assert rel_err(out[0][1], out[2][1]) < 1e-9
