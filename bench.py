@@ -90,10 +90,11 @@ def modes(capi, n, warmup=3, steps=8):
     res = {"workload": "lognormal K (sigma 1, corr. length 0.05), M=20, no dispersion, %d^2, rel-res 1e-10" % n,
            "learned_training": train_info}
     for name, prec, guess in [("fp64_zero", "fp64", "zero"), ("mixed_zero", "mixed", "zero"),
-                              ("mixed_previous", "mixed", "previous"), ("mixed_learned", "mixed", "learned")]:
-        p = lognormal_params(capi, n, prec, guess if guess != "learned" else "zero")
-        if guess == "learned":
-            p.initial_guess = capi.GUESS_LEARNED
+                              ("mixed_previous", "mixed", "previous"), ("mixed_extrapolated", "mixed", "extrapolate"),
+                              ("mixed_learned", "mixed", "learned")]:
+        p = lognormal_params(capi, n, prec, "zero")
+        p.initial_guess = {"zero": capi.GUESS_ZERO, "previous": capi.GUESS_PREVIOUS,
+                           "extrapolate": capi.GUESS_EXTRAPOLATE, "learned": capi.GUESS_LEARNED}[guess]
         ctx = capi.Context(p)
         ctx.set_permeability(K, None)
         ctx.set_source(one_cell_source(n))
@@ -113,11 +114,14 @@ def modes(capi, n, warmup=3, steps=8):
         res[name] = {"ms_per_step": 1e3 * el / steps, "cell_updates_per_s": n * n * steps / el,
                      "vcycles_per_step": (s1.total_its - s0.total_its) / steps,
                      "last_relres": s1.last_relres}
-        if guess == "learned":
-            res[name]["last_step_length"] = s1.last_guess_alpha
+        if guess in ("learned", "extrapolate"):
+            res[name]["last_step_lengths"] = {"network_increment": s1.last_guess_alpha,
+                                              "last_change": s1.last_guess_gamma}
         ctx.close()
     net.close()
     res["speedup_mixed_learned_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_learned"]["ms_per_step"]
+    res["speedup_mixed_extrapolated_vs_fp64"] = (res["fp64_zero"]["ms_per_step"]
+                                                 / res["mixed_extrapolated"]["ms_per_step"])
     res["speedup_mixed_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_zero"]["ms_per_step"]
     return res
 

@@ -27,7 +27,7 @@ SPLIT_ALTERNATING, SPLIT_X_FIRST, SPLIT_Y_FIRST, SPLIT_STRANG, SPLIT_PARALLEL = 
 HYPER_MUSCL, HYPER_UPWIND = 0, 1
 PARA_EXPLICIT, PARA_CN = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED = 0, 1, 2
+GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED, GUESS_EXTRAPOLATE = 0, 1, 2, 3
 
 _INT_FIELDS = ["N1", "N2", "num_ghostcells", "source_type", "average_procedure", "dim_splitting",
                "hyper_method", "diffusion", "para_method", "is_test", "alpha_absolute",
@@ -47,7 +47,7 @@ class Stats(C.Structure):
                 ("total_its", C.c_int), ("calls", C.c_int), ("reserved_i", C.c_int * 3),
                 ("last_relres", C.c_double), ("last_parabolic_relres", C.c_double),
                 ("total_mass_outflux", C.c_double), ("last_guess_alpha", C.c_double),
-                ("reserved_d", C.c_double * 4)]
+                ("last_guess_gamma", C.c_double), ("reserved_d", C.c_double * 3)]
 
 
 # every entry point declared in include/impec_hip.h: name -> (argtypes, restype)

@@ -155,7 +155,7 @@ int impec::ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out) {
     if (P->mg_rows < 0 || P->mg_rows > 4096) return IMPEC_EINVAL;
     if (P->mg_variant < 0 || P->mg_variant > 2) return IMPEC_EINVAL;
     if (P->cn_guess != 0 && P->cn_guess != 1) return IMPEC_EINVAL;
-    if (P->initial_guess < IMPEC_GUESS_ZERO || P->initial_guess > IMPEC_GUESS_LEARNED) return IMPEC_EINVAL;
+    if (P->initial_guess < IMPEC_GUESS_ZERO || P->initial_guess > IMPEC_GUESS_EXTRAPOLATE) return IMPEC_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IMPEC_ENODEV;
 
@@ -209,10 +209,14 @@ int impec::ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out) {
     if (!st) st = dalloc(ctx, &ctx->v, nv);
     if (!st) st = dalloc(ctx, &ctx->q, nq);
     if (!st) st = dalloc(ctx, &ctx->scal, S_COUNT);
-    if (!st) st = dalloc(ctx, &ctx->part, (size_t)RED_BLOCKS * 4);
+    if (!st) st = dalloc(ctx, &ctx->part, (size_t)RED_BLOCKS * PART_VECS);
     if (!st && hipHostMalloc((void **)&ctx->h_scal, sizeof(double) * S_COUNT, hipHostMallocDefault) != hipSuccess)
         st = set_error(ctx, IMPEC_ENOMEM, "pinned host scalars");
     if (!st) st = pressure_setup_levels(ctx);
+    if (!st && P->initial_guess == IMPEC_GUESS_EXTRAPOLATE) {   // the two previous pressures
+        st = dalloc(ctx, &ctx->p_prev, ns);
+        if (!st) st = dalloc(ctx, &ctx->p_old, ns);
+    }
     if (!st && P->diffusion) st = ensure_cn(ctx);
     if (!st) update_symmetry(ctx);
     if (!st && hipDeviceSynchronize() != hipSuccess) st = set_error(ctx, IMPEC_EHIP, "initial synchronisation");
@@ -232,7 +236,8 @@ int impec_destroy(impec_ctx *ctx) {
     double *bufs[] = {ctx->c, ctx->p, ctx->u, ctx->v, ctx->q, ctx->K11, ctx->K22 != ctx->K11 ? ctx->K22 : nullptr,
                       ctx->w, ctx->w22, ctx->t1, ctx->t2, ctx->t3, ctx->scal, ctx->part, ctx->d11, ctx->d12,
                       ctx->d22, ctx->d21, ctx->cn_rhs, ctx->cn_x, ctx->cn_r, ctx->cn_r0, ctx->cn_p, ctx->cn_v,
-                      ctx->cn_s, ctx->cn_t, ctx->cn_y, ctx->cn_z, ctx->cn_dg, ctx->src_q, ctx->tr_md, ctx->p_prev};
+                      ctx->cn_s, ctx->cn_t, ctx->cn_y, ctx->cn_z, ctx->cn_dg, ctx->src_q, ctx->tr_md, ctx->p_prev,
+                      ctx->p_old};
     for (double *b : bufs)
         if (b) hipFree(b);
     if (ctx->src_idx) hipFree(ctx->src_idx);
@@ -276,6 +281,7 @@ int impec_get_concentration(impec_ctx *ctx, double *c) {
 int impec_set_pressure(impec_ctx *ctx, const double *p) {
     if (!ctx || !p) return IMPEC_EINVAL;
     ctx->have_prev_p = 1;
+    ctx->n_solves = 1;   // no earlier pressure to extrapolate from
     return put2d(ctx, ctx->p, ctx->lds, p, ctx->N1 + 2 * ctx->B, ctx->N2 + 2 * ctx->B);
 }
 int impec_get_pressure(impec_ctx *ctx, double *p) {

@@ -12,6 +12,7 @@ namespace impec {
 constexpr int GHOST = 2;          // num_ghostcells of every shipped config
 constexpr int MAX_LEVELS = 24;
 constexpr int RED_BLOCKS = 65536; // max partial sums per reduction pass
+constexpr int PART_VECS = 6;      // partial-sum vectors (stride RED_BLOCKS) in a context's part buffer
 #ifndef IMPEC_VEC_BLOCKS
 #define IMPEC_VEC_BLOCKS 65536
 #endif
@@ -28,6 +29,7 @@ enum Scal {
     S_MAXVEL = 20, S_MINDT = 21, S_MINDT_DIFF = 22,  // CFL pass
     S_OUTFLUX = 24, S_TMO = 25, S_CMAX = 26, S_CMIN = 27,
     S_GUESS_ALPHA = 28,                              // step length of the learned increment
+    S_GUESS_GAMMA = 29,                              // step length of the last step's change
     // per-member control of an ensemble's pressure solve (device-side loop state)
     S_EST = 32,        // 0 done, 1 iterating, 2 residual recheck due
     S_EITS = 33,       // iterations of this solve
@@ -39,6 +41,7 @@ enum Scal {
     S_ESTATUS = 39,    // 0 ok, 1 limit reached, 2 non-finite residual
     S_DT = 40,         // time step of the ensemble step
     S_ZEROP = 41,      // start the next solve from zero
+    S_G0 = 42,         // 42..46: partial products of the warm-start projection
     S_COUNT = 48
 };
 enum { EST_DONE = 0, EST_ITER = 1, EST_CHECK = 2 };
@@ -129,6 +132,8 @@ struct impec_ctx {
     // learned initial guess of the pressure solve
     impec_cnn *net;            // not owned
     double *p_prev;            // pressure before the latest solve (scalar layout)
+    double *p_old;             // pressure before the solve before that
+    int n_solves;              // pressure solves so far
 
     int prof_on;
     impec::ProfEntry prof[32];

@@ -882,10 +882,15 @@ int impec_cnn_train(impec_cnn *n, int iters, int batch, float lr, double *loss) 
 int impec_set_warmstart(impec_ctx *ctx, impec_cnn *n) {
     if (!ctx) return IMPEC_EINVAL;
     ctx->net = n;
+    const size_t bytes = sizeof(double) * (size_t)(ctx->N1 + 2 * ctx->B) * ctx->lds;
     if (n && !ctx->p_prev) {
-        const size_t bytes = sizeof(double) * (size_t)(ctx->N1 + 2 * ctx->B) * ctx->lds;
         IMPEC_HIP_CHECK(ctx, hipMalloc(&ctx->p_prev, bytes));
         IMPEC_HIP_CHECK(ctx, hipMemset(ctx->p_prev, 0, bytes));
+    }
+    // the learned guess also steps along the last step's change p_n - p_(n-1)
+    if (n && ctx->P.initial_guess == IMPEC_GUESS_LEARNED && !ctx->p_old) {
+        IMPEC_HIP_CHECK(ctx, hipMalloc(&ctx->p_old, bytes));
+        IMPEC_HIP_CHECK(ctx, hipMemset(ctx->p_old, 0, bytes));
     }
     return IMPEC_OK;
 }

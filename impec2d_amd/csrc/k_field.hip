@@ -442,7 +442,7 @@ __global__ void k_diag(const double *__restrict__ c, const double *__restrict__ 
     }
     block_reduce<RED_SUM, 3>(s);
     if (tid_linear() == 0)
-        for (int k = 0; k < 3; k++) part[k * RED_BLOCKS + blockIdx.x] = s[k];
+        for (int k = 0; k < 3; k++) part[k * RED_BLOCKS + blockIdx.x] = s[k];   // 3 <= PART_VECS
 }
 
 int launch_diagnostics(impec_ctx *ctx, int which) {

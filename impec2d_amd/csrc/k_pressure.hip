@@ -452,45 +452,81 @@ __global__ void k_cg_init(LevArgs L, CgArgs c, double *__restrict__ p, const dou
     cg_init_body<DIAG>(HwGrid{}, L, c, p, q, r, part);
 }
 
-// Learned initial guess: the network's increment d is used as a search
-// direction from the previous pressure, p = p_prev + alpha d with
-// alpha = d.r0 / d.A_N d, r0 = b~ - A_N p_prev (b~ the consistent right-hand
-// side), the step that minimises the A_N-norm of the error along d -- so the
-// guess is never worse than p_prev in that norm.  Partials (d.r0, d.A_N d).
-__global__ void k_guess_dots(LevArgs L, CgArgs c, const double *__restrict__ p, const double *__restrict__ q,
-                             const double *__restrict__ d, double *__restrict__ part) {
-    double s[2] = {0., 0.};
+// Warm starts from the previous solutions: p = p_n + alpha d + gamma e, with
+// d the network's increment (learned guess) and e = p_n - p_(n-1) the last
+// step's change.  alpha and gamma minimise the 2-norm of the consistent
+// residual r0 - alpha A_N d - gamma A_N e over the directions present -- the
+// quantity the stopping test measures -- so the guess's residual is never
+// larger than that of p_n (r0 = b~ - A_N p_n, b~ the consistent right-hand
+// side; A_N e = A_N p_n - A_N p_(n-1)).
+// Partials: (A_N d.r0, A_N e.r0, A_N d.A_N d, A_N d.A_N e, A_N e.A_N e).
+__global__ void k_guess_dots(LevArgs L, CgArgs c, const double *__restrict__ p, const double *__restrict__ pold,
+                             const double *__restrict__ q, const double *__restrict__ d, double *__restrict__ part) {
+    double s[5] = {0., 0., 0., 0., 0.};
     const double *x = p + (long)c.B * c.ldp + c.B;
+    const double *xo = pold ? pold + (long)c.B * c.ldp + c.B : nullptr;
     for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
         int i, j;
         cg_cell(c, id, i, j);
         double a[9];
         op_row5(L, i, j, a);
-        double r0 = -q[id] - row_dot<false>(a, x, c.ldp, i, j, c.n1, c.n2);
+        const double Ap = row_dot<false>(a, x, c.ldp, i, j, c.n1, c.n2);
+        double r0 = -q[id] - Ap;
         if (i == c.pin_i && j == c.pin_j) r0 -= c.bsum;
-        const double dc = d[(long)i * c.ld + j];
-        s[0] += dc * r0;
-        s[1] += dc * row_dot<false>(a, d, c.ld, i, j, c.n1, c.n2);
+        const double Ad = d ? row_dot<false>(a, d, c.ld, i, j, c.n1, c.n2) : 0.;
+        const double Ae = xo ? Ap - row_dot<false>(a, xo, c.ldp, i, j, c.n1, c.n2) : 0.;
+        s[0] += Ad * r0;
+        s[1] += Ae * r0;
+        s[2] += Ad * Ad;
+        s[3] += Ad * Ae;
+        s[4] += Ae * Ae;
     }
-    block_reduce<RED_SUM, 2>(s);
-    if (tid_linear() == 0) {
-        part[blockIdx.x] = s[0];
-        part[RED_BLOCKS + blockIdx.x] = s[1];
-    }
+    static_assert(5 <= PART_VECS, "five partial-sum vectors");
+    block_reduce<RED_SUM, 5>(s);
+    if (tid_linear() == 0)
+        for (int k = 0; k < 5; k++) part[k * RED_BLOCKS + blockIdx.x] = s[k];
 }
 
-// p += alpha d (alpha = 1 without the projection)
-__global__ void k_guess_apply(CgArgs c, double *__restrict__ p, const double *__restrict__ d, double *__restrict__ S,
-                              int project) {
-    double alpha = 1.;
-    // A_N is negative definite (C < 0) on mean-free vectors: the ratio is the
-    // minimiser of the (-A_N)-norm of the error either way
-    if (project) alpha = S[S_TT] != 0. ? S[S_TS] / S[S_TT] : 0.;
-    if (blockIdx.x == 0 && threadIdx.x == 0) S[S_GUESS_ALPHA] = alpha;
+// the minimal-residual step lengths from the five products in S[S_G0..]:
+// the 2 x 2 normal equations when both directions are present (the better
+// single direction when they are nearly parallel), else the 1-D minimiser
+__global__ void k_guess_solve(double *__restrict__ S, int use_d, int use_e) {
+    const double b0 = S[S_G0], b1 = S[S_G0 + 1], m00 = S[S_G0 + 2], m01 = S[S_G0 + 3], m11 = S[S_G0 + 4];
+    const double a1 = m00 > 0. ? b0 / m00 : 0., g1 = m11 > 0. ? b1 / m11 : 0.;
+    double a = use_d ? a1 : 0., g = use_e ? g1 : 0.;
+    if (use_d && use_e) {
+        const double det = m00 * m11 - m01 * m01;
+        if (det > 1e-12 * m00 * m11) {
+            a = (b0 * m11 - b1 * m01) / det;
+            g = (m00 * b1 - m01 * b0) / det;
+        } else if (a1 * b0 >= g1 * b1) {   // residual decreases b0^2/m00 and b1^2/m11
+            g = 0.;
+        } else {
+            a = 0.;
+        }
+    }
+    S[S_GUESS_ALPHA] = a;
+    S[S_GUESS_GAMMA] = g;
+}
+
+// p += alpha d + gamma (p - p_old) with the step lengths of k_guess_solve, or
+// unit steps (project = 0: no consistent residual to minimise)
+__global__ void k_guess_apply(CgArgs c, double *__restrict__ p, const double *__restrict__ pold,
+                              const double *__restrict__ d, double *__restrict__ S, int project) {
+    const double alpha = d ? (project ? S[S_GUESS_ALPHA] : 1.) : 0.;
+    const double gamma = pold ? (project ? S[S_GUESS_GAMMA] : 1.) : 0.;
+    if (!project && blockIdx.x == 0 && threadIdx.x == 0) {
+        S[S_GUESS_ALPHA] = alpha;
+        S[S_GUESS_GAMMA] = gamma;
+    }
     for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
         int i, j;
         cg_cell(c, id, i, j);
-        p[(long)(i + c.B) * c.ldp + (j + c.B)] += alpha * d[(long)i * c.ld + j];
+        const long o = (long)(i + c.B) * c.ldp + (j + c.B);
+        double v = p[o];
+        if (d) v += alpha * d[(long)i * c.ld + j];
+        if (pold) v += gamma * (p[o] - pold[o]);
+        p[o] = v;
     }
 }
 
@@ -1985,21 +2021,28 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     const double bb = ctx->bnorm * ctx->bnorm;
     const int maxit = ctx->P.max_iterations > 0 ? ctx->P.max_iterations : 10000;
 
-    const bool warm = (ctx->P.initial_guess == IMPEC_GUESS_PREVIOUS || ctx->P.initial_guess == IMPEC_GUESS_LEARNED) &&
-                      ctx->have_prev_p;
+    const int guess = ctx->P.initial_guess;
+    const bool warm = guess != IMPEC_GUESS_ZERO && ctx->have_prev_p;
     if (!warm) hipLaunchKernelGGL(k_zero_interior, dim3(nb), dim3(256), 0, s, c, ctx->p);
-    else if (ctx->P.initial_guess == IMPEC_GUESS_LEARNED && ctx->net) {
+    else if (guess == IMPEC_GUESS_LEARNED || guess == IMPEC_GUESS_EXTRAPOLATE) {
         prof_begin(ctx);
-        double *d = ctx->pd + L0.ld + 1;   // free until the CG starts
-        st = cnn_increment(ctx, d, L0.ld);
-        if (st) return st;
-        const bool project = !DIAG && ctx->symmetric;
-        if (project) {
-            hipLaunchKernelGGL(k_guess_dots, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q, d, ctx->part);
-            if ((st = finish_sum(ctx, nb, 2, S_TS))) return st;
+        double *d = nullptr;   // the network's increment, in pd (free until the CG starts)
+        if (guess == IMPEC_GUESS_LEARNED && ctx->net) {
+            d = ctx->pd + L0.ld + 1;
+            st = cnn_increment(ctx, d, L0.ld);
+            if (st) return st;
         }
-        hipLaunchKernelGGL(k_guess_apply, dim3(nb), dim3(256), 0, s, c, ctx->p, d, ctx->scal, project ? 1 : 0);
-        IMPEC_HIP_CHECK(ctx, hipGetLastError());
+        const double *pold = (ctx->p_old && ctx->n_solves >= 2) ? ctx->p_old : nullptr;
+        const bool project = !DIAG && ctx->symmetric;
+        if (d || pold) {
+            if (project) {
+                hipLaunchKernelGGL(k_guess_dots, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, pold, ctx->q, d, ctx->part);
+                if ((st = finish_sum(ctx, nb, 5, S_G0))) return st;
+                hipLaunchKernelGGL(k_guess_solve, dim3(1), dim3(1), 0, s, ctx->scal, d ? 1 : 0, pold ? 1 : 0);
+            }
+            hipLaunchKernelGGL(k_guess_apply, dim3(nb), dim3(256), 0, s, c, ctx->p, pold, d, ctx->scal, project ? 1 : 0);
+            IMPEC_HIP_CHECK(ctx, hipGetLastError());
+        }
         prof_end(ctx, "cnn_guess");
     }
     int its = 0, status = IMPEC_OK;
@@ -2040,9 +2083,13 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
         relres = ctx->bnorm > 0 ? sqrt(ctx->h_scal[S_TT]) / ctx->bnorm : 0.;
         if (status != IMPEC_OK || relres <= tol) break;
     }
-    ctx->stats.last_guess_alpha =
-        (warm && ctx->P.initial_guess == IMPEC_GUESS_LEARNED && ctx->net) ? ctx->h_scal[S_GUESS_ALPHA] : 0.;
+    const bool used_d = warm && guess == IMPEC_GUESS_LEARNED && ctx->net;
+    const bool used_e = warm && (guess == IMPEC_GUESS_LEARNED || guess == IMPEC_GUESS_EXTRAPOLATE) && ctx->p_old &&
+                        ctx->n_solves >= 2;
+    ctx->stats.last_guess_alpha = used_d ? ctx->h_scal[S_GUESS_ALPHA] : 0.;
+    ctx->stats.last_guess_gamma = used_e ? ctx->h_scal[S_GUESS_GAMMA] : 0.;
     ctx->have_prev_p = 1;
+    ctx->n_solves += 1;
     ctx->stats.last_its = its;
     ctx->stats.last_vcycles = its;
     ctx->stats.last_relres = relres;
@@ -2058,6 +2105,11 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
 
 int pressure_solve(impec_ctx *ctx, int *its, double *relres) {
     // the pressure before this solve (training pairs and the learned guess)
+    if (ctx->p_old && ctx->p_prev) {   // p_old <- the pressure before the previous solve
+        double *t = ctx->p_old;
+        ctx->p_old = ctx->p_prev;
+        ctx->p_prev = t;
+    }
     if (ctx->p_prev && ctx->have_prev_p)
         IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(ctx->p_prev, ctx->p, sizeof(double) * (size_t)(ctx->N1 + 2 * ctx->B) *
                                                                      ctx->lds,

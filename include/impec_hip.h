@@ -53,6 +53,8 @@ extern "C" {
 #define IMPEC_GUESS_ZERO 0   /* reference semantics (MGSolver.cpp:573-590)   */
 #define IMPEC_GUESS_PREVIOUS 1
 #define IMPEC_GUESS_LEARNED 2   /* previous p plus the network's increment   */
+                                /* and the last step's change, both scaled   */
+#define IMPEC_GUESS_EXTRAPOLATE 3   /* previous p plus the last step's change */
 
 /* Parameter block.  Field meanings follow SimParams (SimParams.h:29-164);
  * fields marked "ext" extend the reference. */
@@ -105,7 +107,9 @@ typedef struct impec_stats {
     double total_mass_outflux; /* HyperbolicSolver::GetTotalMassOutflux        */
     double last_guess_alpha;   /* ext: step length given to the learned        */
                                /* increment in the last solve (0: not used)    */
-    double reserved_d[4];
+    double last_guess_gamma;   /* ext: step length given to the last step's    */
+                               /* change p_n - p_(n-1) (0: not used)           */
+    double reserved_d[3];
 } impec_stats;
 
 typedef struct impec_ctx impec_ctx;

@@ -1,6 +1,7 @@
 """Warm starts: the previous pressure as the MG-CG initial guess
-(initial_guess = previous) and the current concentration as the dispersion
-solve's initial guess (cn_guess = 1).  Both keep the stopping rule relative to
+(initial_guess = previous), the previous pressure plus the last step's change
+with a minimal-residual step length (initial_guess = extrapolate), and the
+current concentration as the dispersion solve's initial guess (cn_guess = 1).  Both keep the stopping rule relative to
 ||b||, so the solutions agree with the zero-guess (reference) solves to the
 solver tolerance, with fewer iterations."""
 import numpy as np
@@ -34,8 +35,12 @@ def test_previous_pressure_and_current_c_guesses(gpu_lib, n):
     print("warm start", n, flush=True)
     p0, c0, t0, its0, pits0 = _run(p, 8)
     p1, c1, t1, its1, pits1 = _run(p, 8, initial_guess=capi.GUESS_PREVIOUS, cn_guess=1)
-    print("  its zero", its0, "previous", its1, "; parabolic its zero", pits0, "current c", pits1, flush=True)
+    p2, c2, t2, its2, _ = _run(p, 8, initial_guess=capi.GUESS_EXTRAPOLATE)
+    print("  its zero", its0, "previous", its1, "extrapolated", its2, "; parabolic its zero", pits0,
+          "current c", pits1, flush=True)
     assert rel_err(p1, p0) < 1e-9 and rel_err(c1, c0) < 1e-9
-    assert abs(t1 - t0) <= 1e-9 * t0
+    assert rel_err(p2, p0) < 1e-9 and rel_err(c2, c0) < 1e-9
+    assert abs(t1 - t0) <= 1e-9 * t0 and abs(t2 - t0) <= 1e-9 * t0
     assert sum(its1[1:]) <= sum(its0[1:])
+    assert sum(its2[2:]) <= sum(its1[2:]) + 1
     assert sum(pits1) <= sum(pits0) + 1
