@@ -1649,47 +1649,112 @@ static Grid2 vec_grid(int n1, int n2) {
     return g;
 }
 
-// d_new = (x0 - mean) + beta d_old on the cell and its four neighbours,
-// q = A_N d_new (pinned row zeroed in the eliminated formulation), d.q partials
-template <typename T, class G>
-__device__ __forceinline__ void cg_dir_matvec_body(const G &gr_, LevArgs L, CgArgs c, const T *__restrict__ x0, const double *__restrict__ dold,
-                                double *__restrict__ dnew, double *__restrict__ qv, const double *__restrict__ S,
-                                int first, double *__restrict__ part) {
-    const double mean = S[S_ZSUM] / (double)c.n;
-    const double beta = first ? 0. : S[S_RHO] / S[S_RHO_OLD];
-    const int j = gr_.bx() * 64 + threadIdx.x;
-    double s[1] = {0.};
-    const long ld = c.ld;
-    for (int i = gr_.by() * 4 + threadIdx.y; i < c.n1; i += gr_.gy() * 4) {
-        if (j >= c.n2) break;
-        const long o = (long)(i + 1) * ld + (j + 1);
-#define DN(off_, ii_, jj_)                                                                                  \
-    ((!c.shift && (ii_) == c.pin_i && (jj_) == c.pin_j) ? 0.                                                \
-                                                         : (first ? (double)x0[o + (off_)] - mean           \
-                                                                  : ((double)x0[o + (off_)] - mean) + beta * dold[o + (off_)]))
-        const double dc = DN(0, i, j);
-        double a[9];
-        op_row5(L, i, j, a);
-        double y = a[ST_C] * dc;
-        if (i > 0) y += a[ST_W] * DN(-ld, i - 1, j);
-        if (j > 0) y += a[ST_S] * DN(-1, i, j - 1);
-        if (i < c.n1 - 1) y += a[ST_E] * DN(ld, i + 1, j);
-        if (j < c.n2 - 1) y += a[ST_N] * DN(1, i, j + 1);
-#undef DN
-        if (!c.shift && i == c.pin_i && j == c.pin_j) y = 0.;
-        dnew[o] = dc;
-        qv[o] = y;
-        s[0] += dc * y;
-    }
-    block_reduce<RED_SUM, 1>(s);
-    if (tid_linear() == 0) part[gr_.by() * gr_.gx() + gr_.bx()] = s[0];
-}
+// Direction update and matvec of one CG iteration: d_new = (x0 - mean) +
+// beta d_old (d_new = x0 - mean on a (re)start), q = A_N d_new (pinned row
+// zeroed in the eliminated formulation) and the d_new.q partials.
+// A wavefront owns 64 columns, one per lane, writes the middle 62 (lanes
+// 1..62) and marches down a chunk of R rows, keeping the new direction of
+// rows i-1, i, i+1 in registers; column neighbours come from the adjacent
+// lanes by DPP wave shifts and the inputs of the next DM_PF rows are in
+// flight (branch-free clamped loads).  Each cell's x0 and d_old are read once
+// instead of five times.  Per-wave partials of d.q.
+constexpr int DM_OUT = 62, DM_PF = 4;
+#ifndef IMPEC_DM_WAVES
+#define IMPEC_DM_WAVES 8192
+#endif
+
 template <typename T>
-__global__ void k_cg_dir_matvec(LevArgs L, CgArgs c, const T *__restrict__ x0, const double *__restrict__ dold,
-                                double *__restrict__ dnew, double *__restrict__ qv, const double *__restrict__ S,
-                                int first, double *__restrict__ part) {
-    cg_dir_matvec_body<T>(HwGrid{}, L, c, x0, dold, dnew, qv, S, first, part);
+struct DirMvArgs {
+    LevArgs L;
+    CgArgs c;
+    const T *x0;          // V-cycle output, level-0 ring-1 layout
+    double *d0, *d1;      // the two direction arrays (ensembles pick by iteration parity)
+    double *q;
+    const double *S;
+    double *part;
+    int R, nstrip, nwave;
+};
+
+template <typename T>
+__device__ __forceinline__ void dir_mv_body(const DirMvArgs<T> &A, const double *__restrict__ dold,
+                                            double *__restrict__ dnew, const int first, const int wid) {
+    if (wid >= A.nwave) return;
+    const CgArgs &c = A.c;
+    const int n1 = c.n1, n2 = c.n2;
+    const int lane = threadIdx.x & 63;
+    const int strip = wid % A.nstrip, chunk = wid / A.nstrip;
+    const int j = strip * DM_OUT - 1 + lane;
+    const bool jv = j >= 0 && j < n2;
+    const int jc = j < 0 ? 0 : (j > n2 - 1 ? n2 - 1 : j);
+    const bool outl = lane >= 1 && lane <= DM_OUT && jv;
+    const int i0 = chunk * A.R, i1 = i0 + A.R < n1 ? i0 + A.R : n1;
+    const double mean = A.S[S_ZSUM] / (double)c.n;
+    const double beta = first ? 0. : A.S[S_RHO] / A.S[S_RHO_OLD];
+    const long ld = c.ld;
+    const T *__restrict__ x0 = A.x0;
+    auto row_of = [&](int i) -> long { const int ic = i < 0 ? 0 : (i > n1 - 1 ? n1 - 1 : i); return (long)(ic + 1) * ld + (jc + 1); };
+    // the new direction at (i, j) from its loaded inputs (0 outside the level)
+    auto dn = [&](int i, double xv, double dv) -> double {
+        if (i < 0 || i >= n1 || !jv) return 0.;
+        if (!c.shift && i == c.pin_i && j == c.pin_j) return 0.;
+        return first ? xv - mean : (xv - mean) + beta * dv;
+    };
+    double RX[DM_PF], RD[DM_PF];
+#pragma unroll
+    for (int u = 0; u < DM_PF; u++) {
+        const long o = row_of(i0 + 1 + u);
+        RX[u] = (double)x0[o];
+        RD[u] = first ? 0. : dold[o];
+    }
+    double dm, dcur;
+    {
+        const long om = row_of(i0 - 1), oc = row_of(i0);
+        dm = dn(i0 - 1, (double)x0[om], first ? 0. : dold[om]);
+        dcur = dn(i0, (double)x0[oc], first ? 0. : dold[oc]);
+    }
+    double acc = 0.;
+    for (int kb = i0; kb < i1; kb += DM_PF) {
+#pragma unroll
+        for (int u = 0; u < DM_PF; u++) {
+            const int i = kb + u;
+            const double dnext = dn(i + 1, RX[u], RD[u]);
+            {   // refill the slot with row i + 1 + DM_PF
+                const long o = row_of(i + 1 + DM_PF);
+                RX[u] = (double)x0[o];
+                RD[u] = first ? 0. : dold[o];
+            }
+            if (i < i1) {
+                double a[9];
+                op_row5(A.L, i, jc, a);
+                const double dl = lane_prev(dcur), dr = lane_next(dcur);
+                double y = a[ST_C] * dcur;
+                if (i > 0) y += a[ST_W] * dm;
+                if (j > 0) y += a[ST_S] * dl;
+                if (i < n1 - 1) y += a[ST_E] * dnext;
+                if (j < n2 - 1) y += a[ST_N] * dr;
+                if (!c.shift && i == c.pin_i && j == c.pin_j) y = 0.;
+                if (outl) {
+                    const long o = (long)(i + 1) * ld + (j + 1);
+                    dnew[o] = dcur;
+                    A.q[o] = y;
+                    acc += dcur * y;
+                }
+            }
+            dm = dcur;
+            dcur = dnext;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) A.part[wid] = acc;
 }
+
+template <typename T>
+__global__ void __launch_bounds__(64 * SW_WAVES) k_dir_mv(DirMvArgs<T> A, const double *__restrict__ dold,
+                                                          double *__restrict__ dnew, int first) {
+    dir_mv_body<T>(A, dold, dnew, first, hw_wave());
+}
+
 
 // x += alpha d, r -= alpha q; partials (r.r, sum r)
 template <class G>
@@ -1952,6 +2017,26 @@ static int cg_iteration_unfused(impec_ctx *ctx, const LevArgs &a0, const CgArgs 
     return IMPEC_OK;
 }
 
+template <typename T>
+static DirMvArgs<T> dir_mv_args(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c) {
+    DirMvArgs<T> A;
+    memset(&A, 0, sizeof(A));
+    A.L = a0;
+    A.c = c;
+    A.x0 = LevelArrays<T>::r(ctx->lev[0]);
+    A.d0 = ctx->pd;
+    A.d1 = ctx->pz;
+    A.q = ctx->pq;
+    A.S = ctx->scal;
+    A.part = ctx->part;
+    A.nstrip = (c.n2 + DM_OUT - 1) / DM_OUT;
+    long R = ((long)c.n1 * A.nstrip + IMPEC_DM_WAVES - 1) / IMPEC_DM_WAVES;
+    if (R < 8) R = 8;
+    A.R = (int)R;
+    A.nwave = A.nstrip * ((c.n1 + A.R - 1) / A.R);
+    return A;
+}
+
 // one CG iteration with the streaming V-cycle (nu = 1) in storage type T
 // (double: plain FP64; float: the FP32 cycle of the mixed-precision solve);
 // the CG vectors, the residual and every reduction stay FP64; d ping-pongs
@@ -1966,11 +2051,12 @@ static int cg_iteration_fused(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c
     int st;
     if ((st = vcycle_fused<T>(ctx))) return st;
     hipLaunchKernelGGL(k_finish_rho, dim3(1), dim3(1024), 0, s, ctx->part, nwave0, inv_n, ctx->scal);
+    const DirMvArgs<T> dm = dir_mv_args<T>(ctx, a0, c);
     prof_begin(ctx);
-    hipLaunchKernelGGL((k_cg_dir_matvec<T>), vg.grid, vg.block, 0, s, a0, c, LevelArrays<T>::r(L0), *dold, *dnew,
-                       ctx->pq, ctx->scal, first, ctx->part);
+    hipLaunchKernelGGL((k_dir_mv<T>), dim3((dm.nwave + SW_WAVES - 1) / SW_WAVES), dim3(64 * SW_WAVES), 0, s, dm, *dold,
+                       *dnew, first ? 1 : 0);
     prof_end(ctx, "cg_dir_matvec");
-    if ((st = finish_sum(ctx, vg.nblocks, 1, S_DQ))) return st;
+    if ((st = finish_sum(ctx, dm.nwave, 1, S_DQ))) return st;
     prof_begin(ctx);
     hipLaunchKernelGGL(k_cg_update2, vg.grid, vg.block, 0, s, c, ctx->p, *dnew, ctx->pr, ctx->pq, ctx->scal,
                        ctx->part);
@@ -2210,16 +2296,6 @@ struct RhoRec {
     double inv_n;
     double *S;
 };
-template <typename T>
-struct DirRec {
-    LevArgs L;
-    CgArgs c;
-    const T *x0;
-    double *d0, *d1, *q;
-    const double *S;
-    double *part;
-    int gx, gy;
-};
 struct Upd2Rec {
     CgArgs c;
     double *p, *d0, *d1, *r;
@@ -2254,7 +2330,7 @@ struct LegTabs {
     Batch<FusedArgsT<T>> down[MAX_LEVELS], up[MAX_LEVELS];
     Batch<CoarsestRec<T>> coarsest;    // members without a bottom kernel
     Batch<BottomArgs<T>> bottom;       // members with one
-    Batch<DirRec<T>> dir;
+    Batch<DirMvArgs<T>> dir;
     void free_all() {
         for (int l = 0; l < MAX_LEVELS; l++) {
             batch_free(down[l]);
@@ -2279,8 +2355,9 @@ struct ListDesc {
 struct EnsPressTabs {
     int mixed;
     int lmax;                           // most levels of any member
-    // compacted lists: iter[0] the level-0 vector kernels, iter[1 + l] the legs
-    // of level l; check: the residual recomputation
+    // compacted lists: iter[0] the level-0 vector update, iter[1 + l] the legs
+    // of level l, iter[lmax] the direction update and matvec; check: the
+    // residual recomputation
     ListDesc *d_iter = nullptr, *d_check = nullptr;
     int n_iter = 0;
     std::vector<ListDesc> h_iter, h_check;
@@ -2366,17 +2443,6 @@ __global__ void __launch_bounds__(1024) kb_finish_rho(const RhoRec *__restrict__
     const RhoRec R = tab[blockIdx.x];
     if (!in_state(R.S, need)) return;
     finish_rho_body(VGrid{0, 0, 1, 1}, R.part, R.nblocks, R.inv_n, R.S);
-}
-// the direction arrays alternate every iteration of a solve (d_old, d_new) =
-// (d0, d1) on even iterations and (d1, d0) on odd ones, as in solve_cg
-template <typename T>
-__global__ void kb_dir_matvec(const DirRec<T> *__restrict__ tab, const int *__restrict__ start, int nmem, int need) {
-    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
-        if (!in_state(R.S, need)) return;
-        const bool odd = ((long long)R.S[S_EITS]) & 1;
-        const int first = R.S[S_ESINCE] == 0.;
-        cg_dir_matvec_body<T>(G, R.L, R.c, R.x0, odd ? R.d1 : R.d0, odd ? R.d0 : R.d1, R.q, R.S, first, R.part);
-    });
 }
 __global__ void kb_update2(const Upd2Rec *__restrict__ tab, const int *__restrict__ start, int nmem, int need) {
     batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
@@ -2533,12 +2599,15 @@ __global__ void __launch_bounds__(64 * SW_WAVES) kp_mg_up(const FusedArgsT<T> *_
         mg_up_body<FINE0, false, T>(A, loc);
     });
 }
+// the direction arrays alternate every iteration of a solve: (d_old, d_new) =
+// (d0, d1) on even iterations and (d1, d0) on odd ones, as in solve_cg
 template <typename T>
-__global__ void kp_dir_matvec(const DirRec<T> *__restrict__ tab, ActList act) {
-    act_for(tab, act, [&](const DirRec<T> &R, const VGrid &G) {
-        const bool odd = ((long long)R.S[S_EITS]) & 1;
-        const int first = R.S[S_ESINCE] == 0.;
-        cg_dir_matvec_body<T>(G, R.L, R.c, R.x0, odd ? R.d1 : R.d0, odd ? R.d0 : R.d1, R.q, R.S, first, R.part);
+__global__ void __launch_bounds__(64 * SW_WAVES) kp_dir_mv(const DirMvArgs<T> *__restrict__ tab, ActList act) {
+    act_wave_for(act, [&](int m, int loc) {
+        const DirMvArgs<T> &A = tab[m];
+        const bool odd = ((long long)A.S[S_EITS]) & 1;
+        const int first = A.S[S_ESINCE] == 0.;
+        dir_mv_body<T>(A, odd ? A.d1 : A.d0, odd ? A.d0 : A.d1, first, loc);
     });
 }
 __global__ void kp_update2(const Upd2Rec *__restrict__ tab, ActList act) {
@@ -2610,7 +2679,7 @@ static int build_legs(impec_ensemble *e, LegTabs<T> &t) {
     }
     std::vector<CoarsestRec<T>> cr(n);
     std::vector<BottomArgs<T>> br(n);
-    std::vector<DirRec<T>> dr(n);
+    std::vector<DirMvArgs<T>> dr(n);
     std::vector<int> u_co(n), u_bo(n), u_vec(n);
     for (int k = 0; k < n; k++) {
         impec_ctx *ctx = e->m[k];
@@ -2625,10 +2694,8 @@ static int build_legs(impec_ensemble *e, LegTabs<T> &t) {
             a0.fx = ctx->fx;
             a0.fy = ctx->fy;
         }
-        const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
-        dr[k] = DirRec<T>{a0, cg_args(ctx), LA::r(ctx->lev[0]), ctx->pd, ctx->pz, ctx->pq, ctx->scal, ctx->part,
-                          (int)vg.grid.x, (int)vg.grid.y};
-        u_vec[k] = vg.nblocks;
+        dr[k] = dir_mv_args<T>(ctx, a0, cg_args(ctx));
+        u_vec[k] = dr[k].nwave;
     }
     ENS_CHECK(e, batch_upload(t.coarsest, cr, u_co));
     ENS_CHECK(e, batch_upload(t.bottom, br, u_bo));
@@ -2669,12 +2736,13 @@ static int make_list(impec_ensemble *e, const int *bstart, const std::vector<int
 static int build_lists(impec_ensemble *e) {
     EnsPressTabs *t = e->pres;
     const int n = e->n;
-    t->h_iter.assign(t->lmax, ListDesc());
+    t->h_iter.assign(t->lmax + 1, ListDesc());
     t->h_check.assign(1, ListDesc());
     std::vector<int> all(n, 1);
     int st;
-    const int *vstart = t->mixed ? t->l32.dir.start : t->l64.dir.start;
-    if ((st = make_list(e, vstart, all, t->h_iter[0]))) return st;
+    if ((st = make_list(e, t->upd.start, all, t->h_iter[0]))) return st;
+    const int *dstart = t->mixed ? t->l32.dir.start : t->l64.dir.start;
+    if ((st = make_list(e, dstart, all, t->h_iter[t->lmax]))) return st;
     for (int l = 0; l + 1 < t->lmax; l++) {
         std::vector<int> u(n, 0);
         for (int k = 0; k < n; k++) u[k] = l < top_level(e->m[k]);
@@ -2682,7 +2750,7 @@ static int build_lists(impec_ensemble *e) {
         if ((st = make_list(e, ls, u, t->h_iter[1 + l]))) return st;
     }
     if ((st = make_list(e, t->init.start, all, t->h_check[0]))) return st;
-    t->n_iter = t->lmax;
+    t->n_iter = t->lmax + 1;
     void *pi, *pc;
     ENS_CHECK(e, hipMalloc(&pi, sizeof(ListDesc) * t->h_iter.size()));
     t->list_mem.push_back(pi);
@@ -2769,7 +2837,7 @@ int ens_press_build(impec_ensemble *e) {
         u_red[k] = nb;
         u_vec[k] = vg.nblocks;
         fi[k] = FinishRec{ctx->part, ctx->scal, nb, 2, (int)S_RR, (int)RED_SUM};
-        fd[k] = FinishRec{ctx->part, ctx->scal, vg.nblocks, 1, (int)S_DQ, (int)RED_SUM};
+        fd[k] = FinishRec{ctx->part, ctx->scal, dir_mv_args<double>(ctx, a0, c).nwave, 1, (int)S_DQ, (int)RED_SUM};
         frr[k] = FinishRec{ctx->part, ctx->scal, vg.nblocks, 2, (int)S_RR, (int)RED_SUM};
         ft[k] = FinishRec{ctx->part, ctx->scal, nb, 1, (int)S_TT, (int)RED_SUM};
         const int nwave0 = t->mixed ? fused_args<float>(ctx, 0).nwave : fused_args<double>(ctx, 0).nwave;
@@ -2902,7 +2970,8 @@ static int ens_cg_slot_t(impec_ensemble *e, const LegTabs<T> &lt) {
     ENS_CHECK(e, hipGetLastError());
     ENS_CHECK(e, launch_m(kb_finish_rho, t->rho, dim3(1024), s, (int)EST_ITER));
     const ActList &av = t->h_iter[0].act;
-    hipLaunchKernelGGL(kp_dir_matvec<T>, dim3(persist_grid(lt.dir.total, PERSIST_VEC_WG)), vblk, 0, s, lt.dir.tab, av);
+    hipLaunchKernelGGL(kp_dir_mv<T>, dim3(persist_grid((lt.dir.total + SW_WAVES - 1) / SW_WAVES, PERSIST_LEG_WG)), lblk,
+                       0, s, lt.dir.tab, t->h_iter[lmax].act);
     ENS_CHECK(e, hipGetLastError());
     ENS_CHECK(e, ens_finish(s, t->fin_dq, EST_ITER));
     hipLaunchKernelGGL(kp_update2, dim3(persist_grid(t->upd.total, PERSIST_VEC_WG)), vblk, 0, s, t->upd.tab, av);

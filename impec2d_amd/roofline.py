@@ -28,7 +28,7 @@ KERNELS = {
     "mg_up_l0": dict(rocprof="k_mg_up<true, false", bytes_per_cell=34, bytes_per_cell_fp32=21),
     # CG direction + matvec: read z, d_old, two face-mobility arrays (4 x 8),
     # write d, q (2 x 8); the FP32 cycle's z is 4 B
-    "cg_dir_matvec": dict(rocprof="k_cg_dir_matvec", bytes_per_cell=48, bytes_per_cell_fp32=44),
+    "cg_dir_matvec": dict(rocprof="k_dir_mv", bytes_per_cell=48, bytes_per_cell_fp32=44),
     # CG update: read p, d, r, q (4 x 8), write p, r (2 x 8)
     "cg_update": dict(rocprof="k_cg_update2", bytes_per_cell=48),
     # MUSCL-Hancock x / y sweep: read c and one velocity component, write c
