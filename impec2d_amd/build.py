@@ -75,16 +75,19 @@ HOST_RUN = os.path.join(HERE, "impec2d_run")
 GXX_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
 
+HOST_SOURCES = ["impec_solvers.cpp", "impec_config.cpp", "impec_output.cpp"]
+
+
 def build_host(verbose=False):
-    """The C++ solver classes (host/impec_solvers.cpp) as libimpec_host.so and
+    """The C++ solver classes, configuration reader and output path (host/) as libimpec_host.so and
     the impec2d_run driver, both linked against libimpec_hip.so (g++ only: no
     device code on this side)."""
     gxx = os.environ.get("CXX", "g++")
     inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host")]
     link = ["-L", HERE, "-limpec_hip", "-Wl,-rpath,$ORIGIN"]
     steps = [
-        [gxx] + GXX_FLAGS + ["-shared"] + inc + [os.path.join(HERE, "host", "impec_solvers.cpp"), "-o",
-                                                 HOST_LIB + ".tmp"] + link,
+        [gxx] + GXX_FLAGS + ["-shared"] + inc + [os.path.join(HERE, "host", f) for f in HOST_SOURCES] +
+        ["-o", HOST_LIB + ".tmp"] + link,
         [gxx] + GXX_FLAGS + inc + [os.path.join(HERE, "host", "impec2d_run.cpp"), "-o", HOST_RUN + ".tmp",
                                    "-L", HERE, "-limpec_host"] + link,
     ]

@@ -3,6 +3,7 @@
 // runs the step loop of Impec2DSolver::Solve and optionally writes the final
 // fields and diagnostics.
 //
+//   impec2d_run --config file.cfg [options below, which override the file]
 //   impec2d_run [--N n] [--N1 n --N2 n] [--steps k] [--tfinal t] [--stype diagonal|parallel]
 //               [--nu x] [--mu-res x] [--mu-inj x] [--cfl x] [--eps x] [--para-eps x]
 //               [--diffusion] [--alpha-l x] [--alpha-t x] [--d-m x] [--alpha-absolute]
@@ -10,9 +11,14 @@
 //               [--meanconc] [--porosity x] [--guess zero|previous] [--quiet]
 //               [--checks] [--dump file] [--diagnostics file]
 //
+// --config reads an Impec2D configuration file (libconfig syntax, the keys of
+// SimParams.cpp:45-463) and, as the reference does, writes the final fields
+// and diagnostics.dat into its results_dir and, with log_to_file, the
+// concentration every log_skip_steps steps into log_dir.
 // --dump writes c, p (each (N1+4) x (N2+4)) and v ((N1+5) x (N2+5) x 2) as raw
 // little-endian doubles, in that order.  Exit status 0 on success, 1 on a
-// solver error, 2 on a usage error, 3 when no GPU can be used.
+// solver error, 2 on a usage or configuration error, 3 when no GPU can be used.
+#include "impec_config.hpp"
 #include "impec_solvers.hpp"
 
 #include <cstdio>
@@ -25,7 +31,7 @@
 using namespace impec_host;
 
 static void usage() {
-    std::cerr << "usage: impec2d_run [--N n] [--steps k] [--tfinal t] [--stype diagonal|parallel] [--nu x]\n"
+    std::cerr << "usage: impec2d_run [--config file.cfg] [--N n] [--steps k] [--tfinal t] [--stype diagonal|parallel] [--nu x]\n"
                  "                   [--diffusion] [--alpha-l x] [--alpha-t x] [--d-m x] [--alpha-absolute]\n"
                  "                   [--upwind] [--split mode] [--meanconc] [--guess zero|previous]\n"
                  "                   [--quiet] [--checks] [--dump file] [--diagnostics file]\n";
@@ -36,6 +42,17 @@ int main(int argc, char **argv) {
     sp.tfinal = 1e30;
     sp.max_steps = 10;
     std::string dump, diag;
+    bool print_params = false;
+    for (int a = 1; a + 1 < argc; a++)   // a configuration file first; the options override it
+        if (std::string(argv[a]) == "--config") {
+            std::string err;
+            if (!ReadConfigFile(argv[a + 1], sp, err)) {
+                std::cerr << argv[a + 1] << ": " << err << std::endl;
+                return 2;
+            }
+            sp.max_steps = 0;
+            sp.write_results = true;
+        }
     for (int a = 1; a < argc; a++) {
         const std::string k = argv[a];
         auto val = [&]() -> const char * {
@@ -45,7 +62,8 @@ int main(int argc, char **argv) {
             }
             return argv[++a];
         };
-        if (k == "--N") sp.N1 = sp.N2 = std::atoi(val());
+        if (k == "--config") val();
+        else if (k == "--N") sp.N1 = sp.N2 = std::atoi(val());
         else if (k == "--N1") sp.N1 = std::atoi(val());
         else if (k == "--N2") sp.N2 = std::atoi(val());
         else if (k == "--steps") sp.max_steps = std::atoi(val());
@@ -77,6 +95,9 @@ int main(int argc, char **argv) {
         else if (k == "--quiet") sp.verbose = false;
         else if (k == "--checks") sp.check_mass_conservation = sp.check_noflow_boundary = sp.check_divergence = true;
         else if (k == "--dump") dump = val();
+        else if (k == "--results-dir") sp.results_dir = val();
+        else if (k == "--log-dir") sp.log_dir = val();
+        else if (k == "--print-params") print_params = true;
         else if (k == "--diagnostics") diag = val();
         else {
             usage();
@@ -86,6 +107,29 @@ int main(int argc, char **argv) {
     if (sp.N1 < 2 || sp.N2 < 2) {
         usage();
         return 2;
+    }
+    if (print_params) {   // the parameters as read, one "name value" line each; no device needed
+        sp.Finalize();
+        std::cout.precision(17);
+        std::cout << "N1 " << sp.N1 << "\nN2 " << sp.N2 << "\nc_bounds " << sp.c_bounds << "\nx_lb " << sp.x_lb
+                  << "\nx_ub " << sp.x_ub << "\ny_lb " << sp.y_lb << "\ny_ub " << sp.y_ub << "\ndx " << sp.dx
+                  << "\ndy " << sp.dy << "\ntstart " << sp.tstart << "\ntfinal " << sp.tfinal << "\nI " << sp.I
+                  << "\nstype " << (sp.stype == PARALLEL ? "parallel" : "diagonal") << "\neps " << sp.eps
+                  << "\npara_eps " << sp.para_eps << "\nmu_res " << sp.mu_res << "\nmu_inj " << sp.mu_inj
+                  << "\navgproc " << (sp.avgproc == HARMONIC ? "harmonic" : "mean_concentration")
+                  << "\nnu " << sp.nu << "\ncfl " << sp.cfl << "\nporosity " << sp.porosity
+                  << "\ninj_fluid_conc " << sp.inj_fluid_conc << "\ndiffusion " << sp.diffusion
+                  << "\nalpha_l " << sp.alpha_l << "\nalpha_t " << sp.alpha_t << "\ndim_splitting "
+                  << (int)sp.dim_splitting << "\nwell_profile " << (int)sp.well_profile << "\nwell_rc "
+                  << sp.well_rc << "\nlog_to_file " << sp.log_to_file << "\nlog_skip_steps " << sp.log_skip_steps
+                  << "\nlog_dir " << sp.log_dir << "\nresults_dir " << sp.results_dir
+                  << "\ncheck_mass_conservation " << sp.check_mass_conservation << "\ncheck_noflow_boundary "
+                  << sp.check_noflow_boundary << "\ncheck_divergence " << sp.check_divergence << "\ntimes";
+        for (double t : sp.times) std::cout << " " << t;
+        std::cout << "\ncounters";
+        for (int c : sp.counters) std::cout << " " << c;
+        std::cout << std::endl;
+        return 0;
     }
     try {
         Impec2DSolver solver(sp);

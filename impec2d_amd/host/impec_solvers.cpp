@@ -3,6 +3,7 @@
 // libimpec_hip.so; this file only orders the calls and keeps host arrays
 // coherent with the device.
 #include "impec_solvers.hpp"
+#include "impec_output.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -466,11 +467,19 @@ void Impec2DSolver::Initialize() {
     InitializeConcentration();
 }
 
-// one_cell wells: +I/(dx dy) at the injectors, -I/(dx dy) at the producers
-// (Impec2DSolver.cpp:241-280; InitOneCellDiagonal / InitOneCellParallel, Util.cpp:565-586)
+// Impec2DSolver::InitializeSource (Impec2DSolver.cpp:241-280): one-cell wells
+// of +-I/(dx dy) (InitOneCellDiagonal / InitOneCellParallel, Util.cpp:565-586)
+// or exact quarter-circle wells of radius well_rc (x_ub - x_lb) and density
+// 4 I / (pi r^2) (Util.cpp:444-559); a Gaussian profile leaves q = 0, as there
 void Impec2DSolver::InitializeSource() {
     const int N1 = sp.N1, N2 = sp.N2;
     q.assign((size_t)N1 * N2, 0.);
+    if (sp.well_profile == CIRCULAR) {
+        const double r = sp.well_rc * (sp.x_ub - sp.x_lb);
+        InitQuarterCircle(sp, q, r, sp.I * 4. / (M_PI * r * r));
+        return;
+    }
+    if (sp.well_profile != ONECELL) return;
     const double val = sp.I / (sp.dx * sp.dy);
     if (sp.stype == DIAGONAL) {
         q[0] = val;
@@ -480,6 +489,45 @@ void Impec2DSolver::InitializeSource() {
         q[(size_t)(N1 - 1) * N2 + (N2 - 1)] = val;
         q[(size_t)(N2 - 1)] = -val;
         q[(size_t)(N1 - 1) * N2] = -val;
+    }
+}
+
+bool Impec2DSolver::IsLoggingStep(int counter) const {
+    return std::find(sp.counters.begin(), sp.counters.end(), counter) != sp.counters.end();
+}
+
+bool Impec2DSolver::IsLoggingTime(double t) const {
+    return std::find(sp.times.begin(), sp.times.end(), t) != sp.times.end();
+}
+
+void Impec2DSolver::LogResultsMatch(double postfix, bool logTime) {
+    std::string dir, suffix;
+    if (postfix < 0) {   // the end of the simulation
+        dir = sp.results_dir;
+        suffix = ".dat";
+    } else {
+        dir = sp.log_dir;
+        std::stringstream ss;
+        ss << "_";
+        if (logTime) ss << "t_";
+        ss << postfix;
+        suffix = ss.str() + ".dat";
+    }
+    const std::string cfile = dir + "/concentration" + suffix, pfile = dir + "/pressure" + suffix;
+    const bool final = postfix < 0;
+    if (sp.stype == PARALLEL) {
+        PrintProjectionToPlotFile(cfile, ProjectParallelToDiagonal(c, sp.x_lb, sp.y_lb, sp.x_ub, sp.y_ub), sp.x_lb,
+                                  sp.y_lb);
+        if (final)
+            PrintProjectionToPlotFile(pfile, ProjectParallelToDiagonal(p, sp.x_lb, sp.y_lb, sp.x_ub, sp.y_ub),
+                                      sp.x_lb, sp.y_lb);
+    } else {
+        PrintDToPlotFile(cfile, c, sp.x_lb, sp.y_lb, sp.dx, sp.dy);
+        if (final) {
+            PrintDToPlotFile(pfile, p, sp.x_lb, sp.y_lb, sp.dx, sp.dy);
+            PrintVelToPlotFile(dir + "/velocity_u" + suffix, dir + "/velocity_v" + suffix, v, sp.x_lb, sp.y_lb,
+                               sp.dx, sp.dy);
+        }
     }
 }
 
@@ -580,6 +628,13 @@ bool Impec2DSolver::Solve() {
         ds.diff_err = sp.check_divergence ? CalculateDivergenceError() : -1;
         diagstates.push_back(ds);
         if (sp.verbose) OutputState(ds);
+        if (sp.log_to_file) {   // Impec2DSolver.cpp:420-428
+            if (IsLoggingTime(t)) LogResultsMatch(t, true);
+            // log_skip_steps = 0: only the steps of the list
+            if (IsLoggingStep(counter) ||
+                (sp.log_skip_steps != 0 && (counter == 1 || counter % sp.log_skip_steps == 0)))
+                LogResultsMatch(counter, false);
+        }
         double cmax = 0, cmin = 0;
         if (dev->ok(impec_minmax_concentration(dev->ctx(), &cmax, &cmin), "Impec2DSolver::Solve")) {
             maxc = std::max(maxc, cmax);
@@ -590,6 +645,10 @@ bool Impec2DSolver::Solve() {
         std::cout << "overall maximum: " << maxc << std::endl;
         std::cout << "overall minimum: " << minc << std::endl;
         std::cout << "Average no. of Iterations: " << es->GetAvgIts() << std::endl;
+    }
+    if (sp.write_results) {   // Impec2DSolver.cpp:461-462
+        LogResultsMatch(-1, false);
+        WriteDiagnostics(sp.results_dir + "/diagnostics.dat");
     }
     return state;
 }

@@ -37,6 +37,7 @@ enum DimSplitting {   // SimParams.h:21, same order
 };
 enum HyperbolicMethod { MUSCL = IMPEC_HYPER_MUSCL, UPWIND = IMPEC_HYPER_UPWIND };
 enum ParabolicMethod { EXPLICIT = IMPEC_PARA_EXPLICIT, CRANKNICOLSON = IMPEC_PARA_CN };
+enum WellProfile { ONECELL, CIRCULAR, GAUSSIAN };   // SimParams.h well_profile; GAUSSIAN leaves q = 0
 
 // The SimParams fields the hot path reads (SimParams.h:29-164), with the
 // reference's names.
@@ -59,6 +60,17 @@ struct SimParams {
     bool check_mass_conservation = false, check_noflow_boundary = false, check_divergence = false;
     bool verbose = true;                         // console line per step (Impec2DSolver::OutputState)
     int precision = IMPEC_PREC_FP64, initial_guess = IMPEC_GUESS_ZERO, max_iterations = 10000;
+    // wells (Impec2DSolver::InitializeSource, Impec2DSolver.cpp:241-280)
+    WellProfile well_profile = ONECELL;
+    double well_rc = 0.05;                       // circle radius as a fraction of x_ub - x_lb
+    // output (Impec2DSolver::LogResultsMatch / WriteDiagnostics)
+    bool log_to_file = false;
+    int log_skip_steps = 0;                      // 0: only the steps in counters
+    std::vector<int> counters;                   // log_counter_list
+    bool time_matching = false, log_mass_conservation = false;
+    double max_mass_frac_error = 0.;
+    std::string log_dir = ".", results_dir = ".";
+    bool write_results = false;                  // final fields and diagnostics.dat into results_dir
 
     void Finalize();                             // dx, dy from the domain and N1, N2
     impec_params ToImpec() const;
@@ -95,6 +107,7 @@ class Array2d {
     int n1() const { return n1_; }
     int n2() const { return n2_; }
     int bounds() const { return b_; }
+    int ld() const { return ld_; }   // row pitch of data()
 
   private:
     friend class Device;
@@ -133,6 +146,10 @@ class Array3d {
         for_read();
         return buf_.data();
     }
+    int n1() const { return n1_; }   // cells: the faces run to n1 and n2
+    int n2() const { return n2_; }
+    int bounds() const { return b_; }
+    int ld() const { return ld_; }   // row pitch of data(), in (u, v) pairs
 
   private:
     friend class Device;
@@ -339,8 +356,14 @@ class Impec2DSolver {
 
   private:
     void Initialize();
-    void InitializeSource();          // one-cell wells (Impec2DSolver.cpp:241-280, Util.cpp:565-586)
+    void InitializeSource();          // one-cell or circle wells (Impec2DSolver.cpp:241-280, Util.cpp:444-586)
     void InitializeConcentration();   // c = 0 (Impec2DSolver.cpp:191-233 without an init file)
+    // plot files of c (and, at the end, p and v) into log_dir / results_dir;
+    // parallel grids are projected onto the diagonal grid first
+    // (LogResultsMatch, Impec2DSolver.cpp:622-717)
+    void LogResultsMatch(double postfix, bool logTime);
+    bool IsLoggingStep(int counter) const;   // Impec2DSolver.cpp:594-604
+    bool IsLoggingTime(double t) const;
     double CalculateMassError(double t);
     double CalculateVelocityBoundaryError();
     double CalculateDivergenceError();
