@@ -13,9 +13,46 @@
 
 namespace impec {
 
+// Build-time option, off by default.  The dispatcher hands the workgroups of a
+// launch to the eight XCDs in turn (workgroup k to XCD k mod 8), and each XCD
+// has its own L2.  With IMPEC_XCD_SWIZZLE=1, xcd_block renumbers the first
+// 8 * floor(nb / 8) workgroups so that each XCD receives one consecutive range
+// of logical workgroups, and tiles that share cache lines (edge columns of
+// neighbouring strips, halo rows of neighbouring tiles) meet in one L2.
+// Measured at 4096^2 on MI355X: FETCH_SIZE drops by up to 46% per kernel
+// (the sweeps, velocities, CN tensors, CG update), yet every kernel runs 2-15%
+// slower and the step 5% slower.  The duplicate fetches of the round-robin
+// order are served by the Infinity Cache, while eight XCDs streaming eight
+// distant regions lose DRAM locality.  A bijection on [0, nb); no kernel of
+// this library waits on another workgroup, so the order carries no progress
+// guarantee either way.
+#ifndef IMPEC_XCD_SWIZZLE
+#define IMPEC_XCD_SWIZZLE 0
+#endif
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+#if IMPEC_XCD_SWIZZLE
+    const int per = nb >> 3;
+    return b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
+#else
+    (void)nb;
+    return b;
+#endif
+}
+
+// the hardware grid (renumbered by xcd_block when that is enabled, x fastest
+// as the dispatcher walks them); partial sums indexed by by() * gx() + bx()
+// land where the plain launch puts them
 struct HwGrid {
+#if IMPEC_XCD_SWIZZLE
+    __device__ __forceinline__ int lin() const {
+        return xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    }
+    __device__ __forceinline__ int bx() const { return gridDim.y == 1 ? lin() : lin() % gridDim.x; }
+    __device__ __forceinline__ int by() const { return gridDim.y == 1 ? 0 : lin() / gridDim.x; }
+#else
     __device__ __forceinline__ int bx() const { return blockIdx.x; }
     __device__ __forceinline__ int by() const { return blockIdx.y; }
+#endif
     __device__ __forceinline__ int gx() const { return gridDim.x; }
     __device__ __forceinline__ int gy() const { return gridDim.y; }
 };

@@ -56,8 +56,9 @@ struct CnConst {
 __global__ void k_cn_tensor(CnConst k, const double *__restrict__ u, const double *__restrict__ v,
                             double *__restrict__ d11, double *__restrict__ d12, double *__restrict__ d22,
                             double *__restrict__ d21) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const HwGrid g_;
+    const int j = g_.bx() * blockDim.x + threadIdx.x;
+    const int i = g_.by() * blockDim.y + threadIdx.y;
     if (i > k.N1 || j > k.N2) return;
     const long L = k.ldv;
     const long o = (long)(i + k.B) * L + (j + k.B);
@@ -115,8 +116,9 @@ __device__ __forceinline__ void cn_row(const CnConst &k, const double *__restric
 __global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ d11,
                          const double *__restrict__ d12, const double *__restrict__ d22,
                          const double *__restrict__ d21, double *__restrict__ rhs, double *__restrict__ diag) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const HwGrid g_;
+    const int j = g_.bx() * blockDim.x + threadIdx.x;
+    const int i = g_.by() * blockDim.y + threadIdx.y;
     if (i >= k.N1 || j >= k.N2) return;
     const long L = k.ldv, t = (long)i * L + j;
     const double d11m = d11[t], d12m = d12[t], d11p = d11[t + L], d12p = d12[t + L];
@@ -175,7 +177,7 @@ struct CnArgs {
 template <int MODE>
 __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
     const int lane = threadIdx.x & 63;
-    const int wid = blockIdx.x * CN_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wid = xcd_block(blockIdx.x, gridDim.x) * CN_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wid >= A.nwave) return;
     const CnConst &k = A.k;
     const int N1 = k.N1, N2 = k.N2;
@@ -316,10 +318,11 @@ static dim3 cn_vgrid(int N1, int N2) {
 __global__ void k_bicg_init(int N1, int N2, const double *__restrict__ b, double *__restrict__ r,
                             double *__restrict__ r0, double *__restrict__ p, double *__restrict__ vv,
                             double *__restrict__ x, double *__restrict__ part) {
-    const int j = blockIdx.x * 64 + threadIdx.x;
+    const HwGrid g_;
+    const int j = g_.bx() * 64 + threadIdx.x;
     double s[2] = {0., 0.};
     if (j < N2)
-        for (int i = blockIdx.y * 4 + threadIdx.y; i < N1; i += gridDim.y * 4) {
+        for (int i = g_.by() * 4 + threadIdx.y; i < N1; i += g_.gy() * 4) {
             const long id = (long)i * N2 + j;
             const double bv = b[id];
             r[id] = bv;
@@ -332,7 +335,7 @@ __global__ void k_bicg_init(int N1, int N2, const double *__restrict__ b, double
         }
     block_reduce<RED_SUM, 2>(s);
     if (tid_linear() == 0) {
-        const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+        const int blk = g_.by() * g_.gx() + g_.bx();
         part[blk] = s[0];
         part[RED_BLOCKS + blk] = s[1];
     }
@@ -344,10 +347,11 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
                              const double *__restrict__ d12, const double *__restrict__ d22,
                              const double *__restrict__ d21, const double *__restrict__ b, double *__restrict__ x,
                              double *__restrict__ r, double *__restrict__ r0, double *__restrict__ part) {
-    const int j = blockIdx.x * 64 + threadIdx.x;
+    const HwGrid g_;
+    const int j = g_.bx() * 64 + threadIdx.x;
     double s[2] = {0., 0.};
     if (j < k.N2)
-        for (int i = blockIdx.y * 4 + threadIdx.y; i < k.N1; i += gridDim.y * 4) {
+        for (int i = g_.by() * 4 + threadIdx.y; i < k.N1; i += g_.gy() * 4) {
             double a[9];
             cn_row(k, d11, d12, d22, d21, i, j, a);
             const double *cc = c + (long)(i + k.B) * lds + (j + k.B);
@@ -372,7 +376,7 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
         }
     block_reduce<RED_SUM, 2>(s);
     if (tid_linear() == 0) {
-        const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+        const int blk = g_.by() * g_.gx() + g_.bx();
         part[blk] = s[0];
         part[RED_BLOCKS + blk] = s[1];
     }
@@ -383,11 +387,12 @@ __global__ void k_bicg_xr(int N1, int N2, double *__restrict__ x, const double *
                           const double *__restrict__ sv, const double *__restrict__ dinv, double *__restrict__ r,
                           const double *__restrict__ t, const double *__restrict__ r0, const double *__restrict__ S,
                           double *__restrict__ part) {
+    const HwGrid g_;
     const double alpha = S[S_ALPHA], om = S[S_OMEGA];
-    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int j = g_.bx() * 64 + threadIdx.x;
     double s[2] = {0., 0.};
     if (j < N2)
-        for (int i = blockIdx.y * 4 + threadIdx.y; i < N1; i += gridDim.y * 4) {
+        for (int i = g_.by() * 4 + threadIdx.y; i < N1; i += g_.gy() * 4) {
             const long id = (long)i * N2 + j;
             const double d = dinv[id], sval = sv[id];
             x[id] += alpha * (p[id] * d) + om * (sval * d);
@@ -398,7 +403,7 @@ __global__ void k_bicg_xr(int N1, int N2, double *__restrict__ x, const double *
         }
     block_reduce<RED_SUM, 2>(s);
     if (tid_linear() == 0) {
-        const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+        const int blk = g_.by() * g_.gx() + g_.bx();
         part[blk] = s[0];
         part[RED_BLOCKS + blk] = s[1];
     }
@@ -438,8 +443,9 @@ __global__ void k_bicg_finish(const double *__restrict__ part, int nb, int stage
 
 // solution into the interior of c
 __global__ void k_cn_store(const double *__restrict__ x, double *__restrict__ c, int N1, int N2, int B, int lds) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const HwGrid g_;
+    const int j = g_.bx() * blockDim.x + threadIdx.x;
+    const int i = g_.by() * blockDim.y + threadIdx.y;
     if (i >= N1 || j >= N2) return;
     c[(long)(i + B) * lds + (j + B)] = x[(long)i * N2 + j];
 }
@@ -447,8 +453,9 @@ __global__ void k_cn_store(const double *__restrict__ x, double *__restrict__ c,
 // matrix dump: nine N1*N2 arrays
 __global__ void k_cn_dump(CnConst k, const double *__restrict__ d11, const double *__restrict__ d12,
                           const double *__restrict__ d22, const double *__restrict__ d21, double *__restrict__ A) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const HwGrid g_;
+    const int j = g_.bx() * blockDim.x + threadIdx.x;
+    const int i = g_.by() * blockDim.y + threadIdx.y;
     if (i >= k.N1 || j >= k.N2) return;
     double a[9];
     cn_row(k, d11, d12, d22, d21, i, j, a);

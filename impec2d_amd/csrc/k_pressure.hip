@@ -1043,6 +1043,12 @@ struct Chunk {
 __device__ __forceinline__ int hw_wave() {
     return blockIdx.x * SW_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 }
+// the same with the workgroups renumbered by xcd_block, for the launches over
+// one grid: the strips of a chunk, which share their edge columns, then run on
+// one XCD
+__device__ __forceinline__ int xcd_wave() {
+    return xcd_block(blockIdx.x, gridDim.x) * SW_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
 
 // strip and chunk of wave `wid` of a leg's launch over one level
 template <typename Args>
@@ -1229,7 +1235,7 @@ __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int w
 
 template <bool FINE0, bool ANISO, typename T>
 __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgsT<T> A) {
-    mg_down_body<FINE0, ANISO, T>(A, hw_wave());
+    mg_down_body<FINE0, ANISO, T>(A, xcd_wave());
 }
 
 template <bool FINE0, bool ANISO, typename T>
@@ -1358,7 +1364,7 @@ __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid
 
 template <bool FINE0, bool ANISO, typename T>
 __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgsT<T> A) {
-    mg_up_body<FINE0, ANISO, T>(A, hw_wave());
+    mg_up_body<FINE0, ANISO, T>(A, xcd_wave());
 }
 
 // ===========================================================================
@@ -1752,7 +1758,7 @@ __device__ __forceinline__ void dir_mv_body(const DirMvArgs<T> &A, const double 
 template <typename T>
 __global__ void __launch_bounds__(64 * SW_WAVES) k_dir_mv(DirMvArgs<T> A, const double *__restrict__ dold,
                                                           double *__restrict__ dnew, int first) {
-    dir_mv_body<T>(A, dold, dnew, first, hw_wave());
+    dir_mv_body<T>(A, dold, dnew, first, xcd_wave());
 }
 
 
