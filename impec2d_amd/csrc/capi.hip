@@ -68,10 +68,14 @@ static int ensure_cn(impec_ctx *ctx) {
     if ((st = dalloc(ctx, &ctx->d11, nt)) || (st = dalloc(ctx, &ctx->d12, nt)) || (st = dalloc(ctx, &ctx->d22, nt)) ||
         (st = dalloc(ctx, &ctx->d21, nt)))
         return st;
-    double **v[11] = {&ctx->cn_rhs, &ctx->cn_x, &ctx->cn_r, &ctx->cn_r0, &ctx->cn_p, &ctx->cn_v,
-                      &ctx->cn_s, &ctx->cn_t, &ctx->cn_y, &ctx->cn_z, &ctx->cn_dg};
-    for (int k = 0; k < 11; k++)
+    double **v[10] = {&ctx->cn_rhs, &ctx->cn_x, &ctx->cn_r, &ctx->cn_r0, &ctx->cn_p, &ctx->cn_v,
+                      &ctx->cn_s, &ctx->cn_t, &ctx->cn_y, &ctx->cn_z};
+    for (int k = 0; k < 10; k++)
         if ((st = dalloc(ctx, v[k], n))) return st;
+    if (hipMalloc((void **)&ctx->cn_dg, n * sizeof(float)) != hipSuccess) {
+        ctx->cn_dg = nullptr;
+        return set_error(ctx, IMPEC_ENOMEM, "device allocation of %zu floats failed", n);
+    }
     return IMPEC_OK;
 }
 
@@ -236,10 +240,11 @@ int impec_destroy(impec_ctx *ctx) {
     double *bufs[] = {ctx->c, ctx->p, ctx->u, ctx->v, ctx->q, ctx->K11, ctx->K22 != ctx->K11 ? ctx->K22 : nullptr,
                       ctx->w, ctx->w22, ctx->t1, ctx->t2, ctx->t3, ctx->scal, ctx->part, ctx->d11, ctx->d12,
                       ctx->d22, ctx->d21, ctx->cn_rhs, ctx->cn_x, ctx->cn_r, ctx->cn_r0, ctx->cn_p, ctx->cn_v,
-                      ctx->cn_s, ctx->cn_t, ctx->cn_y, ctx->cn_z, ctx->cn_dg, ctx->src_q, ctx->tr_md, ctx->p_prev,
+                      ctx->cn_s, ctx->cn_t, ctx->cn_y, ctx->cn_z, ctx->src_q, ctx->tr_md, ctx->p_prev,
                       ctx->p_old};
     for (double *b : bufs)
         if (b) hipFree(b);
+    if (ctx->cn_dg) hipFree(ctx->cn_dg);
     if (ctx->src_idx) hipFree(ctx->src_idx);
     if (ctx->nlev) pressure_free_levels(ctx);
     if (ctx->h_scal) hipHostFree(ctx->h_scal);

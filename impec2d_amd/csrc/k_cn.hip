@@ -51,8 +51,38 @@ struct CnConst {
     double cx2, cy2;        // dt/(2.*dx*dx), dt/(2.*dy*dy)
 };
 
-// SetupDiffusionTensorF / SetupDiffusionTensorG (MGSolver.cpp:846-919) on
-// every x-face (i in [0,N1], j in [0,N2)) and y-face (i in [0,N1), j in [0,N2])
+// SetupDiffusionTensorF (MGSolver.cpp:846-882) at one x-face from u and the
+// four-point average vb of v
+__device__ __forceinline__ void xface_tensor(const CnConst &k, double uu, double vb, double &d11, double &d12) {
+    const double u2 = uu * uu, v2 = vb * vb;
+    const double s = sqrt(u2 + v2);
+    double fl = k.phal / s, ft = k.phat / s;
+    if (k.guard && !(s > 0)) { fl = 0.; ft = 0.; }
+    const double a11 = fl * u2, a12 = fl * (uu * vb);
+    const double b11 = ft * v2, b12 = ft * (-uu * vb);
+    double x11 = a11 + b11;
+    if (k.phdm != 0.) x11 = x11 + k.phdm;
+    d11 = x11;
+    d12 = a12 + b12;
+}
+
+// SetupDiffusionTensorG (MGSolver.cpp:884-919) at one y-face from the
+// four-point average ub of u and v
+__device__ __forceinline__ void yface_tensor(const CnConst &k, double ub, double vv, double &d22, double &d21) {
+    const double u2 = ub * ub, v2 = vv * vv;
+    const double s = sqrt(u2 + v2);
+    double fl = k.phal / s, ft = k.phat / s;
+    if (k.guard && !(s > 0)) { fl = 0.; ft = 0.; }
+    const double a21 = fl * (ub * vv), a22 = fl * v2;
+    const double b21 = ft * (-ub * vv), b22 = ft * u2;
+    d21 = a21 + b21;
+    double x22 = a22 + b22;
+    if (k.phdm != 0.) x22 = x22 + k.phdm;
+    d22 = x22;
+}
+
+// the face tensors on every x-face (i in [0,N1], j in [0,N2)) and y-face
+// (i in [0,N1), j in [0,N2]); u, v ghost-filled
 __global__ void k_cn_tensor(CnConst k, const double *__restrict__ u, const double *__restrict__ v,
                             double *__restrict__ d11, double *__restrict__ d12, double *__restrict__ d22,
                             double *__restrict__ d21) {
@@ -64,32 +94,12 @@ __global__ void k_cn_tensor(CnConst k, const double *__restrict__ u, const doubl
     const long o = (long)(i + k.B) * L + (j + k.B);
     const long t = (long)i * L + j;
     if (j < k.N2) {
-        const double uu = u[o];
         const double vb = 1. / 4 * (v[o - L + 1] + v[o + 1] + v[o - L] + v[o]);
-        const double u2 = uu * uu, v2 = vb * vb;
-        const double s = sqrt(u2 + v2);
-        double fl = k.phal / s, ft = k.phat / s;
-        if (k.guard && !(s > 0)) { fl = 0.; ft = 0.; }
-        const double a11 = fl * u2, a12 = fl * (uu * vb);
-        const double b11 = ft * v2, b12 = ft * (-uu * vb);
-        double x11 = a11 + b11;
-        if (k.phdm != 0.) x11 = x11 + k.phdm;
-        d11[t] = x11;
-        d12[t] = a12 + b12;
+        xface_tensor(k, u[o], vb, d11[t], d12[t]);
     }
     if (i < k.N1) {
         const double ub = 1. / 4 * (u[o + L - 1] + u[o + L] + u[o - 1] + u[o]);
-        const double vv = v[o];
-        const double u2 = ub * ub, v2 = vv * vv;
-        const double s = sqrt(u2 + v2);
-        double fl = k.phal / s, ft = k.phat / s;
-        if (k.guard && !(s > 0)) { fl = 0.; ft = 0.; }
-        const double a21 = fl * (ub * vv), a22 = fl * v2;
-        const double b21 = ft * (-ub * vv), b22 = ft * u2;
-        d21[t] = a21 + b21;
-        double x22 = a22 + b22;
-        if (k.phdm != 0.) x22 = x22 + k.phdm;
-        d22[t] = x22;
+        yface_tensor(k, ub, v[o], d22[t], d21[t]);
     }
 }
 
@@ -115,7 +125,7 @@ __device__ __forceinline__ void cn_row(const CnConst &k, const double *__restric
 // SetRHS_Parabolic (MGSolver.cpp:1324-1370); c ghost-filled
 __global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ d11,
                          const double *__restrict__ d12, const double *__restrict__ d22,
-                         const double *__restrict__ d21, double *__restrict__ rhs, double *__restrict__ diag) {
+                         const double *__restrict__ d21, double *__restrict__ rhs, float *__restrict__ diag) {
     const HwGrid g_;
     const int j = g_.bx() * blockDim.x + threadIdx.x;
     const int i = g_.by() * blockDim.y + threadIdx.y;
@@ -136,24 +146,35 @@ __global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const
     rhs[(long)i * k.N2 + j] = r;
     double a[9];
     cn_row(k, d11, d12, d22, d21, i, j, a);
-    diag[(long)i * k.N2 + j] = 1. / a[ST_C];   // Jacobi preconditioner
+    diag[(long)i * k.N2 + j] = (float)(1. / a[ST_C]);   // Jacobi preconditioner, rounded to FP32
 }
 
 // ---------------------------------------------------------------------------
-// Jacobi-preconditioned BiCGStab, streamed.  The two operator applications of
-// an iteration carry the vector update that produces their input:
-//   k_cn_stream<0>: p = r + beta (p - omega v),  v = A (p / D),  partial r0.v
-//   k_cn_stream<1>: s = r - alpha v,             t = A (s / D),  partials t.s, t.t
-// and k_bicg_xr finishes the iteration (x += alpha p/D + omega s/D,
-// r = s - omega t, partials r.r and r0.r).  A wavefront owns a strip of 64
-// columns (one per lane, 62 written) and a chunk of rows and marches down the
-// rows with the input vector of three rows in registers; the column
-// neighbours come from the adjacent lanes.  The scalars live in device memory
-// (rho_new = r0.r, rho, alpha, omega), so an iteration needs one host
+// Jacobi-preconditioned BiCGStab, streamed.  Each operator application carries
+// the vector updates on either side of it, so an iteration is two launches:
+//   k_cn_stream<CN_S>:  s = r - alpha v,  t = A (s / D);
+//                       partials t.s, t.t, r0.t, r0.s
+//   k_cn_stream<CN_P>:  r = s - omega t and x += alpha p/D + omega s/D (the
+//                       end of the previous iteration), p = r + beta (p - omega v),
+//                       v = A (p / D); partials r0.v, r.r
+//   k_cn_stream<CN_P0>: the first direction p = r, v = A (p / D).
+// rho_new = r0.r of the next residual is formed from the s-launch's partials
+// as r0.s - omega r0.t, so beta is known before the p-launch; the convergence
+// test uses the p-launch's r.r, the recursive residual itself.  The last
+// p-launch of a solve computes a direction nobody uses: one operator
+// application per solve, against a separate vector pass per iteration.
+// 1/D is stored in FP32: the preconditioner is diag(1/d~) with d~ the rounded
+// inverse, applied identically to the matvec input and to the solution
+// update, so the iteration solves the FP64 system to the same residual.
+// A wavefront owns a strip of 64 columns (one per lane, 62 written) and a
+// chunk of rows and marches down the rows with the input vector of three rows
+// in registers; the column neighbours come from the adjacent lanes.  The
+// scalars live in device memory, so an iteration needs one host
 // synchronisation, for the convergence test.
 // ---------------------------------------------------------------------------
 constexpr int S_RHO_NEW = 13;
 constexpr int CN_OUT = 62, CN_WAVES = 4;
+enum { CN_P = 0, CN_S = 1, CN_P0 = 2 };
 // build-time tuning: rows in flight per wavefront, wavefronts per launch
 #ifndef IMPEC_CN_PF
 #define IMPEC_CN_PF 4
@@ -161,14 +182,24 @@ constexpr int CN_OUT = 62, CN_WAVES = 4;
 #ifndef IMPEC_CN_TARGET_WAVES
 #define IMPEC_CN_TARGET_WAVES 4096
 #endif
+// 1: the operator applications form the face tensors from u and v as they
+// march (16 B per cell instead of the four stored tensors' 32 B, for two
+// square roots and four divisions per cell); 0: they read the stored tensors
+#ifndef IMPEC_CN_FACES
+#define IMPEC_CN_FACES 1
+#endif
 
 struct CnArgs {
     CnConst k;
     const double *d11, *d12, *d22, *d21;   // face tensors, pitch ldv
-    const double *dinv;                    // inverse of the folded diagonal (flat N1 x N2)
-    const double *r, *pin, *vin, *r0;      // flat N1 x N2
-    double *pout;                          // <0>: new p; <1>: s
-    double *out;                           // <0>: v; <1>: t
+    const double *u, *v;                   // face velocities, ghosted (ghost width k.B, pitch ldv)
+    const float *dinv;                     // rounded inverse of the folded diagonal (flat N1 x N2)
+    const double *r;                       // CN_S, CN_P0: the residual
+    const double *s, *t;                   // CN_P: s and t of the previous iteration
+    const double *pin, *vin, *r0;          // flat N1 x N2
+    double *x, *rout;                      // CN_P: the solution (in place) and the new residual
+    double *pout;                          // CN_P, CN_P0: new p; CN_S: s
+    double *out;                           // CN_P, CN_P0: v; CN_S: t
     const double *S;
     double *part;                          // per-wave partials
     int R, nstrip, nwave;
@@ -190,14 +221,19 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
     const bool outl = lane >= 1 && lane <= CN_OUT && jv;
     const int i0 = chunk * A.R, i1 = i0 + A.R < N1 ? i0 + A.R : N1;
     double beta = 0., om = 0., alpha = 0.;
-    if (MODE == 0) {
+    if (MODE == CN_P) {
         beta = (A.S[S_RHO_NEW] / A.S[S_RHO]) * (A.S[S_ALPHA] / A.S[S_OMEGA]);
         om = A.S[S_OMEGA];
-    } else {
+        alpha = A.S[S_ALPHA];
+    } else if (MODE == CN_S) {
         alpha = A.S[S_ALPHA];
     }
     // raw row loads from clamped addresses (no lane-dependent branches)
     auto cell = [&](const double *p, int m) -> double {
+        const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
+        return p[(long)mc * N2 + jc];
+    };
+    auto cellf = [&](const float *p, int m) -> float {
         const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
         return p[(long)mc * N2 + jc];
     };
@@ -209,21 +245,46 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
         const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
         return p[(long)mc * L + jf];
     };
-    // ring of two prefetched rows: r, p, v, D^-1, r0 of row m; x-faces of row m
-    // (between m-1 and m); y-faces of row m
+    // face velocities: rows -B..N1+B and columns -1..N2 of the ghosted arrays
+    const int B = k.B;
+    const int jg = j < -1 ? -1 : (j > N2 ? N2 : j);
+    auto vel = [&](const double *p, int m) -> double {
+        const int mc = m < -B ? -B : (m > N1 + B ? N1 + B : m);
+        return p[(long)(mc + B) * L + (jg + B)];
+    };
+    // ring of prefetched rows: the cell vectors of row m (CN_P: s, t, p, v, x;
+    // CN_S: r, v; CN_P0: r), 1/D and r0 of row m; either u and v of row m
+    // (IMPEC_CN_FACES) or the stored x-face tensors of row m (between m-1 and
+    // m) and y-face tensors of row m
     constexpr int PF = IMPEC_CN_PF;   // rows in flight; a slot is refilled at the end of the step that used it
-    double Pr[PF] = {}, Pp[PF] = {}, Pv[PF] = {}, Pd[PF] = {}, P0[PF] = {}, P11[PF] = {}, P12[PF] = {},
-           P22[PF] = {}, P21[PF] = {};
+    double Pa[PF] = {}, Pb[PF] = {}, Pp[PF] = {}, Pv[PF] = {}, Px[PF] = {}, P0[PF] = {};
+#if IMPEC_CN_FACES
+    double Pu[PF] = {}, Pw[PF] = {};
+#else
+    double P11[PF] = {}, P12[PF] = {}, P22[PF] = {}, P21[PF] = {};
+#endif
+    float Pd[PF] = {};
     auto fetch = [&](int u, int m) {
-        Pr[u] = cell(A.r, m);
-        if (MODE == 0) Pp[u] = cell(A.pin, m);
-        Pv[u] = cell(A.vin, m);
-        Pd[u] = cell(A.dinv, m);
-        if (MODE == 0) P0[u] = cell(A.r0, m);
+        if (MODE == CN_P) {
+            Pa[u] = cell(A.s, m);
+            Pb[u] = cell(A.t, m);
+            Pp[u] = cell(A.pin, m);
+            Px[u] = cell(A.x, m);
+        } else {
+            Pa[u] = cell(A.r, m);
+        }
+        if (MODE != CN_P0) Pv[u] = cell(A.vin, m);
+        Pd[u] = cellf(A.dinv, m);
+        P0[u] = cell(A.r0, m);
+#if IMPEC_CN_FACES
+        Pu[u] = vel(A.u, m);
+        Pw[u] = vel(A.v, m);
+#else
         P11[u] = xface(A.d11, m);
         P12[u] = xface(A.d12, m);
         P22[u] = yface(A.d22, m);
         P21[u] = yface(A.d21, m);
+#endif
     };
     const int mstart = i0 - 1, mend = i1;
     const int mstop = mstart + (mend - mstart + PF) / PF * PF;
@@ -232,30 +293,70 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
     // windows: input vector y of rows m-2, m-1, m; x-faces of rows m-1, m;
     // y-faces of row m-1; the written vector (p or s) and r0 of row m-1
     double Y0 = 0., Y1 = 0., Y2 = 0.;
-    double f11a = 0., f12a = 0., f11b = 0., f12b = 0., f22a = 0., f21a = 0.;
+    double f11a = 0., f12a = 0., f11b = 0., f12b = 0.;
+#if IMPEC_CN_FACES
+    // u and v of the previous row, and their lane shifts
+    double uprev = vel(A.u, mstart - 1), vprev = vel(A.v, mstart - 1);
+    double uprev_l = lane_prev(uprev), vprev_n = lane_next(vprev);
+#else
+    double f22a = 0., f21a = 0.;
+#endif
     double wprev = 0., r0prev = 0.;
-    double s0 = 0., s1 = 0.;
+    double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
     for (int mb = mstart; mb < mstop; mb += PF) {
 #pragma unroll
         for (int u = 0; u < PF; u++) {
             const int m = mb + u;
-            const double rv = Pr[u], pv = Pp[u], vv = Pv[u], dv = Pd[u], r0v = P0[u];
-            const double e11 = P11[u], e12 = P12[u], e22 = P22[u], e21 = P21[u];
+            const double dv = (double)Pd[u], r0v = P0[u];
+            // x-face tensors of row m (e11, e12) and y-face tensors of row m - 1 (g22, g21)
+            double e11, e12, g22, g21;
+#if IMPEC_CN_FACES
+            {
+                const double um = Pu[u], vm = Pw[u];
+                const double um_l = lane_prev(um), vm_n = lane_next(vm);
+                // the four-point averages of k_cn_tensor, in its summation order
+                const double vb = 1. / 4 * (vprev_n + vm_n + vprev + vm);
+                xface_tensor(k, um, vb, e11, e12);
+                const double ub = 1. / 4 * (um_l + um + uprev_l + uprev);
+                yface_tensor(k, ub, vprev, g22, g21);
+                uprev = um; vprev = vm; uprev_l = um_l; vprev_n = vm_n;
+            }
+#else
+            e11 = P11[u]; e12 = P12[u];
+            g22 = f22a; g21 = f21a;
+            f22a = P22[u]; f21a = P21[u];
+#endif
             const bool rin = m >= 0 && m < N1;
-            // the input vector on row m (and the written vector)
-            double wv;
-            if (MODE == 0) wv = rv + beta * (pv - om * vv);
-            else wv = rv - alpha * vv;
+            // the written vector on row m (p or s) and, for the p-launches, the residual
+            double wv, rv = 0.;
+            if (MODE == CN_P) {
+                rv = Pa[u] - om * Pb[u];
+                wv = rv + beta * (Pp[u] - om * Pv[u]);
+            } else if (MODE == CN_S) {
+                wv = Pa[u] - alpha * Pv[u];
+            } else {
+                rv = Pa[u];
+                wv = rv;
+            }
             const double yv = (rin && jv) ? wv * dv : 0.;
-            if (rin && m >= i0 && m < i1 && outl) A.pout[(long)m * N2 + j] = wv;
+            if (rin && m >= i0 && m < i1 && outl) {
+                const long o = (long)m * N2 + j;
+                A.pout[o] = wv;
+                if (MODE == CN_P) {
+                    A.rout[o] = rv;
+                    A.x[o] = Px[u] + (alpha * (Pp[u] * dv) + om * (Pa[u] * dv));
+                }
+                if (MODE == CN_S) s3 += r0v * wv;
+                else s1 += rv * rv;
+            }
             Y0 = Y1; Y1 = Y2; Y2 = yv;
             f11a = f11b; f12a = f12b; f11b = e11; f12b = e12;
             // row q = m - 1: GetCoefficients_Parabolic with the faces in registers
             const int q = m - 1;
-            const double d22p = lane_next(f22a), d21p = lane_next(f21a);
+            const double d22p = lane_next(g22), d21p = lane_next(g21);
             if (q >= i0 && q < i1) {
                 double a[9];
-                const double d11m = f11a, d11p = f11b, d12m = f12a, d12p = f12b, d22m = f22a, d21m = f21a;
+                const double d11m = f11a, d11p = f11b, d12m = f12a, d12p = f12b, d22m = g22, d21m = g21;
                 a[ST_NW] = k.cA * (d12m + d21p);
                 a[ST_NE] = k.mcA * (d12p + d21p);
                 a[ST_N] = k.mdt8 * (k.c4dy2 * d22p + k.c1dxdy * (-d12m + d12p));
@@ -280,14 +381,15 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
                 if (we && no) acc += a[ST_NW] * yr0;
                 if (outl) {
                     A.out[(long)q * N2 + j] = acc;
-                    if (MODE == 0) s0 += r0prev * acc;
-                    else {
+                    if (MODE == CN_S) {
                         s0 += acc * wprev;
                         s1 += acc * acc;
+                        s2 += r0prev * acc;
+                    } else {
+                        s0 += r0prev * acc;
                     }
                 }
             }
-            f22a = e22; f21a = e21;
             wprev = wv;
             r0prev = r0v;
             fetch(u, m + PF);
@@ -297,12 +399,21 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
     for (int o = 32; o > 0; o >>= 1) {
         s0 += __shfl_xor(s0, o);
         s1 += __shfl_xor(s1, o);
+        if (MODE == CN_S) {
+            s2 += __shfl_xor(s2, o);
+            s3 += __shfl_xor(s3, o);
+        }
     }
     if (lane == 0) {
         A.part[wid] = s0;
         A.part[RED_BLOCKS + wid] = s1;
+        if (MODE == CN_S) {
+            A.part[2 * RED_BLOCKS + wid] = s2;
+            A.part[3 * RED_BLOCKS + wid] = s3;
+        }
     }
 }
+static_assert(4 <= PART_VECS, "the s-launch writes four partial vectors");
 
 // launch shape of the element-wise BiCGStab kernels: 64 x 4 threads over
 // (column, row), rows strided so that there are at most CN_VBLOCKS blocks
@@ -314,10 +425,9 @@ static dim3 cn_vgrid(int N1, int N2) {
     return dim3(bx, by);
 }
 
-// r = r0 = b, p = v = x = 0; partials b.b (twice: ||b||^2 and r0.r)
+// r = r0 = b, x = 0; partials b.b (twice: ||b||^2 and r0.r)
 __global__ void k_bicg_init(int N1, int N2, const double *__restrict__ b, double *__restrict__ r,
-                            double *__restrict__ r0, double *__restrict__ p, double *__restrict__ vv,
-                            double *__restrict__ x, double *__restrict__ part) {
+                            double *__restrict__ r0, double *__restrict__ x, double *__restrict__ part) {
     const HwGrid g_;
     const int j = g_.bx() * 64 + threadIdx.x;
     double s[2] = {0., 0.};
@@ -327,8 +437,6 @@ __global__ void k_bicg_init(int N1, int N2, const double *__restrict__ b, double
             const double bv = b[id];
             r[id] = bv;
             r0[id] = bv;
-            p[id] = 0.;
-            vv[id] = 0.;
             x[id] = 0.;
             s[0] += bv * bv;
             s[1] += bv * bv;
@@ -341,8 +449,8 @@ __global__ void k_bicg_init(int N1, int N2, const double *__restrict__ b, double
     }
 }
 
-// initial guess x0 = c (interior): x = x0, r = r0 = b - A x0; p, v were zeroed
-// by k_bicg_init; partials r.r (twice)
+// initial guess x0 = c (interior): x = x0, r = r0 = b - A x0; partials r.r
+// (twice)
 __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ d11,
                              const double *__restrict__ d12, const double *__restrict__ d22,
                              const double *__restrict__ d21, const double *__restrict__ b, double *__restrict__ x,
@@ -382,42 +490,19 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
     }
 }
 
-// x += alpha p/D + omega s/D, r = s - omega t; partials r.r, r0.r
-__global__ void k_bicg_xr(int N1, int N2, double *__restrict__ x, const double *__restrict__ p,
-                          const double *__restrict__ sv, const double *__restrict__ dinv, double *__restrict__ r,
-                          const double *__restrict__ t, const double *__restrict__ r0, const double *__restrict__ S,
-                          double *__restrict__ part) {
-    const HwGrid g_;
-    const double alpha = S[S_ALPHA], om = S[S_OMEGA];
-    const int j = g_.bx() * 64 + threadIdx.x;
-    double s[2] = {0., 0.};
-    if (j < N2)
-        for (int i = g_.by() * 4 + threadIdx.y; i < N1; i += g_.gy() * 4) {
-            const long id = (long)i * N2 + j;
-            const double d = dinv[id], sval = sv[id];
-            x[id] += alpha * (p[id] * d) + om * (sval * d);
-            const double rv = sval - om * t[id];
-            r[id] = rv;
-            s[0] += rv * rv;
-            s[1] += r0[id] * rv;
-        }
-    block_reduce<RED_SUM, 2>(s);
-    if (tid_linear() == 0) {
-        const int blk = g_.by() * g_.gx() + g_.bx();
-        part[blk] = s[0];
-        part[RED_BLOCKS + blk] = s[1];
-    }
-}
-
-// second pass of the BiCGStab reductions and the scalar recurrences:
-//  stage 0 (init):    rr = bb = b.b, rho = alpha = omega = 1, rho_new = b.b
-//  stage 4 (guess):   rr = rho_new = r0.r0 of the initial residual
-//  stage 1 (after <0>): r0.v; rho = rho_new; alpha = rho / r0.v
-//  stage 2 (after <1>): t.s, t.t; omega = t.s / t.t (0 when t = 0)
-//  stage 3 (after xr):  rr = r.r, rho_new = r0.r
+// second pass of the BiCGStab reductions (NV partial vectors) and the scalar
+// recurrences:
+//  stage 0 (init):     rr = bb = b.b, rho = alpha = omega = 1, rho_new = b.b
+//  stage 4 (guess):    rr = rho_new = r0.r0 of the initial residual
+//  stage 1 (p-launch): r0.v, r.r; rho = rho_new; alpha = rho / r0.v; rr = r.r
+//  stage 2 (s-launch): t.s, t.t, r0.t, r0.s; omega = t.s / t.t (0 when t = 0);
+//                      rho_new = r0.r of r = s - omega t, as r0.s - omega r0.t
+template <int NV>
 __global__ void k_bicg_finish(const double *__restrict__ part, int nb, int stage, double *__restrict__ S) {
-    double v[2] = {strided_combine<RED_SUM>(part, nb), strided_combine<RED_SUM>(part + RED_BLOCKS, nb)};
-    block_reduce<RED_SUM, 2>(v);
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) v[q] = strided_combine<RED_SUM>(part + q * RED_BLOCKS, nb);
+    block_reduce<RED_SUM, NV>(v);
     if (threadIdx.x != 0) return;
     if (stage == 0) {
         S[S_RR] = v[0];
@@ -431,13 +516,13 @@ __global__ void k_bicg_finish(const double *__restrict__ part, int nb, int stage
         S[S_R0V] = v[0];
         S[S_RHO] = S[S_RHO_NEW];
         S[S_ALPHA] = S[S_RHO] / v[0];
-    } else if (stage == 2) {
+        S[S_RR] = v[1];
+    } else if (NV >= 4) {
         S[S_TS] = v[0];
         S[S_TT] = v[1];
-        S[S_OMEGA] = v[1] > 0 ? v[0] / v[1] : 0.;
-    } else {
-        S[S_RR] = v[0];
-        S[S_RHO_NEW] = v[1];
+        const double om = v[1] > 0 ? v[0] / v[1] : 0.;
+        S[S_OMEGA] = om;
+        S[S_RHO_NEW] = v[NV - 1] - om * v[NV - 2];
     }
 }
 
@@ -516,6 +601,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     const int N1 = ctx->N1, N2 = ctx->N2;
     hipStream_t s = ctx->stream;
     double *b = ctx->cn_rhs, *x = ctx->cn_x, *r = ctx->cn_r, *r0 = ctx->cn_r0, *sv = ctx->cn_s, *t = ctx->cn_t;
+    // p, v ping-pong: a strip reads its neighbours' old values at the halo
     double *pb[2] = {ctx->cn_p, ctx->cn_y}, *vb[2] = {ctx->cn_v, ctx->cn_z};
     prof_begin(ctx);
     int st = tensors_and_rhs(ctx, k);
@@ -523,12 +609,12 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     if (st) return st;
     const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
     const int nvb = (int)(vg.x * vg.y);
-    hipLaunchKernelGGL(k_bicg_init, vg, vblk, 0, s, N1, N2, b, r, r0, pb[0], vb[0], x, ctx->part);
-    hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 0, ctx->scal);
+    hipLaunchKernelGGL(k_bicg_init, vg, vblk, 0, s, N1, N2, b, r, r0, x, ctx->part);
+    hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 0, ctx->scal);
     if (ctx->P.cn_guess == 1) {   // start from the current concentration
         hipLaunchKernelGGL(k_bicg_guess, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->d11, ctx->d12, ctx->d22, ctx->d21,
                            b, x, r, r0, ctx->part);
-        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 4, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 4, ctx->scal);
     }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     if ((st = fetch(ctx))) return st;
@@ -538,8 +624,10 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     const int maxit = ctx->P.max_iterations > 0 ? ctx->P.max_iterations : 10000;
 
     CnArgs a;
+    memset(&a, 0, sizeof(a));
     a.k = k;
     a.d11 = ctx->d11; a.d12 = ctx->d12; a.d22 = ctx->d22; a.d21 = ctx->d21;
+    a.u = ctx->u; a.v = ctx->v;
     a.dinv = ctx->cn_dg;
     a.r = r; a.r0 = r0;
     a.S = ctx->scal;
@@ -556,26 +644,32 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     if (a.nwave > RED_BLOCKS) return set_error(ctx, IMPEC_EINVAL, "dispersion solve: grid too large");
     const dim3 sg((a.nwave + CN_WAVES - 1) / CN_WAVES), sblk(64 * CN_WAVES);
     int cur = 0, its = 0, status = IMPEC_OK;
+    if (!(rr <= tol * tol * bb) && rr == rr) {   // the first direction p = r
+        CnArgs ap = a;
+        ap.pout = pb[1]; ap.out = vb[1];
+        prof_begin(ctx);
+        hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, sblk, 0, s, ap);
+        prof_end(ctx, "cn_apply_p0");
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 1, ctx->scal);
+        IMPEC_HIP_CHECK(ctx, hipGetLastError());
+        cur = 1;
+    }
     while (!(rr <= tol * tol * bb)) {
         if (its >= maxit) { status = IMPEC_ENOCONV; break; }
         if (!(rr == rr)) { status = IMPEC_ENAN; break; }
-        // p, v ping-pong: a strip reads its neighbours' old values at the halo
-        a.pin = pb[cur]; a.vin = vb[cur]; a.pout = pb[cur ^ 1]; a.out = vb[cur ^ 1];
+        CnArgs as = a;
+        as.vin = vb[cur]; as.pout = sv; as.out = t;
         prof_begin(ctx);
-        hipLaunchKernelGGL(k_cn_stream<0>, sg, sblk, 0, s, a);
-        prof_end(ctx, "cn_apply_p");
-        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 1, ctx->scal);
-        CnArgs a2 = a;
-        a2.pin = nullptr; a2.vin = vb[cur ^ 1]; a2.pout = sv; a2.out = t;
-        prof_begin(ctx);
-        hipLaunchKernelGGL(k_cn_stream<1>, sg, sblk, 0, s, a2);
+        hipLaunchKernelGGL(k_cn_stream<CN_S>, sg, sblk, 0, s, as);
         prof_end(ctx, "cn_apply_s");
-        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 2, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<4>, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 2, ctx->scal);
+        CnArgs ap = a;
+        ap.s = sv; ap.t = t; ap.pin = pb[cur]; ap.vin = vb[cur]; ap.x = x; ap.rout = r;
+        ap.pout = pb[cur ^ 1]; ap.out = vb[cur ^ 1];
         prof_begin(ctx);
-        hipLaunchKernelGGL(k_bicg_xr, vg, vblk, 0, s, N1, N2, x, pb[cur ^ 1], sv, ctx->cn_dg, r, t, r0, ctx->scal,
-                           ctx->part);
-        prof_end(ctx, "cn_vector");
-        hipLaunchKernelGGL(k_bicg_finish, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 3, ctx->scal);
+        hipLaunchKernelGGL(k_cn_stream<CN_P>, sg, sblk, 0, s, ap);
+        prof_end(ctx, "cn_apply_p");
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 1, ctx->scal);
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = fetch(ctx))) return st;
         rr = ctx->h_scal[S_RR];

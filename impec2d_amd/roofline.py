@@ -12,14 +12,14 @@ import re
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 
 KERNELS = {
-    # BiCGStab step 1 (k_cn_stream<0>): p = r + beta (p - omega v), v = A (p / D),
-    # r0.v; read r, p, v, 1/D, r0 and the four face tensors (9 x 8), write p, v (2 x 8)
-    "cn_apply_p": dict(rocprof="k_cn_stream<0>", bytes_per_cell=88),
-    # BiCGStab step 2 (k_cn_stream<1>): s = r - alpha v, t = A (s / D), t.s, t.t;
-    # read r, v, 1/D and the four face tensors (7 x 8), write s, t (2 x 8)
-    "cn_apply_s": dict(rocprof="k_cn_stream<1>", bytes_per_cell=72),
-    # BiCGStab update (k_bicg_xr): read x, p, s, 1/D, t, r0 (6 x 8), write x, r (2 x 8)
-    "cn_vector": dict(rocprof="k_bicg_xr", bytes_per_cell=64),
+    # BiCGStab p-launch (k_cn_stream<0>): r = s - omega t, x += alpha p/D + omega s/D,
+    # p = r + beta (p - omega v), v = A (p / D), r0.v, r.r; read s, t, p, v, x, r0
+    # (6 x 8), 1/D (FP32, 4) and the face velocities u, v (2 x 8; the face tensors
+    # are formed in registers), write r, x, p, v (4 x 8)
+    "cn_apply_p": dict(rocprof="k_cn_stream<0>", bytes_per_cell=100),
+    # BiCGStab s-launch (k_cn_stream<1>): s = r - alpha v, t = A (s / D), t.s, t.t,
+    # r0.t, r0.s; read r, v, r0 (3 x 8), 1/D (4) and u, v (2 x 8), write s, t (2 x 8)
+    "cn_apply_s": dict(rocprof="k_cn_stream<1>", bytes_per_cell=60),
     # streaming V-cycle, finest level, down leg: read w, r (2 x 8), write x (8)
     # and the coarse right-hand side (8 / 4); FP32 cycle: w 4, r 8, x 4, b_c 1
     "mg_down_l0": dict(rocprof="k_mg_down<true, false", bytes_per_cell=34, bytes_per_cell_fp32=17),
