@@ -63,11 +63,8 @@ static size_t vel_elems(const impec_ctx *ctx) { return (size_t)(ctx->N1 + 1 + 2 
 
 static int ensure_cn(impec_ctx *ctx) {
     if (ctx->cn_rhs) return IMPEC_OK;
-    const size_t nt = (size_t)(ctx->N1 + 2) * ctx->ldv, n = (size_t)ctx->N1 * ctx->N2;
+    const size_t n = (size_t)ctx->N1 * ctx->N2;
     int st;
-    if ((st = dalloc(ctx, &ctx->d11, nt)) || (st = dalloc(ctx, &ctx->d12, nt)) || (st = dalloc(ctx, &ctx->d22, nt)) ||
-        (st = dalloc(ctx, &ctx->d21, nt)))
-        return st;
     double **v[10] = {&ctx->cn_rhs, &ctx->cn_x, &ctx->cn_r, &ctx->cn_r0, &ctx->cn_p, &ctx->cn_v,
                       &ctx->cn_s, &ctx->cn_t, &ctx->cn_y, &ctx->cn_z};
     for (int k = 0; k < 10; k++)
@@ -238,8 +235,8 @@ int impec_destroy(impec_ctx *ctx) {
     if (!ctx) return IMPEC_OK;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     double *bufs[] = {ctx->c, ctx->p, ctx->u, ctx->v, ctx->q, ctx->K11, ctx->K22 != ctx->K11 ? ctx->K22 : nullptr,
-                      ctx->w, ctx->w22, ctx->t1, ctx->t2, ctx->t3, ctx->scal, ctx->part, ctx->d11, ctx->d12,
-                      ctx->d22, ctx->d21, ctx->cn_rhs, ctx->cn_x, ctx->cn_r, ctx->cn_r0, ctx->cn_p, ctx->cn_v,
+                      ctx->w, ctx->w22, ctx->t1, ctx->t2, ctx->t3, ctx->scal, ctx->part,
+                      ctx->cn_rhs, ctx->cn_x, ctx->cn_r, ctx->cn_r0, ctx->cn_p, ctx->cn_v,
                       ctx->cn_s, ctx->cn_t, ctx->cn_y, ctx->cn_z, ctx->src_q, ctx->tr_md, ctx->p_prev,
                       ctx->p_old};
     for (double *b : bufs)

@@ -116,7 +116,6 @@ struct impec_ctx {
     int have_prev_p;
 
     // Crank-Nicolson
-    double *d11, *d12, *d22, *d21;   // face tensors (N1+1) x (N2+1), pitch ldv
     double *cn_rhs, *cn_x, *cn_r, *cn_r0, *cn_p, *cn_v, *cn_s, *cn_t, *cn_y, *cn_z;
     float *cn_dg;                 // rounded inverse of the folded CN diagonal
 

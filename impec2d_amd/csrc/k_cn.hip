@@ -81,71 +81,60 @@ __device__ __forceinline__ void yface_tensor(const CnConst &k, double ub, double
     d22 = x22;
 }
 
-// the face tensors on every x-face (i in [0,N1], j in [0,N2)) and y-face
-// (i in [0,N1), j in [0,N2]); u, v ghost-filled
-__global__ void k_cn_tensor(CnConst k, const double *__restrict__ u, const double *__restrict__ v,
-                            double *__restrict__ d11, double *__restrict__ d12, double *__restrict__ d22,
-                            double *__restrict__ d21) {
-    const HwGrid g_;
-    const int j = g_.bx() * blockDim.x + threadIdx.x;
-    const int i = g_.by() * blockDim.y + threadIdx.y;
-    if (i > k.N1 || j > k.N2) return;
+// the eight face tensors around cell (i, j) (west / east x-faces, south /
+// north y-faces) from the ghost-filled u and v, with the four-point averages
+// of SetupDiffusionTensorF/G in their summation order
+struct CellTensors {
+    double d11m, d12m, d11p, d12p, d22m, d21m, d22p, d21p;
+};
+__device__ __forceinline__ CellTensors cell_tensors(const CnConst &k, const double *__restrict__ u,
+                                                    const double *__restrict__ v, int i, int j) {
     const long L = k.ldv;
     const long o = (long)(i + k.B) * L + (j + k.B);
-    const long t = (long)i * L + j;
-    if (j < k.N2) {
-        const double vb = 1. / 4 * (v[o - L + 1] + v[o + 1] + v[o - L] + v[o]);
-        xface_tensor(k, u[o], vb, d11[t], d12[t]);
-    }
-    if (i < k.N1) {
-        const double ub = 1. / 4 * (u[o + L - 1] + u[o + L] + u[o - 1] + u[o]);
-        yface_tensor(k, ub, v[o], d22[t], d21[t]);
-    }
+    CellTensors T;
+    xface_tensor(k, u[o], 1. / 4 * (v[o - L + 1] + v[o + 1] + v[o - L] + v[o]), T.d11m, T.d12m);
+    xface_tensor(k, u[o + L], 1. / 4 * (v[o + 1] + v[o + L + 1] + v[o] + v[o + L]), T.d11p, T.d12p);
+    yface_tensor(k, 1. / 4 * (u[o + L - 1] + u[o + L] + u[o - 1] + u[o]), v[o], T.d22m, T.d21m);
+    yface_tensor(k, 1. / 4 * (u[o + L] + u[o + L + 1] + u[o] + u[o + 1]), v[o + 1], T.d22p, T.d21p);
+    return T;
 }
 
 // GetCoefficients_Parabolic (MGSolver.cpp:923-948) + folding
-__device__ __forceinline__ void cn_row(const CnConst &k, const double *__restrict__ d11, const double *__restrict__ d12,
-                                       const double *__restrict__ d22, const double *__restrict__ d21, int i, int j,
-                                       double a[9]) {
-    const long L = k.ldv, t = (long)i * L + j;
-    const double d11m = d11[t], d12m = d12[t], d11p = d11[t + L], d12p = d12[t + L];
-    const double d22m = d22[t], d21m = d21[t], d22p = d22[t + 1], d21p = d21[t + 1];
-    a[ST_NW] = k.cA * (d12m + d21p);
-    a[ST_NE] = k.mcA * (d12p + d21p);
-    a[ST_N] = k.mdt8 * (k.c4dy2 * d22p + k.c1dxdy * (-d12m + d12p));
-    a[ST_SW] = k.mcA * (d12m + d21m);
-    a[ST_SE] = k.cA * (d12p + d21m);
-    a[ST_S] = k.mdt8 * (k.c4dy2 * d22m + k.c1dxdy * (-d12p + d12m));
-    a[ST_W] = k.mdt8 * (k.c4dx2 * d11m + k.c1dxdy * (-d21p + d21m));
-    a[ST_C] = 1. + k.cx2 * (d11p + d11m) + k.cy2 * (d22p + d22m);
-    a[ST_E] = k.mdt8 * (k.c4dx2 * d11p + k.c1dxdy * (-d21m + d21p));
+__device__ __forceinline__ void cn_coeffs(const CnConst &k, const CellTensors &T, int i, int j, double a[9]) {
+    a[ST_NW] = k.cA * (T.d12m + T.d21p);
+    a[ST_NE] = k.mcA * (T.d12p + T.d21p);
+    a[ST_N] = k.mdt8 * (k.c4dy2 * T.d22p + k.c1dxdy * (-T.d12m + T.d12p));
+    a[ST_SW] = k.mcA * (T.d12m + T.d21m);
+    a[ST_SE] = k.cA * (T.d12p + T.d21m);
+    a[ST_S] = k.mdt8 * (k.c4dy2 * T.d22m + k.c1dxdy * (-T.d12p + T.d12m));
+    a[ST_W] = k.mdt8 * (k.c4dx2 * T.d11m + k.c1dxdy * (-T.d21p + T.d21m));
+    a[ST_C] = 1. + k.cx2 * (T.d11p + T.d11m) + k.cy2 * (T.d22p + T.d22m);
+    a[ST_E] = k.mdt8 * (k.c4dx2 * T.d11p + k.c1dxdy * (-T.d21m + T.d21p));
     cn_fold(i, j, k.N1, k.N2, a);
 }
 
-// SetRHS_Parabolic (MGSolver.cpp:1324-1370); c ghost-filled
-__global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ d11,
-                         const double *__restrict__ d12, const double *__restrict__ d22,
-                         const double *__restrict__ d21, double *__restrict__ rhs, float *__restrict__ diag) {
+// SetRHS_Parabolic (MGSolver.cpp:1324-1370) and the Jacobi preconditioner;
+// c, u, v ghost-filled
+__global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ u,
+                         const double *__restrict__ v, double *__restrict__ rhs, float *__restrict__ diag) {
     const HwGrid g_;
     const int j = g_.bx() * blockDim.x + threadIdx.x;
     const int i = g_.by() * blockDim.y + threadIdx.y;
     if (i >= k.N1 || j >= k.N2) return;
-    const long L = k.ldv, t = (long)i * L + j;
-    const double d11m = d11[t], d12m = d12[t], d11p = d11[t + L], d12p = d12[t + L];
-    const double d22m = d22[t], d21m = d21[t], d22p = d22[t + 1], d21p = d21[t + 1];
+    const CellTensors T = cell_tensors(k, u, v, i, j);
     const double *cc = c + (long)(i + k.B) * lds + (j + k.B);
 #define C_(a, b) cc[(long)(a) * lds + (b)]
     const double r = C_(0, 0) + k.dt8 * (
-        k.c4dx2 * (d11p * (C_(1, 0) - C_(0, 0)) - d11m * (C_(0, 0) - C_(-1, 0)))
-        + k.c1dxdy * (d12p * (C_(0, 1) + C_(1, 1) - C_(0, -1) - C_(1, -1))
-                      - d12m * (C_(0, 1) + C_(-1, 1) - C_(-1, -1) - C_(0, -1)))
-        + k.c1dxdy * (d21p * (C_(1, 0) + C_(1, 1) - C_(-1, 1) - C_(-1, 0))
-                      - d21m * (C_(1, 0) + C_(1, -1) - C_(-1, 0) - C_(-1, -1)))
-        + k.c4dy2 * (d22p * (C_(0, 1) - C_(0, 0)) - d22m * (C_(0, 0) - C_(0, -1))));
+        k.c4dx2 * (T.d11p * (C_(1, 0) - C_(0, 0)) - T.d11m * (C_(0, 0) - C_(-1, 0)))
+        + k.c1dxdy * (T.d12p * (C_(0, 1) + C_(1, 1) - C_(0, -1) - C_(1, -1))
+                      - T.d12m * (C_(0, 1) + C_(-1, 1) - C_(-1, -1) - C_(0, -1)))
+        + k.c1dxdy * (T.d21p * (C_(1, 0) + C_(1, 1) - C_(-1, 1) - C_(-1, 0))
+                      - T.d21m * (C_(1, 0) + C_(1, -1) - C_(-1, 0) - C_(-1, -1)))
+        + k.c4dy2 * (T.d22p * (C_(0, 1) - C_(0, 0)) - T.d22m * (C_(0, 0) - C_(0, -1))));
 #undef C_
     rhs[(long)i * k.N2 + j] = r;
     double a[9];
-    cn_row(k, d11, d12, d22, d21, i, j, a);
+    cn_coeffs(k, T, i, j, a);
     diag[(long)i * k.N2 + j] = (float)(1. / a[ST_C]);   // Jacobi preconditioner, rounded to FP32
 }
 
@@ -182,16 +171,9 @@ enum { CN_P = 0, CN_S = 1, CN_P0 = 2 };
 #ifndef IMPEC_CN_TARGET_WAVES
 #define IMPEC_CN_TARGET_WAVES 4096
 #endif
-// 1: the operator applications form the face tensors from u and v as they
-// march (16 B per cell instead of the four stored tensors' 32 B, for two
-// square roots and four divisions per cell); 0: they read the stored tensors
-#ifndef IMPEC_CN_FACES
-#define IMPEC_CN_FACES 1
-#endif
 
 struct CnArgs {
     CnConst k;
-    const double *d11, *d12, *d22, *d21;   // face tensors, pitch ldv
     const double *u, *v;                   // face velocities, ghosted (ghost width k.B, pitch ldv)
     const float *dinv;                     // rounded inverse of the folded diagonal (flat N1 x N2)
     const double *r;                       // CN_S, CN_P0: the residual
@@ -217,7 +199,6 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
     const int j = strip * CN_OUT - 1 + lane;
     const bool jv = j >= 0 && j < N2;
     const int jc = j < 0 ? 0 : (j > N2 - 1 ? N2 - 1 : j);   // cell column for loads
-    const int jf = j < 0 ? 0 : (j > N2 ? N2 : j);           // face column for loads (faces run to N2)
     const bool outl = lane >= 1 && lane <= CN_OUT && jv;
     const int i0 = chunk * A.R, i1 = i0 + A.R < N1 ? i0 + A.R : N1;
     double beta = 0., om = 0., alpha = 0.;
@@ -237,14 +218,6 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
         const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
         return p[(long)mc * N2 + jc];
     };
-    auto xface = [&](const double *p, int m) -> double {   // x-faces: rows 0..N1
-        const int mc = m < 0 ? 0 : (m > N1 ? N1 : m);
-        return p[(long)mc * L + jc];
-    };
-    auto yface = [&](const double *p, int m) -> double {   // y-faces: columns 0..N2
-        const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
-        return p[(long)mc * L + jf];
-    };
     // face velocities: rows -B..N1+B and columns -1..N2 of the ghosted arrays
     const int B = k.B;
     const int jg = j < -1 ? -1 : (j > N2 ? N2 : j);
@@ -253,16 +226,9 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
         return p[(long)(mc + B) * L + (jg + B)];
     };
     // ring of prefetched rows: the cell vectors of row m (CN_P: s, t, p, v, x;
-    // CN_S: r, v; CN_P0: r), 1/D and r0 of row m; either u and v of row m
-    // (IMPEC_CN_FACES) or the stored x-face tensors of row m (between m-1 and
-    // m) and y-face tensors of row m
+    // CN_S: r, v; CN_P0: r), 1/D, r0 and the face velocities u, v of row m
     constexpr int PF = IMPEC_CN_PF;   // rows in flight; a slot is refilled at the end of the step that used it
-    double Pa[PF] = {}, Pb[PF] = {}, Pp[PF] = {}, Pv[PF] = {}, Px[PF] = {}, P0[PF] = {};
-#if IMPEC_CN_FACES
-    double Pu[PF] = {}, Pw[PF] = {};
-#else
-    double P11[PF] = {}, P12[PF] = {}, P22[PF] = {}, P21[PF] = {};
-#endif
+    double Pa[PF] = {}, Pb[PF] = {}, Pp[PF] = {}, Pv[PF] = {}, Px[PF] = {}, P0[PF] = {}, Pu[PF] = {}, Pw[PF] = {};
     float Pd[PF] = {};
     auto fetch = [&](int u, int m) {
         if (MODE == CN_P) {
@@ -276,15 +242,8 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
         if (MODE != CN_P0) Pv[u] = cell(A.vin, m);
         Pd[u] = cellf(A.dinv, m);
         P0[u] = cell(A.r0, m);
-#if IMPEC_CN_FACES
         Pu[u] = vel(A.u, m);
         Pw[u] = vel(A.v, m);
-#else
-        P11[u] = xface(A.d11, m);
-        P12[u] = xface(A.d12, m);
-        P22[u] = yface(A.d22, m);
-        P21[u] = yface(A.d21, m);
-#endif
     };
     const int mstart = i0 - 1, mend = i1;
     const int mstop = mstart + (mend - mstart + PF) / PF * PF;
@@ -294,13 +253,9 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
     // y-faces of row m-1; the written vector (p or s) and r0 of row m-1
     double Y0 = 0., Y1 = 0., Y2 = 0.;
     double f11a = 0., f12a = 0., f11b = 0., f12b = 0.;
-#if IMPEC_CN_FACES
     // u and v of the previous row, and their lane shifts
     double uprev = vel(A.u, mstart - 1), vprev = vel(A.v, mstart - 1);
     double uprev_l = lane_prev(uprev), vprev_n = lane_next(vprev);
-#else
-    double f22a = 0., f21a = 0.;
-#endif
     double wprev = 0., r0prev = 0.;
     double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
     for (int mb = mstart; mb < mstop; mb += PF) {
@@ -310,22 +265,16 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
             const double dv = (double)Pd[u], r0v = P0[u];
             // x-face tensors of row m (e11, e12) and y-face tensors of row m - 1 (g22, g21)
             double e11, e12, g22, g21;
-#if IMPEC_CN_FACES
             {
                 const double um = Pu[u], vm = Pw[u];
                 const double um_l = lane_prev(um), vm_n = lane_next(vm);
-                // the four-point averages of k_cn_tensor, in its summation order
+                // the four-point averages of cell_tensors, in its summation order
                 const double vb = 1. / 4 * (vprev_n + vm_n + vprev + vm);
                 xface_tensor(k, um, vb, e11, e12);
                 const double ub = 1. / 4 * (um_l + um + uprev_l + uprev);
                 yface_tensor(k, ub, vprev, g22, g21);
                 uprev = um; vprev = vm; uprev_l = um_l; vprev_n = vm_n;
             }
-#else
-            e11 = P11[u]; e12 = P12[u];
-            g22 = f22a; g21 = f21a;
-            f22a = P22[u]; f21a = P21[u];
-#endif
             const bool rin = m >= 0 && m < N1;
             // the written vector on row m (p or s) and, for the p-launches, the residual
             double wv, rv = 0.;
@@ -356,17 +305,8 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
             const double d22p = lane_next(g22), d21p = lane_next(g21);
             if (q >= i0 && q < i1) {
                 double a[9];
-                const double d11m = f11a, d11p = f11b, d12m = f12a, d12p = f12b, d22m = g22, d21m = g21;
-                a[ST_NW] = k.cA * (d12m + d21p);
-                a[ST_NE] = k.mcA * (d12p + d21p);
-                a[ST_N] = k.mdt8 * (k.c4dy2 * d22p + k.c1dxdy * (-d12m + d12p));
-                a[ST_SW] = k.mcA * (d12m + d21m);
-                a[ST_SE] = k.cA * (d12p + d21m);
-                a[ST_S] = k.mdt8 * (k.c4dy2 * d22m + k.c1dxdy * (-d12p + d12m));
-                a[ST_W] = k.mdt8 * (k.c4dx2 * d11m + k.c1dxdy * (-d21p + d21m));
-                a[ST_C] = 1. + k.cx2 * (d11p + d11m) + k.cy2 * (d22p + d22m);
-                a[ST_E] = k.mdt8 * (k.c4dx2 * d11p + k.c1dxdy * (-d21m + d21p));
-                cn_fold(q, j, N1, N2, a);
+                const CellTensors T = {f11a, f12a, f11b, f12b, g22, g21, d22p, d21p};
+                cn_coeffs(k, T, q, j, a);
                 const double yl0 = lane_prev(Y0), yl1 = lane_prev(Y1), yl2 = lane_prev(Y2);
                 const double yr0 = lane_next(Y0), yr1 = lane_next(Y1), yr2 = lane_next(Y2);
                 const bool we = q > 0, ea = q < N1 - 1, so = j > 0, no = j < N2 - 1;
@@ -451,9 +391,8 @@ __global__ void k_bicg_init(int N1, int N2, const double *__restrict__ b, double
 
 // initial guess x0 = c (interior): x = x0, r = r0 = b - A x0; partials r.r
 // (twice)
-__global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ d11,
-                             const double *__restrict__ d12, const double *__restrict__ d22,
-                             const double *__restrict__ d21, const double *__restrict__ b, double *__restrict__ x,
+__global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ u,
+                             const double *__restrict__ v, const double *__restrict__ b, double *__restrict__ x,
                              double *__restrict__ r, double *__restrict__ r0, double *__restrict__ part) {
     const HwGrid g_;
     const int j = g_.bx() * 64 + threadIdx.x;
@@ -461,7 +400,7 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
     if (j < k.N2)
         for (int i = g_.by() * 4 + threadIdx.y; i < k.N1; i += g_.gy() * 4) {
             double a[9];
-            cn_row(k, d11, d12, d22, d21, i, j, a);
+            cn_coeffs(k, cell_tensors(k, u, v, i, j), i, j, a);
             const double *cc = c + (long)(i + k.B) * lds + (j + k.B);
             const long L = lds;
             const bool we = i > 0, ea = i < k.N1 - 1, so = j > 0, no = j < k.N2 - 1;
@@ -536,14 +475,14 @@ __global__ void k_cn_store(const double *__restrict__ x, double *__restrict__ c,
 }
 
 // matrix dump: nine N1*N2 arrays
-__global__ void k_cn_dump(CnConst k, const double *__restrict__ d11, const double *__restrict__ d12,
-                          const double *__restrict__ d22, const double *__restrict__ d21, double *__restrict__ A) {
+__global__ void k_cn_dump(CnConst k, const double *__restrict__ u, const double *__restrict__ v,
+                          double *__restrict__ A) {
     const HwGrid g_;
     const int j = g_.bx() * blockDim.x + threadIdx.x;
     const int i = g_.by() * blockDim.y + threadIdx.y;
     if (i >= k.N1 || j >= k.N2) return;
     double a[9];
-    cn_row(k, d11, d12, d22, d21, i, j, a);
+    cn_coeffs(k, cell_tensors(k, u, v, i, j), i, j, a);
     const long M = (long)k.N1 * k.N2;
     for (int q = 0; q < 9; q++) A[q * M + (long)i * k.N2 + j] = a[q];
 }
@@ -572,15 +511,10 @@ static CnConst cn_const(const impec_ctx *ctx, double dt) {
     return k;
 }
 
-// face tensors, right-hand side (into cn_rhs) and inverse folded diagonal (into cn_dg)
+// right-hand side (into cn_rhs) and rounded inverse folded diagonal (into cn_dg)
 static int tensors_and_rhs(impec_ctx *ctx, const CnConst &k) {
-    dim3 blk(64, 4);
-    dim3 gt((ctx->N2 + 1 + 63) / 64, (ctx->N1 + 1 + 3) / 4);
-    hipLaunchKernelGGL(k_cn_tensor, gt, blk, 0, ctx->stream, k, ctx->u, ctx->v, ctx->d11, ctx->d12, ctx->d22,
-                       ctx->d21);
-    dim3 gc((ctx->N2 + 63) / 64, (ctx->N1 + 3) / 4);
-    hipLaunchKernelGGL(k_cn_rhs, gc, blk, 0, ctx->stream, k, ctx->c, ctx->lds, ctx->d11, ctx->d12, ctx->d22, ctx->d21,
-                       ctx->cn_rhs, ctx->cn_dg);
+    dim3 blk(64, 4), gc((ctx->N2 + 63) / 64, (ctx->N1 + 3) / 4);
+    hipLaunchKernelGGL(k_cn_rhs, gc, blk, 0, ctx->stream, k, ctx->c, ctx->lds, ctx->u, ctx->v, ctx->cn_rhs, ctx->cn_dg);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
 }
@@ -605,15 +539,15 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     double *pb[2] = {ctx->cn_p, ctx->cn_y}, *vb[2] = {ctx->cn_v, ctx->cn_z};
     prof_begin(ctx);
     int st = tensors_and_rhs(ctx, k);
-    prof_end(ctx, "cn_tensor_rhs");
+    prof_end(ctx, "cn_rhs");
     if (st) return st;
     const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
     const int nvb = (int)(vg.x * vg.y);
     hipLaunchKernelGGL(k_bicg_init, vg, vblk, 0, s, N1, N2, b, r, r0, x, ctx->part);
     hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 0, ctx->scal);
     if (ctx->P.cn_guess == 1) {   // start from the current concentration
-        hipLaunchKernelGGL(k_bicg_guess, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->d11, ctx->d12, ctx->d22, ctx->d21,
-                           b, x, r, r0, ctx->part);
+        hipLaunchKernelGGL(k_bicg_guess, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, x, r, r0,
+                           ctx->part);
         hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 4, ctx->scal);
     }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
@@ -626,7 +560,6 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     CnArgs a;
     memset(&a, 0, sizeof(a));
     a.k = k;
-    a.d11 = ctx->d11; a.d12 = ctx->d12; a.d22 = ctx->d22; a.d21 = ctx->d21;
     a.u = ctx->u; a.v = ctx->v;
     a.dinv = ctx->cn_dg;
     a.r = r; a.r0 = r0;
@@ -696,7 +629,7 @@ int cn_operator_dump(impec_ctx *ctx, double dt, double *A9_host, double *rhs_hos
     double *dA = nullptr;
     IMPEC_HIP_CHECK(ctx, hipMalloc(&dA, sizeof(double) * 9 * n));
     dim3 blk(64, 4), gc((ctx->N2 + 63) / 64, (ctx->N1 + 3) / 4);
-    hipLaunchKernelGGL(k_cn_dump, gc, blk, 0, ctx->stream, k, ctx->d11, ctx->d12, ctx->d22, ctx->d21, dA);
+    hipLaunchKernelGGL(k_cn_dump, gc, blk, 0, ctx->stream, k, ctx->u, ctx->v, dA);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && A9_host) e = hipMemcpyAsync(A9_host, dA, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess && rhs_host)
