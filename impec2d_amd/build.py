@@ -75,7 +75,7 @@ HOST_RUN = os.path.join(HERE, "impec2d_run")
 GXX_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
 
-HOST_SOURCES = ["impec_solvers.cpp", "impec_config.cpp", "impec_output.cpp"]
+HOST_SOURCES = ["impec_solvers.cpp", "impec_config.cpp", "impec_output.cpp", "impec_tests.cpp"]
 
 
 def build_host(verbose=False):

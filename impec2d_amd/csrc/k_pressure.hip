@@ -1797,11 +1797,14 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
 // rows per chunk of the streaming legs: about IMPEC_MG_WAVES wavefronts on large
 // levels, short chunks on small ones; even, since the march starts on an
 // even row
+#ifndef IMPEC_MG_MINROWS
+#define IMPEC_MG_MINROWS 8
+#endif
 static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip) {
     if (l == 0 && ctx->P.mg_rows > 0) return (ctx->P.mg_rows + 1) & ~1;
     long r = ((long)n1 * nstrip + IMPEC_MG_WAVES - 1) / IMPEC_MG_WAVES;
     r = (r + 1) & ~1L;
-    if (r < 8) r = 8;
+    if (r < IMPEC_MG_MINROWS) r = IMPEC_MG_MINROWS;
     if (r > 64) r = 64;
     return (int)r;
 }

@@ -14,7 +14,9 @@
 // --config reads an Impec2D configuration file (libconfig syntax, the keys of
 // SimParams.cpp:45-463) and, as the reference does, writes the final fields
 // and diagnostics.dat into its results_dir and, with log_to_file, the
-// concentration every log_skip_steps steps into log_dir.
+// concentration every log_skip_steps steps into log_dir.  A file with
+// is_test = true runs the convergence test its test section names
+// (Impec2DSolver::TestComponent) and writes the norm and plot files there.
 // --dump writes c, p (each (N1+4) x (N2+4)) and v ((N1+5) x (N2+5) x 2) as raw
 // little-endian doubles, in that order.  Exit status 0 on success, 1 on a
 // solver error, 2 on a usage or configuration error, 3 when no GPU can be used.
@@ -128,8 +130,18 @@ int main(int argc, char **argv) {
         for (double t : sp.times) std::cout << " " << t;
         std::cout << "\ncounters";
         for (int c : sp.counters) std::cout << " " << c;
+        std::cout << "\nis_test " << sp.is_test;
+        if (sp.is_test)
+            std::cout << "\ntest_component " << (sp.test_component == ELLIPTIC ? "elliptic" : "hyperbolic")
+                      << "\ntestno " << sp.testno << "\ninit_x_cells " << sp.init_x_cells << "\ninit_y_cells "
+                      << sp.init_y_cells << "\nlevels " << sp.levels << "\nhyp_test " << (int)sp.hyp_test
+                      << "\nhyp_norm " << (int)sp.hyp_norm << "\nnorm_file " << sp.norm_file;
         std::cout << std::endl;
         return 0;
+    }
+    if (sp.is_test && !dump.empty()) {
+        std::cerr << "--dump is not available in test mode (the fields change size between levels)" << std::endl;
+        return 2;
     }
     try {
         Impec2DSolver solver(sp);

@@ -374,9 +374,44 @@ bool ReadConfigText(const std::string &text, SimParams &sp, std::string &err) {
         }
         sp.times.push_back(e.kind == CfgValue::FLOAT ? e.f : (double)e.i);
     }
-    if (sp.is_test) {
-        err = "is_test = true (the convergence tests of TestComponent) is not run from configuration files here";
-        return false;
+    if (sp.is_test) {   // SimParams.cpp:252-316
+        const CfgValue *ts = root.path("application.test");
+        std::string comp, sprofile;
+        int ihypnorm = 0;
+        if (!(ts && ts->get("component", comp) && ts->get("testno", sp.testno) &&
+              ts->get("num_result_file", sp.num_result_file) && ts->get("ex_result_file", sp.ex_result_file) &&
+              ts->get("norm_file", sp.norm_file) && ts->get("initial_x_cells", sp.init_x_cells) &&
+              ts->get("initial_y_cells", sp.init_y_cells) && ts->get("num_u_file", sp.num_u_file) &&
+              ts->get("ex_u_file", sp.ex_u_file) && ts->get("num_v_file", sp.num_v_file) &&
+              ts->get("ex_v_file", sp.ex_v_file) && ts->get("levels", sp.levels) &&
+              ts->get("profile", sprofile) && ts->get("hyp_norm", ihypnorm))) {
+            err = "Could not find one or more items in section <<test>>!";
+            return false;
+        }
+        if (comp == "elliptic") sp.test_component = ELLIPTIC;
+        else if (comp == "hyperbolic") sp.test_component = HYPERBOLIC;
+        else {
+            err = "TestComponent> Config::General: source type is not valid!";
+            return false;
+        }
+        if (sprofile == "sine") sp.hyp_test = SINE;
+        else if (sprofile == "tophat") sp.hyp_test = TOPHAT;
+        else if (sprofile == "gaussbell") sp.hyp_test = GAUSS;
+        else if (sprofile == "horizontal_sine") sp.hyp_test = HSINE;
+        else {
+            err = "TestComponent> Config:: Test profile is not valid!";
+            return false;
+        }
+        if (ihypnorm == 1) sp.hyp_norm = L1;
+        else if (ihypnorm == 2) sp.hyp_norm = L2;
+        else {
+            err = "TestComponent> Config:: Norm is not supported!";
+            return false;
+        }
+        if (sp.init_x_cells < 2 || sp.init_y_cells < 2 || sp.levels < 1) {
+            err = "TestComponent> Config:: initial_x_cells, initial_y_cells and levels must be positive";
+            return false;
+        }
     }
     std::string sti;
     double ti_eps = 0.;

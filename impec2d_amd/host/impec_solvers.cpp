@@ -585,7 +585,8 @@ void Impec2DSolver::WriteDiagnostics(const std::string &fname) const {
     }
 }
 
-bool Impec2DSolver::Run() { return Solve(); }
+// Impec2DSolver::Run (Impec2DSolver.cpp:747-760), without the gnuplot pause
+bool Impec2DSolver::Run() { return sp.is_test ? TestComponent() : Solve(); }
 
 // the step loop (Impec2DSolver.cpp:293-466)
 bool Impec2DSolver::Solve() {

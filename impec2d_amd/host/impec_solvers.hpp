@@ -38,6 +38,9 @@ enum DimSplitting {   // SimParams.h:21, same order
 enum HyperbolicMethod { MUSCL = IMPEC_HYPER_MUSCL, UPWIND = IMPEC_HYPER_UPWIND };
 enum ParabolicMethod { EXPLICIT = IMPEC_PARA_EXPLICIT, CRANKNICOLSON = IMPEC_PARA_CN };
 enum WellProfile { ONECELL, CIRCULAR, GAUSSIAN };   // SimParams.h well_profile; GAUSSIAN leaves q = 0
+enum SolverTest { ELLIPTIC, HYPERBOLIC };            // SimParams.h:17
+enum HypTest { SINE, TOPHAT, GAUSS, HSINE };         // SimParams.h:18
+enum HypNorm { L1, L2 };                             // SimParams.h:19
 
 // The SimParams fields the hot path reads (SimParams.h:29-164), with the
 // reference's names.
@@ -71,6 +74,13 @@ struct SimParams {
     double max_mass_frac_error = 0.;
     std::string log_dir = ".", results_dir = ".";
     bool write_results = false;                  // final fields and diagnostics.dat into results_dir
+    // test mode (SimParams.cpp:252-316): the convergence tests of
+    // Impec2DSolver::TestComponent, run when is_test is set
+    SolverTest test_component = ELLIPTIC;
+    int testno = 1, init_x_cells = 0, init_y_cells = 0, levels = 1;
+    HypTest hyp_test = SINE;
+    HypNorm hyp_norm = L1;
+    std::string num_result_file, ex_result_file, norm_file, num_u_file, num_v_file, ex_u_file, ex_v_file;
 
     void Finalize();                             // dx, dy from the domain and N1, N2
     impec_params ToImpec() const;
@@ -211,6 +221,11 @@ class EllipticSolver {
     int GetLastIts();
     int GetLastParabolicIts();
     double GetAvgIts();
+    // the manufactured-solution convergence test (EllipticSolver.cpp:155-311):
+    // levels resolutions from init_x_cells x init_y_cells, doubling; L1 errors
+    // of p, u and v into norm_file, the fields of the first level into the
+    // test's plot files.  No gnuplot window (the reference's GnuPlotter).
+    bool TestNeumann();
 
   protected:
     SimParams *sp;
@@ -344,8 +359,16 @@ class Impec2DSolver {
   public:
     explicit Impec2DSolver(const SimParams &params);   // throws std::runtime_error without a device
     virtual ~Impec2DSolver();
-    bool Run();                                        // Solve() from the initial state
+    bool Run();                                        // TestComponent() when is_test, else Solve()
     bool Solve();
+    bool TestComponent();                              // Impec2DSolver.cpp:719-745
+    // the periodic advection convergence test (Impec2DSolver.cpp:773-843):
+    // levels resolutions, Advection() each, log(dx) and log(norm) into norm_file
+    bool TestAdvection();
+    // one resolution of it (Impec2DSolver.cpp:845-990): the profile carried with
+    // the constant velocity (-(x_ub - x_lb), y_ub - y_lb) from tstart to tfinal;
+    // returns the L1 or L2 error against the initial profile
+    double Advection();
     const std::vector<DiagState> &Diagnostics() const { return diagstates; }
     void WriteDiagnostics(const std::string &fname) const;   // diagnostics.dat columns
     Array2d &Concentration() { return c; }

@@ -58,10 +58,39 @@ def test_shipped_configurations_read(name):
     assert float(d["dx"]) == float(d["x_ub"]) / int(d["N1"])               # SimParams.cpp: dx = (x_ub - x_lb) / N1
 
 
-@pytest.mark.parametrize("name", ["elliptic_test.cfg", "hyperbolic_test.cfg"])
-def test_test_configurations_refused_with_reason(name):
+# the test sections of the two test configurations (SimParams.cpp:252-316)
+TESTS = {
+    "elliptic_test.cfg": dict(test_component="elliptic", testno="4", init_x_cells="20", init_y_cells="30",
+                              levels="6", hyp_test="1", hyp_norm="1"),
+    "hyperbolic_test.cfg": dict(test_component="hyperbolic", testno="1", init_x_cells="50", init_y_cells="50",
+                                levels="4", hyp_test="0", hyp_norm="0"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(TESTS))
+def test_test_configurations_read(name):
     rc, out, err = _params("--config", os.path.join(CFG, name))
-    assert rc == 2 and "is_test" in err
+    assert rc == 0, err
+    d = _parse(out)
+    assert d["is_test"] == "1"
+    for k, v in TESTS[name].items():
+        assert d[k] == v, (k, d[k], v)
+
+
+def _test_text():
+    with open(os.path.join(CFG, "hyperbolic_test.cfg")) as f:
+        return f.read()
+
+
+@pytest.mark.parametrize("key,value,message", [
+    ("component", '"parabolic"', "source type is not valid"),
+    ("profile", '"square"', "Test profile is not valid"),
+    ("hyp_norm", "3", "Norm is not supported"),
+    ("levels", None, "section <<test>>"),
+])
+def test_invalid_test_settings_reported(tmp_path, key, value, message):
+    rc, out, err = _run_text(tmp_path, _set(_test_text(), key, value))
+    assert rc == 2 and message in err, err
 
 
 def _base_text():

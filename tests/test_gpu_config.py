@@ -123,3 +123,95 @@ def test_diagonal_dispersion_configuration(gpu_lib, tmp_path):
     assert _plot_values(str(out / "velocity_u.dat")).shape == ((N1 + 1) * N2, 3)
     assert _plot_values(str(out / "velocity_v.dat")).shape == (N1 * (N2 + 1), 3)
     assert np.loadtxt(str(out / "diagnostics.dat")).shape == (steps, 8)
+
+
+# ---------------------------------------------------------------------------
+# test mode: the two test configurations run unchanged except for their
+# output paths (TestNeumann / TestAdvection through impec2d_run)
+# ---------------------------------------------------------------------------
+def _retarget(name, tmp_path):
+    """The configuration with every output file and directory moved into tmp_path."""
+    import re
+    text = open(os.path.join(CFG, name)).read()
+    for key in ("num_result_file", "ex_result_file", "norm_file", "num_u_file", "num_v_file", "ex_u_file",
+                "ex_v_file", "log_dir", "results_dir"):
+        pat = re.compile(r'(^|\s)' + key + r'\s*=\s*"([^"]*)"\s*;', re.M)
+        m = pat.search(text)
+        assert m, key
+        target = str(tmp_path) if key.endswith("_dir") else str(tmp_path / os.path.basename(m.group(2)))
+        text = pat.sub(lambda mm: '%s%s = "%s";' % (mm.group(1), key, target), text, count=1)
+    path = tmp_path / name
+    path.write_text(text)
+    return str(path)
+
+
+def _norms(tmp_path, basename):
+    return np.loadtxt(str(tmp_path / basename), ndmin=2)
+
+
+def test_elliptic_test_configuration(gpu_lib, tmp_path):
+    """elliptic_test.cfg: TestNeumann of test 4 (anisotropic K) from 20 x 30
+    cells over 6 levels, nu = 1, mean-concentration faces, eps 1e-12: second
+    order in p, u and v (examples/elliptic.eps), and the first level's L1
+    errors equal the oracle's on the same manufactured problem."""
+    cfg = _retarget("elliptic_test.cfg", tmp_path)
+    _say("run elliptic_test.cfg")
+    r = subprocess.run([RUN, "--config", cfg], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "Executing test no. 4 for the ELLIPTIC Solver" in r.stdout
+    nrm = _norms(tmp_path, "Elliptic_1e-10_test4_L2_nu_0.66.out")
+    assert nrm.shape == (6, 4)
+    orders = [np.polyfit(nrm[2:, 0], nrm[2:, c], 1)[0] for c in (1, 2, 3)]
+    _say("  orders p %.3f u %.3f v %.3f" % tuple(orders))
+    assert 1.9 < orders[0] < 2.1 and 1.85 < orders[1] < 2.15 and 1.85 < orders[2] < 2.15, orders
+    # level 0 against the oracle (EllipticSolver.cpp:357-415 restated with numpy)
+    from test_oracle import _K, _q
+    N1, N2, L = 20, 30, 2 * math.pi
+    dx, dy = L / N1, L / N2
+    p = po.default_params(N1=N1, N2=N2, nu=1.0, x_ub=L, y_ub=L, mu_res=1.0, mu_inj=1.0, avgproc=1, eps=1e-12)
+    xc = (np.arange(-2, N1 + 2) + 0.5) * dx
+    yc = (np.arange(-2, N2 + 2) + 0.5) * dy
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    K11, K22 = _K(4, X, Y)
+    o = po.Oracle(p, np.ascontiguousarray(K11), np.ascontiguousarray(K22))
+    Xi, Yi = np.meshgrid((np.arange(N1) + 0.5) * dx, (np.arange(N2) + 0.5) * dy, indexing="ij")
+    q = _q(4, Xi, Yi).reshape(-1)
+    pex = np.cos(Xi) * np.cos(Yi)
+    o.set_kappa(-dx * dy * q[1 * N2 + 1] / pex[1, 1])
+    c, pr, v = o.scalar(), o.scalar(), o.velocity()
+    o.fill(c)
+    assert o.pressure_solve(c, pr, q) == 0
+    o.velocities(pr, v, c)
+    ep = np.abs(o.interior(pr) - pex).sum() * dx * dy
+    Xu, Yu = np.meshgrid(np.arange(N1 + 1) * dx, (np.arange(N2) + 0.5) * dy, indexing="ij")
+    eu = np.abs(v[2:N1 + 3, 2:N2 + 2, 0] - _K(4, Xu, Yu)[0] * np.sin(Xu) * np.cos(Yu)).sum() * dx * dy
+    _say("  level 0: p %.6e (oracle %.6e)  u %.6e (oracle %.6e)" % (math.exp(nrm[0, 1]), ep, math.exp(nrm[0, 2]), eu))
+    assert abs(math.exp(nrm[0, 1]) - ep) <= 1e-8 * ep and abs(math.exp(nrm[0, 2]) - eu) <= 1e-8 * eu
+    assert abs(nrm[0, 0] - math.log(dx)) < 1e-14
+    # the first level's plot files: p numerical / exact, u, v
+    assert _plot_values(str(tmp_path / "ell_test2.out")).shape == (N1 * N2, 3)
+    assert _plot_values(str(tmp_path / "ell_exact2_u.out")).shape == ((N1 + 1) * N2, 3)
+
+
+def test_hyperbolic_test_configuration(gpu_lib, tmp_path):
+    """hyperbolic_test.cfg: TestAdvection of the sine profile, MUSCL with
+    parallel splitting, CFL 0.95, 50^2 to 400^2: order 1.7-2.0 in L1
+    (examples/hyperbolic.eps: 1.85), and the first level's error equals the
+    oracle's."""
+    cfg = _retarget("hyperbolic_test.cfg", tmp_path)
+    _say("run hyperbolic_test.cfg")
+    r = subprocess.run([RUN, "--config", cfg, "--quiet"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "Executing test no. 1 for the Hyperbolic Solver" in r.stdout
+    nrm = _norms(tmp_path, "Hyper_parallel_CFL0.9_norm_muscl.out")
+    assert nrm.shape == (4, 2)
+    k = np.polyfit(nrm[:, 0], nrm[:, 1], 1)[0]
+    _say("  order %.3f, errors %s" % (k, " ".join("%.3e" % math.exp(e) for e in nrm[:, 1])))
+    assert 1.7 < k < 2.0, k
+    from test_oracle import advection_l1
+    e50, _ = advection_l1(50, 0, cfl=0.95)
+    assert abs(math.exp(nrm[0, 1]) - e50) <= 1e-10 * e50, (math.exp(nrm[0, 1]), e50)
+    for n in (50, 100, 200, 400):
+        assert os.path.exists(str(tmp_path / ("hyp_test1.out%d" % n)))
+        assert os.path.exists(str(tmp_path / ("hyp_test1.outinit%d" % n)))
+    assert os.path.exists(str(tmp_path / "concentration_1.dat"))   # log_skip_steps = 100 logs step 1
