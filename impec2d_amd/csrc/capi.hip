@@ -125,7 +125,7 @@ int impec_params_default(impec_params *p) {
     p->max_iterations = 10000;   // MGSolver.cpp:89
     p->precision = IMPEC_PREC_FP64;
     p->initial_guess = IMPEC_GUESS_ZERO;
-    p->mg_pre = 1; p->mg_post = 1;
+    p->mg_pre = 2; p->mg_post = 2;
     p->x_lb = 0.; p->x_ub = 1.; p->y_lb = 0.; p->y_ub = 1.;
     p->nu = 2. / 3.;
     p->mu_res = 41.; p->mu_inj = 1.;

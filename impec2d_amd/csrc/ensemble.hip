@@ -182,7 +182,10 @@ int impec_ensemble_create(const impec_params *members, int count, impec_ensemble
         if (P.precision != P0.precision || P.hyper_method != P0.hyper_method ||
             (P.porosity == 1.) != (P0.porosity == 1.))
             return IMPEC_EINVAL;
-        if (P.mg_pre > 1 || P.mg_post > 1 || P.mg_variant != 0) return IMPEC_EINVAL;
+        // one or two red-black pairs per leg, the same for every member
+        const bool one = P.mg_pre <= 1 && P.mg_post <= 1, two = P.mg_pre == 2 && P.mg_post == 2;
+        const bool one0 = P0.mg_pre <= 1 && P0.mg_post <= 1;
+        if (!(one || two) || one != one0 || P.mg_variant != 0) return IMPEC_EINVAL;
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IMPEC_ENODEV;

@@ -931,7 +931,18 @@ __global__ void k_set_pin(CgArgs c, double *__restrict__ p) {
 // with the boundary faces dropped (the Neumann folding), and the diagonal is
 // minus the sum of the four faces.
 // ===========================================================================
-constexpr int SW_OUT = 120, SW_HALO = 4, SW_WAVES = 4;
+constexpr int SW_WAVES = 4;
+// A strip is 128 columns, two per lane.  Each Gauss-Seidel sweep of a leg
+// spoils one more column from the strip's edges (the neighbours beyond them
+// are not in the wave), so a leg with NS red-black pairs keeps NS + 1 lanes
+// of halo on each side: NS = 1 writes 120 columns, NS = 2 writes 116.
+template <int NS>
+struct Strip {
+    static constexpr int HALO_LANES = NS + 1;
+    static constexpr int HALO = 2 * HALO_LANES;   // columns
+    static constexpr int OUT = 128 - 2 * HALO;    // written columns
+};
+constexpr int SW_OUT = Strip<1>::OUT, SW_HALO = Strip<1>::HALO;
 
 // T is the storage and arithmetic type of the cycle: double (FP64) or float
 // (the FP32 V-cycles of the mixed-precision solve, inside the FP64 CG)
@@ -1050,14 +1061,15 @@ __device__ __forceinline__ int xcd_wave() {
     return xcd_block(blockIdx.x, gridDim.x) * SW_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 }
 
-// strip and chunk of wave `wid` of a leg's launch over one level
-template <typename Args>
+// strip and chunk of wave `wid` of a leg's launch over one level, for legs of
+// NS red-black pairs
+template <int NS, typename Args>
 __device__ __forceinline__ bool wave_setup(const Args &A, int wid, Lane &L, Chunk &K) {
     const int lane = threadIdx.x & 63;
     if (wid >= A.nwave) return false;
     const int strip = wid % A.nstrip, chunk = wid / A.nstrip;
     const int n2 = A.g.n2;
-    L.g0 = strip * SW_OUT - SW_HALO + 2 * lane;
+    L.g0 = strip * Strip<NS>::OUT - Strip<NS>::HALO + 2 * lane;
     L.v0 = L.g0 >= 0 && L.g0 < n2;
     L.v1 = L.g0 >= 0 && L.g0 + 1 < n2;
     L.vR = L.g0 >= 0 && L.g0 + 2 < n2;
@@ -1065,7 +1077,7 @@ __device__ __forceinline__ bool wave_setup(const Args &A, int wid, Lane &L, Chun
     L.c1 = L.g0 + 1 < 0 ? 0 : (L.g0 + 1 > n2 - 1 ? n2 - 1 : L.g0 + 1);
     K.i0 = chunk * A.R;
     K.i1 = K.i0 + A.R < A.g.n1 ? K.i0 + A.R : A.g.n1;
-    K.out = lane >= 2 && lane < 62;
+    K.out = lane >= Strip<NS>::HALO_LANES && lane < 64 - Strip<NS>::HALO_LANES;
     return true;
 }
 
@@ -1094,12 +1106,17 @@ __device__ __forceinline__ void mask_pair(const Lane &L, int m, int n1, V fill, 
 #endif
 constexpr int SW_PF = IMPEC_SW_PF;
 
-template <bool FINE0, bool ANISO, typename T>
+template <bool FINE0, bool ANISO, typename T, int NS = 1>
 __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int wid) {
 #pragma clang fp contract(fast)
+    // NS red-black pairs: red from zero on row k, then the other 2 NS - 1
+    // colour sweeps on rows k-1 .. k-2NS+1, the residual on row k-2NS
+    constexpr int NX = 2 + 2 * NS;   // x and W-face window: rows k-NX+1 .. k
+    constexpr int NF = 1 + 2 * NS;   // face and rhs window: rows k-NF+1 .. k
+    constexpr int LAG = 2 * NS;      // residual and output row k - LAG
     Lane L;
     Chunk K;
-    if (!wave_setup(A, wid, L, K)) return;
+    if (!wave_setup<NS>(A, wid, L, K)) return;
     const int n1 = A.g.n1, n2 = A.g.n2;
     constexpr bool aniso = ANISO;
     const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
@@ -1115,7 +1132,7 @@ __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int w
     const T wj1 = T(L.v1 ? pw(L.g0 + 1, J, A.m2) : 0.);
     const T wjR = T(L.vR ? pw(L.g0 + 2, J, A.m2) : 0.);
     const int Ilo = K.i0 >> 1, Ihi = (K.i1 + 1) >> 1;
-    const int kstart = K.i0 - 4, kend = 2 * Ihi + 2;
+    const int kstart = K.i0 - 2 - LAG, kend = 2 * Ihi + LAG;
 
     T PW[SW_PF][2], PY[SW_PF][2];
     Src PB[SW_PF][2];
@@ -1130,11 +1147,11 @@ __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int w
     load_pair(A.w, A.ldw, L, kstart, n1, wc0, wc1);
     mask_pair(L, kstart, n1, T(1), wc0, wc1);
 
-    T X[4][2] = {};   // rows k-3 .. k
-    T B[3][2] = {};   // rows k-2 .. k
-    FRowT<T> F[3];    // rows k-2 .. k
-    T E[4][2] = {};   // E of rows k-3 .. k
-    for (int q = 0; q < 3; q++) F[q] = FRowT<T>{T(0), T(0), T(0), T(0), T(0)};
+    T X[NX][2] = {};   // rows k-NX+1 .. k
+    T B[NF][2] = {};   // rows k-NF+1 .. k
+    FRowT<T> F[NF];    // rows k-NF+1 .. k
+    T E[NX][2] = {};   // E of rows k-NX+1 .. k
+    for (int q = 0; q < NF; q++) F[q] = FRowT<T>{T(0), T(0), T(0), T(0), T(0)};
     T acc_c = T(0), acc_n = T(0);
 
     // whole groups of SW_PF steps (no exit inside the unrolled group, so the
@@ -1159,31 +1176,41 @@ __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int w
             wc0 = wn0;
             wc1 = wn1;
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
+            for (int q = 0; q < NX - 1; q++) {
                 X[q][0] = X[q + 1][0]; X[q][1] = X[q + 1][1];
                 E[q][0] = E[q + 1][0]; E[q][1] = E[q + 1][1];
             }
-            F[0] = F[1]; F[1] = F[2]; F[2] = fk;
-            B[0][0] = B[1][0]; B[0][1] = B[1][1]; B[1][0] = B[2][0]; B[1][1] = B[2][1];
-            B[2][0] = bk[0]; B[2][1] = bk[1];
-            E[3][0] = fk.E0; E[3][1] = fk.E1;
+#pragma unroll
+            for (int q = 0; q < NF - 1; q++) {
+                F[q] = F[q + 1];
+                B[q][0] = B[q + 1][0]; B[q][1] = B[q + 1][1];
+            }
+            F[NF - 1] = fk;
+            B[NF - 1][0] = bk[0]; B[NF - 1][1] = bk[1];
+            E[NX - 1][0] = fk.E0; E[NX - 1][1] = fk.E1;
             // red from zero on row k (its black cell stays zero)
             const int c = u & 1;   // k & 1 (kstart is even)
             {
-                const T C0 = -((E[2][0] + fk.E0) + (fk.S0 + fk.M));
-                const T C1 = -((E[2][1] + fk.E1) + (fk.M + fk.N1));
+                const T C0 = -((E[NX - 2][0] + fk.E0) + (fk.S0 + fk.M));
+                const T C1 = -((E[NX - 2][1] + fk.E1) + (fk.M + fk.N1));
                 if (c == 0) {
-                    X[3][0] = C0 != T(0) ? bk[0] * recip(C0) : T(0);
-                    X[3][1] = T(0);
+                    X[NX - 1][0] = C0 != T(0) ? bk[0] * recip(C0) : T(0);
+                    X[NX - 1][1] = T(0);
                 } else {
-                    X[3][0] = T(0);
-                    X[3][1] = C1 != T(0) ? bk[1] * recip(C1) : T(0);
+                    X[NX - 1][0] = T(0);
+                    X[NX - 1][1] = C1 != T(0) ? bk[1] * recip(C1) : T(0);
                 }
             }
-            // black on row k-1
-            if (c == 0) X[2][0] = gs_cell<0>(F[1], E[1], B[1][0], X[1][0], X[3][0], X[2]);
-            else X[2][1] = gs_cell<1>(F[1], E[1], B[1][1], X[1][1], X[3][1], X[2]);
-            // residual on row k-2
+            // black on row k-1, then red on row k-2, black on row k-3 (NS = 2);
+            // on each of these rows the cell of colour the sweep updates is
+            // the lane's cell c
+#pragma unroll
+            for (int l = 1; l < 2 * NS; l++) {
+                const int xi = NX - 1 - l, fi = NF - 1 - l;
+                if (c == 0) X[xi][0] = gs_cell<0>(F[fi], E[xi - 1], B[fi][0], X[xi - 1][0], X[xi + 1][0], X[xi]);
+                else X[xi][1] = gs_cell<1>(F[fi], E[xi - 1], B[fi][1], X[xi - 1][1], X[xi + 1][1], X[xi]);
+            }
+            // residual on row k-LAG (window rows 0..2 of X, 0 of F, B and E)
             T r0, r1;
             {
                 const FRowT<T> &fr = F[0];
@@ -1205,7 +1232,7 @@ __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int w
                 s += N * xr;
                 r1 = B[0][1] - s;
             }
-            const int f = k - 2;
+            const int f = k - LAG;
             if (K.out && f >= K.i0 && f < K.i1) {
                 T *xp = A.x + (long)f * A.ld + L.g0;
                 if (L.v0) xp[0] = X[1][0];
@@ -1233,17 +1260,22 @@ __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int w
     }
 }
 
-template <bool FINE0, bool ANISO, typename T>
+template <bool FINE0, bool ANISO, typename T, int NS>
 __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgsT<T> A) {
-    mg_down_body<FINE0, ANISO, T>(A, xcd_wave());
+    mg_down_body<FINE0, ANISO, T, NS>(A, xcd_wave());
 }
 
-template <bool FINE0, bool ANISO, typename T>
+template <bool FINE0, bool ANISO, typename T, int NS = 1>
 __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid) {
 #pragma clang fp contract(fast)
+    // NS black-red pairs after the correction of row k: black on row k-1, red
+    // on row k-2, ..., red on row k-2NS, the output row
+    constexpr int NX = 2 + 2 * NS;
+    constexpr int NF = 1 + 2 * NS;
+    constexpr int LAG = 2 * NS;
     Lane L;
     Chunk K;
-    if (!wave_setup(A, wid, L, K)) return;
+    if (!wave_setup<NS>(A, wid, L, K)) return;
     const int n1 = A.g.n1, m1 = A.m1, m2 = A.m2;
     constexpr bool aniso = ANISO;
     const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
@@ -1260,7 +1292,7 @@ __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid
         I = I < 0 ? 0 : (I > m1 - 1 ? m1 - 1 : I);
         return A.xc[(long)I * A.ldc + Jc];
     };
-    const int kstart = K.i0 - 2, kend = K.i1 + 1;
+    const int kstart = K.i0 - LAG, kend = K.i1 + LAG - 1;
 
     T PX[SW_PF][2], PW[SW_PF][2], PY[SW_PF][2], PC[SW_PF][2];
     Src PB[SW_PF][2];
@@ -1280,11 +1312,11 @@ __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid
     load_pair(A.w, A.ldw, L, kstart, n1, wc0, wc1);
     mask_pair(L, kstart, n1, T(1), wc0, wc1);
 
-    T X[4][2] = {};     // rows k-3 .. k
-    Src B[3][2] = {};   // raw rhs rows k-2 .. k
-    FRowT<T> F[3];
-    T E[4][2] = {};
-    for (int q = 0; q < 3; q++) F[q] = FRowT<T>{T(0), T(0), T(0), T(0), T(0)};
+    T X[NX][2] = {};     // rows k-NX+1 .. k
+    Src B[NF][2] = {};   // raw rhs rows k-NF+1 .. k
+    FRowT<T> F[NF];
+    T E[NX][2] = {};
+    for (int q = 0; q < NF; q++) F[q] = FRowT<T>{T(0), T(0), T(0), T(0), T(0)};
     double acc0 = 0., acc1 = 0.;
 
     // whole groups of SW_PF steps (no exit inside the unrolled group, so the
@@ -1311,14 +1343,18 @@ __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid
             wc0 = wn0;
             wc1 = wn1;
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
+            for (int q = 0; q < NX - 1; q++) {
                 X[q][0] = X[q + 1][0]; X[q][1] = X[q + 1][1];
                 E[q][0] = E[q + 1][0]; E[q][1] = E[q + 1][1];
             }
-            F[0] = F[1]; F[1] = F[2]; F[2] = fk;
-            B[0][0] = B[1][0]; B[0][1] = B[1][1]; B[1][0] = B[2][0]; B[1][1] = B[2][1];
-            B[2][0] = bk0; B[2][1] = bk1;
-            E[3][0] = fk.E0; E[3][1] = fk.E1;
+#pragma unroll
+            for (int q = 0; q < NF - 1; q++) {
+                F[q] = F[q + 1];
+                B[q][0] = B[q + 1][0]; B[q][1] = B[q + 1][1];
+            }
+            F[NF - 1] = fk;
+            B[NF - 1][0] = bk0; B[NF - 1][1] = bk1;
+            E[NX - 1][0] = fk.E0; E[NX - 1][1] = fk.E1;
             // x + P e on row k from coarse rows I0 (o0) and I1 (o1)
             {
                 const T pl0 = lane_prev(o0), pl1 = lane_prev(o1), pr0 = lane_next(o0), pr1 = lane_next(o1);
@@ -1327,16 +1363,20 @@ __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid
                 const T e0 = T(0.5625) * o0 + T(0.1875) * l0 + T(0.1875) * o1 + T(0.0625) * l1;
                 const T e1 = T(0.5625) * o0 + T(0.1875) * r0 + T(0.1875) * o1 + T(0.0625) * r1;
                 const bool rin = k >= 0 && k < n1;
-                X[3][0] = (rin && L.v0) ? xk0 + e0 : T(0);
-                X[3][1] = (rin && L.v1) ? xk1 + e1 : T(0);
+                X[NX - 1][0] = (rin && L.v0) ? xk0 + e0 : T(0);
+                X[NX - 1][1] = (rin && L.v1) ? xk1 + e1 : T(0);
             }
-            // black on row k-1
-            if (c == 0) X[2][0] = gs_cell<0>(F[1], E[1], T(B[1][0] - mean), X[1][0], X[3][0], X[2]);
-            else X[2][1] = gs_cell<1>(F[1], E[1], T(B[1][1] - mean), X[1][1], X[3][1], X[2]);
-            // red on row k-2
-            if (c == 0) X[1][0] = gs_cell<0>(F[0], E[0], T(B[0][0] - mean), X[0][0], X[2][0], X[1]);
-            else X[1][1] = gs_cell<1>(F[0], E[0], T(B[0][1] - mean), X[0][1], X[2][1], X[1]);
-            const int f = k - 2;
+            // black on row k-1, red on row k-2 (then black on k-3, red on k-4
+            // for NS = 2); on each row the updated cell is the lane's cell c
+#pragma unroll
+            for (int l = 1; l <= 2 * NS; l++) {
+                const int xi = NX - 1 - l, fi = NF - 1 - l;
+                if (c == 0)
+                    X[xi][0] = gs_cell<0>(F[fi], E[xi - 1], T(B[fi][0] - mean), X[xi - 1][0], X[xi + 1][0], X[xi]);
+                else
+                    X[xi][1] = gs_cell<1>(F[fi], E[xi - 1], T(B[fi][1] - mean), X[xi - 1][1], X[xi + 1][1], X[xi]);
+            }
+            const int f = k - LAG;
             if (K.out && f >= K.i0 && f < K.i1) {
                 T *xp = A.xo + (long)f * A.ld + L.g0;
                 if (L.v0) xp[0] = X[1][0];
@@ -1362,9 +1402,9 @@ __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid
     }
 }
 
-template <bool FINE0, bool ANISO, typename T>
+template <bool FINE0, bool ANISO, typename T, int NS>
 __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgsT<T> A) {
-    mg_up_body<FINE0, ANISO, T>(A, xcd_wave());
+    mg_up_body<FINE0, ANISO, T, NS>(A, xcd_wave());
 }
 
 // ===========================================================================
@@ -1395,6 +1435,7 @@ struct BottomArgs {
     int ldb;
     T *out;                        // its corrected, smoothed x at (0,0), pitch ldo
     int ldo;
+    int ns;                        // red-black pairs per leg, as the streaming legs
     const double *S;               // loop state (ensembles)
 };
 
@@ -1475,6 +1516,14 @@ __device__ void mg_bottom_body(const BottomArgs<T> &A) {
             if ((i + j) & 1) x[id] = bottom_gs(x, w, b[id], n1, n2, i, j, cx, cy, hm);
         }
         __syncthreads();
+        for (int pr = 1; pr < A.ns; pr++)   // further red-black pairs
+            for (int c = 0; c <= 1; c++) {
+                for (int id = t; id < n; id += nt) {
+                    const int i = id / n2, j = id - i * n2;
+                    if (((i + j) & 1) == c) x[id] = bottom_gs(x, w, b[id], n1, n2, i, j, cx, cy, hm);
+                }
+                __syncthreads();
+            }
         for (int id = t; id < n; id += nt) {
             const int i = id / n2, j = id - i * n2;
             T W, E, Sf, N;
@@ -1580,13 +1629,14 @@ __device__ void mg_bottom_body(const BottomArgs<T> &A) {
             x[id] = x[id] + e;
         }
         __syncthreads();
-        for (int c = 1; c >= 0; c--) {
-            for (int id = t; id < n; id += nt) {
-                const int i = id / n2, j = id - i * n2;
-                if (((i + j) & 1) == c) x[id] = bottom_gs(x, w, b[id], n1, n2, i, j, cx, cy, hm);
+        for (int pr = 0; pr < A.ns; pr++)
+            for (int c = 1; c >= 0; c--) {
+                for (int id = t; id < n; id += nt) {
+                    const int i = id / n2, j = id - i * n2;
+                    if (((i + j) & 1) == c) x[id] = bottom_gs(x, w, b[id], n1, n2, i, j, cx, cy, hm);
+                }
+                __syncthreads();
             }
-            __syncthreads();
-        }
     }
     {
         const int n1 = A.lev[0].n1, n2 = A.lev[0].n2;
@@ -1822,6 +1872,10 @@ template <> struct LevelArrays<float> {
     static float *r(const Level &L) { return L.rf; }
 };
 
+// red-black pairs per leg of the streaming legs: 2 for mg_pre = mg_post = 2,
+// else 1 (the single-workgroup bottom of the cycle always runs one pair)
+static int mg_ns(const impec_ctx *ctx) { return ctx->P.mg_pre == 2 && ctx->P.mg_post == 2 ? 2 : 1; }
+
 template <typename T>
 static FusedArgsT<T> fused_args(impec_ctx *ctx, int l) {
     using LA = LevelArrays<T>;
@@ -1856,7 +1910,8 @@ static FusedArgsT<T> fused_args(impec_ctx *ctx, int l) {
         A.ldc = Cl.ld;
     }
     A.part = ctx->part;
-    A.nstrip = (L.n2 + SW_OUT - 1) / SW_OUT;
+    const int out = mg_ns(ctx) == 2 ? Strip<2>::OUT : Strip<1>::OUT;
+    A.nstrip = (L.n2 + out - 1) / out;
     A.R = stream_rows(ctx, l, L.n1, A.nstrip);
     A.nwave = A.nstrip * ((L.n1 + A.R - 1) / A.R);
     return A;
@@ -1892,12 +1947,13 @@ static BottomArgs<T> bottom_args(impec_ctx *ctx, int lb) {
     A.ldb = B.ld;
     A.out = LA::r(B) + B.ld + 1;
     A.ldo = B.ld;
+    A.ns = mg_ns(ctx);
     A.S = ctx->scal;
     return A;
 }
 
-template <typename T>
-static int vcycle_fused(impec_ctx *ctx) {
+template <typename T, int NS>
+static int vcycle_fused_ns(impec_ctx *ctx) {
     using LA = LevelArrays<T>;
     const int L = ctx->nlev;
     const dim3 blk(64 * SW_WAVES);
@@ -1907,9 +1963,9 @@ static int vcycle_fused(impec_ctx *ctx) {
         FusedArgsT<T> A = fused_args<T>(ctx, l);
         prof_begin(ctx);
         if (l == 0 && A.w2 != A.w)
-            hipLaunchKernelGGL((k_mg_down<true, true, T>), stream_grid(A), blk, 0, ctx->stream, A);
-        else if (l == 0) hipLaunchKernelGGL((k_mg_down<true, false, T>), stream_grid(A), blk, 0, ctx->stream, A);
-        else hipLaunchKernelGGL((k_mg_down<false, false, T>), stream_grid(A), blk, 0, ctx->stream, A);
+            hipLaunchKernelGGL((k_mg_down<true, true, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
+        else if (l == 0) hipLaunchKernelGGL((k_mg_down<true, false, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
+        else hipLaunchKernelGGL((k_mg_down<false, false, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
         prof_end(ctx, l == 0 ? "mg_down_l0" : "mg_coarse");
     }
     if (lb > 0) {
@@ -1928,13 +1984,18 @@ static int vcycle_fused(impec_ctx *ctx) {
         FusedArgsT<T> A = fused_args<T>(ctx, l);
         prof_begin(ctx);
         if (l == 0 && A.w2 != A.w)
-            hipLaunchKernelGGL((k_mg_up<true, true, T>), stream_grid(A), blk, 0, ctx->stream, A);
-        else if (l == 0) hipLaunchKernelGGL((k_mg_up<true, false, T>), stream_grid(A), blk, 0, ctx->stream, A);
-        else hipLaunchKernelGGL((k_mg_up<false, false, T>), stream_grid(A), blk, 0, ctx->stream, A);
+            hipLaunchKernelGGL((k_mg_up<true, true, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
+        else if (l == 0) hipLaunchKernelGGL((k_mg_up<true, false, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
+        else hipLaunchKernelGGL((k_mg_up<false, false, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
         prof_end(ctx, l == 0 ? "mg_up_l0" : "mg_coarse");
     }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
+}
+
+template <typename T>
+static int vcycle_fused(impec_ctx *ctx) {
+    return mg_ns(ctx) == 2 ? vcycle_fused_ns<T, 2>(ctx) : vcycle_fused_ns<T, 1>(ctx);
 }
 
 // FP32 copy of a mobility array (rows x cols cells from (0,0) of each)
@@ -2210,7 +2271,7 @@ int pressure_solve(impec_ctx *ctx, int *its, double *relres) {
                                                                      ctx->lds,
                                             hipMemcpyDeviceToDevice, ctx->stream));
     if (ctx->P.nu != 1.) return solve_cg<true, false>(ctx, its, relres);
-    const bool fused_ok = ctx->P.mg_pre <= 1 && ctx->P.mg_post <= 1 &&
+    const bool fused_ok = ((ctx->P.mg_pre <= 1 && ctx->P.mg_post <= 1) || mg_ns(ctx) == 2) &&
                           fused_args<double>(ctx, 0).nwave <= RED_BLOCKS && !ctx->force_unfused;
     // mixed precision: FP32 V-cycles as the preconditioner of the FP64 CG (a
     // Krylov-accelerated defect correction: residual, reductions, update and
@@ -2416,7 +2477,7 @@ __global__ void kb_cg_init(const CgInitRec *__restrict__ tab, const int *__restr
         cg_init_body<false>(G, R.L, R.c, R.p, R.q, R.r, R.part);
     });
 }
-template <bool FINE0, typename T>
+template <bool FINE0, typename T, int NS>
 __global__ void __launch_bounds__(64 * SW_WAVES)
     kb_mg_down(const FusedArgsT<T> *__restrict__ tab, const int *__restrict__ start, int nmem, int total, int need) {
     const int gw = hw_wave();
@@ -2424,9 +2485,9 @@ __global__ void __launch_bounds__(64 * SW_WAVES)
     const int m = member_of(start, nmem, gw);
     const FusedArgsT<T> A = tab[m];
     if (!in_state(A.S, need)) return;
-    mg_down_body<FINE0, false, T>(A, gw - start[m]);
+    mg_down_body<FINE0, false, T, NS>(A, gw - start[m]);
 }
-template <bool FINE0, typename T>
+template <bool FINE0, typename T, int NS>
 __global__ void __launch_bounds__(64 * SW_WAVES)
     kb_mg_up(const FusedArgsT<T> *__restrict__ tab, const int *__restrict__ start, int nmem, int total, int need) {
     const int gw = hw_wave();
@@ -2434,7 +2495,7 @@ __global__ void __launch_bounds__(64 * SW_WAVES)
     const int m = member_of(start, nmem, gw);
     const FusedArgsT<T> A = tab[m];
     if (!in_state(A.S, need)) return;
-    mg_up_body<FINE0, false, T>(A, gw - start[m]);
+    mg_up_body<FINE0, false, T, NS>(A, gw - start[m]);
 }
 template <typename T>
 __global__ void __launch_bounds__(BOTTOM_THREADS) kb_mg_bottom(const BottomArgs<T> *__restrict__ tab, int need) {
@@ -2594,18 +2655,18 @@ __device__ __forceinline__ void act_wave_for(const ActList &A, F &&f) {
     }
 }
 
-template <bool FINE0, typename T>
+template <bool FINE0, typename T, int NS>
 __global__ void __launch_bounds__(64 * SW_WAVES) kp_mg_down(const FusedArgsT<T> *__restrict__ tab, ActList act) {
     act_wave_for(act, [&](int m, int loc) {
         const FusedArgsT<T> A = tab[m];
-        mg_down_body<FINE0, false, T>(A, loc);
+        mg_down_body<FINE0, false, T, NS>(A, loc);
     });
 }
-template <bool FINE0, typename T>
+template <bool FINE0, typename T, int NS>
 __global__ void __launch_bounds__(64 * SW_WAVES) kp_mg_up(const FusedArgsT<T> *__restrict__ tab, ActList act) {
     act_wave_for(act, [&](int m, int loc) {
         const FusedArgsT<T> A = tab[m];
-        mg_up_body<FINE0, false, T>(A, loc);
+        mg_up_body<FINE0, false, T, NS>(A, loc);
     });
 }
 // the direction arrays alternate every iteration of a solve: (d_old, d_new) =
@@ -2906,9 +2967,10 @@ static hipError_t launch_t(void (*k)(const E *, int), const Batch<E> &b, hipStre
     return hipGetLastError();
 }
 
-// the legs of one V-cycle of every member in state `need` (-1: all)
-template <typename T>
-static int ens_vcycle(impec_ensemble *e, const LegTabs<T> &t, int need) {
+// the legs of one V-cycle of every member in state `need` (-1: all); the
+// members share the number of red-black pairs per leg
+template <typename T, int NS>
+static int ens_vcycle_ns(impec_ensemble *e, const LegTabs<T> &t, int need) {
     const dim3 blk(64 * SW_WAVES);
     const int lmax = e->pres->lmax;
     hipStream_t s = e->stream;
@@ -2916,8 +2978,8 @@ static int ens_vcycle(impec_ensemble *e, const LegTabs<T> &t, int need) {
         const Batch<FusedArgsT<T>> &b = t.down[l];
         if (b.nmem == 0) continue;
         const dim3 g((b.total + SW_WAVES - 1) / SW_WAVES);
-        if (l == 0) hipLaunchKernelGGL((kb_mg_down<true, T>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
-        else hipLaunchKernelGGL((kb_mg_down<false, T>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
+        if (l == 0) hipLaunchKernelGGL((kb_mg_down<true, T, NS>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
+        else hipLaunchKernelGGL((kb_mg_down<false, T, NS>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
     }
     ENS_CHECK(e, launch_m(kb_coarsest<T>, t.coarsest, dim3(64), s, need));
     ENS_CHECK(e, launch_m(kb_mg_bottom<T>, t.bottom, dim3(BOTTOM_THREADS), s, need));
@@ -2925,11 +2987,15 @@ static int ens_vcycle(impec_ensemble *e, const LegTabs<T> &t, int need) {
         const Batch<FusedArgsT<T>> &b = t.up[l];
         if (b.nmem == 0) continue;
         const dim3 g((b.total + SW_WAVES - 1) / SW_WAVES);
-        if (l == 0) hipLaunchKernelGGL((kb_mg_up<true, T>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
-        else hipLaunchKernelGGL((kb_mg_up<false, T>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
+        if (l == 0) hipLaunchKernelGGL((kb_mg_up<true, T, NS>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
+        else hipLaunchKernelGGL((kb_mg_up<false, T, NS>), g, blk, 0, s, b.tab, b.start, b.nmem, b.total, need);
     }
     ENS_CHECK(e, hipGetLastError());
     return IMPEC_OK;
+}
+template <typename T>
+static int ens_vcycle(impec_ensemble *e, const LegTabs<T> &t, int need) {
+    return mg_ns(e->m[0]) == 2 ? ens_vcycle_ns<T, 2>(e, t, need) : ens_vcycle_ns<T, 1>(e, t, need);
 }
 
 int ens_solve_begin(impec_ensemble *e) {
@@ -2958,13 +3024,19 @@ static int ens_cg_slot_t(impec_ensemble *e, const LegTabs<T> &lt) {
     hipStream_t s = e->stream;
     const dim3 lblk(64 * SW_WAVES), vblk(64, 4);
     const int lmax = t->lmax;
+    const int ns = mg_ns(e->m[0]);
     for (int l = 0; l + 1 < lmax; l++) {
         const Batch<FusedArgsT<T>> &b = lt.down[l];
         if (b.nmem == 0) continue;
         const dim3 g(persist_grid((b.total + SW_WAVES - 1) / SW_WAVES, PERSIST_LEG_WG));
         const ActList &a = t->h_iter[1 + l].act;
-        if (l == 0) hipLaunchKernelGGL((kp_mg_down<true, T>), g, lblk, 0, s, b.tab, a);
-        else hipLaunchKernelGGL((kp_mg_down<false, T>), g, lblk, 0, s, b.tab, a);
+        if (ns == 2) {
+            if (l == 0) hipLaunchKernelGGL((kp_mg_down<true, T, 2>), g, lblk, 0, s, b.tab, a);
+            else hipLaunchKernelGGL((kp_mg_down<false, T, 2>), g, lblk, 0, s, b.tab, a);
+        } else {
+            if (l == 0) hipLaunchKernelGGL((kp_mg_down<true, T, 1>), g, lblk, 0, s, b.tab, a);
+            else hipLaunchKernelGGL((kp_mg_down<false, T, 1>), g, lblk, 0, s, b.tab, a);
+        }
     }
     ENS_CHECK(e, launch_m(kb_coarsest<T>, lt.coarsest, dim3(64), s, (int)EST_ITER));
     ENS_CHECK(e, launch_m(kb_mg_bottom<T>, lt.bottom, dim3(BOTTOM_THREADS), s, (int)EST_ITER));
@@ -2973,8 +3045,13 @@ static int ens_cg_slot_t(impec_ensemble *e, const LegTabs<T> &lt) {
         if (b.nmem == 0) continue;
         const dim3 g(persist_grid((b.total + SW_WAVES - 1) / SW_WAVES, PERSIST_LEG_WG));
         const ActList &a = t->h_iter[1 + l].act;
-        if (l == 0) hipLaunchKernelGGL((kp_mg_up<true, T>), g, lblk, 0, s, b.tab, a);
-        else hipLaunchKernelGGL((kp_mg_up<false, T>), g, lblk, 0, s, b.tab, a);
+        if (ns == 2) {
+            if (l == 0) hipLaunchKernelGGL((kp_mg_up<true, T, 2>), g, lblk, 0, s, b.tab, a);
+            else hipLaunchKernelGGL((kp_mg_up<false, T, 2>), g, lblk, 0, s, b.tab, a);
+        } else {
+            if (l == 0) hipLaunchKernelGGL((kp_mg_up<true, T, 1>), g, lblk, 0, s, b.tab, a);
+            else hipLaunchKernelGGL((kp_mg_up<false, T, 1>), g, lblk, 0, s, b.tab, a);
+        }
     }
     ENS_CHECK(e, hipGetLastError());
     ENS_CHECK(e, launch_m(kb_finish_rho, t->rho, dim3(1024), s, (int)EST_ITER));

@@ -72,7 +72,10 @@ typedef struct impec_params {
     int max_iterations;        /* linear solver cap (reference: 10000)         */
     int precision;             /* ext: IMPEC_PREC_*                            */
     int initial_guess;         /* ext: IMPEC_GUESS_* of the pressure solve     */
-    int mg_pre, mg_post;       /* smoothing sweeps per V-cycle leg             */
+    int mg_pre, mg_post;       /* red-black Gauss-Seidel sweeps per V-cycle    */
+                               /* leg (default 2 and 2); the streaming cycle  */
+                               /* runs 1/1 or 2/2, other counts the one-      */
+                               /* kernel-per-pass cycle                        */
     int mg_variant;            /* 0: streaming fused V-cycle where available,  */
                                /* 1: one kernel per smoothing/transfer pass,   */
                                /* 2: streaming legs down to the coarsest level */

@@ -285,3 +285,54 @@ def test_fused_and_unfused_multigrid_agree(gpu_lib, n1, n2):
     assert abs(out[0][0] - out[2][0]) <= 1, (out[0][0], out[2][0])
     assert rel_err(out[0][1], out[1][1]) < 1e-9
     assert rel_err(out[0][1], out[2][1]) < 1e-9
+
+
+@pytest.mark.parametrize("n1,n2", [(256, 256), (200, 120), (77, 45), (2048, 2048)])
+def test_two_sweep_legs_match_the_unfused_cycle(gpu_lib, n1, n2):
+    """mg_pre = mg_post = 2: the streaming legs with two red-black pairs each
+    (wider strip halos, the bottom kernel with two pairs too) against the same
+    legs run to the coarsest level, the one-kernel-per-pass V(2,2) cycle and
+    the V(1,1) legs: all reach the tolerance with the same p, the two-pair
+    cycles within one iteration of each other and no slower than V(1,1)."""
+    p = params(N1=n1, N2=n2, nu=1.0, eps=1e-11, mu_inj=0.1)
+    _say("two-sweep legs", n1, n2)
+    out = []
+    for variant, ns in ((0, 2), (2, 2), (1, 2), (0, 1)):
+        o, g = pair(p, mg_variant=variant, mg_pre=ns, mg_post=ns)
+        c = smooth_c(o, seed=17)
+        g.set_concentration(c)
+        g.set_source(o.source("one_cell"))
+        its, rr = g.pressure_solve()
+        assert rr <= 1e-11
+        out.append((its, o.interior(g.get_pressure())))
+    _say("  iterations: legs V(2,2) %d, legs to the coarsest %d, unfused V(2,2) %d, legs V(1,1) %d"
+         % tuple(x[0] for x in out))
+    assert abs(out[0][0] - out[2][0]) <= 1 and out[0][0] <= out[3][0], [x[0] for x in out]
+    assert abs(out[0][0] - out[1][0]) <= 1
+    for x in out[1:]:
+        assert rel_err(out[0][1], x[1]) < 1e-9
+
+
+@pytest.mark.parametrize("kw", [dict(nu=1.0), dict(nu=1.0, diffusion=1, alpha_abs=1, alpha_l=1e-3, alpha_t=1e-4,
+                                                   d_m=1e-5)])
+def test_steps_with_two_sweep_legs(gpu_lib, kw):
+    """Ten IMPEC steps with the two-pair streaming legs on a grid large enough
+    for them (96^2: two streaming levels above the bottom kernel) agree with
+    the oracle to 1e-10 relative."""
+    p = params(**dict(dict(N1=96, N2=96, eps=1e-13, para_eps=1e-13, mu_inj=0.1), **kw))
+    _say("steps, two-sweep legs", kw)
+    o, g = pair(p, mg_pre=2, mg_post=2)
+    q = o.source("one_cell")
+    c, pr, v = o.scalar(), o.scalar(), o.velocity()
+    g.set_source(q)
+    g.set_concentration(c)
+    t_ref = t_gpu = 0.0
+    for k in range(1, 11):
+        st, dt_ref = o.step(c, pr, v, q, t_ref)
+        assert st == 0
+        t_ref += dt_ref
+        t_gpu += g.step(t_gpu)
+    cg, pg, vg = g.get_concentration(), g.get_pressure(), g.get_velocity()
+    assert rel_err(o.interior(pg), o.interior(pr)) < 1e-10
+    assert rel_err(vg, v) < 1e-10
+    assert rel_err(o.interior(cg), o.interior(c)) < 1e-10
