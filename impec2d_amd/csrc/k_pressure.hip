@@ -1848,7 +1848,7 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
 // levels, short chunks on small ones; even, since the march starts on an
 // even row
 #ifndef IMPEC_MG_MINROWS
-#define IMPEC_MG_MINROWS 8
+#define IMPEC_MG_MINROWS 4
 #endif
 static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip) {
     if (l == 0 && ctx->P.mg_rows > 0) return (ctx->P.mg_rows + 1) & ~1;
