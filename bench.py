@@ -122,6 +122,7 @@ def modes(capi, n, warmup=3, steps=8):
     res["speedup_mixed_learned_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_learned"]["ms_per_step"]
     res["speedup_mixed_extrapolated_vs_fp64"] = (res["fp64_zero"]["ms_per_step"]
                                                  / res["mixed_extrapolated"]["ms_per_step"])
+    res["speedup_mixed_previous_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_previous"]["ms_per_step"]
     res["speedup_mixed_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_zero"]["ms_per_step"]
     return res
 
