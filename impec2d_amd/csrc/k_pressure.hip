@@ -452,6 +452,30 @@ __global__ void k_cg_init(LevArgs L, CgArgs c, double *__restrict__ p, const dou
     cg_init_body<DIAG>(HwGrid{}, L, c, p, q, r, part);
 }
 
+// k_cg_init of the shift formulation from the zero iterate (zero initial
+// guess): p = 0 on the interior and r = b = -q (minus sum(b) at the pinned
+// cell), with no operator application (A 0 = 0); the same partials in the same
+// order
+__global__ void k_cg_init_zero(CgArgs c, double *__restrict__ p, const double *__restrict__ q,
+                               double *__restrict__ r, double *__restrict__ part) {
+    double s[2] = {0., 0.};
+    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
+        int i, j;
+        cg_cell(c, id, i, j);
+        p[(long)(i + c.B) * c.ldp + (j + c.B)] = 0.;
+        double rv = -q[id];
+        if (i == c.pin_i && j == c.pin_j) rv -= c.bsum;
+        r[(long)(i + 1) * c.ld + (j + 1)] = rv;
+        s[0] += rv * rv;
+        s[1] += rv;
+    }
+    block_reduce<RED_SUM, 2>(s);
+    if (tid_linear() == 0) {
+        part[blockIdx.x] = s[0];
+        part[RED_BLOCKS + blockIdx.x] = s[1];
+    }
+}
+
 // Warm starts from the previous solutions: p = p_n + alpha d + gamma e, with
 // d the network's increment (learned guess) and e = p_n - p_(n-1) the last
 // step's change.  alpha and gamma minimise the 2-norm of the consistent
@@ -2179,7 +2203,10 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
 
     const int guess = ctx->P.initial_guess;
     const bool warm = guess != IMPEC_GUESS_ZERO && ctx->have_prev_p;
-    if (!warm) hipLaunchKernelGGL(k_zero_interior, dim3(nb), dim3(256), 0, s, c, ctx->p);
+    // the zero iterate: in the shift formulation its residual is b itself, formed
+    // together with p = 0 (k_cg_init_zero) instead of zeroing p and applying A
+    const bool zero_start = !warm && ctx->symmetric;
+    if (!warm && !zero_start) hipLaunchKernelGGL(k_zero_interior, dim3(nb), dim3(256), 0, s, c, ctx->p);
     else if (guess == IMPEC_GUESS_LEARNED || guess == IMPEC_GUESS_EXTRAPOLATE) {
         prof_begin(ctx);
         double *d = nullptr;   // the network's increment, in pd (free until the CG starts)
@@ -2206,7 +2233,17 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     double *dold = ctx->pd, *dnew = ctx->pz;
     for (int phase = ctx->symmetric ? 0 : 1; phase < 2; phase++) {
         c.shift = phase == 0;
-        if ((st = cg_residual<DIAG>(ctx, a0, c, nb, &rr))) return st;
+        if (phase == 0 && zero_start) {
+            prof_begin(ctx);
+            hipLaunchKernelGGL(k_cg_init_zero, dim3(nb), dim3(256), 0, s, c, ctx->p, ctx->q, ctx->pr, ctx->part);
+            IMPEC_HIP_CHECK(ctx, hipGetLastError());
+            st = finish_sum(ctx, nb, 2, S_RR);
+            prof_end(ctx, "cg_init");
+            if (st || (st = fetch_scal(ctx))) return st;
+            rr = ctx->h_scal[S_RR];
+        } else if ((st = cg_residual<DIAG>(ctx, a0, c, nb, &rr))) {
+            return st;
+        }
         int since = 0, restarts = 0;
         for (;;) {
             while (!(rr <= tol * tol * bb)) {
