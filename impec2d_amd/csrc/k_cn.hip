@@ -114,28 +114,41 @@ __device__ __forceinline__ void cn_coeffs(const CnConst &k, const CellTensors &T
 }
 
 // SetRHS_Parabolic (MGSolver.cpp:1324-1370) and the Jacobi preconditioner;
-// c, u, v ghost-filled
+// c, u, v ghost-filled.  64 x 4 threads per workgroup over (column, row), rows
+// strided; per-workgroup partials of b.b (twice: ||b||^2 and, for the zero
+// initial guess, r0.r) for the solver's start
 __global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ u,
-                         const double *__restrict__ v, double *__restrict__ rhs, float *__restrict__ diag) {
+                         const double *__restrict__ v, double *__restrict__ rhs, float *__restrict__ diag,
+                         double *__restrict__ part) {
     const HwGrid g_;
-    const int j = g_.bx() * blockDim.x + threadIdx.x;
-    const int i = g_.by() * blockDim.y + threadIdx.y;
-    if (i >= k.N1 || j >= k.N2) return;
-    const CellTensors T = cell_tensors(k, u, v, i, j);
-    const double *cc = c + (long)(i + k.B) * lds + (j + k.B);
+    const int j = g_.bx() * 64 + threadIdx.x;
+    double sq = 0.;
+    if (j < k.N2)
+        for (int i = g_.by() * 4 + threadIdx.y; i < k.N1; i += g_.gy() * 4) {
+            const CellTensors T = cell_tensors(k, u, v, i, j);
+            const double *cc = c + (long)(i + k.B) * lds + (j + k.B);
 #define C_(a, b) cc[(long)(a) * lds + (b)]
-    const double r = C_(0, 0) + k.dt8 * (
-        k.c4dx2 * (T.d11p * (C_(1, 0) - C_(0, 0)) - T.d11m * (C_(0, 0) - C_(-1, 0)))
-        + k.c1dxdy * (T.d12p * (C_(0, 1) + C_(1, 1) - C_(0, -1) - C_(1, -1))
-                      - T.d12m * (C_(0, 1) + C_(-1, 1) - C_(-1, -1) - C_(0, -1)))
-        + k.c1dxdy * (T.d21p * (C_(1, 0) + C_(1, 1) - C_(-1, 1) - C_(-1, 0))
-                      - T.d21m * (C_(1, 0) + C_(1, -1) - C_(-1, 0) - C_(-1, -1)))
-        + k.c4dy2 * (T.d22p * (C_(0, 1) - C_(0, 0)) - T.d22m * (C_(0, 0) - C_(0, -1))));
+            const double r = C_(0, 0) + k.dt8 * (
+                k.c4dx2 * (T.d11p * (C_(1, 0) - C_(0, 0)) - T.d11m * (C_(0, 0) - C_(-1, 0)))
+                + k.c1dxdy * (T.d12p * (C_(0, 1) + C_(1, 1) - C_(0, -1) - C_(1, -1))
+                              - T.d12m * (C_(0, 1) + C_(-1, 1) - C_(-1, -1) - C_(0, -1)))
+                + k.c1dxdy * (T.d21p * (C_(1, 0) + C_(1, 1) - C_(-1, 1) - C_(-1, 0))
+                              - T.d21m * (C_(1, 0) + C_(1, -1) - C_(-1, 0) - C_(-1, -1)))
+                + k.c4dy2 * (T.d22p * (C_(0, 1) - C_(0, 0)) - T.d22m * (C_(0, 0) - C_(0, -1))));
 #undef C_
-    rhs[(long)i * k.N2 + j] = r;
-    double a[9];
-    cn_coeffs(k, T, i, j, a);
-    diag[(long)i * k.N2 + j] = (float)(1. / a[ST_C]);   // Jacobi preconditioner, rounded to FP32
+            rhs[(long)i * k.N2 + j] = r;
+            double a[9];
+            cn_coeffs(k, T, i, j, a);
+            diag[(long)i * k.N2 + j] = (float)(1. / a[ST_C]);   // Jacobi preconditioner, rounded to FP32
+            sq += r * r;
+        }
+    double s2[2] = {sq, sq};
+    block_reduce<RED_SUM, 2>(s2);
+    if (tid_linear() == 0) {
+        const int blk = g_.by() * g_.gx() + g_.bx();
+        part[blk] = s2[0];
+        part[RED_BLOCKS + blk] = s2[1];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -185,6 +198,7 @@ struct CnArgs {
     const double *S;
     double *part;                          // per-wave partials
     int R, nstrip, nwave;
+    int xzero;                             // CN_P: x is zero (the first update of a zero-guess solve)
 };
 
 template <int MODE>
@@ -235,7 +249,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
             Pa[u] = cell(A.s, m);
             Pb[u] = cell(A.t, m);
             Pp[u] = cell(A.pin, m);
-            Px[u] = cell(A.x, m);
+            if (!A.xzero) Px[u] = cell(A.x, m);
         } else {
             Pa[u] = cell(A.r, m);
         }
@@ -293,7 +307,8 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
                 A.pout[o] = wv;
                 if (MODE == CN_P) {
                     A.rout[o] = rv;
-                    A.x[o] = Px[u] + (alpha * (Pp[u] * dv) + om * (Pa[u] * dv));
+                    const double dx_ = alpha * (Pp[u] * dv) + om * (Pa[u] * dv);
+                    A.x[o] = A.xzero ? dx_ : Px[u] + dx_;
                 }
                 if (MODE == CN_S) s3 += r0v * wv;
                 else s1 += rv * rv;
@@ -465,6 +480,14 @@ __global__ void k_bicg_finish(const double *__restrict__ part, int nb, int stage
     }
 }
 
+// x = 0 (a zero-guess solve that needed no iteration)
+__global__ void k_cn_zero(int N1, int N2, double *__restrict__ x) {
+    const HwGrid g_;
+    const int j = g_.bx() * 64 + threadIdx.x;
+    if (j < N2)
+        for (int i = g_.by() * 4 + threadIdx.y; i < N1; i += g_.gy() * 4) x[(long)i * N2 + j] = 0.;
+}
+
 // solution into the interior of c
 __global__ void k_cn_store(const double *__restrict__ x, double *__restrict__ c, int N1, int N2, int B, int lds) {
     const HwGrid g_;
@@ -513,8 +536,9 @@ static CnConst cn_const(const impec_ctx *ctx, double dt) {
 
 // right-hand side (into cn_rhs) and rounded inverse folded diagonal (into cn_dg)
 static int tensors_and_rhs(impec_ctx *ctx, const CnConst &k) {
-    dim3 blk(64, 4), gc((ctx->N2 + 63) / 64, (ctx->N1 + 3) / 4);
-    hipLaunchKernelGGL(k_cn_rhs, gc, blk, 0, ctx->stream, k, ctx->c, ctx->lds, ctx->u, ctx->v, ctx->cn_rhs, ctx->cn_dg);
+    const dim3 vg = cn_vgrid(ctx->N1, ctx->N2), blk(64, 4);
+    hipLaunchKernelGGL(k_cn_rhs, vg, blk, 0, ctx->stream, k, ctx->c, ctx->lds, ctx->u, ctx->v, ctx->cn_rhs, ctx->cn_dg,
+                       ctx->part);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
 }
@@ -543,9 +567,12 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     if (st) return st;
     const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
     const int nvb = (int)(vg.x * vg.y);
-    hipLaunchKernelGGL(k_bicg_init, vg, vblk, 0, s, N1, N2, b, r, r0, x, ctx->part);
+    // zero initial guess: r = r0 = b and x = 0 are not formed; the first
+    // launches read b, r0 stays b, and the first p-launch writes x
+    const bool zero = ctx->P.cn_guess != 1;
+    if (!zero) hipLaunchKernelGGL(k_bicg_init, vg, vblk, 0, s, N1, N2, b, r, r0, x, ctx->part);
     hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 0, ctx->scal);
-    if (ctx->P.cn_guess == 1) {   // start from the current concentration
+    if (!zero) {   // start from the current concentration
         hipLaunchKernelGGL(k_bicg_guess, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, x, r, r0,
                            ctx->part);
         hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 4, ctx->scal);
@@ -562,7 +589,8 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     a.k = k;
     a.u = ctx->u; a.v = ctx->v;
     a.dinv = ctx->cn_dg;
-    a.r = r; a.r0 = r0;
+    a.r = zero ? b : r;
+    a.r0 = zero ? b : r0;
     a.S = ctx->scal;
     a.part = ctx->part;
     a.nstrip = (N2 + CN_OUT - 1) / CN_OUT;
@@ -591,6 +619,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         if (its >= maxit) { status = IMPEC_ENOCONV; break; }
         if (!(rr == rr)) { status = IMPEC_ENAN; break; }
         CnArgs as = a;
+        as.r = its == 0 ? a.r : r;   // b before the first p-launch of a zero-guess solve
         as.vin = vb[cur]; as.pout = sv; as.out = t;
         prof_begin(ctx);
         hipLaunchKernelGGL(k_cn_stream<CN_S>, sg, sblk, 0, s, as);
@@ -598,6 +627,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         hipLaunchKernelGGL(k_bicg_finish<4>, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 2, ctx->scal);
         CnArgs ap = a;
         ap.s = sv; ap.t = t; ap.pin = pb[cur]; ap.vin = vb[cur]; ap.x = x; ap.rout = r;
+        ap.xzero = zero && its == 0;
         ap.pout = pb[cur ^ 1]; ap.out = vb[cur ^ 1];
         prof_begin(ctx);
         hipLaunchKernelGGL(k_cn_stream<CN_P>, sg, sblk, 0, s, ap);
@@ -610,6 +640,8 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         its++;
     }
     dim3 blk(64, 4), gc((N2 + 63) / 64, (N1 + 3) / 4);
+    if (zero && its == 0)   // b already met the tolerance: x = 0 was never written
+        hipLaunchKernelGGL(k_cn_zero, vg, vblk, 0, s, N1, N2, x);
     hipLaunchKernelGGL(k_cn_store, gc, blk, 0, s, x, ctx->c, N1, N2, ctx->B, ctx->lds);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     ctx->stats.last_parabolic_its = its;
