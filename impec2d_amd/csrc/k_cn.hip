@@ -193,6 +193,7 @@ struct CnArgs {
     const double *s, *t;                   // CN_P: s and t of the previous iteration
     const double *pin, *vin, *r0;          // flat N1 x N2
     double *x, *rout;                      // CN_P: the solution (in place) and the new residual
+    long ldx;                              // row pitch of x (the interior of the ghosted c)
     double *pout;                          // CN_P, CN_P0: new p; CN_S: s
     double *out;                           // CN_P, CN_P0: v; CN_S: t
     const double *S;
@@ -228,6 +229,10 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
         const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
         return p[(long)mc * N2 + jc];
     };
+    auto cellx = [&](const double *p, int m) -> double {   // x: the interior of c, pitch ldx
+        const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
+        return p[(long)mc * A.ldx + jc];
+    };
     auto cellf = [&](const float *p, int m) -> float {
         const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
         return p[(long)mc * N2 + jc];
@@ -249,7 +254,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
             Pa[u] = cell(A.s, m);
             Pb[u] = cell(A.t, m);
             Pp[u] = cell(A.pin, m);
-            if (!A.xzero) Px[u] = cell(A.x, m);
+            if (!A.xzero) Px[u] = cellx(A.x, m);
         } else {
             Pa[u] = cell(A.r, m);
         }
@@ -308,7 +313,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
                 if (MODE == CN_P) {
                     A.rout[o] = rv;
                     const double dx_ = alpha * (Pp[u] * dv) + om * (Pa[u] * dv);
-                    A.x[o] = A.xzero ? dx_ : Px[u] + dx_;
+                    A.x[(long)m * A.ldx + j] = A.xzero ? dx_ : Px[u] + dx_;
                 }
                 if (MODE == CN_S) s3 += r0v * wv;
                 else s1 += rv * rv;
@@ -380,35 +385,11 @@ static dim3 cn_vgrid(int N1, int N2) {
     return dim3(bx, by);
 }
 
-// r = r0 = b, x = 0; partials b.b (twice: ||b||^2 and r0.r)
-__global__ void k_bicg_init(int N1, int N2, const double *__restrict__ b, double *__restrict__ r,
-                            double *__restrict__ r0, double *__restrict__ x, double *__restrict__ part) {
-    const HwGrid g_;
-    const int j = g_.bx() * 64 + threadIdx.x;
-    double s[2] = {0., 0.};
-    if (j < N2)
-        for (int i = g_.by() * 4 + threadIdx.y; i < N1; i += g_.gy() * 4) {
-            const long id = (long)i * N2 + j;
-            const double bv = b[id];
-            r[id] = bv;
-            r0[id] = bv;
-            x[id] = 0.;
-            s[0] += bv * bv;
-            s[1] += bv * bv;
-        }
-    block_reduce<RED_SUM, 2>(s);
-    if (tid_linear() == 0) {
-        const int blk = g_.by() * g_.gx() + g_.bx();
-        part[blk] = s[0];
-        part[RED_BLOCKS + blk] = s[1];
-    }
-}
-
-// initial guess x0 = c (interior): x = x0, r = r0 = b - A x0; partials r.r
-// (twice)
+// initial guess x0 = c (the solution is formed in place in c's interior):
+// r = r0 = b - A x0; partials r.r (twice)
 __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ u,
-                             const double *__restrict__ v, const double *__restrict__ b, double *__restrict__ x,
-                             double *__restrict__ r, double *__restrict__ r0, double *__restrict__ part) {
+                             const double *__restrict__ v, const double *__restrict__ b, double *__restrict__ r,
+                             double *__restrict__ r0, double *__restrict__ part) {
     const HwGrid g_;
     const int j = g_.bx() * 64 + threadIdx.x;
     double s[2] = {0., 0.};
@@ -430,7 +411,6 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
             if (we && no) acc += a[ST_NW] * cc[-L + 1];
             const long id = (long)i * k.N2 + j;
             const double rv = b[id] - acc;
-            x[id] = cc[0];
             r[id] = rv;
             r0[id] = rv;
             s[0] += rv * rv;
@@ -480,22 +460,14 @@ __global__ void k_bicg_finish(const double *__restrict__ part, int nb, int stage
     }
 }
 
-// x = 0 (a zero-guess solve that needed no iteration)
-__global__ void k_cn_zero(int N1, int N2, double *__restrict__ x) {
+// x = 0 (a zero-guess solve that needed no iteration), pitch ldx
+__global__ void k_cn_zero(int N1, int N2, double *__restrict__ x, long ldx) {
     const HwGrid g_;
     const int j = g_.bx() * 64 + threadIdx.x;
     if (j < N2)
-        for (int i = g_.by() * 4 + threadIdx.y; i < N1; i += g_.gy() * 4) x[(long)i * N2 + j] = 0.;
+        for (int i = g_.by() * 4 + threadIdx.y; i < N1; i += g_.gy() * 4) x[(long)i * ldx + j] = 0.;
 }
 
-// solution into the interior of c
-__global__ void k_cn_store(const double *__restrict__ x, double *__restrict__ c, int N1, int N2, int B, int lds) {
-    const HwGrid g_;
-    const int j = g_.bx() * blockDim.x + threadIdx.x;
-    const int i = g_.by() * blockDim.y + threadIdx.y;
-    if (i >= N1 || j >= N2) return;
-    c[(long)(i + B) * lds + (j + B)] = x[(long)i * N2 + j];
-}
 
 // matrix dump: nine N1*N2 arrays
 __global__ void k_cn_dump(CnConst k, const double *__restrict__ u, const double *__restrict__ v,
@@ -558,7 +530,9 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     const CnConst k = cn_const(ctx, dt);
     const int N1 = ctx->N1, N2 = ctx->N2;
     hipStream_t s = ctx->stream;
-    double *b = ctx->cn_rhs, *x = ctx->cn_x, *r = ctx->cn_r, *r0 = ctx->cn_r0, *sv = ctx->cn_s, *t = ctx->cn_t;
+    // the solution is formed in place in the interior of c (x0 = c for cn_guess = 1)
+    double *b = ctx->cn_rhs, *x = ctx->c + (long)ctx->B * ctx->lds + ctx->B, *r = ctx->cn_r, *r0 = ctx->cn_r0;
+    double *sv = ctx->cn_s, *t = ctx->cn_t;
     // p, v ping-pong: a strip reads its neighbours' old values at the halo
     double *pb[2] = {ctx->cn_p, ctx->cn_y}, *vb[2] = {ctx->cn_v, ctx->cn_z};
     prof_begin(ctx);
@@ -568,13 +542,12 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
     const int nvb = (int)(vg.x * vg.y);
     // zero initial guess: r = r0 = b and x = 0 are not formed; the first
-    // launches read b, r0 stays b, and the first p-launch writes x
+    // launches read b, r0 stays b, and the first p-launch writes x.  The
+    // b.b partials come from the right-hand side kernel.
     const bool zero = ctx->P.cn_guess != 1;
-    if (!zero) hipLaunchKernelGGL(k_bicg_init, vg, vblk, 0, s, N1, N2, b, r, r0, x, ctx->part);
     hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 0, ctx->scal);
     if (!zero) {   // start from the current concentration
-        hipLaunchKernelGGL(k_bicg_guess, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, x, r, r0,
-                           ctx->part);
+        hipLaunchKernelGGL(k_bicg_guess, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, r, r0, ctx->part);
         hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 4, ctx->scal);
     }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
@@ -591,6 +564,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     a.dinv = ctx->cn_dg;
     a.r = zero ? b : r;
     a.r0 = zero ? b : r0;
+    a.ldx = ctx->lds;
     a.S = ctx->scal;
     a.part = ctx->part;
     a.nstrip = (N2 + CN_OUT - 1) / CN_OUT;
@@ -639,10 +613,8 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         cur ^= 1;
         its++;
     }
-    dim3 blk(64, 4), gc((N2 + 63) / 64, (N1 + 3) / 4);
     if (zero && its == 0)   // b already met the tolerance: x = 0 was never written
-        hipLaunchKernelGGL(k_cn_zero, vg, vblk, 0, s, N1, N2, x);
-    hipLaunchKernelGGL(k_cn_store, gc, blk, 0, s, x, ctx->c, N1, N2, ctx->B, ctx->lds);
+        hipLaunchKernelGGL(k_cn_zero, vg, vblk, 0, s, N1, N2, x, (long)ctx->lds);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     ctx->stats.last_parabolic_its = its;
     ctx->stats.last_parabolic_relres = bb > 0 ? sqrt(rr / bb) : 0.;

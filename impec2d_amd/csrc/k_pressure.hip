@@ -2244,7 +2244,24 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
         } else if ((st = cg_residual<DIAG>(ctx, a0, c, nb, &rr))) {
             return st;
         }
+        // p to the reference's pinned system (the shift, or the pinned value) and
+        // that system's true relative residual
+        auto finish_phase = [&]() -> int {
+            if (c.shift) {
+                hipLaunchKernelGGL(k_read_pin, dim3(1), dim3(1), 0, s, c, ctx->p, ctx->scal);
+                hipLaunchKernelGGL(k_shift, dim3(nb), dim3(256), 0, s, c, ctx->p, ctx->scal);
+            } else {
+                hipLaunchKernelGGL(k_set_pin, dim3(1), dim3(1), 0, s, c, ctx->p);
+            }
+            hipLaunchKernelGGL(k_true_residual<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q, ctx->part);
+            IMPEC_HIP_CHECK(ctx, hipGetLastError());
+            int e = finish_sum(ctx, nb, 1, S_TT);
+            if (!e) e = fetch_scal(ctx);
+            if (!e) relres = ctx->bnorm > 0 ? sqrt(ctx->h_scal[S_TT]) / ctx->bnorm : 0.;
+            return e;
+        };
         int since = 0, restarts = 0;
+        bool finished = false;
         for (;;) {
             while (!(rr <= tol * tol * bb)) {
                 if (its >= maxit) { status = IMPEC_ENOCONV; break; }
@@ -2258,22 +2275,22 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
                 since++;
             }
             if (status != IMPEC_OK) break;
+            if (c.shift) {
+                // the consistent formulation confirms convergence on the pinned
+                // system itself: shift, then its true residual.  When that fails
+                // (drift of the recursion, or a shift too large for the rows'
+                // rounding) the elimination phase continues from this iterate
+                // and restarts on drift itself.
+                if ((st = finish_phase())) return st;
+                finished = true;
+                break;
+            }
             if ((st = cg_residual<DIAG>(ctx, a0, c, nb, &rr))) return st;
             if (rr <= tol * tol * bb || restarts >= 3) break;
             restarts++;
             since = 0;
         }
-        if (c.shift) {
-            hipLaunchKernelGGL(k_read_pin, dim3(1), dim3(1), 0, s, c, ctx->p, ctx->scal);
-            hipLaunchKernelGGL(k_shift, dim3(nb), dim3(256), 0, s, c, ctx->p, ctx->scal);
-        } else {
-            hipLaunchKernelGGL(k_set_pin, dim3(1), dim3(1), 0, s, c, ctx->p);
-        }
-        hipLaunchKernelGGL(k_true_residual<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q, ctx->part);
-        IMPEC_HIP_CHECK(ctx, hipGetLastError());
-        if ((st = finish_sum(ctx, nb, 1, S_TT))) return st;
-        if ((st = fetch_scal(ctx))) return st;
-        relres = ctx->bnorm > 0 ? sqrt(ctx->h_scal[S_TT]) / ctx->bnorm : 0.;
+        if (!finished && (st = finish_phase())) return st;
         if (status != IMPEC_OK || relres <= tol) break;
     }
     const bool used_d = warm && guess == IMPEC_GUESS_LEARNED && ctx->net;
