@@ -378,12 +378,18 @@ static_assert(4 <= PART_VECS, "the s-launch writes four partial vectors");
 // launch shape of the element-wise BiCGStab kernels: 64 x 4 threads over
 // (column, row), rows strided so that there are at most CN_VBLOCKS blocks
 constexpr int CN_VBLOCKS = 2048;
-static dim3 cn_vgrid(int N1, int N2) {
+static dim3 cn_vgrid(int N1, int N2, int cap = CN_VBLOCKS) {
     const int bx = (N2 + 63) / 64;
     int by = (N1 + 3) / 4;
-    if ((long)bx * by > CN_VBLOCKS) by = CN_VBLOCKS / bx > 0 ? CN_VBLOCKS / bx : 1;
+    if ((long)bx * by > cap) by = cap / bx > 0 ? cap / bx : 1;
     return dim3(bx, by);
 }
+// workgroups of the right-hand side kernel (build-time tunable; at most
+// RED_BLOCKS, the partial-sum capacity)
+#ifndef IMPEC_CN_RHS_BLOCKS
+#define IMPEC_CN_RHS_BLOCKS 2048
+#endif
+static_assert(IMPEC_CN_RHS_BLOCKS <= RED_BLOCKS, "partials of the right-hand side kernel");
 
 // initial guess x0 = c (the solution is formed in place in c's interior):
 // r = r0 = b - A x0; partials r.r (twice)
@@ -506,12 +512,14 @@ static CnConst cn_const(const impec_ctx *ctx, double dt) {
     return k;
 }
 
-// right-hand side (into cn_rhs) and rounded inverse folded diagonal (into cn_dg)
-static int tensors_and_rhs(impec_ctx *ctx, const CnConst &k) {
-    const dim3 vg = cn_vgrid(ctx->N1, ctx->N2), blk(64, 4);
+// right-hand side (into cn_rhs) and rounded inverse folded diagonal (into
+// cn_dg); *nblocks: the workgroups that wrote b.b partials
+static int tensors_and_rhs(impec_ctx *ctx, const CnConst &k, int *nblocks = nullptr) {
+    const dim3 vg = cn_vgrid(ctx->N1, ctx->N2, IMPEC_CN_RHS_BLOCKS), blk(64, 4);
     hipLaunchKernelGGL(k_cn_rhs, vg, blk, 0, ctx->stream, k, ctx->c, ctx->lds, ctx->u, ctx->v, ctx->cn_rhs, ctx->cn_dg,
                        ctx->part);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    if (nblocks) *nblocks = (int)(vg.x * vg.y);
     return IMPEC_OK;
 }
 
@@ -536,7 +544,8 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     // p, v ping-pong: a strip reads its neighbours' old values at the halo
     double *pb[2] = {ctx->cn_p, ctx->cn_y}, *vb[2] = {ctx->cn_v, ctx->cn_z};
     prof_begin(ctx);
-    int st = tensors_and_rhs(ctx, k);
+    int nrb = 0;
+    int st = tensors_and_rhs(ctx, k, &nrb);
     prof_end(ctx, "cn_rhs");
     if (st) return st;
     const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
@@ -545,7 +554,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     // launches read b, r0 stays b, and the first p-launch writes x.  The
     // b.b partials come from the right-hand side kernel.
     const bool zero = ctx->P.cn_guess != 1;
-    hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 0, ctx->scal);
+    hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nrb, 0, ctx->scal);
     if (!zero) {   // start from the current concentration
         hipLaunchKernelGGL(k_bicg_guess, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, r, r0, ctx->part);
         hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 4, ctx->scal);
