@@ -65,9 +65,9 @@ static int ensure_cn(impec_ctx *ctx) {
     if (ctx->cn_rhs) return IMPEC_OK;
     const size_t n = (size_t)ctx->N1 * ctx->N2;
     int st;
-    double **v[9] = {&ctx->cn_rhs, &ctx->cn_r, &ctx->cn_r0, &ctx->cn_p, &ctx->cn_v,
-                     &ctx->cn_s, &ctx->cn_t, &ctx->cn_y, &ctx->cn_z};
-    for (int k = 0; k < 9; k++)
+    double **v[8] = {&ctx->cn_rhs, &ctx->cn_r, &ctx->cn_r0, &ctx->cn_p, &ctx->cn_v,
+                     &ctx->cn_r2, &ctx->cn_y, &ctx->cn_z};
+    for (int k = 0; k < 8; k++)
         if ((st = dalloc(ctx, v[k], n))) return st;
     if (hipMalloc((void **)&ctx->cn_dg, n * sizeof(float)) != hipSuccess) {
         ctx->cn_dg = nullptr;
@@ -237,7 +237,7 @@ int impec_destroy(impec_ctx *ctx) {
     double *bufs[] = {ctx->c, ctx->p, ctx->u, ctx->v, ctx->q, ctx->K11, ctx->K22 != ctx->K11 ? ctx->K22 : nullptr,
                       ctx->w, ctx->w22, ctx->t1, ctx->t2, ctx->t3, ctx->scal, ctx->part,
                       ctx->cn_rhs, ctx->cn_r, ctx->cn_r0, ctx->cn_p, ctx->cn_v,
-                      ctx->cn_s, ctx->cn_t, ctx->cn_y, ctx->cn_z, ctx->src_q, ctx->tr_md, ctx->p_prev,
+                      ctx->cn_r2, ctx->cn_y, ctx->cn_z, ctx->src_q, ctx->tr_md, ctx->p_prev,
                       ctx->p_old};
     for (double *b : bufs)
         if (b) hipFree(b);

@@ -116,7 +116,7 @@ struct impec_ctx {
     int have_prev_p;
 
     // Crank-Nicolson
-    double *cn_rhs, *cn_r, *cn_r0, *cn_p, *cn_v, *cn_s, *cn_t, *cn_y, *cn_z;   // the solution forms in c
+    double *cn_rhs, *cn_r, *cn_r0, *cn_p, *cn_v, *cn_r2, *cn_y, *cn_z;   // the solution forms in c; r ping-pongs with cn_r2
     float *cn_dg;                 // rounded inverse of the folded CN diagonal
 
     // sources

@@ -154,32 +154,39 @@ __global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const
 // ---------------------------------------------------------------------------
 // Jacobi-preconditioned BiCGStab, streamed.  Each operator application carries
 // the vector updates on either side of it, so an iteration is two launches:
-//   k_cn_stream<CN_S>:  s = r - alpha v,  t = A (s / D);
-//                       partials t.s, t.t, r0.t, r0.s
-//   k_cn_stream<CN_P>:  r = s - omega t and x += alpha p/D + omega s/D (the
-//                       end of the previous iteration), p = r + beta (p - omega v),
-//                       v = A (p / D); partials r0.v, r.r
+//   k_cn_stream<CN_S>:  s = r - alpha v,  t = A (s / D); partials t.s, t.t,
+//                       r0.t, r0.s (s and t are not stored)
+//   k_cn_pstep:         s and t again from r and v, then r' = s - omega t and
+//                       x += alpha p/D + omega s/D (the end of the iteration),
+//                       p' = r' + beta (p - omega v), v' = A (p' / D);
+//                       partials r0.v', r'.r'
 //   k_cn_stream<CN_P0>: the first direction p = r, v = A (p / D).
-// rho_new = r0.r of the next residual is formed from the s-launch's partials
-// as r0.s - omega r0.t, so beta is known before the p-launch; the convergence
-// test uses the p-launch's r.r, the recursive residual itself.  The last
+// rho_new = r0.r' is formed from the s-launch's partials as r0.s - omega r0.t,
+// so beta is known before the p-launch; the convergence test uses the
+// p-launch's r'.r', the recursive residual itself.  The p-launch forms s and t
+// with the s-launch's arithmetic, bit for bit, so omega belongs to the t it
+// uses; recomputing them costs an operator application and saves storing and
+// re-reading two vectors (24 of 160 bytes per cell and iteration).  The last
 // p-launch of a solve computes a direction nobody uses: one operator
 // application per solve, against a separate vector pass per iteration.
 // 1/D is stored in FP32: the preconditioner is diag(1/d~) with d~ the rounded
 // inverse, applied identically to the matvec input and to the solution
 // update, so the iteration solves the FP64 system to the same residual.
-// A wavefront owns a strip of 64 columns (one per lane, 62 written) and a
-// chunk of rows and marches down the rows with the input vector of three rows
-// in registers; the column neighbours come from the adjacent lanes.  The
-// scalars live in device memory, so an iteration needs one host
-// synchronisation, for the convergence test.
+// A wavefront owns a strip of 64 columns (one per lane) and a chunk of rows
+// and marches down the rows with the operator's input vectors of three rows in
+// registers; the column neighbours come from the adjacent lanes.  The scalars
+// live in device memory, so an iteration needs one host synchronisation, for
+// the convergence test.
 // ---------------------------------------------------------------------------
 constexpr int S_RHO_NEW = 13;
-constexpr int CN_OUT = 62, CN_WAVES = 4;
-enum { CN_P = 0, CN_S = 1, CN_P0 = 2 };
+constexpr int CN_OUT = 62, CNP_OUT = 60, CN_WAVES = 4;
+enum { CN_S = 1, CN_P0 = 2 };
 // build-time tuning: rows in flight per wavefront, wavefronts per launch
 #ifndef IMPEC_CN_PF
 #define IMPEC_CN_PF 4
+#endif
+#ifndef IMPEC_CNP_PF
+#define IMPEC_CNP_PF 2
 #endif
 #ifndef IMPEC_CN_TARGET_WAVES
 #define IMPEC_CN_TARGET_WAVES 4096
@@ -189,19 +196,66 @@ struct CnArgs {
     CnConst k;
     const double *u, *v;                   // face velocities, ghosted (ghost width k.B, pitch ldv)
     const float *dinv;                     // rounded inverse of the folded diagonal (flat N1 x N2)
-    const double *r;                       // CN_S, CN_P0: the residual
-    const double *s, *t;                   // CN_P: s and t of the previous iteration
+    const double *r;                       // the residual
     const double *pin, *vin, *r0;          // flat N1 x N2
-    double *x, *rout;                      // CN_P: the solution (in place) and the new residual
+    double *x, *rout;                      // p-step: the solution (in place) and the new residual
     long ldx;                              // row pitch of x (the interior of the ghosted c)
-    double *pout;                          // CN_P, CN_P0: new p; CN_S: s
-    double *out;                           // CN_P, CN_P0: v; CN_S: t
+    double *pout;                          // p-step, CN_P0: new p
+    double *out;                           // p-step, CN_P0: new v
     const double *S;
     double *part;                          // per-wave partials
     int R, nstrip, nwave;
-    int xzero;                             // CN_P: x is zero (the first update of a zero-guess solve)
+    int xzero;                             // p-step: x is zero (the first update of a zero-guess solve)
 };
 
+// the folded 9-point operator of row q (coefficients a) applied at column j
+// to y, whose rows q-1, q, q+1 are Y0, Y1, Y2 (column neighbours from the
+// adjacent lanes; every lane of the wave executes it)
+__device__ __forceinline__ double cn_apply(const double a[9], double Y0, double Y1, double Y2, int q, int j, int N1,
+                                           int N2) {
+    const double yl0 = lane_prev(Y0), yl1 = lane_prev(Y1), yl2 = lane_prev(Y2);
+    const double yr0 = lane_next(Y0), yr1 = lane_next(Y1), yr2 = lane_next(Y2);
+    const bool we = q > 0, ea = q < N1 - 1, so = j > 0, no = j < N2 - 1;
+    double acc = a[ST_C] * Y1;
+    if (we) acc += a[ST_W] * Y0;
+    if (we && so) acc += a[ST_SW] * yl0;
+    if (so) acc += a[ST_S] * yl1;
+    if (ea && so) acc += a[ST_SE] * yl2;
+    if (ea) acc += a[ST_E] * Y2;
+    if (ea && no) acc += a[ST_NE] * yr2;
+    if (no) acc += a[ST_N] * yr1;
+    if (we && no) acc += a[ST_NW] * yr0;
+    return acc;
+}
+
+// the face tensors streamed down a strip: push(row m) forms the x-face tensors
+// of face m (from u and v of rows m-1 and m) and the y-face tensors of row m-1
+// (from u of rows m-1 and m), the four-point averages of cell_tensors in its
+// summation order; cell(...) then gives row m-1's eight tensors
+struct FaceStream {
+    double uprev, vprev, uprev_l, vprev_n;   // u, v of row m-1 and their lane shifts
+    double f11a = 0., f12a = 0., f11b = 0., f12b = 0.;   // x-faces m-1 and m
+    double g22 = 0., g21 = 0.;                           // y-faces of row m-1
+    __device__ __forceinline__ void start(double u_, double v_) {
+        uprev = u_; vprev = v_; uprev_l = lane_prev(u_); vprev_n = lane_next(v_);
+    }
+    __device__ __forceinline__ void push(const CnConst &k, double um, double vm) {
+        const double um_l = lane_prev(um), vm_n = lane_next(vm);
+        double e11, e12;
+        const double vb = 1. / 4 * (vprev_n + vm_n + vprev + vm);
+        xface_tensor(k, um, vb, e11, e12);
+        const double ub = 1. / 4 * (um_l + um + uprev_l + uprev);
+        yface_tensor(k, ub, vprev, g22, g21);
+        uprev = um; vprev = vm; uprev_l = um_l; vprev_n = vm_n;
+        f11a = f11b; f12a = f12b; f11b = e11; f12b = e12;
+    }
+    __device__ __forceinline__ CellTensors cell() const {
+        return CellTensors{f11a, f12a, f11b, f12b, g22, g21, lane_next(g22), lane_next(g21)};
+    }
+};
+
+// CN_S (s = r - alpha v, t = A (s / D), partials only) and CN_P0 (p = r,
+// v = A (p / D)); lanes 1..62 of a strip are written
 template <int MODE>
 __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
     const int lane = threadIdx.x & 63;
@@ -216,22 +270,11 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
     const int jc = j < 0 ? 0 : (j > N2 - 1 ? N2 - 1 : j);   // cell column for loads
     const bool outl = lane >= 1 && lane <= CN_OUT && jv;
     const int i0 = chunk * A.R, i1 = i0 + A.R < N1 ? i0 + A.R : N1;
-    double beta = 0., om = 0., alpha = 0.;
-    if (MODE == CN_P) {
-        beta = (A.S[S_RHO_NEW] / A.S[S_RHO]) * (A.S[S_ALPHA] / A.S[S_OMEGA]);
-        om = A.S[S_OMEGA];
-        alpha = A.S[S_ALPHA];
-    } else if (MODE == CN_S) {
-        alpha = A.S[S_ALPHA];
-    }
+    const double alpha = MODE == CN_S ? A.S[S_ALPHA] : 0.;
     // raw row loads from clamped addresses (no lane-dependent branches)
     auto cell = [&](const double *p, int m) -> double {
         const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
         return p[(long)mc * N2 + jc];
-    };
-    auto cellx = [&](const double *p, int m) -> double {   // x: the interior of c, pitch ldx
-        const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
-        return p[(long)mc * A.ldx + jc];
     };
     auto cellf = [&](const float *p, int m) -> float {
         const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
@@ -244,37 +287,28 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
         const int mc = m < -B ? -B : (m > N1 + B ? N1 + B : m);
         return p[(long)(mc + B) * L + (jg + B)];
     };
-    // ring of prefetched rows: the cell vectors of row m (CN_P: s, t, p, v, x;
-    // CN_S: r, v; CN_P0: r), 1/D, r0 and the face velocities u, v of row m
+    // ring of prefetched rows: r and (CN_S) v of row m, 1/D, r0 and the face
+    // velocities u, v of row m
     constexpr int PF = IMPEC_CN_PF;   // rows in flight; a slot is refilled at the end of the step that used it
-    double Pa[PF] = {}, Pb[PF] = {}, Pp[PF] = {}, Pv[PF] = {}, Px[PF] = {}, P0[PF] = {}, Pu[PF] = {}, Pw[PF] = {};
+    double Pa[PF] = {}, Pv[PF] = {}, P0[PF] = {}, Pu[PF] = {}, Pw[PF] = {};
     float Pd[PF] = {};
     auto fetch = [&](int u, int m) {
-        if (MODE == CN_P) {
-            Pa[u] = cell(A.s, m);
-            Pb[u] = cell(A.t, m);
-            Pp[u] = cell(A.pin, m);
-            if (!A.xzero) Px[u] = cellx(A.x, m);
-        } else {
-            Pa[u] = cell(A.r, m);
-        }
-        if (MODE != CN_P0) Pv[u] = cell(A.vin, m);
+        Pa[u] = cell(A.r, m);
+        if (MODE == CN_S) Pv[u] = cell(A.vin, m);
         Pd[u] = cellf(A.dinv, m);
         P0[u] = cell(A.r0, m);
         Pu[u] = vel(A.u, m);
         Pw[u] = vel(A.v, m);
     };
-    const int mstart = i0 - 1, mend = i1;
-    const int mstop = mstart + (mend - mstart + PF) / PF * PF;
+    const int mstart = i0 - 1, mend = i1 + 1;
+    const int mstop = mstart + (mend - mstart + PF - 1) / PF * PF;
 #pragma unroll
     for (int u = 0; u < PF; u++) fetch(u, mstart + u);
-    // windows: input vector y of rows m-2, m-1, m; x-faces of rows m-1, m;
-    // y-faces of row m-1; the written vector (p or s) and r0 of row m-1
+    // windows: the operator's input y of rows m-2, m-1, m; the vector of row
+    // m-1 (s or p) and r0 of row m-1
     double Y0 = 0., Y1 = 0., Y2 = 0.;
-    double f11a = 0., f12a = 0., f11b = 0., f12b = 0.;
-    // u and v of the previous row, and their lane shifts
-    double uprev = vel(A.u, mstart - 1), vprev = vel(A.v, mstart - 1);
-    double uprev_l = lane_prev(uprev), vprev_n = lane_next(vprev);
+    FaceStream fs;
+    fs.start(vel(A.u, mstart - 1), vel(A.v, mstart - 1));
     double wprev = 0., r0prev = 0.;
     double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
     for (int mb = mstart; mb < mstop; mb += PF) {
@@ -282,70 +316,33 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
         for (int u = 0; u < PF; u++) {
             const int m = mb + u;
             const double dv = (double)Pd[u], r0v = P0[u];
-            // x-face tensors of row m (e11, e12) and y-face tensors of row m - 1 (g22, g21)
-            double e11, e12, g22, g21;
-            {
-                const double um = Pu[u], vm = Pw[u];
-                const double um_l = lane_prev(um), vm_n = lane_next(vm);
-                // the four-point averages of cell_tensors, in its summation order
-                const double vb = 1. / 4 * (vprev_n + vm_n + vprev + vm);
-                xface_tensor(k, um, vb, e11, e12);
-                const double ub = 1. / 4 * (um_l + um + uprev_l + uprev);
-                yface_tensor(k, ub, vprev, g22, g21);
-                uprev = um; vprev = vm; uprev_l = um_l; vprev_n = vm_n;
-            }
+            fs.push(k, Pu[u], Pw[u]);
             const bool rin = m >= 0 && m < N1;
-            // the written vector on row m (p or s) and, for the p-launches, the residual
-            double wv, rv = 0.;
-            if (MODE == CN_P) {
-                rv = Pa[u] - om * Pb[u];
-                wv = rv + beta * (Pp[u] - om * Pv[u]);
-            } else if (MODE == CN_S) {
-                wv = Pa[u] - alpha * Pv[u];
-            } else {
-                rv = Pa[u];
-                wv = rv;
-            }
+            // the vector on row m: s, or the first direction p = r
+            const double wv = MODE == CN_S ? Pa[u] - alpha * Pv[u] : Pa[u];
             const double yv = (rin && jv) ? wv * dv : 0.;
             if (rin && m >= i0 && m < i1 && outl) {
-                const long o = (long)m * N2 + j;
-                A.pout[o] = wv;
-                if (MODE == CN_P) {
-                    A.rout[o] = rv;
-                    const double dx_ = alpha * (Pp[u] * dv) + om * (Pa[u] * dv);
-                    A.x[(long)m * A.ldx + j] = A.xzero ? dx_ : Px[u] + dx_;
-                }
                 if (MODE == CN_S) s3 += r0v * wv;
-                else s1 += rv * rv;
+                else {
+                    A.pout[(long)m * N2 + j] = wv;
+                    s1 += wv * wv;
+                }
             }
             Y0 = Y1; Y1 = Y2; Y2 = yv;
-            f11a = f11b; f12a = f12b; f11b = e11; f12b = e12;
             // row q = m - 1: GetCoefficients_Parabolic with the faces in registers
             const int q = m - 1;
-            const double d22p = lane_next(g22), d21p = lane_next(g21);
+            const CellTensors T = fs.cell();
             if (q >= i0 && q < i1) {
                 double a[9];
-                const CellTensors T = {f11a, f12a, f11b, f12b, g22, g21, d22p, d21p};
                 cn_coeffs(k, T, q, j, a);
-                const double yl0 = lane_prev(Y0), yl1 = lane_prev(Y1), yl2 = lane_prev(Y2);
-                const double yr0 = lane_next(Y0), yr1 = lane_next(Y1), yr2 = lane_next(Y2);
-                const bool we = q > 0, ea = q < N1 - 1, so = j > 0, no = j < N2 - 1;
-                double acc = a[ST_C] * Y1;
-                if (we) acc += a[ST_W] * Y0;
-                if (we && so) acc += a[ST_SW] * yl0;
-                if (so) acc += a[ST_S] * yl1;
-                if (ea && so) acc += a[ST_SE] * yl2;
-                if (ea) acc += a[ST_E] * Y2;
-                if (ea && no) acc += a[ST_NE] * yr2;
-                if (no) acc += a[ST_N] * yr1;
-                if (we && no) acc += a[ST_NW] * yr0;
+                const double acc = cn_apply(a, Y0, Y1, Y2, q, j, N1, N2);
                 if (outl) {
-                    A.out[(long)q * N2 + j] = acc;
                     if (MODE == CN_S) {
                         s0 += acc * wprev;
                         s1 += acc * acc;
                         s2 += r0prev * acc;
                     } else {
+                        A.out[(long)q * N2 + j] = acc;
                         s0 += r0prev * acc;
                     }
                 }
@@ -371,6 +368,125 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
             A.part[2 * RED_BLOCKS + wid] = s2;
             A.part[3 * RED_BLOCKS + wid] = s3;
         }
+    }
+}
+
+// The p-launch: row m streams in r, v, 1/D and the face velocities; s and
+// y_s = s/D are formed on row m, t = A y_s, r', p' and x on row m-1, and
+// v' = A (p'/D) on row m-2.  Two chained operators need a halo of two rows
+// and two lanes: lanes 2..61 of a strip are written, rows i0-2 .. i1+1 are
+// streamed.  r' goes to a buffer other than r (neighbouring waves read r in
+// their halos); x is read and written by its own lane only.
+__global__ void __launch_bounds__(64 * CN_WAVES) k_cn_pstep(CnArgs A) {
+    const int lane = threadIdx.x & 63;
+    const int wid = xcd_block(blockIdx.x, gridDim.x) * CN_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wid >= A.nwave) return;
+    const CnConst &k = A.k;
+    const int N1 = k.N1, N2 = k.N2;
+    const long L = k.ldv;
+    const int strip = wid % A.nstrip, chunk = wid / A.nstrip;
+    const int j = strip * CNP_OUT - 2 + lane;
+    const bool jv = j >= 0 && j < N2;
+    const int jc = j < 0 ? 0 : (j > N2 - 1 ? N2 - 1 : j);
+    const bool outl = lane >= 2 && lane < 2 + CNP_OUT && jv;
+    const int i0 = chunk * A.R, i1 = i0 + A.R < N1 ? i0 + A.R : N1;
+    const double beta = (A.S[S_RHO_NEW] / A.S[S_RHO]) * (A.S[S_ALPHA] / A.S[S_OMEGA]);
+    const double om = A.S[S_OMEGA], alpha = A.S[S_ALPHA];
+    auto cell = [&](const double *p, int m) -> double {
+        const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
+        return p[(long)mc * N2 + jc];
+    };
+    auto cellx = [&](const double *p, int m) -> double {   // x: the interior of c, pitch ldx
+        const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
+        return p[(long)mc * A.ldx + jc];
+    };
+    auto cellf = [&](const float *p, int m) -> float {
+        const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
+        return p[(long)mc * N2 + jc];
+    };
+    const int B = k.B;
+    const int jg = j < -1 ? -1 : (j > N2 ? N2 : j);
+    auto vel = [&](const double *p, int m) -> double {
+        const int mc = m < -B ? -B : (m > N1 + B ? N1 + B : m);
+        return p[(long)(mc + B) * L + (jg + B)];
+    };
+    // ring: r, v, 1/D, u, v of row m; p and x of row m-1; r0 of row m-2
+    constexpr int PF = IMPEC_CNP_PF;
+    double Pr[PF] = {}, Pv[PF] = {}, Pu[PF] = {}, Pw[PF] = {}, Pp[PF] = {}, Px[PF] = {}, P0[PF] = {};
+    float Pd[PF] = {};
+    auto fetch = [&](int u, int m) {
+        Pr[u] = cell(A.r, m);
+        Pv[u] = cell(A.vin, m);
+        Pd[u] = cellf(A.dinv, m);
+        Pu[u] = vel(A.u, m);
+        Pw[u] = vel(A.v, m);
+        Pp[u] = cell(A.pin, m - 1);
+        if (!A.xzero) Px[u] = cellx(A.x, m - 1);
+        P0[u] = cell(A.r0, m - 2);
+    };
+    const int mstart = i0 - 2, mend = i1 + 2;
+    const int mstop = mstart + (mend - mstart + PF - 1) / PF * PF;
+#pragma unroll
+    for (int u = 0; u < PF; u++) fetch(u, mstart + u);
+    double YS0 = 0., YS1 = 0., YS2 = 0.;   // s/D of rows m-2, m-1, m
+    double YP0 = 0., YP1 = 0., YP2 = 0.;   // p'/D of rows m-3, m-2, m-1
+    double sprev = 0., vprev = 0., dprev = 0.;   // s, v and 1/D of row m-1
+    double aprev[9] = {};                  // coefficients of row m-2
+    FaceStream fs;
+    fs.start(vel(A.u, mstart - 1), vel(A.v, mstart - 1));
+    double s0 = 0., s1 = 0.;
+    for (int mb = mstart; mb < mstop; mb += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int m = mb + u;
+            fs.push(k, Pu[u], Pw[u]);
+            // row m: s = r - alpha v and s/D (the s-launch's arithmetic)
+            const bool rin = m >= 0 && m < N1;
+            const double dv = (double)Pd[u];
+            const double sm = Pr[u] - alpha * Pv[u];
+            YS0 = YS1; YS1 = YS2; YS2 = (rin && jv) ? sm * dv : 0.;
+            // row q = m - 1: t, r', p', x
+            const int q = m - 1;
+            const bool qin = q >= 0 && q < N1;
+            double a[9];
+            cn_coeffs(k, fs.cell(), q, j, a);
+            const double t = cn_apply(a, YS0, YS1, YS2, q, j, N1, N2);
+            const double rq = sprev - om * t;
+            const double pq = rq + beta * (Pp[u] - om * vprev);
+            if (qin && q >= i0 && q < i1 && outl) {
+                const long o = (long)q * N2 + j;
+                A.pout[o] = pq;
+                A.rout[o] = rq;
+                const double dx_ = alpha * (Pp[u] * dprev) + om * (sprev * dprev);
+                A.x[(long)q * A.ldx + j] = A.xzero ? dx_ : Px[u] + dx_;
+                s1 += rq * rq;
+            }
+            YP0 = YP1; YP1 = YP2; YP2 = (qin && jv) ? pq * dprev : 0.;
+            // row q2 = m - 2: v' = A (p'/D)
+            const int q2 = m - 2;
+            if (q2 >= i0 && q2 < i1) {
+                const double acc = cn_apply(aprev, YP0, YP1, YP2, q2, j, N1, N2);
+                if (outl) {
+                    A.out[(long)q2 * N2 + j] = acc;
+                    s0 += P0[u] * acc;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 9; e++) aprev[e] = a[e];
+            sprev = sm;
+            vprev = Pv[u];
+            dprev = dv;
+            fetch(u, m + PF);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+    }
+    if (lane == 0) {
+        A.part[wid] = s0;
+        A.part[RED_BLOCKS + wid] = s1;
     }
 }
 static_assert(4 <= PART_VECS, "the s-launch writes four partial vectors");
@@ -540,9 +656,8 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     hipStream_t s = ctx->stream;
     // the solution is formed in place in the interior of c (x0 = c for cn_guess = 1)
     double *b = ctx->cn_rhs, *x = ctx->c + (long)ctx->B * ctx->lds + ctx->B, *r = ctx->cn_r, *r0 = ctx->cn_r0;
-    double *sv = ctx->cn_s, *t = ctx->cn_t;
-    // p, v ping-pong: a strip reads its neighbours' old values at the halo
-    double *pb[2] = {ctx->cn_p, ctx->cn_y}, *vb[2] = {ctx->cn_v, ctx->cn_z};
+    // r, p, v ping-pong: a strip reads its neighbours' old values at the halo
+    double *pb[2] = {ctx->cn_p, ctx->cn_y}, *vb[2] = {ctx->cn_v, ctx->cn_z}, *rb[2] = {ctx->cn_r, ctx->cn_r2};
     prof_begin(ctx);
     int nrb = 0;
     int st = tensors_and_rhs(ctx, k, &nrb);
@@ -585,8 +700,17 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         a.R = (int)R;
     }
     a.nwave = a.nstrip * ((N1 + a.R - 1) / a.R);
-    if (a.nwave > RED_BLOCKS) return set_error(ctx, IMPEC_EINVAL, "dispersion solve: grid too large");
+    // the p-step's strips of CNP_OUT columns, chunks of the same rows
+    CnArgs a2 = a;
+    a2.nstrip = (N2 + CNP_OUT - 1) / CNP_OUT;
+    a2.nwave = a2.nstrip * ((N1 + a.R - 1) / a.R);
+    if (a.nwave > RED_BLOCKS || a2.nwave > RED_BLOCKS)
+        return set_error(ctx, IMPEC_EINVAL, "dispersion solve: grid too large");
     const dim3 sg((a.nwave + CN_WAVES - 1) / CN_WAVES), sblk(64 * CN_WAVES);
+    const dim3 pg((a2.nwave + CN_WAVES - 1) / CN_WAVES);
+    // the current residual: b before the first p-step of a zero-guess solve
+    const double *rcur = a.r;
+    int rnext = 0;
     int cur = 0, its = 0, status = IMPEC_OK;
     if (!(rr <= tol * tol * bb) && rr == rr) {   // the first direction p = r
         CnArgs ap = a;
@@ -602,20 +726,24 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         if (its >= maxit) { status = IMPEC_ENOCONV; break; }
         if (!(rr == rr)) { status = IMPEC_ENAN; break; }
         CnArgs as = a;
-        as.r = its == 0 ? a.r : r;   // b before the first p-launch of a zero-guess solve
-        as.vin = vb[cur]; as.pout = sv; as.out = t;
+        as.r = rcur;
+        as.vin = vb[cur];
         prof_begin(ctx);
         hipLaunchKernelGGL(k_cn_stream<CN_S>, sg, sblk, 0, s, as);
         prof_end(ctx, "cn_apply_s");
         hipLaunchKernelGGL(k_bicg_finish<4>, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 2, ctx->scal);
-        CnArgs ap = a;
-        ap.s = sv; ap.t = t; ap.pin = pb[cur]; ap.vin = vb[cur]; ap.x = x; ap.rout = r;
+        CnArgs ap = a2;
+        ap.r = rcur; ap.pin = pb[cur]; ap.vin = vb[cur]; ap.x = x;
+        if (rb[rnext] == rcur) rnext ^= 1;   // r' never overwrites the r it is formed from
+        ap.rout = rb[rnext];
         ap.xzero = zero && its == 0;
         ap.pout = pb[cur ^ 1]; ap.out = vb[cur ^ 1];
         prof_begin(ctx);
-        hipLaunchKernelGGL(k_cn_stream<CN_P>, sg, sblk, 0, s, ap);
+        hipLaunchKernelGGL(k_cn_pstep, pg, sblk, 0, s, ap);
         prof_end(ctx, "cn_apply_p");
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 1, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, a2.nwave, 1, ctx->scal);
+        rcur = rb[rnext];
+        rnext ^= 1;
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = fetch(ctx))) return st;
         rr = ctx->h_scal[S_RR];

@@ -222,7 +222,7 @@ __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int
 // ===========================================================================
 constexpr int PS_OUT = 60, PS_WAVES = 4;
 #ifndef IMPEC_PS_PF
-#define IMPEC_PS_PF 4
+#define IMPEC_PS_PF 2
 #endif
 #ifndef IMPEC_PS_TARGET_WAVES
 #define IMPEC_PS_TARGET_WAVES 4096
@@ -235,27 +235,49 @@ struct ParArgs {
     int R, nstrip, nwave;       // output rows per chunk, strips, wavefronts
 };
 
-// the update of one cell by a sweep along its line: values and faces of the
-// line at offsets -2..+2 and -1..+2 (the arithmetic of k_sweep_x / k_sweep_y)
+// Y sweep of one row held across the lanes (column j in lane j), with the
+// arithmetic of k_sweep_y: each cell's half-step states and each face's flux
+// are formed once, in the lane of the cell (of the face's right cell), and
+// reach the neighbours by lane shifts.  Cell j's update needs lanes j-2..j+2.
 template <bool MUSCL, bool PHI1>
-__device__ __forceinline__ double line_update(const SweepConst &k, double cm2, double cm1, double c0, double cp1,
-                                              double cp2, double um1, double u0, double up1, double up2) {
-    double Fi, Fip1;
+__device__ __forceinline__ double ysweep_lanes(const SweepConst &k, double c, double v) {
+    const double cl = lane_prev(c);
+    double G;   // flux through the face left of the lane's cell
     if (MUSCL) {
-        double lA, rA, lB, rB, lC, rC;
-        hancock<PHI1>(cm2, cm1, c0, um1, u0, k, lA, rA);   // cell -1
-        hancock<PHI1>(cm1, c0, cp1, u0, up1, k, lB, rB);   // cell 0
-        hancock<PHI1>(c0, cp1, cp2, up1, up2, k, lC, rC);  // cell +1
-        (void)rA; (void)lC; (void)cm2; (void)up2;
-        Fi = upwind<PHI1>(u0, lA, rB, k.phi);
-        Fip1 = upwind<PHI1>(up1, lB, rC, k.phi);
+        double lo, ro;
+        hancock<PHI1>(cl, c, lane_next(c), v, lane_next(v), k, lo, ro);
+        G = upwind<PHI1>(v, lane_prev(lo), ro, k.phi);
     } else {
-        (void)cm2; (void)cp2; (void)um1; (void)up2;
-        Fi = upwind<PHI1>(u0, cm1, c0, k.phi);
-        Fip1 = upwind<PHI1>(up1, c0, cp1, k.phi);
+        G = upwind<PHI1>(v, cl, c, k.phi);
     }
-    return c0 + k.dtdx * (Fi - Fip1);
+    return c + k.dtdx * (G - lane_next(G));
 }
+
+// X sweep of a lane's column as rows stream past, with the arithmetic of
+// k_sweep_x: push(row k, faces k-1 and k) forms the half-step states of row
+// k-1 and the flux through face k-1, and returns the update of row k-2.
+// After push(row k) the fields hold:
+struct XStream {
+    double cm2 = 0., cm1 = 0., c0 = 0.;   // rows k-2, k-1, k
+    double lo = 0.;                       // left state of face k (from row k-1)
+    double F = 0.;                        // flux through face k-1
+    template <bool MUSCL, bool PHI1>
+    __device__ __forceinline__ double push(const SweepConst &k, double ck, double ukm1, double uk) {
+        cm2 = cm1; cm1 = c0; c0 = ck;
+        double Fn;   // flux through face k-1
+        if (MUSCL) {
+            double lo_n, ro_n;
+            hancock<PHI1>(cm2, cm1, c0, ukm1, uk, k, lo_n, ro_n);
+            Fn = upwind<PHI1>(ukm1, lo, ro_n, k.phi);
+            lo = lo_n;
+        } else {
+            Fn = upwind<PHI1>(ukm1, cm2, cm1, k.phi);
+        }
+        const double out = cm2 + k.dtdx * (F - Fn);
+        F = Fn;
+        return out;
+    }
+};
 
 template <bool MUSCL, bool PHI1>
 __global__ void __launch_bounds__(64 * PS_WAVES) k_advect_parallel(ParArgs A) {
@@ -287,50 +309,33 @@ __global__ void __launch_bounds__(64 * PS_WAVES) k_advect_parallel(ParArgs A) {
         Pu[s_] = fload(A.u, m);
         Pv[s_] = fload(A.v, m);
     };
+    // output row i = k - 2 needs rows i-2 .. i+2 of the state and faces i-1 .. i+2
     const int kstart = i0 - 2, kend = i1 + 2;
     const int kstop = kstart + (kend - kstart + PF - 1) / PF * PF;
 #pragma unroll
     for (int s_ = 0; s_ < PF; s_++) fetch(s_, kstart + s_);
-    // windows: c and its Y-swept rows k-4 .. k, u rows k-3 .. k, v rows k-2 .. k
-    double C[5] = {}, C3[5] = {}, U[4] = {}, V[3] = {};
+    XStream xy, yx;                       // X sweeps of the state (XY) and of its Y sweep (YX)
+    double ukm1 = 0., vkm1 = 0., vkm2 = 0.;
     for (int kb = kstart; kb < kstop; kb += PF) {
 #pragma unroll
         for (int s_ = 0; s_ < PF; s_++) {
             const int k = kb + s_;
             const double ck = Pc[s_], uk = Pu[s_], vk = Pv[s_];
-            // YX: the Y sweep of row k (every row; columns outside [0, N2) are copied)
-            double c3k = ck;
-            {
-                const double l1 = lane_prev(ck), l2 = lane_prev(l1), r1 = lane_next(ck), r2 = lane_next(r1);
-                const double vl = lane_prev(vk), vr1 = lane_next(vk), vr2 = lane_next(vr1);
-                if (ycol) c3k = line_update<MUSCL, PHI1>(ky, l2, l1, ck, r1, r2, vl, vk, vr1, vr2);
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                C[q] = C[q + 1];
-                C3[q] = C3[q + 1];
-            }
-            C[4] = ck;
-            C3[4] = c3k;
-            U[0] = U[1]; U[1] = U[2]; U[2] = U[3]; U[3] = uk;
-            V[0] = V[1]; V[1] = V[2]; V[2] = vk;
-            // row i = k - 2: the X sweeps (rows [0, N1); other rows are copied)
             const int i = k - 2;
-            const bool xrow = i >= 0 && i < N1;
-            double c1 = C[2], c4 = C3[2];
-            if (xrow) {
-                c1 = line_update<MUSCL, PHI1>(kx, C[0], C[1], C[2], C[3], C[4], U[0], U[1], U[2], U[3]);
-                c4 = line_update<MUSCL, PHI1>(kx, C3[0], C3[1], C3[2], C3[3], C3[4], U[0], U[1], U[2], U[3]);
-            }
-            // XY: the Y sweep of the X-swept row i
-            double c2 = c1;
-            {
-                const double vi = V[0];
-                const double l1 = lane_prev(c1), l2 = lane_prev(l1), r1 = lane_next(c1), r2 = lane_next(r1);
-                const double vl = lane_prev(vi), vr1 = lane_next(vi), vr2 = lane_next(vr1);
-                if (ycol) c2 = line_update<MUSCL, PHI1>(ky, l2, l1, c1, r1, r2, vl, vi, vr1, vr2);
-            }
+            const bool xrow = i >= 0 && i < N1;                  // X sweeps update this row
+            // XY: X sweep of the state, then Y sweep of row i
+            const double x1 = xy.push<MUSCL, PHI1>(kx, ck, ukm1, uk);
+            const double c1 = xrow ? x1 : xy.cm2;
+            const double y2 = ysweep_lanes<MUSCL, PHI1>(ky, c1, vkm2);
+            const double c2 = ycol ? y2 : c1;
+            // YX: Y sweep of row k, then X sweep of row i
+            const double y3 = ysweep_lanes<MUSCL, PHI1>(ky, ck, vk);
+            const double x4 = yx.push<MUSCL, PHI1>(kx, ycol ? y3 : ck, ukm1, uk);
+            const double c4 = xrow ? x4 : yx.cm2;
             if (outl && i >= i0 && i < i1) A.out[(long)(i + B) * LS + (j + B)] = 0.5 * (c2 + c4);
+            ukm1 = uk;
+            vkm2 = vkm1;
+            vkm1 = vk;
             fetch(s_, k + PF);
         }
     }
