@@ -113,10 +113,24 @@ __device__ __forceinline__ void cn_coeffs(const CnConst &k, const CellTensors &T
     cn_fold(i, j, k.N1, k.N2, a);
 }
 
-// SetRHS_Parabolic (MGSolver.cpp:1324-1370) and the Jacobi preconditioner;
-// c, u, v ghost-filled.  64 x 4 threads per workgroup over (column, row), rows
-// strided; per-workgroup partials of b.b (twice: ||b||^2 and, for the zero
-// initial guess, r0.r) for the solver's start
+// SetRHS_Parabolic (MGSolver.cpp:1324-1370) at one cell: c at row offsets
+// m/0/p (-1, 0, +1) and column offsets m/0/p, in the reference's grouping
+__device__ __forceinline__ double cn_rhs_at(const CnConst &k, const CellTensors &T, double cmm, double cm0,
+                                            double cmp, double c0m, double c00, double c0p, double cpm, double cp0,
+                                            double cpp) {
+    return c00 + k.dt8 * (
+        k.c4dx2 * (T.d11p * (cp0 - c00) - T.d11m * (c00 - cm0))
+        + k.c1dxdy * (T.d12p * (c0p + cpp - c0m - cpm)
+                      - T.d12m * (c0p + cmp - cmm - c0m))
+        + k.c1dxdy * (T.d21p * (cp0 + cpp - cmp - cm0)
+                      - T.d21m * (cp0 + cpm - cm0 - cmm))
+        + k.c4dy2 * (T.d22p * (c0p - c00) - T.d22m * (c00 - c0m)));
+}
+
+// The right-hand side and the Jacobi preconditioner; c, u, v ghost-filled.
+// 64 x 4 threads per workgroup over (column, row), rows strided;
+// per-workgroup partials of b.b (twice: ||b||^2 and rho) for the solver's
+// start
 __global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ u,
                          const double *__restrict__ v, double *__restrict__ rhs, float *__restrict__ diag,
                          double *__restrict__ part) {
@@ -128,13 +142,8 @@ __global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const
             const CellTensors T = cell_tensors(k, u, v, i, j);
             const double *cc = c + (long)(i + k.B) * lds + (j + k.B);
 #define C_(a, b) cc[(long)(a) * lds + (b)]
-            const double r = C_(0, 0) + k.dt8 * (
-                k.c4dx2 * (T.d11p * (C_(1, 0) - C_(0, 0)) - T.d11m * (C_(0, 0) - C_(-1, 0)))
-                + k.c1dxdy * (T.d12p * (C_(0, 1) + C_(1, 1) - C_(0, -1) - C_(1, -1))
-                              - T.d12m * (C_(0, 1) + C_(-1, 1) - C_(-1, -1) - C_(0, -1)))
-                + k.c1dxdy * (T.d21p * (C_(1, 0) + C_(1, 1) - C_(-1, 1) - C_(-1, 0))
-                              - T.d21m * (C_(1, 0) + C_(1, -1) - C_(-1, 0) - C_(-1, -1)))
-                + k.c4dy2 * (T.d22p * (C_(0, 1) - C_(0, 0)) - T.d22m * (C_(0, 0) - C_(0, -1))));
+            const double r = cn_rhs_at(k, T, C_(-1, -1), C_(-1, 0), C_(-1, 1), C_(0, -1), C_(0, 0), C_(0, 1),
+                                       C_(1, -1), C_(1, 0), C_(1, 1));
 #undef C_
             rhs[(long)i * k.N2 + j] = r;
             double a[9];
@@ -489,6 +498,101 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_pstep(CnArgs A) {
         A.part[RED_BLOCKS + wid] = s1;
     }
 }
+// The start of a zero-guess solve in one pass (x0 = 0, so r = r0 = p = b):
+// row m streams in c and the face velocities; the right-hand side b and the
+// rounded 1/D are formed on row m-1 (k_cn_rhs's arithmetic), the first
+// direction's v = A (b/D) on row m-2 (CN_P0's).  Writes b, 1/D and v;
+// partials b.v and b.b.  As in the p-launch, lanes 2..61 of a strip and
+// rows i0-2 .. i1+1 are streamed for the two chained stencils.
+__global__ void __launch_bounds__(64 * CN_WAVES) k_cn_start(CnArgs A, const double *__restrict__ c, int lds,
+                                                            double *__restrict__ b, float *__restrict__ dinv) {
+    const int lane = threadIdx.x & 63;
+    const int wid = xcd_block(blockIdx.x, gridDim.x) * CN_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wid >= A.nwave) return;
+    const CnConst &k = A.k;
+    const int N1 = k.N1, N2 = k.N2, B = k.B;
+    const long L = k.ldv;
+    const int strip = wid % A.nstrip, chunk = wid / A.nstrip;
+    const int j = strip * CNP_OUT - 2 + lane;
+    const bool jv = j >= 0 && j < N2;
+    const bool outl = lane >= 2 && lane < 2 + CNP_OUT && jv;
+    const int i0 = chunk * A.R, i1 = i0 + A.R < N1 ? i0 + A.R : N1;
+    const int jcg = j < -B ? -B : (j > N2 + B - 1 ? N2 + B - 1 : j);   // ghosted c column for loads
+    auto crow = [&](int m) -> double {
+        const int mc = m < -B ? -B : (m > N1 + B - 1 ? N1 + B - 1 : m);
+        return c[(long)(mc + B) * lds + (jcg + B)];
+    };
+    const int jg = j < -1 ? -1 : (j > N2 ? N2 : j);
+    auto vel = [&](const double *p, int m) -> double {
+        const int mc = m < -B ? -B : (m > N1 + B ? N1 + B : m);
+        return p[(long)(mc + B) * L + (jg + B)];
+    };
+    constexpr int PF = IMPEC_CNP_PF;
+    double Pc[PF] = {}, Pu[PF] = {}, Pw[PF] = {};
+    auto fetch = [&](int u, int m) {
+        Pc[u] = crow(m);
+        Pu[u] = vel(A.u, m);
+        Pw[u] = vel(A.v, m);
+    };
+    const int mstart = i0 - 2, mend = i1 + 2;
+    const int mstop = mstart + (mend - mstart + PF - 1) / PF * PF;
+#pragma unroll
+    for (int u = 0; u < PF; u++) fetch(u, mstart + u);
+    double C0 = 0., C1 = 0., C2 = 0.;      // c of rows m-2, m-1, m
+    double Y0 = 0., Y1 = 0., Y2 = 0.;      // b/D of rows m-3, m-2, m-1
+    double bprev = 0.;                     // b of row m-2
+    double aprev[9] = {};                  // coefficients of row m-2
+    FaceStream fs;
+    fs.start(vel(A.u, mstart - 1), vel(A.v, mstart - 1));
+    double s0 = 0., s1 = 0.;
+    for (int mb = mstart; mb < mstop; mb += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int m = mb + u;
+            fs.push(k, Pu[u], Pw[u]);
+            C0 = C1; C1 = C2; C2 = Pc[u];
+            // row q = m - 1: b and 1/D
+            const int q = m - 1;
+            const bool qin = q >= 0 && q < N1;
+            const CellTensors T = fs.cell();
+            const double bq = cn_rhs_at(k, T, lane_prev(C0), C0, lane_next(C0), lane_prev(C1), C1, lane_next(C1),
+                                        lane_prev(C2), C2, lane_next(C2));
+            double a[9];
+            cn_coeffs(k, T, q, j, a);
+            const float dq = (float)(1. / a[ST_C]);
+            if (qin && q >= i0 && q < i1 && outl) {
+                const long o = (long)q * N2 + j;
+                b[o] = bq;
+                dinv[o] = dq;
+                s1 += bq * bq;
+            }
+            Y0 = Y1; Y1 = Y2; Y2 = (qin && jv) ? bq * (double)dq : 0.;
+            // row q2 = m - 2: v = A (b/D)
+            const int q2 = m - 2;
+            if (q2 >= i0 && q2 < i1) {
+                const double acc = cn_apply(aprev, Y0, Y1, Y2, q2, j, N1, N2);
+                if (outl) {
+                    A.out[(long)q2 * N2 + j] = acc;
+                    s0 += bprev * acc;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 9; e++) aprev[e] = a[e];
+            bprev = bq;
+            fetch(u, m + PF);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+    }
+    if (lane == 0) {
+        A.part[wid] = s0;
+        A.part[RED_BLOCKS + wid] = s1;
+    }
+}
+
 static_assert(4 <= PART_VECS, "the s-launch writes four partial vectors");
 
 // launch shape of the element-wise BiCGStab kernels: 64 x 4 threads over
@@ -550,6 +654,7 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
 // recurrences:
 //  stage 0 (init):     rr = bb = b.b, rho = alpha = omega = 1, rho_new = b.b
 //  stage 4 (guess):    rr = rho_new = r0.r0 of the initial residual
+//  stage 5 (start):    b.v, b.b of a zero-guess start: stages 0 and 1 at once
 //  stage 1 (p-launch): r0.v, r.r; rho = rho_new; alpha = rho / r0.v; rr = r.r
 //  stage 2 (s-launch): t.s, t.t, r0.t, r0.s; omega = t.s / t.t (0 when t = 0);
 //                      rho_new = r0.r of r = s - omega t, as r0.s - omega r0.t
@@ -568,6 +673,14 @@ __global__ void k_bicg_finish(const double *__restrict__ part, int nb, int stage
     } else if (stage == 4) {
         S[S_RR] = v[0];
         S[S_RHO_NEW] = v[1];
+    } else if (stage == 5) {   // a zero-guess start: v[0] = b.v, v[1] = b.b
+        S[S_RR] = v[1];
+        S[S_BB] = v[1];
+        S[S_OMEGA] = 1.;
+        S[S_RHO] = v[1];
+        S[S_RHO_NEW] = v[1];
+        S[S_R0V] = v[0];
+        S[S_ALPHA] = v[1] / v[0];
     } else if (stage == 1) {
         S[S_R0V] = v[0];
         S[S_RHO] = S[S_RHO_NEW];
@@ -658,29 +771,9 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     double *b = ctx->cn_rhs, *x = ctx->c + (long)ctx->B * ctx->lds + ctx->B, *r = ctx->cn_r, *r0 = ctx->cn_r0;
     // r, p, v ping-pong: a strip reads its neighbours' old values at the halo
     double *pb[2] = {ctx->cn_p, ctx->cn_y}, *vb[2] = {ctx->cn_v, ctx->cn_z}, *rb[2] = {ctx->cn_r, ctx->cn_r2};
-    prof_begin(ctx);
-    int nrb = 0;
-    int st = tensors_and_rhs(ctx, k, &nrb);
-    prof_end(ctx, "cn_rhs");
-    if (st) return st;
-    const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
-    const int nvb = (int)(vg.x * vg.y);
-    // zero initial guess: r = r0 = b and x = 0 are not formed; the first
-    // launches read b, r0 stays b, and the first p-launch writes x.  The
-    // b.b partials come from the right-hand side kernel.
     const bool zero = ctx->P.cn_guess != 1;
-    hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nrb, 0, ctx->scal);
-    if (!zero) {   // start from the current concentration
-        hipLaunchKernelGGL(k_bicg_guess, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, r, r0, ctx->part);
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 4, ctx->scal);
-    }
-    IMPEC_HIP_CHECK(ctx, hipGetLastError());
-    if ((st = fetch(ctx))) return st;
-    const double bb = ctx->h_scal[S_BB];
-    double rr = ctx->h_scal[S_RR];
     const double tol = ctx->P.para_eps;
     const int maxit = ctx->P.max_iterations > 0 ? ctx->P.max_iterations : 10000;
-
     CnArgs a;
     memset(&a, 0, sizeof(a));
     a.k = k;
@@ -700,7 +793,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         a.R = (int)R;
     }
     a.nwave = a.nstrip * ((N1 + a.R - 1) / a.R);
-    // the p-step's strips of CNP_OUT columns, chunks of the same rows
+    // the p-launch's strips of CNP_OUT columns, chunks of the same rows
     CnArgs a2 = a;
     a2.nstrip = (N2 + CNP_OUT - 1) / CNP_OUT;
     a2.nwave = a2.nstrip * ((N1 + a.R - 1) / a.R);
@@ -708,20 +801,55 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         return set_error(ctx, IMPEC_EINVAL, "dispersion solve: grid too large");
     const dim3 sg((a.nwave + CN_WAVES - 1) / CN_WAVES), sblk(64 * CN_WAVES);
     const dim3 pg((a2.nwave + CN_WAVES - 1) / CN_WAVES);
-    // the current residual: b before the first p-step of a zero-guess solve
+    int cur = 0, its = 0, status = IMPEC_OK, st;
+    // the first p-launch's p: b for a zero guess, whose first direction is
+    // never stored
+    const double *pfirst = pb[1];
+    double bb, rr;
+    if (zero) {
+        // x0 = 0: r = r0 = p = b, none of them formed; one pass writes b, 1/D
+        // and the first direction's v, and the first p-launch writes x
+        CnArgs az = a2;
+        az.out = vb[1];
+        prof_begin(ctx);
+        hipLaunchKernelGGL(k_cn_start, pg, sblk, 0, s, az, ctx->c, ctx->lds, b, ctx->cn_dg);
+        prof_end(ctx, "cn_start");
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, a2.nwave, 5, ctx->scal);
+        IMPEC_HIP_CHECK(ctx, hipGetLastError());
+        if ((st = fetch(ctx))) return st;
+        bb = ctx->h_scal[S_BB];
+        rr = ctx->h_scal[S_RR];
+        cur = 1;
+        pfirst = b;
+    } else {
+        // x0 = the current concentration
+        prof_begin(ctx);
+        int nrb = 0;
+        if ((st = tensors_and_rhs(ctx, k, &nrb))) return st;
+        prof_end(ctx, "cn_rhs");
+        const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
+        const int nvb = (int)(vg.x * vg.y);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nrb, 0, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_guess, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, r, r0, ctx->part);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 4, ctx->scal);
+        IMPEC_HIP_CHECK(ctx, hipGetLastError());
+        if ((st = fetch(ctx))) return st;
+        bb = ctx->h_scal[S_BB];
+        rr = ctx->h_scal[S_RR];
+        if (!(rr <= tol * tol * bb) && rr == rr) {   // the first direction p = r
+            CnArgs ap = a;
+            ap.pout = pb[1]; ap.out = vb[1];
+            prof_begin(ctx);
+            hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, sblk, 0, s, ap);
+            prof_end(ctx, "cn_apply_p0");
+            hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 1, ctx->scal);
+            IMPEC_HIP_CHECK(ctx, hipGetLastError());
+            cur = 1;
+        }
+    }
+    // the current residual: b before the first p-launch of a zero-guess solve
     const double *rcur = a.r;
     int rnext = 0;
-    int cur = 0, its = 0, status = IMPEC_OK;
-    if (!(rr <= tol * tol * bb) && rr == rr) {   // the first direction p = r
-        CnArgs ap = a;
-        ap.pout = pb[1]; ap.out = vb[1];
-        prof_begin(ctx);
-        hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, sblk, 0, s, ap);
-        prof_end(ctx, "cn_apply_p0");
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 1, ctx->scal);
-        IMPEC_HIP_CHECK(ctx, hipGetLastError());
-        cur = 1;
-    }
     while (!(rr <= tol * tol * bb)) {
         if (its >= maxit) { status = IMPEC_ENOCONV; break; }
         if (!(rr == rr)) { status = IMPEC_ENAN; break; }
@@ -733,7 +861,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         prof_end(ctx, "cn_apply_s");
         hipLaunchKernelGGL(k_bicg_finish<4>, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 2, ctx->scal);
         CnArgs ap = a2;
-        ap.r = rcur; ap.pin = pb[cur]; ap.vin = vb[cur]; ap.x = x;
+        ap.r = rcur; ap.pin = its == 0 ? pfirst : pb[cur]; ap.vin = vb[cur]; ap.x = x;
         if (rb[rnext] == rcur) rnext ^= 1;   // r' never overwrites the r it is formed from
         ap.rout = rb[rnext];
         ap.xzero = zero && its == 0;
@@ -751,7 +879,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         its++;
     }
     if (zero && its == 0)   // b already met the tolerance: x = 0 was never written
-        hipLaunchKernelGGL(k_cn_zero, vg, vblk, 0, s, N1, N2, x, (long)ctx->lds);
+        hipLaunchKernelGGL(k_cn_zero, cn_vgrid(N1, N2), dim3(64, 4), 0, s, N1, N2, x, (long)ctx->lds);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     ctx->stats.last_parabolic_its = its;
     ctx->stats.last_parabolic_relres = bb > 0 ? sqrt(rr / bb) : 0.;

@@ -12,14 +12,18 @@ import re
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 
 KERNELS = {
-    # BiCGStab p-launch (k_cn_stream<0>): r = s - omega t, x += alpha p/D + omega s/D,
-    # p = r + beta (p - omega v), v = A (p / D), r0.v, r.r; read s, t, p, v, x, r0
-    # (6 x 8), 1/D (FP32, 4) and the face velocities u, v (2 x 8; the face tensors
-    # are formed in registers), write r, x, p, v (4 x 8)
-    "cn_apply_p": dict(rocprof="k_cn_stream<0>", bytes_per_cell=100),
+    # BiCGStab p-launch (k_cn_pstep): s = r - alpha v and t = A (s / D) again,
+    # r' = s - omega t, x += alpha p/D + omega s/D, p' = r' + beta (p - omega v),
+    # v' = A (p' / D), r0.v', r'.r'; read r, v, p, x, r0 (5 x 8), 1/D (FP32, 4)
+    # and the face velocities u, v (2 x 8; the face tensors are formed in
+    # registers), write r', x, p', v' (4 x 8)
+    "cn_apply_p": dict(rocprof="k_cn_pstep", bytes_per_cell=92),
     # BiCGStab s-launch (k_cn_stream<1>): s = r - alpha v, t = A (s / D), t.s, t.t,
-    # r0.t, r0.s; read r, v, r0 (3 x 8), 1/D (4) and u, v (2 x 8), write s, t (2 x 8)
-    "cn_apply_s": dict(rocprof="k_cn_stream<1>", bytes_per_cell=60),
+    # r0.t, r0.s; read r, v, r0 (3 x 8), 1/D (4) and u, v (2 x 8); no vector written
+    "cn_apply_s": dict(rocprof="k_cn_stream<1>", bytes_per_cell=44),
+    # zero-guess start (k_cn_start): b, 1/D and v = A (b / D); read c, u, v
+    # (3 x 8), write b (8), 1/D (4), v (8)
+    "cn_start": dict(rocprof="k_cn_start", bytes_per_cell=44),
     # streaming V-cycle, finest level, down leg: read w, r (2 x 8), write x (8)
     # and the coarse right-hand side (8 / 4); FP32 cycle: w 4, r 8, x 4, b_c 1
     "mg_down_l0": dict(rocprof="k_mg_down<true, false", bytes_per_cell=34, bytes_per_cell_fp32=17),
@@ -31,6 +35,9 @@ KERNELS = {
     "cg_dir_matvec": dict(rocprof="k_dir_mv", bytes_per_cell=48, bytes_per_cell_fp32=44),
     # CG update: read p, d, r, q (4 x 8), write p, r (2 x 8)
     "cg_update": dict(rocprof="k_cg_update2", bytes_per_cell=48),
+    # symmetric parallel splitting in one pass (k_advect_parallel): read c, u, v
+    # (3 x 8), write the averaged c (8)
+    "advect_parallel": dict(rocprof="k_advect_parallel", bytes_per_cell=32),
     # MUSCL-Hancock x / y sweep: read c and one velocity component, write c
     "sweep_x": dict(rocprof="k_sweep_x", bytes_per_cell=24),
     "sweep_y": dict(rocprof="k_sweep_y", bytes_per_cell=24),
