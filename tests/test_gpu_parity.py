@@ -113,7 +113,11 @@ def test_source_split_bitexact(gpu_lib, profile, stype):
 
 @pytest.mark.parametrize("kw", [dict(dim_split=4), dict(dim_split=0), dict(dim_split=1), dict(dim_split=2),
                                 dict(dim_split=3), dict(dim_split=4, hyper=1), dict(dim_split=4, porosity=0.7),
-                                dict(dim_split=4, is_test=1), dict(dim_split=4, N1=21, N2=11)])
+                                dict(dim_split=4, is_test=1), dict(dim_split=4, N1=21, N2=11),
+                                # the one-pass parallel splitting: widths across its 60-column
+                                # strips, ragged row chunks, tiny grids
+                                dict(dim_split=4, N1=7, N2=61), dict(dim_split=4, N1=70, N2=121, hyper=1),
+                                dict(dim_split=4, N1=3, N2=2), dict(dim_split=4, N1=130, N2=57, porosity=0.7)])
 def test_advect_split_bitexact(gpu_lib, kw):
     p = params(**dict(dict(N1=24, N2=24), **kw))
     _say("advect", kw)
@@ -174,6 +178,32 @@ def test_parabolic_solve_matches_dense(gpu_lib):
     o, g = pair(p)
     c = smooth_c(o, seed=13)
     rng = np.random.default_rng(14)
+    v = rng.standard_normal(o.velocity().shape) + 0.5
+    o.fill_velocity(v)
+    g.set_concentration(c)
+    g.set_velocity(v)
+    dt = 3e-3
+    its = g.dispersion_cn(dt)
+    x = po.dense_solve9(o.N1, o.N2, o.parabolic_matrix(v, dt), o.parabolic_rhs(c, v, dt))
+    cg = o.interior(g.get_concentration())
+    assert its > 0
+    assert rel_err(cg.reshape(-1), x) < 1e-12
+
+
+@pytest.mark.parametrize("n1,n2", [(5, 3), (9, 61), (21, 63), (13, 125), (33, 62)])
+@pytest.mark.parametrize("cn_guess", [0, 1])
+def test_parabolic_solve_ragged_strips(gpu_lib, n1, n2, cn_guess):
+    """The streamed BiCGStab on grids whose widths cross the 60- and
+    62-column strips of its launches and whose heights leave ragged row
+    chunks (one-row chunks included), from the zero guess (the one-pass
+    start) and from the current concentration, matches the dense solve of
+    the oracle's operator to 1e-12."""
+    from oracle import pyoracle as po
+    p = params(N1=n1, N2=n2, diffusion=1, alpha_l=0.5, alpha_t=0.2, para_eps=1e-14)
+    _say("parabolic solve ragged", n1, n2, cn_guess)
+    o, g = pair(p, cn_guess=cn_guess)
+    c = smooth_c(o, seed=21)
+    rng = np.random.default_rng(22)
     v = rng.standard_normal(o.velocity().shape) + 0.5
     o.fill_velocity(v)
     g.set_concentration(c)
