@@ -37,7 +37,7 @@ def workload_params(capi, n, precision="fp64", guess="zero", mg_rows=0, cn_guess
         para_method=capi.PARA_CN, alpha_absolute=1, x_lb=0.0, x_ub=1.0, y_lb=0.0, y_ub=1.0,
         nu=1.0, mu_res=1.0, mu_inj=0.1, porosity=1.0, cfl=0.5, inj_fluid_conc=1.0,
         alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5, eps=1e-10, para_eps=1e-10, I=1.0,
-        precision=capi.PREC_MIXED if precision == "mixed" else capi.PREC_FP64,
+        precision={"mixed": capi.PREC_MIXED, "mixed32": capi.PREC_MIXED32}.get(precision, capi.PREC_FP64),
         initial_guess=capi.GUESS_PREVIOUS if guess == "previous" else capi.GUESS_ZERO, mg_rows=mg_rows,
         cn_guess=cn_guess)
 
@@ -91,7 +91,8 @@ def modes(capi, n, warmup=3, steps=8):
            "learned_training": train_info}
     for name, prec, guess in [("fp64_zero", "fp64", "zero"), ("mixed_zero", "mixed", "zero"),
                               ("mixed_previous", "mixed", "previous"), ("mixed_extrapolated", "mixed", "extrapolate"),
-                              ("mixed_learned", "mixed", "learned")]:
+                              ("mixed_learned", "mixed", "learned"), ("mixed32_zero", "mixed32", "zero"),
+                              ("mixed32_previous", "mixed32", "previous"), ("mixed32_learned", "mixed32", "learned")]:
         p = lognormal_params(capi, n, prec, "zero")
         p.initial_guess = {"zero": capi.GUESS_ZERO, "previous": capi.GUESS_PREVIOUS,
                            "extrapolate": capi.GUESS_EXTRAPOLATE, "learned": capi.GUESS_LEARNED}[guess]
@@ -124,6 +125,9 @@ def modes(capi, n, warmup=3, steps=8):
                                                  / res["mixed_extrapolated"]["ms_per_step"])
     res["speedup_mixed_previous_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_previous"]["ms_per_step"]
     res["speedup_mixed_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_zero"]["ms_per_step"]
+    for g in ("zero", "previous", "learned"):
+        res["speedup_mixed32_%s_vs_fp64" % g] = (res["fp64_zero"]["ms_per_step"]
+                                                 / res["mixed32_%s" % g]["ms_per_step"])
     return res
 
 
@@ -241,7 +245,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "mixed32"], default="fp64")
     ap.add_argument("--guess", choices=["zero", "previous"], default="zero")
     ap.add_argument("--mg-rows", type=int, default=0,
                     help="rows per wavefront chunk of the finest multigrid level (0: automatic)")
@@ -303,7 +307,7 @@ def main():
 
     if not args.no_roofline:
         from impec2d_amd import roofline
-        out["roofline"] = roofline.measure(ctx, n, t, fp32_cycle=args.precision == "mixed")
+        out["roofline"] = roofline.measure(ctx, n, t, fp32_cycle=args.precision in ("mixed", "mixed32"))
         out["roofline_by_kernel"] = roofline.measure.last_table
     ctx.close()
     if not args.no_modes:

@@ -26,7 +26,7 @@ AVG_HARMONIC, AVG_MEANCONC = 0, 1
 SPLIT_ALTERNATING, SPLIT_X_FIRST, SPLIT_Y_FIRST, SPLIT_STRANG, SPLIT_PARALLEL = range(5)
 HYPER_MUSCL, HYPER_UPWIND = 0, 1
 PARA_EXPLICIT, PARA_CN = 0, 1
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED32 = 0, 1, 2
 GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED, GUESS_EXTRAPOLATE = 0, 1, 2, 3
 
 _INT_FIELDS = ["N1", "N2", "num_ghostcells", "source_type", "average_procedure", "dim_splitting",

@@ -152,7 +152,8 @@ int impec::ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out) {
     if (!(P->nu >= 0. && P->nu <= 1.)) return IMPEC_EINVAL;   // EllipticSolver::SetNu
     if (!(P->x_ub > P->x_lb && P->y_ub > P->y_lb)) return IMPEC_EINVAL;
     if (P->dim_splitting < 0 || P->dim_splitting > 4) return IMPEC_EINVAL;
-    if (P->precision != IMPEC_PREC_FP64 && P->precision != IMPEC_PREC_MIXED) return IMPEC_EINVAL;
+    if (P->precision != IMPEC_PREC_FP64 && P->precision != IMPEC_PREC_MIXED && P->precision != IMPEC_PREC_MIXED32)
+        return IMPEC_EINVAL;
     if (P->mg_rows < 0 || P->mg_rows > 4096) return IMPEC_EINVAL;
     if (P->mg_variant < 0 || P->mg_variant > 2) return IMPEC_EINVAL;
     if (P->cn_guess != 0 && P->cn_guess != 1) return IMPEC_EINVAL;

@@ -179,6 +179,7 @@ int impec_ensemble_create(const impec_params *members, int count, impec_ensemble
         const impec_params &P = members[k];
         if (P.nu != 1. || P.diffusion || P.is_test || P.dim_splitting != IMPEC_SPLIT_PARALLEL) return IMPEC_EINVAL;
         if (P.initial_guess != IMPEC_GUESS_ZERO && P.initial_guess != IMPEC_GUESS_PREVIOUS) return IMPEC_EINVAL;
+        if (P.precision == IMPEC_PREC_MIXED32) return IMPEC_EINVAL;   // single contexts only
         if (P.precision != P0.precision || P.hyper_method != P0.hyper_method ||
             (P.porosity == 1.) != (P0.porosity == 1.))
             return IMPEC_EINVAL;

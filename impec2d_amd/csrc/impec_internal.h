@@ -105,6 +105,10 @@ struct impec_ctx {
     impec::Level lev[impec::MAX_LEVELS];
     double *pr, *pz, *pd, *pq; // PCG vectors on the level-0 ring-1 layout
     double *fx, *fy;           // level-0 face mobilities of the 5-point operator (per solve)
+    // the FP32-vector CG (IMPEC_PREC_MIXED32), level-0 ring-1 layout: residual,
+    // correction since the last reliable update, direction pair, matvec, and
+    // FP32 copies of fx / fy
+    float *pr32, *pe32, *pd32, *pz32, *pq32, *fx32, *fy32;
     double *scal;              // device scalars (Scal)
     double *part;              // reduction partials
     double *h_scal;            // pinned host mirror of scal

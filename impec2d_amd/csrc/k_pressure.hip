@@ -817,6 +817,11 @@ void pressure_free_levels(impec_ctx *ctx) {
     ctx->nlev = 0;
     hipFree(ctx->pr); hipFree(ctx->pz); hipFree(ctx->pd); hipFree(ctx->pq);
     hipFree(ctx->fx); hipFree(ctx->fy);
+    float **v32[7] = {&ctx->pr32, &ctx->pe32, &ctx->pd32, &ctx->pz32, &ctx->pq32, &ctx->fx32, &ctx->fy32};
+    for (float **v : v32) {
+        if (*v) hipFree(*v);
+        *v = nullptr;
+    }
 }
 
 static dim3 grid2(int n1, int n2, dim3 blk) { return dim3((n2 + blk.x - 1) / blk.x, (n1 + blk.y - 1) / blk.y); }
@@ -978,6 +983,7 @@ struct FusedArgsT {
     int ld;                 // pitch of this level's x/b (ring-1 arrays at cell (0,0))
     const T *b;             // level rhs (levels >= 1)
     const double *r;        // level 0: CG residual (FP64), b = r - S[S_RSUM]/n
+    const float *r32;       // level 0: the FP32 residual of the FP32-vector CG (R32 legs)
     const double *S;
     double inv_n;
     T *x;                   // this level's x (down: written; up: read)
@@ -1130,7 +1136,7 @@ __device__ __forceinline__ void mask_pair(const Lane &L, int m, int n1, V fill, 
 #endif
 constexpr int SW_PF = IMPEC_SW_PF;
 
-template <bool FINE0, bool ANISO, typename T, int NS = 1>
+template <bool FINE0, bool ANISO, typename T, int NS = 1, bool R32 = false>
 __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int wid) {
 #pragma clang fp contract(fast)
     // NS red-black pairs: red from zero on row k, then the other 2 NS - 1
@@ -1144,9 +1150,11 @@ __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int w
     const int n1 = A.g.n1, n2 = A.g.n2;
     constexpr bool aniso = ANISO;
     const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
-    using Src = typename std::conditional<FINE0, double, T>::type;   // level 0 reads the FP64 CG residual
+    // level 0 reads the CG residual: FP64, or FP32 for the FP32-vector CG (R32)
+    using Src = typename std::conditional<FINE0, typename std::conditional<R32, float, double>::type, T>::type;
     const Src *src;
-    if constexpr (FINE0) src = A.r;
+    if constexpr (FINE0 && R32) src = A.r32;
+    else if constexpr (FINE0) src = A.r;
     else src = A.b;
     // full weighting: this lane's coarse column and the weights of its four fine columns
     const int J = L.g0 >> 1;
@@ -1284,12 +1292,13 @@ __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int w
     }
 }
 
-template <bool FINE0, bool ANISO, typename T, int NS>
+template <bool FINE0, bool ANISO, typename T, int NS, bool R32 = false>
 __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_down(FusedArgsT<T> A) {
-    mg_down_body<FINE0, ANISO, T, NS>(A, xcd_wave());
+    static_assert(FINE0 || !R32, "only level 0 reads the CG residual");
+    mg_down_body<FINE0, ANISO, T, NS, R32>(A, xcd_wave());
 }
 
-template <bool FINE0, bool ANISO, typename T, int NS = 1>
+template <bool FINE0, bool ANISO, typename T, int NS = 1, bool R32 = false>
 __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid) {
 #pragma clang fp contract(fast)
     // NS black-red pairs after the correction of row k: black on row k-1, red
@@ -1303,9 +1312,11 @@ __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid
     const int n1 = A.g.n1, m1 = A.m1, m2 = A.m2;
     constexpr bool aniso = ANISO;
     const double mean = FINE0 ? A.S[S_RSUM] * A.inv_n : 0.;
-    using Src = typename std::conditional<FINE0, double, T>::type;   // level 0 reads the FP64 CG residual
+    // level 0 reads the CG residual: FP64, or FP32 for the FP32-vector CG (R32)
+    using Src = typename std::conditional<FINE0, typename std::conditional<R32, float, double>::type, T>::type;
     const Src *src;
-    if constexpr (FINE0) src = A.r;
+    if constexpr (FINE0 && R32) src = A.r32;
+    else if constexpr (FINE0) src = A.r;
     else src = A.b;
     // prolongation: this lane's coarse column J and its clamped neighbours
     const int J = L.g0 >> 1;
@@ -1426,9 +1437,10 @@ __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid
     }
 }
 
-template <bool FINE0, bool ANISO, typename T, int NS>
+template <bool FINE0, bool ANISO, typename T, int NS, bool R32 = false>
 __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgsT<T> A) {
-    mg_up_body<FINE0, ANISO, T, NS>(A, xcd_wave());
+    static_assert(FINE0 || !R32, "only level 0 reads the CG residual");
+    mg_up_body<FINE0, ANISO, T, NS, R32>(A, xcd_wave());
 }
 
 // ===========================================================================
@@ -1743,21 +1755,37 @@ constexpr int DM_OUT = 62, DM_PF = 4;
 #define IMPEC_DM_WAVES 8192
 #endif
 
-template <typename T>
+// DT: storage of d and q, double for the FP64 CG, float for the FP32-vector CG
+// (whose matvec reads FP32 copies of the face mobilities, fx32 / fy32)
+template <typename T, typename DT = double>
 struct DirMvArgs {
     LevArgs L;
     CgArgs c;
     const T *x0;          // V-cycle output, level-0 ring-1 layout
-    double *d0, *d1;      // the two direction arrays (ensembles pick by iteration parity)
-    double *q;
+    DT *d0, *d1;          // the two direction arrays (ensembles pick by iteration parity)
+    DT *q;
+    const float *fx32, *fy32;
     const double *S;
     double *part;
     int R, nstrip, nwave;
 };
 
-template <typename T>
-__device__ __forceinline__ void dir_mv_body(const DirMvArgs<T> &A, const double *__restrict__ dold,
-                                            double *__restrict__ dnew, const int first, const int wid) {
+// op_row5 with FP32 face mobilities (the FP32-vector CG's operator)
+__device__ __forceinline__ void op_row5_f32(const LevArgs &L, const float *fx, const float *fy, int i, int j,
+                                            double a[9]) {
+    const OpGeom &g = L.g;
+    const long o = (long)i * L.ld + j;
+    const double fxm = fx[o], fxp = fx[o + L.ld], fym = fy[o], fyp = fy[o + 1];
+    const double Wp = g.cx * fxm, Ep = g.cx * fxp, Sp = g.cy * fym, Np = g.cy * fyp;
+    const double Cp = g.mcy * (fym + fyp) - g.cx * (fxm + fxp);
+    a[ST_C] = Cp; a[ST_W] = Wp; a[ST_SW] = 0.; a[ST_S] = Sp; a[ST_SE] = 0.;
+    a[ST_E] = Ep; a[ST_NE] = 0.; a[ST_N] = Np; a[ST_NW] = 0.;
+    fold_row(i, j, g.n1, g.n2, a);
+}
+
+template <typename T, typename DT = double>
+__device__ __forceinline__ void dir_mv_body(const DirMvArgs<T, DT> &A, const DT *__restrict__ dold,
+                                            DT *__restrict__ dnew, const int first, const int wid) {
     if (wid >= A.nwave) return;
     const CgArgs &c = A.c;
     const int n1 = c.n1, n2 = c.n2;
@@ -1777,7 +1805,8 @@ __device__ __forceinline__ void dir_mv_body(const DirMvArgs<T> &A, const double 
     auto dn = [&](int i, double xv, double dv) -> double {
         if (i < 0 || i >= n1 || !jv) return 0.;
         if (!c.shift && i == c.pin_i && j == c.pin_j) return 0.;
-        return first ? xv - mean : (xv - mean) + beta * dv;
+        // rounded to the storage type: the matvec applies A to the stored d
+        return (double)(DT)(first ? xv - mean : (xv - mean) + beta * dv);
     };
     double RX[DM_PF], RD[DM_PF];
 #pragma unroll
@@ -1805,7 +1834,8 @@ __device__ __forceinline__ void dir_mv_body(const DirMvArgs<T> &A, const double 
             }
             if (i < i1) {
                 double a[9];
-                op_row5(A.L, i, jc, a);
+                if constexpr (std::is_same<DT, float>::value) op_row5_f32(A.L, A.fx32, A.fy32, i, jc, a);
+                else op_row5(A.L, i, jc, a);
                 const double dl = lane_prev(dcur), dr = lane_next(dcur);
                 double y = a[ST_C] * dcur;
                 if (i > 0) y += a[ST_W] * dm;
@@ -1815,9 +1845,10 @@ __device__ __forceinline__ void dir_mv_body(const DirMvArgs<T> &A, const double 
                 if (!c.shift && i == c.pin_i && j == c.pin_j) y = 0.;
                 if (outl) {
                     const long o = (long)(i + 1) * ld + (j + 1);
-                    dnew[o] = dcur;
-                    A.q[o] = y;
-                    acc += dcur * y;
+                    const double yq = (double)(DT)y;   // q as stored
+                    dnew[o] = (DT)dcur;
+                    A.q[o] = (DT)yq;
+                    acc += dcur * yq;
                 }
             }
             dm = dcur;
@@ -1829,10 +1860,68 @@ __device__ __forceinline__ void dir_mv_body(const DirMvArgs<T> &A, const double 
     if (lane == 0) A.part[wid] = acc;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(64 * SW_WAVES) k_dir_mv(DirMvArgs<T> A, const double *__restrict__ dold,
-                                                          double *__restrict__ dnew, int first) {
-    dir_mv_body<T>(A, dold, dnew, first, xcd_wave());
+template <typename T, typename DT = double>
+__global__ void __launch_bounds__(64 * SW_WAVES) k_dir_mv(DirMvArgs<T, DT> A, const DT *__restrict__ dold,
+                                                          DT *__restrict__ dnew, int first) {
+    dir_mv_body<T, DT>(A, dold, dnew, first, xcd_wave());
+}
+
+// --- the FP32-vector CG (precision IMPEC_PREC_MIXED32) ----------------------
+// e += alpha d, r -= alpha q in FP32 storage (alpha and the arithmetic in
+// FP64, each result rounded once); partials (r.r, sum r) of the stored r
+__global__ void k_cg_update32(CgArgs c, float *__restrict__ e, const float *__restrict__ d, float *__restrict__ r,
+                              const float *__restrict__ qv, const double *__restrict__ S, double *__restrict__ part) {
+    const double alpha = S[S_RHO] / S[S_DQ];
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    double s[2] = {0., 0.};
+    if (j < c.n2)
+        for (int i = blockIdx.y * 4 + threadIdx.y; i < c.n1; i += gridDim.y * 4) {
+            const long o = (long)(i + 1) * c.ld + (j + 1);
+            e[o] = (float)((double)e[o] + alpha * (double)d[o]);
+            const float rv = (float)((double)r[o] - alpha * (double)qv[o]);
+            r[o] = rv;
+            s[0] += (double)rv * (double)rv;
+            s[1] += (double)rv;
+        }
+    block_reduce<RED_SUM, 2>(s);
+    if (tid_linear() == 0) {
+        const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+        part[blk] = s[0];
+        part[RED_BLOCKS + blk] = s[1];
+    }
+}
+
+// a reliable update's restart of the FP32 recursion: r32 = the FP64 residual
+// rounded, e = 0; partials (r32.r32, sum r32)
+__global__ void k_cg_restart32(CgArgs c, const double *__restrict__ r64, float *__restrict__ r32,
+                               float *__restrict__ e, double *__restrict__ part) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    double s[2] = {0., 0.};
+    if (j < c.n2)
+        for (int i = blockIdx.y * 4 + threadIdx.y; i < c.n1; i += gridDim.y * 4) {
+            const long o = (long)(i + 1) * c.ld + (j + 1);
+            const float rv = (float)r64[o];
+            r32[o] = rv;
+            e[o] = 0.f;
+            s[0] += (double)rv * (double)rv;
+            s[1] += (double)rv;
+        }
+    block_reduce<RED_SUM, 2>(s);
+    if (tid_linear() == 0) {
+        const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+        part[blk] = s[0];
+        part[RED_BLOCKS + blk] = s[1];
+    }
+}
+
+// a reliable update's FP64 accumulation: p += e (pressure pitch ldp, ghost B)
+__global__ void k_cg_fold32(CgArgs c, double *__restrict__ p, const float *__restrict__ e) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    if (j < c.n2)
+        for (int i = blockIdx.y * 4 + threadIdx.y; i < c.n1; i += gridDim.y * 4) {
+            double *xp = p + (long)(i + c.B) * c.ldp + (j + c.B);
+            *xp = *xp + (double)e[(long)(i + 1) * c.ld + (j + 1)];
+        }
 }
 
 
@@ -1920,6 +2009,7 @@ static FusedArgsT<T> fused_args(impec_ctx *ctx, int l) {
     A.ld = L.ld;
     A.b = LA::b(L) + L.ld + 1;
     A.r = ctx->pr + L.ld + 1;
+    A.r32 = ctx->pr32 ? ctx->pr32 + L.ld + 1 : nullptr;
     A.S = ctx->scal;
     A.inv_n = 1.0 / ((double)ctx->N1 * ctx->N2);
     A.x = LA::x(L) + L.ld + 1;
@@ -1976,8 +2066,10 @@ static BottomArgs<T> bottom_args(impec_ctx *ctx, int lb) {
     return A;
 }
 
+// r32: level 0 reads the FP32 residual of the FP32-vector CG (T = float only)
 template <typename T, int NS>
-static int vcycle_fused_ns(impec_ctx *ctx) {
+static int vcycle_fused_ns(impec_ctx *ctx, bool r32) {
+    constexpr bool F = std::is_same<T, float>::value;
     using LA = LevelArrays<T>;
     const int L = ctx->nlev;
     const dim3 blk(64 * SW_WAVES);
@@ -1986,7 +2078,11 @@ static int vcycle_fused_ns(impec_ctx *ctx) {
     for (int l = 0; l < ltop; l++) {
         FusedArgsT<T> A = fused_args<T>(ctx, l);
         prof_begin(ctx);
-        if (l == 0 && A.w2 != A.w)
+        if (F && r32 && l == 0 && A.w2 != A.w)
+            hipLaunchKernelGGL((k_mg_down<true, true, T, NS, F>), stream_grid(A), blk, 0, ctx->stream, A);
+        else if (F && r32 && l == 0)
+            hipLaunchKernelGGL((k_mg_down<true, false, T, NS, F>), stream_grid(A), blk, 0, ctx->stream, A);
+        else if (l == 0 && A.w2 != A.w)
             hipLaunchKernelGGL((k_mg_down<true, true, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
         else if (l == 0) hipLaunchKernelGGL((k_mg_down<true, false, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
         else hipLaunchKernelGGL((k_mg_down<false, false, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
@@ -2007,7 +2103,11 @@ static int vcycle_fused_ns(impec_ctx *ctx) {
     for (int l = ltop - 1; l >= 0; l--) {
         FusedArgsT<T> A = fused_args<T>(ctx, l);
         prof_begin(ctx);
-        if (l == 0 && A.w2 != A.w)
+        if (F && r32 && l == 0 && A.w2 != A.w)
+            hipLaunchKernelGGL((k_mg_up<true, true, T, NS, F>), stream_grid(A), blk, 0, ctx->stream, A);
+        else if (F && r32 && l == 0)
+            hipLaunchKernelGGL((k_mg_up<true, false, T, NS, F>), stream_grid(A), blk, 0, ctx->stream, A);
+        else if (l == 0 && A.w2 != A.w)
             hipLaunchKernelGGL((k_mg_up<true, true, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
         else if (l == 0) hipLaunchKernelGGL((k_mg_up<true, false, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
         else hipLaunchKernelGGL((k_mg_up<false, false, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
@@ -2018,8 +2118,8 @@ static int vcycle_fused_ns(impec_ctx *ctx) {
 }
 
 template <typename T>
-static int vcycle_fused(impec_ctx *ctx) {
-    return mg_ns(ctx) == 2 ? vcycle_fused_ns<T, 2>(ctx) : vcycle_fused_ns<T, 1>(ctx);
+static int vcycle_fused(impec_ctx *ctx, bool r32 = false) {
+    return mg_ns(ctx) == 2 ? vcycle_fused_ns<T, 2>(ctx, r32) : vcycle_fused_ns<T, 1>(ctx, r32);
 }
 
 // FP32 copy of a mobility array (rows x cols cells from (0,0) of each)
@@ -2063,6 +2163,38 @@ static int mixed_setup(impec_ctx *ctx) {
     }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
+}
+
+// the FP32-vector CG's arrays (first use) and this solve's FP32 face copies
+static int mixed32_setup(impec_ctx *ctx) {
+    const size_t bytes = sizeof(float) * (size_t)(ctx->N1 + 2) * ctx->lev[0].ld;
+    float **v32[7] = {&ctx->pr32, &ctx->pe32, &ctx->pd32, &ctx->pz32, &ctx->pq32, &ctx->fx32, &ctx->fy32};
+    for (float **v : v32) {
+        if (*v) continue;
+        if (hipMalloc(v, bytes) != hipSuccess) {
+            *v = nullptr;
+            return set_error(ctx, IMPEC_ENOMEM, "FP32 CG vector allocation failed");
+        }
+        hipMemset(*v, 0, bytes);
+    }
+    // fx: faces i = 0..N1 of columns 0..N2-1; fy: rows 0..N1-1, faces j = 0..N2
+    const int ld = ctx->lev[0].ld;
+    const dim3 blk(64, 4);
+    hipLaunchKernelGGL(k_to_f32, grid2(ctx->N1 + 1, ctx->N2 + 1, blk), blk, 0, ctx->stream, ctx->fx, ld, ctx->fx32, ld,
+                       ctx->N1 + 1, ctx->N2 + 1);
+    hipLaunchKernelGGL(k_to_f32, grid2(ctx->N1 + 1, ctx->N2 + 1, blk), blk, 0, ctx->stream, ctx->fy, ld, ctx->fy32, ld,
+                       ctx->N1 + 1, ctx->N2 + 1);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
+// restart of the FP32 recursion from the FP64 residual in pr: r32 = r, e = 0,
+// S_RR and S_RSUM of r32
+static int cg_restart32(impec_ctx *ctx, const CgArgs &c) {
+    const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
+    hipLaunchKernelGGL(k_cg_restart32, vg.grid, vg.block, 0, ctx->stream, c, ctx->pr, ctx->pr32, ctx->pe32, ctx->part);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return finish_sum(ctx, vg.nblocks, 2, S_RR);
 }
 
 // r = b~ - A_N x recomputed from the current iterate (the CG start-up
@@ -2163,6 +2295,65 @@ static int cg_iteration_fused(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c
     return IMPEC_OK;
 }
 
+// one iteration of the FP32-vector CG (IMPEC_PREC_MIXED32): the FP32 V-cycle
+// on the FP32 residual, the FP32 direction and matvec (FP32 faces), the FP32
+// updates of the residual and of the correction accumulated since the last
+// reliable update; every dot product and scalar in FP64
+static int cg_iteration_mixed32(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, bool first, float **dold,
+                                float **dnew) {
+    hipStream_t s = ctx->stream;
+    const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
+    const int nwave0 = fused_args<float>(ctx, 0).nwave;
+    const double inv_n = 1.0 / (double)c.n;
+    int st;
+    if ((st = vcycle_fused<float>(ctx, true))) return st;
+    hipLaunchKernelGGL(k_finish_rho, dim3(1), dim3(1024), 0, s, ctx->part, nwave0, inv_n, ctx->scal);
+    const DirMvArgs<float> d64 = dir_mv_args<float>(ctx, a0, c);
+    DirMvArgs<float, float> dm;
+    memset(&dm, 0, sizeof(dm));
+    dm.L = d64.L; dm.c = d64.c; dm.x0 = d64.x0;
+    dm.d0 = ctx->pd32; dm.d1 = ctx->pz32; dm.q = ctx->pq32;
+    dm.fx32 = ctx->fx32; dm.fy32 = ctx->fy32;
+    dm.S = d64.S; dm.part = d64.part;
+    dm.R = d64.R; dm.nstrip = d64.nstrip; dm.nwave = d64.nwave;
+    prof_begin(ctx);
+    hipLaunchKernelGGL((k_dir_mv<float, float>), dim3((dm.nwave + SW_WAVES - 1) / SW_WAVES), dim3(64 * SW_WAVES), 0,
+                       s, dm, *dold, *dnew, first ? 1 : 0);
+    prof_end(ctx, "cg_dir_matvec");
+    if ((st = finish_sum(ctx, dm.nwave, 1, S_DQ))) return st;
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_cg_update32, vg.grid, vg.block, 0, s, c, ctx->pe32, *dnew, ctx->pr32, ctx->pq32, ctx->scal,
+                       ctx->part);
+    prof_end(ctx, "cg_update");
+    if ((st = finish_sum(ctx, vg.nblocks, 2, S_RR))) return st;
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    float *t = *dold;
+    *dold = *dnew;
+    *dnew = t;
+    return IMPEC_OK;
+}
+
+// a reliable update of the FP32-vector CG: p += e in FP64, the FP64 residual
+// of the current formulation recomputed (||r||^2 into *rr), and the FP32
+// recursion restarted from it; the direction is kept
+template <bool DIAG>
+static int cg_reliable32(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, int nb, double *rr) {
+    const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_cg_fold32, vg.grid, vg.block, 0, ctx->stream, c, ctx->p, ctx->pe32);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    prof_end(ctx, "cg_reliable");
+    int st = cg_residual<DIAG>(ctx, a0, c, nb, rr);
+    if (st) return st;
+    return cg_restart32(ctx, c);
+}
+
+// a reliable update falls due when the FP32 residual has dropped by this factor
+// (in norm) since the last one, or meets the tolerance
+#ifndef IMPEC_RU_DELTA
+#define IMPEC_RU_DELTA 1e-4
+#endif
+
 // Multigrid-preconditioned CG for the reference's pinned system.
 //  phase 0: consistent Neumann formulation (when A_N is symmetric), then the
 //           shift to the pinned value;
@@ -2179,15 +2370,18 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     st = build_hierarchy(ctx);
     if (st) return st;
     if (std::is_same<T, float>::value && (st = mixed_setup(ctx))) return st;
+    // the FP32-vector CG with reliable FP64 updates (streaming 5-point path)
+    const bool m32 = std::is_same<T, float>::value && FUSED && !DIAG && ctx->P.precision == IMPEC_PREC_MIXED32;
     const Level &L0 = ctx->lev[0];
     LevArgs a0 = lev_args(ctx, 0);
-    if (!DIAG && !IMPEC_NO_FACES) {
+    if (!DIAG && (!IMPEC_NO_FACES || m32)) {
         dim3 blk(64, 4);
         hipLaunchKernelGGL(k_faces, grid2(ctx->N1 + 1, ctx->N2 + 1, blk), blk, 0, ctx->stream, a0, ctx->fx, ctx->fy);
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         a0.fx = ctx->fx;
         a0.fy = ctx->fy;
     }
+    if (m32 && (st = mixed32_setup(ctx))) return st;
     CgArgs c;
     c.n1 = ctx->N1; c.n2 = ctx->N2; c.ld = L0.ld; c.ldp = ctx->lds; c.B = ctx->B;
     c.n = (long)ctx->N1 * ctx->N2;
@@ -2231,8 +2425,14 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     int its = 0, status = IMPEC_OK;
     double relres = 0., rr = 0.;
     double *dold = ctx->pd, *dnew = ctx->pz;
+    float *dold32 = ctx->pd32, *dnew32 = ctx->pz32;
+    double rr_ref = 0.;   // FP64 residual norm^2 at the FP32 recursion's last restart
     for (int phase = ctx->symmetric ? 0 : 1; phase < 2; phase++) {
         c.shift = phase == 0;
+        // the FP32-vector CG runs the consistent phase; the elimination phase, a
+        // short continuation at the FP64 rounding floor of a worse-conditioned
+        // system, keeps FP64 vectors (its FP32 recursion stalls there)
+        const bool ru = m32 && c.shift;
         if (phase == 0 && zero_start) {
             prof_begin(ctx);
             hipLaunchKernelGGL(k_cg_init_zero, dim3(nb), dim3(256), 0, s, c, ctx->p, ctx->q, ctx->pr, ctx->part);
@@ -2243,6 +2443,10 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
             rr = ctx->h_scal[S_RR];
         } else if ((st = cg_residual<DIAG>(ctx, a0, c, nb, &rr))) {
             return st;
+        }
+        if (ru) {
+            if ((st = cg_restart32(ctx, c))) return st;
+            rr_ref = rr;
         }
         // p to the reference's pinned system (the shift, or the pinned value) and
         // that system's true relative residual
@@ -2266,13 +2470,36 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
             while (!(rr <= tol * tol * bb)) {
                 if (its >= maxit) { status = IMPEC_ENOCONV; break; }
                 if (!(rr == rr)) { status = IMPEC_ENAN; break; }
-                if (FUSED && !DIAG) st = cg_iteration_fused<T>(ctx, a0, c, since == 0, &dold, &dnew);
+                if (ru) st = cg_iteration_mixed32(ctx, a0, c, since == 0, &dold32, &dnew32);
+                else if (FUSED && !DIAG) st = cg_iteration_fused<T>(ctx, a0, c, since == 0, &dold, &dnew);
                 else st = cg_iteration_unfused<DIAG>(ctx, a0, c, nb, since == 0);
                 if (st) return st;
                 if ((st = fetch_scal(ctx))) return st;
                 rr = ctx->h_scal[S_RR];
                 its++;
                 since++;
+                // FP32-vector CG: the correction goes into p and the FP64 residual
+                // replaces the FP32 one when the latter has fallen by IMPEC_RU_DELTA,
+                // meets the tolerance or is not finite; only an FP64 residual ends
+                // the loop
+                if (ru && (rr <= tol * tol * bb || rr <= IMPEC_RU_DELTA * IMPEC_RU_DELTA * rr_ref || !(rr == rr))) {
+#ifdef IMPEC_RU_TRACE
+                    const double rr32 = rr;
+#endif
+                    if ((st = cg_reliable32<DIAG>(ctx, a0, c, nb, &rr))) return st;
+#ifdef IMPEC_RU_TRACE
+                    if (its < 200)
+                        fprintf(stderr, "ru phase %d its %d rr32/bb %.3e rr64/bb %.3e ref/bb %.3e\n", phase, its,
+                                rr32 / bb, rr / bb, rr_ref / bb);
+#endif
+                    rr_ref = rr;
+                }
+            }
+            if (ru && status != IMPEC_OK) {   // keep the corrections made so far
+                const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
+                hipLaunchKernelGGL(k_cg_fold32, vg.grid, vg.block, 0, s, c, ctx->p, ctx->pe32);
+                hipLaunchKernelGGL(k_cg_restart32, vg.grid, vg.block, 0, s, c, ctx->pr, ctx->pr32, ctx->pe32,
+                                   ctx->part);
             }
             if (status != IMPEC_OK) break;
             if (c.shift) {
@@ -2286,6 +2513,10 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
                 break;
             }
             if ((st = cg_residual<DIAG>(ctx, a0, c, nb, &rr))) return st;
+            if (ru) {
+                if ((st = cg_restart32(ctx, c))) return st;
+                rr_ref = rr;
+            }
             if (rr <= tol * tol * bb || restarts >= 3) break;
             restarts++;
             since = 0;
@@ -2330,7 +2561,8 @@ int pressure_solve(impec_ctx *ctx, int *its, double *relres) {
     // mixed precision: FP32 V-cycles as the preconditioner of the FP64 CG (a
     // Krylov-accelerated defect correction: residual, reductions, update and
     // the coarsest solve in FP64); on the streaming 5-point path only
-    if (fused_ok && ctx->P.precision == IMPEC_PREC_MIXED) return solve_cg<false, true, float>(ctx, its, relres);
+    if (fused_ok && (ctx->P.precision == IMPEC_PREC_MIXED || ctx->P.precision == IMPEC_PREC_MIXED32))
+        return solve_cg<false, true, float>(ctx, its, relres);
     if (fused_ok) return solve_cg<false, true>(ctx, its, relres);
     return solve_cg<false, false>(ctx, its, relres);
 }

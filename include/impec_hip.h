@@ -50,6 +50,8 @@ extern "C" {
 #define IMPEC_PARA_CN 1
 #define IMPEC_PREC_FP64 0    /* plain FP64 multigrid-preconditioned CG       */
 #define IMPEC_PREC_MIXED 1   /* FP32 V-cycles inside FP64 defect correction  */
+#define IMPEC_PREC_MIXED32 2 /* FP32 V-cycles and FP32 CG vectors, restarted  */
+                             /* from the FP64 residual by reliable updates    */
 #define IMPEC_GUESS_ZERO 0   /* reference semantics (MGSolver.cpp:573-590)   */
 #define IMPEC_GUESS_PREVIOUS 1
 #define IMPEC_GUESS_LEARNED 2   /* previous p plus the network's increment   */

@@ -82,7 +82,7 @@ def test_invalid_params_rejected():
     lib = capi.load()
     h = C.c_void_p()
     for kw in [dict(num_ghostcells=3), dict(N1=1), dict(nu=1.5), dict(source_type=2), dict(dim_splitting=7),
-               dict(mg_variant=3)]:
+               dict(mg_variant=3), dict(precision=3)]:
         p = capi.default_params(**kw)
         assert lib.impec_create(C.byref(p), C.byref(h)) == capi.IMPEC_EINVAL
 
@@ -90,7 +90,8 @@ def test_invalid_params_rejected():
 def test_ensemble_member_validation():
     """impec_ensemble_create checks every member before touching a device:
     members outside the batched path (nu < 1, dispersion, other splittings,
-    test mode, learned guess, mixed precisions or schemes) are refused, an
+    test mode, learned guess, mixed precisions or schemes, the FP32-vector CG)
+    are refused, an
     acceptable set without a device fails with IMPEC_ENODEV."""
     import ctypes as C
     lib = capi.load()
@@ -106,6 +107,9 @@ def test_ensemble_member_validation():
                dict(initial_guess=capi.GUESS_LEARNED), dict(mg_variant=1)]:
         assert create(base, capi.default_params(**dict(good, **kw))) == capi.IMPEC_EINVAL, kw
     assert create(base, capi.default_params(**dict(good, precision=capi.PREC_MIXED))) == capi.IMPEC_EINVAL
+    # the FP32-vector CG runs on single contexts only, even when every member asks for it
+    m32 = capi.default_params(**dict(good, precision=capi.PREC_MIXED32))
+    assert create(m32, m32) == capi.IMPEC_EINVAL
     assert create(base, capi.default_params(**dict(good, hyper_method=1))) == capi.IMPEC_EINVAL
     assert lib.impec_ensemble_create(None, 0, C.byref(h)) == capi.IMPEC_EINVAL
     if lib.impec_device_count() == 0:

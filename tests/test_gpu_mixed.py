@@ -1,8 +1,9 @@
 """Mixed precision: FP32 V-cycles as the preconditioner of the FP64 CG (the
 FP64 defect-correction loop keeps the residual, the reductions, the update and
-the coarsest solve in FP64).  The solve must reach the same true FP64 relative
-residual as the plain FP64 method, in about as many cycles, and the per-field
-error against the FP64 oracle is reported."""
+the coarsest solve in FP64), and the FP32-vector CG whose FP32 recursion is
+restarted from the FP64 residual by reliable updates.  Each solve must reach
+the same true FP64 relative residual as the plain FP64 method, in about as
+many cycles, and the per-field error against the FP64 oracle is reported."""
 import numpy as np
 import pytest
 
@@ -16,16 +17,18 @@ def _say(*a):
     print(*a, flush=True)
 
 
+@pytest.mark.parametrize("mode", ["mixed", "mixed32"])
 @pytest.mark.parametrize("n1,n2,aniso", [(256, 256, False), (200, 120, False), (77, 45, False),
                                          (128, 128, True), (1024, 1024, False)])
-def test_mixed_pressure_matches_fp64(gpu_lib, n1, n2, aniso):
+def test_mixed_pressure_matches_fp64(gpu_lib, n1, n2, aniso, mode):
     from impec2d_amd import capi
+    prec_m = capi.PREC_MIXED if mode == "mixed" else capi.PREC_MIXED32
     p = params(N1=n1, N2=n2, nu=1.0, eps=1e-10, mu_inj=1.0 / 20)
     K11 = fields.lognormal_permeability(n1, n2, sigma=1.0, corr_len=0.05, seed=1)
     K22 = fields.lognormal_permeability(n1, n2, sigma=1.0, corr_len=0.05, seed=2) if aniso else None
-    _say("mixed pressure", n1, n2, aniso)
+    _say("mixed pressure", mode, n1, n2, aniso)
     out = {}
-    for prec in (capi.PREC_FP64, capi.PREC_MIXED):
+    for prec in (capi.PREC_FP64, prec_m):
         o, g = pair(p, K11, K22, precision=prec)
         rng = np.random.default_rng(5)
         c = np.clip(rng.random(o.scalar().shape), 0, 1)
@@ -36,16 +39,17 @@ def test_mixed_pressure_matches_fp64(gpu_lib, n1, n2, aniso):
         out[prec] = (its, o.interior(g.get_pressure()))
         g.close()
     its64, p64 = out[capi.PREC_FP64]
-    its32, p32 = out[capi.PREC_MIXED]
-    _say("  its fp64 %d mixed %d, |p32-p64|/|p64| %.2e" % (its64, its32, rel_err(p32, p64)))
+    its32, p32 = out[prec_m]
+    _say("  its fp64 %d %s %d, |p32-p64|/|p64| %.2e" % (its64, mode, its32, rel_err(p32, p64)))
     # anisotropic K: the coarse levels coarsen the x-mobility only, the FP64
     # cycle needs ~100 iterations and FP32 rounding adds a few
     assert its32 <= (its64 * 1.1 + 2 if aniso else its64 + 2)
     assert rel_err(p32, p64) < 1e-8
 
 
+@pytest.mark.parametrize("mode", ["mixed", "mixed32"])
 @pytest.mark.parametrize("kw", [dict(), dict(diffusion=1, alpha_abs=1, alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5)])
-def test_mixed_steps_error_per_field(gpu_lib, kw):
+def test_mixed_steps_error_per_field(gpu_lib, kw, mode):
     """Ten IMPEC steps with FP32 V-cycles against the FP64 oracle, pressure
     solved to 1e-13: every field within 1e-10 relative, the same bar as the FP64
     method (the preconditioner's precision does not limit the FP64 residual)."""
@@ -53,7 +57,7 @@ def test_mixed_steps_error_per_field(gpu_lib, kw):
     n = 64
     p = params(**dict(dict(N1=n, N2=n, eps=1e-13, para_eps=1e-13, mu_inj=1.0 / 20), **kw))
     K = fields.lognormal_permeability(n, n, sigma=1.0, corr_len=0.05, seed=7)
-    o, g = pair(p, K, None, precision=capi.PREC_MIXED)
+    o, g = pair(p, K, None, precision=capi.PREC_MIXED if mode == "mixed" else capi.PREC_MIXED32)
     q = o.source("one_cell")
     c, pr, v = o.scalar(), o.scalar(), o.velocity()
     g.set_source(q)
@@ -67,5 +71,5 @@ def test_mixed_steps_error_per_field(gpu_lib, kw):
     ep = rel_err(o.interior(g.get_pressure()), o.interior(pr))
     ev = rel_err(g.get_velocity(), v)
     ec = rel_err(o.interior(g.get_concentration()), o.interior(c))
-    _say("mixed vs FP64 oracle after 10 steps: p %.2e v %.2e c %.2e" % (ep, ev, ec))
+    _say("%s vs FP64 oracle after 10 steps: p %.2e v %.2e c %.2e" % (mode, ep, ev, ec))
     assert ep < 1e-10 and ev < 1e-10 and ec < 1e-10
