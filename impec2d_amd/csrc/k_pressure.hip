@@ -301,10 +301,12 @@ __global__ void k_fill_zero(double *__restrict__ a, long n) {
 }
 
 // coarse mobility quantity: mean over the existing fine cells of each 2x2
-// block, on the interior and (clamped) ghost ring of the coarse level
+// block, on the interior and (clamped) ghost ring of the coarse level; wcf
+// (optional, same layout) receives the interior rounded to FP32 for the
+// mixed-precision cycle
 template <class G>
 __device__ __forceinline__ void coarsen_w_body(const G &gr_, const double *__restrict__ wf, int ldf, int n1, int n2, double *__restrict__ wc, int ldc,
-                            int m1, int m2) {
+                            int m1, int m2, float *__restrict__ wcf = nullptr) {
     const int J = gr_.bx() * blockDim.x + threadIdx.x - 1;
     const int I = gr_.by() * blockDim.y + threadIdx.y - 1;
     if (I > m1 || J > m2) return;
@@ -320,11 +322,13 @@ __device__ __forceinline__ void coarsen_w_body(const G &gr_, const double *__res
                 cnt++;
             }
         }
-    wc[(long)I * ldc + J] = s / cnt;
+    const double v = s / cnt;
+    wc[(long)I * ldc + J] = v;
+    if (wcf && I >= 0 && I < m1 && J >= 0 && J < m2) wcf[(long)I * ldc + J] = (float)v;
 }
 __global__ void k_coarsen_w(const double *__restrict__ wf, int ldf, int n1, int n2, double *__restrict__ wc, int ldc,
-                            int m1, int m2) {
-    coarsen_w_body(HwGrid{}, wf, ldf, n1, n2, wc, ldc, m1, m2);
+                            int m1, int m2, float *__restrict__ wcf) {
+    coarsen_w_body(HwGrid{}, wf, ldf, n1, n2, wc, ldc, m1, m2, wcf);
 }
 
 // coarsest level (at most 2 x 2 cells): exact solve of the singular Neumann
@@ -833,8 +837,9 @@ static int red_grid(long n) {
     return (int)b;
 }
 
-// coarse-level mobility quantities for the current fine w
-static int build_hierarchy(impec_ctx *ctx) {
+// coarse-level mobility quantities for the current fine w (f32: with the FP32
+// copies of the mixed-precision cycle, whose arrays exist)
+static int build_hierarchy(impec_ctx *ctx, bool f32 = false) {
     dim3 blk(64, 4);
     for (int l = 1; l < ctx->nlev; l++) {
         const Level &F = ctx->lev[l - 1];
@@ -849,7 +854,7 @@ static int build_hierarchy(impec_ctx *ctx) {
             ldf = F.ld;
         }
         hipLaunchKernelGGL(k_coarsen_w, grid2(Cl.n1 + 2, Cl.n2 + 2, blk), blk, 0, ctx->stream, wf, ldf, F.n1, F.n2,
-                           Cl.w + Cl.ld + 1, Cl.ld, Cl.n1, Cl.n2);
+                           Cl.w + Cl.ld + 1, Cl.ld, Cl.n1, Cl.n2, f32 ? Cl.wf + Cl.ld + 1 : nullptr);
     }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
@@ -2135,9 +2140,8 @@ __global__ void k_to_f32(const double *__restrict__ src, int lds, float *__restr
     to_f32_body(HwGrid{}, src, lds, dst, ldd, rows, cols);
 }
 
-// FP32 level arrays of the mixed-precision cycle (first use) and the FP32
-// copies of this solve's mobility hierarchy
-static int mixed_setup(impec_ctx *ctx) {
+// FP32 level arrays of the mixed-precision cycle (first use)
+static int mixed_alloc(impec_ctx *ctx) {
     const bool aniso = ctx->K22 && ctx->K22 != ctx->K11;
     for (int l = 0; l < ctx->nlev; l++) {
         Level &L = ctx->lev[l];
@@ -2151,8 +2155,14 @@ static int mixed_setup(impec_ctx *ctx) {
             hipMemset(*arrs[k], 0, bytes);
         }
     }
+    return IMPEC_OK;
+}
+
+// FP32 copies of levels [0, lend) of this solve's mobility hierarchy
+static int mixed_convert(impec_ctx *ctx, int lend) {
+    const bool aniso = ctx->K22 && ctx->K22 != ctx->K11;
     dim3 blk(64, 4);
-    for (int l = 0; l < ctx->nlev; l++) {
+    for (int l = 0; l < lend; l++) {
         Level &L = ctx->lev[l];
         LevArgs la = lev_args(ctx, l);
         hipLaunchKernelGGL(k_to_f32, grid2(L.n1, L.n2, blk), blk, 0, ctx->stream, la.w1, la.ldw, L.wf + L.ld + 1,
@@ -2163,6 +2173,12 @@ static int mixed_setup(impec_ctx *ctx) {
     }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
+}
+
+// FP32 level arrays and the FP32 copies of the whole hierarchy
+static int mixed_setup(impec_ctx *ctx) {
+    const int st = mixed_alloc(ctx);
+    return st ? st : mixed_convert(ctx, ctx->nlev);
 }
 
 // the FP32-vector CG's arrays (first use) and this solve's FP32 face copies
@@ -2365,11 +2381,14 @@ static int cg_reliable32(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, int
 // it when the recursion has drifted.
 template <bool DIAG, bool FUSED, typename T = double>
 static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
+    constexpr bool F32 = std::is_same<T, float>::value;
     int st = launch_mobility(ctx);
     if (st) return st;
-    st = build_hierarchy(ctx);
+    // mixed precision: the coarsening writes the FP32 copies of levels >= 1
+    if (F32 && (st = mixed_alloc(ctx))) return st;
+    st = build_hierarchy(ctx, F32);
     if (st) return st;
-    if (std::is_same<T, float>::value && (st = mixed_setup(ctx))) return st;
+    if (F32 && (st = mixed_convert(ctx, 1))) return st;
     // the FP32-vector CG with reliable FP64 updates (streaming 5-point path)
     const bool m32 = std::is_same<T, float>::value && FUSED && !DIAG && ctx->P.precision == IMPEC_PREC_MIXED32;
     const Level &L0 = ctx->lev[0];
