@@ -407,10 +407,14 @@ __device__ __forceinline__ void cg_cell(const CgArgs &c, long id, int &i, int &j
     j = (int)(id - (long)i * c.n2);
 }
 
-// r = b - A_N x with b = -q; pin handling; partials (r.r, sum r)
+// r = b - A_N x with b = -q; pin handling; partials (r.r, sum r).  With r32
+// (the FP32-vector CG's restart) the residual is stored rounded to FP32 in r32
+// instead of r, the correction e is cleared, and the partials are those of the
+// FP64 residual.
 template <bool DIAG, class G>
 __device__ __forceinline__ void cg_init_body(const G &gr_, LevArgs L, CgArgs c, double *__restrict__ p, const double *__restrict__ q,
-                          double *__restrict__ r, double *__restrict__ part) {
+                          double *__restrict__ r, double *__restrict__ part, float *__restrict__ r32 = nullptr,
+                          float *__restrict__ e32 = nullptr) {
     double s[2] = {0., 0.};
     double *x = p + (long)c.B * c.ldp + c.B;
     for (long id = (long)gr_.bx() * blockDim.x + threadIdx.x; id < c.n; id += (long)gr_.gx() * blockDim.x) {
@@ -440,7 +444,13 @@ __device__ __forceinline__ void cg_init_body(const G &gr_, LevArgs L, CgArgs c, 
             rv = bij - row_dot<DIAG>(a, x, c.ldp, i, j, c.n1, c.n2);
             if (pin) rv -= c.bsum;
         }
-        r[(long)(i + 1) * c.ld + (j + 1)] = rv;
+        const long o = (long)(i + 1) * c.ld + (j + 1);
+        if (r32) {
+            r32[o] = (float)rv;
+            e32[o] = 0.f;
+        } else {
+            r[o] = rv;
+        }
         s[0] += rv * rv;
         s[1] += rv;
     }
@@ -454,6 +464,11 @@ template <bool DIAG>
 __global__ void k_cg_init(LevArgs L, CgArgs c, double *__restrict__ p, const double *__restrict__ q,
                           double *__restrict__ r, double *__restrict__ part) {
     cg_init_body<DIAG>(HwGrid{}, L, c, p, q, r, part);
+}
+// the FP32-vector CG's restart from the iterate: k_cg_init into r32, e cleared
+__global__ void k_cg_init32(LevArgs L, CgArgs c, double *__restrict__ p, const double *__restrict__ q,
+                            float *__restrict__ r32, float *__restrict__ e32, double *__restrict__ part) {
+    cg_init_body<false>(HwGrid{}, L, c, p, q, nullptr, part, r32, e32);
 }
 
 // k_cg_init of the shift formulation from the zero iterate (zero initial
@@ -2349,19 +2364,29 @@ static int cg_iteration_mixed32(impec_ctx *ctx, const LevArgs &a0, const CgArgs 
     return IMPEC_OK;
 }
 
-// a reliable update of the FP32-vector CG: p += e in FP64, the FP64 residual
-// of the current formulation recomputed (||r||^2 into *rr), and the FP32
-// recursion restarted from it; the direction is kept
-template <bool DIAG>
+// the FP32-vector CG's (re)start from the iterate in p: the FP64 residual of
+// the shift formulation (||r||^2 into *rr) stored rounded in r32, e cleared
+static int cg_residual32(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, int nb, double *rr) {
+    prof_begin(ctx);
+    hipLaunchKernelGGL(k_cg_init32, dim3(nb), dim3(256), 0, ctx->stream, a0, c, ctx->p, ctx->q, ctx->pr32, ctx->pe32,
+                       ctx->part);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    int st = finish_sum(ctx, nb, 2, S_RR);
+    prof_end(ctx, "cg_init");
+    if (st || (st = fetch_scal(ctx))) return st;
+    *rr = ctx->h_scal[S_RR];
+    return IMPEC_OK;
+}
+
+// a reliable update of the FP32-vector CG: p += e in FP64, then the restart
+// from the FP64 residual; the direction is kept
 static int cg_reliable32(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, int nb, double *rr) {
     const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
     prof_begin(ctx);
     hipLaunchKernelGGL(k_cg_fold32, vg.grid, vg.block, 0, ctx->stream, c, ctx->p, ctx->pe32);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     prof_end(ctx, "cg_reliable");
-    int st = cg_residual<DIAG>(ctx, a0, c, nb, rr);
-    if (st) return st;
-    return cg_restart32(ctx, c);
+    return cg_residual32(ctx, a0, c, nb, rr);
 }
 
 // a reliable update falls due when the FP32 residual has dropped by this factor
@@ -2460,13 +2485,13 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
             prof_end(ctx, "cg_init");
             if (st || (st = fetch_scal(ctx))) return st;
             rr = ctx->h_scal[S_RR];
+            if (ru && (st = cg_restart32(ctx, c))) return st;
+        } else if (ru) {
+            if ((st = cg_residual32(ctx, a0, c, nb, &rr))) return st;
         } else if ((st = cg_residual<DIAG>(ctx, a0, c, nb, &rr))) {
             return st;
         }
-        if (ru) {
-            if ((st = cg_restart32(ctx, c))) return st;
-            rr_ref = rr;
-        }
+        rr_ref = rr;
         // p to the reference's pinned system (the shift, or the pinned value) and
         // that system's true relative residual
         auto finish_phase = [&]() -> int {
@@ -2505,7 +2530,7 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
 #ifdef IMPEC_RU_TRACE
                     const double rr32 = rr;
 #endif
-                    if ((st = cg_reliable32<DIAG>(ctx, a0, c, nb, &rr))) return st;
+                    if ((st = cg_reliable32(ctx, a0, c, nb, &rr))) return st;
 #ifdef IMPEC_RU_TRACE
                     if (its < 200)
                         fprintf(stderr, "ru phase %d its %d rr32/bb %.3e rr64/bb %.3e ref/bb %.3e\n", phase, its,
