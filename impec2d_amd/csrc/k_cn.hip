@@ -219,21 +219,24 @@ struct CnArgs {
 
 // the folded 9-point operator of row q (coefficients a) applied at column j
 // to y, whose rows q-1, q, q+1 are Y0, Y1, Y2 (column neighbours from the
-// adjacent lanes; every lane of the wave executes it)
+// adjacent lanes; every lane of the wave executes it).  No boundary masks:
+// the fold zeroes every coefficient that points outside the grid (corners and
+// edges, MGSolver.cpp:1033-1296), and y is zero outside, so those terms add
+// +0 -- interior results are those of the masked sum bit for bit.
 __device__ __forceinline__ double cn_apply(const double a[9], double Y0, double Y1, double Y2, int q, int j, int N1,
                                            int N2) {
+    (void)q; (void)j; (void)N1; (void)N2;
     const double yl0 = lane_prev(Y0), yl1 = lane_prev(Y1), yl2 = lane_prev(Y2);
     const double yr0 = lane_next(Y0), yr1 = lane_next(Y1), yr2 = lane_next(Y2);
-    const bool we = q > 0, ea = q < N1 - 1, so = j > 0, no = j < N2 - 1;
     double acc = a[ST_C] * Y1;
-    if (we) acc += a[ST_W] * Y0;
-    if (we && so) acc += a[ST_SW] * yl0;
-    if (so) acc += a[ST_S] * yl1;
-    if (ea && so) acc += a[ST_SE] * yl2;
-    if (ea) acc += a[ST_E] * Y2;
-    if (ea && no) acc += a[ST_NE] * yr2;
-    if (no) acc += a[ST_N] * yr1;
-    if (we && no) acc += a[ST_NW] * yr0;
+    acc += a[ST_W] * Y0;
+    acc += a[ST_SW] * yl0;
+    acc += a[ST_S] * yl1;
+    acc += a[ST_SE] * yl2;
+    acc += a[ST_E] * Y2;
+    acc += a[ST_NE] * yr2;
+    acc += a[ST_N] * yr1;
+    acc += a[ST_NW] * yr0;
     return acc;
 }
 
