@@ -164,7 +164,9 @@ __device__ __forceinline__ void vertex_w(const VelConst &k, const double *__rest
 #undef W2
 }
 
-template <class G>
+// PAR: nu = 1, the diagonal-grid term (weight 1 - nu = 0) is not formed; the
+// stored value nu * par equals nu * par + 0 * dg for every finite dg
+template <class G, bool PAR = false>
 __device__ __forceinline__ void velocity_body(const G &gr_, VelConst k, const double *__restrict__ p, const double *__restrict__ w11,
                            const double *__restrict__ w22, double *__restrict__ u, double *__restrict__ v) {
     const int j = gr_.bx() * blockDim.x + threadIdx.x;
@@ -174,37 +176,46 @@ __device__ __forceinline__ void velocity_body(const G &gr_, VelConst k, const do
 #define P_(i, j) p[(size_t)((i) + B) * ld + ((j) + B)]
 #define W1(i, j) w11[(size_t)((i) + B) * ld + ((j) + B)]
 #define W2(i, j) w22[(size_t)((i) + B) * ld + ((j) + B)]
-    double w1m, w2m;  // vertex (i-1/2, j-1/2), shared by u and v
-    vertex_w(k, p, w11, w22, i - 1, j - 1, w1m, w2m);
+    double w1m = 0., w2m = 0.;  // vertex (i-1/2, j-1/2), shared by u and v
+    if (!PAR) vertex_w(k, p, w11, w22, i - 1, j - 1, w1m, w2m);
     if (j < k.N2) {
         // GetParallelU1_im12j + GetDiagonalU1_im12j
         const double lam = face_mob(W1(i - 1, j), W1(i, j), k.harmonic);
         const double par = -lam * (P_(i, j) - P_(i - 1, j)) / k.dx;
+        if (PAR) {
+            u[(size_t)(i + B) * k.ldv + (j + B)] = k.nu * par;
+        } else {
         double w1p, w2p;
         vertex_w(k, p, w11, w22, i - 1, j, w1p, w2p);
         const double u1m = k.cost * w1m - k.sina * w2m;
         const double u1p = k.cost * w1p - k.sina * w2p;
         const double dg = 0.5 * (u1p + u1m);
         u[(size_t)(i + B) * k.ldv + (j + B)] = k.nu * par + k.one_m_nu * dg;
+        }
     }
     if (i < k.N1) {
         // GetParallelU2_ijm12 + GetDiagonalU2_ijm12
         const double lam = face_mob(W2(i, j - 1), W2(i, j), k.harmonic);
         const double par = -lam * (P_(i, j) - P_(i, j - 1)) / k.dy;
+        if (PAR) {
+            v[(size_t)(i + B) * k.ldv + (j + B)] = k.nu * par;
+        } else {
         double w1p, w2p;
         vertex_w(k, p, w11, w22, i, j - 1, w1p, w2p);
         const double u2m = k.sint * w1m + k.cosa * w2m;
         const double u2p = k.sint * w1p + k.cosa * w2p;
         const double dg = 0.5 * (u2m + u2p);
         v[(size_t)(i + B) * k.ldv + (j + B)] = k.nu * par + k.one_m_nu * dg;
+        }
     }
 #undef P_
 #undef W1
 #undef W2
 }
+template <bool PAR>
 __global__ void k_velocity(VelConst k, const double *__restrict__ p, const double *__restrict__ w11,
                            const double *__restrict__ w22, double *__restrict__ u, double *__restrict__ v) {
-    velocity_body(HwGrid{}, k, p, w11, w22, u, v);
+    velocity_body<HwGrid, PAR>(HwGrid{}, k, p, w11, w22, u, v);
 }
 
 int launch_velocity(impec_ctx *ctx) {
@@ -223,7 +234,9 @@ int launch_velocity(impec_ctx *ctx) {
     const double *w22 = ctx->K22 && ctx->K22 != ctx->K11 ? ctx->w22 : ctx->w;
     dim3 blk(64, 4), grd((ctx->N2 + 1 + 63) / 64, (ctx->N1 + 1 + 3) / 4);
     prof_begin(ctx);
-    hipLaunchKernelGGL(k_velocity, grd, blk, 0, ctx->stream, k, ctx->p, ctx->w, w22, ctx->u, ctx->v);
+    if (ctx->P.nu == 1.)
+        hipLaunchKernelGGL(k_velocity<true>, grd, blk, 0, ctx->stream, k, ctx->p, ctx->w, w22, ctx->u, ctx->v);
+    else hipLaunchKernelGGL(k_velocity<false>, grd, blk, 0, ctx->stream, k, ctx->p, ctx->w, w22, ctx->u, ctx->v);
     prof_end(ctx, "velocity");
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return launch_fill_velocity(ctx);
@@ -614,9 +627,11 @@ __global__ void kb_mobility(const MobRec *__restrict__ tab, const int *__restric
         mobility_body(G, R.c, R.K11, R.K22, R.w, R.w22, R.rows, R.cols, R.ld, R.mob1over4, R.mu_res, R.harmonic);
     });
 }
+// ensemble members are nu = 1 (checked at creation): the single-context
+// kernel's nu = 1 path
 __global__ void kb_velocity(const VelRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
     batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
-        velocity_body(G, R.k, R.p, R.w11, R.w22, R.u, R.v);
+        velocity_body<VGrid, true>(G, R.k, R.p, R.w11, R.w22, R.u, R.v);
     });
 }
 __global__ void kb_fill_velocity(const FillVRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
