@@ -200,6 +200,11 @@ enum { CN_S = 1, CN_P0 = 2 };
 #ifndef IMPEC_CN_TARGET_WAVES
 #define IMPEC_CN_TARGET_WAVES 4096
 #endif
+// build-time tuning: minimum waves per SIMD requested for the p-launch (1 lets
+// the compiler use the registers it wants: 144, three waves per SIMD)
+#ifndef IMPEC_CNP_MIN_WAVES
+#define IMPEC_CNP_MIN_WAVES 1
+#endif
 
 struct CnArgs {
     CnConst k;
@@ -389,7 +394,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
 // and two lanes: lanes 2..61 of a strip are written, rows i0-2 .. i1+1 are
 // streamed.  r' goes to a buffer other than r (neighbouring waves read r in
 // their halos); x is read and written by its own lane only.
-__global__ void __launch_bounds__(64 * CN_WAVES) k_cn_pstep(CnArgs A) {
+__global__ void __launch_bounds__(64 * CN_WAVES, IMPEC_CNP_MIN_WAVES) k_cn_pstep(CnArgs A) {
     const int lane = threadIdx.x & 63;
     const int wid = xcd_block(blockIdx.x, gridDim.x) * CN_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wid >= A.nwave) return;
