@@ -222,15 +222,13 @@ struct CnArgs {
     int xzero;                             // p-step: x is zero (the first update of a zero-guess solve)
 };
 
-// the folded 9-point operator of row q (coefficients a) applied at column j
-// to y, whose rows q-1, q, q+1 are Y0, Y1, Y2 (column neighbours from the
+// the folded 9-point operator of row i (coefficients a) applied to y, whose
+// rows i-1, i, i+1 are Y0, Y1, Y2 (column neighbours from the
 // adjacent lanes; every lane of the wave executes it).  No boundary masks:
 // the fold zeroes every coefficient that points outside the grid (corners and
 // edges, MGSolver.cpp:1033-1296), and y is zero outside, so those terms add
 // +0 -- interior results are those of the masked sum bit for bit.
-__device__ __forceinline__ double cn_apply(const double a[9], double Y0, double Y1, double Y2, int q, int j, int N1,
-                                           int N2) {
-    (void)q; (void)j; (void)N1; (void)N2;
+__device__ __forceinline__ double cn_apply(const double a[9], double Y0, double Y1, double Y2) {
     const double yl0 = lane_prev(Y0), yl1 = lane_prev(Y1), yl2 = lane_prev(Y2);
     const double yr0 = lane_next(Y0), yr1 = lane_next(Y1), yr2 = lane_next(Y2);
     double acc = a[ST_C] * Y1;
@@ -352,7 +350,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
             if (q >= i0 && q < i1) {
                 double a[9];
                 cn_coeffs(k, T, q, j, a);
-                const double acc = cn_apply(a, Y0, Y1, Y2, q, j, N1, N2);
+                const double acc = cn_apply(a, Y0, Y1, Y2);
                 if (outl) {
                     if (MODE == CN_S) {
                         s0 += acc * wprev;
@@ -467,7 +465,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES, IMPEC_CNP_MIN_WAVES) k_cn_pstep
             const bool qin = q >= 0 && q < N1;
             double a[9];
             cn_coeffs(k, fs.cell(), q, j, a);
-            const double t = cn_apply(a, YS0, YS1, YS2, q, j, N1, N2);
+            const double t = cn_apply(a, YS0, YS1, YS2);
             const double rq = sprev - om * t;
             const double pq = rq + beta * (Pp[u] - om * vprev);
             if (qin && q >= i0 && q < i1 && outl) {
@@ -482,7 +480,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES, IMPEC_CNP_MIN_WAVES) k_cn_pstep
             // row q2 = m - 2: v' = A (p'/D)
             const int q2 = m - 2;
             if (q2 >= i0 && q2 < i1) {
-                const double acc = cn_apply(aprev, YP0, YP1, YP2, q2, j, N1, N2);
+                const double acc = cn_apply(aprev, YP0, YP1, YP2);
                 if (outl) {
                     A.out[(long)q2 * N2 + j] = acc;
                     s0 += P0[u] * acc;
@@ -578,7 +576,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_start(CnArgs A, const doub
             // row q2 = m - 2: v = A (b/D)
             const int q2 = m - 2;
             if (q2 >= i0 && q2 < i1) {
-                const double acc = cn_apply(aprev, Y0, Y1, Y2, q2, j, N1, N2);
+                const double acc = cn_apply(aprev, Y0, Y1, Y2);
                 if (outl) {
                     A.out[(long)q2 * N2 + j] = acc;
                     s0 += bprev * acc;
