@@ -73,7 +73,16 @@ __device__ __forceinline__ double strided_combine(const double *__restrict__ p, 
 #pragma unroll
         for (int u = 0; u < 8; u++) acc[u] = red_combine<OP>(acc[u], v[u]);
     }
-    for (; b < n; b += T) acc[0] = red_combine<OP>(acc[0], p[b]);
+    // the rest (fewer than eight strides): loads issued together, combined in
+    // order into acc[0]; a missing one is the identity, which leaves the
+    // combination unchanged
+    {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = b + u * T < n ? p[b + u * T] : red_identity<OP>();
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc[0] = red_combine<OP>(acc[0], v[u]);
+    }
 #pragma unroll
     for (int u = 1; u < 8; u++) acc[0] = red_combine<OP>(acc[0], acc[u]);
     return acc[0];
