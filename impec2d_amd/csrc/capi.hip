@@ -212,6 +212,7 @@ int impec::ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out) {
     if (!st) st = dalloc(ctx, &ctx->q, nq);
     if (!st) st = dalloc(ctx, &ctx->scal, S_COUNT);
     if (!st) st = dalloc(ctx, &ctx->part, (size_t)RED_BLOCKS * PART_VECS);
+    if (!st) st = dalloc(ctx, &ctx->part_mid, (size_t)FIN_THREADS * PART_VECS);
     if (!st && hipHostMalloc((void **)&ctx->h_scal, sizeof(double) * S_COUNT, hipHostMallocDefault) != hipSuccess)
         st = set_error(ctx, IMPEC_ENOMEM, "pinned host scalars");
     if (!st) st = pressure_setup_levels(ctx);
@@ -236,7 +237,7 @@ int impec_destroy(impec_ctx *ctx) {
     if (!ctx) return IMPEC_OK;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     double *bufs[] = {ctx->c, ctx->p, ctx->u, ctx->v, ctx->q, ctx->K11, ctx->K22 != ctx->K11 ? ctx->K22 : nullptr,
-                      ctx->w, ctx->w22, ctx->t1, ctx->t2, ctx->t3, ctx->scal, ctx->part,
+                      ctx->w, ctx->w22, ctx->t1, ctx->t2, ctx->t3, ctx->scal, ctx->part, ctx->part_mid,
                       ctx->cn_rhs, ctx->cn_r, ctx->cn_r0, ctx->cn_p, ctx->cn_v,
                       ctx->cn_r2, ctx->cn_y, ctx->cn_z, ctx->src_q, ctx->tr_md, ctx->p_prev,
                       ctx->p_old};

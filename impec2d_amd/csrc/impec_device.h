@@ -56,16 +56,27 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV]) {
     }
 }
 
-// this thread's share of a combine over n partials p[0..n) read with stride
-// blockDim.x, eight independent loads in flight (a fixed order: the result
-// does not depend on timing)
-template <int OP>
-__device__ __forceinline__ double strided_combine(const double *__restrict__ p, int n) {
-    const int T = blockDim.x;
+// the share of thread t of T threads in a combine over n partials p[0..n)
+// read with stride T, eight independent loads in flight (a fixed order: the
+// result does not depend on timing)
+// DEPTH: strides of eight loaded before any is combined (more loads in flight,
+// the same combination order)
+template <int OP, int DEPTH = 1>
+__device__ __forceinline__ double strided_combine_at(const double *__restrict__ p, int n, int t, int T) {
     double acc[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) acc[u] = red_identity<OP>();
-    int b = threadIdx.x;
+    int b = t;
+    if (DEPTH > 1)
+        for (; b + (8 * DEPTH - 1) * T < n; b += 8 * DEPTH * T) {
+            double v[8 * DEPTH];
+#pragma unroll
+            for (int u = 0; u < 8 * DEPTH; u++) v[u] = p[b + u * T];
+#pragma unroll
+            for (int q = 0; q < DEPTH; q++)
+#pragma unroll
+                for (int u = 0; u < 8; u++) acc[u] = red_combine<OP>(acc[u], v[8 * q + u]);
+        }
     for (; b + 7 * T < n; b += 8 * T) {
         double v[8];
 #pragma unroll
@@ -86,6 +97,11 @@ __device__ __forceinline__ double strided_combine(const double *__restrict__ p, 
 #pragma unroll
     for (int u = 1; u < 8; u++) acc[0] = red_combine<OP>(acc[0], acc[u]);
     return acc[0];
+}
+// the share of thread threadIdx.x of blockDim.x threads
+template <int OP>
+__device__ __forceinline__ double strided_combine(const double *__restrict__ p, int n) {
+    return strided_combine_at<OP>(p, n, threadIdx.x, blockDim.x);
 }
 
 // mu(c) of EllipticSolver.cpp:692-695

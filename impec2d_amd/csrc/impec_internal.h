@@ -13,6 +13,7 @@ constexpr int GHOST = 2;          // num_ghostcells of every shipped config
 constexpr int MAX_LEVELS = 24;
 constexpr int RED_BLOCKS = 65536; // max partial sums per reduction pass
 constexpr int PART_VECS = 6;      // partial-sum vectors (stride RED_BLOCKS) in a context's part buffer
+constexpr int FIN_THREADS = 1024; // threads of the one-workgroup combine of partials
 #ifndef IMPEC_VEC_BLOCKS
 #define IMPEC_VEC_BLOCKS 65536
 #endif
@@ -111,6 +112,7 @@ struct impec_ctx {
     float *pr32, *pe32, *pd32, *pz32, *pq32, *fx32, *fy32;
     double *scal;              // device scalars (Scal)
     double *part;              // reduction partials
+    double *part_mid;          // per-thread shares of a two-stage finish (FIN_THREADS x PART_VECS)
     double *h_scal;            // pinned host mirror of scal
     int symmetric;             // shift formulation usable
     int force_unfused;         // use the unfused multigrid kernels (testing)

@@ -263,9 +263,42 @@ __global__ void k_finish(const double *__restrict__ part, int nblocks, int nv, d
     finish_body<OP>(HwGrid{}, part, nblocks, nv, scal, dst);
 }
 
+// The same combine in two launches, for many partials: one workgroup would
+// read them all through one compute unit (about 24 us for 2 x 65536).  The
+// first launch computes the FIN_THREADS per-thread shares of k_finish
+// (strided_combine_at with the same thread index and count, 64 loads in
+// flight) on 16 workgroups; the second runs k_finish's block reduction over
+// them, so the result is k_finish's bit for bit.
+constexpr int FIN_STAGE_WG = 64;
+template <int OP>
+__global__ void __launch_bounds__(FIN_STAGE_WG) k_finish_shares(const double *__restrict__ part, int nblocks, int nv,
+                                                               double *__restrict__ mid) {
+    const int t = blockIdx.x * FIN_STAGE_WG + threadIdx.x;
+    for (int k = 0; k < nv; k++)
+        mid[k * FIN_THREADS + t] = strided_combine_at<OP, 8>(part + (long)k * RED_BLOCKS, nblocks, t, FIN_THREADS);
+}
+template <int OP>
+__global__ void __launch_bounds__(FIN_THREADS) k_finish_reduce(const double *__restrict__ mid, int nv,
+                                                              double *__restrict__ scal, int dst) {
+    for (int k = 0; k < nv; k++) {
+        double v[1] = {mid[k * FIN_THREADS + threadIdx.x]};
+        block_reduce<OP, 1>(v);
+        if (threadIdx.x == 0) scal[dst + k] = v[0];
+        __syncthreads();
+    }
+}
+
 template <int OP>
 static int finish_op(impec_ctx *ctx, int nblocks, int nv, int dst) {
-    hipLaunchKernelGGL(k_finish<OP>, dim3(1), dim3(1024), 0, ctx->stream, ctx->part, nblocks, nv, ctx->scal, dst);
+    if (nblocks > 8 * FIN_THREADS && nv <= PART_VECS) {
+        hipLaunchKernelGGL(k_finish_shares<OP>, dim3(FIN_THREADS / FIN_STAGE_WG), dim3(FIN_STAGE_WG), 0, ctx->stream,
+                           ctx->part, nblocks, nv, ctx->part_mid);
+        hipLaunchKernelGGL(k_finish_reduce<OP>, dim3(1), dim3(FIN_THREADS), 0, ctx->stream, ctx->part_mid, nv,
+                           ctx->scal, dst);
+    } else {
+        hipLaunchKernelGGL(k_finish<OP>, dim3(1), dim3(FIN_THREADS), 0, ctx->stream, ctx->part, nblocks, nv, ctx->scal,
+                           dst);
+    }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
 }

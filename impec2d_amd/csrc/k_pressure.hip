@@ -2326,38 +2326,6 @@ static int cg_iteration_fused(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c
     return IMPEC_OK;
 }
 
-// One stage of a combine of partials in a fixed order: workgroup g combines
-// its contiguous share of each of the nv partial vectors (stride RED_BLOCKS)
-// into out[k * ostride + g].
-__global__ void k_finish_stage(const double *__restrict__ part, int nblocks, int nv, double *__restrict__ out,
-                               long ostride) {
-    const int G = gridDim.x, g = blockIdx.x;
-    const int per = (nblocks + G - 1) / G;
-    const int lo = g * per < nblocks ? g * per : nblocks;
-    const int hi = lo + per < nblocks ? lo + per : nblocks;
-    for (int k = 0; k < nv; k++) {
-        double v[1] = {strided_combine<RED_SUM>(part + (long)k * RED_BLOCKS + lo, hi - lo)};
-        block_reduce<RED_SUM, 1>(v);
-        if (threadIdx.x == 0) out[(long)k * ostride + g] = v[0];
-        __syncthreads();
-    }
-}
-
-// finish_sum for many partials: one workgroup would read them all in a chain
-// of latencies (about 24 us for 2 x 65536); 64 workgroups, then one.  The
-// partials of vectors 2 .. 2+nv-1 of the buffer hold the intermediate sums.
-static int finish_sum_wide(impec_ctx *ctx, int nblocks, int nv, int dst) {
-    constexpr int G = 64;
-    static_assert(2 + 2 <= PART_VECS, "intermediate sums of two vectors");
-    if (nblocks <= 8 * 1024 || nv > 2) return finish_sum(ctx, nblocks, nv, dst);
-    double *mid = ctx->part + 2 * (long)RED_BLOCKS;
-    hipLaunchKernelGGL(k_finish_stage, dim3(G), dim3(1024), 0, ctx->stream, ctx->part, nblocks, nv, mid,
-                       (long)RED_BLOCKS);
-    hipLaunchKernelGGL(k_finish_stage, dim3(1), dim3(1024), 0, ctx->stream, mid, G, nv, ctx->scal + dst, 1L);
-    IMPEC_HIP_CHECK(ctx, hipGetLastError());
-    return IMPEC_OK;
-}
-
 // one iteration of the FP32-vector CG (IMPEC_PREC_MIXED32): the FP32 V-cycle
 // on the FP32 residual, the FP32 direction and matvec (FP32 faces), the FP32
 // updates of the residual and of the correction accumulated since the last
@@ -2388,7 +2356,7 @@ static int cg_iteration_mixed32(impec_ctx *ctx, const LevArgs &a0, const CgArgs 
     hipLaunchKernelGGL(k_cg_update32, vg.grid, vg.block, 0, s, c, ctx->pe32, *dnew, ctx->pr32, ctx->pq32, ctx->scal,
                        ctx->part);
     prof_end(ctx, "cg_update");
-    if ((st = finish_sum_wide(ctx, vg.nblocks, 2, S_RR))) return st;
+    if ((st = finish_sum(ctx, vg.nblocks, 2, S_RR))) return st;
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     float *t = *dold;
     *dold = *dnew;
@@ -2403,7 +2371,7 @@ static int cg_residual32(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, int
     hipLaunchKernelGGL(k_cg_init32, dim3(nb), dim3(256), 0, ctx->stream, a0, c, ctx->p, ctx->q, ctx->pr32, ctx->pe32,
                        ctx->part);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
-    int st = finish_sum_wide(ctx, nb, 2, S_RR);
+    int st = finish_sum(ctx, nb, 2, S_RR);
     prof_end(ctx, "cg_init");
     if (st || (st = fetch_scal(ctx))) return st;
     *rr = ctx->h_scal[S_RR];
