@@ -107,6 +107,29 @@ def test_members_equal_single_contexts_bitwise(gpu_lib, precision, guess):
             assert np.array_equal(c_e, c_s)
 
 
+def test_large_member_equal_single_context_bitwise(gpu_lib):
+    """A 2048^2 member leaves more partials than one workgroup combines quickly
+    (16384 after the CG update): the single context combines them in two
+    launches, the ensemble in one workgroup per member, and the two must agree
+    bit for bit."""
+    steps = 2
+    specs = [(2048, 2048, "par", 10.0, 1.0), (64, 48, "diag", 20.0, 1.0)]
+    _say("ensemble with a 2048^2 member vs single contexts")
+    ens, Ks = _make_ensemble(specs)
+    res = _ensemble_steps(ens, steps)
+    ens.close()
+    for k, (p, K) in enumerate(Ks):
+        _, _, single = _run_single(p, K, steps)
+        for s in range(steps):
+            dt_e, its_e, (c_e, p_e, v_e) = res[s][0][k], res[s][1][k], res[s][2][k]
+            dt_s, its_s, c_s, p_s, v_s = single[s]
+            _say("  member %d step %d: its %d/%d" % (k, s, its_e, its_s))
+            assert its_e == its_s and dt_e == dt_s
+            assert np.array_equal(p_e, p_s)
+            assert np.array_equal(v_e, v_s)
+            assert np.array_equal(c_e, c_s)
+
+
 def test_graph_replays_match_eager_launches(gpu_lib):
     """The same ensemble stepped with eager launches, with graphs of one CG
     iteration per replay and with graphs of four: identical fields after
