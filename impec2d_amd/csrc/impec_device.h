@@ -104,6 +104,47 @@ __device__ __forceinline__ double strided_combine(const double *__restrict__ p, 
     return strided_combine_at<OP>(p, n, threadIdx.x, blockDim.x);
 }
 
+// strided_combine of NV vectors p + q * vstride (q < NV) at once: each vector
+// combined in strided_combine's order, the loads of all NV issued together
+template <int OP, int NV>
+__device__ __forceinline__ void strided_combine_nv(const double *__restrict__ p, long vstride, int n, double (&out)[NV]) {
+    const int T = blockDim.x;
+    double acc[NV][8];
+#pragma unroll
+    for (int q = 0; q < NV; q++)
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc[q][u] = red_identity<OP>();
+    int b = threadIdx.x;
+    for (; b + 7 * T < n; b += 8 * T) {
+        double v[NV][8];
+#pragma unroll
+        for (int q = 0; q < NV; q++)
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[q][u] = p[q * vstride + b + u * T];
+#pragma unroll
+        for (int q = 0; q < NV; q++)
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc[q][u] = red_combine<OP>(acc[q][u], v[q][u]);
+    }
+    {
+        double v[NV][8];
+#pragma unroll
+        for (int q = 0; q < NV; q++)
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[q][u] = b + u * T < n ? p[q * vstride + b + u * T] : red_identity<OP>();
+#pragma unroll
+        for (int q = 0; q < NV; q++)
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc[q][0] = red_combine<OP>(acc[q][0], v[q][u]);
+    }
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+#pragma unroll
+        for (int u = 1; u < 8; u++) acc[q][0] = red_combine<OP>(acc[q][0], acc[q][u]);
+        out[q] = acc[q][0];
+    }
+}
+
 // mu(c) of EllipticSolver.cpp:692-695
 __device__ __forceinline__ double mu_of(double c, double mob1over4, double mu_res) {
     double x = (1. - c + mob1over4 * c);

@@ -667,8 +667,7 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
 template <int NV>
 __global__ void k_bicg_finish(const double *__restrict__ part, int nb, int stage, double *__restrict__ S) {
     double v[NV];
-#pragma unroll
-    for (int q = 0; q < NV; q++) v[q] = strided_combine<RED_SUM>(part + q * RED_BLOCKS, nb);
+    strided_combine_nv<RED_SUM, NV>(part, RED_BLOCKS, nb, v);
     block_reduce<RED_SUM, NV>(v);
     if (threadIdx.x != 0) return;
     if (stage == 0) {

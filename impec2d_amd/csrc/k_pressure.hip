@@ -1731,7 +1731,8 @@ static int top_level(const impec_ctx *ctx) {
 // sums of the V-cycle output and rho = r.z with z = x0 - mean(x0)
 template <class G>
 __device__ __forceinline__ void finish_rho_body(const G &gr_, const double *__restrict__ part, int nblocks, double inv_n, double *__restrict__ S) {
-    double v[2] = {strided_combine<RED_SUM>(part, nblocks), strided_combine<RED_SUM>(part + RED_BLOCKS, nblocks)};
+    double v[2];
+    strided_combine_nv<RED_SUM, 2>(part, RED_BLOCKS, nblocks, v);
     block_reduce<RED_SUM, 2>(v);
     if (threadIdx.x == 0) {
         S[S_ZSUM] = v[0];
