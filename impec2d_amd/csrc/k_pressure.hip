@@ -1155,6 +1155,8 @@ __device__ __forceinline__ void mask_pair(const Lane &L, int m, int n1, V fill, 
 #define IMPEC_SW_PF 4
 #endif
 constexpr int SW_PF = IMPEC_SW_PF;
+static_assert(SW_PF % 2 == 0, "the unrolled steps take the row parity (colour, prolongation neighbour) from the slot "
+                              "index, so the prefetch depth must be even");
 
 template <bool FINE0, bool ANISO, typename T, int NS = 1, bool R32 = false>
 __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int wid) {
