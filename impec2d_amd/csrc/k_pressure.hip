@@ -1986,6 +1986,7 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
 #ifndef IMPEC_MG_MINROWS
 #define IMPEC_MG_MINROWS 4
 #endif
+static_assert(IMPEC_MG_MINROWS % 2 == 0 && IMPEC_MG_MINROWS >= 2, "chunks of the streaming legs start on even rows");
 static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip) {
     if (l == 0 && ctx->P.mg_rows > 0) return (ctx->P.mg_rows + 1) & ~1;
     long r = ((long)n1 * nstrip + IMPEC_MG_WAVES - 1) / IMPEC_MG_WAVES;
