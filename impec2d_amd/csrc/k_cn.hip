@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
     FaceStream fs;
     fs.start(vel(A.u, mstart - 1), vel(A.v, mstart - 1));
     double wprev = 0., r0prev = 0.;
-    double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+    double s0 = 0., s1 = 0., s2 = 0., s3 = 0., s4 = 0.;
     for (int mb = mstart; mb < mstop; mb += PF) {
 #pragma unroll
         for (int u = 0; u < PF; u++) {
@@ -337,8 +337,10 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
             const double wv = MODE == CN_S ? Pa[u] - alpha * Pv[u] : Pa[u];
             const double yv = (rin && jv) ? wv * dv : 0.;
             if (rin && m >= i0 && m < i1 && outl) {
-                if (MODE == CN_S) s3 += r0v * wv;
-                else {
+                if (MODE == CN_S) {
+                    s3 += r0v * wv;
+                    s4 += wv * wv;
+                } else {
                     A.pout[(long)m * N2 + j] = wv;
                     s1 += wv * wv;
                 }
@@ -374,6 +376,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
         if (MODE == CN_S) {
             s2 += __shfl_xor(s2, o);
             s3 += __shfl_xor(s3, o);
+            s4 += __shfl_xor(s4, o);
         }
     }
     if (lane == 0) {
@@ -382,6 +385,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
         if (MODE == CN_S) {
             A.part[2 * RED_BLOCKS + wid] = s2;
             A.part[3 * RED_BLOCKS + wid] = s3;
+            A.part[4 * RED_BLOCKS + wid] = s4;
         }
     }
 }
@@ -599,7 +603,26 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_start(CnArgs A, const doub
     }
 }
 
-static_assert(4 <= PART_VECS, "the s-launch writes four partial vectors");
+static_assert(5 <= PART_VECS, "the s-launch writes five partial vectors");
+
+// The last iteration's x update alone (the p-launch's arithmetic): s = r -
+// alpha v, x += alpha p/D + omega s/D.  Once the s-launch's partials show the
+// next residual meets the tolerance, the direction and matvec of a full
+// p-launch would go unused.
+__global__ void k_cn_xfinal(CnArgs A) {
+    const int N1 = A.k.N1, N2 = A.k.N2;
+    const double alpha = A.S[S_ALPHA], om = A.S[S_OMEGA];
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    if (j >= N2) return;
+    for (int i = blockIdx.y * 4 + threadIdx.y; i < N1; i += gridDim.y * 4) {
+        const long o = (long)i * N2 + j;
+        const double dv = (double)A.dinv[o];
+        const double sm = A.r[o] - alpha * A.vin[o];
+        const double dx_ = alpha * (A.pin[o] * dv) + om * (sm * dv);
+        double *xp = A.x + (long)i * A.ldx + j;
+        *xp = A.xzero ? dx_ : *xp + dx_;
+    }
+}
 
 // launch shape of the element-wise BiCGStab kernels: 64 x 4 threads over
 // (column, row), rows strided so that there are at most CN_VBLOCKS blocks
@@ -662,10 +685,15 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
 //  stage 4 (guess):    rr = rho_new = r0.r0 of the initial residual
 //  stage 5 (start):    b.v, b.b of a zero-guess start: stages 0 and 1 at once
 //  stage 1 (p-launch): r0.v, r.r; rho = rho_new; alpha = rho / r0.v; rr = r.r
-//  stage 2 (s-launch): t.s, t.t, r0.t, r0.s; omega = t.s / t.t (0 when t = 0);
-//                      rho_new = r0.r of r = s - omega t, as r0.s - omega r0.t
+//  stage 2 (s-launch): t.s, t.t, r0.t, r0.s, s.s; omega = t.s / t.t (0 when
+//                      t = 0); rho_new = r0.r of r = s - omega t, as
+//                      r0.s - omega r0.t; rr = r.r as s.s - 2 omega t.s + omega^2 t.t
+// launched with BICG_FIN_THREADS threads: one wave per SIMD, so the loads of
+// all NV vectors fit in registers
+constexpr int BICG_FIN_THREADS = 256;
 template <int NV>
-__global__ void k_bicg_finish(const double *__restrict__ part, int nb, int stage, double *__restrict__ S) {
+__global__ void __launch_bounds__(BICG_FIN_THREADS) k_bicg_finish(const double *__restrict__ part, int nb, int stage,
+                                                                  double *__restrict__ S) {
     double v[NV];
     strided_combine_nv<RED_SUM, NV>(part, RED_BLOCKS, nb, v);
     block_reduce<RED_SUM, NV>(v);
@@ -691,12 +719,13 @@ __global__ void k_bicg_finish(const double *__restrict__ part, int nb, int stage
         S[S_RHO] = S[S_RHO_NEW];
         S[S_ALPHA] = S[S_RHO] / v[0];
         S[S_RR] = v[1];
-    } else if (NV >= 4) {
+    } else if (NV >= 5) {
         S[S_TS] = v[0];
         S[S_TT] = v[1];
         const double om = v[1] > 0 ? v[0] / v[1] : 0.;
         S[S_OMEGA] = om;
-        S[S_RHO_NEW] = v[NV - 1] - om * v[NV - 2];
+        S[S_RHO_NEW] = v[3] - om * v[2];
+        S[S_RR] = v[4] - 2. * om * v[0] + om * om * v[1];
     }
 }
 
@@ -819,7 +848,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         prof_begin(ctx);
         hipLaunchKernelGGL(k_cn_start, pg, sblk, 0, s, az, ctx->c, ctx->lds, b, ctx->cn_dg);
         prof_end(ctx, "cn_start");
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, a2.nwave, 5, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a2.nwave, 5, ctx->scal);
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = fetch(ctx))) return st;
         bb = ctx->h_scal[S_BB];
@@ -834,9 +863,9 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         prof_end(ctx, "cn_rhs");
         const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
         const int nvb = (int)(vg.x * vg.y);
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nrb, 0, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nrb, 0, ctx->scal);
         hipLaunchKernelGGL(k_bicg_guess, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, r, r0, ctx->part);
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, nvb, 4, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nvb, 4, ctx->scal);
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = fetch(ctx))) return st;
         bb = ctx->h_scal[S_BB];
@@ -847,7 +876,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             prof_begin(ctx);
             hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, sblk, 0, s, ap);
             prof_end(ctx, "cn_apply_p0");
-            hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 1, ctx->scal);
+            hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 1, ctx->scal);
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
             cur = 1;
         }
@@ -864,22 +893,32 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         prof_begin(ctx);
         hipLaunchKernelGGL(k_cn_stream<CN_S>, sg, sblk, 0, s, as);
         prof_end(ctx, "cn_apply_s");
-        hipLaunchKernelGGL(k_bicg_finish<4>, dim3(1), dim3(1024), 0, s, ctx->part, a.nwave, 2, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<5>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 2, ctx->scal);
+        IMPEC_HIP_CHECK(ctx, hipGetLastError());
+        // the next residual's norm from the s-launch: the last iteration updates x only
+        if ((st = fetch(ctx))) return st;
+        rr = ctx->h_scal[S_RR];
         CnArgs ap = a2;
         ap.r = rcur; ap.pin = its == 0 ? pfirst : pb[cur]; ap.vin = vb[cur]; ap.x = x;
+        ap.xzero = zero && its == 0;
+        if (rr <= tol * tol * bb || !(rr == rr)) {
+            prof_begin(ctx);
+            hipLaunchKernelGGL(k_cn_xfinal, cn_vgrid(N1, N2), dim3(64, 4), 0, s, ap);
+            prof_end(ctx, "cn_xfinal");
+            IMPEC_HIP_CHECK(ctx, hipGetLastError());
+            its++;
+            continue;   // the loop test ends the solve (or reports the non-finite residual)
+        }
         if (rb[rnext] == rcur) rnext ^= 1;   // r' never overwrites the r it is formed from
         ap.rout = rb[rnext];
-        ap.xzero = zero && its == 0;
         ap.pout = pb[cur ^ 1]; ap.out = vb[cur ^ 1];
         prof_begin(ctx);
         hipLaunchKernelGGL(k_cn_pstep, pg, sblk, 0, s, ap);
         prof_end(ctx, "cn_apply_p");
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(1024), 0, s, ctx->part, a2.nwave, 1, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a2.nwave, 1, ctx->scal);
         rcur = rb[rnext];
         rnext ^= 1;
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
-        if ((st = fetch(ctx))) return st;
-        rr = ctx->h_scal[S_RR];
         cur ^= 1;
         its++;
     }
@@ -887,7 +926,8 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         hipLaunchKernelGGL(k_cn_zero, cn_vgrid(N1, N2), dim3(64, 4), 0, s, N1, N2, x, (long)ctx->lds);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     ctx->stats.last_parabolic_its = its;
-    ctx->stats.last_parabolic_relres = bb > 0 ? sqrt(rr / bb) : 0.;
+    // rr from s.s - 2 omega t.s + omega^2 t.t can round below zero at convergence
+    ctx->stats.last_parabolic_relres = bb > 0 && rr > 0 ? sqrt(rr / bb) : 0.;
     if (its_out) *its_out = its;
     if (status == IMPEC_ENOCONV)
         return set_error(ctx, status, "dispersion solve: no convergence after %d iterations", its);
