@@ -19,7 +19,7 @@ KERNELS = {
     # registers), write r', x, p', v' (4 x 8)
     "cn_apply_p": dict(rocprof="k_cn_pstep", bytes_per_cell=92),
     # BiCGStab s-launch (k_cn_stream<1>): s = r - alpha v, t = A (s / D), t.s, t.t,
-    # r0.t, r0.s; read r, v, r0 (3 x 8), 1/D (4) and u, v (2 x 8); no vector written
+    # r0.t, r0.s, s.s; read r, v, r0 (3 x 8), 1/D (4) and u, v (2 x 8); no vector written
     "cn_apply_s": dict(rocprof="k_cn_stream<1>", bytes_per_cell=44),
     # zero-guess start (k_cn_start): b, 1/D and v = A (b / D); read c, u, v
     # (3 x 8), write b (8), 1/D (4), v (8)
