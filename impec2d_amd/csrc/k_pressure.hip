@@ -2531,15 +2531,7 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
                 // meets the tolerance or is not finite; only an FP64 residual ends
                 // the loop
                 if (ru && (rr <= tol * tol * bb || rr <= IMPEC_RU_DELTA * IMPEC_RU_DELTA * rr_ref || !(rr == rr))) {
-#ifdef IMPEC_RU_TRACE
-                    const double rr32 = rr;
-#endif
                     if ((st = cg_reliable32(ctx, a0, c, nb, &rr))) return st;
-#ifdef IMPEC_RU_TRACE
-                    if (its < 200)
-                        fprintf(stderr, "ru phase %d its %d rr32/bb %.3e rr64/bb %.3e ref/bb %.3e\n", phase, its,
-                                rr32 / bb, rr / bb, rr_ref / bb);
-#endif
                     rr_ref = rr;
                 }
             }
