@@ -183,7 +183,8 @@ __global__ void k_flux_y(SweepConst k, const double *__restrict__ c, const doubl
     G[(long)(i + k.B) * (k.N2 + 1) + j] = Gj;
 }
 
-// c = 0.5 * (a + b) over the whole ghosted array (HyperbolicSplitSolver.cpp:97-101)
+// c = 0.5 * (a + b) over the whole ghosted array (HyperbolicSplitSolver.cpp:97-101);
+// the ensemble's average (single contexts average inside k_advect_parallel)
 template <class G>
 __device__ __forceinline__ void average_body(const G &gr_, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c, int rows,
                           int cols, int ld) {
@@ -192,10 +193,6 @@ __device__ __forceinline__ void average_body(const G &gr_, const double *__restr
     if (i >= rows || j >= cols) return;
     const long o = (long)i * ld + j;
     c[o] = 0.5 * (a[o] + b[o]);
-}
-__global__ void k_average(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c, int rows,
-                          int cols, int ld) {
-    average_body(HwGrid{}, a, b, c, rows, cols, ld);
 }
 
 __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int rows, int cols, int ld) {
