@@ -147,6 +147,7 @@ int impec::ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out) {
     if (!P || !out) return IMPEC_EINVAL;
     *out = nullptr;
     if (P->N1 < 2 || P->N2 < 2) return IMPEC_EINVAL;
+    if ((long long)P->N1 * P->N2 >= (1LL << 31)) return IMPEC_EINVAL;   // cell indices are 32-bit
     if (P->num_ghostcells != GHOST) return IMPEC_EINVAL;
     if (P->source_type != IMPEC_SRC_PARALLEL && P->source_type != IMPEC_SRC_DIAGONAL) return IMPEC_EINVAL;
     if (!(P->nu >= 0. && P->nu <= 1.)) return IMPEC_EINVAL;   // EllipticSolver::SetNu

@@ -402,9 +402,12 @@ struct CgArgs {
     double bsum;        // sum of b = -q
 };
 
+// cell (i, j) of flat index id < n1 * n2 < 2^31 (checked at creation): 32-bit
+// division, not the 64-bit sequence
 __device__ __forceinline__ void cg_cell(const CgArgs &c, long id, int &i, int &j) {
-    i = (int)(id / c.n2);
-    j = (int)(id - (long)i * c.n2);
+    const int k = (int)id;
+    i = k / c.n2;
+    j = k - i * c.n2;
 }
 
 // r = b - A_N x with b = -q; pin handling; partials (r.r, sum r).  With r32

@@ -82,7 +82,7 @@ def test_invalid_params_rejected():
     lib = capi.load()
     h = C.c_void_p()
     for kw in [dict(num_ghostcells=3), dict(N1=1), dict(nu=1.5), dict(source_type=2), dict(dim_splitting=7),
-               dict(mg_variant=3), dict(precision=3)]:
+               dict(mg_variant=3), dict(precision=3), dict(N1=65536, N2=32768)]:
         p = capi.default_params(**kw)
         assert lib.impec_create(C.byref(p), C.byref(h)) == capi.IMPEC_EINVAL
 
