@@ -277,9 +277,8 @@ struct XStream {
 };
 
 template <bool MUSCL, bool PHI1>
-__global__ void __launch_bounds__(64 * PS_WAVES) k_advect_parallel(ParArgs A) {
+__device__ __forceinline__ void advect_parallel_body(const ParArgs &A, const int wid) {
     const int lane = threadIdx.x & 63;
-    const int wid = blockIdx.x * PS_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wid >= A.nwave) return;
     const SweepConst &kx = A.kx, &ky = A.ky;
     const int N1 = kx.N1, N2 = kx.N2, B = kx.B;
@@ -337,6 +336,33 @@ __global__ void __launch_bounds__(64 * PS_WAVES) k_advect_parallel(ParArgs A) {
         }
     }
 }
+template <bool MUSCL, bool PHI1>
+__global__ void __launch_bounds__(64 * PS_WAVES) k_advect_parallel(ParArgs A) {
+    advect_parallel_body<MUSCL, PHI1>(A, blockIdx.x * PS_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
+}
+
+// The ensemble's one-pass parallel splitting: one launch over every member's
+// wavefronts (member m owns waves [start[m], start[m+1])), the step's factors
+// formed from the member's device dt as sweep_const forms them
+struct ParRec {
+    ParArgs a;       // dt factors zero; out is the member's t1
+    double dx, dy;
+    const double *S;
+};
+template <bool MUSCL, bool PHI1>
+__global__ void __launch_bounds__(64 * PS_WAVES)
+    kp_advect_parallel(const ParRec *__restrict__ tab, const int *__restrict__ start, int nmem, int total) {
+    const int gw = blockIdx.x * PS_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (gw >= total) return;
+    const int m = member_of(start, nmem, gw);
+    ParRec R = tab[m];
+    const double dt = R.S[S_DT];
+    R.a.kx.hdtdx = 0.5 * dt / R.dx;
+    R.a.kx.dtdx = dt / R.dx;
+    R.a.ky.hdtdx = 0.5 * dt / R.dy;
+    R.a.ky.dtdx = dt / R.dy;
+    advect_parallel_body<MUSCL, PHI1>(R.a, gw - start[m]);
+}
 
 static SweepConst sweep_const(const impec_ctx *ctx, double dt, double h) {
     SweepConst k;
@@ -345,6 +371,23 @@ static SweepConst sweep_const(const impec_ctx *ctx, double dt, double h) {
     k.dtdx = dt / h;
     k.phi = ctx->P.porosity;
     return k;
+}
+
+// arguments of the one-pass parallel splitting of c into out: strips of
+// PS_OUT columns, chunks of rows for about IMPEC_PS_TARGET_WAVES wavefronts
+static ParArgs par_args(const impec_ctx *ctx, double dt, const double *c, double *out) {
+    ParArgs a;
+    a.kx = sweep_const(ctx, dt, ctx->dx);
+    a.ky = sweep_const(ctx, dt, ctx->dy);
+    a.c = c; a.u = ctx->u; a.v = ctx->v; a.out = out;
+    const int rows = ctx->N1 + 2 * ctx->B, cols = ctx->N2 + 2 * ctx->B;
+    a.nstrip = (cols + PS_OUT - 1) / PS_OUT;
+    long R = ((long)rows * a.nstrip + IMPEC_PS_TARGET_WAVES - 1) / IMPEC_PS_TARGET_WAVES;
+    if (R < 8) R = 8;
+    if (R > rows) R = rows;
+    a.R = (int)R;
+    a.nwave = a.nstrip * ((rows + a.R - 1) / a.R);
+    return a;
 }
 
 template <bool MUSCL, bool PHI1>
@@ -448,17 +491,8 @@ int advect_split(impec_ctx *ctx, double dt) {
         // X then Y and Y then X from c, averaged: one streaming pass into t1,
         // then t1 back into c
         (void)t2; (void)t3;
-        ParArgs a;
-        a.kx = sweep_const(ctx, dt, ctx->dx);
-        a.ky = sweep_const(ctx, dt, ctx->dy);
-        a.c = c; a.u = ctx->u; a.v = ctx->v; a.out = t1;
-        const int rows = ctx->N1 + 2 * ctx->B, cols = ctx->N2 + 2 * ctx->B;
-        a.nstrip = (cols + PS_OUT - 1) / PS_OUT;
-        long R = ((long)rows * a.nstrip + IMPEC_PS_TARGET_WAVES - 1) / IMPEC_PS_TARGET_WAVES;
-        if (R < 8) R = 8;
-        if (R > rows) R = rows;
-        a.R = (int)R;
-        a.nwave = a.nstrip * ((rows + a.R - 1) / a.R);
+        const ParArgs a = par_args(ctx, dt, c, t1);
+        const int rows = ctx->N1 + 2 * ctx->B;
         const bool muscl = ctx->P.hyper_method == IMPEC_HYPER_MUSCL, phi1 = ctx->P.porosity == 1.;
         const dim3 g((a.nwave + PS_WAVES - 1) / PS_WAVES), blk(64 * PS_WAVES);
         prof_begin(ctx);
@@ -490,15 +524,6 @@ int advect_split(impec_ctx *ctx, double dt) {
                                  hipGetErrorString(e_));                                       \
     } while (0)
 
-struct SweepRec {
-    SweepConst k;           // dt-dependent factors formed in the kernel
-    double h;               // dx (x sweeps) or dy (y sweeps)
-    const double *cin;
-    double *cout;
-    const double *vel;
-    const double *S;
-    int gx, gy;
-};
 struct AvgRec {
     const double *a, *b;
     double *c;
@@ -507,31 +532,10 @@ struct AvgRec {
 
 struct EnsTransTabs {
     int muscl, phi1;
-    Batch<SweepRec> x1, y1, y2, x2;   // c->t1 (x), t1->t2 (y), c->t1 (y), t1->t3 (x)
-    Batch<AvgRec> avg;
+    Batch<ParRec> par;      // c -> t1, both orders averaged (units: wavefronts)
+    Batch<AvgRec> back;     // t1 -> c as 0.5 (t1 + t1), which is t1 exactly
 };
 
-// sweep_const's factors from the member's step
-__device__ __forceinline__ SweepConst sweep_factors(const SweepRec &R) {
-    SweepConst k = R.k;
-    const double dt = R.S[S_DT];
-    k.hdtdx = 0.5 * dt / R.h;
-    k.dtdx = dt / R.h;
-    return k;
-}
-
-template <bool MUSCL, bool PHI1>
-__global__ void kb_sweep_x(const SweepRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
-        sweep_x_body<MUSCL, PHI1>(G, sweep_factors(R), R.cin, R.cout, R.vel);
-    });
-}
-template <bool MUSCL, bool PHI1>
-__global__ void kb_sweep_y(const SweepRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
-        sweep_y_body<MUSCL, PHI1>(G, sweep_factors(R), R.cin, R.cout, R.vel);
-    });
-}
 __global__ void kb_average(const AvgRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
     batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
         average_body(G, R.a, R.b, R.c, R.rows, R.cols, R.ld);
@@ -541,7 +545,7 @@ __global__ void kb_average(const AvgRec *__restrict__ tab, const int *__restrict
 void ens_trans_free(impec_ensemble *e) {
     if (!e->trans) return;
     EnsTransTabs *t = e->trans;
-    batch_free(t->x1); batch_free(t->y1); batch_free(t->y2); batch_free(t->x2); batch_free(t->avg);
+    batch_free(t->par); batch_free(t->back);
     delete t;
     e->trans = nullptr;
 }
@@ -553,53 +557,42 @@ int ens_trans_build(impec_ensemble *e) {
     const int n = e->n;
     t->muscl = e->m[0]->P.hyper_method == IMPEC_HYPER_MUSCL;
     t->phi1 = e->m[0]->P.porosity == 1.;
-    std::vector<SweepRec> x1(n), y1(n), y2(n), x2(n);
-    std::vector<AvgRec> av(n);
-    std::vector<int> u(n);
+    std::vector<ParRec> pr(n);
+    std::vector<AvgRec> bk(n);
+    std::vector<int> wv(n), u(n);
     for (int k = 0; k < n; k++) {
         impec_ctx *ctx = e->m[k];
-        const SweepConst base = sweep_const(ctx, 0., 1.);
+        pr[k] = ParRec{par_args(ctx, 0., ctx->c, ctx->t1), ctx->dx, ctx->dy, ctx->scal};
+        wv[k] = pr[k].a.nwave;
         const int gx = (ctx->N2 + 2 * ctx->B + 63) / 64, gy = (ctx->N1 + 2 * ctx->B + 3) / 4;
-        x1[k] = SweepRec{base, ctx->dx, ctx->c, ctx->t1, ctx->u, ctx->scal, gx, gy};
-        y1[k] = SweepRec{base, ctx->dy, ctx->t1, ctx->t2, ctx->v, ctx->scal, gx, gy};
-        y2[k] = SweepRec{base, ctx->dy, ctx->c, ctx->t1, ctx->v, ctx->scal, gx, gy};
-        x2[k] = SweepRec{base, ctx->dx, ctx->t1, ctx->t3, ctx->u, ctx->scal, gx, gy};
-        av[k] = AvgRec{ctx->t2, ctx->t3, ctx->c, ctx->N1 + 2 * ctx->B, ctx->N2 + 2 * ctx->B, ctx->lds, gx, gy};
+        bk[k] = AvgRec{ctx->t1, ctx->t1, ctx->c, ctx->N1 + 2 * ctx->B, ctx->N2 + 2 * ctx->B, ctx->lds, gx, gy};
         u[k] = gx * gy;
     }
-    ENS_CHECK(e, batch_upload(t->x1, x1, u));
-    ENS_CHECK(e, batch_upload(t->y1, y1, u));
-    ENS_CHECK(e, batch_upload(t->y2, y2, u));
-    ENS_CHECK(e, batch_upload(t->x2, x2, u));
-    ENS_CHECK(e, batch_upload(t->avg, av, u));
+    ENS_CHECK(e, batch_upload(t->par, pr, wv));
+    ENS_CHECK(e, batch_upload(t->back, bk, u));
     return IMPEC_OK;
 }
 
 template <bool MUSCL, bool PHI1>
-static int ens_sweeps(impec_ensemble *e) {
+static int ens_parallel(impec_ensemble *e) {
     EnsTransTabs *t = e->trans;
-    const dim3 blk(64, 4);
     hipStream_t s = e->stream;
-    const Batch<SweepRec> *order[4] = {&t->x1, &t->y1, &t->y2, &t->x2};
-    for (int q = 0; q < 4; q++) {
-        const Batch<SweepRec> &b = *order[q];
-        if (b.nmem == 0) continue;
-        if (q == 0 || q == 3)
-            hipLaunchKernelGGL((kb_sweep_x<MUSCL, PHI1>), dim3(batch_grid(b.total)), blk, 0, s, b.tab, b.start, b.nmem);
-        else hipLaunchKernelGGL((kb_sweep_y<MUSCL, PHI1>), dim3(batch_grid(b.total)), blk, 0, s, b.tab, b.start, b.nmem);
-    }
-    if (t->avg.nmem)
-        hipLaunchKernelGGL(kb_average, dim3(batch_grid(t->avg.total)), blk, 0, s, t->avg.tab, t->avg.start, t->avg.nmem);
+    if (t->par.nmem)
+        hipLaunchKernelGGL((kp_advect_parallel<MUSCL, PHI1>), dim3((t->par.total + PS_WAVES - 1) / PS_WAVES),
+                           dim3(64 * PS_WAVES), 0, s, t->par.tab, t->par.start, t->par.nmem, t->par.total);
+    if (t->back.nmem)
+        hipLaunchKernelGGL(kb_average, dim3(batch_grid(t->back.total)), dim3(64, 4), 0, s, t->back.tab,
+                           t->back.start, t->back.nmem);
     ENS_CHECK(e, hipGetLastError());
     return IMPEC_OK;
 }
 
 int ens_advect_parallel(impec_ensemble *e) {
     const EnsTransTabs *t = e->trans;
-    if (t->muscl && t->phi1) return ens_sweeps<true, true>(e);
-    if (t->muscl) return ens_sweeps<true, false>(e);
-    if (t->phi1) return ens_sweeps<false, true>(e);
-    return ens_sweeps<false, false>(e);
+    if (t->muscl && t->phi1) return ens_parallel<true, true>(e);
+    if (t->muscl) return ens_parallel<true, false>(e);
+    if (t->phi1) return ens_parallel<false, true>(e);
+    return ens_parallel<false, false>(e);
 }
 
 }  // namespace impec
