@@ -192,7 +192,7 @@ constexpr int CN_OUT = 62, CNP_OUT = 60, CN_WAVES = 4;
 enum { CN_S = 1, CN_P0 = 2 };
 // build-time tuning: rows in flight per wavefront, wavefronts per launch
 #ifndef IMPEC_CN_PF
-#define IMPEC_CN_PF 4
+#define IMPEC_CN_PF 2
 #endif
 #ifndef IMPEC_CNP_PF
 #define IMPEC_CNP_PF 2
