@@ -200,6 +200,10 @@ enum { CN_S = 1, CN_P0 = 2 };
 #ifndef IMPEC_CN_TARGET_WAVES
 #define IMPEC_CN_TARGET_WAVES 4096
 #endif
+// the p-launch and the one-pass start (their own chunks of rows)
+#ifndef IMPEC_CNP_TARGET_WAVES
+#define IMPEC_CNP_TARGET_WAVES 8192
+#endif
 // build-time tuning: minimum waves per SIMD requested for the p-launch (1 lets
 // the compiler use the registers it wants: 144, three waves per SIMD)
 #ifndef IMPEC_CNP_MIN_WAVES
@@ -827,10 +831,16 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         a.R = (int)R;
     }
     a.nwave = a.nstrip * ((N1 + a.R - 1) / a.R);
-    // the p-launch's strips of CNP_OUT columns, chunks of the same rows
+    // the p-launch's strips of CNP_OUT columns and its own chunks of rows
     CnArgs a2 = a;
     a2.nstrip = (N2 + CNP_OUT - 1) / CNP_OUT;
-    a2.nwave = a2.nstrip * ((N1 + a.R - 1) / a.R);
+    {
+        long R = ((long)N1 * a2.nstrip + IMPEC_CNP_TARGET_WAVES - 1) / IMPEC_CNP_TARGET_WAVES;
+        if (R < 8) R = 8;
+        if (R > N1) R = N1;
+        a2.R = (int)R;
+    }
+    a2.nwave = a2.nstrip * ((N1 + a2.R - 1) / a2.R);
     if (a.nwave > RED_BLOCKS || a2.nwave > RED_BLOCKS)
         return set_error(ctx, IMPEC_EINVAL, "dispersion solve: grid too large");
     const dim3 sg((a.nwave + CN_WAVES - 1) / CN_WAVES), sblk(64 * CN_WAVES);

@@ -1778,7 +1778,7 @@ static Grid2 vec_grid(int n1, int n2) {
 // instead of five times.  Per-wave partials of d.q.
 constexpr int DM_OUT = 62, DM_PF = 4;
 #ifndef IMPEC_DM_WAVES
-#define IMPEC_DM_WAVES 8192
+#define IMPEC_DM_WAVES 4096
 #endif
 
 // DT: storage of d and q, double for the FP64 CG, float for the FP32-vector CG
