@@ -1983,6 +1983,12 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
 #ifndef IMPEC_MG_WAVES
 #define IMPEC_MG_WAVES 2048
 #endif
+// the finest level's target in the FP32 cycle (its legs are compute-bound and
+// gain from more waves; the FP64 legs and the small levels lose from shorter
+// chunks)
+#ifndef IMPEC_MG_WAVES0_F32
+#define IMPEC_MG_WAVES0_F32 4096
+#endif
 // rows per chunk of the streaming legs: about IMPEC_MG_WAVES wavefronts on large
 // levels, short chunks on small ones; even, since the march starts on an
 // even row
@@ -1990,9 +1996,10 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
 #define IMPEC_MG_MINROWS 4
 #endif
 static_assert(IMPEC_MG_MINROWS % 2 == 0 && IMPEC_MG_MINROWS >= 2, "chunks of the streaming legs start on even rows");
-static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip) {
+static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip, bool f32) {
     if (l == 0 && ctx->P.mg_rows > 0) return (ctx->P.mg_rows + 1) & ~1;
-    long r = ((long)n1 * nstrip + IMPEC_MG_WAVES - 1) / IMPEC_MG_WAVES;
+    const long target = l == 0 && f32 ? IMPEC_MG_WAVES0_F32 : IMPEC_MG_WAVES;
+    long r = ((long)n1 * nstrip + target - 1) / target;
     r = (r + 1) & ~1L;
     if (r < IMPEC_MG_MINROWS) r = IMPEC_MG_MINROWS;
     if (r > 64) r = 64;
@@ -2053,7 +2060,7 @@ static FusedArgsT<T> fused_args(impec_ctx *ctx, int l) {
     A.part = ctx->part;
     const int out = mg_ns(ctx) == 2 ? Strip<2>::OUT : Strip<1>::OUT;
     A.nstrip = (L.n2 + out - 1) / out;
-    A.R = stream_rows(ctx, l, L.n1, A.nstrip);
+    A.R = stream_rows(ctx, l, L.n1, A.nstrip, std::is_same<T, float>::value);
     A.nwave = A.nstrip * ((L.n1 + A.R - 1) / A.R);
     return A;
 }
