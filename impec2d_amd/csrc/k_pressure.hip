@@ -1989,6 +1989,13 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
 #ifndef IMPEC_MG_WAVES0_F32
 #define IMPEC_MG_WAVES0_F32 4096
 #endif
+// the target of the FP64 cycle's coarser levels (level 1 and below): longer
+// chunks there (1024 waves at 2048^2 and 1024^2) run those legs 8-16% faster
+// than 2048 waves, 512 waves slower; the FP32 cycle's coarse levels lose about
+// 1% of a mixed step with them and keep IMPEC_MG_WAVES
+#ifndef IMPEC_MG_WAVES_COARSE
+#define IMPEC_MG_WAVES_COARSE 1024
+#endif
 // rows per chunk of the streaming legs: about IMPEC_MG_WAVES wavefronts on large
 // levels, short chunks on small ones; even, since the march starts on an
 // even row
@@ -1998,7 +2005,8 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
 static_assert(IMPEC_MG_MINROWS % 2 == 0 && IMPEC_MG_MINROWS >= 2, "chunks of the streaming legs start on even rows");
 static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip, bool f32) {
     if (l == 0 && ctx->P.mg_rows > 0) return (ctx->P.mg_rows + 1) & ~1;
-    const long target = l == 0 && f32 ? IMPEC_MG_WAVES0_F32 : IMPEC_MG_WAVES;
+    const long target = f32 ? (l == 0 ? IMPEC_MG_WAVES0_F32 : IMPEC_MG_WAVES)
+                            : (l == 0 ? IMPEC_MG_WAVES : IMPEC_MG_WAVES_COARSE);
     long r = ((long)n1 * nstrip + target - 1) / target;
     r = (r + 1) & ~1L;
     if (r < IMPEC_MG_MINROWS) r = IMPEC_MG_MINROWS;
