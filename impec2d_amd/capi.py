@@ -10,8 +10,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# IMPEC_HIP_LIB selects another build of the library (e.g. one with other tuning constants)
-LIB_PATH = os.environ.get("IMPEC_HIP_LIB") or os.path.join(HERE, "libimpec_hip.so")
+LIB_PATH = os.path.join(HERE, "libimpec_hip.so")
 
 IMPEC_OK = 0
 IMPEC_EINVAL = -1
@@ -123,13 +122,15 @@ SIGNATURES = {
 _lib = None
 
 
-def load():
-    """Loads libimpec_hip.so; raises if it has not been built."""
+def load(path=None):
+    """Loads the in-tree libimpec_hip.so (or the build at `path`, which must
+    then be passed explicitly on the first call); raises if it is not built."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError("libimpec_hip.so is not built; run `python -m impec2d_amd.build`")
-        lib = C.CDLL(LIB_PATH)
+        path = path or LIB_PATH
+        if not os.path.exists(path):
+            raise RuntimeError("%s is not built; run `python -m impec2d_amd.build`" % path)
+        lib = C.CDLL(path)
         for name, (args, res) in SIGNATURES.items():
             fn = getattr(lib, name)
             fn.argtypes = args

@@ -31,6 +31,7 @@ enum Scal {
     S_OUTFLUX = 24, S_TMO = 25, S_CMAX = 26, S_CMIN = 27,
     S_GUESS_ALPHA = 28,                              // step length of the learned increment
     S_GUESS_GAMMA = 29,                              // step length of the last step's change
+    S_CNRES = 30,                                    // true ||b - A x||^2 of the dispersion solve
     // per-member control of an ensemble's pressure solve (device-side loop state)
     S_EST = 32,        // 0 done, 1 iterating, 2 residual recheck due
     S_EITS = 33,       // iterations of this solve

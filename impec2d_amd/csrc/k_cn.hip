@@ -644,8 +644,11 @@ static dim3 cn_vgrid(int N1, int N2, int cap = CN_VBLOCKS) {
 #endif
 static_assert(IMPEC_CN_RHS_BLOCKS <= RED_BLOCKS, "partials of the right-hand side kernel");
 
-// initial guess x0 = c (the solution is formed in place in c's interior):
-// r = r0 = b - A x0; partials r.r (twice)
+// residual of the iterate x in c's interior: r = b - A x; partials r.r
+// (twice).  STORE: also r and r0 = r (the initial guess x0 = c, or a restart
+// of the recursion from the current iterate); otherwise the partials alone
+// (the true residual that confirms convergence)
+template <bool STORE>
 __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ u,
                              const double *__restrict__ v, const double *__restrict__ b, double *__restrict__ r,
                              double *__restrict__ r0, double *__restrict__ part) {
@@ -670,8 +673,10 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
             if (we && no) acc += a[ST_NW] * cc[-L + 1];
             const long id = (long)i * k.N2 + j;
             const double rv = b[id] - acc;
-            r[id] = rv;
-            r0[id] = rv;
+            if (STORE) {
+                r[id] = rv;
+                r0[id] = rv;
+            }
             s[0] += rv * rv;
             s[1] += rv * rv;
         }
@@ -692,6 +697,7 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
 //  stage 2 (s-launch): t.s, t.t, r0.t, r0.s, s.s; omega = t.s / t.t (0 when
 //                      t = 0); rho_new = r0.r of r = s - omega t, as
 //                      r0.s - omega r0.t; rr = r.r as s.s - 2 omega t.s + omega^2 t.t
+//  stage 6 (check):    the true residual's r.r
 // launched with BICG_FIN_THREADS threads: one wave per SIMD, so the loads of
 // all NV vectors fit in registers
 constexpr int BICG_FIN_THREADS = 256;
@@ -718,6 +724,8 @@ __global__ void __launch_bounds__(BICG_FIN_THREADS) k_bicg_finish(const double *
         S[S_RHO_NEW] = v[1];
         S[S_R0V] = v[0];
         S[S_ALPHA] = v[1] / v[0];
+    } else if (stage == 6) {
+        S[S_CNRES] = v[0];
     } else if (stage == 1) {
         S[S_R0V] = v[0];
         S[S_RHO] = S[S_RHO_NEW];
@@ -809,7 +817,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     double *b = ctx->cn_rhs, *x = ctx->c + (long)ctx->B * ctx->lds + ctx->B, *r = ctx->cn_r, *r0 = ctx->cn_r0;
     // r, p, v ping-pong: a strip reads its neighbours' old values at the halo
     double *pb[2] = {ctx->cn_p, ctx->cn_y}, *vb[2] = {ctx->cn_v, ctx->cn_z}, *rb[2] = {ctx->cn_r, ctx->cn_r2};
-    const bool zero = ctx->P.cn_guess != 1;
+    bool zero = ctx->P.cn_guess != 1;
     const double tol = ctx->P.para_eps;
     const int maxit = ctx->P.max_iterations > 0 ? ctx->P.max_iterations : 10000;
     CnArgs a;
@@ -874,7 +882,8 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
         const int nvb = (int)(vg.x * vg.y);
         hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nrb, 0, ctx->scal);
-        hipLaunchKernelGGL(k_bicg_guess, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, r, r0, ctx->part);
+        hipLaunchKernelGGL(k_bicg_guess<true>, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, r, r0,
+                           ctx->part);
         hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nvb, 4, ctx->scal);
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = fetch(ctx))) return st;
@@ -893,54 +902,95 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     }
     // the current residual: b before the first p-launch of a zero-guess solve
     const double *rcur = a.r;
-    int rnext = 0;
-    while (!(rr <= tol * tol * bb)) {
-        if (its >= maxit) { status = IMPEC_ENOCONV; break; }
-        if (!(rr == rr)) { status = IMPEC_ENAN; break; }
-        CnArgs as = a;
-        as.r = rcur;
-        as.vin = vb[cur];
+    int rnext = 0, restarts = 0;
+    double rt = 0.;   // true ||b - A x||^2 of the final iterate
+    const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
+    const int nvb = (int)(vg.x * vg.y);
+    for (;;) {
+        while (!(rr <= tol * tol * bb)) {
+            if (its >= maxit) { status = IMPEC_ENOCONV; break; }
+            if (!(rr == rr)) { status = IMPEC_ENAN; break; }
+            CnArgs as = a;
+            as.r = rcur;
+            as.vin = vb[cur];
+            prof_begin(ctx);
+            hipLaunchKernelGGL(k_cn_stream<CN_S>, sg, sblk, 0, s, as);
+            prof_end(ctx, "cn_apply_s");
+            hipLaunchKernelGGL(k_bicg_finish<5>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 2,
+                               ctx->scal);
+            IMPEC_HIP_CHECK(ctx, hipGetLastError());
+            // the next residual's norm from the s-launch: the last iteration updates x only
+            if ((st = fetch(ctx))) return st;
+            rr = ctx->h_scal[S_RR];
+            CnArgs ap = a2;
+            ap.r = rcur; ap.pin = its == 0 ? pfirst : pb[cur]; ap.vin = vb[cur]; ap.x = x;
+            ap.xzero = zero && its == 0;
+            if (rr <= tol * tol * bb || !(rr == rr)) {
+                prof_begin(ctx);
+                hipLaunchKernelGGL(k_cn_xfinal, cn_vgrid(N1, N2), dim3(64, 4), 0, s, ap);
+                prof_end(ctx, "cn_xfinal");
+                IMPEC_HIP_CHECK(ctx, hipGetLastError());
+                its++;
+                continue;   // the loop test ends the solve (or reports the non-finite residual)
+            }
+            if (rb[rnext] == rcur) rnext ^= 1;   // r' never overwrites the r it is formed from
+            ap.rout = rb[rnext];
+            ap.pout = pb[cur ^ 1]; ap.out = vb[cur ^ 1];
+            prof_begin(ctx);
+            hipLaunchKernelGGL(k_cn_pstep, pg, sblk, 0, s, ap);
+            prof_end(ctx, "cn_apply_p");
+            hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a2.nwave, 1,
+                               ctx->scal);
+            rcur = rb[rnext];
+            rnext ^= 1;
+            IMPEC_HIP_CHECK(ctx, hipGetLastError());
+            cur ^= 1;
+            its++;
+        }
+        if (status != IMPEC_OK) break;
+        if (zero && its == 0)   // b already met the tolerance: x = 0 was never written
+            hipLaunchKernelGGL(k_cn_zero, cn_vgrid(N1, N2), dim3(64, 4), 0, s, N1, N2, x, (long)ctx->lds);
+        // the recursion's residual (s.s - 2 omega t.s + omega^2 t.t from the
+        // s-launch's partials) can cancel at convergence: the true residual of
+        // the iterate decides, as the pressure solve's does
         prof_begin(ctx);
-        hipLaunchKernelGGL(k_cn_stream<CN_S>, sg, sblk, 0, s, as);
-        prof_end(ctx, "cn_apply_s");
-        hipLaunchKernelGGL(k_bicg_finish<5>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 2, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_guess<false>, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, nullptr,
+                           nullptr, ctx->part);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nvb, 6, ctx->scal);
+        prof_end(ctx, "cn_check");
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
-        // the next residual's norm from the s-launch: the last iteration updates x only
+        if ((st = fetch(ctx))) return st;
+        rt = ctx->h_scal[S_CNRES];
+        if (rt <= tol * tol * bb) break;
+        if (!(rt == rt)) { status = IMPEC_ENAN; break; }
+        if (restarts >= 3) { status = IMPEC_ENOCONV; break; }
+        // restart the recursion from the current iterate: r = r0 = b - A x,
+        // the first direction p = r
+        restarts++;
+        a.r0 = a2.r0 = r0;
+        hipLaunchKernelGGL(k_bicg_guess<true>, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, r, r0,
+                           ctx->part);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nvb, 4, ctx->scal);
+        CnArgs ap = a;
+        ap.r = r;
+        ap.pout = pb[1]; ap.out = vb[1];
+        hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, sblk, 0, s, ap);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 1, ctx->scal);
+        IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = fetch(ctx))) return st;
         rr = ctx->h_scal[S_RR];
-        CnArgs ap = a2;
-        ap.r = rcur; ap.pin = its == 0 ? pfirst : pb[cur]; ap.vin = vb[cur]; ap.x = x;
-        ap.xzero = zero && its == 0;
-        if (rr <= tol * tol * bb || !(rr == rr)) {
-            prof_begin(ctx);
-            hipLaunchKernelGGL(k_cn_xfinal, cn_vgrid(N1, N2), dim3(64, 4), 0, s, ap);
-            prof_end(ctx, "cn_xfinal");
-            IMPEC_HIP_CHECK(ctx, hipGetLastError());
-            its++;
-            continue;   // the loop test ends the solve (or reports the non-finite residual)
-        }
-        if (rb[rnext] == rcur) rnext ^= 1;   // r' never overwrites the r it is formed from
-        ap.rout = rb[rnext];
-        ap.pout = pb[cur ^ 1]; ap.out = vb[cur ^ 1];
-        prof_begin(ctx);
-        hipLaunchKernelGGL(k_cn_pstep, pg, sblk, 0, s, ap);
-        prof_end(ctx, "cn_apply_p");
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a2.nwave, 1, ctx->scal);
-        rcur = rb[rnext];
-        rnext ^= 1;
-        IMPEC_HIP_CHECK(ctx, hipGetLastError());
-        cur ^= 1;
-        its++;
+        rcur = r;
+        rnext = 1;
+        cur = 1;
+        zero = false;
+        if (its == 0) its = 1;   // the first p-launch reads p = pb[cur], never b
     }
-    if (zero && its == 0)   // b already met the tolerance: x = 0 was never written
-        hipLaunchKernelGGL(k_cn_zero, cn_vgrid(N1, N2), dim3(64, 4), 0, s, N1, N2, x, (long)ctx->lds);
-    IMPEC_HIP_CHECK(ctx, hipGetLastError());
     ctx->stats.last_parabolic_its = its;
-    // rr from s.s - 2 omega t.s + omega^2 t.t can round below zero at convergence
-    ctx->stats.last_parabolic_relres = bb > 0 && rr > 0 ? sqrt(rr / bb) : 0.;
+    ctx->stats.last_parabolic_relres = bb > 0 && rt > 0 ? sqrt(rt / bb) : 0.;
     if (its_out) *its_out = its;
     if (status == IMPEC_ENOCONV)
-        return set_error(ctx, status, "dispersion solve: no convergence after %d iterations", its);
+        return set_error(ctx, status, "dispersion solve: no convergence after %d iterations, %d restarts (relres %.3e)",
+                         its, restarts, ctx->stats.last_parabolic_relres);
     if (status == IMPEC_ENAN) return set_error(ctx, status, "dispersion solve: non-finite residual");
     return IMPEC_OK;
 }

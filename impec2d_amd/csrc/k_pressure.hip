@@ -538,6 +538,10 @@ __global__ void k_guess_dots(LevArgs L, CgArgs c, const double *__restrict__ p, 
 // single direction when they are nearly parallel), else the 1-D minimiser
 __global__ void k_guess_solve(double *__restrict__ S, int use_d, int use_e) {
     const double b0 = S[S_G0], b1 = S[S_G0 + 1], m00 = S[S_G0 + 2], m01 = S[S_G0 + 3], m11 = S[S_G0 + 4];
+    // a direction with non-finite products (a diverged network's increment)
+    // takes no part in the guess
+    if (!isfinite(b0) || !isfinite(m00) || !isfinite(m01)) use_d = 0;
+    if (!isfinite(b1) || !isfinite(m11) || (use_d && !isfinite(m01))) use_e = 0;
     const double a1 = m00 > 0. ? b0 / m00 : 0., g1 = m11 > 0. ? b1 / m11 : 0.;
     double a = use_d ? a1 : 0., g = use_e ? g1 : 0.;
     if (use_d && use_e) {
@@ -570,8 +574,8 @@ __global__ void k_guess_apply(CgArgs c, double *__restrict__ p, const double *__
         cg_cell(c, id, i, j);
         const long o = (long)(i + c.B) * c.ldp + (j + c.B);
         double v = p[o];
-        if (d) v += alpha * d[(long)i * c.ld + j];
-        if (pold) v += gamma * (p[o] - pold[o]);
+        if (d && alpha != 0.) v += alpha * d[(long)i * c.ld + j];   // 0 x inf would poison p
+        if (pold && gamma != 0.) v += gamma * (p[o] - pold[o]);
         p[o] = v;
     }
 }
@@ -2582,6 +2586,13 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
         if (!finished && (st = finish_phase())) return st;
         if (status != IMPEC_OK || relres <= tol) break;
     }
+    // the elimination phase is the last: its restarts are spent and the
+    // pinned system's true residual still misses the tolerance
+    bool missed = false;
+    if (status == IMPEC_OK && !(relres <= tol)) {
+        status = IMPEC_ENOCONV;
+        missed = true;
+    }
     const bool used_d = warm && guess == IMPEC_GUESS_LEARNED && ctx->net;
     const bool used_e = warm && (guess == IMPEC_GUESS_LEARNED || guess == IMPEC_GUESS_EXTRAPOLATE) && ctx->p_old &&
                         ctx->n_solves >= 2;
@@ -2596,6 +2607,10 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     ctx->stats.calls += 1;
     if (its_out) *its_out = its;
     if (relres_out) *relres_out = relres;
+    if (missed)
+        return set_error(ctx, status,
+                         "pressure solve: true residual %.3e above eps %.3e after %d iterations and 3 restarts", relres,
+                         tol, its);
     if (status == IMPEC_ENOCONV)
         return set_error(ctx, status, "pressure solve: no convergence after %d iterations (relres %.3e)", its, relres);
     if (status == IMPEC_ENAN) return set_error(ctx, status, "pressure solve: non-finite residual");

@@ -264,12 +264,20 @@ double EllipticSolver::GetAvgIts() { return calls ? (double)total_its / calls : 
 MGSolver::MGSolver(SimParams *sp_, Device *dev_) : EllipticSolver(sp_, dev_) {}
 MGSolver::~MGSolver() {}
 
+bool MGSolver::keep_unconverged(int status, const char *where) {
+    if (status != IMPEC_ENOCONV) return dev->ok(status, where);
+    if (missed++ == 0)
+        std::cerr << "WARNING::" << where << ": " << impec_last_error(dev->ctx())
+                  << " -- the iterate is kept, as the reference keeps its solver's (reported once)" << std::endl;
+    return true;
+}
+
 bool MGSolver::SolveLinearMultigrid(Array2d &c_c, int, Array2d &p_c, int, double const *const q_v) {
     if (!dev->use(c_c, Device::C) || !dev->use_source(q_v)) return false;
     if (!dev->use(p_c, Device::P)) return false;   // the previous p (warm start) when the caller wrote one
     int its = 0;
     double relres = 0.;
-    const bool ok = dev->ok(impec_pressure_solve(dev->ctx(), &its, &relres), "MGSolver::SolveLinearMultigrid");
+    const bool ok = keep_unconverged(impec_pressure_solve(dev->ctx(), &its, &relres), "MGSolver::SolveLinearMultigrid");
     last_its = its;
     last_relres = relres;
     dev->written(p_c);
@@ -284,7 +292,7 @@ bool MGSolver::Solve_Parabolic_cc_to_c(Array2d &c_c, int, const Array3d &v, int,
     }
     if (!dev->use(c_c, Device::C) || !dev->use(v)) return false;
     int its = 0;
-    const bool ok = dev->ok(impec_dispersion_cn(dev->ctx(), dt, &its), "MGSolver::Solve_Parabolic_cc_to_c");
+    const bool ok = keep_unconverged(impec_dispersion_cn(dev->ctx(), dt, &its), "MGSolver::Solve_Parabolic_cc_to_c");
     last_parabolic_its = its;
     dev->written(c_c);
     return ok;

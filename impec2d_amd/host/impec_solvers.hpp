@@ -247,9 +247,17 @@ class MGSolver : public EllipticSolver {
     bool Solve_Parabolic_cc_to_c(Array2d &c_c, int c_bounds, const Array3d &v, int v_bounds,
                                  Array2d const *const ls, int ls_bounds, const double dt);
     double GetLastRelres() const { return last_relres; }
+    // solves that missed their tolerance (IMPEC_ENOCONV) and were kept
+    int GetMissedSolves() const { return missed; }
 
   private:
+    // the reference keeps whatever iterate its solver returns
+    // (Solve_Elliptic_cc_to_c and Solve_Parabolic_cc_to_c return true,
+    // MGSolver.cpp:223-248, 952-976): a tolerance below the FP64 rounding
+    // floor (the shipped configurations ask for 1e-14) must not stop the run
+    bool keep_unconverged(int status, const char *where);
     double last_relres = 0.;
+    int missed = 0;
 };
 
 // ---------------------------------------------------------------------------

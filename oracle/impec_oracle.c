@@ -37,6 +37,34 @@
 #include <stdio.h>
 
 /* ------------------------------------------------------------------------- */
+/* Planted mistakes (tests only).  or_set_mutation(m) makes one restated      */
+/* formula deliberately wrong, so that the tests that pin this oracle can     */
+/* show they fail on it; 0 (the default) is the faithful restatement.         */
+/* ------------------------------------------------------------------------- */
+enum {
+    MUT_NONE = 0,
+    MUT_CN_SWAP_ALPHA = 1,      /* alpha_l and alpha_t exchanged in the tensors   */
+    MUT_CN_DROP_PHI = 2,        /* porosity left out of the tensors               */
+    MUT_CN_AVG_F = 3,           /* x-face four-point average: v(i+1,j+1) for v(i-1,j+1) */
+    MUT_CN_AVG_G = 4,           /* y-face four-point average: u(i-1,j-1) for u(i+1,j-1) */
+    MUT_CN_DM_CROSS = 5,        /* d_m added to d12 / d21 as well                  */
+    MUT_CN_RHS_CROSS = 6,       /* right-hand side: sign of the d12 cross term     */
+    MUT_SPLIT_XFIRST = 7,       /* x_sweep_first runs the y sweep first            */
+    MUT_SPLIT_YFIRST = 8,       /* y_sweep_first runs the x sweep first            */
+    MUT_SPLIT_STRANG = 9,       /* strang: full x steps around the y step          */
+    MUT_SPLIT_ALT = 10,         /* alternating: the first sweep never changes      */
+    MUT_SPLIT_PARALLEL = 11,    /* parallel: the Y-then-X copy left out (no average) */
+    MUT_COUNT
+};
+static int or_mutation = MUT_NONE;
+int or_set_mutation(int m)
+{
+    if (m < 0 || m >= MUT_COUNT) return -1;
+    or_mutation = m;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* parameters (SimParams.h:29-164 field meanings)                             */
 /* ------------------------------------------------------------------------- */
 typedef struct {
@@ -1215,25 +1243,36 @@ void or_advect_split(or_ctx *ctx, double *c, const double *v, double dt)
     switch (ctx->P.dim_split) {
     case 0:
         if (ctx->prev_first_sweep == 1) {
-            or_advect_x(ctx, c, v, dt); or_advect_y(ctx, c, v, dt); ctx->prev_first_sweep = 0;
+            or_advect_x(ctx, c, v, dt); or_advect_y(ctx, c, v, dt);
+            ctx->prev_first_sweep = or_mutation == MUT_SPLIT_ALT ? 1 : 0;
         } else {
             or_advect_y(ctx, c, v, dt); or_advect_x(ctx, c, v, dt); ctx->prev_first_sweep = 1;
         }
         break;
     case 1:
-        or_advect_x(ctx, c, v, dt); or_advect_y(ctx, c, v, dt); ctx->prev_first_sweep = 0; break;
+        if (or_mutation == MUT_SPLIT_XFIRST) { or_advect_y(ctx, c, v, dt); or_advect_x(ctx, c, v, dt); }
+        else { or_advect_x(ctx, c, v, dt); or_advect_y(ctx, c, v, dt); }
+        ctx->prev_first_sweep = 0;
+        break;
     case 2:
-        or_advect_y(ctx, c, v, dt); or_advect_x(ctx, c, v, dt); ctx->prev_first_sweep = 1; break;
-    case 3:
-        or_advect_x(ctx, c, v, 0.5 * dt); or_advect_y(ctx, c, v, dt); or_advect_x(ctx, c, v, 0.5 * dt);
-        ctx->prev_first_sweep = 0; break;
+        if (or_mutation == MUT_SPLIT_YFIRST) { or_advect_x(ctx, c, v, dt); or_advect_y(ctx, c, v, dt); }
+        else { or_advect_y(ctx, c, v, dt); or_advect_x(ctx, c, v, dt); }
+        ctx->prev_first_sweep = 1;
+        break;
+    case 3: {
+        const double hx = or_mutation == MUT_SPLIT_STRANG ? dt : 0.5 * dt;
+        or_advect_x(ctx, c, v, hx); or_advect_y(ctx, c, v, dt); or_advect_x(ctx, c, v, hx);
+        ctx->prev_first_sweep = 0;
+        break;
+    }
     default: {
         const size_t n = (size_t)(ctx->N1 + 2 * ctx->b) * ctx->ld;
         double *ct = malloc(sizeof(double) * n);
         memcpy(ct, c, sizeof(double) * n);
         or_advect_x(ctx, c, v, dt); or_advect_y(ctx, c, v, dt);
         or_advect_y(ctx, ct, v, dt); or_advect_x(ctx, ct, v, dt);
-        for (size_t k = 0; k < n; k++) c[k] = 0.5 * (c[k] + ct[k]);
+        if (or_mutation != MUT_SPLIT_PARALLEL)
+            for (size_t k = 0; k < n; k++) c[k] = 0.5 * (c[k] + ct[k]);
         free(ct);
         ctx->prev_first_sweep = 0;
     }
@@ -1248,13 +1287,15 @@ static void tensor_F(const or_ctx *ctx, const double *v, int i, int j, double *d
 {
     const double dx = ctx->dx, dy = ctx->dy;
     double u = v[V_(ctx, i, j, 0)];
-    double vb = 1. / 4 * (v[V_(ctx, i - 1, j + 1, 1)] + v[V_(ctx, i, j + 1, 1)]
+    const int ia = or_mutation == MUT_CN_AVG_F ? i + 1 : i - 1;
+    double vb = 1. / 4 * (v[V_(ctx, ia, j + 1, 1)] + v[V_(ctx, i, j + 1, 1)]
                           + v[V_(ctx, i - 1, j, 1)] + v[V_(ctx, i, j, 1)]);
     double u2 = u * u, v2 = vb * vb;
-    double phi = ctx->P.porosity;
+    double phi = or_mutation == MUT_CN_DROP_PHI ? 1. : ctx->P.porosity;
     double maxdx = dx > dy ? dx : dy;   /* std::max(dx, dy) */
     double al = ctx->P.alpha_abs ? ctx->P.alpha_l : ctx->P.alpha_l * maxdx;
     double at = ctx->P.alpha_abs ? ctx->P.alpha_t : ctx->P.alpha_t * maxdx;
+    if (or_mutation == MUT_CN_SWAP_ALPHA) { double t_ = al; al = at; at = t_; }
     double s = sqrt(u2 + v2);
     double fl = phi * al / s, ft = phi * at / s;
     if (ctx->P.alpha_abs && !(s > 0)) { fl = 0; ft = 0; }
@@ -1263,20 +1304,23 @@ static void tensor_F(const or_ctx *ctx, const double *v, int i, int j, double *d
     *d11 = d11l + d11t;
     *d12 = d12l + d12t;
     if (ctx->P.d_m != 0) *d11 = *d11 + phi * ctx->P.d_m;
+    if (or_mutation == MUT_CN_DM_CROSS) *d12 = *d12 + phi * ctx->P.d_m;
 }
 
 /* SetupDiffusionTensorG (MGSolver.cpp:888-919) at y-face (i, j-1/2). */
 static void tensor_G(const or_ctx *ctx, const double *v, int i, int j, double *d22, double *d21)
 {
     const double dx = ctx->dx, dy = ctx->dy;
-    double ub = 1. / 4 * (v[V_(ctx, i + 1, j - 1, 0)] + v[V_(ctx, i + 1, j, 0)]
+    const int ia = or_mutation == MUT_CN_AVG_G ? i - 1 : i + 1;
+    double ub = 1. / 4 * (v[V_(ctx, ia, j - 1, 0)] + v[V_(ctx, i + 1, j, 0)]
                           + v[V_(ctx, i, j - 1, 0)] + v[V_(ctx, i, j, 0)]);
     double vv = v[V_(ctx, i, j, 1)];
     double u2 = ub * ub, v2 = vv * vv;
-    double phi = ctx->P.porosity;
+    double phi = or_mutation == MUT_CN_DROP_PHI ? 1. : ctx->P.porosity;
     double maxdx = dx > dy ? dx : dy;
     double al = ctx->P.alpha_abs ? ctx->P.alpha_l : ctx->P.alpha_l * maxdx;
     double at = ctx->P.alpha_abs ? ctx->P.alpha_t : ctx->P.alpha_t * maxdx;
+    if (or_mutation == MUT_CN_SWAP_ALPHA) { double t_ = al; al = at; at = t_; }
     double s = sqrt(u2 + v2);
     double fl = phi * al / s, ft = phi * at / s;
     if (ctx->P.alpha_abs && !(s > 0)) { fl = 0; ft = 0; }
@@ -1285,6 +1329,7 @@ static void tensor_G(const or_ctx *ctx, const double *v, int i, int j, double *d
     *d21 = d21l + d21t;
     *d22 = d22l + d22t;
     if (ctx->P.d_m != 0) *d22 = *d22 + phi * ctx->P.d_m;
+    if (or_mutation == MUT_CN_DM_CROSS) *d21 = *d21 + phi * ctx->P.d_m;
 }
 
 /* GetCoefficients_Parabolic (MGSolver.cpp:923-948). */
@@ -1337,7 +1382,8 @@ void or_rhs_parabolic(const or_ctx *ctx, const double *c, const double *v, doubl
                 4. / (dx * dx) * (d11_ip12j * (C_(i + 1, j) - C_(i, j))
                                   - d11_im12j * (C_(i, j) - C_(i - 1, j)))
                 + 1. / (dx * dy) * (d12_ip12j * (C_(i, j + 1) + C_(i + 1, j + 1) - C_(i, j - 1) - C_(i + 1, j - 1))
-                                    - d12_im12j * (C_(i, j + 1) + C_(i - 1, j + 1) - C_(i - 1, j - 1) - C_(i, j - 1)))
+                                    + (or_mutation == MUT_CN_RHS_CROSS ? 1. : -1.) * d12_im12j
+                                          * (C_(i, j + 1) + C_(i - 1, j + 1) - C_(i - 1, j - 1) - C_(i, j - 1)))
                 + 1. / (dx * dy) * (d21_ijp12 * (C_(i + 1, j) + C_(i + 1, j + 1) - C_(i - 1, j + 1) - C_(i - 1, j))
                                     - d21_ijm12 * (C_(i + 1, j) + C_(i + 1, j - 1) - C_(i - 1, j) - C_(i - 1, j - 1)))
                 + 4. / (dy * dy) * (d22_ijp12 * (C_(i, j + 1) - C_(i, j))

@@ -43,6 +43,8 @@ def lib():
         dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
         vp = C.c_void_p
         L.or_ctx_size.restype = C.c_size_t
+        L.or_set_mutation.argtypes = [C.c_int]
+        L.or_set_mutation.restype = C.c_int
         L.or_params_size.restype = C.c_size_t
         L.or_init.argtypes = [vp, C.POINTER(OrParams), vp, vp]
         for fn in ["or_fill_neumann_scalar", "or_fill_periodic_scalar", "or_fill_boundaries",
@@ -258,6 +260,28 @@ class Oracle:
 
     def set_kappa(self, kappa):
         self.L.or_set_kappa(self.ctx, kappa)
+
+
+# planted mistakes of the oracle (impec_oracle.c, or_set_mutation): tests use
+# them to show that their pins fail on a wrong formula
+MUTATIONS = {"cn_swap_alpha": 1, "cn_drop_phi": 2, "cn_avg_f": 3, "cn_avg_g": 4, "cn_dm_cross": 5,
+             "cn_rhs_cross": 6, "split_xfirst": 7, "split_yfirst": 8, "split_strang": 9, "split_alt": 10,
+             "split_parallel": 11}
+
+
+class mutated:
+    """Context manager: the oracle with one planted mistake (TEST ONLY)."""
+
+    def __init__(self, name):
+        self.m = MUTATIONS[name]
+
+    def __enter__(self):
+        assert lib().or_set_mutation(self.m) == 0
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_mutation(0)
+        return False
 
 
 def matvec9(N1, N2, A, x):

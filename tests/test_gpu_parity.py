@@ -285,6 +285,48 @@ def test_steps_1_and_10(gpu_lib, kw):
             assert rel_err(vg, v) < 1e-10, k
             assert rel_err(o.interior(cg), o.interior(c)) < 1e-10, k
             assert abs(dt_gpu - dt_ref) <= 1e-10 * dt_ref
+            # diagnostics of the same states (Impec2DSolver.cpp:530-620): mass
+            # and produced mass to 1e-10 relative, the balance and the
+            # divergence / wall residuals (rounding-level numbers) absolutely
+            assert abs(g.total_mass() - o.mass(c)) <= 1e-10 * o.mass(c)
+            tmo_g, tmo_o = g.stats().total_mass_outflux, o.total_mass_outflux
+            assert abs(tmo_g - tmo_o) <= 1e-10 * max(tmo_o, 1e-300) or tmo_o == tmo_g == 0.0
+            assert abs(g.mass_error(t_gpu) - o.mass_error(c, t_ref)) <= 1e-11
+            dq = 2.0 * p["I"]   # sum |q| dx dy of the two wells
+            assert abs(g.divergence_error() - o.div_error(v, q)) <= 1e-9 * dq
+            assert abs(g.velocity_boundary_error() - o.noflow_error(v)) <= 1e-12
+
+
+@pytest.mark.parametrize("kw", [dict(nu=1.0), dict(nu=2 / 3.0, stype=0, x_ub=2 ** 0.5, y_ub=2 ** 0.5),
+                                dict(nu=1.0, N1=37, N2=21)])
+def test_diagnostics_match_oracle(gpu_lib, kw):
+    """impec_total_mass, impec_mass_error, impec_divergence_error and
+    impec_velocity_boundary_error (Impec2DSolver.cpp:530-620) against the
+    oracle's values on a state where none of them is near zero: random c,
+    random face velocities including the wall faces, the well sources."""
+    p = params(**dict(dict(N1=32, N2=32), **kw))
+    _say("diagnostics", kw)
+    o, g = pair(p)
+    c = random_c(o, seed=21)
+    rng = np.random.default_rng(22)
+    v = rng.uniform(-3.0, 3.0, o.velocity().shape)
+    q = o.source("one_cell")
+    g.set_concentration(c)
+    g.set_velocity(v)
+    g.set_source(q)
+    m_ref = o.mass(c)
+    assert m_ref > 0
+    assert abs(g.total_mass() - m_ref) <= 1e-13 * m_ref
+    for t in (0.37, 1.25):
+        e_ref = o.mass_error(c, t)
+        assert e_ref > 1e-3
+        assert abs(g.mass_error(t) - e_ref) <= 1e-13 * e_ref
+    d_ref = o.div_error(v, q)
+    assert d_ref > 1e-3
+    assert abs(g.divergence_error() - d_ref) <= 1e-13 * d_ref
+    w_ref = o.noflow_error(v)
+    assert w_ref > 1e-3
+    assert abs(g.velocity_boundary_error() - w_ref) <= 1e-13 * w_ref
 
 
 @pytest.mark.parametrize("n1,n2", [(256, 256), (200, 120), (77, 45), (2048, 2048)])

@@ -57,6 +57,9 @@ def test_large_grid_invariants(gpu_lib, n, steps):
         t += ctx.step(t)
         s = ctx.stats()
         assert s.last_relres <= 1e-10
+        # the dispersion solve's true residual ||b - A x|| / ||b||, recomputed
+        # after the BiCGStab recursion reports convergence
+        assert 0.0 < s.last_parabolic_relres <= 1e-10
     assert ctx.mass_error(t) < 1e-9
     assert ctx.velocity_boundary_error() == 0.0        # nu = 1: exact zero normal flux
     h = 1.0 / n
