@@ -30,11 +30,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def workload_params(capi, n, precision="fp64", guess="zero", mg_rows=0, cn_guess=0):
+def workload_params(capi, n, precision="fp64", guess="zero", mg_rows=0, cn_guess=0, grid="diagonal"):
+    """The headline physics at n^2.  grid "diagonal": the unit square with the
+    injector at (0,0) and the producer at (1,1) (reference source_type
+    "diagonal"); "rotated": the 45-degree rotated square of side sqrt(2) with
+    injectors at two opposite corners and producers at the other two
+    (reference source_type "parallel")."""
+    ext = 1.0 if grid == "diagonal" else 2 ** 0.5
     return capi.default_params(
-        N1=n, N2=n, source_type=capi.SRC_DIAGONAL, average_procedure=capi.AVG_HARMONIC,
+        N1=n, N2=n, source_type=capi.SRC_DIAGONAL if grid == "diagonal" else capi.SRC_PARALLEL,
+        average_procedure=capi.AVG_HARMONIC,
         dim_splitting=capi.SPLIT_PARALLEL, hyper_method=capi.HYPER_MUSCL, diffusion=1,
-        para_method=capi.PARA_CN, alpha_absolute=1, x_lb=0.0, x_ub=1.0, y_lb=0.0, y_ub=1.0,
+        para_method=capi.PARA_CN, alpha_absolute=1, x_lb=0.0, x_ub=ext, y_lb=0.0, y_ub=ext,
         nu=1.0, mu_res=1.0, mu_inj=0.1, porosity=1.0, cfl=0.5, inj_fluid_conc=1.0,
         alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5, eps=1e-10, para_eps=1e-10, I=1.0,
         precision={"mixed": capi.PREC_MIXED, "mixed32": capi.PREC_MIXED32}.get(precision, capi.PREC_FP64),
@@ -89,7 +96,8 @@ def modes(capi, n, warmup=3, steps=8):
     net, train_info = train_guess_network(capi)
     res = {"workload": "lognormal K (sigma 1, corr. length 0.05), M=20, no dispersion, %d^2, rel-res 1e-10" % n,
            "learned_training": train_info}
-    for name, prec, guess in [("fp64_zero", "fp64", "zero"), ("mixed_zero", "mixed", "zero"),
+    for name, prec, guess in [("fp64_zero", "fp64", "zero"), ("fp64_previous", "fp64", "previous"),
+                              ("mixed_zero", "mixed", "zero"),
                               ("mixed_previous", "mixed", "previous"), ("mixed_extrapolated", "mixed", "extrapolate"),
                               ("mixed_learned", "mixed", "learned"), ("mixed32_zero", "mixed32", "zero"),
                               ("mixed32_previous", "mixed32", "previous"), ("mixed32_learned", "mixed32", "learned")]:
@@ -125,10 +133,39 @@ def modes(capi, n, warmup=3, steps=8):
                                                  / res["mixed_extrapolated"]["ms_per_step"])
     res["speedup_mixed_previous_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_previous"]["ms_per_step"]
     res["speedup_mixed_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_zero"]["ms_per_step"]
+    # the warm start's gain alone (FP64) and the precision's gain alone (previous guess)
+    res["speedup_fp64_previous_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["fp64_previous"]["ms_per_step"]
+    res["speedup_mixed_previous_vs_fp64_previous"] = (res["fp64_previous"]["ms_per_step"]
+                                                      / res["mixed_previous"]["ms_per_step"])
     for g in ("zero", "previous", "learned"):
         res["speedup_mixed32_%s_vs_fp64" % g] = (res["fp64_zero"]["ms_per_step"]
                                                  / res["mixed32_%s" % g]["ms_per_step"])
     return res
+
+
+def rotated(capi, n, warmup=3, steps=10):
+    """The headline physics on the 45-degree rotated grid (reference
+    source_type "parallel": the rotated square of side sqrt(2), injectors at
+    two opposite corners, producers at the other two) at n^2, FP64."""
+    ctx = capi.Context(workload_params(capi, n, grid="rotated"))
+    ctx.set_source(corner_wells_source(n))
+    t = 0.0
+    for _ in range(warmup):
+        t += ctx.step(t)
+    ctx.synchronize()
+    s0 = ctx.stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t += ctx.step(t)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    s1 = ctx.stats()
+    ctx.close()
+    return {"workload": "rotated square %d^2, two injectors and two producers at the corners, otherwise the "
+                        "headline workload (M=10, Peaceman CN dispersion, FP64, zero guesses)" % n,
+            "ms_per_step": 1e3 * el / steps, "cell_updates_per_s": n * n * steps / el,
+            "vcycles_per_step": (s1.total_its - s0.total_its) / steps, "parabolic_its_last": s1.last_parabolic_its,
+            "pressure_relres_last": s1.last_relres, "parabolic_relres_last": s1.last_parabolic_relres}
 
 
 def corner_wells_source(n, I=1.0, extent=2 ** 0.5):
@@ -218,25 +255,60 @@ def ensemble(capi, members=256, warmup=2, steps=10, loop_steps=1, seed=2024, gra
     }
 
 
-def cpu_baseline(n=1024, steps=8):
-    """Times the C oracle (FP64 restatement of the reference algorithm,
-    one thread) on `steps` steps of the same workload at n^2."""
-    from oracle import pyoracle as po
-    P = po.default_params(N1=n, N2=n, mu_res=1.0, mu_inj=0.1, nu=1.0, eps=1e-10, para_eps=1e-10,
-                          diffusion=1, alpha_abs=1, alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5)
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_rate(po, P, steps):
+    """cell-updates/s of the oracle's IMPEC step on P after one untimed step."""
     o = po.Oracle(P)
     c, p, v, q = o.scalar(), o.scalar(), o.velocity(), o.source()
     t = 0.0
-    st, dt = o.step(c, p, v, q, t)      # untimed first step
+    st, dt = o.step(c, p, v, q, t)
     t += dt
     t0 = time.perf_counter()
     for _ in range(steps):
         st, dt = o.step(c, p, v, q, t)
         t += dt
     el = time.perf_counter() - t0
-    return {"value": n * n * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": "oracle/impec_oracle.c, %d steps at %d^2 after 1 untimed step, same physics and "
-                      "tolerances, single thread, %.1f s" % (steps, n, el)}
+    return P["N1"] * P["N2"] * steps / el, el
+
+
+def cpu_baseline(n=1024, steps=4, c1_steps=100):
+    """Times the C oracle (FP64 restatement of the reference algorithm, its
+    per-cell loops under OpenMP) on the host cores: `steps` steps of the
+    headline workload at n^2 and `c1_steps` steps of the CPU configuration
+    (64^2, M=10, no dispersion), each with all cores and with one."""
+    import os
+    from oracle import pyoracle as po
+    ncores = len(os.sched_getaffinity(0))
+    # the process's CPU share: OMP_NUM_THREADS when the host sets it
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and 0 < int(env) < ncores:
+        ncores = int(env)
+    head = po.default_params(N1=n, N2=n, mu_res=1.0, mu_inj=0.1, nu=1.0, eps=1e-10, para_eps=1e-10,
+                             diffusion=1, alpha_abs=1, alpha_l=1e-3, alpha_t=1e-4, d_m=1e-5)
+    c1 = po.default_params(N1=64, N2=64, mu_res=1.0, mu_inj=0.1, nu=1.0, eps=1e-10)
+    res = {}
+    for threads in (ncores, 1):
+        used = po.set_threads(threads)
+        res[threads] = (used, _oracle_rate(po, head, steps), _oracle_rate(po, c1, c1_steps))
+    po.set_threads(ncores)
+    used, (rate, el), (rate64, el64) = res[ncores]
+    _, (rate1, el1), (rate64_1, _) = res[1]
+    return {"value": rate, "unit": "cell-updates/s", "cores": used, "kind": "port",
+            "sample": "oracle/impec_oracle.c, %d steps of the headline workload at %d^2 after 1 untimed step "
+                      "(same physics and tolerances), %d OpenMP threads, %.1f s" % (steps, n, used, el),
+            "cpu_model": _cpu_model(), "nproc": ncores,
+            "single_core_value": rate1, "single_core_s": el1,
+            "c1_64": {"workload": "64^2, M=10, no dispersion, %d steps" % c1_steps,
+                      "all_cores": rate64, "single_core": rate64_1}}
 
 
 def main():
@@ -253,6 +325,7 @@ def main():
     ap.add_argument("--no-modes", action="store_true",
                     help="skip the plain / mixed / warm-start comparison on the lognormal-K workload")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-rotated", action="store_true", help="skip the rotated-grid line")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched-ensemble workload (256 realisations of mixed sizes)")
     ap.add_argument("--ensemble-members", type=int, default=256)
@@ -302,7 +375,8 @@ def main():
                                "MUSCL parallel splitting, CFL 0.5, rel-res 1e-10, %s, %s initial guess"
                                % (n, args.precision, args.guess),
                    "N": n, "vcycles_per_step": vcyc,
-                   "parabolic_its_last": s1.last_parabolic_its, "pressure_relres_last": s1.last_relres},
+                   "parabolic_its_last": s1.last_parabolic_its, "pressure_relres_last": s1.last_relres,
+                   "parabolic_relres_last": s1.last_parabolic_relres},
     }
 
     if not args.no_roofline:
@@ -310,6 +384,8 @@ def main():
         out["roofline"] = roofline.measure(ctx, n, t, fp32_cycle=args.precision in ("mixed", "mixed32"))
         out["roofline_by_kernel"] = roofline.measure.last_table
     ctx.close()
+    if not args.no_rotated:
+        out["rotated_grid"] = rotated(capi, n)
     if not args.no_modes:
         out["modes"] = modes(capi, n)
     if not args.no_ensemble:

@@ -9,7 +9,7 @@
 //               [--diffusion] [--alpha-l x] [--alpha-t x] [--d-m x] [--alpha-absolute]
 //               [--upwind] [--split alternating|x_first|y_first|strang|parallel]
 //               [--meanconc] [--porosity x] [--guess zero|previous] [--quiet]
-//               [--checks] [--dump file] [--diagnostics file]
+//               [--checks] [--dump file] [--diagnostics file] [--recovery file]
 //
 // --config reads an Impec2D configuration file (libconfig syntax, the keys of
 // SimParams.cpp:45-463) and, as the reference does, writes the final fields
@@ -18,8 +18,11 @@
 // is_test = true runs the convergence test its test section names
 // (Impec2DSolver::TestComponent) and writes the norm and plot files there.
 // --dump writes c, p (each (N1+4) x (N2+4)) and v ((N1+5) x (N2+5) x 2) as raw
-// little-endian doubles, in that order.  Exit status 0 on success, 1 on a
-// solver error, 2 on a usage or configuration error, 3 when no GPU can be used.
+// little-endian doubles, in that order.  --recovery writes the long-run
+// measures of every step (t, produced mass, recovered fraction, produced
+// concentration; impec2d_amd/recovery.py defines them) as text.  Exit status
+// 0 on success, 1 on a solver error, 2 on a usage or configuration error, 3
+// when no GPU can be used.
 #include "impec_config.hpp"
 #include "impec_solvers.hpp"
 
@@ -36,14 +39,14 @@ static void usage() {
     std::cerr << "usage: impec2d_run [--config file.cfg] [--N n] [--steps k] [--tfinal t] [--stype diagonal|parallel] [--nu x]\n"
                  "                   [--diffusion] [--alpha-l x] [--alpha-t x] [--d-m x] [--alpha-absolute]\n"
                  "                   [--upwind] [--split mode] [--meanconc] [--guess zero|previous]\n"
-                 "                   [--quiet] [--checks] [--dump file] [--diagnostics file]\n";
+                 "                   [--quiet] [--checks] [--dump file] [--diagnostics file] [--recovery file]\n";
 }
 
 int main(int argc, char **argv) {
     SimParams sp;
     sp.tfinal = 1e30;
     sp.max_steps = 10;
-    std::string dump, diag;
+    std::string dump, diag, recovery;
     bool print_params = false;
     for (int a = 1; a + 1 < argc; a++)   // a configuration file first; the options override it
         if (std::string(argv[a]) == "--config") {
@@ -101,6 +104,7 @@ int main(int argc, char **argv) {
         else if (k == "--log-dir") sp.log_dir = val();
         else if (k == "--print-params") print_params = true;
         else if (k == "--diagnostics") diag = val();
+        else if (k == "--recovery") recovery = val();
         else {
             usage();
             return 2;
@@ -147,6 +151,7 @@ int main(int argc, char **argv) {
         Impec2DSolver solver(sp);
         const bool ok = solver.Run();
         if (!diag.empty()) solver.WriteDiagnostics(diag);
+        if (!recovery.empty()) solver.WriteRecovery(recovery);
         if (!dump.empty()) {
             FILE *f = std::fopen(dump.c_str(), "wb");
             if (!f) {

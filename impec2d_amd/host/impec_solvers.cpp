@@ -593,6 +593,30 @@ void Impec2DSolver::WriteDiagnostics(const std::string &fname) const {
     }
 }
 
+void Impec2DSolver::WriteRecovery(const std::string &fname) const {
+    std::ofstream file(fname.c_str());
+    if (!file.is_open()) {
+        std::cerr << "Unable to write to recovery-file: " << fname << std::endl;
+        return;
+    }
+    file.precision(17);
+    // injected mass per unit time in the mass-balance units of CalculateMassError
+    // (Impec2DSolver.cpp:530-540): I / (dx dy), twice for two injectors
+    const double rate = sp.I / (sp.dx * sp.dy) * (sp.stype == PARALLEL ? 2. : 1.);
+    const double pv = (double)sp.N1 * sp.N2 * sp.porosity;
+    file << "# counter t dt outflux injected recovered produced_conc\n";
+    double prev_out = 0., prev_inj = 0.;
+    for (const DiagState &ds : diagstates) {
+        const double inj = (ds.t - sp.tstart) * rate;
+        const double d_inj = inj - prev_inj;
+        const double cp = d_inj > 0 ? (ds.outflux - prev_out) / d_inj : 0.;
+        file << ds.ctr << " " << ds.t << " " << ds.dt << " " << ds.outflux << " " << inj << " "
+             << (inj - ds.outflux) / pv << " " << cp << "\n";
+        prev_out = ds.outflux;
+        prev_inj = inj;
+    }
+}
+
 // Impec2DSolver::Run (Impec2DSolver.cpp:747-760), without the gnuplot pause
 bool Impec2DSolver::Run() { return sp.is_test ? TestComponent() : Solve(); }
 
@@ -632,6 +656,7 @@ bool Impec2DSolver::Solve() {
         ds.dt = dt;
         ds.its = es->GetLastIts();
         ds.fillperdt = dt * sp.I / (sp.dx * sp.dy);
+        ds.outflux = hs->GetTotalMassOutflux();
         ds.mass_err = sp.check_mass_conservation ? CalculateMassError(t) : -1;
         ds.vel_err = sp.check_noflow_boundary ? CalculateVelocityBoundaryError() : -1;
         ds.diff_err = sp.check_divergence ? CalculateDivergenceError() : -1;

@@ -361,6 +361,7 @@ struct DiagState {   // Impec2DSolver.h
     double diff_err;
     double vel_err;
     double fillperdt;
+    double outflux;   // HyperbolicSolver::GetTotalMassOutflux after the step
 };
 
 class Impec2DSolver {
@@ -379,6 +380,10 @@ class Impec2DSolver {
     double Advection();
     const std::vector<DiagState> &Diagnostics() const { return diagstates; }
     void WriteDiagnostics(const std::string &fname) const;   // diagnostics.dat columns
+    // the long-run measures of every step (impec2d_amd/recovery.py defines
+    // them): counter, t, dt, outflux, injected, recovered fraction of the
+    // resident fluid, produced concentration
+    void WriteRecovery(const std::string &fname) const;
     Array2d &Concentration() { return c; }
     Array2d &Pressure() { return p; }
     Array3d &Velocity() { return v; }

@@ -25,6 +25,11 @@
  * Extensions (off by default, flagged in or_params): max_steps, per-cell K
  * arrays, molecular dispersion d_m, absolute dispersivities.
  *
+ * Threads: the per-cell loops that write disjoint outputs run under OpenMP
+ * (or_set_threads), so every result is the same bit for bit whatever the
+ * thread count.  Sums over vectors of 65536 or more entries add fixed chunks
+ * of 16384 in order; shorter ones, and every other reduction, stay serial.
+ *
  * Array layouts are the reference's (GenSimDefs.h:22-31, Impec2DSolver.cpp:151-182):
  *   scalar field  (N1+2b) x (N2+2b) doubles, row-major, j contiguous, base -b
  *   velocity      (N1+1+2b) x (N2+1+2b) x 2 doubles, component last, base -b
@@ -35,6 +40,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ------------------------------------------------------------------------- */
 /* Planted mistakes (tests only).  or_set_mutation(m) makes one restated      */
@@ -57,6 +65,17 @@ enum {
     MUT_COUNT
 };
 static int or_mutation = MUT_NONE;
+int or_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
 int or_set_mutation(int m)
 {
     if (m < 0 || m >= MUT_COUNT) return -1;
@@ -451,9 +470,10 @@ static void fold_boundary(int i, int j, int N1, int N2, const double a[9], doubl
 void or_assemble_elliptic(const or_ctx *ctx, const double *c, double *A)
 {
     const int N1 = ctx->N1, N2 = ctx->N2, M = N1 * N2;
-    double a[9], f[9];
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < N1; i++)
         for (int j = 0; j < N2; j++) {
+            double a[9], f[9];
             weighted_coeffs(ctx, c, i, j, a);
             fold_boundary(i, j, N1, N2, a, f);
             for (int k = 0; k < 9; k++) A[k * M + i * N2 + j] = f[k];
@@ -468,6 +488,7 @@ void or_assemble_elliptic(const or_ctx *ctx, const double *c, double *A)
 void or_matvec9(int N1, int N2, const double *A, const double *x, double *y)
 {
     const int M = N1 * N2;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < N1; i++)
         for (int j = 0; j < N2; j++) {
             double s = 0;
@@ -516,6 +537,7 @@ static void level_assemble(const or_ctx *ctx, or_level *L, const double *w, int 
     const double dxi = dx / cos(ctx->theta), deta = dy / cos(ctx->alpha);
     const int harm = ctx->P.avgproc == 0;
     const double s2 = ctx->secamt2, nu = ctx->P.nu;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n1; i++)
         for (int j = 0; j < n2; j++) {
             double fx[2], fy[2], vt[4];
@@ -631,6 +653,9 @@ static int color_of(int i, int j, int ncolor)
 static void gs_color(or_level *L, int color, int ncolor)
 {
     const int n1 = L->n1, n2 = L->n2, M = n1 * n2;
+    /* the cells of one colour do not read each other (or read them with a
+     * zero coefficient), so the colour sweep runs in any order */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n1; i++)
         for (int j = 0; j < n2; j++) {
             if (color_of(i, j, ncolor) != color) continue;
@@ -651,6 +676,7 @@ static void project_mean(double *x, int n)
     double s = 0;
     for (int k = 0; k < n; k++) s += x[k];
     s /= n;
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < n; k++) x[k] -= s;
 }
 
@@ -681,22 +707,38 @@ static void vcycle(or_mg *mg, int l)
     }
     for (int c = 0; c < nc; c++) gs_color(L, c, nc);
     or_matvec9(n1, n2, L->A, L->x, L->r);
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < M; k++) L->r[k] = L->b[k] - L->r[k];
     or_level *Lc = &mg->L[l + 1];
     const int m1 = Lc->n1, m2 = Lc->n2;
-    memset(Lc->b, 0, sizeof(double) * m1 * m2);
-    for (int i = 0; i < n1; i++) {
-        int Ia, Ib; prolong1d(i, m1, &Ia, &Ib);
-        for (int j = 0; j < n2; j++) {
-            int Ja, Jb; prolong1d(j, m2, &Ja, &Jb);
-            double r = L->r[i * n2 + j] * 0.25;
-            Lc->b[Ia * m2 + Ja] += 0.5625 * r;
-            Lc->b[Ia * m2 + Jb] += 0.1875 * r;
-            Lc->b[Ib * m2 + Ja] += 0.1875 * r;
-            Lc->b[Ib * m2 + Jb] += 0.0625 * r;
+    /* full weighting, the transpose of the prolongation: each fine residual
+     * r/4 goes to its four coarse cells with weights 9/16, 3/16, 3/16, 1/16.
+     * Every coarse cell gathers its contributions in the order a row-major
+     * scatter over the fine cells would add them (fine rows, then columns,
+     * then the four targets in turn), so the sums do not depend on threads. */
+#pragma omp parallel for schedule(static)
+    for (int I = 0; I < m1; I++)
+        for (int J = 0; J < m2; J++) {
+            double acc = 0;
+            const int i0 = 2 * I - 3 > 0 ? 2 * I - 3 : 0, i1 = 2 * I + 4 < n1 - 1 ? 2 * I + 4 : n1 - 1;
+            const int j0 = 2 * J - 3 > 0 ? 2 * J - 3 : 0, j1 = 2 * J + 4 < n2 - 1 ? 2 * J + 4 : n2 - 1;
+            for (int i = i0; i <= i1; i++) {
+                int Ia, Ib; prolong1d(i, m1, &Ia, &Ib);
+                if (Ia != I && Ib != I) continue;
+                for (int j = j0; j <= j1; j++) {
+                    int Ja, Jb; prolong1d(j, m2, &Ja, &Jb);
+                    if (Ja != J && Jb != J) continue;
+                    double r = L->r[i * n2 + j] * 0.25;
+                    if (Ia == I && Ja == J) acc += 0.5625 * r;
+                    if (Ia == I && Jb == J) acc += 0.1875 * r;
+                    if (Ib == I && Ja == J) acc += 0.1875 * r;
+                    if (Ib == I && Jb == J) acc += 0.0625 * r;
+                }
+            }
+            Lc->b[I * m2 + J] = acc;
         }
-    }
     vcycle(mg, l + 1);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n1; i++) {
         int Ia, Ib; prolong1d(i, m1, &Ia, &Ib);
         for (int j = 0; j < n2; j++) {
@@ -711,10 +753,26 @@ static void vcycle(or_mg *mg, int l)
 /* ------------------------------------------------------------------------- */
 /* linear solves                                                              */
 /* ------------------------------------------------------------------------- */
+#define DOT_CHUNK 16384
 static double dot(const double *a, const double *b, int n)
 {
+    if (n < 4 * DOT_CHUNK) {
+        double s = 0;
+        for (int k = 0; k < n; k++) s += a[k] * b[k];
+        return s;
+    }
+    const int nch = (n + DOT_CHUNK - 1) / DOT_CHUNK;
+    double *part = malloc(sizeof(double) * nch);
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < nch; q++) {
+        const int k1 = (q + 1) * DOT_CHUNK < n ? (q + 1) * DOT_CHUNK : n;
+        double s = 0;
+        for (int k = q * DOT_CHUNK; k < k1; k++) s += a[k] * b[k];
+        part[q] = s;
+    }
     double s = 0;
-    for (int k = 0; k < n; k++) s += a[k] * b[k];
+    for (int q = 0; q < nch; q++) s += part[q];
+    free(part);
     return s;
 }
 
@@ -786,11 +844,13 @@ static int pcg_pinned(or_ctx *ctx, const double *A, const double *A_N, const dou
         if (its == 0) memcpy(d, z, sizeof(double) * M);
         else {
             double beta = rho / rho_old;
+#pragma omp parallel for schedule(static)
             for (int k = 0; k < M; k++) d[k] = z[k] + beta * d[k];
         }
         or_matvec9(N1, N2, A_N, d, q);
         if (!shift) q[pin] = 0;
         double alpha = rho / dot(d, q, M);
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < M; k++) { x[k] += alpha * d[k]; r[k] -= alpha * q[k]; }
         rho_old = rho;
         its++;
@@ -832,15 +892,22 @@ static int bicgstab(or_ctx *ctx, const double *A, const double *b, double *x, do
         double rho_new = dot(r0, r, M);
         if (rho_new == 0) { status = 2; break; }
         double beta = (rho_new / rho) * (alpha / omega);
-        for (int k = 0; k < M; k++) p[k] = r[k] + beta * (p[k] - omega * v[k]);
-        for (int k = 0; k < M; k++) y[k] = p[k] / A[ST_C * M + k];
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < M; k++) {
+            p[k] = r[k] + beta * (p[k] - omega * v[k]);
+            y[k] = p[k] / A[ST_C * M + k];
+        }
         or_matvec9(N1, N2, A, y, v);
         alpha = rho_new / dot(r0, v, M);
-        for (int k = 0; k < M; k++) s[k] = r[k] - alpha * v[k];
-        for (int k = 0; k < M; k++) zz[k] = s[k] / A[ST_C * M + k];
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < M; k++) {
+            s[k] = r[k] - alpha * v[k];
+            zz[k] = s[k] / A[ST_C * M + k];
+        }
         or_matvec9(N1, N2, A, zz, t);
         double tt = dot(t, t, M);
         omega = tt > 0 ? dot(t, s, M) / tt : 0;
+#pragma omp parallel for schedule(static)
         for (int k = 0; k < M; k++) { x[k] += alpha * y[k] + omega * zz[k]; r[k] = s[k] - omega * t[k]; }
         rho = rho_new;
         its++;
@@ -1004,9 +1071,11 @@ void or_calc_velocities(const or_ctx *ctx, double *p, double *v, const double *c
 {
     or_fill_neumann_scalar(ctx, p);
     const double nu = ctx->P.nu;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i <= ctx->N1; i++)
         for (int j = 0; j < ctx->N2; j++)
             v[V_(ctx, i, j, 0)] = nu * par_u1(ctx, c, p, i, j) + (1. - nu) * diag_u1(ctx, c, p, i, j);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < ctx->N1; i++)
         for (int j = 0; j <= ctx->N2; j++)
             v[V_(ctx, i, j, 1)] = nu * par_u2(ctx, c, p, i, j) + (1. - nu) * diag_u2(ctx, c, p, i, j);
@@ -1153,6 +1222,7 @@ static void flux_F(const or_ctx *ctx, double *F, const double *c, const double *
     const int N1 = ctx->N1, N2 = ctx->N2, b = ctx->b, ldf = N2 + 2 * b;
 #define F_(i, j) F[(i) * ldf + (j) + b]
     if (ctx->P.hyper == 1) {
+#pragma omp parallel for schedule(static)
         for (int i = 0; i <= N1; i++)
             for (int j = -b; j < N2 + b; j++)
                 F_(i, j) = upwind(ctx, v[V_(ctx, i, j, 0)], c[S_(ctx, i - 1, j)], c[S_(ctx, i, j)]);
@@ -1162,6 +1232,7 @@ static void flux_F(const or_ctx *ctx, double *F, const double *c, const double *
     const int ldu = N2 + 1 + 2 * b;
     double *U = calloc((size_t)(N1 + 3) * ldu * 2, sizeof(double));
 #define U_(i, j, k) U[(((i) + 1) * ldu + ((j) + b)) * 2 + (k)]
+#pragma omp parallel for schedule(static)
     for (int i = -1; i <= N1; i++)
         for (int j = -b; j < N2 + b; j++) {
             double UL, UR;
@@ -1170,6 +1241,7 @@ static void flux_F(const or_ctx *ctx, double *F, const double *c, const double *
             U_(i + 1, j, 0) = UR + df;
             U_(i, j, 1) = UL + df;
         }
+#pragma omp parallel for schedule(static)
     for (int i = 0; i <= N1; i++)
         for (int j = -b; j < N2 + b; j++)
             F_(i, j) = upwind(ctx, v[V_(ctx, i, j, 0)], U_(i, j, 0), U_(i, j, 1));
@@ -1184,6 +1256,7 @@ static void flux_G(const or_ctx *ctx, double *G, const double *c, const double *
     const int N1 = ctx->N1, N2 = ctx->N2, b = ctx->b, ldg = N2 + 1;
 #define G_(i, j) G[((i) + b) * ldg + (j)]
     if (ctx->P.hyper == 1) {
+#pragma omp parallel for schedule(static)
         for (int i = -b; i < N1 + b; i++)
             for (int j = 0; j <= N2; j++)
                 G_(i, j) = upwind(ctx, v[V_(ctx, i, j, 1)], c[S_(ctx, i, j - 1)], c[S_(ctx, i, j)]);
@@ -1192,6 +1265,7 @@ static void flux_G(const or_ctx *ctx, double *G, const double *c, const double *
     const int ldu = N2 + 3;
     double *U = calloc((size_t)(N1 + 1 + 2 * b) * ldu * 2, sizeof(double));
 #define U_(i, j, k) U[(((i) + b) * ldu + ((j) + 1)) * 2 + (k)]
+#pragma omp parallel for schedule(static)
     for (int i = -b; i < N1 + b; i++)
         for (int j = -1; j <= N2; j++) {
             double UL, UR;
@@ -1200,6 +1274,7 @@ static void flux_G(const or_ctx *ctx, double *G, const double *c, const double *
             U_(i, j + 1, 0) = UR + df;
             U_(i, j, 1) = UL + df;
         }
+#pragma omp parallel for schedule(static)
     for (int i = -b; i < N1 + b; i++)
         for (int j = 0; j <= N2; j++)
             G_(i, j) = upwind(ctx, v[V_(ctx, i, j, 1)], U_(i, j, 0), U_(i, j, 1));
@@ -1218,6 +1293,7 @@ void or_advect_x(const or_ctx *ctx, double *c, const double *v, double dt)
     const int N1 = ctx->N1, N2 = ctx->N2, b = ctx->b, ldf = N2 + 2 * b;
     double *F = calloc((size_t)(N1 + 1) * ldf, sizeof(double));
     flux_F(ctx, F, c, v, dt);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < N1; i++)
         for (int j = -b; j < N2 + b; j++)
             c[S_(ctx, i, j)] += dt / ctx->dx * (F[i * ldf + j + b] - F[(i + 1) * ldf + j + b]);
@@ -1230,6 +1306,7 @@ void or_advect_y(const or_ctx *ctx, double *c, const double *v, double dt)
     const int N1 = ctx->N1, N2 = ctx->N2, b = ctx->b, ldg = N2 + 1;
     double *G = calloc((size_t)(N1 + 2 * b) * ldg, sizeof(double));
     flux_G(ctx, G, c, v, dt);
+#pragma omp parallel for schedule(static)
     for (int i = -b; i < N1 + b; i++)
         for (int j = 0; j < N2; j++)
             c[S_(ctx, i, j)] += dt / ctx->dy * (G[(i + b) * ldg + j] - G[(i + b) * ldg + j + 1]);
@@ -1357,9 +1434,10 @@ static void para_coeffs(const or_ctx *ctx, const double *v, int i, int j, double
 void or_assemble_parabolic(const or_ctx *ctx, const double *v, double dt, double *A)
 {
     const int N1 = ctx->N1, N2 = ctx->N2, M = N1 * N2;
-    double a[9], f[9];
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < N1; i++)
         for (int j = 0; j < N2; j++) {
+            double a[9], f[9];
             para_coeffs(ctx, v, i, j, dt, a);
             fold_boundary(i, j, N1, N2, a, f);
             for (int k = 0; k < 9; k++) A[k * M + i * N2 + j] = f[k];
@@ -1371,6 +1449,7 @@ void or_rhs_parabolic(const or_ctx *ctx, const double *c, const double *v, doubl
 {
     const double dx = ctx->dx, dy = ctx->dy;
 #define C_(i, j) c[S_(ctx, i, j)]
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < ctx->N1; i++)
         for (int j = 0; j < ctx->N2; j++) {
             double d11_im12j, d12_im12j, d22_ijm12, d21_ijm12, d21_ijp12, d22_ijp12, d12_ip12j, d11_ip12j;

@@ -43,6 +43,8 @@ def lib():
         dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
         vp = C.c_void_p
         L.or_ctx_size.restype = C.c_size_t
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_threads.restype = C.c_int
         L.or_set_mutation.argtypes = [C.c_int]
         L.or_set_mutation.restype = C.c_int
         L.or_params_size.restype = C.c_size_t
@@ -260,6 +262,12 @@ class Oracle:
 
     def set_kappa(self, kappa):
         self.L.or_set_kappa(self.ctx, kappa)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's per-cell loops (results do not depend on
+    it); returns the count in effect."""
+    return lib().or_set_threads(int(n))
 
 
 # planted mistakes of the oracle (impec_oracle.c, or_set_mutation): tests use

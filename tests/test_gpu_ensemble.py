@@ -7,7 +7,7 @@ picked by the autotuner keep that equality."""
 import numpy as np
 import pytest
 
-from helpers import gpu_params, pair, params, smooth_c
+from helpers import gpu_params, pair, params, rel_err, smooth_c
 from impec2d_amd import fields
 from oracle import pyoracle as po
 
@@ -170,3 +170,106 @@ def test_autotuned_rows_keep_member_equality(gpu_lib):
             assert res[s][1][k] == single[s][1]
             c_e, p_e, v_e = res[s][2][k]
             assert np.array_equal(p_e, single[s][3]) and np.array_equal(c_e, single[s][2])
+
+
+def test_members_match_oracle(gpu_lib):
+    """One member of each grid type against the oracle stepped with the same
+    lognormal K: p, u, v, c after 3 steps within 1e-10 (solves at 1e-13)."""
+    from impec2d_amd import capi
+    specs = [(40, 40, "diag", 10.0, 1.0), (36, 44, "par", 30.0, 1.5), (48, 48, "diag", 3.0, 0.5)]
+    plist, setups = [], []
+    for k, (n1, n2, grid, M, sigma) in enumerate(specs):
+        kw = dict(N1=n1, N2=n2, nu=1.0, eps=1e-13, mu_inj=1.0 / M)
+        if grid == "par":
+            kw.update(stype=0, x_ub=R2, y_ub=R2)
+        p = params(**kw)
+        K = fields.lognormal_permeability(n1, n2, sigma=sigma, corr_len=0.05, seed=300 + k,
+                                          x_extent=kw.get("x_ub", 1.0), y_extent=kw.get("y_ub", 1.0))
+        plist.append(gpu_params(p))
+        setups.append((p, K))
+    ens = capi.Ensemble(plist)
+    refs = []
+    for k, (p, K) in enumerate(setups):
+        o = po.Oracle(p, K, None)
+        q = o.source("one_cell")
+        c0 = smooth_c(o, seed=4)
+        m = ens.members[k]
+        m.set_permeability(K, None)
+        m.set_source(q)
+        m.set_concentration(c0)
+        refs.append((o, q, c0.copy(), o.scalar(), o.velocity()))
+    t_ref = [0.0] * len(specs)
+    for s in range(3):
+        dt, its = ens.step()
+        for k, (o, q, c, pr, v) in enumerate(refs):
+            st, dtr = o.step(c, pr, v, q, t_ref[k])
+            assert st == 0
+            t_ref[k] += dtr
+            assert abs(dt[k] - dtr) <= 1e-10 * dtr
+    for k, (o, q, c, pr, v) in enumerate(refs):
+        m = ens.members[k]
+        ec = rel_err(o.interior(m.get_concentration()), o.interior(c))
+        ep = rel_err(o.interior(m.get_pressure()), o.interior(pr))
+        ev = rel_err(m.get_velocity(), v)
+        _say("  member %d %s: c %.2e p %.2e v %.2e" % (k, specs[k][2], ec, ep, ev))
+        assert ec < 1e-10 and ep < 1e-10 and ev < 1e-10
+    ens.close()
+
+
+def test_full_size_ensemble_invariants(gpu_lib):
+    """The benchmark's ensemble at full size: 256 members, sizes uniform from
+    {128, 256, 512, 1024}^2, M in [2, 50], sigma in [0.5, 2], both grids,
+    mixed precision with the previous-pressure guess, 3 steps.  Permeabilities
+    are made transpose-symmetric (geometric mean of K and K^T), so every
+    member's concentration must stay symmetric about the diagonal; every
+    member keeps mass balance, div v = q and zero wall flux, and its pressure
+    solves reach 1e-10."""
+    from impec2d_amd import capi
+    import bench
+    rng = np.random.default_rng(77)
+    members = 256
+    sizes = rng.choice([128, 256, 512, 1024], members)
+    sigmas = rng.uniform(0.5, 2.0, members)
+    Ms = rng.uniform(2.0, 50.0, members)
+    grids = rng.integers(0, 2, members)
+    plist = []
+    for k in range(members):
+        n, ext = int(sizes[k]), (1.0 if grids[k] == 0 else R2)
+        plist.append(capi.default_params(
+            N1=n, N2=n, source_type=capi.SRC_DIAGONAL if grids[k] == 0 else capi.SRC_PARALLEL, nu=1.0,
+            dim_splitting=capi.SPLIT_PARALLEL, diffusion=0, x_ub=ext, y_ub=ext, mu_res=1.0, mu_inj=1.0 / Ms[k],
+            cfl=0.5, eps=1e-10, precision=capi.PREC_MIXED, initial_guess=capi.GUESS_PREVIOUS))
+    _say("full-size ensemble: %d members, %d cells" % (members, int(np.sum(sizes.astype(np.int64) ** 2))))
+    ens = capi.Ensemble(plist)
+    for k in range(members):
+        n, ext = int(sizes[k]), (1.0 if grids[k] == 0 else R2)
+        K = fields.lognormal_permeability(n, n, sigma=sigmas[k], corr_len=0.05, seed=5000 + k, x_extent=ext,
+                                          y_extent=ext)
+        K = np.sqrt(K * K.T)
+        m = ens.members[k]
+        m.set_permeability(np.ascontiguousarray(K), None)
+        m.set_source(bench.one_cell_source(n) if grids[k] == 0 else bench.corner_wells_source(n))
+    t = np.zeros(members)
+    for s in range(3):
+        dt, its = ens.step()
+        t += dt
+        _say("  step %d: CG iterations %d..%d" % (s, its.min(), its.max()))
+    worst = dict(mass=0.0, div=0.0, sym=0.0)
+    for k in range(members):
+        m = ens.members[k]
+        st = m.stats()
+        assert st.last_relres <= 1e-10, k
+        me = m.mass_error(t[k])
+        worst["mass"] = max(worst["mass"], me)
+        assert me < 1e-9, (k, me)
+        assert m.velocity_boundary_error() == 0.0, k
+        de = m.divergence_error()
+        worst["div"] = max(worst["div"], de)
+        assert de < 1e-6, (k, de)
+        c = m.get_concentration()[2:-2, 2:-2]
+        sym = np.abs(c - c.T).max() / max(np.abs(c).max(), 1e-300)
+        worst["sym"] = max(worst["sym"], sym)
+        assert sym < 1e-8, (k, sym)
+    _say("  worst mass error %.2e, divergence error %.2e, asymmetry %.2e" % (worst["mass"], worst["div"],
+                                                                             worst["sym"]))
+    ens.close()

@@ -37,21 +37,29 @@ def test_gpu_matches_golden(gpu_lib, case):
             assert abs(t - float(z["t%d" % k])) <= 1e-10 * float(z["t%d" % k])
 
 
-def _bench_ctx(n):
+def _bench_ctx(n, grid="diagonal"):
     import bench
     from impec2d_amd import capi
-    ctx = capi.Context(bench.workload_params(capi, n))
-    ctx.set_source(bench.one_cell_source(n))
+    if grid == "diagonal":
+        ctx = capi.Context(bench.workload_params(capi, n))
+        ctx.set_source(bench.one_cell_source(n))
+    else:   # the 45-degree rotated grid: the rotated square, four corner wells
+        ctx = capi.Context(bench.workload_params(capi, n, grid="rotated"))
+        ctx.set_source(bench.corner_wells_source(n))
     return ctx
 
 
+@pytest.mark.parametrize("grid", ["diagonal", "rotated"])
 @pytest.mark.parametrize("n,steps", [(1024, 6), (4096, 2)])
-def test_large_grid_invariants(gpu_lib, n, steps):
-    """Benchmark workload: every pressure solve reaches rel-res 1e-10, mass is
-    conserved, div v = q away from the pinned cell, walls carry no flow, and
-    the quarter-five-spot stays symmetric about the diagonal."""
-    print("invariants", n, flush=True)
-    ctx = _bench_ctx(n)
+def test_large_grid_invariants(gpu_lib, n, steps, grid):
+    """Benchmark workload on both grids (the quarter-five-spot, and the
+    rotated square with two injectors and two producers at its corners):
+    every pressure solve reaches rel-res 1e-10 and every dispersion solve
+    para_eps on its true residual, mass is conserved, div v = q away from the
+    pinned cell, walls carry no flow, and the concentration stays symmetric
+    about the diagonal."""
+    print("invariants", n, grid, flush=True)
+    ctx = _bench_ctx(n, grid)
     t = 0.0
     for _ in range(steps):
         t += ctx.step(t)
