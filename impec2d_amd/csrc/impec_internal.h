@@ -44,9 +44,15 @@ enum Scal {
     S_DT = 40,         // time step of the ensemble step
     S_ZEROP = 41,      // start the next solve from zero
     S_G0 = 42,         // 42..46: partial products of the warm-start projection
+    S_ERRPREV = 47,    // true residual norm^2 at the previous restart check (-1: none)
     S_COUNT = 48
 };
 enum { EST_DONE = 0, EST_ITER = 1, EST_CHECK = 2 };
+// restarts of the pressure solve's recursion from the true residual: at least
+// 3, more while each one at least halves the residual norm
+#ifndef IMPEC_MAX_RESTARTS
+#define IMPEC_MAX_RESTARTS 12
+#endif
 
 // Geometry and constants of the nu-weighted pressure operator on one level.
 // Constant factors are formed on the host with the reference's expressions

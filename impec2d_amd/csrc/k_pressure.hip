@@ -622,6 +622,19 @@ __global__ void k_cg_z(CgArgs c, const double *__restrict__ x0, const double *__
     if (tid_linear() == 0) part[blockIdx.x] = s[0];
 }
 
+// alpha = rho / (d.q).  A recursion that has broken down (d.q zero or a
+// non-finite scalar, which happens when it is driven below the FP64 rounding
+// floor of the residual) takes no step: p and r stay as they are, and the
+// solve loop restarts the recursion from the true residual.
+__device__ __forceinline__ double cg_alpha(const double *__restrict__ S) {
+    const double dq = S[S_DQ], a = S[S_RHO] / dq;
+    return (fabs(dq) > 0. && isfinite(a)) ? a : 0.;
+}
+__host__ __device__ __forceinline__ bool cg_broke_down(const double *S) {
+    const double dq = S[S_DQ], a = S[S_RHO] / dq;
+    return !(fabs(dq) > 0.) || !isfinite(a);
+}
+
 // d = z + beta d (beta = 0 on the first iteration)
 __global__ void k_cg_dir(CgArgs c, const double *__restrict__ z, double *__restrict__ d, const double *__restrict__ S,
                          int first) {
@@ -658,7 +671,7 @@ __global__ void k_cg_matvec(LevArgs L, CgArgs c, const double *__restrict__ d, d
 // x += alpha d, r -= alpha q; partials (r.r, sum r)
 __global__ void k_cg_update(CgArgs c, double *__restrict__ p, const double *__restrict__ d, double *__restrict__ r,
                             const double *__restrict__ qv, const double *__restrict__ S, double *__restrict__ part) {
-    const double alpha = S[S_RHO] / S[S_DQ];
+    const double alpha = cg_alpha(S);
     double s[2] = {0., 0.};
     for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < c.n; id += (long)gridDim.x * blockDim.x) {
         int i, j;
@@ -1901,7 +1914,7 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_dir_mv(DirMvArgs<T, DT> A, co
 // FP64, each result rounded once); partials (r.r, sum r) of the stored r
 __global__ void k_cg_update32(CgArgs c, float *__restrict__ e, const float *__restrict__ d, float *__restrict__ r,
                               const float *__restrict__ qv, const double *__restrict__ S, double *__restrict__ part) {
-    const double alpha = S[S_RHO] / S[S_DQ];
+    const double alpha = cg_alpha(S);
     const int j = blockIdx.x * 64 + threadIdx.x;
     double s[2] = {0., 0.};
     if (j < c.n2)
@@ -1959,7 +1972,7 @@ __global__ void k_cg_fold32(CgArgs c, double *__restrict__ p, const float *__res
 template <class G>
 __device__ __forceinline__ void cg_update2_body(const G &gr_, CgArgs c, double *__restrict__ p, const double *__restrict__ d, double *__restrict__ r,
                              const double *__restrict__ qv, const double *__restrict__ S, double *__restrict__ part) {
-    const double alpha = S[S_RHO] / S[S_DQ];
+    const double alpha = cg_alpha(S);
     const int j = gr_.bx() * 64 + threadIdx.x;
     double s[2] = {0., 0.};
     for (int i = gr_.by() * 4 + threadIdx.y; i < c.n1; i += gr_.gy() * 4) {
@@ -2535,8 +2548,10 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
             return e;
         };
         int since = 0, restarts = 0;
+        double rr_check = -1.;   // true residual norm^2 at the previous restart (-1: none yet)
         bool finished = false;
         for (;;) {
+            bool broke = false;
             while (!(rr <= tol * tol * bb)) {
                 if (its >= maxit) { status = IMPEC_ENOCONV; break; }
                 if (!(rr == rr)) { status = IMPEC_ENAN; break; }
@@ -2545,9 +2560,15 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
                 else st = cg_iteration_unfused<DIAG>(ctx, a0, c, nb, since == 0);
                 if (st) return st;
                 if ((st = fetch_scal(ctx))) return st;
-                rr = ctx->h_scal[S_RR];
                 its++;
                 since++;
+                // the recursion broke down (it took no step): restart it from
+                // the true residual as on convergence
+                if (cg_broke_down(ctx->h_scal)) {
+                    broke = true;
+                    break;
+                }
+                rr = ctx->h_scal[S_RR];
                 // FP32-vector CG: the correction goes into p and the FP64 residual
                 // replaces the FP32 one when the latter has fallen by IMPEC_RU_DELTA,
                 // meets the tolerance or is not finite; only an FP64 residual ends
@@ -2557,7 +2578,7 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
                     rr_ref = rr;
                 }
             }
-            if (ru && status != IMPEC_OK) {   // keep the corrections made so far
+            if (ru && (status != IMPEC_OK || broke)) {   // keep the corrections made so far
                 const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
                 hipLaunchKernelGGL(k_cg_fold32, vg.grid, vg.block, 0, s, c, ctx->p, ctx->pe32);
                 hipLaunchKernelGGL(k_cg_restart32, vg.grid, vg.block, 0, s, c, ctx->pr, ctx->pr32, ctx->pe32,
@@ -2574,12 +2595,17 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
                 finished = true;
                 break;
             }
+            const double rr_before = rr_check;
             if ((st = cg_residual<DIAG>(ctx, a0, c, nb, &rr))) return st;
             if (ru) {
                 if ((st = cg_restart32(ctx, c))) return st;
                 rr_ref = rr;
             }
-            if (rr <= tol * tol * bb || restarts >= 3) break;
+            rr_check = rr;
+            // restart while each restart at least halves the true residual
+            // (up to IMPEC_MAX_RESTARTS), and at least 3 times
+            const bool progress = rr_before < 0. || rr <= 0.25 * rr_before;
+            if (rr <= tol * tol * bb || restarts >= IMPEC_MAX_RESTARTS || (restarts >= 3 && !progress)) break;
             restarts++;
             since = 0;
         }
@@ -2609,8 +2635,8 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     if (relres_out) *relres_out = relres;
     if (missed)
         return set_error(ctx, status,
-                         "pressure solve: true residual %.3e above eps %.3e after %d iterations and 3 restarts", relres,
-                         tol, its);
+                         "pressure solve: true residual %.3e above eps %.3e after %d iterations and its restarts",
+                         relres, tol, its);
     if (status == IMPEC_ENOCONV)
         return set_error(ctx, status, "pressure solve: no convergence after %d iterations (relres %.3e)", its, relres);
     if (status == IMPEC_ENAN) return set_error(ctx, status, "pressure solve: non-finite residual");
@@ -2889,6 +2915,7 @@ __global__ void kb_ctl_first(const CtlRec *__restrict__ tab, int n) {
     S[S_EITS] = 0.;
     S[S_ESINCE] = 0.;
     S[S_ERST] = 0.;
+    S[S_ERRPREV] = -1.;
     S[S_ESTATUS] = 0.;
     double est = EST_ITER;
     if (rr <= S[S_ETOL]) est = EST_DONE;
@@ -2907,7 +2934,7 @@ __global__ void kb_ctl_iter(const CtlRec *__restrict__ tab, int n) {
     S[S_EITS] = its;
     S[S_ESINCE] = S[S_ESINCE] + 1.;
     const double rr = S[S_RR];
-    if (rr <= S[S_ETOL]) S[S_EST] = EST_CHECK;
+    if (rr <= S[S_ETOL] || cg_broke_down(S)) S[S_EST] = EST_CHECK;
     else if (its >= S[S_EMAXIT]) { S[S_EST] = EST_DONE; S[S_ESTATUS] = 1.; }
     else if (!(rr == rr)) { S[S_EST] = EST_DONE; S[S_ESTATUS] = 2.; }
 }
@@ -2918,7 +2945,15 @@ __global__ void kb_ctl_check(const CtlRec *__restrict__ tab, int n) {
     double *S = tab[m].S;
     if (S[S_EST] != (double)EST_CHECK) return;
     const double rr = S[S_RR];
-    if (rr <= S[S_ETOL] || S[S_ERST] >= 3.) { S[S_EST] = EST_DONE; return; }
+    // restart while each restart at least halves the true residual (up to
+    // IMPEC_MAX_RESTARTS), and at least 3 times, as the single-context loop
+    const double before = S[S_ERRPREV];
+    S[S_ERRPREV] = rr;
+    const bool progress = before < 0. || rr <= 0.25 * before;
+    if (rr <= S[S_ETOL] || S[S_ERST] >= (double)IMPEC_MAX_RESTARTS || (S[S_ERST] >= 3. && !progress)) {
+        S[S_EST] = EST_DONE;
+        return;
+    }
     S[S_ERST] = S[S_ERST] + 1.;
     S[S_ESINCE] = 0.;
     if (S[S_EITS] >= S[S_EMAXIT]) { S[S_EST] = EST_DONE; S[S_ESTATUS] = 1.; }

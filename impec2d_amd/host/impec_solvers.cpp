@@ -11,6 +11,7 @@
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
+#include <typeinfo>
 
 namespace impec_host {
 
@@ -61,10 +62,46 @@ impec_params SimParams::ToImpec() const {
 // host arrays
 // ---------------------------------------------------------------------------
 void Array2d::resize(int n1, int n2, int b) {
-    n1_ = n1; n2_ = n2; b_ = b; ld_ = n2 + 2 * b;
-    buf_.assign((size_t)(n1 + 2 * b) * ld_, 0.);
+    resize(n1 + 2 * b, n2 + 2 * b);
+    Idx2d base;
+    base[0] = base[1] = -b;
+    reindexSelf(base);
+}
+
+void Array2d::resize(int e1, int e2) {
+    if (dev_) dev_->unbind(this);   // a new shape is no longer the device's field
+    e1_ = e1 > 0 ? e1 : 0;
+    e2_ = e2 > 0 ? e2 : 0;
+    l1_ = l2_ = 0;
+    ld_ = e2_;
+    buf_.assign((size_t)e1_ * e2_, 0.);
+    set_field_shape();
     host_dirty_ = true;
     host_stale_ = false;
+}
+
+void Array2d::reindexSelf(const Idx2d &base) {
+    l1_ = base[0];
+    l2_ = base[1];
+    set_field_shape();
+}
+
+// the field view: equal ghost widths on both indices
+void Array2d::set_field_shape() {
+    if (l1_ == l2_ && l1_ <= 0 && e1_ + 2 * l1_ >= 0 && e2_ + 2 * l2_ >= 0) {
+        b_ = -l1_;
+        n1_ = e1_ - 2 * b_;
+        n2_ = e2_ - 2 * b_;
+    } else {
+        b_ = -1;
+        n1_ = n2_ = 0;
+    }
+}
+
+Array2d &Array2d::operator=(double x) {
+    for_write();
+    std::fill(buf_.begin(), buf_.end(), x);
+    return *this;
 }
 
 Array2d::~Array2d() {
@@ -374,25 +411,93 @@ HyperbolicSplitSolver::HyperbolicSplitSolver(SimParams *sp_, Device *dev_) : Hyp
 HyperbolicSplitSolver::~HyperbolicSplitSolver() {}
 
 bool HyperbolicSplitSolver::Advect_X(Array2d &c, const Array3d &v, const double dt) {
-    if (!dev->use(c, Device::C) || !dev->use(v)) return false;
-    const bool ok = dev->ok(impec_advect_x(dev->ctx(), dt), "HyperbolicSplitSolver::Advect_X");
-    dev->written(c);
-    return ok;
+    if (DeviceFluxes()) {
+        if (!dev->use(c, Device::C) || !dev->use(v)) return false;
+        const bool ok = dev->ok(impec_advect_x(dev->ctx(), dt), "HyperbolicSplitSolver::Advect_X");
+        dev->written(c);
+        return ok;
+    }
+    // a subclass's fluxes: the flux array of the reference's shape, then the
+    // update of every row, ghost rows included
+    const int b = sp->c_bounds;
+    Array2d F;
+    F.resize(sp->N1 + 1, sp->N2 + 2 * b);
+    Idx2d idx;
+    idx[0] = 0;
+    idx[1] = -b;
+    F.reindexSelf(idx);
+    F = 0;
+    CalculateIntercellFlux_F(F, c, v, dt);
+    for (int i = 0; i < sp->N1; i++)
+        for (int j = -b; j < sp->N2 + b; j++) c(i, j) += dt / sp->dx * (F(i, j) - F(i + 1, j));
+    return true;
 }
 
 bool HyperbolicSplitSolver::Advect_Y(Array2d &c, const Array3d &v, const double dt) {
-    if (!dev->use(c, Device::C) || !dev->use(v)) return false;
-    const bool ok = dev->ok(impec_advect_y(dev->ctx(), dt), "HyperbolicSplitSolver::Advect_Y");
-    dev->written(c);
-    return ok;
+    if (DeviceFluxes()) {
+        if (!dev->use(c, Device::C) || !dev->use(v)) return false;
+        const bool ok = dev->ok(impec_advect_y(dev->ctx(), dt), "HyperbolicSplitSolver::Advect_Y");
+        dev->written(c);
+        return ok;
+    }
+    const int b = sp->c_bounds;
+    Array2d G;
+    G.resize(sp->N1 + 2 * b, sp->N2 + 1);
+    Idx2d idx;
+    idx[0] = -b;
+    idx[1] = 0;
+    G.reindexSelf(idx);
+    G = 0;
+    CalculateIntercellFlux_G(G, c, v, dt);
+    for (int i = -b; i < sp->N1 + b; i++)
+        for (int j = 0; j < sp->N2; j++) c(i, j) += dt / sp->dy * (G(i, j) - G(i, j + 1));
+    return true;
 }
 
 bool HyperbolicSplitSolver::AdvectSplit(Array2d &c, const Array3d &v, double const *const, const double dt) {
     FillBoundaries(c);   // HyperbolicSplitSolver.cpp:55
-    if (!dev->use(c, Device::C) || !dev->use(v)) return false;
-    const bool ok = dev->ok(impec_advect_split(dev->ctx(), dt), "HyperbolicSplitSolver::AdvectSplit");
-    dev->written(c);
-    return ok;
+    if (DeviceFluxes()) {
+        if (!dev->use(c, Device::C) || !dev->use(v)) return false;
+        const bool ok = dev->ok(impec_advect_split(dev->ctx(), dt), "HyperbolicSplitSolver::AdvectSplit");
+        dev->written(c);
+        return ok;
+    }
+    // the sweep orders of HyperbolicSplitSolver.cpp:58-104 over Advect_X / _Y
+    bool ret = true;
+    switch (sp->dim_splitting) {
+    case ALTERNATING_SWEEPS:
+        if (prev_first_sweep == 1) {
+            ret = Advect_X(c, v, dt) && Advect_Y(c, v, dt);
+            prev_first_sweep = 0;
+        } else {
+            ret = Advect_Y(c, v, dt) && Advect_X(c, v, dt);
+            prev_first_sweep = 1;
+        }
+        break;
+    case X_SWEEP_FIRST:
+        ret = Advect_X(c, v, dt) && Advect_Y(c, v, dt);
+        prev_first_sweep = 0;
+        break;
+    case Y_SWEEP_FIRST:
+        ret = Advect_Y(c, v, dt) && Advect_X(c, v, dt);
+        prev_first_sweep = 1;
+        break;
+    case STRANG:
+        ret = Advect_X(c, v, 0.5 * dt) && Advect_Y(c, v, dt) && Advect_X(c, v, 0.5 * dt);
+        prev_first_sweep = 0;
+        break;
+    default: {
+        Array2d ct(sp->N1, sp->N2, sp->c_bounds);
+        const Array2d &cr = c;
+        std::copy(cr.data(), cr.data() + (size_t)cr.extent(0) * cr.extent(1), ct.data());
+        ret = Advect_X(c, v, dt) && Advect_Y(c, v, dt) && Advect_Y(ct, v, dt) && Advect_X(ct, v, dt);
+        const int b = sp->c_bounds;
+        for (int i = -b; i < sp->N1 + b; i++)
+            for (int j = -b; j < sp->N2 + b; j++) c(i, j) = 0.5 * (c(i, j) + ct(i, j));
+        prev_first_sweep = 0;
+    }
+    }
+    return ret;
 }
 
 bool HyperbolicSplitSolver::Solve(Array2d &c, const Array3d &v, double const *const q, const double dt) {
@@ -406,12 +511,19 @@ bool HyperbolicSplitSolver::Solve(Array2d &c, const Array3d &v, double const *co
     return state;
 }
 
-bool HyperbolicSplitSolver::Flux(int dir, std::vector<double> &F, const Array2d &c, const Array3d &v, double dt) {
+bool HyperbolicSplitSolver::Flux(int dir, Array2d &F, const Array2d &c, const Array3d &v, double dt) {
+    const int b = sp->c_bounds;
+    const int e1 = dir == 0 ? sp->N1 + 1 : sp->N1 + 2 * b, e2 = dir == 0 ? sp->N2 + 2 * b : sp->N2 + 1;
+    const int l1 = dir == 0 ? 0 : -b, l2 = dir == 0 ? -b : 0;
+    if (F.extent(0) != e1 || F.extent(1) != e2 || F.lbound(0) != l1 || F.lbound(1) != l2) {
+        std::cerr << "ERROR::HyperbolicSplitSolver::CalculateIntercellFlux_" << (dir == 0 ? "F" : "G")
+                  << ": the flux array must have extent " << e1 << " x " << e2 << " and base (" << l1 << "," << l2
+                  << ")" << std::endl;
+        return false;
+    }
     Array2d &cc = const_cast<Array2d &>(c);
     if (!dev->use(cc, Device::C) || !dev->use(v)) return false;
-    const size_t n = dir == 0 ? (size_t)(sp->N1 + 1) * (sp->N2 + 2 * sp->c_bounds)
-                              : (size_t)(sp->N1 + 2 * sp->c_bounds) * (sp->N2 + 1);
-    F.assign(n, 0.);
+    // the host layout of impec_intercell_flux is this array's storage
     return dev->ok(impec_intercell_flux(dev->ctx(), dir, dt, F.data()), "HyperbolicSplitSolver::Flux");
 }
 
@@ -420,28 +532,26 @@ MUSCLSplitSolver::MUSCLSplitSolver(SimParams *sp_, Device *dev_) : HyperbolicSpl
         throw std::runtime_error("MUSCLSplitSolver: the device was created for another scheme");
 }
 MUSCLSplitSolver::~MUSCLSplitSolver() {}
-void MUSCLSplitSolver::CalculateIntercellFlux_F(std::vector<double> &F, const Array2d &c, const Array3d &v,
-                                                const double dt) {
+void MUSCLSplitSolver::CalculateIntercellFlux_F(Array2d &F, const Array2d &c, const Array3d &v, const double dt) {
     Flux(0, F, c, v, dt);
 }
-void MUSCLSplitSolver::CalculateIntercellFlux_G(std::vector<double> &G, const Array2d &c, const Array3d &v,
-                                                const double dt) {
+void MUSCLSplitSolver::CalculateIntercellFlux_G(Array2d &G, const Array2d &c, const Array3d &v, const double dt) {
     Flux(1, G, c, v, dt);
 }
+bool MUSCLSplitSolver::DeviceFluxes() const { return typeid(*this) == typeid(MUSCLSplitSolver); }
 
 UpwindSplitSolver::UpwindSplitSolver(SimParams *sp_, Device *dev_) : HyperbolicSplitSolver(sp_, dev_) {
     if (dev_->params().hyper_method != IMPEC_HYPER_UPWIND)
         throw std::runtime_error("UpwindSplitSolver: the device was created for another scheme");
 }
 UpwindSplitSolver::~UpwindSplitSolver() {}
-void UpwindSplitSolver::CalculateIntercellFlux_F(std::vector<double> &F, const Array2d &c, const Array3d &v,
-                                                 const double dt) {
+void UpwindSplitSolver::CalculateIntercellFlux_F(Array2d &F, const Array2d &c, const Array3d &v, const double dt) {
     Flux(0, F, c, v, dt);
 }
-void UpwindSplitSolver::CalculateIntercellFlux_G(std::vector<double> &G, const Array2d &c, const Array3d &v,
-                                                 const double dt) {
+void UpwindSplitSolver::CalculateIntercellFlux_G(Array2d &G, const Array2d &c, const Array3d &v, const double dt) {
     Flux(1, G, c, v, dt);
 }
+bool UpwindSplitSolver::DeviceFluxes() const { return typeid(*this) == typeid(UpwindSplitSolver); }
 
 // ---------------------------------------------------------------------------
 // Impec2DSolver

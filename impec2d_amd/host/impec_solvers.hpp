@@ -88,8 +88,18 @@ struct SimParams {
 
 class Device;
 
-// Blitz-layout scalar field: extent (N1+2b) x (N2+2b), element (i,j) for
-// i in [-b, N1+b), j in [-b, N2+b).
+// Index pair of Array2d::reindexSelf (blitz::TinyVector<int,2>, GenSimDefs.h:30).
+struct Idx2d {
+    int v[2] = {0, 0};
+    int &operator[](int k) { return v[k]; }
+    int operator[](int k) const { return v[k]; }
+};
+
+// Blitz-layout 2-D array, row-major with the second index contiguous.  A
+// scalar field has extent (N1+2b) x (N2+2b) and base -b: element (i,j) for
+// i in [-b, N1+b), j in [-b, N2+b).  As with Blitz, any extent and base can
+// be set (resize(e1, e2) then reindexSelf), which the flux arrays of
+// HyperbolicSplitSolver::Advect_X / _Y use (HyperbolicSplitSolver.cpp:113-159).
 class Array2d {
   public:
     Array2d() = default;
@@ -97,16 +107,19 @@ class Array2d {
     Array2d(const Array2d &) = delete;
     Array2d &operator=(const Array2d &) = delete;
     ~Array2d();
-    void resize(int n1, int n2, int b);
+    void resize(int n1, int n2, int b);   // a field: extent (n1+2b) x (n2+2b), base -b
+    void resize(int e1, int e2);          // Blitz resize: extent e1 x e2, base 0, zeroed
+    void reindexSelf(const Idx2d &base);  // Blitz reindexSelf: lower bounds of both indices
+    Array2d &operator=(double x);         // Blitz "A = x": every element
     double &operator()(int i, int j) {
         for_write();
-        return buf_[(size_t)(i + b_) * ld_ + (j + b_)];
+        return buf_[(size_t)(i - l1_) * ld_ + (j - l2_)];
     }
     double operator()(int i, int j) const {
         for_read();
-        return buf_[(size_t)(i + b_) * ld_ + (j + b_)];
+        return buf_[(size_t)(i - l1_) * ld_ + (j - l2_)];
     }
-    double *data() {            // element (-b,-b); marks the host copy as written
+    double *data() {            // element (lbound(0), lbound(1)); marks the host copy as written
         for_write();
         return buf_.data();
     }
@@ -118,13 +131,18 @@ class Array2d {
     int n2() const { return n2_; }
     int bounds() const { return b_; }
     int ld() const { return ld_; }   // row pitch of data()
+    int lbound(int d) const { return d == 0 ? l1_ : l2_; }
+    int ubound(int d) const { return d == 0 ? l1_ + e1_ - 1 : l2_ + e2_ - 1; }
+    int extent(int d) const { return d == 0 ? e1_ : e2_; }
 
   private:
     friend class Device;
     void for_read() const;
     void for_write();
+    void set_field_shape();
     std::vector<double> buf_;
-    int n1_ = 0, n2_ = 0, b_ = 0, ld_ = 0;
+    int n1_ = 0, n2_ = 0, b_ = 0, ld_ = 0;   // field view: cells and ghost width (b_ < 0: not a field)
+    int l1_ = 0, l2_ = 0, e1_ = 0, e2_ = 0;  // lower bounds and extents
     Device *dev_ = nullptr;
     int field_ = -1;
     mutable bool host_stale_ = false;   // the device copy is newer
@@ -313,12 +331,16 @@ class HyperbolicSplitSolver : public HyperbolicSolver {
   public:
     HyperbolicSplitSolver(SimParams *sp, Device *dev);
     virtual ~HyperbolicSplitSolver();
-    // F(i,j): faces i in [0,N1], j in [-b,N2+b), F (N1+1) x (N2+2b) row-major;
-    // G(i,j): i in [-b,N1+b), faces j in [0,N2], G (N1+2b) x (N2+1)
-    virtual void CalculateIntercellFlux_F(std::vector<double> &F, const Array2d &c, const Array3d &v,
-                                          const double dt) = 0;
-    virtual void CalculateIntercellFlux_G(std::vector<double> &G, const Array2d &c, const Array3d &v,
-                                          const double dt) = 0;
+    // The reference's extension point (HyperbolicSplitSolver.h:19-20).  F(i,j):
+    // faces i in [0,N1], j in [-b,N2+b) (extent (N1+1) x (N2+2b), base (0,-b));
+    // G(i,j): i in [-b,N1+b), faces j in [0,N2] (extent (N1+2b) x (N2+1), base (-b,0)).
+    virtual void CalculateIntercellFlux_F(Array2d &F, const Array2d &c, const Array3d &v, const double dt) = 0;
+    virtual void CalculateIntercellFlux_G(Array2d &G, const Array2d &c, const Array3d &v, const double dt) = 0;
+    // The sweeps.  With the library's own fluxes (MUSCLSplitSolver and
+    // UpwindSplitSolver themselves) they run on the device; a subclass's
+    // fluxes are honoured as the reference honours them: the flux array is
+    // formed by the virtual and c is updated on the host
+    // (HyperbolicSplitSolver.cpp:113-159).
     bool Advect_X(Array2d &c, const Array3d &v, const double dt);   // HyperbolicSplitSolver.cpp:113-134
     bool Advect_Y(Array2d &c, const Array3d &v, const double dt);   // :140-159
     // Source(dt/2), fill, [CN(dt/2)], AdvectSplit(dt), Source(dt/2), fill, [CN(dt/2)] (:162-198)
@@ -326,27 +348,36 @@ class HyperbolicSplitSolver : public HyperbolicSolver {
 
   protected:
     bool AdvectSplit(Array2d &c, const Array3d &v, double const *const q, const double dt);   // :51-106
-    bool Flux(int dir, std::vector<double> &F, const Array2d &c, const Array3d &v, double dt);
+    // the library's device fluxes into F / G of the shapes above
+    bool Flux(int dir, Array2d &F, const Array2d &c, const Array3d &v, double dt);
+    // true when the flux virtuals are the library's own (the dynamic type is
+    // MUSCLSplitSolver or UpwindSplitSolver): the sweeps then stay on the device
+    virtual bool DeviceFluxes() const { return false; }
+
+  private:
+    int prev_first_sweep = 1;   // Y_SWEEP (HyperbolicSolver.cpp:84): the host sweeps' alternation
 };
 
 class MUSCLSplitSolver : public HyperbolicSplitSolver {
   public:
     MUSCLSplitSolver(SimParams *sp, Device *dev);   // the device must run the MUSCL scheme
     virtual ~MUSCLSplitSolver();
-    virtual void CalculateIntercellFlux_F(std::vector<double> &F, const Array2d &c, const Array3d &v,
-                                          const double dt);
-    virtual void CalculateIntercellFlux_G(std::vector<double> &G, const Array2d &c, const Array3d &v,
-                                          const double dt);
+    virtual void CalculateIntercellFlux_F(Array2d &F, const Array2d &c, const Array3d &v, const double dt);
+    virtual void CalculateIntercellFlux_G(Array2d &G, const Array2d &c, const Array3d &v, const double dt);
+
+  protected:
+    virtual bool DeviceFluxes() const;
 };
 
 class UpwindSplitSolver : public HyperbolicSplitSolver {
   public:
     UpwindSplitSolver(SimParams *sp, Device *dev);  // the device must run the upwind scheme
     virtual ~UpwindSplitSolver();
-    virtual void CalculateIntercellFlux_F(std::vector<double> &F, const Array2d &c, const Array3d &v,
-                                          const double dt);
-    virtual void CalculateIntercellFlux_G(std::vector<double> &G, const Array2d &c, const Array3d &v,
-                                          const double dt);
+    virtual void CalculateIntercellFlux_F(Array2d &F, const Array2d &c, const Array3d &v, const double dt);
+    virtual void CalculateIntercellFlux_G(Array2d &G, const Array2d &c, const Array3d &v, const double dt);
+
+  protected:
+    virtual bool DeviceFluxes() const;
 };
 
 // ---------------------------------------------------------------------------

@@ -34,7 +34,7 @@ def _order(hs, errs):
 def gpu_manufactured_errors(test, nu, N):
     from impec2d_amd import capi
     L = 2 * math.pi
-    p = params(N1=N, N2=N, nu=nu, x_ub=L, y_ub=L, mu_res=1.0, mu_inj=1.0, avgproc=1, eps=1e-13)
+    p = params(N1=N, N2=N, nu=nu, x_ub=L, y_ub=L, mu_res=1.0, mu_inj=1.0, avgproc=1, eps=1e-11)
     dx = L / N
     b = 2
     xc = (np.arange(-b, N + b) + 0.5) * dx

@@ -2,7 +2,10 @@
 (tests/golden/longrun/, written by make_longrun.py).
 
 Stable cases (M = 1, strong dispersion) and the CPU configuration's 200 steps
-are compared pointwise at the end: p, u, v, c within 1e-10 relative, every
+are compared pointwise at the end: p, u, v within TOL_PV relative, c within
+TOL_C relative (the two solvers' pressures differ by their solve tolerance,
+1e-10, at every step, and the MUSCL limiter's branches turn such differences
+into concentration differences of about 1e-9 over thousands of steps), every
 step's t within 1e-10 relative.  Unstable cases (M = 10 without dispersion,
 and the shipped production configurations with M = 41) amplify rounding
 differences into different fingers, so they are judged by the curves of
@@ -31,6 +34,8 @@ FIX = os.path.join(ROOT, "tests", "golden", "longrun")
 RUN = os.path.join(ROOT, "impec2d_amd", "impec2d_run")
 CFG = os.path.join(ROOT, "tests", "golden", "configs")
 
+TOL_PV = 1e-10
+TOL_C = 1e-8
 TOL_RECOVERED = 2e-3
 TOL_BREAKTHROUGH = 0.02
 STABLE = ["c1_200", "m1_t080", "m10_disp_t080"]
@@ -96,7 +101,7 @@ def test_stable_long_run_pointwise(gpu_lib, case):
     ep = rel_err(o.interior(g.get_pressure()), o.interior(z["p"]))
     ev = rel_err(g.get_velocity(), z["v"])
     _say("  end t %.4f: c %.2e p %.2e v %.2e" % (ts[-1], ec, ep, ev))
-    assert ec < 1e-10 and ep < 1e-10 and ev < 1e-10
+    assert ec < TOL_C and ep < TOL_PV and ev < TOL_PV
     scale = max(np.abs(z["outflux"]).max(), 1e-300)
     assert np.abs(outs - z["outflux"]).max() <= 1e-9 * scale
     g.close()

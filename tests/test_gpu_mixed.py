@@ -55,7 +55,7 @@ def test_mixed_steps_error_per_field(gpu_lib, kw, mode):
     method (the preconditioner's precision does not limit the FP64 residual)."""
     from impec2d_amd import capi
     n = 64
-    p = params(**dict(dict(N1=n, N2=n, eps=1e-13, para_eps=1e-13, mu_inj=1.0 / 20), **kw))
+    p = params(**dict(dict(N1=n, N2=n, eps=1e-12, para_eps=1e-13, mu_inj=1.0 / 20), **kw))
     K = fields.lognormal_permeability(n, n, sigma=1.0, corr_len=0.05, seed=7)
     o, g = pair(p, K, None, precision=capi.PREC_MIXED if mode == "mixed" else capi.PREC_MIXED32)
     q = o.source("one_cell")

@@ -289,8 +289,11 @@ def test_steps_1_and_10(gpu_lib, kw):
             # and produced mass to 1e-10 relative, the balance and the
             # divergence / wall residuals (rounding-level numbers) absolutely
             assert abs(g.total_mass() - o.mass(c)) <= 1e-10 * o.mass(c)
+            # produced mass: relative to itself once it is more than a trace of
+            # the injected mass (dispersion carries traces of 1e-90 to the sink)
             tmo_g, tmo_o = g.stats().total_mass_outflux, o.total_mass_outflux
-            assert abs(tmo_g - tmo_o) <= 1e-10 * max(tmo_o, 1e-300) or tmo_o == tmo_g == 0.0
+            injected = t_ref * p["I"] / (o.dx * o.dy)
+            assert abs(tmo_g - tmo_o) <= 1e-10 * max(tmo_o, 1e-12 * injected)
             assert abs(g.mass_error(t_gpu) - o.mass_error(c, t_ref)) <= 1e-11
             dq = 2.0 * p["I"]   # sum |q| dx dy of the two wells
             assert abs(g.divergence_error() - o.div_error(v, q)) <= 1e-9 * dq
