@@ -91,17 +91,21 @@ def train_guess_network(capi, n=512, steps=120, crops=32, iters=3000):
 def modes(capi, n, warmup=3, steps=8):
     """Plain FP64 against mixed precision and the warm starts on the lognormal-K
     workload at n^2: ms/step and V-cycles/step of each."""
+    import numpy as np
     from impec2d_amd import fields
     K = fields.lognormal_permeability(n, n, sigma=1.0, corr_len=0.05, seed=1)
     net, train_info = train_guess_network(capi)
+    ref_fields = None
     res = {"workload": "lognormal K (sigma 1, corr. length 0.05), M=20, no dispersion, %d^2, rel-res 1e-10" % n,
            "learned_training": train_info}
     for name, prec, guess in [("fp64_zero", "fp64", "zero"), ("fp64_previous", "fp64", "previous"),
                               ("mixed_zero", "mixed", "zero"),
                               ("mixed_previous", "mixed", "previous"), ("mixed_extrapolated", "mixed", "extrapolate"),
                               ("mixed_learned", "mixed", "learned"), ("mixed32_zero", "mixed32", "zero"),
-                              ("mixed32_previous", "mixed32", "previous"), ("mixed32_learned", "mixed32", "learned")]:
+                              ("mixed32_previous", "mixed32", "previous"), ("mixed32_learned", "mixed32", "learned"),
+                              ("mixed_previous_transport32", "mixed", "previous")]:
         p = lognormal_params(capi, n, prec, "zero")
+        p.transport32 = 1 if name.endswith("transport32") else 0
         p.initial_guess = {"zero": capi.GUESS_ZERO, "previous": capi.GUESS_PREVIOUS,
                            "extrapolate": capi.GUESS_EXTRAPOLATE, "learned": capi.GUESS_LEARNED}[guess]
         ctx = capi.Context(p)
@@ -123,6 +127,18 @@ def modes(capi, n, warmup=3, steps=8):
         res[name] = {"ms_per_step": 1e3 * el / steps, "cell_updates_per_s": n * n * steps / el,
                      "vcycles_per_step": (s1.total_its - s0.total_its) / steps,
                      "last_relres": s1.last_relres}
+        # the error per field against the plain FP64 run after the same steps
+        fields_now = (ctx.get_pressure()[2:-2, 2:-2], ctx.get_velocity(), ctx.get_concentration()[2:-2, 2:-2])
+        if name == "fp64_zero":
+            ref_fields = fields_now
+        elif name in ("mixed_previous", "mixed32_previous", "mixed_previous_transport32"):
+            err = {}
+            for key, a, b in (("p", fields_now[0], ref_fields[0]), ("u", fields_now[1][..., 0], ref_fields[1][..., 0]),
+                              ("v", fields_now[1][..., 1], ref_fields[1][..., 1]),
+                              ("c", fields_now[2], ref_fields[2])):
+                err[key] = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+            res[name]["error_vs_fp64"] = err
+        del fields_now
         if guess in ("learned", "extrapolate"):
             res[name]["last_step_lengths"] = {"network_increment": s1.last_guess_alpha,
                                               "last_change": s1.last_guess_gamma}
@@ -133,6 +149,8 @@ def modes(capi, n, warmup=3, steps=8):
                                                  / res["mixed_extrapolated"]["ms_per_step"])
     res["speedup_mixed_previous_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_previous"]["ms_per_step"]
     res["speedup_mixed_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["mixed_zero"]["ms_per_step"]
+    res["speedup_mixed_previous_transport32_vs_fp64"] = (res["fp64_zero"]["ms_per_step"]
+                                                         / res["mixed_previous_transport32"]["ms_per_step"])
     # the warm start's gain alone (FP64) and the precision's gain alone (previous guess)
     res["speedup_fp64_previous_vs_fp64"] = res["fp64_zero"]["ms_per_step"] / res["fp64_previous"]["ms_per_step"]
     res["speedup_mixed_previous_vs_fp64_previous"] = (res["fp64_previous"]["ms_per_step"]
@@ -384,15 +402,25 @@ def main():
         out["roofline"] = roofline.measure(ctx, n, t, fp32_cycle=args.precision in ("mixed", "mixed32"))
         out["roofline_by_kernel"] = roofline.measure.last_table
     ctx.close()
+
+    # the secondary workloads: a failure is reported in the line, never loses it
+    def extra(key, fn):
+        print("bench.py: %s ..." % key, file=sys.stderr, flush=True)
+        try:
+            out[key] = fn()
+        except Exception as e:   # noqa: BLE001
+            out[key] = {"error": "%s: %s" % (type(e).__name__, e)}
+            print("bench.py: %s failed: %s" % (key, e), file=sys.stderr, flush=True)
+
     if not args.no_rotated:
-        out["rotated_grid"] = rotated(capi, n)
+        extra("rotated_grid", lambda: rotated(capi, n))
     if not args.no_modes:
-        out["modes"] = modes(capi, n)
+        extra("modes", lambda: modes(capi, n))
     if not args.no_ensemble:
-        out["ensemble"] = ensemble(capi, members=args.ensemble_members, warmup=args.ensemble_warmup,
-                                   steps=args.ensemble_steps, graphs=not args.ensemble_eager)
+        extra("ensemble", lambda: ensemble(capi, members=args.ensemble_members, warmup=args.ensemble_warmup,
+                                           steps=args.ensemble_steps, graphs=not args.ensemble_eager))
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        extra("cpu_baseline", cpu_baseline)
     print(json.dumps(out), flush=True)
     return 0
 

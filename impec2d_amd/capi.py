@@ -30,23 +30,26 @@ GUESS_ZERO, GUESS_PREVIOUS, GUESS_LEARNED, GUESS_EXTRAPOLATE = 0, 1, 2, 3
 
 _INT_FIELDS = ["N1", "N2", "num_ghostcells", "source_type", "average_procedure", "dim_splitting",
                "hyper_method", "diffusion", "para_method", "is_test", "alpha_absolute",
-               "max_iterations", "precision", "initial_guess", "mg_pre", "mg_post", "mg_variant", "mg_rows", "cn_guess"]
+               "max_iterations", "precision", "initial_guess", "mg_pre", "mg_post", "mg_variant", "mg_rows", "cn_guess",
+               "transport32"]
 _DBL_FIELDS = ["x_lb", "x_ub", "y_lb", "y_ub", "nu", "mu_res", "mu_inj", "porosity", "cfl",
                "inj_fluid_conc", "alpha_l", "alpha_t", "dt_diff_limiter", "d_m", "eps", "para_eps",
                "I", "tstart", "inner_tol"]
 
 
 class Params(C.Structure):
-    _fields_ = ([(n, C.c_int) for n in _INT_FIELDS] + [("reserved_i", C.c_int * 1)]
+    _fields_ = ([(n, C.c_int) for n in _INT_FIELDS]
                 + [(n, C.c_double) for n in _DBL_FIELDS] + [("reserved_d", C.c_double * 3)])
 
 
 class Stats(C.Structure):
     _fields_ = [("last_its", C.c_int), ("last_vcycles", C.c_int), ("last_parabolic_its", C.c_int),
-                ("total_its", C.c_int), ("calls", C.c_int), ("reserved_i", C.c_int * 3),
+                ("total_its", C.c_int), ("calls", C.c_int), ("last_restarts", C.c_int),
+                ("last_elim_its", C.c_int), ("reserved_i", C.c_int * 1),
                 ("last_relres", C.c_double), ("last_parabolic_relres", C.c_double),
                 ("total_mass_outflux", C.c_double), ("last_guess_alpha", C.c_double),
-                ("last_guess_gamma", C.c_double), ("reserved_d", C.c_double * 3)]
+                ("last_guess_gamma", C.c_double), ("last_relres_floor", C.c_double),
+                ("reserved_d", C.c_double * 2)]
 
 
 # every entry point declared in include/impec_hip.h: name -> (argtypes, restype)

@@ -158,6 +158,8 @@ int impec::ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out) {
     if (P->mg_rows < 0 || P->mg_rows > 4096) return IMPEC_EINVAL;
     if (P->mg_variant < 0 || P->mg_variant > 2) return IMPEC_EINVAL;
     if (P->cn_guess != 0 && P->cn_guess != 1) return IMPEC_EINVAL;
+    // FP32 transport: the one-pass parallel splitting only
+    if (P->transport32 != 0 && (P->transport32 != 1 || P->dim_splitting != IMPEC_SPLIT_PARALLEL)) return IMPEC_EINVAL;
     if (P->initial_guess < IMPEC_GUESS_ZERO || P->initial_guess > IMPEC_GUESS_EXTRAPOLATE) return IMPEC_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IMPEC_ENODEV;

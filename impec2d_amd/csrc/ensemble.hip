@@ -58,15 +58,17 @@ __global__ void kb_count_active(double *const *__restrict__ S, int n, int *__res
     if (threadIdx.x == 0) out[0] = cnt;
 }
 
-// per member: iterations, status, ||b - A p||^2, time step
+// per member: iterations, status, ||b - A p||^2, time step, || |b| + |A| |p| ||^2
+constexpr int GATHER = 5;
 __global__ void kb_gather(double *const *__restrict__ S, int n, double *__restrict__ out) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n) return;
     const double *s = S[m];
-    out[4 * m + 0] = s[S_EITS];
-    out[4 * m + 1] = s[S_ESTATUS];
-    out[4 * m + 2] = s[S_TT];
-    out[4 * m + 3] = s[S_DT];
+    out[GATHER * m + 0] = s[S_EITS];
+    out[GATHER * m + 1] = s[S_ESTATUS];
+    out[GATHER * m + 2] = s[S_TT];
+    out[GATHER * m + 3] = s[S_DT];
+    out[GATHER * m + 4] = s[S_FLOOR];
 }
 
 static void drop_graphs(impec_ensemble *e) {
@@ -114,7 +116,34 @@ static int stage_post(impec_ensemble *e) {
     return IMPEC_OK;
 }
 
-static int capture(impec_ensemble *e, int (*stage)(impec_ensemble *), hipGraphExec_t *out) {
+// Every node of a captured stage must be a kernel launch with a non-empty
+// grid, a workgroup of 1 to 1024 threads and at most 64 KiB of dynamic LDS:
+// the stages record no copies, fills or host nodes.  Checked once per
+// capture, before instantiation.
+static int check_graph(impec_ensemble *e, hipGraph_t g, const char *name) {
+    size_t n = 0;
+    ENS_CHECK(e, hipGraphGetNodes(g, nullptr, &n));
+    std::vector<hipGraphNode_t> nodes(n);
+    if (n) ENS_CHECK(e, hipGraphGetNodes(g, nodes.data(), &n));
+    for (size_t k = 0; k < n; k++) {
+        hipGraphNodeType type;
+        ENS_CHECK(e, hipGraphNodeGetType(nodes[k], &type));
+        if (type != hipGraphNodeTypeKernel)
+            return ens_set_error(e, IMPEC_EHIP, "graph %s: node %zu has type %d, not a kernel launch", name, k,
+                                 (int)type);
+        hipKernelNodeParams kp;
+        ENS_CHECK(e, hipGraphKernelNodeGetParams(nodes[k], &kp));
+        const long long grid = (long long)kp.gridDim.x * kp.gridDim.y * kp.gridDim.z;
+        const long long block = (long long)kp.blockDim.x * kp.blockDim.y * kp.blockDim.z;
+        if (grid < 1 || block < 1 || block > 1024 || kp.sharedMemBytes > 65536 || !kp.func)
+            return ens_set_error(e, IMPEC_EHIP, "graph %s: kernel node %zu with grid %lld, block %lld, LDS %u",
+                                 name, k, grid, block, (unsigned)kp.sharedMemBytes);
+    }
+    e->graph_nodes += (long long)n;
+    return IMPEC_OK;
+}
+
+static int capture(impec_ensemble *e, int (*stage)(impec_ensemble *), hipGraphExec_t *out, const char *name) {
     hipGraph_t g = nullptr;
     ENS_CHECK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     const int st = stage(e);
@@ -124,6 +153,11 @@ static int capture(impec_ensemble *e, int (*stage)(impec_ensemble *), hipGraphEx
         return st;
     }
     ENS_CHECK(e, he);
+    const int sc = check_graph(e, g, name);
+    if (sc) {
+        (void)hipGraphDestroy(g);
+        return sc;
+    }
     const hipError_t hi = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     ENS_CHECK(e, hi);
@@ -179,7 +213,7 @@ int impec_ensemble_create(const impec_params *members, int count, impec_ensemble
         const impec_params &P = members[k];
         if (P.nu != 1. || P.diffusion || P.is_test || P.dim_splitting != IMPEC_SPLIT_PARALLEL) return IMPEC_EINVAL;
         if (P.initial_guess != IMPEC_GUESS_ZERO && P.initial_guess != IMPEC_GUESS_PREVIOUS) return IMPEC_EINVAL;
-        if (P.precision == IMPEC_PREC_MIXED32) return IMPEC_EINVAL;   // single contexts only
+        if (P.precision == IMPEC_PREC_MIXED32 || P.transport32) return IMPEC_EINVAL;   // single contexts only
         if (P.precision != P0.precision || P.hyper_method != P0.hyper_method ||
             (P.porosity == 1.) != (P0.porosity == 1.))
             return IMPEC_EINVAL;
@@ -208,9 +242,9 @@ int impec_ensemble_create(const impec_params *members, int count, impec_ensemble
         for (int k = 0; k < count; k++) S[k] = e->m[k]->scal;
         if (hipMalloc(&e->d_S, sizeof(double *) * count) != hipSuccess ||
             hipMemcpy(e->d_S, S.data(), sizeof(double *) * count, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMalloc(&e->d_out, sizeof(double) * 4 * count) != hipSuccess ||
+            hipMalloc(&e->d_out, sizeof(double) * GATHER * count) != hipSuccess ||
             hipMalloc(&e->d_active, sizeof(int)) != hipSuccess ||
-            hipHostMalloc((void **)&e->h_out, sizeof(double) * 4 * count, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&e->h_out, sizeof(double) * GATHER * count, hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc((void **)&e->h_active, sizeof(int), hipHostMallocDefault) != hipSuccess)
             st = IMPEC_ENOMEM;
     }
@@ -301,8 +335,10 @@ int impec_ensemble_step(impec_ensemble *e, double *dt, int *its) {
     int st = prepare(e);
     if (st) return st;
     if (e->use_graphs && !e->have_graphs) {
-        if ((st = capture(e, stage_pre, &e->g_pre)) || (st = capture(e, stage_loop, &e->g_loop)) ||
-            (st = capture(e, stage_end, &e->g_end)) || (st = capture(e, stage_post, &e->g_post))) {
+        e->graph_nodes = 0;
+        if ((st = capture(e, stage_pre, &e->g_pre, "set-up")) || (st = capture(e, stage_loop, &e->g_loop, "iterations")) ||
+            (st = capture(e, stage_end, &e->g_end, "end of solve")) ||
+            (st = capture(e, stage_post, &e->g_post, "transport"))) {
             drop_graphs(e);
             return st;
         }
@@ -318,17 +354,19 @@ int impec_ensemble_step(impec_ensemble *e, double *dt, int *its) {
         if (++loops > e->max_loop) return ens_set_error(e, IMPEC_ENOCONV, "ensemble pressure solve did not finish");
     }
     if ((st = run(e, stage_end, e->g_end))) return st;
-    ENS_CHECK(e, hipMemcpyAsync(e->h_out, e->d_out, sizeof(double) * 4 * e->n, hipMemcpyDeviceToHost, e->stream));
+    ENS_CHECK(e, hipMemcpyAsync(e->h_out, e->d_out, sizeof(double) * GATHER * e->n, hipMemcpyDeviceToHost,
+                                e->stream));
     ENS_CHECK(e, hipStreamSynchronize(e->stream));
     for (int k = 0; k < e->n; k++) {
         impec_ctx *c = e->m[k];
-        int mits = (int)e->h_out[4 * k + 0];
-        const int status = (int)e->h_out[4 * k + 1];
-        double relres = c->bnorm > 0 ? sqrt(e->h_out[4 * k + 2]) / c->bnorm : 0.;
+        int mits = (int)e->h_out[GATHER * k + 0];
+        const int status = (int)e->h_out[GATHER * k + 1];
+        double relres = c->bnorm > 0 ? sqrt(e->h_out[GATHER * k + 2]) / c->bnorm : 0.;
+        double floor_rel = c->bnorm > 0 ? IMPEC_FLOOR_C * 0x1p-53 * sqrt(e->h_out[GATHER * k + 4]) / c->bnorm : 0.;
         if (status == 1)
             return ens_set_error(e, IMPEC_ENOCONV, "member %d: no convergence after %d iterations", k, mits);
         if (status == 2) return ens_set_error(e, IMPEC_ENAN, "member %d: non-finite residual", k);
-        if (relres > c->P.eps) {
+        if (relres > c->P.eps && relres > floor_rel) {
             // the shifted iterate misses the pinned system's tolerance: the
             // member's own solve (consistent phase, then the eliminated one)
             int its1 = 0;
@@ -336,6 +374,7 @@ int impec_ensemble_step(impec_ensemble *e, double *dt, int *its) {
             if ((st = pressure_solve(c, &its1, &relres)))
                 return ens_set_error(e, st, "member %d: %s", k, c->err);
             mits = its1;
+            floor_rel = c->stats.last_relres_floor;
             c->stats.total_its -= its1;   // counted once below
             c->stats.calls -= 1;
         }
@@ -343,15 +382,17 @@ int impec_ensemble_step(impec_ensemble *e, double *dt, int *its) {
         c->stats.last_its = mits;
         c->stats.last_vcycles = mits;
         c->stats.last_relres = relres;
+        c->stats.last_relres_floor = floor_rel;
         c->stats.total_its += mits;
         c->stats.calls += 1;
         if (its) its[k] = mits;
     }
     if ((st = run(e, stage_post, e->g_post))) return st;
     if (dt) {
-        ENS_CHECK(e, hipMemcpyAsync(e->h_out, e->d_out, sizeof(double) * 4 * e->n, hipMemcpyDeviceToHost, e->stream));
+        ENS_CHECK(e, hipMemcpyAsync(e->h_out, e->d_out, sizeof(double) * GATHER * e->n, hipMemcpyDeviceToHost,
+                                    e->stream));
         ENS_CHECK(e, hipStreamSynchronize(e->stream));
-        for (int k = 0; k < e->n; k++) dt[k] = e->h_out[4 * k + 3];
+        for (int k = 0; k < e->n; k++) dt[k] = e->h_out[GATHER * k + 3];
     }
     return IMPEC_OK;
 }

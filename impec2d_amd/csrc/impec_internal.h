@@ -25,7 +25,9 @@ constexpr int VEC_BLOCKS = IMPEC_VEC_BLOCKS;
 // except the convergence test.
 enum Scal {
     S_RHO = 0, S_RHO_OLD = 1, S_DQ = 2, S_RR = 3, S_RSUM = 4, S_ZSUM = 5, S_ALPHA = 6, S_BETA = 7,
-    S_OMEGA = 8, S_TS = 9, S_TT = 10, S_R0V = 11, S_XPIN = 12, S_RX = 14, S_BB = 15,
+    S_OMEGA = 8, S_TS = 9, S_TT = 10,
+    S_FLOOR = 11,     // with S_TT: || |b| + |A| |p| ||^2, the rounding scale of the true residual
+    S_XPIN = 12, S_RX = 14, S_BB = 15, S_R0V = 19,
     S_MASS = 16, S_DIV = 17, S_NOFLOW = 18,          // written together by the diagnostics pass
     S_MAXVEL = 20, S_MINDT = 21, S_MINDT_DIFF = 22,  // CFL pass
     S_OUTFLUX = 24, S_TMO = 25, S_CMAX = 26, S_CMIN = 27,
@@ -45,13 +47,40 @@ enum Scal {
     S_ZEROP = 41,      // start the next solve from zero
     S_G0 = 42,         // 42..46: partial products of the warm-start projection
     S_ERRPREV = 47,    // true residual norm^2 at the previous restart check (-1: none)
-    S_COUNT = 48
+    S_EMARK = 48,      // recursive residual norm^2 when it last halved (in norm)
+    S_ESTALL = 49,     // iterations since then
+    S_COUNT = 50
 };
 enum { EST_DONE = 0, EST_ITER = 1, EST_CHECK = 2 };
+// The FP64 rounding floor of the pressure solve's true residual: evaluating
+// b - A p rounds every row at about u (|b| + |A| |p|), so a residual norm below
+// IMPEC_FLOOR_C u || |b| + |A| |p| || cannot be told from zero and counts as
+// converged whatever the tolerance (u = 2^-53).
+#ifndef IMPEC_FLOOR_C
+#define IMPEC_FLOOR_C 8.0
+#endif
 // restarts of the pressure solve's recursion from the true residual: at least
 // 3, more while each one at least halves the residual norm
 #ifndef IMPEC_MAX_RESTARTS
 #define IMPEC_MAX_RESTARTS 12
+#endif
+// A recursion whose residual norm has not halved in IMPEC_STALL_ITS iterations
+// has its true residual and rounding floor evaluated: converged at the floor,
+// it stops; stalled within IMPEC_STALL_GATE times the floor, it is restarted
+// from the true residual (a recursion continued at the floor drifts away from
+// the true residual); stalled above that (a slow stretch of a poorly
+// preconditioned system), it goes on.
+#ifndef IMPEC_STALL_ITS
+#define IMPEC_STALL_ITS 20
+#endif
+#ifndef IMPEC_STALL_GATE
+#define IMPEC_STALL_GATE 4.0
+#endif
+// the dispersion solve's BiCGStab (about 15 iterations per solve): a recursion
+// whose residual norm has not halved in this many iterations gets the true
+// residual check and restart of a converged one
+#ifndef IMPEC_CN_STALL_ITS
+#define IMPEC_CN_STALL_ITS 50
 #endif
 
 // Geometry and constants of the nu-weighted pressure operator on one level.
@@ -173,6 +202,7 @@ struct impec_ensemble {
     int use_graphs;            // replay captured graphs (else launch eagerly)
     int slots;                 // CG iterations per replay of the loop graph
     int have_graphs;
+    long long graph_nodes;     // kernel nodes of the captured stages (checked at capture)
     hipGraphExec_t g_pre, g_loop, g_end, g_post;
     impec::EnsFieldTabs *field;
     impec::EnsPressTabs *pres;

@@ -170,6 +170,8 @@ __global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const
 //                       p' = r' + beta (p - omega v), v' = A (p' / D);
 //                       partials r0.v', r'.r'
 //   k_cn_stream<CN_P0>: the first direction p = r, v = A (p / D).
+//   k_cn_stream<CN_R>:  the true residual of the iterate, partials of
+//                       ||b - A x||^2 (x read from c's interior, b from r).
 // rho_new = r0.r' is formed from the s-launch's partials as r0.s - omega r0.t,
 // so beta is known before the p-launch; the convergence test uses the
 // p-launch's r'.r', the recursive residual itself.  The p-launch forms s and t
@@ -189,7 +191,7 @@ __global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const
 // ---------------------------------------------------------------------------
 constexpr int S_RHO_NEW = 13;
 constexpr int CN_OUT = 62, CNP_OUT = 60, CN_WAVES = 4;
-enum { CN_S = 1, CN_P0 = 2 };
+enum { CN_S = 1, CN_P0 = 2, CN_R = 3 };
 // build-time tuning: rows in flight per wavefront, wavefronts per launch
 #ifndef IMPEC_CN_PF
 #define IMPEC_CN_PF 2
@@ -215,7 +217,9 @@ struct CnArgs {
     const double *u, *v;                   // face velocities, ghosted (ghost width k.B, pitch ldv)
     const float *dinv;                     // rounded inverse of the folded diagonal (flat N1 x N2)
     const double *r;                       // the residual
-    const double *pin, *vin, *r0;          // flat N1 x N2
+    const double *pin, *vin;               // flat N1 x N2
+    const float *r0;                       // the shadow residual, rounded to FP32 (any fixed r0 with
+                                           // r0.r != 0 serves BiCGStab; 4 B per read instead of 8)
     double *x, *rout;                      // p-step: the solution (in place) and the new residual
     long ldx;                              // row pitch of x (the interior of the ghosted c)
     double *pout;                          // p-step, CN_P0: new p
@@ -273,8 +277,9 @@ struct FaceStream {
     }
 };
 
-// CN_S (s = r - alpha v, t = A (s / D), partials only) and CN_P0 (p = r,
-// v = A (p / D)); lanes 1..62 of a strip are written
+// CN_S (s = r - alpha v, t = A (s / D), partials only), CN_P0 (p = r,
+// v = A (p / D)) and CN_R (partials of ||r - A x||^2 with r = b; nothing
+// written); lanes 1..62 of a strip are written
 template <int MODE>
 __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
     const int lane = threadIdx.x & 63;
@@ -311,11 +316,19 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
     constexpr int PF = IMPEC_CN_PF;   // rows in flight; a slot is refilled at the end of the step that used it
     double Pa[PF] = {}, Pv[PF] = {}, P0[PF] = {}, Pu[PF] = {}, Pw[PF] = {};
     float Pd[PF] = {};
+    auto cellx = [&](const double *p, int m) -> double {   // x: the interior of c, pitch ldx
+        const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
+        return p[(long)mc * A.ldx + jc];
+    };
     auto fetch = [&](int u, int m) {
         Pa[u] = cell(A.r, m);
         if (MODE == CN_S) Pv[u] = cell(A.vin, m);
-        Pd[u] = cellf(A.dinv, m);
-        P0[u] = cell(A.r0, m);
+        if (MODE == CN_R) {
+            Pv[u] = cellx(A.x, m);
+        } else {
+            Pd[u] = cellf(A.dinv, m);
+            P0[u] = (double)cellf(A.r0, m);
+        }
         Pu[u] = vel(A.u, m);
         Pw[u] = vel(A.v, m);
     };
@@ -328,7 +341,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
     double Y0 = 0., Y1 = 0., Y2 = 0.;
     FaceStream fs;
     fs.start(vel(A.u, mstart - 1), vel(A.v, mstart - 1));
-    double wprev = 0., r0prev = 0.;
+    double wprev = 0., r0prev = 0., bprev = 0.;
     double s0 = 0., s1 = 0., s2 = 0., s3 = 0., s4 = 0.;
     for (int mb = mstart; mb < mstop; mb += PF) {
 #pragma unroll
@@ -337,10 +350,10 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
             const double dv = (double)Pd[u], r0v = P0[u];
             fs.push(k, Pu[u], Pw[u]);
             const bool rin = m >= 0 && m < N1;
-            // the vector on row m: s, or the first direction p = r
-            const double wv = MODE == CN_S ? Pa[u] - alpha * Pv[u] : Pa[u];
-            const double yv = (rin && jv) ? wv * dv : 0.;
-            if (rin && m >= i0 && m < i1 && outl) {
+            // the vector on row m: s, the first direction p = r, or the iterate x
+            const double wv = MODE == CN_S ? Pa[u] - alpha * Pv[u] : (MODE == CN_R ? Pv[u] : Pa[u]);
+            const double yv = (rin && jv) ? (MODE == CN_R ? wv : wv * dv) : 0.;
+            if (MODE != CN_R && rin && m >= i0 && m < i1 && outl) {
                 if (MODE == CN_S) {
                     s3 += r0v * wv;
                     s4 += wv * wv;
@@ -362,6 +375,9 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
                         s0 += acc * wprev;
                         s1 += acc * acc;
                         s2 += r0prev * acc;
+                    } else if (MODE == CN_R) {
+                        const double rv = bprev - acc;
+                        s0 += rv * rv;
                     } else {
                         A.out[(long)q * N2 + j] = acc;
                         s0 += r0prev * acc;
@@ -370,6 +386,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
             }
             wprev = wv;
             r0prev = r0v;
+            bprev = Pa[u];
             fetch(u, m + PF);
         }
     }
@@ -445,7 +462,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES, IMPEC_CNP_MIN_WAVES) k_cn_pstep
         Pw[u] = vel(A.v, m);
         Pp[u] = cell(A.pin, m - 1);
         if (!A.xzero) Px[u] = cellx(A.x, m - 1);
-        P0[u] = cell(A.r0, m - 2);
+        P0[u] = (double)cellf(A.r0, m - 2);
     };
     const int mstart = i0 - 2, mend = i1 + 2;
     const int mstop = mstart + (mend - mstart + PF - 1) / PF * PF;
@@ -519,7 +536,8 @@ __global__ void __launch_bounds__(64 * CN_WAVES, IMPEC_CNP_MIN_WAVES) k_cn_pstep
 // partials b.v and b.b.  As in the p-launch, lanes 2..61 of a strip and
 // rows i0-2 .. i1+1 are streamed for the two chained stencils.
 __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_start(CnArgs A, const double *__restrict__ c, int lds,
-                                                            double *__restrict__ b, float *__restrict__ dinv) {
+                                                            double *__restrict__ b, float *__restrict__ dinv,
+                                                            float *__restrict__ r0) {
     const int lane = threadIdx.x & 63;
     const int wid = xcd_block(blockIdx.x, gridDim.x) * CN_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wid >= A.nwave) return;
@@ -554,11 +572,11 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_start(CnArgs A, const doub
     for (int u = 0; u < PF; u++) fetch(u, mstart + u);
     double C0 = 0., C1 = 0., C2 = 0.;      // c of rows m-2, m-1, m
     double Y0 = 0., Y1 = 0., Y2 = 0.;      // b/D of rows m-3, m-2, m-1
-    double bprev = 0.;                     // b of row m-2
+    double bprev = 0.;                     // r0 = b rounded to FP32, of row m-2
     double aprev[9] = {};                  // coefficients of row m-2
     FaceStream fs;
     fs.start(vel(A.u, mstart - 1), vel(A.v, mstart - 1));
-    double s0 = 0., s1 = 0.;
+    double s0 = 0., s1 = 0., s2 = 0.;
     for (int mb = mstart; mb < mstop; mb += PF) {
 #pragma unroll
         for (int u = 0; u < PF; u++) {
@@ -574,11 +592,14 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_start(CnArgs A, const doub
             double a[9];
             cn_coeffs(k, T, q, j, a);
             const float dq = (float)(1. / a[ST_C]);
+            const float r0q = (float)bq;
             if (qin && q >= i0 && q < i1 && outl) {
                 const long o = (long)q * N2 + j;
                 b[o] = bq;
                 dinv[o] = dq;
+                r0[o] = r0q;
                 s1 += bq * bq;
+                s2 += (double)r0q * bq;
             }
             Y0 = Y1; Y1 = Y2; Y2 = (qin && jv) ? bq * (double)dq : 0.;
             // row q2 = m - 2: v = A (b/D)
@@ -592,7 +613,7 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_start(CnArgs A, const doub
             }
 #pragma unroll
             for (int e = 0; e < 9; e++) aprev[e] = a[e];
-            bprev = bq;
+            bprev = (double)r0q;
             fetch(u, m + PF);
         }
     }
@@ -600,10 +621,12 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_start(CnArgs A, const doub
     for (int o = 32; o > 0; o >>= 1) {
         s0 += __shfl_xor(s0, o);
         s1 += __shfl_xor(s1, o);
+        s2 += __shfl_xor(s2, o);
     }
     if (lane == 0) {
         A.part[wid] = s0;
         A.part[RED_BLOCKS + wid] = s1;
+        A.part[2 * RED_BLOCKS + wid] = s2;
     }
 }
 
@@ -651,7 +674,7 @@ static_assert(IMPEC_CN_RHS_BLOCKS <= RED_BLOCKS, "partials of the right-hand sid
 template <bool STORE>
 __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, const double *__restrict__ u,
                              const double *__restrict__ v, const double *__restrict__ b, double *__restrict__ r,
-                             double *__restrict__ r0, double *__restrict__ part) {
+                             float *__restrict__ r0, double *__restrict__ part) {
     const HwGrid g_;
     const int j = g_.bx() * 64 + threadIdx.x;
     double s[2] = {0., 0.};
@@ -673,12 +696,13 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
             if (we && no) acc += a[ST_NW] * cc[-L + 1];
             const long id = (long)i * k.N2 + j;
             const double rv = b[id] - acc;
+            const float r0v = (float)rv;   // the shadow residual, rounded
             if (STORE) {
                 r[id] = rv;
-                r0[id] = rv;
+                r0[id] = r0v;
             }
             s[0] += rv * rv;
-            s[1] += rv * rv;
+            s[1] += (double)r0v * rv;
         }
     block_reduce<RED_SUM, 2>(s);
     if (tid_linear() == 0) {
@@ -691,8 +715,9 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
 // second pass of the BiCGStab reductions (NV partial vectors) and the scalar
 // recurrences:
 //  stage 0 (init):     rr = bb = b.b, rho = alpha = omega = 1, rho_new = b.b
-//  stage 4 (guess):    rr = rho_new = r0.r0 of the initial residual
-//  stage 5 (start):    b.v, b.b of a zero-guess start: stages 0 and 1 at once
+//  stage 4 (guess):    rr = r.r, rho_new = r0.r of the initial residual
+//  stage 5 (start):    r0.v, b.b, r0.b of a zero-guess start (r = p = b, r0 = b
+//                      rounded to FP32): stages 0 and 1 at once
 //  stage 1 (p-launch): r0.v, r.r; rho = rho_new; alpha = rho / r0.v; rr = r.r
 //  stage 2 (s-launch): t.s, t.t, r0.t, r0.s, s.s; omega = t.s / t.t (0 when
 //                      t = 0); rho_new = r0.r of r = s - omega t, as
@@ -716,14 +741,14 @@ __global__ void __launch_bounds__(BICG_FIN_THREADS) k_bicg_finish(const double *
     } else if (stage == 4) {
         S[S_RR] = v[0];
         S[S_RHO_NEW] = v[1];
-    } else if (stage == 5) {   // a zero-guess start: v[0] = b.v, v[1] = b.b
+    } else if (stage == 5 && NV >= 3) {   // a zero-guess start: v[0] = r0.v, v[1] = b.b, v[2] = r0.b
         S[S_RR] = v[1];
         S[S_BB] = v[1];
         S[S_OMEGA] = 1.;
-        S[S_RHO] = v[1];
-        S[S_RHO_NEW] = v[1];
+        S[S_RHO] = v[2];
+        S[S_RHO_NEW] = v[2];
         S[S_R0V] = v[0];
-        S[S_ALPHA] = v[1] / v[0];
+        S[S_ALPHA] = v[2] / v[0];
     } else if (stage == 6) {
         S[S_CNRES] = v[0];
     } else if (stage == 1) {
@@ -814,7 +839,8 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     const int N1 = ctx->N1, N2 = ctx->N2;
     hipStream_t s = ctx->stream;
     // the solution is formed in place in the interior of c (x0 = c for cn_guess = 1)
-    double *b = ctx->cn_rhs, *x = ctx->c + (long)ctx->B * ctx->lds + ctx->B, *r = ctx->cn_r, *r0 = ctx->cn_r0;
+    double *b = ctx->cn_rhs, *x = ctx->c + (long)ctx->B * ctx->lds + ctx->B, *r = ctx->cn_r;
+    float *r0 = (float *)ctx->cn_r0;   // FP32 shadow residual in the FP64 buffer's first half
     // r, p, v ping-pong: a strip reads its neighbours' old values at the halo
     double *pb[2] = {ctx->cn_p, ctx->cn_y}, *vb[2] = {ctx->cn_v, ctx->cn_z}, *rb[2] = {ctx->cn_r, ctx->cn_r2};
     bool zero = ctx->P.cn_guess != 1;
@@ -826,7 +852,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     a.u = ctx->u; a.v = ctx->v;
     a.dinv = ctx->cn_dg;
     a.r = zero ? b : r;
-    a.r0 = zero ? b : r0;
+    a.r0 = r0;
     a.ldx = ctx->lds;
     a.S = ctx->scal;
     a.part = ctx->part;
@@ -864,9 +890,9 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         CnArgs az = a2;
         az.out = vb[1];
         prof_begin(ctx);
-        hipLaunchKernelGGL(k_cn_start, pg, sblk, 0, s, az, ctx->c, ctx->lds, b, ctx->cn_dg);
+        hipLaunchKernelGGL(k_cn_start, pg, sblk, 0, s, az, ctx->c, ctx->lds, b, ctx->cn_dg, r0);
         prof_end(ctx, "cn_start");
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a2.nwave, 5, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<3>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a2.nwave, 5, ctx->scal);
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = fetch(ctx))) return st;
         bb = ctx->h_scal[S_BB];
@@ -907,6 +933,9 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
     const int nvb = (int)(vg.x * vg.y);
     for (;;) {
+        double rr_mark = rr;   // the recursive residual when it last halved
+        int stall = 0;
+        bool stalled = false;
         while (!(rr <= tol * tol * bb)) {
             if (its >= maxit) { status = IMPEC_ENOCONV; break; }
             if (!(rr == rr)) { status = IMPEC_ENAN; break; }
@@ -925,12 +954,23 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             CnArgs ap = a2;
             ap.r = rcur; ap.pin = its == 0 ? pfirst : pb[cur]; ap.vin = vb[cur]; ap.x = x;
             ap.xzero = zero && its == 0;
-            if (rr <= tol * tol * bb || !(rr == rr)) {
+            // a recursion whose residual norm has not halved in
+            // IMPEC_CN_STALL_ITS iterations (at the rounding floor, or stuck)
+            // is checked against the true residual and restarted like a
+            // converged one
+            if (rr <= 0.25 * rr_mark) {
+                rr_mark = rr;
+                stall = 0;
+            } else {
+                stalled = ++stall >= IMPEC_CN_STALL_ITS;
+            }
+            if (rr <= tol * tol * bb || !(rr == rr) || stalled) {
                 prof_begin(ctx);
                 hipLaunchKernelGGL(k_cn_xfinal, cn_vgrid(N1, N2), dim3(64, 4), 0, s, ap);
                 prof_end(ctx, "cn_xfinal");
                 IMPEC_HIP_CHECK(ctx, hipGetLastError());
                 its++;
+                if (stalled) break;
                 continue;   // the loop test ends the solve (or reports the non-finite residual)
             }
             if (rb[rnext] == rcur) rnext ^= 1;   // r' never overwrites the r it is formed from
@@ -954,9 +994,13 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         // s-launch's partials) can cancel at convergence: the true residual of
         // the iterate decides, as the pressure solve's does
         prof_begin(ctx);
-        hipLaunchKernelGGL(k_bicg_guess<false>, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, nullptr,
-                           nullptr, ctx->part);
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nvb, 6, ctx->scal);
+        {
+            CnArgs ar = a;
+            ar.r = b;
+            ar.x = x;
+            hipLaunchKernelGGL(k_cn_stream<CN_R>, sg, sblk, 0, s, ar);
+        }
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 6, ctx->scal);
         prof_end(ctx, "cn_check");
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = fetch(ctx))) return st;

@@ -136,6 +136,24 @@ __device__ __forceinline__ double row_dot(const double a[9], const double *__res
     return s;
 }
 
+// sum |a_k x_k| over the row (the rounding scale of row_dot)
+template <bool DIAG>
+__device__ __forceinline__ double row_absdot(const double a[9], const double *__restrict__ x, int ld, int i, int j,
+                                             int n1, int n2) {
+    const double *xc = x + (long)i * ld + j;
+    double s = fabs(a[ST_C] * xc[0]);
+    const bool w = i > 0, e = i < n1 - 1, so = j > 0, no = j < n2 - 1;
+    if (w) s += fabs(a[ST_W] * xc[-ld]);
+    if (DIAG && w && so) s += fabs(a[ST_SW] * xc[-ld - 1]);
+    if (so) s += fabs(a[ST_S] * xc[-1]);
+    if (DIAG && e && so) s += fabs(a[ST_SE] * xc[ld - 1]);
+    if (e) s += fabs(a[ST_E] * xc[ld]);
+    if (DIAG && e && no) s += fabs(a[ST_NE] * xc[ld + 1]);
+    if (no) s += fabs(a[ST_N] * xc[1]);
+    if (DIAG && w && no) s += fabs(a[ST_NW] * xc[-ld + 1]);
+    return s;
+}
+
 // off-diagonal part for Gauss-Seidel: b - sum_{k != C} a_k x_nb
 template <bool DIAG>
 __device__ __forceinline__ double gs_rhs(const double a[9], double b, const double *__restrict__ x, int ld, int i,
@@ -678,9 +696,12 @@ __global__ void k_cg_update(CgArgs c, double *__restrict__ p, const double *__re
         cg_cell(c, id, i, j);
         const long o = (long)(i + 1) * c.ld + (j + 1);
         double *xp = p + (long)(i + c.B) * c.ldp + (j + c.B);
-        *xp = *xp + alpha * d[o];
-        const double rv = r[o] - alpha * qv[o];
-        r[o] = rv;
+        double rv = r[o];
+        if (alpha != 0.) {   // no step: a broken-down direction may hold non-finite values
+            *xp = *xp + alpha * d[o];
+            rv = rv - alpha * qv[o];
+            r[o] = rv;
+        }
         s[0] += rv * rv;
         s[1] += rv;
     }
@@ -720,23 +741,30 @@ __global__ void k_shift(CgArgs c, double *__restrict__ p, const double *__restri
 template <bool DIAG, class G>
 __device__ __forceinline__ void true_residual_body(const G &gr_, LevArgs L, CgArgs c, const double *__restrict__ p, const double *__restrict__ q,
                                 double *__restrict__ part) {
-    double s[1] = {0.};
+    // partials of ||b - A p||^2 and of || |b| + |A| |p| ||^2 (the rounding scale)
+    double s[2] = {0., 0.};
     const double *x = p + (long)c.B * c.ldp + c.B;
     for (long id = (long)gr_.bx() * blockDim.x + threadIdx.x; id < c.n; id += (long)gr_.gx() * blockDim.x) {
         int i, j;
         cg_cell(c, id, i, j);
-        double rv;
+        double rv, mag;
         if (i == c.pin_i && j == c.pin_j) {
             rv = -q[id] - c.dpin * x[(long)i * c.ldp + j];
+            mag = fabs(q[id]) + fabs(c.dpin * x[(long)i * c.ldp + j]);
         } else {
             double a[9];
             op_row_L<DIAG>(L, i, j, a);
             rv = -q[id] - row_dot<DIAG>(a, x, c.ldp, i, j, c.n1, c.n2);
+            mag = fabs(q[id]) + row_absdot<DIAG>(a, x, c.ldp, i, j, c.n1, c.n2);
         }
         s[0] += rv * rv;
+        s[1] += mag * mag;
     }
-    block_reduce<RED_SUM, 1>(s);
-    if (tid_linear() == 0) part[gr_.bx()] = s[0];
+    block_reduce<RED_SUM, 2>(s);
+    if (tid_linear() == 0) {
+        part[gr_.bx()] = s[0];
+        part[RED_BLOCKS + gr_.bx()] = s[1];
+    }
 }
 template <bool DIAG>
 __global__ void k_true_residual(LevArgs L, CgArgs c, const double *__restrict__ p, const double *__restrict__ q,
@@ -1920,9 +1948,12 @@ __global__ void k_cg_update32(CgArgs c, float *__restrict__ e, const float *__re
     if (j < c.n2)
         for (int i = blockIdx.y * 4 + threadIdx.y; i < c.n1; i += gridDim.y * 4) {
             const long o = (long)(i + 1) * c.ld + (j + 1);
-            e[o] = (float)((double)e[o] + alpha * (double)d[o]);
-            const float rv = (float)((double)r[o] - alpha * (double)qv[o]);
-            r[o] = rv;
+            float rv = r[o];
+            if (alpha != 0.) {   // no step: a broken-down direction may hold non-finite values
+                e[o] = (float)((double)e[o] + alpha * (double)d[o]);
+                rv = (float)((double)r[o] - alpha * (double)qv[o]);
+                r[o] = rv;
+            }
             s[0] += (double)rv * (double)rv;
             s[1] += (double)rv;
         }
@@ -1979,9 +2010,12 @@ __device__ __forceinline__ void cg_update2_body(const G &gr_, CgArgs c, double *
         if (j >= c.n2) break;
         const long o = (long)(i + 1) * c.ld + (j + 1);
         double *xp = p + (long)(i + c.B) * c.ldp + (j + c.B);
-        *xp = *xp + alpha * d[o];
-        const double rv = r[o] - alpha * qv[o];
-        r[o] = rv;
+        double rv = r[o];
+        if (alpha != 0.) {   // no step: a broken-down direction may hold non-finite values
+            *xp = *xp + alpha * d[o];
+            rv = rv - alpha * qv[o];
+            r[o] = rv;
+        }
         s[0] += rv * rv;
         s[1] += rv;
     }
@@ -2507,11 +2541,14 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     }
     int its = 0, status = IMPEC_OK;
     double relres = 0., rr = 0.;
+    double floor_rel = 0., tol_eff = tol;   // the true residual's rounding floor, max(tol, floor)
     double *dold = ctx->pd, *dnew = ctx->pz;
     float *dold32 = ctx->pd32, *dnew32 = ctx->pz32;
     double rr_ref = 0.;   // FP64 residual norm^2 at the FP32 recursion's last restart
+    int restarts_all = 0, elim_its = 0;
     for (int phase = ctx->symmetric ? 0 : 1; phase < 2; phase++) {
         c.shift = phase == 0;
+        const int its_at_start = its;
         // the FP32-vector CG runs the consistent phase; the elimination phase, a
         // short continuation at the FP64 rounding floor of a worse-conditioned
         // system, keeps FP64 vectors (its FP32 recursion stalls there)
@@ -2542,16 +2579,22 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
             }
             hipLaunchKernelGGL(k_true_residual<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q, ctx->part);
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
-            int e = finish_sum(ctx, nb, 1, S_TT);
+            int e = finish_sum(ctx, nb, 2, S_TT);
             if (!e) e = fetch_scal(ctx);
-            if (!e) relres = ctx->bnorm > 0 ? sqrt(ctx->h_scal[S_TT]) / ctx->bnorm : 0.;
+            if (!e) {
+                relres = ctx->bnorm > 0 ? sqrt(ctx->h_scal[S_TT]) / ctx->bnorm : 0.;
+                floor_rel = ctx->bnorm > 0 ? IMPEC_FLOOR_C * 0x1p-53 * sqrt(ctx->h_scal[S_FLOOR]) / ctx->bnorm : 0.;
+                tol_eff = tol > floor_rel ? tol : floor_rel;
+            }
             return e;
         };
         int since = 0, restarts = 0;
         double rr_check = -1.;   // true residual norm^2 at the previous restart (-1: none yet)
-        bool finished = false;
+        bool finished = false, at_floor = false;
         for (;;) {
             bool broke = false;
+            double rr_mark = rr;   // the recursive residual when it last halved
+            int stall = 0;
             while (!(rr <= tol * tol * bb)) {
                 if (its >= maxit) { status = IMPEC_ENOCONV; break; }
                 if (!(rr == rr)) { status = IMPEC_ENAN; break; }
@@ -2577,6 +2620,31 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
                     if ((st = cg_reliable32(ctx, a0, c, nb, &rr))) return st;
                     rr_ref = rr;
                 }
+                if (rr <= 0.25 * rr_mark) {
+                    rr_mark = rr;
+                    stall = 0;
+                } else if (++stall >= IMPEC_STALL_ITS && rr == rr && !(rr <= tol * tol * bb)) {
+                    // the pinned system's true residual and floor (p moves by a
+                    // constant in the consistent phase, its pinned value is kept
+                    // in the elimination phase: the recursion is unchanged)
+                    if (ru) {   // the FP32 corrections into p first
+                        if ((st = cg_reliable32(ctx, a0, c, nb, &rr))) return st;
+                        rr_ref = rr;
+                    }
+                    if ((st = finish_phase())) return st;
+                    if (relres <= tol_eff) {
+                        at_floor = true;
+                        break;
+                    }
+                    const double gate = IMPEC_STALL_GATE * floor_rel;
+                    if (rr <= gate * gate * bb) break;   // restart from the true residual
+                    rr_mark = rr;
+                    stall = 0;
+                }
+            }
+            if (at_floor) {
+                finished = true;
+                break;
             }
             if (ru && (status != IMPEC_OK || broke)) {   // keep the corrections made so far
                 const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
@@ -2605,17 +2673,20 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
             // restart while each restart at least halves the true residual
             // (up to IMPEC_MAX_RESTARTS), and at least 3 times
             const bool progress = rr_before < 0. || rr <= 0.25 * rr_before;
-            if (rr <= tol * tol * bb || restarts >= IMPEC_MAX_RESTARTS || (restarts >= 3 && !progress)) break;
+            if (rr <= tol_eff * tol_eff * bb || restarts >= IMPEC_MAX_RESTARTS || (restarts >= 3 && !progress)) break;
             restarts++;
             since = 0;
         }
         if (!finished && (st = finish_phase())) return st;
-        if (status != IMPEC_OK || relres <= tol) break;
+        restarts_all += restarts;
+        if (phase == 1) elim_its = its - its_at_start;
+        if (status != IMPEC_OK || relres <= tol_eff) break;
     }
     // the elimination phase is the last: its restarts are spent and the
-    // pinned system's true residual still misses the tolerance
+    // pinned system's true residual is still above the tolerance and above
+    // what FP64 can resolve
     bool missed = false;
-    if (status == IMPEC_OK && !(relres <= tol)) {
+    if (status == IMPEC_OK && !(relres <= tol_eff)) {
         status = IMPEC_ENOCONV;
         missed = true;
     }
@@ -2629,6 +2700,9 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
     ctx->stats.last_its = its;
     ctx->stats.last_vcycles = its;
     ctx->stats.last_relres = relres;
+    ctx->stats.last_relres_floor = floor_rel;
+    ctx->stats.last_restarts = restarts_all;
+    ctx->stats.last_elim_its = elim_its;
     ctx->stats.total_its += its;
     ctx->stats.calls += 1;
     if (its_out) *its_out = its;
@@ -2916,6 +2990,8 @@ __global__ void kb_ctl_first(const CtlRec *__restrict__ tab, int n) {
     S[S_ESINCE] = 0.;
     S[S_ERST] = 0.;
     S[S_ERRPREV] = -1.;
+    S[S_EMARK] = rr;
+    S[S_ESTALL] = 0.;
     S[S_ESTATUS] = 0.;
     double est = EST_ITER;
     if (rr <= S[S_ETOL]) est = EST_DONE;
@@ -2924,7 +3000,9 @@ __global__ void kb_ctl_first(const CtlRec *__restrict__ tab, int n) {
     S[S_EST] = est;
     if (!tab[m].always_zero) S[S_ZEROP] = 0.;
 }
-// after an iteration: count it; converged recursions get their residual recomputed
+// after an iteration: count it; converged, broken-down and stalled recursions
+// (residual norm not halved in IMPEC_STALL_ITS iterations) get their residual
+// recomputed
 __global__ void kb_ctl_iter(const CtlRec *__restrict__ tab, int n) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n) return;
@@ -2934,11 +3012,20 @@ __global__ void kb_ctl_iter(const CtlRec *__restrict__ tab, int n) {
     S[S_EITS] = its;
     S[S_ESINCE] = S[S_ESINCE] + 1.;
     const double rr = S[S_RR];
+    bool stalled = false;
+    if (rr <= 0.25 * S[S_EMARK]) {
+        S[S_EMARK] = rr;
+        S[S_ESTALL] = 0.;
+    } else {
+        S[S_ESTALL] = S[S_ESTALL] + 1.;
+        stalled = S[S_ESTALL] >= (double)IMPEC_STALL_ITS;
+    }
     if (rr <= S[S_ETOL] || cg_broke_down(S)) S[S_EST] = EST_CHECK;
     else if (its >= S[S_EMAXIT]) { S[S_EST] = EST_DONE; S[S_ESTATUS] = 1.; }
     else if (!(rr == rr)) { S[S_EST] = EST_DONE; S[S_ESTATUS] = 2.; }
+    else if (stalled) S[S_EST] = EST_CHECK;
 }
-// after a recomputed residual: done, or restart the recursion (at most 3 times)
+// after a recomputed residual: done, or restart the recursion
 __global__ void kb_ctl_check(const CtlRec *__restrict__ tab, int n) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n) return;
@@ -2956,6 +3043,8 @@ __global__ void kb_ctl_check(const CtlRec *__restrict__ tab, int n) {
     }
     S[S_ERST] = S[S_ERST] + 1.;
     S[S_ESINCE] = 0.;
+    S[S_EMARK] = rr;
+    S[S_ESTALL] = 0.;
     if (S[S_EITS] >= S[S_EMAXIT]) { S[S_EST] = EST_DONE; S[S_ESTATUS] = 1.; }
     else if (!(rr == rr)) { S[S_EST] = EST_DONE; S[S_ESTATUS] = 2.; }
     else S[S_EST] = EST_ITER;
@@ -3303,7 +3392,7 @@ int ens_press_build(impec_ensemble *e) {
         fi[k] = FinishRec{ctx->part, ctx->scal, nb, 2, (int)S_RR, (int)RED_SUM};
         fd[k] = FinishRec{ctx->part, ctx->scal, dir_mv_args<double>(ctx, a0, c).nwave, 1, (int)S_DQ, (int)RED_SUM};
         frr[k] = FinishRec{ctx->part, ctx->scal, vg.nblocks, 2, (int)S_RR, (int)RED_SUM};
-        ft[k] = FinishRec{ctx->part, ctx->scal, nb, 1, (int)S_TT, (int)RED_SUM};
+        ft[k] = FinishRec{ctx->part, ctx->scal, nb, 2, (int)S_TT, (int)RED_SUM};
         const int nwave0 = t->mixed ? fused_args<float>(ctx, 0).nwave : fused_args<double>(ctx, 0).nwave;
         rh[k] = RhoRec{ctx->part, nwave0, 1.0 / (double)c.n, ctx->scal};
         up[k] = Upd2Rec{c, ctx->p, ctx->pd, ctx->pz, ctx->pr, ctx->pq, ctx->scal, ctx->part, (int)vg.grid.x,

@@ -2,7 +2,9 @@
 // ratio-form limiter and upwind Riemann flux (MUSCLSplitSolver.cpp:54-189),
 // first-order upwind (UpwindSplitSolver.cpp:43-65), conservative updates
 // (HyperbolicSplitSolver.cpp:113-159) and the splitting modes
-// (HyperbolicSplitSolver.cpp:51-106).  FP64.
+// (HyperbolicSplitSolver.cpp:51-106).  FP64; the one-pass parallel
+// splitting also in FP32 arithmetic (reconstruction, limiter, half step and
+// fluxes) with FP64 states and updates, the mixed-precision configuration.
 //
 // Sweeps are out of place: cells a sweep updates are written from the input
 // state, every other cell of the ghosted array is copied, which equals the
@@ -13,51 +15,59 @@
 
 namespace impec {
 
-struct SweepConst {
+// the sweep's constants in the arithmetic type T of its fluxes
+template <typename T>
+struct SweepConstT {
     int N1, N2, B, lds, ldv;
-    double hdtdx;   // 0.5 * dt / dx   (MUSCLSplitSolver.cpp:118)
-    double dtdx;    // dt / dx         (HyperbolicSplitSolver.cpp:131)
-    double phi;     // porosity
+    T hdtdx;   // 0.5 * dt / dx   (MUSCLSplitSolver.cpp:118)
+    T dtdx;    // dt / dx         (HyperbolicSplitSolver.cpp:131)
+    T phi;     // porosity
 };
+using SweepConst = SweepConstT<double>;
 
-template <bool PHI1>
-__device__ __forceinline__ double flux(double vel, double conc, double phi) {
+template <bool PHI1, typename T>
+__device__ __forceinline__ T flux(T vel, T conc, T phi) {
     // HyperbolicSolver::Flux (HyperbolicSolver.cpp:208-210); x / 1.0 == x
     return PHI1 ? vel * conc : vel * conc / phi;
 }
 
-template <bool PHI1>
-__device__ __forceinline__ double upwind(double speed, double UL, double UR, double phi) {
+template <bool PHI1, typename T>
+__device__ __forceinline__ T upwind(T speed, T UL, T UR, T phi) {
     return (speed > 0) ? flux<PHI1>(speed, UL, phi) : flux<PHI1>(speed, UR, phi);
 }
+
+template <typename T> __device__ __forceinline__ T max_finite();
+template <> __device__ __forceinline__ double max_finite<double>() { return DBL_MAX; }
+template <> __device__ __forceinline__ float max_finite<float>() { return FLT_MAX; }
 
 // Reconstruct + GetSlope + SlopeLimiter (MUSCLSplitSolver.cpp:54-93).  The
 // limiter min(2r/(1+r), 2/(1+r)) is evaluated with one division: both
 // candidates share the rounded denominator 1+r and correctly rounded division
 // is monotone in the numerator, so the smaller quotient is the one whose
 // numerator min(2r, 2) is smaller -- bit-identical to evaluating both.
-__device__ __forceinline__ void reconstruct(double cm, double c0, double cp, double &UL, double &UR) {
-    const double dprev = c0 - cm, dnext = cp - c0;
-    double lim = 0.;
+template <typename T>
+__device__ __forceinline__ void reconstruct(T cm, T c0, T cp, T &UL, T &UR) {
+    const T dprev = c0 - cm, dnext = cp - c0;
+    T lim = 0;
     if (dprev * dnext > 0) {
-        double r = dprev / dnext;
-        if (r > DBL_MAX) r = DBL_MAX;
-        const double num = r < 1. ? 2. * r : 2.;
-        lim = num / (1. + r);
+        T r = dprev / dnext;
+        if (r > max_finite<T>()) r = max_finite<T>();
+        const T num = r < T(1) ? T(2) * r : T(2);
+        lim = num / (T(1) + r);
     }
-    const double d = lim * 0.5 * (dprev + dnext);
-    UR = c0 + 0.5 * d;
-    UL = c0 - 0.5 * d;
+    const T d = lim * T(0.5) * (dprev + dnext);
+    UR = c0 + T(0.5) * d;
+    UL = c0 - T(0.5) * d;
 }
 
 // half-step evolved interface states of cell k: left state of face k+1 and
 // right state of face k
-template <bool PHI1>
-__device__ __forceinline__ void hancock(double cm, double c0, double cp, double vl, double vr, const SweepConst &k,
-                                        double &left_of_next, double &right_of_this) {
-    double UL, UR;
-    reconstruct(cm, c0, cp, UL, UR);
-    const double df = k.hdtdx * (flux<PHI1>(vl, UL, k.phi) - flux<PHI1>(vr, UR, k.phi));
+template <bool PHI1, typename T>
+__device__ __forceinline__ void hancock(T cm, T c0, T cp, T vl, T vr, const SweepConstT<T> &k, T &left_of_next,
+                                        T &right_of_this) {
+    T UL, UR;
+    reconstruct<T>(cm, c0, cp, UL, UR);
+    const T df = k.hdtdx * (flux<PHI1>(vl, UL, k.phi) - flux<PHI1>(vr, UR, k.phi));
     left_of_next = UR + df;
     right_of_this = UL + df;
 }
@@ -225,62 +235,79 @@ constexpr int PS_OUT = 60, PS_WAVES = 4;
 #define IMPEC_PS_TARGET_WAVES 4096
 #endif
 
-struct ParArgs {
-    SweepConst kx, ky;          // x sweeps (dt / dx), y sweeps (dt / dy)
+// T: the arithmetic of the reconstructions, half steps and fluxes; states,
+// updates and the average stay FP64 whatever T is
+template <typename T>
+struct ParArgsT {
+    SweepConstT<T> kx, ky;      // x sweeps (dt / dx), y sweeps (dt / dy)
     const double *c, *u, *v;    // ghosted: c pitch lds, u and v pitch ldv
     double *out;                // the averaged state, pitch lds
     int R, nstrip, nwave;       // output rows per chunk, strips, wavefronts
 };
+using ParArgs = ParArgsT<double>;
 
 // Y sweep of one row held across the lanes (column j in lane j), with the
 // arithmetic of k_sweep_y: each cell's half-step states and each face's flux
 // are formed once, in the lane of the cell (of the face's right cell), and
 // reach the neighbours by lane shifts.  Cell j's update needs lanes j-2..j+2.
-template <bool MUSCL, bool PHI1>
-__device__ __forceinline__ double ysweep_lanes(const SweepConst &k, double c, double v) {
-    const double cl = lane_prev(c);
-    double G;   // flux through the face left of the lane's cell
+// the conservative update c + dtdx (F_left - F_right).  FP64: the reference's
+// expression.  FP32 fluxes: each face's dtdx F is rounded once and both cells
+// of the face take that same value, so the increments telescope and the FP64
+// state keeps its mass to FP64 rounding
+__device__ __forceinline__ double flux_update(double c, double dtdx, double Fl, double Fr) {
+    return c + dtdx * (Fl - Fr);
+}
+__device__ __forceinline__ double flux_update(double c, float dtdx, float Fl, float Fr) {
+    return c + ((double)(dtdx * Fl) - (double)(dtdx * Fr));
+}
+
+template <bool MUSCL, bool PHI1, typename T>
+__device__ __forceinline__ double ysweep_lanes(const SweepConstT<T> &k, double c, double v) {
+    const T ct = (T)c, vt = (T)v;   // no-ops in FP64
+    const T cl = lane_prev(ct);
+    T G;   // flux through the face left of the lane's cell
     if (MUSCL) {
-        double lo, ro;
-        hancock<PHI1>(cl, c, lane_next(c), v, lane_next(v), k, lo, ro);
-        G = upwind<PHI1>(v, lane_prev(lo), ro, k.phi);
+        T lo, ro;
+        hancock<PHI1>(cl, ct, lane_next(ct), vt, lane_next(vt), k, lo, ro);
+        G = upwind<PHI1>(vt, lane_prev(lo), ro, k.phi);
     } else {
-        G = upwind<PHI1>(v, cl, c, k.phi);
+        G = upwind<PHI1>(vt, cl, ct, k.phi);
     }
-    return c + k.dtdx * (G - lane_next(G));
+    return flux_update(c, k.dtdx, G, lane_next(G));
 }
 
 // X sweep of a lane's column as rows stream past, with the arithmetic of
 // k_sweep_x: push(row k, faces k-1 and k) forms the half-step states of row
 // k-1 and the flux through face k-1, and returns the update of row k-2.
 // After push(row k) the fields hold:
+template <typename T>
 struct XStream {
-    double cm2 = 0., cm1 = 0., c0 = 0.;   // rows k-2, k-1, k
-    double lo = 0.;                       // left state of face k (from row k-1)
-    double F = 0.;                        // flux through face k-1
+    double cm2 = 0., cm1 = 0., c0 = 0.;   // rows k-2, k-1, k (the states, FP64)
+    T lo = 0;                             // left state of face k (from row k-1)
+    T F = 0;                              // flux through face k-1
     template <bool MUSCL, bool PHI1>
-    __device__ __forceinline__ double push(const SweepConst &k, double ck, double ukm1, double uk) {
+    __device__ __forceinline__ double push(const SweepConstT<T> &k, double ck, double ukm1, double uk) {
         cm2 = cm1; cm1 = c0; c0 = ck;
-        double Fn;   // flux through face k-1
+        T Fn;   // flux through face k-1
         if (MUSCL) {
-            double lo_n, ro_n;
-            hancock<PHI1>(cm2, cm1, c0, ukm1, uk, k, lo_n, ro_n);
-            Fn = upwind<PHI1>(ukm1, lo, ro_n, k.phi);
+            T lo_n, ro_n;
+            hancock<PHI1>((T)cm2, (T)cm1, (T)c0, (T)ukm1, (T)uk, k, lo_n, ro_n);
+            Fn = upwind<PHI1>((T)ukm1, lo, ro_n, k.phi);
             lo = lo_n;
         } else {
-            Fn = upwind<PHI1>(ukm1, cm2, cm1, k.phi);
+            Fn = upwind<PHI1>((T)ukm1, (T)cm2, (T)cm1, k.phi);
         }
-        const double out = cm2 + k.dtdx * (F - Fn);
+        const double out = flux_update(cm2, k.dtdx, F, Fn);
         F = Fn;
         return out;
     }
 };
 
-template <bool MUSCL, bool PHI1>
-__device__ __forceinline__ void advect_parallel_body(const ParArgs &A, const int wid) {
+template <bool MUSCL, bool PHI1, typename T>
+__device__ __forceinline__ void advect_parallel_body(const ParArgsT<T> &A, const int wid) {
     const int lane = threadIdx.x & 63;
     if (wid >= A.nwave) return;
-    const SweepConst &kx = A.kx, &ky = A.ky;
+    const SweepConstT<T> &kx = A.kx, &ky = A.ky;
     const int N1 = kx.N1, N2 = kx.N2, B = kx.B;
     const long LS = kx.lds, LV = kx.ldv;
     const int strip = wid % A.nstrip, chunk = wid / A.nstrip;
@@ -310,7 +337,7 @@ __device__ __forceinline__ void advect_parallel_body(const ParArgs &A, const int
     const int kstop = kstart + (kend - kstart + PF - 1) / PF * PF;
 #pragma unroll
     for (int s_ = 0; s_ < PF; s_++) fetch(s_, kstart + s_);
-    XStream xy, yx;                       // X sweeps of the state (XY) and of its Y sweep (YX)
+    XStream<T> xy, yx;                    // X sweeps of the state (XY) and of its Y sweep (YX)
     double ukm1 = 0., vkm1 = 0., vkm2 = 0.;
     for (int kb = kstart; kb < kstop; kb += PF) {
 #pragma unroll
@@ -320,13 +347,13 @@ __device__ __forceinline__ void advect_parallel_body(const ParArgs &A, const int
             const int i = k - 2;
             const bool xrow = i >= 0 && i < N1;                  // X sweeps update this row
             // XY: X sweep of the state, then Y sweep of row i
-            const double x1 = xy.push<MUSCL, PHI1>(kx, ck, ukm1, uk);
+            const double x1 = xy.template push<MUSCL, PHI1>(kx, ck, ukm1, uk);
             const double c1 = xrow ? x1 : xy.cm2;
             const double y2 = ysweep_lanes<MUSCL, PHI1>(ky, c1, vkm2);
             const double c2 = ycol ? y2 : c1;
             // YX: Y sweep of row k, then X sweep of row i
             const double y3 = ysweep_lanes<MUSCL, PHI1>(ky, ck, vk);
-            const double x4 = yx.push<MUSCL, PHI1>(kx, ycol ? y3 : ck, ukm1, uk);
+            const double x4 = yx.template push<MUSCL, PHI1>(kx, ycol ? y3 : ck, ukm1, uk);
             const double c4 = xrow ? x4 : yx.cm2;
             if (outl && i >= i0 && i < i1) A.out[(long)(i + B) * LS + (j + B)] = 0.5 * (c2 + c4);
             ukm1 = uk;
@@ -336,8 +363,8 @@ __device__ __forceinline__ void advect_parallel_body(const ParArgs &A, const int
         }
     }
 }
-template <bool MUSCL, bool PHI1>
-__global__ void __launch_bounds__(64 * PS_WAVES) k_advect_parallel(ParArgs A) {
+template <bool MUSCL, bool PHI1, typename T>
+__global__ void __launch_bounds__(64 * PS_WAVES) k_advect_parallel(ParArgsT<T> A) {
     advect_parallel_body<MUSCL, PHI1>(A, blockIdx.x * PS_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
 }
 
@@ -373,12 +400,25 @@ static SweepConst sweep_const(const impec_ctx *ctx, double dt, double h) {
     return k;
 }
 
+// the sweep constants rounded to the arithmetic type T
+template <typename T>
+static SweepConstT<T> sweep_const_t(const impec_ctx *ctx, double dt, double h) {
+    const SweepConst d = sweep_const(ctx, dt, h);
+    SweepConstT<T> k;
+    k.N1 = d.N1; k.N2 = d.N2; k.B = d.B; k.lds = d.lds; k.ldv = d.ldv;
+    k.hdtdx = (T)d.hdtdx;
+    k.dtdx = (T)d.dtdx;
+    k.phi = (T)d.phi;
+    return k;
+}
+
 // arguments of the one-pass parallel splitting of c into out: strips of
 // PS_OUT columns, chunks of rows for about IMPEC_PS_TARGET_WAVES wavefronts
-static ParArgs par_args(const impec_ctx *ctx, double dt, const double *c, double *out) {
-    ParArgs a;
-    a.kx = sweep_const(ctx, dt, ctx->dx);
-    a.ky = sweep_const(ctx, dt, ctx->dy);
+template <typename T = double>
+static ParArgsT<T> par_args(const impec_ctx *ctx, double dt, const double *c, double *out) {
+    ParArgsT<T> a;
+    a.kx = sweep_const_t<T>(ctx, dt, ctx->dx);
+    a.ky = sweep_const_t<T>(ctx, dt, ctx->dy);
     a.c = c; a.u = ctx->u; a.v = ctx->v; a.out = out;
     const int rows = ctx->N1 + 2 * ctx->B, cols = ctx->N2 + 2 * ctx->B;
     a.nstrip = (cols + PS_OUT - 1) / PS_OUT;
@@ -458,6 +498,15 @@ int copy_scalar(impec_ctx *ctx, const double *src, double *dst) {
     return IMPEC_OK;
 }
 
+template <typename T>
+static void launch_parallel(impec_ctx *ctx, const ParArgsT<T> &a, bool muscl, bool phi1) {
+    const dim3 g((a.nwave + PS_WAVES - 1) / PS_WAVES), blk(64 * PS_WAVES);
+    if (muscl && phi1) hipLaunchKernelGGL((k_advect_parallel<true, true, T>), g, blk, 0, ctx->stream, a);
+    else if (muscl) hipLaunchKernelGGL((k_advect_parallel<true, false, T>), g, blk, 0, ctx->stream, a);
+    else if (phi1) hipLaunchKernelGGL((k_advect_parallel<false, true, T>), g, blk, 0, ctx->stream, a);
+    else hipLaunchKernelGGL((k_advect_parallel<false, false, T>), g, blk, 0, ctx->stream, a);
+}
+
 // AdvectSplit (HyperbolicSplitSolver.cpp:51-106) after its FillBoundaries
 int advect_split(impec_ctx *ctx, double dt) {
     int st = launch_fill_scalar(ctx, ctx->c, ctx->P.is_test);
@@ -491,15 +540,11 @@ int advect_split(impec_ctx *ctx, double dt) {
         // X then Y and Y then X from c, averaged: one streaming pass into t1,
         // then t1 back into c
         (void)t2; (void)t3;
-        const ParArgs a = par_args(ctx, dt, c, t1);
         const int rows = ctx->N1 + 2 * ctx->B;
         const bool muscl = ctx->P.hyper_method == IMPEC_HYPER_MUSCL, phi1 = ctx->P.porosity == 1.;
-        const dim3 g((a.nwave + PS_WAVES - 1) / PS_WAVES), blk(64 * PS_WAVES);
         prof_begin(ctx);
-        if (muscl && phi1) hipLaunchKernelGGL((k_advect_parallel<true, true>), g, blk, 0, ctx->stream, a);
-        else if (muscl) hipLaunchKernelGGL((k_advect_parallel<true, false>), g, blk, 0, ctx->stream, a);
-        else if (phi1) hipLaunchKernelGGL((k_advect_parallel<false, true>), g, blk, 0, ctx->stream, a);
-        else hipLaunchKernelGGL((k_advect_parallel<false, false>), g, blk, 0, ctx->stream, a);
+        if (ctx->P.transport32) launch_parallel<float>(ctx, par_args<float>(ctx, dt, c, t1), muscl, phi1);
+        else launch_parallel<double>(ctx, par_args<double>(ctx, dt, c, t1), muscl, phi1);
         prof_end(ctx, "advect_parallel");
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(c, t1, sizeof(double) * (size_t)rows * ctx->lds, hipMemcpyDeviceToDevice,

@@ -14,16 +14,17 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 KERNELS = {
     # BiCGStab p-launch (k_cn_pstep): s = r - alpha v and t = A (s / D) again,
     # r' = s - omega t, x += alpha p/D + omega s/D, p' = r' + beta (p - omega v),
-    # v' = A (p' / D), r0.v', r'.r'; read r, v, p, x, r0 (5 x 8), 1/D (FP32, 4)
-    # and the face velocities u, v (2 x 8; the face tensors are formed in
-    # registers), write r', x, p', v' (4 x 8)
-    "cn_apply_p": dict(rocprof="k_cn_pstep", bytes_per_cell=92),
+    # v' = A (p' / D), r0.v', r'.r'; read r, v, p, x (4 x 8), the shadow residual
+    # r0 and 1/D (FP32, 2 x 4) and the face velocities u, v (2 x 8; the face
+    # tensors are formed in registers), write r', x, p', v' (4 x 8)
+    "cn_apply_p": dict(rocprof="k_cn_pstep", bytes_per_cell=88),
     # BiCGStab s-launch (k_cn_stream<1>): s = r - alpha v, t = A (s / D), t.s, t.t,
-    # r0.t, r0.s, s.s; read r, v, r0 (3 x 8), 1/D (4) and u, v (2 x 8); no vector written
-    "cn_apply_s": dict(rocprof="k_cn_stream<1>", bytes_per_cell=44),
-    # zero-guess start (k_cn_start): b, 1/D and v = A (b / D); read c, u, v
-    # (3 x 8), write b (8), 1/D (4), v (8)
-    "cn_start": dict(rocprof="k_cn_start", bytes_per_cell=44),
+    # r0.t, r0.s, s.s; read r, v (2 x 8), r0 and 1/D (FP32, 2 x 4) and u, v
+    # (2 x 8); no vector written
+    "cn_apply_s": dict(rocprof="k_cn_stream<1>", bytes_per_cell=40),
+    # zero-guess start (k_cn_start): b, 1/D, r0 = b in FP32 and v = A (b / D);
+    # read c, u, v (3 x 8), write b (8), 1/D and r0 (2 x 4), v (8)
+    "cn_start": dict(rocprof="k_cn_start", bytes_per_cell=48),
     # streaming V-cycle, finest level, down leg: read w, r (2 x 8), write x (8)
     # and the coarse right-hand side (8 / 4); FP32 cycle: w 4, r 8, x 4, b_c 1
     "mg_down_l0": dict(rocprof="k_mg_down<true, false", bytes_per_cell=34, bytes_per_cell_fp32=17),

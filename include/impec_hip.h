@@ -86,7 +86,9 @@ typedef struct impec_params {
                                /* level (0: automatic; rounded up to even)     */
     int cn_guess;              /* ext: initial guess of the dispersion solve:  */
                                /* 0 zero (AssignRHS_Parabolic), 1 current c    */
-    int reserved_i[1];
+    int transport32;           /* ext: 1: the parallel-splitting transport     */
+                               /* forms reconstructions, half steps and fluxes */
+                               /* in FP32 (states, updates, mass sums FP64)    */
     double x_lb, x_ub, y_lb, y_ub;
     double nu;                 /* hyperbolic.nu, 5-pt/rotated stencil weight   */
     double mu_res, mu_inj;
@@ -106,7 +108,11 @@ typedef struct impec_stats {
     int last_vcycles;          /* V-cycles in the last pressure solve          */
     int last_parabolic_its;    /* BiCGStab iterations, last CN solve           */
     int total_its, calls;      /* EllipticSolver::GetAvgIts bookkeeping        */
-    int reserved_i[3];
+    int last_restarts;         /* ext: restarts from the true residual in the  */
+                               /* last pressure solve                          */
+    int last_elim_its;         /* ext: of last_its, those of the elimination   */
+                               /* phase (0: the consistent phase converged)    */
+    int reserved_i[1];
     double last_relres;        /* true ||b - A p|| / ||b|| of the pinned system */
     double last_parabolic_relres;
     double total_mass_outflux; /* HyperbolicSolver::GetTotalMassOutflux        */
@@ -114,7 +120,10 @@ typedef struct impec_stats {
                                /* increment in the last solve (0: not used)    */
     double last_guess_gamma;   /* ext: step length given to the last step's    */
                                /* change p_n - p_(n-1) (0: not used)           */
-    double reserved_d[3];
+    double last_relres_floor;  /* the FP64 rounding floor of last_relres:      */
+                               /* 8 u || |b| + |A| |p| || / ||b||; a solve     */
+                               /* converges at max(eps, this)                  */
+    double reserved_d[2];
 } impec_stats;
 
 typedef struct impec_ctx impec_ctx;
@@ -156,7 +165,10 @@ int impec_fill_boundaries(impec_ctx *ctx);
 /* EllipticSolver::Solve -> MGSolver::Solve_Elliptic_cc_to_c
  * (EllipticSolver.cpp:133-142, MGSolver.cpp:223-248): assembles the
  * nu-weighted 9-point operator of the ghost-filled c, solves for p (interior)
- * to ||b - A p||_2 <= eps ||b||_2.  its/relres may be NULL. */
+ * to ||b - A p||_2 <= eps ||b||_2, the residual recomputed from p; a residual
+ * below the FP64 rounding floor of its own evaluation (last_relres_floor)
+ * also counts as converged.  IMPEC_ENOCONV otherwise, with p the best
+ * iterate.  its/relres may be NULL. */
 int impec_pressure_solve(impec_ctx *ctx, int *its, double *relres);
 /* EllipticSolver::CalcVelocities (EllipticSolver.cpp:660-683). */
 int impec_velocities(impec_ctx *ctx);

@@ -126,9 +126,9 @@ def test_unstable_long_run_curves(gpu_lib, case):
 
 @pytest.mark.parametrize("case", sorted(CONFIGS))
 def test_shipped_configuration_to_end_time(gpu_lib, case, tmp_path):
-    """The configuration file unchanged (its eps of 1e-14 lies below the FP64
-    rounding floor: the driver keeps the iterate at the floor, as the
-    reference keeps HYPRE's) to its end time 0.4."""
+    """The configuration file unchanged to its end time 0.4 (its eps of 1e-14
+    lies below the FP64 rounding floor of these systems: the solves stop at
+    that floor, which they report)."""
     meta, z = _load(case)
     out = tmp_path / "out"
     log = tmp_path / "log"
