@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libimpec_hip.so")
 
-SOURCES = ["capi.hip", "k_field.hip", "k_pressure.hip", "k_transport.hip", "k_cn.hip", "k_cnn.hip", "ensemble.hip"]
+SOURCES = ["capi.hip", "k_field.hip", "k_pressure.hip", "k_transport.hip", "k_cn.hip", "k_cnn.hip", "ensemble.hip", "noalloc.hip"]
 
 # FP64 paths must round like the reference's x86-64 build: no contraction of
 # a*b+c into fused multiply-adds.  Packed FP32 instructions are disabled.

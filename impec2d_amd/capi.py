@@ -78,6 +78,7 @@ SIGNATURES = {
     "impec_pressure_solve": ([_vp, _ip, _dp], C.c_int),
     "impec_velocities": ([_vp], C.c_int),
     "impec_max_stable_timestep": ([_vp, _dp], C.c_int),
+    "impec_max_stable_timestep_noalloc": ([C.POINTER(Params), _vp, _vp, _vp], C.c_int),
     "impec_source_split": ([_vp, C.c_double], C.c_int),
     "impec_advect_split": ([_vp, C.c_double], C.c_int),
     "impec_advect_x": ([_vp, C.c_double], C.c_int),

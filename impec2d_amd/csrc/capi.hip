@@ -101,7 +101,8 @@ const char *impec_strerror(int code) {
     }
 }
 
-const char *impec_last_error(const impec_ctx *ctx) { return ctx ? ctx->err : "null context"; }
+// NULL: the message of the last context-free call on this thread
+const char *impec_last_error(const impec_ctx *ctx) { return ctx ? ctx->err : noalloc_last_error(); }
 
 int impec_device_count(void) {
     int n = 0;

@@ -218,6 +218,8 @@ struct impec_ensemble {
 namespace impec {
 
 int set_error(impec_ctx *ctx, int code, const char *fmt, ...);
+// the last error of a context-free entry point on this thread (noalloc.hip)
+const char *noalloc_last_error();
 int ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out);
 
 #define IMPEC_HIP_CHECK(ctx, call)                                                            \

@@ -136,6 +136,8 @@ int impec_params_default(impec_params *p);
  * c = p = v = 0, q = 0. */
 int impec_create(const impec_params *p, impec_ctx **out);
 int impec_destroy(impec_ctx *ctx);
+/* the message of the context's last failure; NULL: of the last context-free
+ * call on this thread */
 const char *impec_last_error(const impec_ctx *ctx);
 const char *impec_strerror(int code);
 int impec_synchronize(impec_ctx *ctx);
@@ -174,6 +176,18 @@ int impec_pressure_solve(impec_ctx *ctx, int *its, double *relres);
 int impec_velocities(impec_ctx *ctx);
 /* HyperbolicSolver::MaxStableTimestep (HyperbolicSolver.cpp:93-198). */
 int impec_max_stable_timestep(impec_ctx *ctx, double *dt);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the time step
+ * of impec_max_stable_timestep for face velocities the caller holds on the
+ * device in the host velocity layout above, written to the device word
+ * *dt_dev on `stream` (a hipStream_t; NULL: the null stream) with no host
+ * synchronisation; P gives the grid, domain, cfl and dispersion settings.
+ * Allocates nothing: the reduction's partials live in a static device array
+ * (3 x 2048 doubles per GPU) that every call on a GPU shares, so calls that
+ * overlap on one GPU race.  Velocity arrays up to 2^31 - 1 doubles; larger
+ * ones, bad grids and null pointers give IMPEC_EINVAL with a message in
+ * impec_last_error(NULL). */
+int impec_max_stable_timestep_noalloc(const impec_params *P, const double *v_dev, double *dt_dev, void *stream);
 /* HyperbolicSolver::SourceSplit -> TRIntegrator::Integrate
  * (HyperbolicSolver.cpp:201-205, TRIntegrator.cpp:49-71). */
 int impec_source_split(impec_ctx *ctx, double dt);
