@@ -197,14 +197,22 @@ enum { CN_S = 1, CN_P0 = 2, CN_R = 3 };
 #define IMPEC_CN_PF 2
 #endif
 #ifndef IMPEC_CNP_PF
-#define IMPEC_CNP_PF 2
+#define IMPEC_CNP_PF 3
 #endif
 #ifndef IMPEC_CN_TARGET_WAVES
 #define IMPEC_CN_TARGET_WAVES 4096
 #endif
-// the p-launch and the one-pass start (their own chunks of rows)
+// the p-launch: short chunks (about 14 rows at 4096^2) run fastest; more
+// halo rows are re-read, yet the launch finishes sooner (measured with three
+// rows in flight: 8192 waves 318 us, 12288 313 us, 20480 303 us, 24576
+// 314 us, 32768 330 us)
 #ifndef IMPEC_CNP_TARGET_WAVES
-#define IMPEC_CNP_TARGET_WAVES 8192
+#define IMPEC_CNP_TARGET_WAVES 20480
+#endif
+// the one-pass start (its own chunks of rows; 8192 waves: 207 us against
+// 212 at 12288 and 214 at 20480)
+#ifndef IMPEC_CNS_TARGET_WAVES
+#define IMPEC_CNS_TARGET_WAVES 8192
 #endif
 // build-time tuning: minimum waves per SIMD requested for the p-launch (1 lets
 // the compiler use the registers it wants: 144, three waves per SIMD)
@@ -387,6 +395,8 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
             wprev = wv;
             r0prev = r0v;
             bprev = Pa[u];
+            // (clamping the refill past mend - 1, as the p-launch does, fetches
+            // 3% less here and runs 1-2% slower: measured)
             fetch(u, m + PF);
         }
     }
@@ -460,9 +470,15 @@ __global__ void __launch_bounds__(64 * CN_WAVES, IMPEC_CNP_MIN_WAVES) k_cn_pstep
         Pd[u] = cellf(A.dinv, m);
         Pu[u] = vel(A.u, m);
         Pw[u] = vel(A.v, m);
-        Pp[u] = cell(A.pin, m - 1);
-        if (!A.xzero) Px[u] = cellx(A.x, m - 1);
-        P0[u] = (double)cellf(A.r0, m - 2);
+        // p is used on rows i0-1 .. i1 (the outputs and the matvec's
+        // neighbours), x and r0 on the chunk's own rows: other rows re-read
+        // one of those (a cache hit) instead of fetching one more
+        const int mp = m - 1 < i0 - 1 ? i0 - 1 : (m - 1 > i1 ? i1 : m - 1);
+        const int mx = m - 1 < i0 ? i0 : (m - 1 > i1 - 1 ? i1 - 1 : m - 1);
+        const int m0 = m - 2 < i0 ? i0 : (m - 2 > i1 - 1 ? i1 - 1 : m - 2);
+        Pp[u] = cell(A.pin, mp);
+        if (!A.xzero) Px[u] = cellx(A.x, mx);
+        P0[u] = (double)cellf(A.r0, m0);
     };
     const int mstart = i0 - 2, mend = i1 + 2;
     const int mstop = mstart + (mend - mstart + PF - 1) / PF * PF;
@@ -516,7 +532,9 @@ __global__ void __launch_bounds__(64 * CN_WAVES, IMPEC_CNP_MIN_WAVES) k_cn_pstep
             sprev = sm;
             vprev = Pv[u];
             dprev = dv;
-            fetch(u, m + PF);
+            // rows past mend - 1 are never used: the refill re-reads row
+            // mend - 1 (a cache hit) instead of fetching the next chunk's rows
+            fetch(u, m + PF < mend ? m + PF : mend - 1);
         }
     }
 #pragma unroll
@@ -614,7 +632,9 @@ __global__ void __launch_bounds__(64 * CN_WAVES) k_cn_start(CnArgs A, const doub
 #pragma unroll
             for (int e = 0; e < 9; e++) aprev[e] = a[e];
             bprev = (double)r0q;
-            fetch(u, m + PF);
+            // rows past mend - 1 are never used: the refill re-reads row
+            // mend - 1 (a cache hit) instead of fetching the next chunk's rows
+            fetch(u, m + PF < mend ? m + PF : mend - 1);
         }
     }
 #pragma unroll
@@ -887,12 +907,22 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     if (zero) {
         // x0 = 0: r = r0 = p = b, none of them formed; one pass writes b, 1/D
         // and the first direction's v, and the first p-launch writes x
+        // (the start's own chunks of rows: it runs fastest with fewer, longer
+        // chunks than the p-launch)
         CnArgs az = a2;
         az.out = vb[1];
+        {
+            long R = ((long)N1 * az.nstrip + IMPEC_CNS_TARGET_WAVES - 1) / IMPEC_CNS_TARGET_WAVES;
+            if (R < 8) R = 8;
+            if (R > N1) R = N1;
+            az.R = (int)R;
+        }
+        az.nwave = az.nstrip * ((N1 + az.R - 1) / az.R);
         prof_begin(ctx);
-        hipLaunchKernelGGL(k_cn_start, pg, sblk, 0, s, az, ctx->c, ctx->lds, b, ctx->cn_dg, r0);
+        hipLaunchKernelGGL(k_cn_start, dim3((az.nwave + CN_WAVES - 1) / CN_WAVES), sblk, 0, s, az, ctx->c, ctx->lds, b,
+                           ctx->cn_dg, r0);
         prof_end(ctx, "cn_start");
-        hipLaunchKernelGGL(k_bicg_finish<3>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a2.nwave, 5, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<3>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, az.nwave, 5, ctx->scal);
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = fetch(ctx))) return st;
         bb = ctx->h_scal[S_BB];

@@ -1356,8 +1356,9 @@ __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int w
                 }
             }
             // refill the slot once its registers are dead (an earlier refill
-            // would copy them out and wait for the load that filled them)
-            fetch(u, k + SW_PF);
+            // would copy them out and wait for the load that filled them);
+            // steps past kend re-read the inputs of step kend (cache hits)
+            fetch(u, k + SW_PF <= kend ? k + SW_PF : kend);
         }
     }
 }
@@ -1491,7 +1492,7 @@ __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid
                     if (L.v1) { acc0 += double(X[1][1]); acc1 += double(B[0][1]) * double(X[1][1]); }
                 }
             }
-            fetch(u, k + SW_PF);   // the slot's registers are dead here
+            fetch(u, k + SW_PF <= kend ? k + SW_PF : kend);   // the slot's registers are dead here; past kend: re-reads
         }
     }
     if (FINE0) {
@@ -1898,8 +1899,8 @@ __device__ __forceinline__ void dir_mv_body(const DirMvArgs<T, DT> &A, const DT 
         for (int u = 0; u < DM_PF; u++) {
             const int i = kb + u;
             const double dnext = dn(i + 1, RX[u], RD[u]);
-            {   // refill the slot with row i + 1 + DM_PF
-                const long o = row_of(i + 1 + DM_PF);
+            {   // refill the slot with row i + 1 + DM_PF (rows past i1 are never used: re-read row i1)
+                const long o = row_of(i + 1 + DM_PF <= i1 ? i + 1 + DM_PF : i1);
                 RX[u] = (double)x0[o];
                 RD[u] = first ? 0. : dold[o];
             }

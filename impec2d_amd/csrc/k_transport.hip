@@ -359,7 +359,7 @@ __device__ __forceinline__ void advect_parallel_body(const ParArgsT<T> &A, const
             ukm1 = uk;
             vkm2 = vkm1;
             vkm1 = vk;
-            fetch(s_, k + PF);
+            fetch(s_, k + PF < kend ? k + PF : kend - 1);   // past kend - 1: a re-read (cache hit)
         }
     }
 }
