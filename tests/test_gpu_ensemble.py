@@ -130,6 +130,39 @@ def test_large_member_equal_single_context_bitwise(gpu_lib):
             assert np.array_equal(c_e, c_s)
 
 
+def test_members_with_eps_below_rounding_floor(gpu_lib):
+    """eps = 1e-15 lies below the FP64 rounding floor of every member's
+    system: a member's recursion can stall short of it.  The device-side loop
+    sends a stalled member to the residual recheck and its restart rules, and
+    the member stops at its floor (or the host finishes it with the member's
+    own solve).  Every solve ends at max(eps, floor) within a bounded number
+    of iterations, and the fields agree with the same problem stepped alone to
+    what two solves stopped at the rounding floor can share (the two loops
+    take their restarts at different iterations there, so bitwise equality is
+    not expected)."""
+    steps = 3
+    _say("ensemble with eps 1e-15 (below the rounding floor)")
+    specs = MEMBERS[:4]
+    ens, Ks = _make_ensemble(specs, eps=1e-15)
+    res = _ensemble_steps(ens, steps)
+    worst = 0
+    for k in range(len(specs)):
+        st = ens.members[k].stats()
+        assert st.last_relres <= max(1e-15, st.last_relres_floor), (k, st.last_relres, st.last_relres_floor)
+    ens.close()
+    for k, (p, K) in enumerate(Ks):
+        _, _, single = _run_single(p, K, steps, eps=1e-15)
+        for s in range(steps):
+            its_e = res[s][1][k]
+            c_e, p_e, v_e = res[s][2][k]
+            _, its_s, c_s, p_s, v_s = single[s]
+            ep, ec = rel_err(p_e, p_s), rel_err(c_e, c_s)
+            _say("  member %d step %d: its %d/%d, p %.2e c %.2e" % (k, s, its_e, its_s, ep, ec))
+            worst = max(worst, its_e)
+            assert ep < 1e-8 and ec < 1e-8
+    assert worst <= 200   # no run to the iteration limit
+
+
 def test_graph_replays_match_eager_launches(gpu_lib):
     """The same ensemble stepped with eager launches, with graphs of one CG
     iteration per replay and with graphs of four: identical fields after
