@@ -186,6 +186,53 @@ def rotated(capi, n, warmup=3, steps=10):
             "pressure_relres_last": s1.last_relres, "parabolic_relres_last": s1.last_parabolic_relres}
 
 
+def fingered_front(n, radius=0.4, amp=0.05, width=0.03, modes=7):
+    """A developed displacement front on the unit square: c = 1 behind a
+    quarter circle of the given radius around the injector at (0, 0), with
+    `modes` fingers of amplitude `amp`, smeared over `width` (ghosted array)."""
+    import numpy as np
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    r, th = np.hypot(X, Y), np.arctan2(Y, X + 1e-300)
+    c = 0.5 * (1.0 - np.tanh((r - radius - amp * np.sin(modes * th)) / width))
+    out = np.zeros((n + 4, n + 4))
+    out[2:-2, 2:-2] = c
+    return out
+
+
+def late_time(capi, n, steps=10, warmup=3):
+    """The headline workload at a developed front.  At n^2 = 4096^2 its first
+    2000 steps reach t = 1.5e-4 and sweep 0.01% of the cells, so the late
+    state is set directly: a fingered front at radius 0.4 (about 12% of the
+    domain swept), then the same timing as the headline line, with the solver
+    iteration counts against those of the first steps."""
+    import numpy as np
+    ctx = capi.Context(workload_params(capi, n))
+    ctx.set_source(one_cell_source(n))
+    ctx.set_concentration(fingered_front(n))
+    t = 0.0
+    for _ in range(warmup):
+        t += ctx.step(t)
+    ctx.synchronize()
+    s0 = ctx.stats()
+    pi = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t += ctx.step(t)
+        pi.append(ctx.stats().last_parabolic_its)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    s1 = ctx.stats()
+    swept = float(np.mean(ctx.get_concentration()[2:-2, 2:-2] > 0.5))
+    ctx.close()
+    return {"workload": "the headline workload (quarter-five-spot %d^2, M=10, Peaceman CN dispersion, FP64, zero "
+                        "guesses) from a fingered front at radius 0.4 (7 fingers, amplitude 0.05, width 0.03)" % n,
+            "swept_fraction": swept, "ms_per_step": 1e3 * el / steps, "cell_updates_per_s": n * n * steps / el,
+            "vcycles_per_step": (s1.total_its - s0.total_its) / steps,
+            "parabolic_its_per_solve": float(np.mean(pi)),
+            "pressure_relres_last": s1.last_relres, "parabolic_relres_last": s1.last_parabolic_relres}
+
+
 def corner_wells_source(n, I=1.0, extent=2 ** 0.5):
     """Reference source_type "parallel" on the rotated square: injectors at
     cells (0,0) and (N-1,N-1), producers at (0,N-1) and (N-1,0), rate I each
@@ -344,6 +391,8 @@ def main():
                     help="skip the plain / mixed / warm-start comparison on the lognormal-K workload")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-rotated", action="store_true", help="skip the rotated-grid line")
+    ap.add_argument("--no-late-time", action="store_true",
+                    help="skip the late-time line (the headline workload from a developed, fingered front)")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched-ensemble workload (256 realisations of mixed sizes)")
     ap.add_argument("--ensemble-members", type=int, default=256)
@@ -414,6 +463,8 @@ def main():
 
     if not args.no_rotated:
         extra("rotated_grid", lambda: rotated(capi, n))
+    if not args.no_late_time:
+        extra("late_time", lambda: late_time(capi, n))
     if not args.no_modes:
         extra("modes", lambda: modes(capi, n))
     if not args.no_ensemble:
