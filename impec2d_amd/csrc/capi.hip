@@ -4,6 +4,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include <vector>
 
 namespace impec {
@@ -217,8 +218,16 @@ int impec::ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out) {
     if (!st) st = dalloc(ctx, &ctx->scal, S_COUNT);
     if (!st) st = dalloc(ctx, &ctx->part, (size_t)RED_BLOCKS * PART_VECS);
     if (!st) st = dalloc(ctx, &ctx->part_mid, (size_t)FIN_THREADS * PART_VECS);
-    if (!st && hipHostMalloc((void **)&ctx->h_scal, sizeof(double) * S_COUNT, hipHostMallocDefault) != hipSuccess)
+    // mapped, coherent host memory: a kernel writes the scalars and then a
+    // sequence number (slot S_COUNT) into it, which the host polls
+    if (!st && hipHostMalloc((void **)&ctx->h_scal, sizeof(double) * (S_COUNT + 1),
+                             hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
         st = set_error(ctx, IMPEC_ENOMEM, "pinned host scalars");
+    if (!st) {
+        memset(ctx->h_scal, 0, sizeof(double) * (S_COUNT + 1));
+        if (hipHostGetDevicePointer((void **)&ctx->d_hscal, ctx->h_scal, 0) != hipSuccess)
+            st = set_error(ctx, IMPEC_EHIP, "device address of the host scalars");
+    }
     if (!st) st = pressure_setup_levels(ctx);
     if (!st && P->initial_guess == IMPEC_GUESS_EXTRAPOLATE) {   // the two previous pressures
         st = dalloc(ctx, &ctx->p_prev, ns);
@@ -534,9 +543,7 @@ int impec_minmax_concentration(impec_ctx *ctx, double *cmax, double *cmin) {
 
 int impec_get_stats(impec_ctx *ctx, impec_stats *st) {
     if (!ctx || !st) return IMPEC_EINVAL;
-    IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost,
-                                        ctx->stream));
-    IMPEC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (int e = sync_scalars(ctx)) return e;
     ctx->stats.total_mass_outflux = ctx->h_scal[S_OUTFLUX];
     *st = ctx->stats;
     return IMPEC_OK;

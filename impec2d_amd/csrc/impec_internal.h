@@ -149,7 +149,9 @@ struct impec_ctx {
     double *scal;              // device scalars (Scal)
     double *part;              // reduction partials
     double *part_mid;          // per-thread shares of a two-stage finish (FIN_THREADS x PART_VECS)
-    double *h_scal;            // pinned host mirror of scal
+    double *h_scal;            // pinned host mirror of scal (mapped: kernels publish into it)
+    double *d_hscal;           // its device address
+    unsigned long long pub_seq;   // sequence number of the last publication (slot S_COUNT of h_scal)
     int symmetric;             // shift formulation usable
     int force_unfused;         // use the unfused multigrid kernels (testing)
     int pin_i, pin_j;
@@ -247,6 +249,9 @@ int launch_velocity(impec_ctx *ctx);
 int launch_cfl(impec_ctx *ctx, double *dt);
 int launch_tr(impec_ctx *ctx, double dt);
 int launch_diagnostics(impec_ctx *ctx, int which);  // results in h_scal after sync
+// h_scal <- scal once the stream's work so far has finished (a publishing
+// kernel and a host poll, with a stream synchronisation as the fallback)
+int sync_scalars(impec_ctx *ctx);
 int launch_minmax(impec_ctx *ctx, double *cmax, double *cmin);
 
 // ---- pressure (k_pressure.hip) ----

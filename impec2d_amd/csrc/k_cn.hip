@@ -843,12 +843,7 @@ static int tensors_and_rhs(impec_ctx *ctx, const CnConst &k, int *nblocks = null
     return IMPEC_OK;
 }
 
-static int fetch(impec_ctx *ctx) {
-    IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost,
-                                        ctx->stream));
-    IMPEC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    return IMPEC_OK;
-}
+static int fetch(impec_ctx *ctx) { return sync_scalars(ctx); }
 
 // Solve_Parabolic_cc_to_c: x0 = 0 (AssignRHS_Parabolic, MGSolver.cpp:1375-1392)
 // or, with cn_guess = 1, x0 = the current c; stop on

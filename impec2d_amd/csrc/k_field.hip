@@ -1,6 +1,7 @@
 // k_field.hip -- ghost fills, mobility, Darcy face velocities, CFL reduction,
 // trapezoidal-rule well sources and diagnostics reductions (FP64).
 #include "impec_device.h"
+#include <chrono>
 #include <stdio.h>
 #include <stdarg.h>
 #include <vector>
@@ -363,6 +364,37 @@ __global__ void k_cfl(CflConst k, const double *__restrict__ u, const double *__
     cfl_body(HwGrid{}, k, u, v, part);
 }
 
+// The scalars into the mapped host mirror, then the sequence number: every
+// thread makes its stores visible system-wide before the barrier, and one
+// thread then releases the number the host waits for.
+__global__ void k_publish(const double *__restrict__ S, double *__restrict__ h, unsigned long long seq) {
+    for (int k = threadIdx.x; k < S_COUNT; k += blockDim.x) h[k] = S[k];
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store((unsigned long long *)(h + S_COUNT), seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+int sync_scalars(impec_ctx *ctx) {
+    const unsigned long long seq = ++ctx->pub_seq;
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, ctx->stream, ctx->scal, ctx->d_hscal, seq);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    // the host polls the sequence number (a few microseconds sooner than a
+    // copy and a stream synchronisation); past 50 ms it leaves the wait to
+    // the runtime, which also reports an earlier failure of the stream
+    unsigned long long *flag = (unsigned long long *)(ctx->h_scal + S_COUNT);
+    const auto t0 = std::chrono::steady_clock::now();
+    while (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq) {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+            IMPEC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+            if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq)
+                return set_error(ctx, IMPEC_EHIP, "scalars of the stream were not published");
+            break;
+        }
+    }
+    return IMPEC_OK;
+}
+
 int launch_cfl(impec_ctx *ctx, double *dt_out) {
     CflConst k;
     k.N1 = ctx->N1; k.N2 = ctx->N2; k.B = ctx->B; k.ld = ctx->ldv;
@@ -382,9 +414,7 @@ int launch_cfl(impec_ctx *ctx, double *dt_out) {
     }
     prof_end(ctx, "cfl");
     if (st) return st;
-    IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost,
-                                        ctx->stream));
-    IMPEC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if ((st = sync_scalars(ctx))) return st;
     const double *S = ctx->h_scal;
     double dt;
     if (ctx->P.is_test) {
@@ -502,10 +532,7 @@ int launch_diagnostics(impec_ctx *ctx, int which) {
     prof_end(ctx, "diagnostics");
     if (st) return st;
     // S_MASS, S_MASS+1 (div sum), S_MASS+2 (noflow sum)
-    IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost,
-                                        ctx->stream));
-    IMPEC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    return IMPEC_OK;
+    return sync_scalars(ctx);
 }
 
 __global__ void k_minmax(const double *__restrict__ c, long long n, double *__restrict__ part) {
@@ -554,9 +581,7 @@ int launch_minmax(impec_ctx *ctx, double *cmax, double *cmin) {
     hipLaunchKernelGGL(k_finish<RED_MIN>, dim3(1), dim3(256), 0, ctx->stream, ctx->part + RED_BLOCKS, nb, 1,
                        ctx->scal, (int)S_CMIN);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
-    IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost,
-                                        ctx->stream));
-    IMPEC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (int e = sync_scalars(ctx)) return e;
     *cmax = ctx->h_scal[S_CMAX];
     *cmin = ctx->h_scal[S_CMIN];
     return IMPEC_OK;

@@ -987,12 +987,7 @@ static int vcycle(impec_ctx *ctx) {
     return IMPEC_OK;
 }
 
-static int fetch_scal(impec_ctx *ctx) {
-    IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_scal, ctx->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost,
-                                        ctx->stream));
-    IMPEC_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    return IMPEC_OK;
-}
+static int fetch_scal(impec_ctx *ctx) { return sync_scalars(ctx); }
 
 template <class G>
 __device__ __forceinline__ void zero_interior_body(const G &gr_, CgArgs c, double *__restrict__ p) {
