@@ -16,7 +16,7 @@ fill, pressure solve, Darcy velocities, CFL time step, source / dispersion /
 advection / source / dispersion splitting.  value = N^2 * steps / seconds with
 all state resident in HBM.
 
-Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+Prints one JSON line.  --gpus N with N > 1 exits: the path runs on one GPU.
 """
 import argparse
 import json
@@ -404,8 +404,7 @@ def main():
                     help="also run one plain 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs the IMPEC path on one GPU; --gpus %d is not supported "
-              "(see DESIGN.md for how the path would shard)" % args.gpus, file=sys.stderr)
+        print("bench.py: the IMPEC path runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
 
     from impec2d_amd import capi

@@ -5,7 +5,7 @@
 // BF16 weights and activations, FP32 accumulation); the 3->32 input layer and
 // the 32->1 output layer are FP32 dot products.  Weight gradients are summed
 // in fixed order (per-block partials, then one pass per weight), so training
-// is deterministic.  Multigrid-CG stays the judge: the network only supplies
+// is deterministic.  Multigrid-CG decides convergence: the network only supplies
 // the initial guess and every solve still runs to the same tolerance.
 //
 // Tensors are NHWC: [image][row][column][channel].  Convolution:

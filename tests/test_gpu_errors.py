@@ -185,7 +185,7 @@ def test_garbage_network_still_reaches_tolerance(gpu_lib, layers):
     """A learned guess from a network whose weights are scaled by 1e6 (with 8
     layers the BF16 activations overflow and the increment is not finite): the
     minimal-residual step lengths discard the increment and the MG-CG solve
-    still reaches the tolerance (plain multigrid is the fallback and the judge)."""
+    still reaches the tolerance (plain multigrid is the fallback and decides convergence)."""
     from impec2d_amd import capi
     _say("learned guess: garbage network, layers", layers)
     n = 64
