@@ -449,6 +449,7 @@ def main():
         from impec2d_amd import roofline
         out["roofline"] = roofline.measure(ctx, n, t, fp32_cycle=args.precision in ("mixed", "mixed32"))
         out["roofline_by_kernel"] = roofline.measure.last_table
+        out["step_roofline"] = roofline.step_summary(n, out["ms_per_step"])
     ctx.close()
 
     # the secondary workloads: a failure is reported in the line, never loses it

@@ -42,6 +42,17 @@ KERNELS = {
     # MUSCL-Hancock x / y sweep: read c and one velocity component, write c
     "sweep_x": dict(rocprof="k_sweep_x", bytes_per_cell=24),
     "sweep_y": dict(rocprof="k_sweep_y", bytes_per_cell=24),
+    # the smaller passes of a step (counted in the whole-step figure):
+    # dispersion: the true-residual pass (read x, b, u, v) and the last
+    # iteration's x update (read x, p, r, v and the FP32 1/D, write x)
+    "cn_check": dict(rocprof="k_cn_stream<3>", bytes_per_cell=32),
+    "cn_xfinal": dict(rocprof="k_cn_xfinal", bytes_per_cell=44),
+    # velocities (read p and w, write u and v), CFL (read u, v), mobility
+    # (read c, write w), the zero-guess CG start (read q, write p and r)
+    "velocity": dict(rocprof="k_velocity", bytes_per_cell=32),
+    "cfl": dict(rocprof="k_cfl", bytes_per_cell=16),
+    "mobility": dict(rocprof="k_mobility", bytes_per_cell=16),
+    "cg_init": dict(rocprof="k_cg_init_zero", bytes_per_cell=24),
 }
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -57,6 +68,23 @@ def committed_traffic(name, n):
     with open(path) as f:
         m = re.search(r"^TRAFFIC_BYTES_PER_LAUNCH\s+([0-9.eE+]+)", f.read(), re.M)
     return float(m.group(1)) if m else None
+
+
+def step_summary(n, ms_per_step):
+    """The whole step against the HBM roofline: the algorithmic bytes of every
+    finest-level pass measured by the last measure() call (launches per step x
+    bytes per cell x n^2) over the step time.  The coarse multigrid levels, the
+    reductions' finishes, ghost fills and source cells are not counted (their
+    bytes are a few percent of the step's), so this is a lower bound."""
+    table, steps = getattr(measure, "last_table", None), getattr(measure, "last_steps", 0)
+    if not table or not steps or not ms_per_step:
+        return None
+    b = sum(v["launches"] * v["bytes_per_cell"] for v in table.values()) / steps * float(n) * n
+    kms = sum(v["launches"] * v["avg_us"] for v in table.values()) / steps / 1e3
+    gbs = b / (ms_per_step * 1e-3) / 1e9
+    return {"algorithmic_bytes_per_step": b, "bytes_per_cell": b / (float(n) * n), "achieved": gbs,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "counted_kernel_ms_per_step": kms, "ms_per_step": ms_per_step}
 
 
 def measure(ctx, n, t, steps=2, fp32_cycle=False):
@@ -84,6 +112,7 @@ def measure(ctx, n, t, steps=2, fp32_cycle=False):
         if best is None or ms > best[2]:
             best = (name, launches, ms)
     measure.last_table = table
+    measure.last_steps = steps
     if best is None:
         return None
     name, launches, ms = best
