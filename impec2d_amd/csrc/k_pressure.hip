@@ -1193,13 +1193,23 @@ __device__ __forceinline__ void mask_pair(const Lane &L, int m, int n1, V fill, 
     b = (rin && L.v1) ? b : fill;
 }
 
-// prefetch distance in rows (one slot per unrolled step); build-time tunable
+// prefetch distance in rows (one slot per unrolled step), per arithmetic
+// type; build-time tunable.  The FP64 legs run at 3 waves per SIMD with 2 rows
+// in flight (4 rows take them to 203 VGPRs and 2 waves, 10% slower); the FP32
+// legs are fastest with 4.
 #ifndef IMPEC_SW_PF
-#define IMPEC_SW_PF 4
+#define IMPEC_SW_PF 2
 #endif
-constexpr int SW_PF = IMPEC_SW_PF;
-static_assert(SW_PF % 2 == 0, "the unrolled steps take the row parity (colour, prolongation neighbour) from the slot "
-                              "index, so the prefetch depth must be even");
+#ifndef IMPEC_SW_PF_F32
+#define IMPEC_SW_PF_F32 4
+#endif
+static_assert(IMPEC_SW_PF % 2 == 0 && IMPEC_SW_PF_F32 % 2 == 0,
+              "the unrolled steps take the row parity (colour, prolongation neighbour) from the slot index, so the "
+              "prefetch depth must be even");
+template <typename T>
+struct SwPf {
+    static constexpr int value = std::is_same<T, float>::value ? IMPEC_SW_PF_F32 : IMPEC_SW_PF;
+};
 
 template <bool FINE0, bool ANISO, typename T, int NS = 1, bool R32 = false>
 __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int wid) {
@@ -1209,6 +1219,7 @@ __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int w
     constexpr int NX = 2 + 2 * NS;   // x and W-face window: rows k-NX+1 .. k
     constexpr int NF = 1 + 2 * NS;   // face and rhs window: rows k-NF+1 .. k
     constexpr int LAG = 2 * NS;      // residual and output row k - LAG
+    constexpr int SW_PF = SwPf<T>::value;
     Lane L;
     Chunk K;
     if (!wave_setup<NS>(A, wid, L, K)) return;
@@ -1372,6 +1383,7 @@ __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid
     constexpr int NX = 2 + 2 * NS;
     constexpr int NF = 1 + 2 * NS;
     constexpr int LAG = 2 * NS;
+    constexpr int SW_PF = SwPf<T>::value;
     Lane L;
     Chunk K;
     if (!wave_setup<NS>(A, wid, L, K)) return;
@@ -1821,6 +1833,12 @@ constexpr int DM_OUT = 62, DM_PF = 4;
 #ifndef IMPEC_DM_WAVES
 #define IMPEC_DM_WAVES 4096
 #endif
+// the FP32-vector CG's pass (half the bytes per cell): shorter chunks, 6144
+// waves (2.8% of a step of that mode faster than 4096; the FP64 pass loses 1%
+// with them)
+#ifndef IMPEC_DM_WAVES_F32
+#define IMPEC_DM_WAVES_F32 6144
+#endif
 
 // DT: storage of d and q, double for the FP64 CG, float for the FP32-vector CG
 // (whose matvec reads FP32 copies of the face mobilities, fx32 / fy32)
@@ -2027,21 +2045,26 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
     cg_update2_body(HwGrid{}, c, p, d, r, qv, S, part);
 }
 
+// the finest level's target in the FP64 cycle (3072 waves, 3 per SIMD: 118 and
+// 149 us per leg at 4096^2 against 131 and 160 at 2048)
 #ifndef IMPEC_MG_WAVES
-#define IMPEC_MG_WAVES 2048
+#define IMPEC_MG_WAVES 3072
 #endif
 // the finest level's target in the FP32 cycle (its legs are compute-bound and
 // gain from more waves; the FP64 legs and the small levels lose from shorter
 // chunks)
 #ifndef IMPEC_MG_WAVES0_F32
-#define IMPEC_MG_WAVES0_F32 4096
+#define IMPEC_MG_WAVES0_F32 6144
 #endif
-// the target of the FP64 cycle's coarser levels (level 1 and below): longer
-// chunks there (1024 waves at 2048^2 and 1024^2) run those legs 8-16% faster
-// than 2048 waves, 512 waves slower; the FP32 cycle's coarse levels lose about
-// 1% of a mixed step with them and keep IMPEC_MG_WAVES
+// the target of the FP64 cycle's coarser levels (level 1 and below): with two
+// rows in flight and 3 waves per SIMD, 2048 waves (1.4% of a step faster
+// than 1024; with four rows in flight 1024 had been the faster)
 #ifndef IMPEC_MG_WAVES_COARSE
-#define IMPEC_MG_WAVES_COARSE 1024
+#define IMPEC_MG_WAVES_COARSE 2048
+#endif
+// the FP32 cycle's coarser levels
+#ifndef IMPEC_MG_WAVES_COARSE_F32
+#define IMPEC_MG_WAVES_COARSE_F32 2048
 #endif
 // rows per chunk of the streaming legs: about IMPEC_MG_WAVES wavefronts on large
 // levels, short chunks on small ones; even, since the march starts on an
@@ -2052,7 +2075,7 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
 static_assert(IMPEC_MG_MINROWS % 2 == 0 && IMPEC_MG_MINROWS >= 2, "chunks of the streaming legs start on even rows");
 static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip, bool f32) {
     if (l == 0 && ctx->P.mg_rows > 0) return (ctx->P.mg_rows + 1) & ~1;
-    const long target = f32 ? (l == 0 ? IMPEC_MG_WAVES0_F32 : IMPEC_MG_WAVES)
+    const long target = f32 ? (l == 0 ? IMPEC_MG_WAVES0_F32 : IMPEC_MG_WAVES_COARSE_F32)
                             : (l == 0 ? IMPEC_MG_WAVES : IMPEC_MG_WAVES_COARSE);
     long r = ((long)n1 * nstrip + target - 1) / target;
     r = (r + 1) & ~1L;
@@ -2344,7 +2367,7 @@ static int cg_iteration_unfused(impec_ctx *ctx, const LevArgs &a0, const CgArgs 
 }
 
 template <typename T>
-static DirMvArgs<T> dir_mv_args(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c) {
+static DirMvArgs<T> dir_mv_args(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, long waves = IMPEC_DM_WAVES) {
     DirMvArgs<T> A;
     memset(&A, 0, sizeof(A));
     A.L = a0;
@@ -2356,7 +2379,7 @@ static DirMvArgs<T> dir_mv_args(impec_ctx *ctx, const LevArgs &a0, const CgArgs 
     A.S = ctx->scal;
     A.part = ctx->part;
     A.nstrip = (c.n2 + DM_OUT - 1) / DM_OUT;
-    long R = ((long)c.n1 * A.nstrip + IMPEC_DM_WAVES - 1) / IMPEC_DM_WAVES;
+    long R = ((long)c.n1 * A.nstrip + waves - 1) / waves;
     if (R < 8) R = 8;
     A.R = (int)R;
     A.nwave = A.nstrip * ((c.n1 + A.R - 1) / A.R);
@@ -2408,7 +2431,7 @@ static int cg_iteration_mixed32(impec_ctx *ctx, const LevArgs &a0, const CgArgs 
     int st;
     if ((st = vcycle_fused<float>(ctx, true))) return st;
     hipLaunchKernelGGL(k_finish_rho, dim3(1), dim3(1024), 0, s, ctx->part, nwave0, inv_n, ctx->scal);
-    const DirMvArgs<float> d64 = dir_mv_args<float>(ctx, a0, c);
+    const DirMvArgs<float> d64 = dir_mv_args<float>(ctx, a0, c, IMPEC_DM_WAVES_F32);
     DirMvArgs<float, float> dm;
     memset(&dm, 0, sizeof(dm));
     dm.L = d64.L; dm.c = d64.c; dm.x0 = d64.x0;
