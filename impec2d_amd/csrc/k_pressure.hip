@@ -1829,7 +1829,10 @@ static Grid2 vec_grid(int n1, int n2) {
 // lanes by DPP wave shifts and the inputs of the next DM_PF rows are in
 // flight (branch-free clamped loads).  Each cell's x0 and d_old are read once
 // instead of five times.  Per-wave partials of d.q.
-constexpr int DM_OUT = 62, DM_PF = 4;
+#ifndef IMPEC_DM_PF
+#define IMPEC_DM_PF 4
+#endif
+constexpr int DM_OUT = 62, DM_PF = IMPEC_DM_PF;
 #ifndef IMPEC_DM_WAVES
 #define IMPEC_DM_WAVES 4096
 #endif
