@@ -190,7 +190,15 @@ __global__ void k_cn_rhs(CnConst k, const double *__restrict__ c, int lds, const
 // the convergence test.
 // ---------------------------------------------------------------------------
 constexpr int S_RHO_NEW = 13;
-constexpr int CN_OUT = 62, CNP_OUT = 60, CN_WAVES = 4;
+#ifndef IMPEC_CN_WAVES
+#define IMPEC_CN_WAVES 4
+#endif
+// the s-launch (k_cn_stream) in workgroups of 8 waves: 181 us against 184
+// with 4 at 4096^2; the p-launch and the start lose 4-7% with 8 and keep 4
+#ifndef IMPEC_CNS_WAVES
+#define IMPEC_CNS_WAVES 8
+#endif
+constexpr int CN_OUT = 62, CNP_OUT = 60, CN_WAVES = IMPEC_CN_WAVES, CNS_WAVES = IMPEC_CNS_WAVES;
 enum { CN_S = 1, CN_P0 = 2, CN_R = 3 };
 // build-time tuning: rows in flight per wavefront, wavefronts per launch
 #ifndef IMPEC_CN_PF
@@ -289,9 +297,9 @@ struct FaceStream {
 // v = A (p / D)) and CN_R (partials of ||r - A x||^2 with r = b; nothing
 // written); lanes 1..62 of a strip are written
 template <int MODE>
-__global__ void __launch_bounds__(64 * CN_WAVES) k_cn_stream(CnArgs A) {
+__global__ void __launch_bounds__(64 * CNS_WAVES) k_cn_stream(CnArgs A) {
     const int lane = threadIdx.x & 63;
-    const int wid = xcd_block(blockIdx.x, gridDim.x) * CN_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wid = xcd_block(blockIdx.x, gridDim.x) * CNS_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wid >= A.nwave) return;
     const CnConst &k = A.k;
     const int N1 = k.N1, N2 = k.N2;
@@ -892,7 +900,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     a2.nwave = a2.nstrip * ((N1 + a2.R - 1) / a2.R);
     if (a.nwave > RED_BLOCKS || a2.nwave > RED_BLOCKS)
         return set_error(ctx, IMPEC_EINVAL, "dispersion solve: grid too large");
-    const dim3 sg((a.nwave + CN_WAVES - 1) / CN_WAVES), sblk(64 * CN_WAVES);
+    const dim3 sg((a.nwave + CNS_WAVES - 1) / CNS_WAVES), ssblk(64 * CNS_WAVES), sblk(64 * CN_WAVES);
     const dim3 pg((a2.nwave + CN_WAVES - 1) / CN_WAVES);
     int cur = 0, its = 0, status = IMPEC_OK, st;
     // the first p-launch's p: b for a zero guess, whose first direction is
@@ -944,7 +952,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             CnArgs ap = a;
             ap.pout = pb[1]; ap.out = vb[1];
             prof_begin(ctx);
-            hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, sblk, 0, s, ap);
+            hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, ssblk, 0, s, ap);
             prof_end(ctx, "cn_apply_p0");
             hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 1, ctx->scal);
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
@@ -968,7 +976,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             as.r = rcur;
             as.vin = vb[cur];
             prof_begin(ctx);
-            hipLaunchKernelGGL(k_cn_stream<CN_S>, sg, sblk, 0, s, as);
+            hipLaunchKernelGGL(k_cn_stream<CN_S>, sg, ssblk, 0, s, as);
             prof_end(ctx, "cn_apply_s");
             hipLaunchKernelGGL(k_bicg_finish<5>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 2,
                                ctx->scal);
@@ -1023,7 +1031,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             CnArgs ar = a;
             ar.r = b;
             ar.x = x;
-            hipLaunchKernelGGL(k_cn_stream<CN_R>, sg, sblk, 0, s, ar);
+            hipLaunchKernelGGL(k_cn_stream<CN_R>, sg, ssblk, 0, s, ar);
         }
         hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 6, ctx->scal);
         prof_end(ctx, "cn_check");
@@ -1043,7 +1051,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         CnArgs ap = a;
         ap.r = r;
         ap.pout = pb[1]; ap.out = vb[1];
-        hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, sblk, 0, s, ap);
+        hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, ssblk, 0, s, ap);
         hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 1, ctx->scal);
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = fetch(ctx))) return st;

@@ -1023,7 +1023,10 @@ __global__ void k_set_pin(CgArgs c, double *__restrict__ p) {
 // with the boundary faces dropped (the Neumann folding), and the diagonal is
 // minus the sum of the four faces.
 // ===========================================================================
-constexpr int SW_WAVES = 4;
+#ifndef IMPEC_SW_WAVES
+#define IMPEC_SW_WAVES 4
+#endif
+constexpr int SW_WAVES = IMPEC_SW_WAVES;
 // A strip is 128 columns, two per lane.  Each Gauss-Seidel sweep of a leg
 // spoils one more column from the strip's edges (the neighbours beyond them
 // are not in the wave), so a leg with NS red-black pairs keeps NS + 1 lanes
