@@ -349,57 +349,101 @@ __global__ void k_coarsen_w(const double *__restrict__ wf, int ldf, int n1, int 
     coarsen_w_body(HwGrid{}, wf, ldf, n1, n2, wc, ldc, m1, m2, wcf);
 }
 
-// coarsest level (at most 2 x 2 cells): exact solve of the singular Neumann
-// system bordered by the zero-mean condition, [A 1; 1^T 0][x; l] = [b; 0],
-// by Gaussian elimination with partial pivoting on one thread
-// (x, b stored as T; the elimination is FP64 in both precisions)
-template <bool DIAG, typename T = double, class G>
-__device__ __forceinline__ void coarsest_body(const G &gr_, LevArgs L, T *__restrict__ x, const T *__restrict__ b) {
-    if (threadIdx.x != 0 || gr_.bx() != 0) return;
-    const int n1 = L.g.n1, n2 = L.g.n2, n = n1 * n2, m = n + 1;
+// Gaussian elimination with partial pivoting, then back substitution, of a
+// bordered system of n = N cells: M holds the (N+1) x (N+1) matrix and, in
+// column N+1, the right-hand side; sol receives the N+1 unknowns.  The sizes
+// are compile-time constants and every loop is unrolled, so M stays in
+// registers (with a run-time size it lives in scratch memory and each of the
+// elimination's dependent accesses is a memory round trip).  The pivot row is
+// exchanged by selects over the candidate rows, which keeps every index
+// static.  FUSED: the updates rounded once, as fused multiply-adds; otherwise
+// as a rounded product and a rounded difference.
+template <int N, bool FUSED>
+__device__ __forceinline__ void bordered_solve(double (&M)[N + 1][N + 2], double (&sol)[N + 1]) {
+    constexpr int m = N + 1;
+#pragma unroll
+    for (int col = 0; col < m; col++) {
+        int piv = col;
+        double best = fabs(M[col][col]);
+#pragma unroll
+        for (int r = col + 1; r < m; r++) {
+            const double a = fabs(M[r][col]);
+            if (a > best) { piv = r; best = a; }
+        }
+#pragma unroll
+        for (int r = col + 1; r < m; r++) {
+            const bool sw = piv == r;
+#pragma unroll
+            for (int q = 0; q <= m; q++) {
+                const double a = M[col][q], b = M[r][q];
+                M[col][q] = sw ? b : a;
+                M[r][q] = sw ? a : b;
+            }
+        }
+        const double d = M[col][col];
+        if (d == 0.) continue;
+#pragma unroll
+        for (int r = col + 1; r < m; r++) {
+            const double f = M[r][col] / d;
+#pragma unroll
+            for (int q = col; q <= m; q++) M[r][q] = FUSED ? fma(-f, M[col][q], M[r][q]) : M[r][q] - f * M[col][q];
+        }
+    }
+#pragma unroll
+    for (int r = m - 1; r >= 0; r--) {
+        double t = M[r][m];
+#pragma unroll
+        for (int q = r + 1; q < m; q++) t = FUSED ? fma(-M[r][q], sol[q], t) : t - M[r][q] * sol[q];
+        sol[r] = M[r][r] != 0. ? t / M[r][r] : 0.;
+    }
+}
+
+// coarsest level of N1 x N2 <= 2 x 2 cells: exact solve of the singular
+// Neumann system bordered by the zero-mean condition,
+// [A 1; 1^T 0][x; l] = [b; 0], on one thread (x, b stored as T; the
+// elimination is FP64 in both precisions)
+template <int N1, int N2, bool DIAG, typename T>
+__device__ __forceinline__ void coarsest_solve(const LevArgs &L, T *__restrict__ x, const T *__restrict__ b) {
+    constexpr int n = N1 * N2, m = n + 1;
     const int dI[9] = {0, -1, -1, 0, 1, 1, 1, 0, -1}, dJ[9] = {0, 0, -1, -1, -1, 0, 1, 1, 1};
-    double M[5][6];
+    double M[m][m + 1];
+#pragma unroll
     for (int r = 0; r < m; r++)
+#pragma unroll
         for (int q = 0; q <= m; q++) M[r][q] = 0.;
-    for (int i = 0; i < n1; i++)
-        for (int j = 0; j < n2; j++) {
-            const int row = i * n2 + j;
+#pragma unroll
+    for (int i = 0; i < N1; i++)
+#pragma unroll
+        for (int j = 0; j < N2; j++) {
+            const int row = i * N2 + j;
             double a[9];
             op_row<DIAG>(L.g, L.w1, L.w2, L.ldw, i, j, a);
+#pragma unroll
             for (int k = 0; k < 9; k++) {
                 const int ii = i + dI[k], jj = j + dJ[k];
-                if (ii < 0 || jj < 0 || ii >= n1 || jj >= n2) continue;
-                M[row][ii * n2 + jj] += a[k];
+                if (ii < 0 || jj < 0 || ii >= N1 || jj >= N2) continue;
+                M[row][ii * N2 + jj] += a[k];
             }
             M[row][n] = 1.;
             M[n][row] = 1.;
             M[row][m] = (double)b[(long)i * L.ld + j];
         }
-    for (int col = 0; col < m; col++) {
-        int piv = col;
-        for (int r = col + 1; r < m; r++)
-            if (fabs(M[r][col]) > fabs(M[piv][col])) piv = r;
-        if (piv != col)
-            for (int q = 0; q <= m; q++) {
-                const double t = M[col][q];
-                M[col][q] = M[piv][q];
-                M[piv][q] = t;
-            }
-        const double d = M[col][col];
-        if (d == 0.) continue;
-        for (int r = col + 1; r < m; r++) {
-            const double f = M[r][col] / d;
-            for (int q = col; q <= m; q++) M[r][q] -= f * M[col][q];
-        }
-    }
-    double sol[5];
-    for (int r = m - 1; r >= 0; r--) {
-        double t = M[r][m];
-        for (int q = r + 1; q < m; q++) t -= M[r][q] * sol[q];
-        sol[r] = M[r][r] != 0. ? t / M[r][r] : 0.;
-    }
-    for (int i = 0; i < n1; i++)
-        for (int j = 0; j < n2; j++) x[(long)i * L.ld + j] = (T)sol[i * n2 + j];
+    double sol[m];
+    bordered_solve<n, false>(M, sol);
+#pragma unroll
+    for (int i = 0; i < N1; i++)
+#pragma unroll
+        for (int j = 0; j < N2; j++) x[(long)i * L.ld + j] = (T)sol[i * N2 + j];
+}
+template <bool DIAG, typename T = double, class G>
+__device__ __forceinline__ void coarsest_body(const G &gr_, LevArgs L, T *__restrict__ x, const T *__restrict__ b) {
+    if (threadIdx.x != 0 || gr_.bx() != 0) return;
+    const int n1 = L.g.n1, n2 = L.g.n2;
+    // the hierarchy is coarsened until both sides are at most 2
+    if (n1 == 2 && n2 == 2) coarsest_solve<2, 2, DIAG>(L, x, b);
+    else if (n1 == 2 && n2 == 1) coarsest_solve<2, 1, DIAG>(L, x, b);
+    else if (n1 == 1 && n2 == 2) coarsest_solve<1, 2, DIAG>(L, x, b);
+    else if (n1 == 1 && n2 == 1) coarsest_solve<1, 1, DIAG>(L, x, b);
 }
 template <bool DIAG, typename T = double>
 __global__ void k_coarsest(LevArgs L, T *__restrict__ x, const T *__restrict__ b) {
@@ -1586,36 +1630,65 @@ __device__ __forceinline__ T bottom_gs(const T *x, const T *w, T bv, int n1, int
     return Cc != T(0) ? s * recip(Cc) : T(0);
 }
 
+// the bottom's coarsest level of N1 x N2 cells: the bordered solve of the
+// legs' operator (faces from w, boundary faces dropped), w, b and x at the
+// level's place in LDS
+template <int N1, int N2, typename T>
+__device__ __forceinline__ void bottom_coarsest(const T *w, const T *b, T *x, T cx, T cy, int hm) {
+    constexpr int n = N1 * N2, m = n + 1;
+    double M[m][m + 1];
+#pragma unroll
+    for (int r = 0; r < m; r++)
+#pragma unroll
+        for (int q = 0; q <= m; q++) M[r][q] = 0.;
+#pragma unroll
+    for (int i = 0; i < N1; i++)
+#pragma unroll
+        for (int j = 0; j < N2; j++) {
+            const int row = i * N2 + j;
+            T W, E, Sf, N;
+            bottom_faces(w, N1, N2, i, j, cx, cy, hm, W, E, Sf, N);
+            M[row][row] += (double)(-((W + E) + (Sf + N)));
+            if (i > 0) M[row][row - N2] += (double)W;
+            if (i < N1 - 1) M[row][row + N2] += (double)E;
+            if (j > 0) M[row][row - 1] += (double)Sf;
+            if (j < N2 - 1) M[row][row + 1] += (double)N;
+            M[row][n] = 1.;
+            M[n][row] = 1.;
+            M[row][m] = (double)b[row];
+        }
+    double sol[m];
+    bordered_solve<n, true>(M, sol);
+#pragma unroll
+    for (int k = 0; k < n; k++) x[k] = (T)sol[k];
+}
+
+// Level offsets in LDS are running sums (an array of them indexed by the
+// level would live in scratch memory).
 template <typename T>
 __device__ void mg_bottom_body(const BottomArgs<T> &A) {
 #pragma clang fp contract(fast)
     __shared__ T sx[BOTTOM_CELLS], sb[BOTTOM_CELLS], sr[BOTTOM_CELLS], sw[BOTTOM_CELLS];
     const int t = threadIdx.x, nt = blockDim.x;
     const int nl = A.nl;
-    int off[BOTTOM_LEVELS];
-    {
-        int o = 0;
-        for (int l = 0; l < nl; l++) {
-            off[l] = o;
-            o += A.lev[l].n1 * A.lev[l].n2;
-        }
-    }
-    for (int l = 0; l < nl; l++) {
+    for (int l = 0, o = 0; l < nl; l++) {
         const int n1 = A.lev[l].n1, n2 = A.lev[l].n2, ldw = A.lev[l].ldw;
         const T *w = A.lev[l].w;
         for (int id = t; id < n1 * n2; id += nt) {
             const int i = id / n2, j = id - i * n2;
-            sw[off[l] + id] = w[(long)i * ldw + j];
+            sw[o + id] = w[(long)i * ldw + j];
             if (l == 0) sb[id] = A.b[(long)i * A.ldb + j];
         }
+        o += n1 * n2;
     }
     __syncthreads();
     // down: red from zero, black, residual, full weighting
+    int o = 0;   // offset of level l
     for (int l = 0; l + 1 < nl; l++) {
         const int n1 = A.lev[l].n1, n2 = A.lev[l].n2, n = n1 * n2, hm = A.lev[l].harmonic;
         const T cx = T(A.lev[l].cx), cy = T(A.lev[l].cy);
-        T *x = sx + off[l], *r = sr + off[l];
-        const T *b = sb + off[l], *w = sw + off[l];
+        T *x = sx + o, *r = sr + o;
+        const T *b = sb + o, *w = sw + o;
         for (int id = t; id < n; id += nt) {
             const int i = id / n2, j = id - i * n2;
             T v = T(0);
@@ -1655,7 +1728,7 @@ __device__ void mg_bottom_body(const BottomArgs<T> &A) {
         }
         __syncthreads();
         const int m1 = A.lev[l + 1].n1, m2 = A.lev[l + 1].n2;
-        T *bc = sb + off[l + 1];
+        T *bc = sb + o + n;
         for (int id = t; id < m1 * m2; id += nt) {
             const int I = id / m2, J = id - I * m2;
             const T wL = T((2 * J - 1 >= 0 && 2 * J - 1 < n2) ? pw(2 * J - 1, J, m2) : 0.);
@@ -1676,55 +1749,21 @@ __device__ void mg_bottom_body(const BottomArgs<T> &A) {
             bc[id] = T(0.25) * acc;
         }
         __syncthreads();
+        o += n;
     }
-    // coarsest: k_coarsest's bordered solve (FP64 elimination) of the operator
-    // the legs use on that level, assembled from the faces in LDS
+    // coarsest (o is its offset): k_coarsest's bordered solve (FP64
+    // elimination) of the operator the legs use on that level, assembled from
+    // the faces in LDS
     if (t == 0) {
-        const int l = nl - 1, n1 = A.lev[l].n1, n2 = A.lev[l].n2, n = n1 * n2, m = n + 1, hm = A.lev[l].harmonic;
+        const int l = nl - 1, n1 = A.lev[l].n1, n2 = A.lev[l].n2, hm = A.lev[l].harmonic;
         const T cx = T(A.lev[l].cx), cy = T(A.lev[l].cy);
-        const T *b = sb + off[l], *w = sw + off[l];
-        T *x = sx + off[l];
-        double M[5][6];
-        for (int rr = 0; rr < m; rr++)
-            for (int q = 0; q <= m; q++) M[rr][q] = 0.;
-        for (int i = 0; i < n1; i++)
-            for (int j = 0; j < n2; j++) {
-                const int row = i * n2 + j;
-                T W, E, Sf, N;
-                bottom_faces(w, n1, n2, i, j, cx, cy, hm, W, E, Sf, N);
-                M[row][row] += (double)(-((W + E) + (Sf + N)));
-                if (i > 0) M[row][row - n2] += (double)W;
-                if (i < n1 - 1) M[row][row + n2] += (double)E;
-                if (j > 0) M[row][row - 1] += (double)Sf;
-                if (j < n2 - 1) M[row][row + 1] += (double)N;
-                M[row][n] = 1.;
-                M[n][row] = 1.;
-                M[row][m] = (double)b[row];
-            }
-        for (int col = 0; col < m; col++) {
-            int piv = col;
-            for (int rr = col + 1; rr < m; rr++)
-                if (fabs(M[rr][col]) > fabs(M[piv][col])) piv = rr;
-            if (piv != col)
-                for (int q = 0; q <= m; q++) {
-                    const double tt = M[col][q];
-                    M[col][q] = M[piv][q];
-                    M[piv][q] = tt;
-                }
-            const double d = M[col][col];
-            if (d == 0.) continue;
-            for (int rr = col + 1; rr < m; rr++) {
-                const double f = M[rr][col] / d;
-                for (int q = col; q <= m; q++) M[rr][q] -= f * M[col][q];
-            }
-        }
-        double sol[5];
-        for (int rr = m - 1; rr >= 0; rr--) {
-            double tt = M[rr][m];
-            for (int q = rr + 1; q < m; q++) tt -= M[rr][q] * sol[q];
-            sol[rr] = M[rr][rr] != 0. ? tt / M[rr][rr] : 0.;
-        }
-        for (int k = 0; k < n; k++) x[k] = (T)sol[k];
+        const T *b = sb + o, *w = sw + o;
+        T *x = sx + o;
+        // the hierarchy is coarsened until both sides are at most 2
+        if (n1 == 2 && n2 == 2) bottom_coarsest<2, 2>(w, b, x, cx, cy, hm);
+        else if (n1 == 2 && n2 == 1) bottom_coarsest<2, 1>(w, b, x, cx, cy, hm);
+        else if (n1 == 1 && n2 == 2) bottom_coarsest<1, 2>(w, b, x, cx, cy, hm);
+        else if (n1 == 1 && n2 == 1) bottom_coarsest<1, 1>(w, b, x, cx, cy, hm);
     }
     __syncthreads();
     // up: x + P e, black, red
@@ -1732,8 +1771,10 @@ __device__ void mg_bottom_body(const BottomArgs<T> &A) {
         const int n1 = A.lev[l].n1, n2 = A.lev[l].n2, n = n1 * n2, hm = A.lev[l].harmonic;
         const int m1 = A.lev[l + 1].n1, m2 = A.lev[l + 1].n2;
         const T cx = T(A.lev[l].cx), cy = T(A.lev[l].cy);
-        T *x = sx + off[l];
-        const T *xc = sx + off[l + 1], *b = sb + off[l], *w = sw + off[l];
+        const int oc = o;   // offset of level l + 1
+        o -= n;
+        T *x = sx + o;
+        const T *xc = sx + oc, *b = sb + o, *w = sw + o;
         for (int id = t; id < n; id += nt) {
             const int i = id / n2, j = id - i * n2;
             int I0 = i >> 1, I1 = (i & 1) ? I0 + 1 : I0 - 1;
