@@ -1571,8 +1571,8 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_mg_up(FusedArgsT<T> A) {
 // ===========================================================================
 // The bottom of the streaming V-cycle: from the first level of at most
 // BOTTOM_N x BOTTOM_N cells down to the coarsest and back up in one
-// workgroup, with every level's x, b, r and w in LDS.  Per cell it does what
-// the streaming legs do (faces from w by the same expressions, red then black
+// workgroup, with every level's x, b, r, w and faces in LDS.  Per cell it does
+// what the streaming legs do (faces from w by the same expressions, red then black
 // from a zero guess, residual, full weighting; bilinear prolongation, black,
 // red), and on the coarsest level k_coarsest's bordered solve of the legs'
 // operator (faces from w, boundary faces dropped) assembled from LDS;
@@ -1612,24 +1612,6 @@ __device__ __forceinline__ void bottom_faces(const T *w, int n1, int n2, int i, 
     N = j < n2 - 1 ? cy * face_mob_pc(wc, w[i * n2 + j + 1], harmonic) : T(0);
 }
 
-// Gauss-Seidel update of cell (i,j) of colour (i + j) & 1, as gs_cell
-template <typename T>
-__device__ __forceinline__ T bottom_gs(const T *x, const T *w, T bv, int n1, int n2, int i, int j, T cx, T cy,
-                                       int harmonic) {
-#pragma clang fp contract(fast)
-    T W, E, Sf, N;
-    bottom_faces(w, n1, n2, i, j, cx, cy, harmonic, W, E, Sf, N);
-    const T xa = i > 0 ? x[(i - 1) * n2 + j] : T(0), xb = i < n1 - 1 ? x[(i + 1) * n2 + j] : T(0);
-    const T xS = j > 0 ? x[i * n2 + j - 1] : T(0), xN = j < n2 - 1 ? x[i * n2 + j + 1] : T(0);
-    const T Cc = -((W + E) + (Sf + N));
-    T s = bv;
-    s -= W * xa;
-    s -= Sf * xS;
-    s -= E * xb;
-    s -= N * xN;
-    return Cc != T(0) ? s * recip(Cc) : T(0);
-}
-
 // the bottom's coarsest level of N1 x N2 cells: the bordered solve of the
 // legs' operator (faces from w, boundary faces dropped), w, b and x at the
 // level's place in LDS
@@ -1663,21 +1645,79 @@ __device__ __forceinline__ void bottom_coarsest(const T *w, const T *b, T *x, T 
     for (int k = 0; k < n; k++) x[k] = (T)sol[k];
 }
 
-// Level offsets in LDS are running sums (an array of them indexed by the
-// level would live in scratch memory).
+// the faces of cell id = (i, j) of an n1 x n2 level from the per-level face
+// arrays in LDS: fw the face to row i-1, fs the face to column j-1 (each zero
+// on the boundary), so the east and north faces are the neighbours' fw and fs;
+// C the diagonal
+template <typename T>
+struct BFaces {
+    T W, E, S, N, C;
+};
+template <typename T>
+__device__ __forceinline__ BFaces<T> bottom_cell_faces(const T *fw, const T *fs, int n1, int n2, int i, int j,
+                                                       int id) {
+#pragma clang fp contract(fast)
+    BFaces<T> f;
+    f.W = fw[id];
+    f.E = i < n1 - 1 ? fw[id + n2] : T(0);
+    f.S = fs[id];
+    f.N = j < n2 - 1 ? fs[id + 1] : T(0);
+    f.C = -((f.W + f.E) + (f.S + f.N));
+    return f;
+}
+
+// Gauss-Seidel update of cell id = (i, j) from the stored faces and the
+// stored reciprocal rc of the diagonal, in gs_cell's operation order
+template <typename T>
+__device__ __forceinline__ T bottom_gs_f(const T *x, const T *fw, const T *fs, const T *rc, T bv, int n1, int n2,
+                                         int i, int j, int id) {
+#pragma clang fp contract(fast)
+    const BFaces<T> f = bottom_cell_faces(fw, fs, n1, n2, i, j, id);
+    const T xa = i > 0 ? x[id - n2] : T(0), xb = i < n1 - 1 ? x[id + n2] : T(0);
+    const T xS = j > 0 ? x[id - 1] : T(0), xN = j < n2 - 1 ? x[id + 1] : T(0);
+    T s = bv;
+    s -= f.W * xa;
+    s -= f.S * xS;
+    s -= f.E * xb;
+    s -= f.N * xN;
+    return f.C != T(0) ? s * rc[id] : T(0);
+}
+
+// Every level has at most BOTTOM_THREADS cells, so a thread holds at most one
+// cell of each and computes its (i, j) once per level.  The faces of all
+// levels and the reciprocals of their diagonals are formed once, after w is
+// loaded, and read from LDS by every sweep.  Level offsets in LDS are running
+// sums (an array of them indexed by the level would live in scratch memory).
+static_assert(BOTTOM_N * BOTTOM_N <= BOTTOM_THREADS, "one bottom cell per thread and level");
 template <typename T>
 __device__ void mg_bottom_body(const BottomArgs<T> &A) {
 #pragma clang fp contract(fast)
     __shared__ T sx[BOTTOM_CELLS], sb[BOTTOM_CELLS], sr[BOTTOM_CELLS], sw[BOTTOM_CELLS];
-    const int t = threadIdx.x, nt = blockDim.x;
+    __shared__ T sfw[BOTTOM_CELLS], sfs[BOTTOM_CELLS], src[BOTTOM_CELLS];
+    const int t = threadIdx.x;
     const int nl = A.nl;
     for (int l = 0, o = 0; l < nl; l++) {
         const int n1 = A.lev[l].n1, n2 = A.lev[l].n2, ldw = A.lev[l].ldw;
         const T *w = A.lev[l].w;
-        for (int id = t; id < n1 * n2; id += nt) {
-            const int i = id / n2, j = id - i * n2;
-            sw[o + id] = w[(long)i * ldw + j];
-            if (l == 0) sb[id] = A.b[(long)i * A.ldb + j];
+        if (t < n1 * n2) {
+            const int i = t / n2, j = t - i * n2;
+            sw[o + t] = w[(long)i * ldw + j];
+            if (l == 0) sb[t] = A.b[(long)i * A.ldb + j];
+        }
+        o += n1 * n2;
+    }
+    __syncthreads();
+    for (int l = 0, o = 0; l < nl; l++) {
+        const int n1 = A.lev[l].n1, n2 = A.lev[l].n2, hm = A.lev[l].harmonic;
+        const T cx = T(A.lev[l].cx), cy = T(A.lev[l].cy);
+        if (t < n1 * n2) {
+            const int i = t / n2, j = t - i * n2;
+            T W, E, Sf, N;
+            bottom_faces(sw + o, n1, n2, i, j, cx, cy, hm, W, E, Sf, N);
+            const T Cc = -((W + E) + (Sf + N));
+            sfw[o + t] = W;
+            sfs[o + t] = Sf;
+            src[o + t] = Cc != T(0) ? recip(Cc) : T(0);
         }
         o += n1 * n2;
     }
@@ -1685,52 +1725,41 @@ __device__ void mg_bottom_body(const BottomArgs<T> &A) {
     // down: red from zero, black, residual, full weighting
     int o = 0;   // offset of level l
     for (int l = 0; l + 1 < nl; l++) {
-        const int n1 = A.lev[l].n1, n2 = A.lev[l].n2, n = n1 * n2, hm = A.lev[l].harmonic;
-        const T cx = T(A.lev[l].cx), cy = T(A.lev[l].cy);
+        const int n1 = A.lev[l].n1, n2 = A.lev[l].n2, n = n1 * n2;
         T *x = sx + o, *r = sr + o;
-        const T *b = sb + o, *w = sw + o;
-        for (int id = t; id < n; id += nt) {
-            const int i = id / n2, j = id - i * n2;
+        const T *b = sb + o, *fw = sfw + o, *fs = sfs + o, *rc = src + o;
+        const bool act = t < n;
+        const int i = t / n2, j = t - i * n2;
+        if (act) {
             T v = T(0);
             if (((i + j) & 1) == 0) {
-                T W, E, Sf, N;
-                bottom_faces(w, n1, n2, i, j, cx, cy, hm, W, E, Sf, N);
-                const T C = -((W + E) + (Sf + N));
-                v = C != T(0) ? b[id] * recip(C) : T(0);
+                const BFaces<T> f = bottom_cell_faces(fw, fs, n1, n2, i, j, t);
+                v = f.C != T(0) ? b[t] * rc[t] : T(0);
             }
-            x[id] = v;
+            x[t] = v;
         }
         __syncthreads();
-        for (int id = t; id < n; id += nt) {
-            const int i = id / n2, j = id - i * n2;
-            if ((i + j) & 1) x[id] = bottom_gs(x, w, b[id], n1, n2, i, j, cx, cy, hm);
-        }
+        if (act && ((i + j) & 1)) x[t] = bottom_gs_f(x, fw, fs, rc, b[t], n1, n2, i, j, t);
         __syncthreads();
         for (int pr = 1; pr < A.ns; pr++)   // further red-black pairs
             for (int c = 0; c <= 1; c++) {
-                for (int id = t; id < n; id += nt) {
-                    const int i = id / n2, j = id - i * n2;
-                    if (((i + j) & 1) == c) x[id] = bottom_gs(x, w, b[id], n1, n2, i, j, cx, cy, hm);
-                }
+                if (act && ((i + j) & 1) == c) x[t] = bottom_gs_f(x, fw, fs, rc, b[t], n1, n2, i, j, t);
                 __syncthreads();
             }
-        for (int id = t; id < n; id += nt) {
-            const int i = id / n2, j = id - i * n2;
-            T W, E, Sf, N;
-            bottom_faces(w, n1, n2, i, j, cx, cy, hm, W, E, Sf, N);
-            const T Cc = -((W + E) + (Sf + N));
-            T s = Cc * x[id];
-            s += W * (i > 0 ? x[id - n2] : T(0));
-            s += Sf * (j > 0 ? x[id - 1] : T(0));
-            s += E * (i < n1 - 1 ? x[id + n2] : T(0));
-            s += N * (j < n2 - 1 ? x[id + 1] : T(0));
-            r[id] = b[id] - s;
+        if (act) {
+            const BFaces<T> f = bottom_cell_faces(fw, fs, n1, n2, i, j, t);
+            T s = f.C * x[t];
+            s += f.W * (i > 0 ? x[t - n2] : T(0));
+            s += f.S * (j > 0 ? x[t - 1] : T(0));
+            s += f.E * (i < n1 - 1 ? x[t + n2] : T(0));
+            s += f.N * (j < n2 - 1 ? x[t + 1] : T(0));
+            r[t] = b[t] - s;
         }
         __syncthreads();
         const int m1 = A.lev[l + 1].n1, m2 = A.lev[l + 1].n2;
         T *bc = sb + o + n;
-        for (int id = t; id < m1 * m2; id += nt) {
-            const int I = id / m2, J = id - I * m2;
+        if (t < m1 * m2) {
+            const int I = t / m2, J = t - I * m2;
             const T wL = T((2 * J - 1 >= 0 && 2 * J - 1 < n2) ? pw(2 * J - 1, J, m2) : 0.);
             const T w0 = T(2 * J < n2 ? pw(2 * J, J, m2) : 0.);
             const T w1 = T(2 * J + 1 < n2 ? pw(2 * J + 1, J, m2) : 0.);
@@ -1746,7 +1775,7 @@ __device__ void mg_bottom_body(const BottomArgs<T> &A) {
                 const T h = ((wL * rL + w0 * r0) + w1 * r1) + wR * rR;
                 acc += wi * h;
             }
-            bc[id] = T(0.25) * acc;
+            bc[t] = T(0.25) * acc;
         }
         __syncthreads();
         o += n;
@@ -1768,15 +1797,15 @@ __device__ void mg_bottom_body(const BottomArgs<T> &A) {
     __syncthreads();
     // up: x + P e, black, red
     for (int l = nl - 2; l >= 0; l--) {
-        const int n1 = A.lev[l].n1, n2 = A.lev[l].n2, n = n1 * n2, hm = A.lev[l].harmonic;
+        const int n1 = A.lev[l].n1, n2 = A.lev[l].n2, n = n1 * n2;
         const int m1 = A.lev[l + 1].n1, m2 = A.lev[l + 1].n2;
-        const T cx = T(A.lev[l].cx), cy = T(A.lev[l].cy);
         const int oc = o;   // offset of level l + 1
         o -= n;
         T *x = sx + o;
-        const T *xc = sx + oc, *b = sb + o, *w = sw + o;
-        for (int id = t; id < n; id += nt) {
-            const int i = id / n2, j = id - i * n2;
+        const T *xc = sx + oc, *b = sb + o, *fw = sfw + o, *fs = sfs + o, *rc = src + o;
+        const bool act = t < n;
+        const int i = t / n2, j = t - i * n2;
+        if (act) {
             int I0 = i >> 1, I1 = (i & 1) ? I0 + 1 : I0 - 1;
             I0 = I0 > m1 - 1 ? m1 - 1 : I0;
             I1 = I1 < 0 ? 0 : (I1 > m1 - 1 ? m1 - 1 : I1);
@@ -1784,23 +1813,20 @@ __device__ void mg_bottom_body(const BottomArgs<T> &A) {
             const int Jn = (j & 1) ? (J == m2 - 1 ? J : J + 1) : (J == 0 ? J : J - 1);
             const T o0 = xc[I0 * m2 + J], o1 = xc[I1 * m2 + J], l0 = xc[I0 * m2 + Jn], l1 = xc[I1 * m2 + Jn];
             const T e = T(0.5625) * o0 + T(0.1875) * l0 + T(0.1875) * o1 + T(0.0625) * l1;
-            x[id] = x[id] + e;
+            x[t] = x[t] + e;
         }
         __syncthreads();
         for (int pr = 0; pr < A.ns; pr++)
             for (int c = 1; c >= 0; c--) {
-                for (int id = t; id < n; id += nt) {
-                    const int i = id / n2, j = id - i * n2;
-                    if (((i + j) & 1) == c) x[id] = bottom_gs(x, w, b[id], n1, n2, i, j, cx, cy, hm);
-                }
+                if (act && ((i + j) & 1) == c) x[t] = bottom_gs_f(x, fw, fs, rc, b[t], n1, n2, i, j, t);
                 __syncthreads();
             }
     }
     {
         const int n1 = A.lev[0].n1, n2 = A.lev[0].n2;
-        for (int id = t; id < n1 * n2; id += nt) {
-            const int i = id / n2, j = id - i * n2;
-            A.out[(long)i * A.ldo + j] = sx[id];
+        if (t < n1 * n2) {
+            const int i = t / n2, j = t - i * n2;
+            A.out[(long)i * A.ldo + j] = sx[t];
         }
     }
 }
