@@ -253,6 +253,26 @@ int launch_diagnostics(impec_ctx *ctx, int which);  // results in h_scal after s
 // h_scal <- scal once the stream's work so far has finished (a publishing
 // kernel and a host poll, with a stream synchronisation as the fallback)
 int sync_scalars(impec_ctx *ctx);
+// A finishing kernel (one workgroup) publishes the scalars itself when given
+// the mapped mirror and the next sequence number, in place of the separate
+// publishing launch: after its own stores the workgroup copies scal into the
+// mirror and releases the number.  pub_next reserves the number; wait_scalars
+// then waits for it as sync_scalars does.
+struct PubArgs {
+    double *h;                 // the mapped mirror, or null: no publication
+    unsigned long long seq;
+};
+PubArgs pub_next(impec_ctx *ctx);
+int wait_scalars(impec_ctx *ctx);
+__device__ __forceinline__ void publish_scalars(const double *S, const PubArgs &p) {
+    if (!p.h) return;   // a kernel argument: the whole workgroup returns
+    __syncthreads();    // the workgroup's stores to S are complete and visible
+    for (int k = threadIdx.x; k < S_COUNT; k += blockDim.x) p.h[k] = S[k];
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store((unsigned long long *)(p.h + S_COUNT), p.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 int launch_minmax(impec_ctx *ctx, double *cmax, double *cmin);
 
 // ---- pressure (k_pressure.hip) ----
@@ -274,7 +294,7 @@ int cn_operator_dump(impec_ctx *ctx, double dt, double *A9_host, double *rhs_hos
 
 // ---- reductions (k_field.hip) ----
 // sum of NV per-block partial vectors into scal[dst .. dst+NV)
-int finish_sum(impec_ctx *ctx, int nblocks, int nv, int dst);
+int finish_sum(impec_ctx *ctx, int nblocks, int nv, int dst, PubArgs pub = PubArgs{nullptr, 0});
 
 // ---- ensembles: batched launches (k_field.hip, k_pressure.hip, k_transport.hip) ----
 int ens_set_error(impec_ensemble *e, int code, const char *fmt, ...);

@@ -752,15 +752,11 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
 //                      r0.s - omega r0.t; rr = r.r as s.s - 2 omega t.s + omega^2 t.t
 //  stage 6 (check):    the true residual's r.r
 // launched with BICG_FIN_THREADS threads: one wave per SIMD, so the loads of
-// all NV vectors fit in registers
+// all NV vectors fit in registers; with pub the workgroup then publishes the
+// scalars for the host
 constexpr int BICG_FIN_THREADS = 256;
 template <int NV>
-__global__ void __launch_bounds__(BICG_FIN_THREADS) k_bicg_finish(const double *__restrict__ part, int nb, int stage,
-                                                                  double *__restrict__ S) {
-    double v[NV];
-    strided_combine_nv<RED_SUM, NV>(part, RED_BLOCKS, nb, v);
-    block_reduce<RED_SUM, NV>(v);
-    if (threadIdx.x != 0) return;
+__device__ __forceinline__ void bicg_scalars(const double (&v)[NV], int stage, double *__restrict__ S) {
     if (stage == 0) {
         S[S_RR] = v[0];
         S[S_BB] = v[0];
@@ -792,6 +788,15 @@ __global__ void __launch_bounds__(BICG_FIN_THREADS) k_bicg_finish(const double *
         S[S_RHO_NEW] = v[3] - om * v[2];
         S[S_RR] = v[4] - 2. * om * v[0] + om * om * v[1];
     }
+}
+template <int NV>
+__global__ void __launch_bounds__(BICG_FIN_THREADS) k_bicg_finish(const double *__restrict__ part, int nb, int stage,
+                                                                  double *__restrict__ S, PubArgs pub) {
+    double v[NV];
+    strided_combine_nv<RED_SUM, NV>(part, RED_BLOCKS, nb, v);
+    block_reduce<RED_SUM, NV>(v);
+    if (threadIdx.x == 0) bicg_scalars<NV>(v, stage, S);
+    publish_scalars(S, pub);
 }
 
 // x = 0 (a zero-guess solve that needed no iteration), pitch ldx
@@ -850,8 +855,6 @@ static int tensors_and_rhs(impec_ctx *ctx, const CnConst &k, int *nblocks = null
     if (nblocks) *nblocks = (int)(vg.x * vg.y);
     return IMPEC_OK;
 }
-
-static int fetch(impec_ctx *ctx) { return sync_scalars(ctx); }
 
 // Solve_Parabolic_cc_to_c: x0 = 0 (AssignRHS_Parabolic, MGSolver.cpp:1375-1392)
 // or, with cn_guess = 1, x0 = the current c; stop on
@@ -925,9 +928,10 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         hipLaunchKernelGGL(k_cn_start, dim3((az.nwave + CN_WAVES - 1) / CN_WAVES), sblk, 0, s, az, ctx->c, ctx->lds, b,
                            ctx->cn_dg, r0);
         prof_end(ctx, "cn_start");
-        hipLaunchKernelGGL(k_bicg_finish<3>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, az.nwave, 5, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<3>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, az.nwave, 5, ctx->scal,
+                           pub_next(ctx));
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
-        if ((st = fetch(ctx))) return st;
+        if ((st = wait_scalars(ctx))) return st;
         bb = ctx->h_scal[S_BB];
         rr = ctx->h_scal[S_RR];
         cur = 1;
@@ -940,12 +944,14 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         prof_end(ctx, "cn_rhs");
         const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
         const int nvb = (int)(vg.x * vg.y);
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nrb, 0, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nrb, 0, ctx->scal,
+                           PubArgs{nullptr, 0});
         hipLaunchKernelGGL(k_bicg_guess<true>, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, r, r0,
                            ctx->part);
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nvb, 4, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nvb, 4, ctx->scal,
+                           pub_next(ctx));
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
-        if ((st = fetch(ctx))) return st;
+        if ((st = wait_scalars(ctx))) return st;
         bb = ctx->h_scal[S_BB];
         rr = ctx->h_scal[S_RR];
         if (!(rr <= tol * tol * bb) && rr == rr) {   // the first direction p = r
@@ -954,7 +960,8 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             prof_begin(ctx);
             hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, ssblk, 0, s, ap);
             prof_end(ctx, "cn_apply_p0");
-            hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 1, ctx->scal);
+            hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 1, ctx->scal,
+                               PubArgs{nullptr, 0});
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
             cur = 1;
         }
@@ -979,10 +986,10 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             hipLaunchKernelGGL(k_cn_stream<CN_S>, sg, ssblk, 0, s, as);
             prof_end(ctx, "cn_apply_s");
             hipLaunchKernelGGL(k_bicg_finish<5>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 2,
-                               ctx->scal);
+                               ctx->scal, pub_next(ctx));
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
             // the next residual's norm from the s-launch: the last iteration updates x only
-            if ((st = fetch(ctx))) return st;
+            if ((st = wait_scalars(ctx))) return st;
             rr = ctx->h_scal[S_RR];
             CnArgs ap = a2;
             ap.r = rcur; ap.pin = its == 0 ? pfirst : pb[cur]; ap.vin = vb[cur]; ap.x = x;
@@ -1013,7 +1020,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             hipLaunchKernelGGL(k_cn_pstep, pg, sblk, 0, s, ap);
             prof_end(ctx, "cn_apply_p");
             hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a2.nwave, 1,
-                               ctx->scal);
+                               ctx->scal, PubArgs{nullptr, 0});
             rcur = rb[rnext];
             rnext ^= 1;
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
@@ -1033,10 +1040,11 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             ar.x = x;
             hipLaunchKernelGGL(k_cn_stream<CN_R>, sg, ssblk, 0, s, ar);
         }
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 6, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 6, ctx->scal,
+                           pub_next(ctx));
         prof_end(ctx, "cn_check");
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
-        if ((st = fetch(ctx))) return st;
+        if ((st = wait_scalars(ctx))) return st;
         rt = ctx->h_scal[S_CNRES];
         if (rt <= tol * tol * bb) break;
         if (!(rt == rt)) { status = IMPEC_ENAN; break; }
@@ -1047,14 +1055,16 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         a.r0 = a2.r0 = r0;
         hipLaunchKernelGGL(k_bicg_guess<true>, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, r, r0,
                            ctx->part);
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nvb, 4, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nvb, 4, ctx->scal,
+                           PubArgs{nullptr, 0});
         CnArgs ap = a;
         ap.r = r;
         ap.pout = pb[1]; ap.out = vb[1];
         hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, ssblk, 0, s, ap);
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 1, ctx->scal);
+        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 1, ctx->scal,
+                           pub_next(ctx));
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
-        if ((st = fetch(ctx))) return st;
+        if ((st = wait_scalars(ctx))) return st;
         rr = ctx->h_scal[S_RR];
         rcur = r;
         rnext = 1;

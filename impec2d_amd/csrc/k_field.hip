@@ -260,8 +260,9 @@ __device__ __forceinline__ void finish_body(const G &gr_, const double *__restri
 }
 template <int OP>
 __global__ void k_finish(const double *__restrict__ part, int nblocks, int nv, double *__restrict__ scal,
-                         int dst) {
+                         int dst, PubArgs pub) {
     finish_body<OP>(HwGrid{}, part, nblocks, nv, scal, dst);
+    publish_scalars(scal, pub);
 }
 
 // The same combine in two launches, for many partials: one workgroup would
@@ -280,31 +281,34 @@ __global__ void __launch_bounds__(FIN_STAGE_WG) k_finish_shares(const double *__
 }
 template <int OP>
 __global__ void __launch_bounds__(FIN_THREADS) k_finish_reduce(const double *__restrict__ mid, int nv,
-                                                              double *__restrict__ scal, int dst) {
+                                                              double *__restrict__ scal, int dst, PubArgs pub) {
     for (int k = 0; k < nv; k++) {
         double v[1] = {mid[k * FIN_THREADS + threadIdx.x]};
         block_reduce<OP, 1>(v);
         if (threadIdx.x == 0) scal[dst + k] = v[0];
         __syncthreads();
     }
+    publish_scalars(scal, pub);
 }
 
 template <int OP>
-static int finish_op(impec_ctx *ctx, int nblocks, int nv, int dst) {
+static int finish_op(impec_ctx *ctx, int nblocks, int nv, int dst, PubArgs pub = PubArgs{nullptr, 0}) {
     if (nblocks > 8 * FIN_THREADS && nv <= PART_VECS) {
         hipLaunchKernelGGL(k_finish_shares<OP>, dim3(FIN_THREADS / FIN_STAGE_WG), dim3(FIN_STAGE_WG), 0, ctx->stream,
                            ctx->part, nblocks, nv, ctx->part_mid);
         hipLaunchKernelGGL(k_finish_reduce<OP>, dim3(1), dim3(FIN_THREADS), 0, ctx->stream, ctx->part_mid, nv,
-                           ctx->scal, dst);
+                           ctx->scal, dst, pub);
     } else {
         hipLaunchKernelGGL(k_finish<OP>, dim3(1), dim3(FIN_THREADS), 0, ctx->stream, ctx->part, nblocks, nv, ctx->scal,
-                           dst);
+                           dst, pub);
     }
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
 }
 
-int finish_sum(impec_ctx *ctx, int nblocks, int nv, int dst) { return finish_op<RED_SUM>(ctx, nblocks, nv, dst); }
+int finish_sum(impec_ctx *ctx, int nblocks, int nv, int dst, PubArgs pub) {
+    return finish_op<RED_SUM>(ctx, nblocks, nv, dst, pub);
+}
 
 static int red_blocks(long long n) {
     long long b = (n + 1023) / 1024;
@@ -367,18 +371,18 @@ __global__ void k_cfl(CflConst k, const double *__restrict__ u, const double *__
 // The scalars into the mapped host mirror, then the sequence number: every
 // thread makes its stores visible system-wide before the barrier, and one
 // thread then releases the number the host waits for.
-__global__ void k_publish(const double *__restrict__ S, double *__restrict__ h, unsigned long long seq) {
-    for (int k = threadIdx.x; k < S_COUNT; k += blockDim.x) h[k] = S[k];
-    __threadfence_system();
-    __syncthreads();
-    if (threadIdx.x == 0)
-        __hip_atomic_store((unsigned long long *)(h + S_COUNT), seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
+__global__ void k_publish(const double *__restrict__ S, PubArgs pub) { publish_scalars(S, pub); }
+
+PubArgs pub_next(impec_ctx *ctx) { return PubArgs{ctx->d_hscal, ++ctx->pub_seq}; }
 
 int sync_scalars(impec_ctx *ctx) {
-    const unsigned long long seq = ++ctx->pub_seq;
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, ctx->stream, ctx->scal, ctx->d_hscal, seq);
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, ctx->stream, ctx->scal, pub_next(ctx));
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return wait_scalars(ctx);
+}
+
+int wait_scalars(impec_ctx *ctx) {
+    const unsigned long long seq = ctx->pub_seq;
     // the host polls the sequence number (a few microseconds sooner than a
     // copy and a stream synchronisation); past 50 ms it leaves the wait to
     // the runtime, which also reports an earlier failure of the stream
@@ -409,12 +413,12 @@ int launch_cfl(impec_ctx *ctx, double *dt_out) {
     if (!st) {
         // min over the two remaining partial rows: shift them down with an offset
         hipLaunchKernelGGL(k_finish<RED_MIN>, dim3(1), dim3(256), 0, ctx->stream, ctx->part + RED_BLOCKS, nb, 2,
-                           ctx->scal, (int)S_MINDT);
+                           ctx->scal, (int)S_MINDT, pub_next(ctx));
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
     }
     prof_end(ctx, "cfl");
     if (st) return st;
-    if ((st = sync_scalars(ctx))) return st;
+    if ((st = wait_scalars(ctx))) return st;
     const double *S = ctx->h_scal;
     double dt;
     if (ctx->P.is_test) {
@@ -528,11 +532,11 @@ int launch_diagnostics(impec_ctx *ctx, int which) {
     hipLaunchKernelGGL(k_diag, dim3(nb), dim3(256), 0, ctx->stream, ctx->c, ctx->u, ctx->v, ctx->q, ctx->N1,
                        ctx->N2, ctx->B, ctx->lds, ctx->ldv, ctx->dx, ctx->dy, ctx->part);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
-    int st = finish_sum(ctx, nb, 3, S_MASS);
+    int st = finish_sum(ctx, nb, 3, S_MASS, pub_next(ctx));
     prof_end(ctx, "diagnostics");
     if (st) return st;
     // S_MASS, S_MASS+1 (div sum), S_MASS+2 (noflow sum)
-    return sync_scalars(ctx);
+    return wait_scalars(ctx);
 }
 
 __global__ void k_minmax(const double *__restrict__ c, long long n, double *__restrict__ part) {
@@ -579,9 +583,9 @@ int launch_minmax(impec_ctx *ctx, double *cmax, double *cmin) {
     int st = finish_op<RED_MAX>(ctx, nb, 1, S_CMAX);
     if (st) return st;
     hipLaunchKernelGGL(k_finish<RED_MIN>, dim3(1), dim3(256), 0, ctx->stream, ctx->part + RED_BLOCKS, nb, 1,
-                       ctx->scal, (int)S_CMIN);
+                       ctx->scal, (int)S_CMIN, pub_next(ctx));
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
-    if (int e = sync_scalars(ctx)) return e;
+    if (int e = wait_scalars(ctx)) return e;
     *cmax = ctx->h_scal[S_CMAX];
     *cmin = ctx->h_scal[S_CMIN];
     return IMPEC_OK;

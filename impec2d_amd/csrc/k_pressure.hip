@@ -2142,9 +2142,15 @@ __global__ void k_cg_update2(CgArgs c, double *__restrict__ p, const double *__r
 // rows per chunk of the streaming legs: about IMPEC_MG_WAVES wavefronts on large
 // levels, short chunks on small ones; even, since the march starts on an
 // even row.  The small levels' legs are a latency chain whose length is the
-// chunk plus the halo rows: chunks of 2 rows make the step 0.4% faster than 4
+// chunk plus the halo rows: on levels of at most IMPEC_MG_SMALL_ROWS rows
+// chunks of 2 rows make the step 0.4% faster than 4; on larger levels the
+// halo work of 2-row chunks costs more than it saves (512^2: 30 us per
+// V-cycle against 26), and they keep 4
 #ifndef IMPEC_MG_MINROWS
 #define IMPEC_MG_MINROWS 2
+#endif
+#ifndef IMPEC_MG_SMALL_ROWS
+#define IMPEC_MG_SMALL_ROWS 256
 #endif
 static_assert(IMPEC_MG_MINROWS % 2 == 0 && IMPEC_MG_MINROWS >= 2, "chunks of the streaming legs start on even rows");
 static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip, bool f32) {
@@ -2153,7 +2159,8 @@ static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip, bool f32
                             : (l == 0 ? IMPEC_MG_WAVES : IMPEC_MG_WAVES_COARSE);
     long r = ((long)n1 * nstrip + target - 1) / target;
     r = (r + 1) & ~1L;
-    if (r < IMPEC_MG_MINROWS) r = IMPEC_MG_MINROWS;
+    const long minr = n1 <= IMPEC_MG_SMALL_ROWS ? IMPEC_MG_MINROWS : (IMPEC_MG_MINROWS > 4 ? IMPEC_MG_MINROWS : 4);
+    if (r < minr) r = minr;
     if (r > 64) r = 64;
     return (int)r;
 }
@@ -2401,10 +2408,10 @@ static int cg_residual(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, int n
     prof_begin(ctx);
     hipLaunchKernelGGL(k_cg_init<DIAG>, dim3(nb), dim3(256), 0, ctx->stream, a0, c, ctx->p, ctx->q, ctx->pr, ctx->part);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
-    int st = finish_sum(ctx, nb, 2, S_RR);
+    int st = finish_sum(ctx, nb, 2, S_RR, pub_next(ctx));
     prof_end(ctx, "cg_init");
     if (st) return st;
-    if ((st = fetch_scal(ctx))) return st;
+    if ((st = wait_scalars(ctx))) return st;
     *rr = ctx->h_scal[S_RR];
     return IMPEC_OK;
 }
@@ -2484,7 +2491,7 @@ static int cg_iteration_fused(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c
     hipLaunchKernelGGL(k_cg_update2, vg.grid, vg.block, 0, s, c, ctx->p, *dnew, ctx->pr, ctx->pq, ctx->scal,
                        ctx->part);
     prof_end(ctx, "cg_update");
-    if ((st = finish_sum(ctx, vg.nblocks, 2, S_RR))) return st;
+    if ((st = finish_sum(ctx, vg.nblocks, 2, S_RR, pub_next(ctx)))) return st;
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     double *t = *dold;
     *dold = *dnew;
@@ -2522,7 +2529,7 @@ static int cg_iteration_mixed32(impec_ctx *ctx, const LevArgs &a0, const CgArgs 
     hipLaunchKernelGGL(k_cg_update32, vg.grid, vg.block, 0, s, c, ctx->pe32, *dnew, ctx->pr32, ctx->pq32, ctx->scal,
                        ctx->part);
     prof_end(ctx, "cg_update");
-    if ((st = finish_sum(ctx, vg.nblocks, 2, S_RR))) return st;
+    if ((st = finish_sum(ctx, vg.nblocks, 2, S_RR, pub_next(ctx)))) return st;
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     float *t = *dold;
     *dold = *dnew;
@@ -2537,9 +2544,9 @@ static int cg_residual32(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, int
     hipLaunchKernelGGL(k_cg_init32, dim3(nb), dim3(256), 0, ctx->stream, a0, c, ctx->p, ctx->q, ctx->pr32, ctx->pe32,
                        ctx->part);
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
-    int st = finish_sum(ctx, nb, 2, S_RR);
+    int st = finish_sum(ctx, nb, 2, S_RR, pub_next(ctx));
     prof_end(ctx, "cg_init");
-    if (st || (st = fetch_scal(ctx))) return st;
+    if (st || (st = wait_scalars(ctx))) return st;
     *rr = ctx->h_scal[S_RR];
     return IMPEC_OK;
 }
@@ -2650,9 +2657,9 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
             prof_begin(ctx);
             hipLaunchKernelGGL(k_cg_init_zero, dim3(nb), dim3(256), 0, s, c, ctx->p, ctx->q, ctx->pr, ctx->part);
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
-            st = finish_sum(ctx, nb, 2, S_RR);
+            st = finish_sum(ctx, nb, 2, S_RR, pub_next(ctx));
             prof_end(ctx, "cg_init");
-            if (st || (st = fetch_scal(ctx))) return st;
+            if (st || (st = wait_scalars(ctx))) return st;
             rr = ctx->h_scal[S_RR];
             if (ru && (st = cg_restart32(ctx, c))) return st;
         } else if (ru) {
@@ -2672,8 +2679,8 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
             }
             hipLaunchKernelGGL(k_true_residual<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q, ctx->part);
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
-            int e = finish_sum(ctx, nb, 2, S_TT);
-            if (!e) e = fetch_scal(ctx);
+            int e = finish_sum(ctx, nb, 2, S_TT, pub_next(ctx));
+            if (!e) e = wait_scalars(ctx);
             if (!e) {
                 relres = ctx->bnorm > 0 ? sqrt(ctx->h_scal[S_TT]) / ctx->bnorm : 0.;
                 floor_rel = ctx->bnorm > 0 ? IMPEC_FLOOR_C * 0x1p-53 * sqrt(ctx->h_scal[S_FLOOR]) / ctx->bnorm : 0.;
@@ -2691,11 +2698,13 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
             while (!(rr <= tol * tol * bb)) {
                 if (its >= maxit) { status = IMPEC_ENOCONV; break; }
                 if (!(rr == rr)) { status = IMPEC_ENAN; break; }
+                // the streaming iterations publish the scalars from their
+                // last reduction; the unfused one is published separately
                 if (ru) st = cg_iteration_mixed32(ctx, a0, c, since == 0, &dold32, &dnew32);
                 else if (FUSED && !DIAG) st = cg_iteration_fused<T>(ctx, a0, c, since == 0, &dold, &dnew);
                 else st = cg_iteration_unfused<DIAG>(ctx, a0, c, nb, since == 0);
                 if (st) return st;
-                if ((st = fetch_scal(ctx))) return st;
+                if ((st = (ru || (FUSED && !DIAG)) ? wait_scalars(ctx) : fetch_scal(ctx))) return st;
                 its++;
                 since++;
                 // the recursion broke down (it took no step): restart it from
