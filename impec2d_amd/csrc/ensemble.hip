@@ -236,6 +236,7 @@ int impec_ensemble_create(const impec_params *members, int count, impec_ensemble
     for (int k = 0; k < count && !st; k++) {
         st = ctx_create(&members[k], e->stream, &e->m[k]);
         if (st) ens_set_error(e, st, "member %d: %s", k, impec_strerror(st));
+        else e->m[k]->fixed_buffers = 1;   // the batched records hold its c and t1
     }
     if (!st) {
         std::vector<double *> S(count);

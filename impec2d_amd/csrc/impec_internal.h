@@ -137,6 +137,10 @@ struct impec_ctx {
     double *K11, *K22;         // nullptr when K == 1
     double *w, *w22;           // mobility quantity of the fine level (scalar layout)
     double *t1, *t2, *t3;      // transport temporaries (scalar layout)
+    // the addresses of c and t1 are held elsewhere (an ensemble's launch
+    // records): the transport copies its result into c instead of exchanging
+    // the two buffers
+    int fixed_buffers;
 
     // pressure solve
     int nlev;

@@ -752,8 +752,9 @@ __global__ void k_bicg_guess(CnConst k, const double *__restrict__ c, int lds, c
 //                      r0.s - omega r0.t; rr = r.r as s.s - 2 omega t.s + omega^2 t.t
 //  stage 6 (check):    the true residual's r.r
 // launched with BICG_FIN_THREADS threads: one wave per SIMD, so the loads of
-// all NV vectors fit in registers; with pub the workgroup then publishes the
-// scalars for the host
+// all NV vectors fit in registers (1024 threads for two vectors, four times
+// the loads in flight over the p-launch's 20480 partials, ran 4% slower);
+// with pub the workgroup then publishes the scalars for the host
 constexpr int BICG_FIN_THREADS = 256;
 template <int NV>
 __device__ __forceinline__ void bicg_scalars(const double (&v)[NV], int stage, double *__restrict__ S) {
@@ -790,13 +791,19 @@ __device__ __forceinline__ void bicg_scalars(const double (&v)[NV], int stage, d
     }
 }
 template <int NV>
-__global__ void __launch_bounds__(BICG_FIN_THREADS) k_bicg_finish(const double *__restrict__ part, int nb, int stage,
-                                                                  double *__restrict__ S, PubArgs pub) {
+__global__ void __launch_bounds__(BICG_FIN_THREADS) k_bicg_finish(const double *__restrict__ part, int nb,
+                                                                        int stage, double *__restrict__ S,
+                                                                        PubArgs pub) {
     double v[NV];
     strided_combine_nv<RED_SUM, NV>(part, RED_BLOCKS, nb, v);
     block_reduce<RED_SUM, NV>(v);
     if (threadIdx.x == 0) bicg_scalars<NV>(v, stage, S);
     publish_scalars(S, pub);
+}
+template <int NV>
+static void bicg_finish(hipStream_t s, const double *part, int nb, int stage, double *S,
+                        PubArgs pub = PubArgs{nullptr, 0}) {
+    hipLaunchKernelGGL(k_bicg_finish<NV>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, part, nb, stage, S, pub);
 }
 
 // x = 0 (a zero-guess solve that needed no iteration), pitch ldx
@@ -928,8 +935,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         hipLaunchKernelGGL(k_cn_start, dim3((az.nwave + CN_WAVES - 1) / CN_WAVES), sblk, 0, s, az, ctx->c, ctx->lds, b,
                            ctx->cn_dg, r0);
         prof_end(ctx, "cn_start");
-        hipLaunchKernelGGL(k_bicg_finish<3>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, az.nwave, 5, ctx->scal,
-                           pub_next(ctx));
+        bicg_finish<3>(s, ctx->part, az.nwave, 5, ctx->scal, pub_next(ctx));
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = wait_scalars(ctx))) return st;
         bb = ctx->h_scal[S_BB];
@@ -944,12 +950,10 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         prof_end(ctx, "cn_rhs");
         const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
         const int nvb = (int)(vg.x * vg.y);
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nrb, 0, ctx->scal,
-                           PubArgs{nullptr, 0});
+        bicg_finish<2>(s, ctx->part, nrb, 0, ctx->scal);
         hipLaunchKernelGGL(k_bicg_guess<true>, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, r, r0,
                            ctx->part);
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nvb, 4, ctx->scal,
-                           pub_next(ctx));
+        bicg_finish<2>(s, ctx->part, nvb, 4, ctx->scal, pub_next(ctx));
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = wait_scalars(ctx))) return st;
         bb = ctx->h_scal[S_BB];
@@ -960,8 +964,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             prof_begin(ctx);
             hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, ssblk, 0, s, ap);
             prof_end(ctx, "cn_apply_p0");
-            hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 1, ctx->scal,
-                               PubArgs{nullptr, 0});
+            bicg_finish<2>(s, ctx->part, a.nwave, 1, ctx->scal);
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
             cur = 1;
         }
@@ -985,8 +988,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             prof_begin(ctx);
             hipLaunchKernelGGL(k_cn_stream<CN_S>, sg, ssblk, 0, s, as);
             prof_end(ctx, "cn_apply_s");
-            hipLaunchKernelGGL(k_bicg_finish<5>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 2,
-                               ctx->scal, pub_next(ctx));
+            bicg_finish<5>(s, ctx->part, a.nwave, 2, ctx->scal, pub_next(ctx));
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
             // the next residual's norm from the s-launch: the last iteration updates x only
             if ((st = wait_scalars(ctx))) return st;
@@ -1019,8 +1021,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             prof_begin(ctx);
             hipLaunchKernelGGL(k_cn_pstep, pg, sblk, 0, s, ap);
             prof_end(ctx, "cn_apply_p");
-            hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a2.nwave, 1,
-                               ctx->scal, PubArgs{nullptr, 0});
+            bicg_finish<2>(s, ctx->part, a2.nwave, 1, ctx->scal);
             rcur = rb[rnext];
             rnext ^= 1;
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
@@ -1040,8 +1041,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             ar.x = x;
             hipLaunchKernelGGL(k_cn_stream<CN_R>, sg, ssblk, 0, s, ar);
         }
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 6, ctx->scal,
-                           pub_next(ctx));
+        bicg_finish<2>(s, ctx->part, a.nwave, 6, ctx->scal, pub_next(ctx));
         prof_end(ctx, "cn_check");
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = wait_scalars(ctx))) return st;
@@ -1055,14 +1055,12 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         a.r0 = a2.r0 = r0;
         hipLaunchKernelGGL(k_bicg_guess<true>, vg, vblk, 0, s, k, ctx->c, ctx->lds, ctx->u, ctx->v, b, r, r0,
                            ctx->part);
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, nvb, 4, ctx->scal,
-                           PubArgs{nullptr, 0});
+        bicg_finish<2>(s, ctx->part, nvb, 4, ctx->scal);
         CnArgs ap = a;
         ap.r = r;
         ap.pout = pb[1]; ap.out = vb[1];
         hipLaunchKernelGGL(k_cn_stream<CN_P0>, sg, ssblk, 0, s, ap);
-        hipLaunchKernelGGL(k_bicg_finish<2>, dim3(1), dim3(BICG_FIN_THREADS), 0, s, ctx->part, a.nwave, 1, ctx->scal,
-                           pub_next(ctx));
+        bicg_finish<2>(s, ctx->part, a.nwave, 1, ctx->scal, pub_next(ctx));
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = wait_scalars(ctx))) return st;
         rr = ctx->h_scal[S_RR];

@@ -538,7 +538,9 @@ int advect_split(impec_ctx *ctx, double dt) {
         break;
     default: {
         // X then Y and Y then X from c, averaged: one streaming pass into t1,
-        // then t1 back into c
+        // which becomes c (the two buffers exchange roles; every user of c
+        // reads ctx->c when it launches), or is copied back into c where the
+        // addresses are held elsewhere
         (void)t2; (void)t3;
         const int rows = ctx->N1 + 2 * ctx->B;
         const bool muscl = ctx->P.hyper_method == IMPEC_HYPER_MUSCL, phi1 = ctx->P.porosity == 1.;
@@ -547,8 +549,13 @@ int advect_split(impec_ctx *ctx, double dt) {
         else launch_parallel<double>(ctx, par_args<double>(ctx, dt, c, t1), muscl, phi1);
         prof_end(ctx, "advect_parallel");
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
-        IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(c, t1, sizeof(double) * (size_t)rows * ctx->lds, hipMemcpyDeviceToDevice,
-                                            ctx->stream));
+        if (ctx->fixed_buffers) {
+            IMPEC_HIP_CHECK(ctx, hipMemcpyAsync(c, t1, sizeof(double) * (size_t)rows * ctx->lds,
+                                                hipMemcpyDeviceToDevice, ctx->stream));
+        } else {
+            ctx->c = t1;
+            ctx->t1 = c;
+        }
         ctx->prev_first_sweep = 0;
     }
     }
