@@ -15,12 +15,10 @@ namespace impec {
 // FillGhostCellsNeumannScalar / FillGhostCellsPeriodicScalar (Util.cpp:215-324):
 // every ghost cell of the ghosted scalar array copies the interior cell its
 // even mirror (or periodic image) maps to; corners compose both maps, which
-// is what the reference's corner loops produce.
-template <class G>
-__device__ __forceinline__ void fill_scalar_body(const G &gr_, double *__restrict__ a, int N1, int N2, int B, int ld, int periodic) {
-    const int j = gr_.bx() * blockDim.x + threadIdx.x - B;
-    const int i = gr_.by() * blockDim.y + threadIdx.y - B;
-    if (i >= N1 + B || j >= N2 + B) return;
+// is what the reference's corner loops produce.  Cell (i, j), interior
+// indices, -B <= i < N1 + B, -B <= j < N2 + B; interior cells are left alone.
+__device__ __forceinline__ void fill_scalar_cell(double *__restrict__ a, int N1, int N2, int B, int ld, int periodic,
+                                                 int i, int j) {
     const bool gi = i < 0 || i >= N1, gj = j < 0 || j >= N2;
     if (!gi && !gj) return;
     int si, sj;
@@ -33,20 +31,22 @@ __device__ __forceinline__ void fill_scalar_body(const G &gr_, double *__restric
     }
     a[(size_t)(i + B) * ld + (j + B)] = a[(size_t)(si + B) * ld + (sj + B)];
 }
-__global__ void k_fill_scalar(double *__restrict__ a, int N1, int N2, int B, int ld, int periodic) {
-    fill_scalar_body(HwGrid{}, a, N1, N2, B, ld, periodic);
+// over the whole ghosted array (the ensemble's batched launch)
+template <class G>
+__device__ __forceinline__ void fill_scalar_body(const G &gr_, double *__restrict__ a, int N1, int N2, int B, int ld, int periodic) {
+    const int j = gr_.bx() * blockDim.x + threadIdx.x - B;
+    const int i = gr_.by() * blockDim.y + threadIdx.y - B;
+    if (i >= N1 + B || j >= N2 + B) return;
+    fill_scalar_cell(a, N1, N2, B, ld, periodic, i, j);
 }
 
 // FillGhostCellsNeumannVelocity (Util.cpp:331-438).  u (x-faces) is odd in x
 // about the faces i = 0 and i = N1 and even in y; v (y-faces) is odd in y about
 // j = 0 and j = N2 and even in x.  Entries the reference never writes
-// (u at j = N2+B, v at i = N1+B) are left untouched.
-template <class G>
-__device__ __forceinline__ void fill_velocity_body(const G &gr_, double *__restrict__ u, double *__restrict__ v, int N1, int N2, int B,
-                                int ld) {
-    const int j = gr_.bx() * blockDim.x + threadIdx.x - B;
-    const int i = gr_.by() * blockDim.y + threadIdx.y - B;
-    if (i > N1 + B || j > N2 + B) return;
+// (u at j = N2+B, v at i = N1+B) are left untouched.  Cell (i, j),
+// -B <= i <= N1 + B, -B <= j <= N2 + B.
+__device__ __forceinline__ void fill_velocity_cell(double *__restrict__ u, double *__restrict__ v, int N1, int N2,
+                                                   int B, int ld, int i, int j) {
     const size_t o = (size_t)(i + B) * ld + (j + B);
     // u component
     if (!((i >= 0 && i <= N1 && j >= 0 && j < N2) || j == N2 + B)) {
@@ -67,26 +67,87 @@ __device__ __forceinline__ void fill_velocity_body(const G &gr_, double *__restr
         v[o] = val;
     }
 }
-__global__ void k_fill_velocity(double *__restrict__ u, double *__restrict__ v, int N1, int N2, int B,
+template <class G>
+__device__ __forceinline__ void fill_velocity_body(const G &gr_, double *__restrict__ u, double *__restrict__ v, int N1, int N2, int B,
                                 int ld) {
-    fill_velocity_body(HwGrid{}, u, v, N1, N2, B, ld);
+    const int j = gr_.bx() * blockDim.x + threadIdx.x - B;
+    const int i = gr_.by() * blockDim.y + threadIdx.y - B;
+    if (i > N1 + B || j > N2 + B) return;
+    fill_velocity_cell(u, v, N1, N2, B, ld, i, j);
+}
+
+// The cells of the frame between an outer rectangle [r0, r1) x [c0, c1) and
+// an inner one [ir0, ir1) x [ic0, ic1): the rows above and below the inner
+// rectangle at full width, then the columns left and right of it, row by
+// row.  A ghost fill launches one thread per frame cell instead of one per
+// cell of the array (at 4096^2 about 33 thousand threads instead of 17
+// million: the launch of the whole grid, not the copies, was its cost).
+struct Frame {
+    int r0, r1, c0, c1, ir0, ir1, ic0, ic1;
+    long cells() const {
+        return (long)(ir0 - r0 + r1 - ir1) * (c1 - c0) + (long)(ir1 - ir0) * (ic0 - c0 + c1 - ic1);
+    }
+};
+__device__ __forceinline__ bool frame_cell(const Frame &f, long t, int &i, int &j) {
+    const long W = f.c1 - f.c0;
+    const long top = (long)(f.ir0 - f.r0) * W, bottom = (long)(f.r1 - f.ir1) * W;
+    if (t < top) {
+        i = f.r0 + (int)(t / W);
+        j = f.c0 + (int)(t % W);
+        return true;
+    }
+    t -= top;
+    if (t < bottom) {
+        i = f.ir1 + (int)(t / W);
+        j = f.c0 + (int)(t % W);
+        return true;
+    }
+    t -= bottom;
+    const int left = f.ic0 - f.c0, side = left + (f.c1 - f.ic1);
+    if (side <= 0 || t >= (long)(f.ir1 - f.ir0) * side) return false;
+    i = f.ir0 + (int)(t / side);
+    const int k = (int)(t % side);
+    j = k < left ? f.c0 + k : f.ic1 + (k - left);
+    return true;
+}
+constexpr int FRAME_THREADS = 256;
+
+__global__ void __launch_bounds__(FRAME_THREADS) k_fill_scalar(Frame f, double *__restrict__ a, int N1, int N2,
+                                                               int B, int ld, int periodic) {
+    int i, j;
+    if (frame_cell(f, (long)blockIdx.x * FRAME_THREADS + threadIdx.x, i, j))
+        fill_scalar_cell(a, N1, N2, B, ld, periodic, i, j);
+}
+__global__ void __launch_bounds__(FRAME_THREADS) k_fill_velocity(Frame f, double *__restrict__ u,
+                                                                 double *__restrict__ v, int N1, int N2, int B,
+                                                                 int ld) {
+    int i, j;
+    if (frame_cell(f, (long)blockIdx.x * FRAME_THREADS + threadIdx.x, i, j))
+        fill_velocity_cell(u, v, N1, N2, B, ld, i, j);
 }
 
 int launch_fill_scalar(impec_ctx *ctx, double *a, int periodic) {
-    dim3 blk(64, 4), grd((ctx->N2 + 2 * ctx->B + 63) / 64, (ctx->N1 + 2 * ctx->B + 3) / 4);
+    const int N1 = ctx->N1, N2 = ctx->N2, B = ctx->B;
+    const Frame f{-B, N1 + B, -B, N2 + B, 0, N1, 0, N2};
+    const long n = f.cells();
+    if (n == 0) return IMPEC_OK;
     prof_begin(ctx);
-    hipLaunchKernelGGL(k_fill_scalar, grd, blk, 0, ctx->stream, a, ctx->N1, ctx->N2, ctx->B, ctx->lds,
-                       periodic);
+    hipLaunchKernelGGL(k_fill_scalar, dim3((unsigned)((n + FRAME_THREADS - 1) / FRAME_THREADS)), dim3(FRAME_THREADS),
+                       0, ctx->stream, f, a, N1, N2, B, ctx->lds, periodic);
     prof_end(ctx, "fill_scalar");
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
 }
 
 int launch_fill_velocity(impec_ctx *ctx) {
-    dim3 blk(64, 4), grd((ctx->N2 + 1 + 2 * ctx->B + 63) / 64, (ctx->N1 + 1 + 2 * ctx->B + 3) / 4);
+    // every cell outside the common interior [0, N1) x [0, N2) of u and v;
+    // fill_velocity_cell leaves the interior ones of each component alone
+    const int N1 = ctx->N1, N2 = ctx->N2, B = ctx->B;
+    const Frame f{-B, N1 + B + 1, -B, N2 + B + 1, 0, N1, 0, N2};
+    const long n = f.cells();
     prof_begin(ctx);
-    hipLaunchKernelGGL(k_fill_velocity, grd, blk, 0, ctx->stream, ctx->u, ctx->v, ctx->N1, ctx->N2, ctx->B,
-                       ctx->ldv);
+    hipLaunchKernelGGL(k_fill_velocity, dim3((unsigned)((n + FRAME_THREADS - 1) / FRAME_THREADS)),
+                       dim3(FRAME_THREADS), 0, ctx->stream, f, ctx->u, ctx->v, N1, N2, B, ctx->ldv);
     prof_end(ctx, "fill_velocity");
     IMPEC_HIP_CHECK(ctx, hipGetLastError());
     return IMPEC_OK;
