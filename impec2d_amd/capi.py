@@ -34,18 +34,18 @@ _INT_FIELDS = ["N1", "N2", "num_ghostcells", "source_type", "average_procedure",
                "transport32"]
 _DBL_FIELDS = ["x_lb", "x_ub", "y_lb", "y_ub", "nu", "mu_res", "mu_inj", "porosity", "cfl",
                "inj_fluid_conc", "alpha_l", "alpha_t", "dt_diff_limiter", "d_m", "eps", "para_eps",
-               "I", "tstart", "inner_tol"]
+               "I", "tstart", "inner_tol", "cn_ahead"]
 
 
 class Params(C.Structure):
     _fields_ = ([(n, C.c_int) for n in _INT_FIELDS]
-                + [(n, C.c_double) for n in _DBL_FIELDS] + [("reserved_d", C.c_double * 3)])
+                + [(n, C.c_double) for n in _DBL_FIELDS] + [("reserved_d", C.c_double * 2)])
 
 
 class Stats(C.Structure):
     _fields_ = [("last_its", C.c_int), ("last_vcycles", C.c_int), ("last_parabolic_its", C.c_int),
                 ("total_its", C.c_int), ("calls", C.c_int), ("last_restarts", C.c_int),
-                ("last_elim_its", C.c_int), ("reserved_i", C.c_int * 1),
+                ("last_elim_its", C.c_int), ("cn_ahead_misses", C.c_int),
                 ("last_relres", C.c_double), ("last_parabolic_relres", C.c_double),
                 ("total_mass_outflux", C.c_double), ("last_guess_alpha", C.c_double),
                 ("last_guess_gamma", C.c_double), ("last_relres_floor", C.c_double),

@@ -77,7 +77,14 @@ enum { EST_DONE = 0, EST_ITER = 1, EST_CHECK = 2 };
 #ifndef IMPEC_STALL_GATE
 #define IMPEC_STALL_GATE 4.0
 #endif
-// the dispersion solve's BiCGStab (about 15 iterations per solve): a recursion
+// dispersion solve: the p-launch of an iteration is queued before the host
+// reads the s-launch's residual while the last known residual norm^2 lies more
+// than this factor above the target (one BiCGStab iteration reduces it by
+// about 10^3 on the benchmark workload)
+#ifndef IMPEC_CN_AHEAD
+#define IMPEC_CN_AHEAD 1e6
+#endif
+// the dispersion solve's BiCGStab (about 7 iterations per solve at 4096^2): a recursion
 // whose residual norm has not halved in this many iterations gets the true
 // residual check and restart of a converged one
 #ifndef IMPEC_CN_STALL_ITS
@@ -141,6 +148,7 @@ struct impec_ctx {
     // records): the transport copies its result into c instead of exchanging
     // the two buffers
     int fixed_buffers;
+    double cn_ahead;           // dispersion lookahead factor (negative: off)
 
     // pressure solve
     int nlev;

@@ -990,12 +990,31 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
             prof_end(ctx, "cn_apply_s");
             bicg_finish<5>(s, ctx->part, a.nwave, 2, ctx->scal, pub_next(ctx));
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
-            // the next residual's norm from the s-launch: the last iteration updates x only
-            if ((st = wait_scalars(ctx))) return st;
-            rr = ctx->h_scal[S_RR];
             CnArgs ap = a2;
             ap.r = rcur; ap.pin = its == 0 ? pfirst : pb[cur]; ap.vin = vb[cur]; ap.x = x;
             ap.xzero = zero && its == 0;
+            // Lookahead: while the last known residual norm^2 lies more than
+            // IMPEC_CN_AHEAD times above the target, this iteration will not
+            // end the solve, and its p-launch is queued before the host reads
+            // the s-launch's scalars, so the device does not wait for the host.
+            // Should it end the solve after all, the p-launch has updated x
+            // exactly as k_cn_xfinal would (same expression, same operands);
+            // its other outputs are not used.
+            const bool ahead = ctx->cn_ahead >= 0. && rr > ctx->cn_ahead * (tol * tol * bb);
+            int rn = rnext;
+            auto p_launch = [&]() {
+                if (rb[rn] == rcur) rn ^= 1;   // r' never overwrites the r it is formed from
+                ap.rout = rb[rn];
+                ap.pout = pb[cur ^ 1]; ap.out = vb[cur ^ 1];
+                prof_begin(ctx);
+                hipLaunchKernelGGL(k_cn_pstep, pg, sblk, 0, s, ap);
+                prof_end(ctx, "cn_apply_p");
+                bicg_finish<2>(s, ctx->part, a2.nwave, 1, ctx->scal);
+            };
+            if (ahead) p_launch();
+            // the next residual's norm from the s-launch: the last iteration updates x only
+            if ((st = wait_scalars(ctx))) return st;
+            rr = ctx->h_scal[S_RR];
             // a recursion whose residual norm has not halved in
             // IMPEC_CN_STALL_ITS iterations (at the rounding floor, or stuck)
             // is checked against the true residual and restarted like a
@@ -1007,24 +1026,21 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
                 stalled = ++stall >= IMPEC_CN_STALL_ITS;
             }
             if (rr <= tol * tol * bb || !(rr == rr) || stalled) {
-                prof_begin(ctx);
-                hipLaunchKernelGGL(k_cn_xfinal, cn_vgrid(N1, N2), dim3(64, 4), 0, s, ap);
-                prof_end(ctx, "cn_xfinal");
+                if (!ahead) {
+                    prof_begin(ctx);
+                    hipLaunchKernelGGL(k_cn_xfinal, cn_vgrid(N1, N2), dim3(64, 4), 0, s, ap);
+                    prof_end(ctx, "cn_xfinal");
+                }
                 IMPEC_HIP_CHECK(ctx, hipGetLastError());
                 its++;
+                if (ahead) ctx->stats.cn_ahead_misses++;
                 if (stalled) break;
                 continue;   // the loop test ends the solve (or reports the non-finite residual)
             }
-            if (rb[rnext] == rcur) rnext ^= 1;   // r' never overwrites the r it is formed from
-            ap.rout = rb[rnext];
-            ap.pout = pb[cur ^ 1]; ap.out = vb[cur ^ 1];
-            prof_begin(ctx);
-            hipLaunchKernelGGL(k_cn_pstep, pg, sblk, 0, s, ap);
-            prof_end(ctx, "cn_apply_p");
-            bicg_finish<2>(s, ctx->part, a2.nwave, 1, ctx->scal);
-            rcur = rb[rnext];
-            rnext ^= 1;
+            if (!ahead) p_launch();
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
+            rcur = rb[rn];
+            rnext = rn ^ 1;
             cur ^= 1;
             its++;
         }

@@ -100,7 +100,12 @@ typedef struct impec_params {
     double I;                  /* general.I, injection rate                    */
     double tstart;             /* general.starttime                            */
     double inner_tol;          /* ext: FP32 inner tolerance (mixed precision)  */
-    double reserved_d[3];
+    double cn_ahead;           /* ext: dispersion lookahead: an iteration's    */
+                               /* p-launch is queued before the host tests     */
+                               /* convergence while the last residual norm^2   */
+                               /* exceeds cn_ahead times the target (0: 1e6;   */
+                               /* negative: never); results do not depend on it */
+    double reserved_d[2];
 } impec_params;
 
 typedef struct impec_stats {
@@ -112,7 +117,9 @@ typedef struct impec_stats {
                                /* last pressure solve                          */
     int last_elim_its;         /* ext: of last_its, those of the elimination   */
                                /* phase (0: the consistent phase converged)    */
-    int reserved_i[1];
+    int cn_ahead_misses;       /* ext: dispersion p-launches queued ahead of   */
+                               /* the host's convergence test for an iteration */
+                               /* that ended its solve (cumulative)            */
     double last_relres;        /* true ||b - A p|| / ||b|| of the pinned system */
     double last_parabolic_relres;
     double total_mass_outflux; /* HyperbolicSolver::GetTotalMassOutflux        */
