@@ -34,12 +34,12 @@ _INT_FIELDS = ["N1", "N2", "num_ghostcells", "source_type", "average_procedure",
                "transport32"]
 _DBL_FIELDS = ["x_lb", "x_ub", "y_lb", "y_ub", "nu", "mu_res", "mu_inj", "porosity", "cfl",
                "inj_fluid_conc", "alpha_l", "alpha_t", "dt_diff_limiter", "d_m", "eps", "para_eps",
-               "I", "tstart", "inner_tol", "cn_ahead"]
+               "I", "tstart", "inner_tol", "cn_ahead", "cg_ahead"]
 
 
 class Params(C.Structure):
     _fields_ = ([(n, C.c_int) for n in _INT_FIELDS]
-                + [(n, C.c_double) for n in _DBL_FIELDS] + [("reserved_d", C.c_double * 2)])
+                + [(n, C.c_double) for n in _DBL_FIELDS] + [("reserved_d", C.c_double * 1)])
 
 
 class Stats(C.Structure):
@@ -49,7 +49,7 @@ class Stats(C.Structure):
                 ("last_relres", C.c_double), ("last_parabolic_relres", C.c_double),
                 ("total_mass_outflux", C.c_double), ("last_guess_alpha", C.c_double),
                 ("last_guess_gamma", C.c_double), ("last_relres_floor", C.c_double),
-                ("reserved_d", C.c_double * 2)]
+                ("cg_ahead_misses", C.c_int), ("reserved_i2", C.c_int), ("reserved_d", C.c_double * 1)]
 
 
 # every entry point declared in include/impec_hip.h: name -> (argtypes, restype)

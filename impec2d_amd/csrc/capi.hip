@@ -160,7 +160,7 @@ int impec::ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out) {
     if (P->mg_rows < 0 || P->mg_rows > 4096) return IMPEC_EINVAL;
     if (P->mg_variant < 0 || P->mg_variant > 2) return IMPEC_EINVAL;
     if (P->cn_guess != 0 && P->cn_guess != 1) return IMPEC_EINVAL;
-    if (P->cn_ahead != P->cn_ahead) return IMPEC_EINVAL;   // NaN
+    if (P->cn_ahead != P->cn_ahead || P->cg_ahead != P->cg_ahead) return IMPEC_EINVAL;   // NaN
     // FP32 transport: the one-pass parallel splitting only
     if (P->transport32 != 0 && (P->transport32 != 1 || P->dim_splitting != IMPEC_SPLIT_PARALLEL)) return IMPEC_EINVAL;
     if (P->initial_guess < IMPEC_GUESS_ZERO || P->initial_guess > IMPEC_GUESS_EXTRAPOLATE) return IMPEC_EINVAL;
@@ -197,6 +197,7 @@ int impec::ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out) {
     ctx->pin_i = 1; ctx->pin_j = 1;   // MGSolver.cpp:109-118
     ctx->force_unfused = P->mg_variant == 1;
     ctx->cn_ahead = P->cn_ahead == 0. ? IMPEC_CN_AHEAD : P->cn_ahead;
+    ctx->cg_ahead = P->cg_ahead == 0. ? IMPEC_CG_AHEAD : P->cg_ahead;
     ctx->prev_first_sweep = 1;        // HyperbolicSolver.cpp:84
     ctx->lds = round_up(ctx->N2 + 2 * ctx->B, 16);
     ctx->ldv = round_up(ctx->N2 + 1 + 2 * ctx->B, 16);

@@ -84,6 +84,12 @@ enum { EST_DONE = 0, EST_ITER = 1, EST_CHECK = 2 };
 #ifndef IMPEC_CN_AHEAD
 #define IMPEC_CN_AHEAD 1e6
 #endif
+// pressure solve: the next V-cycle's first leg is queued before the host reads
+// the CG iteration's residual while the last known residual norm^2 lies more
+// than this factor above the target (an iteration reduces it by about 10^2.5)
+#ifndef IMPEC_CG_AHEAD
+#define IMPEC_CG_AHEAD 1e6
+#endif
 // the dispersion solve's BiCGStab (about 7 iterations per solve at 4096^2): a recursion
 // whose residual norm has not halved in this many iterations gets the true
 // residual check and restart of a converged one
@@ -149,6 +155,7 @@ struct impec_ctx {
     // the two buffers
     int fixed_buffers;
     double cn_ahead;           // dispersion lookahead factor (negative: off)
+    double cg_ahead;           // pressure lookahead factor (negative: off)
 
     // pressure solve
     int nlev;

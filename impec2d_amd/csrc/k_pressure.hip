@@ -2260,15 +2260,16 @@ static BottomArgs<T> bottom_args(impec_ctx *ctx, int lb) {
 }
 
 // r32: level 0 reads the FP32 residual of the FP32-vector CG (T = float only)
+// skip_first: the finest level's down leg is already queued (vcycle_first_leg)
 template <typename T, int NS>
-static int vcycle_fused_ns(impec_ctx *ctx, bool r32) {
+static int vcycle_fused_ns(impec_ctx *ctx, bool r32, bool skip_first = false) {
     constexpr bool F = std::is_same<T, float>::value;
     using LA = LevelArrays<T>;
     const int L = ctx->nlev;
     const dim3 blk(64 * SW_WAVES);
     const int lb = bottom_level(ctx);
     const int ltop = lb > 0 ? lb : L - 1;
-    for (int l = 0; l < ltop; l++) {
+    for (int l = skip_first ? 1 : 0; l < ltop; l++) {
         FusedArgsT<T> A = fused_args<T>(ctx, l);
         prof_begin(ctx);
         if (F && r32 && l == 0 && A.w2 != A.w)
@@ -2311,8 +2312,27 @@ static int vcycle_fused_ns(impec_ctx *ctx, bool r32) {
 }
 
 template <typename T>
-static int vcycle_fused(impec_ctx *ctx, bool r32 = false) {
-    return mg_ns(ctx) == 2 ? vcycle_fused_ns<T, 2>(ctx, r32) : vcycle_fused_ns<T, 1>(ctx, r32);
+static int vcycle_fused(impec_ctx *ctx, bool r32 = false, bool skip_first = false) {
+    return mg_ns(ctx) == 2 ? vcycle_fused_ns<T, 2>(ctx, r32, skip_first) : vcycle_fused_ns<T, 1>(ctx, r32, skip_first);
+}
+
+// the finest level's down leg of vcycle_fused (FP64 CG residual), queued on
+// its own: it reads only the CG residual and writes the finest level's x and
+// the next level's right-hand side, scratch of the preconditioner
+template <typename T, int NS>
+static int vcycle_first_leg_ns(impec_ctx *ctx) {
+    const dim3 blk(64 * SW_WAVES);
+    FusedArgsT<T> A = fused_args<T>(ctx, 0);
+    prof_begin(ctx);
+    if (A.w2 != A.w) hipLaunchKernelGGL((k_mg_down<true, true, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
+    else hipLaunchKernelGGL((k_mg_down<true, false, T, NS>), stream_grid(A), blk, 0, ctx->stream, A);
+    prof_end(ctx, "mg_down_l0");
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+template <typename T>
+static int vcycle_first_leg(impec_ctx *ctx) {
+    return mg_ns(ctx) == 2 ? vcycle_first_leg_ns<T, 2>(ctx) : vcycle_first_leg_ns<T, 1>(ctx);
 }
 
 // FP32 copy of a mobility array (rows x cols cells from (0,0) of each)
@@ -2469,17 +2489,18 @@ static DirMvArgs<T> dir_mv_args(impec_ctx *ctx, const LevArgs &a0, const CgArgs 
 
 // one CG iteration with the streaming V-cycle (nu = 1) in storage type T
 // (double: plain FP64; float: the FP32 cycle of the mixed-precision solve);
-// the CG vectors, the residual and every reduction stay FP64; d ping-pongs
+// the CG vectors, the residual and every reduction stay FP64; d ping-pongs.
+// first_leg_queued: the V-cycle's first leg was queued ahead (vcycle_first_leg)
 template <typename T>
 static int cg_iteration_fused(impec_ctx *ctx, const LevArgs &a0, const CgArgs &c, bool first, double **dold,
-                              double **dnew) {
+                              double **dnew, bool first_leg_queued = false) {
     const Level &L0 = ctx->lev[0];
     hipStream_t s = ctx->stream;
     const Grid2 vg = vec_grid(ctx->N1, ctx->N2);
     const int nwave0 = fused_args<T>(ctx, 0).nwave;
     const double inv_n = 1.0 / (double)c.n;
     int st;
-    if ((st = vcycle_fused<T>(ctx))) return st;
+    if ((st = vcycle_fused<T>(ctx, false, first_leg_queued))) return st;
     hipLaunchKernelGGL(k_finish_rho, dim3(1), dim3(1024), 0, s, ctx->part, nwave0, inv_n, ctx->scal);
     const DirMvArgs<T> dm = dir_mv_args<T>(ctx, a0, c);
     prof_begin(ctx);
@@ -2695,15 +2716,26 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
             bool broke = false;
             double rr_mark = rr;   // the recursive residual when it last halved
             int stall = 0;
+            // Lookahead (streaming iterations with FP64 vectors): while the last
+            // known residual norm^2 lies more than cg_ahead times above the
+            // target, the next V-cycle's first leg is queued before the host
+            // reads the iteration's scalars, so the device does not wait for
+            // the host.  The leg reads only r; every path below that changes r
+            // leaves the loop, and the next entry queues the leg afresh.
+            bool leg_queued = false;
+            const bool lookahead = !ru && FUSED && !DIAG && ctx->cg_ahead >= 0. && top_level(ctx) >= 1;
             while (!(rr <= tol * tol * bb)) {
                 if (its >= maxit) { status = IMPEC_ENOCONV; break; }
                 if (!(rr == rr)) { status = IMPEC_ENAN; break; }
                 // the streaming iterations publish the scalars from their
                 // last reduction; the unfused one is published separately
                 if (ru) st = cg_iteration_mixed32(ctx, a0, c, since == 0, &dold32, &dnew32);
-                else if (FUSED && !DIAG) st = cg_iteration_fused<T>(ctx, a0, c, since == 0, &dold, &dnew);
+                else if (FUSED && !DIAG) st = cg_iteration_fused<T>(ctx, a0, c, since == 0, &dold, &dnew, leg_queued);
                 else st = cg_iteration_unfused<DIAG>(ctx, a0, c, nb, since == 0);
                 if (st) return st;
+                const bool ahead = lookahead && rr > ctx->cg_ahead * (tol * tol * bb);
+                if (ahead && (st = vcycle_first_leg<T>(ctx))) return st;
+                leg_queued = ahead;
                 if ((st = (ru || (FUSED && !DIAG)) ? wait_scalars(ctx) : fetch_scal(ctx))) return st;
                 its++;
                 since++;
@@ -2744,6 +2776,7 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
                     stall = 0;
                 }
             }
+            if (leg_queued) ctx->stats.cg_ahead_misses++;   // queued for an iteration that did not run
             if (at_floor) {
                 finished = true;
                 break;

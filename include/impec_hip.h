@@ -105,7 +105,10 @@ typedef struct impec_params {
                                /* convergence while the last residual norm^2   */
                                /* exceeds cn_ahead times the target (0: 1e6;   */
                                /* negative: never); results do not depend on it */
-    double reserved_d[2];
+    double cg_ahead;           /* ext: pressure lookahead: the next V-cycle's  */
+                               /* first leg is queued likewise (0: 1e6;        */
+                               /* negative: never); results do not depend on it */
+    double reserved_d[1];
 } impec_params;
 
 typedef struct impec_stats {
@@ -130,7 +133,11 @@ typedef struct impec_stats {
     double last_relres_floor;  /* the FP64 rounding floor of last_relres:      */
                                /* 8 u || |b| + |A| |p| || / ||b||; a solve     */
                                /* converges at max(eps, this)                  */
-    double reserved_d[2];
+    int cg_ahead_misses;       /* ext: first legs of a V-cycle queued ahead of  */
+                               /* the host's convergence test for an iteration */
+                               /* that did not run (cumulative)                */
+    int reserved_i2;
+    double reserved_d[1];
 } impec_stats;
 
 typedef struct impec_ctx impec_ctx;

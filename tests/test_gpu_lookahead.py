@@ -1,10 +1,12 @@
-"""The dispersion solve's lookahead: while the residual is far above the
-target, an iteration's p-launch is queued before the host reads the
-s-launch's residual.  When such an iteration turns out to end the solve, the
-p-launch stands in for the final x update (the same expression on the same
-operands), so the fields must not depend on the lookahead factor: never
-(negative), the default (0: 1e6), and 1, which looks ahead in every iteration
-that starts above the target, the converging one included."""
+"""The solvers' lookahead: while the residual is far above the target, work
+of the next step of the iteration is queued before the host reads the
+residual that decides convergence.  Dispersion: an iteration's p-launch; when
+the iteration turns out to end the solve, the p-launch stands in for the final
+x update (the same expression on the same operands).  Pressure: the next
+V-cycle's first leg, which reads only the residual.  The fields must not
+depend on the lookahead factor: never (negative), the default (0: 1e6), and 1,
+which looks ahead in every iteration that starts above the target, the
+converging one included."""
 import numpy as np
 import pytest
 
@@ -69,3 +71,42 @@ def test_lookahead_matches_oracle(gpu_lib):
     assert rel_err(o.interior(g.get_concentration()), o.interior(c)) < 1e-9
     assert g.stats().cn_ahead_misses > 0
     g.close()
+
+
+def _run_pressure(ahead, precision, steps=3):
+    p = params(N1=64, N2=48, eps=1e-11, mu_inj=0.1)
+    o, g = pair(p, cg_ahead=ahead, precision=precision)
+    g.set_source(o.source("one_cell"))
+    g.set_concentration(o.scalar())
+    t = 0.0
+    its = []
+    for _ in range(steps):
+        t += g.step(t)
+        its.append(g.stats().last_its)
+    out = dict(p=g.get_pressure(), u=g.get_velocity(), c=g.get_concentration(), its=its,
+               misses=g.stats().cg_ahead_misses, t=t)
+    g.close()
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pressure_lookahead_leaves_fields_bitwise_unchanged(gpu_lib, precision):
+    """The pressure solve queues the next V-cycle's first leg before the host
+    reads a CG iteration's residual while that residual is far above the
+    target; the leg reads only r, so the fields do not depend on the factor
+    (1: every iteration ahead, each solve ending with a leg that goes unused)."""
+    from impec2d_amd import capi
+    prec = capi.PREC_FP64 if precision == "fp64" else capi.PREC_MIXED
+    runs = {}
+    for ahead in (-1.0, 0.0, 1.0):
+        runs[ahead] = _run_pressure(ahead, prec)
+        print(precision, "factor", ahead, "CG iterations", runs[ahead]["its"], "misses", runs[ahead]["misses"],
+              flush=True)
+    ref = runs[-1.0]
+    assert ref["misses"] == 0
+    assert runs[1.0]["misses"] >= len(ref["its"])
+    for ahead in (0.0, 1.0):
+        r = runs[ahead]
+        assert r["its"] == ref["its"] and r["t"] == ref["t"]
+        for f in ("p", "u", "c"):
+            assert np.array_equal(r[f], ref[f]), (precision, ahead, f)
