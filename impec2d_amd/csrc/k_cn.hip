@@ -198,7 +198,12 @@ constexpr int S_RHO_NEW = 13;
 #ifndef IMPEC_CNS_WAVES
 #define IMPEC_CNS_WAVES 8
 #endif
-constexpr int CN_OUT = 62, CNP_OUT = 60, CN_WAVES = IMPEC_CN_WAVES, CNS_WAVES = IMPEC_CNS_WAVES;
+// the zero-guess start (k_cn_start)
+#ifndef IMPEC_CNZ_WAVES
+#define IMPEC_CNZ_WAVES 4
+#endif
+constexpr int CN_OUT = 62, CNP_OUT = 60, CN_WAVES = IMPEC_CN_WAVES, CNS_WAVES = IMPEC_CNS_WAVES,
+              CNZ_WAVES = IMPEC_CNZ_WAVES;
 enum { CN_S = 1, CN_P0 = 2, CN_R = 3 };
 // build-time tuning: rows in flight per wavefront, wavefronts per launch
 #ifndef IMPEC_CN_PF
@@ -561,11 +566,11 @@ __global__ void __launch_bounds__(64 * CN_WAVES, IMPEC_CNP_MIN_WAVES) k_cn_pstep
 // direction's v = A (b/D) on row m-2 (CN_P0's).  Writes b, 1/D and v;
 // partials b.v and b.b.  As in the p-launch, lanes 2..61 of a strip and
 // rows i0-2 .. i1+1 are streamed for the two chained stencils.
-__global__ void __launch_bounds__(64 * CN_WAVES) k_cn_start(CnArgs A, const double *__restrict__ c, int lds,
-                                                            double *__restrict__ b, float *__restrict__ dinv,
-                                                            float *__restrict__ r0) {
+__global__ void __launch_bounds__(64 * CNZ_WAVES) k_cn_start(CnArgs A, const double *__restrict__ c, int lds,
+                                                             double *__restrict__ b, float *__restrict__ dinv,
+                                                             float *__restrict__ r0) {
     const int lane = threadIdx.x & 63;
-    const int wid = xcd_block(blockIdx.x, gridDim.x) * CN_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wid = xcd_block(blockIdx.x, gridDim.x) * CNZ_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wid >= A.nwave) return;
     const CnConst &k = A.k;
     const int N1 = k.N1, N2 = k.N2, B = k.B;
@@ -932,7 +937,8 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         }
         az.nwave = az.nstrip * ((N1 + az.R - 1) / az.R);
         prof_begin(ctx);
-        hipLaunchKernelGGL(k_cn_start, dim3((az.nwave + CN_WAVES - 1) / CN_WAVES), sblk, 0, s, az, ctx->c, ctx->lds, b,
+        hipLaunchKernelGGL(k_cn_start, dim3((az.nwave + CNZ_WAVES - 1) / CNZ_WAVES), dim3(64 * CNZ_WAVES), 0, s, az,
+                           ctx->c, ctx->lds, b,
                            ctx->cn_dg, r0);
         prof_end(ctx, "cn_start");
         bicg_finish<3>(s, ctx->part, az.nwave, 5, ctx->scal, pub_next(ctx));
