@@ -2024,6 +2024,118 @@ __global__ void __launch_bounds__(64 * SW_WAVES) k_dir_mv(DirMvArgs<T, DT> A, co
     dir_mv_body<T, DT>(A, dold, dnew, first, xcd_wave());
 }
 
+// The pinned system's true residual, streamed like k_dir_mv: a wave owns
+// DM_OUT columns (one per lane, a halo lane on each side) and a chunk of R
+// rows; it reads x and q of each row once, keeps rows i-1, i, i+1 of x in
+// registers and takes the column neighbours by DPP.  Per cell the arithmetic
+// of true_residual_body (op_row5, the terms of row_dot and of row_absdot in
+// their order); per-wave partials of ||b - A x||^2 and || |b| + |A| |x| ||^2.
+// (A grid-stride loop over flat indices, reading x five times, ran at
+// 2.8 TB/s.)  gx: workgroups of SW_WAVES waves (the ensemble's batch units).
+struct TresArgs {
+    LevArgs L;
+    CgArgs c;
+    const double *p, *q;
+    double *part;
+    int R, nstrip, nwave;
+    int gx, gy;
+};
+static TresArgs tres_args(const LevArgs &a0, const CgArgs &c, const double *p, const double *q, double *part) {
+    TresArgs A;
+    memset(&A, 0, sizeof(A));
+    A.L = a0;
+    A.c = c;
+    A.p = p;
+    A.q = q;
+    A.part = part;
+    A.nstrip = (c.n2 + DM_OUT - 1) / DM_OUT;
+    long R = ((long)c.n1 * A.nstrip + IMPEC_DM_WAVES - 1) / IMPEC_DM_WAVES;
+    if (R < 8) R = 8;
+    A.R = (int)R;
+    A.nwave = A.nstrip * ((c.n1 + A.R - 1) / A.R);
+    A.gx = (A.nwave + SW_WAVES - 1) / SW_WAVES;
+    A.gy = 1;
+    return A;
+}
+constexpr int TRES_PF = 4;
+__device__ __forceinline__ void tres_rows_body(const TresArgs &A, const int wid) {
+    if (wid >= A.nwave) return;
+    const CgArgs &c = A.c;
+    const int n1 = c.n1, n2 = c.n2;
+    const int lane = threadIdx.x & 63;
+    const int strip = wid % A.nstrip, chunk = wid / A.nstrip;
+    const int j = strip * DM_OUT - 1 + lane;
+    const bool jv = j >= 0 && j < n2;
+    const int jc = j < 0 ? 0 : (j > n2 - 1 ? n2 - 1 : j);
+    const bool outl = lane >= 1 && lane <= DM_OUT && jv;
+    const int i0 = chunk * A.R, i1 = i0 + A.R < n1 ? i0 + A.R : n1;
+    const double *__restrict__ x = A.p + (long)c.B * c.ldp + c.B;
+    auto xrow = [&](int i) -> double {
+        const int ic = i < 0 ? 0 : (i > n1 - 1 ? n1 - 1 : i);
+        return x[(long)ic * c.ldp + jc];
+    };
+    double RX[TRES_PF], RQ[TRES_PF];
+#pragma unroll
+    for (int u = 0; u < TRES_PF; u++) {
+        RX[u] = xrow(i0 + 1 + u);
+        const int ic = i0 + u < n1 ? i0 + u : n1 - 1;
+        RQ[u] = A.q[(long)ic * n2 + jc];
+    }
+    double xm = xrow(i0 - 1), xc = xrow(i0);
+    double s0 = 0., s1 = 0.;
+    for (int kb = i0; kb < i1; kb += TRES_PF) {
+#pragma unroll
+        for (int u = 0; u < TRES_PF; u++) {
+            const int i = kb + u;
+            const double xn = RX[u], qv = RQ[u];
+            {   // refill the slot (rows past the chunk re-read its last row)
+                const int in = i + 1 + TRES_PF <= i1 ? i + 1 + TRES_PF : i1;
+                RX[u] = xrow(in);
+                const int iq = i + TRES_PF < i1 ? i + TRES_PF : i1 - 1;
+                RQ[u] = A.q[(long)iq * n2 + jc];
+            }
+            const double xl = lane_prev(xc), xr = lane_next(xc);
+            if (i < i1 && outl) {
+                double rv, mag;
+                if (i == c.pin_i && j == c.pin_j) {
+                    rv = -qv - c.dpin * xc;
+                    mag = fabs(qv) + fabs(c.dpin * xc);
+                } else {
+                    double a[9];
+                    op_row5(A.L, i, j, a);
+                    const bool w = i > 0, e = i < n1 - 1, so = j > 0, no = j < n2 - 1;
+                    double sd = a[ST_C] * xc;
+                    if (w) sd += a[ST_W] * xm;
+                    if (so) sd += a[ST_S] * xl;
+                    if (e) sd += a[ST_E] * xn;
+                    if (no) sd += a[ST_N] * xr;
+                    double sa = fabs(a[ST_C] * xc);
+                    if (w) sa += fabs(a[ST_W] * xm);
+                    if (so) sa += fabs(a[ST_S] * xl);
+                    if (e) sa += fabs(a[ST_E] * xn);
+                    if (no) sa += fabs(a[ST_N] * xr);
+                    rv = -qv - sd;
+                    mag = fabs(qv) + sa;
+                }
+                s0 += rv * rv;
+                s1 += mag * mag;
+            }
+            xm = xc;
+            xc = xn;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+    }
+    if (lane == 0) {
+        A.part[wid] = s0;
+        A.part[RED_BLOCKS + wid] = s1;
+    }
+}
+__global__ void __launch_bounds__(64 * SW_WAVES) k_true_rows(TresArgs A) { tres_rows_body(A, xcd_wave()); }
+
 // --- the FP32-vector CG (precision IMPEC_PREC_MIXED32) ----------------------
 // e += alpha d, r -= alpha q in FP32 storage (alpha and the arithmetic in
 // FP64, each result rounded once); partials (r.r, sum r) of the stored r
@@ -2698,9 +2810,17 @@ static int solve_cg(impec_ctx *ctx, int *its_out, double *relres_out) {
             } else {
                 hipLaunchKernelGGL(k_set_pin, dim3(1), dim3(1), 0, s, c, ctx->p);
             }
-            hipLaunchKernelGGL(k_true_residual<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q, ctx->part);
+            int nparts = nb;
+            const TresArgs ta = tres_args(a0, c, ctx->p, ctx->q, ctx->part);
+            if (!DIAG && ta.nwave <= RED_BLOCKS) {   // the 5-point operator: the streamed pass
+                hipLaunchKernelGGL(k_true_rows, dim3(ta.gx), dim3(64 * SW_WAVES), 0, s, ta);
+                nparts = ta.nwave;
+            } else {
+                hipLaunchKernelGGL(k_true_residual<DIAG>, dim3(nb), dim3(256), 0, s, a0, c, ctx->p, ctx->q,
+                                   ctx->part);
+            }
             IMPEC_HIP_CHECK(ctx, hipGetLastError());
-            int e = finish_sum(ctx, nb, 2, S_TT, pub_next(ctx));
+            int e = finish_sum(ctx, nparts, 2, S_TT, pub_next(ctx));
             if (!e) e = wait_scalars(ctx);
             if (!e) {
                 relres = ctx->bnorm > 0 ? sqrt(ctx->h_scal[S_TT]) / ctx->bnorm : 0.;
@@ -2981,13 +3101,6 @@ struct ShiftRec {
     const double *S;
     int gx, gy;
 };
-struct TrueRec {
-    LevArgs L;
-    CgArgs c;
-    const double *p, *q;
-    double *part;
-    int gx, gy;
-};
 
 template <typename T>
 struct LegTabs {
@@ -3039,7 +3152,7 @@ struct EnsPressTabs {
     Batch<CtlRec> ctl;
     Batch<PinRec> pin;
     Batch<ShiftRec> shift;
-    Batch<TrueRec> tres;
+    Batch<TresArgs> tres;
 };
 
 __device__ __forceinline__ bool in_state(const double *S, int need) { return need < 0 || S[S_EST] == (double)need; }
@@ -3195,9 +3308,10 @@ __global__ void kb_shift(const ShiftRec *__restrict__ tab, const int *__restrict
         shift_body(G, R.c, R.p, R.S);
     });
 }
-__global__ void kb_true_residual(const TrueRec *__restrict__ tab, const int *__restrict__ start, int nmem) {
-    batch_for(tab, start, nmem, [&](const auto &R, const VGrid &G) {
-        true_residual_body<false>(G, R.L, R.c, R.p, R.q, R.part);
+__global__ void __launch_bounds__(64 * SW_WAVES) kb_true_rows(const TresArgs *__restrict__ tab,
+                                                               const int *__restrict__ start, int nmem) {
+    batch_for(tab, start, nmem, [&](const TresArgs &R, const VGrid &G) {
+        tres_rows_body(R, G.bx() * SW_WAVES + (int)(threadIdx.x >> 6));
     });
 }
 
@@ -3503,8 +3617,8 @@ int ens_press_build(impec_ensemble *e) {
     std::vector<CtlRec> ct(n);
     std::vector<PinRec> pr(n);
     std::vector<ShiftRec> sh(n);
-    std::vector<TrueRec> tr(n);
-    std::vector<int> u_fa(n), u_red(n), u_vec(n), one(n, 1);
+    std::vector<TresArgs> tr(n);
+    std::vector<int> u_fa(n), u_red(n), u_vec(n), u_tres(n), one(n, 1);
     std::vector<double> init(S_COUNT);
     for (int k = 0; k < n; k++) {
         impec_ctx *ctx = e->m[k];
@@ -3527,7 +3641,9 @@ int ens_press_build(impec_ensemble *e) {
         fi[k] = FinishRec{ctx->part, ctx->scal, nb, 2, (int)S_RR, (int)RED_SUM};
         fd[k] = FinishRec{ctx->part, ctx->scal, dir_mv_args<double>(ctx, a0, c).nwave, 1, (int)S_DQ, (int)RED_SUM};
         frr[k] = FinishRec{ctx->part, ctx->scal, vg.nblocks, 2, (int)S_RR, (int)RED_SUM};
-        ft[k] = FinishRec{ctx->part, ctx->scal, nb, 2, (int)S_TT, (int)RED_SUM};
+        const TresArgs ta = tres_args(a0, c, ctx->p, ctx->q, ctx->part);
+        if (ta.nwave > RED_BLOCKS) return ens_set_error(e, IMPEC_EINVAL, "member %d: grid too wide", k);
+        ft[k] = FinishRec{ctx->part, ctx->scal, ta.nwave, 2, (int)S_TT, (int)RED_SUM};
         const int nwave0 = t->mixed ? fused_args<float>(ctx, 0).nwave : fused_args<double>(ctx, 0).nwave;
         rh[k] = RhoRec{ctx->part, nwave0, 1.0 / (double)c.n, ctx->scal};
         up[k] = Upd2Rec{c, ctx->p, ctx->pd, ctx->pz, ctx->pr, ctx->pq, ctx->scal, ctx->part, (int)vg.grid.x,
@@ -3535,7 +3651,8 @@ int ens_press_build(impec_ensemble *e) {
         ct[k] = CtlRec{ctx->scal, ctx->P.initial_guess == IMPEC_GUESS_ZERO};
         pr[k] = PinRec{c, ctx->p, ctx->scal};
         sh[k] = ShiftRec{c, ctx->p, ctx->scal, nb, 1};
-        tr[k] = TrueRec{a0, c, ctx->p, ctx->q, ctx->part, nb, 1};
+        tr[k] = ta;
+        u_tres[k] = ta.gx;
         // loop constants of solve_cg
         const double tol = ctx->P.eps, bb = ctx->bnorm * ctx->bnorm;
         double v[4];
@@ -3557,7 +3674,7 @@ int ens_press_build(impec_ensemble *e) {
     ENS_CHECK(e, batch_upload(t->ctl, ct, one));
     ENS_CHECK(e, batch_upload(t->pin, pr, one));
     ENS_CHECK(e, batch_upload(t->shift, sh, u_red));
-    ENS_CHECK(e, batch_upload(t->tres, tr, u_red));
+    ENS_CHECK(e, batch_upload(t->tres, tr, u_tres));
     int st = t->mixed ? build_legs<float>(e, t->l32) : build_legs<double>(e, t->l64);
     if (st) return st;
     if ((st = build_lists(e))) return st;
@@ -3702,7 +3819,7 @@ int ens_solve_end(impec_ensemble *e) {
     hipStream_t s = e->stream;
     ENS_CHECK(e, launch_t(kb_read_pin, t->pin, s));
     ENS_CHECK(e, launch_b(kb_shift, t->shift, dim3(256), s));
-    ENS_CHECK(e, launch_b(kb_true_residual, t->tres, dim3(256), s));
+    ENS_CHECK(e, launch_b(kb_true_rows, t->tres, dim3(64 * SW_WAVES), s));
     ENS_CHECK(e, ens_finish(s, t->fin_tt, -1));
     return IMPEC_OK;
 }
