@@ -263,6 +263,7 @@ inline size_t sidx(const impec_ctx *ctx, int i, int j) {
 // ---- field kernels (k_field.hip) ----
 int launch_fill_scalar(impec_ctx *ctx, double *a, int periodic);
 int launch_fill_velocity(impec_ctx *ctx);
+int launch_copy_frame(impec_ctx *ctx, const double *src, double *dst);   // ghost cells of a scalar array
 int launch_mobility(impec_ctx *ctx);
 int cnn_increment(impec_ctx *ctx, double *d, int ldd);
 int launch_velocity(impec_ctx *ctx);

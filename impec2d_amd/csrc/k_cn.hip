@@ -204,7 +204,7 @@ constexpr int S_RHO_NEW = 13;
 #endif
 constexpr int CN_OUT = 62, CNP_OUT = 60, CN_WAVES = IMPEC_CN_WAVES, CNS_WAVES = IMPEC_CNS_WAVES,
               CNZ_WAVES = IMPEC_CNZ_WAVES;
-enum { CN_S = 1, CN_P0 = 2, CN_R = 3 };
+enum { CN_S = 1, CN_P0 = 2, CN_R = 3, CN_RX = 4 };
 // build-time tuning: rows in flight per wavefront, wavefronts per launch
 #ifndef IMPEC_CN_PF
 #define IMPEC_CN_PF 2
@@ -243,6 +243,8 @@ struct CnArgs {
                                            // r0.r != 0 serves BiCGStab; 4 B per read instead of 8)
     double *x, *rout;                      // p-step: the solution (in place) and the new residual
     long ldx;                              // row pitch of x (the interior of the ghosted c)
+    const double *rs;                      // CN_RX: the residual r of the last iteration's s = r - alpha v
+    double *xout;                          // CN_RX: the updated solution (pitch ldx, another buffer)
     double *pout;                          // p-step, CN_P0: new p
     double *out;                           // p-step, CN_P0: new v
     const double *S;
@@ -299,8 +301,12 @@ struct FaceStream {
 };
 
 // CN_S (s = r - alpha v, t = A (s / D), partials only), CN_P0 (p = r,
-// v = A (p / D)) and CN_R (partials of ||r - A x||^2 with r = b; nothing
-// written); lanes 1..62 of a strip are written
+// v = A (p / D)), CN_R (partials of ||r - A x||^2 with r = b; nothing
+// written) and CN_RX (the last iteration's update of x, as k_cn_xfinal forms
+// it, into xout, and the partials of ||b - A x||^2 of the updated x: x is
+// read from its own buffer and the result goes to another, so the halo rows
+// a wave updates again are never ones another wave has already written);
+// lanes 1..62 of a strip are written
 template <int MODE>
 __global__ void __launch_bounds__(64 * CNS_WAVES) k_cn_stream(CnArgs A) {
     const int lane = threadIdx.x & 63;
@@ -315,7 +321,8 @@ __global__ void __launch_bounds__(64 * CNS_WAVES) k_cn_stream(CnArgs A) {
     const int jc = j < 0 ? 0 : (j > N2 - 1 ? N2 - 1 : j);   // cell column for loads
     const bool outl = lane >= 1 && lane <= CN_OUT && jv;
     const int i0 = chunk * A.R, i1 = i0 + A.R < N1 ? i0 + A.R : N1;
-    const double alpha = MODE == CN_S ? A.S[S_ALPHA] : 0.;
+    const double alpha = (MODE == CN_S || MODE == CN_RX) ? A.S[S_ALPHA] : 0.;
+    const double om = MODE == CN_RX ? A.S[S_OMEGA] : 0.;
     // raw row loads from clamped addresses (no lane-dependent branches)
     auto cell = [&](const double *p, int m) -> double {
         const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
@@ -336,6 +343,7 @@ __global__ void __launch_bounds__(64 * CNS_WAVES) k_cn_stream(CnArgs A) {
     // velocities u, v of row m
     constexpr int PF = IMPEC_CN_PF;   // rows in flight; a slot is refilled at the end of the step that used it
     double Pa[PF] = {}, Pv[PF] = {}, P0[PF] = {}, Pu[PF] = {}, Pw[PF] = {};
+    double Pr[PF] = {}, Pp[PF] = {}, Pq[PF] = {};   // CN_RX: r, p, v of the x update
     float Pd[PF] = {};
     auto cellx = [&](const double *p, int m) -> double {   // x: the interior of c, pitch ldx
         const int mc = m < 0 ? 0 : (m > N1 - 1 ? N1 - 1 : m);
@@ -344,11 +352,13 @@ __global__ void __launch_bounds__(64 * CNS_WAVES) k_cn_stream(CnArgs A) {
     auto fetch = [&](int u, int m) {
         Pa[u] = cell(A.r, m);
         if (MODE == CN_S) Pv[u] = cell(A.vin, m);
-        if (MODE == CN_R) {
-            Pv[u] = cellx(A.x, m);
-        } else {
-            Pd[u] = cellf(A.dinv, m);
-            P0[u] = (double)cellf(A.r0, m);
+        if (MODE == CN_R || MODE == CN_RX) Pv[u] = cellx(A.x, m);
+        if (MODE != CN_R) Pd[u] = cellf(A.dinv, m);
+        if (MODE == CN_S || MODE == CN_P0) P0[u] = (double)cellf(A.r0, m);
+        if (MODE == CN_RX) {
+            Pr[u] = cell(A.rs, m);
+            Pp[u] = cell(A.pin, m);
+            Pq[u] = cell(A.vin, m);
         }
         Pu[u] = vel(A.u, m);
         Pw[u] = vel(A.v, m);
@@ -371,10 +381,19 @@ __global__ void __launch_bounds__(64 * CNS_WAVES) k_cn_stream(CnArgs A) {
             const double dv = (double)Pd[u], r0v = P0[u];
             fs.push(k, Pu[u], Pw[u]);
             const bool rin = m >= 0 && m < N1;
-            // the vector on row m: s, the first direction p = r, or the iterate x
-            const double wv = MODE == CN_S ? Pa[u] - alpha * Pv[u] : (MODE == CN_R ? Pv[u] : Pa[u]);
-            const double yv = (rin && jv) ? (MODE == CN_R ? wv : wv * dv) : 0.;
-            if (MODE != CN_R && rin && m >= i0 && m < i1 && outl) {
+            // the vector on row m: s, the first direction p = r, the iterate x,
+            // or x after the last iteration's update (k_cn_xfinal's arithmetic)
+            double wv;
+            if (MODE == CN_RX) {
+                const double sm = Pr[u] - alpha * Pq[u];
+                const double dx_ = alpha * (Pp[u] * dv) + om * (sm * dv);
+                wv = A.xzero ? dx_ : Pv[u] + dx_;
+                if (rin && m >= i0 && m < i1 && outl) A.xout[(long)m * A.ldx + j] = wv;
+            } else {
+                wv = MODE == CN_S ? Pa[u] - alpha * Pv[u] : (MODE == CN_R ? Pv[u] : Pa[u]);
+            }
+            const double yv = (rin && jv) ? ((MODE == CN_R || MODE == CN_RX) ? wv : wv * dv) : 0.;
+            if ((MODE == CN_S || MODE == CN_P0) && rin && m >= i0 && m < i1 && outl) {
                 if (MODE == CN_S) {
                     s3 += r0v * wv;
                     s4 += wv * wv;
@@ -396,7 +415,7 @@ __global__ void __launch_bounds__(64 * CNS_WAVES) k_cn_stream(CnArgs A) {
                         s0 += acc * wprev;
                         s1 += acc * acc;
                         s2 += r0prev * acc;
-                    } else if (MODE == CN_R) {
+                    } else if (MODE == CN_R || MODE == CN_RX) {
                         const double rv = bprev - acc;
                         s0 += rv * rv;
                     } else {
@@ -981,6 +1000,14 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     double rt = 0.;   // true ||b - A x||^2 of the final iterate
     const dim3 vg = cn_vgrid(N1, N2), vblk(64, 4);
     const int nvb = (int)(vg.x * vg.y);
+    // The last iteration's update of x, when its p-launch was not queued
+    // ahead, is left to the true-residual pass (k_cn_stream<CN_RX>), which
+    // writes the updated x into the transport temporary t1; t1 then takes
+    // c's ghost cells and becomes c.  Contexts whose c and t1 addresses are
+    // held elsewhere update x in place (k_cn_xfinal).
+    bool x_pending = false;
+    CnArgs xpend;
+    memset(&xpend, 0, sizeof(xpend));
     for (;;) {
         double rr_mark = rr;   // the recursive residual when it last halved
         int stall = 0;
@@ -1032,7 +1059,10 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
                 stalled = ++stall >= IMPEC_CN_STALL_ITS;
             }
             if (rr <= tol * tol * bb || !(rr == rr) || stalled) {
-                if (!ahead) {
+                if (!ahead && rr == rr && !ctx->fixed_buffers) {
+                    x_pending = true;   // formed by the true-residual pass below
+                    xpend = ap;
+                } else if (!ahead) {
                     prof_begin(ctx);
                     hipLaunchKernelGGL(k_cn_xfinal, cn_vgrid(N1, N2), dim3(64, 4), 0, s, ap);
                     prof_end(ctx, "cn_xfinal");
@@ -1056,15 +1086,29 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         // the recursion's residual (s.s - 2 omega t.s + omega^2 t.t from the
         // s-launch's partials) can cancel at convergence: the true residual of
         // the iterate decides, as the pressure solve's does
+        const char *check_name = x_pending ? "cn_check_x" : "cn_check";
         prof_begin(ctx);
-        {
+        if (x_pending) {
+            CnArgs ar = a;
+            ar.r = b;
+            ar.x = x;
+            ar.rs = xpend.r; ar.pin = xpend.pin; ar.vin = xpend.vin; ar.xzero = xpend.xzero;
+            double *cn = ctx->t1;
+            ar.xout = cn + (long)ctx->B * ctx->lds + ctx->B;
+            hipLaunchKernelGGL(k_cn_stream<CN_RX>, sg, ssblk, 0, s, ar);
+            if ((st = launch_copy_frame(ctx, ctx->c, cn))) return st;
+            ctx->t1 = ctx->c;
+            ctx->c = cn;
+            x = cn + (long)ctx->B * ctx->lds + ctx->B;
+            x_pending = false;
+        } else {
             CnArgs ar = a;
             ar.r = b;
             ar.x = x;
             hipLaunchKernelGGL(k_cn_stream<CN_R>, sg, ssblk, 0, s, ar);
         }
         bicg_finish<2>(s, ctx->part, a.nwave, 6, ctx->scal, pub_next(ctx));
-        prof_end(ctx, "cn_check");
+        prof_end(ctx, check_name);
         IMPEC_HIP_CHECK(ctx, hipGetLastError());
         if ((st = wait_scalars(ctx))) return st;
         rt = ctx->h_scal[S_CNRES];

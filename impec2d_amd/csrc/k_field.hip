@@ -126,6 +126,27 @@ __global__ void __launch_bounds__(FRAME_THREADS) k_fill_velocity(Frame f, double
         fill_velocity_cell(u, v, N1, N2, B, ld, i, j);
 }
 
+// the ghost frame of a ghosted scalar array copied into another of the same
+// layout (the buffer that takes over as c keeps the ghost cells c had)
+__global__ void __launch_bounds__(FRAME_THREADS) k_copy_frame(Frame f, const double *__restrict__ src,
+                                                              double *__restrict__ dst, int B, int ld) {
+    int i, j;
+    if (frame_cell(f, (long)blockIdx.x * FRAME_THREADS + threadIdx.x, i, j)) {
+        const size_t o = (size_t)(i + B) * ld + (j + B);
+        dst[o] = src[o];
+    }
+}
+int launch_copy_frame(impec_ctx *ctx, const double *src, double *dst) {
+    const int N1 = ctx->N1, N2 = ctx->N2, B = ctx->B;
+    const Frame f{-B, N1 + B, -B, N2 + B, 0, N1, 0, N2};
+    const long n = f.cells();
+    if (n == 0) return IMPEC_OK;
+    hipLaunchKernelGGL(k_copy_frame, dim3((unsigned)((n + FRAME_THREADS - 1) / FRAME_THREADS)), dim3(FRAME_THREADS),
+                       0, ctx->stream, f, src, dst, B, ctx->lds);
+    IMPEC_HIP_CHECK(ctx, hipGetLastError());
+    return IMPEC_OK;
+}
+
 int launch_fill_scalar(impec_ctx *ctx, double *a, int periodic) {
     const int N1 = ctx->N1, N2 = ctx->N2, B = ctx->B;
     const Frame f{-B, N1 + B, -B, N2 + B, 0, N1, 0, N2};

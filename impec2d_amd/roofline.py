@@ -47,6 +47,10 @@ KERNELS = {
     # iteration's x update (read x, p, r, v and the FP32 1/D, write x)
     "cn_check": dict(rocprof="k_cn_stream<3>", bytes_per_cell=32),
     "cn_xfinal": dict(rocprof="k_cn_xfinal", bytes_per_cell=44),
+    # the two at once (the last iteration's x update inside the true-residual
+    # pass): read b, x, r, p, v (5 x 8), the FP32 1/D, u and v (2 x 8), write
+    # the updated x (8)
+    "cn_check_x": dict(rocprof="k_cn_stream<4>", bytes_per_cell=68),
     # velocities (read p and w, write u and v), CFL (read u, v), mobility
     # (read c, write w), the zero-guess CG start (read q, write p and r)
     "velocity": dict(rocprof="k_velocity", bytes_per_cell=32),
