@@ -209,6 +209,11 @@ enum { CN_S = 1, CN_P0 = 2, CN_R = 3, CN_RX = 4 };
 #ifndef IMPEC_CN_PF
 #define IMPEC_CN_PF 2
 #endif
+// the true-residual pass with the last x update (k_cn_stream<CN_RX>, seven
+// streams per row)
+#ifndef IMPEC_CNRX_PF
+#define IMPEC_CNRX_PF 2
+#endif
 #ifndef IMPEC_CNP_PF
 #define IMPEC_CNP_PF 3
 #endif
@@ -341,7 +346,8 @@ __global__ void __launch_bounds__(64 * CNS_WAVES) k_cn_stream(CnArgs A) {
     };
     // ring of prefetched rows: r and (CN_S) v of row m, 1/D, r0 and the face
     // velocities u, v of row m
-    constexpr int PF = IMPEC_CN_PF;   // rows in flight; a slot is refilled at the end of the step that used it
+    // rows in flight; a slot is refilled at the end of the step that used it
+    constexpr int PF = MODE == CN_RX ? IMPEC_CNRX_PF : IMPEC_CN_PF;
     double Pa[PF] = {}, Pv[PF] = {}, P0[PF] = {}, Pu[PF] = {}, Pw[PF] = {};
     double Pr[PF] = {}, Pp[PF] = {}, Pq[PF] = {};   // CN_RX: r, p, v of the x update
     float Pd[PF] = {};
