@@ -1250,12 +1250,18 @@ __device__ __forceinline__ void mask_pair(const Lane &L, int m, int n1, V fill, 
 #ifndef IMPEC_SW_PF_F32
 #define IMPEC_SW_PF_F32 4
 #endif
-static_assert(IMPEC_SW_PF % 2 == 0 && IMPEC_SW_PF_F32 % 2 == 0,
+// the FP64 legs of the coarser levels (4 rows in flight measured slower: level
+// 1's down leg 51 us against 44 at 4096^2)
+#ifndef IMPEC_SW_PF_COARSE
+#define IMPEC_SW_PF_COARSE 2
+#endif
+static_assert(IMPEC_SW_PF % 2 == 0 && IMPEC_SW_PF_F32 % 2 == 0 && IMPEC_SW_PF_COARSE % 2 == 0,
               "the unrolled steps take the row parity (colour, prolongation neighbour) from the slot index, so the "
               "prefetch depth must be even");
-template <typename T>
+template <typename T, bool FINE0 = true>
 struct SwPf {
-    static constexpr int value = std::is_same<T, float>::value ? IMPEC_SW_PF_F32 : IMPEC_SW_PF;
+    static constexpr int value =
+        std::is_same<T, float>::value ? IMPEC_SW_PF_F32 : (FINE0 ? IMPEC_SW_PF : IMPEC_SW_PF_COARSE);
 };
 
 template <bool FINE0, bool ANISO, typename T, int NS = 1, bool R32 = false>
@@ -1266,7 +1272,7 @@ __device__ __forceinline__ void mg_down_body(const FusedArgsT<T> &A, const int w
     constexpr int NX = 2 + 2 * NS;   // x and W-face window: rows k-NX+1 .. k
     constexpr int NF = 1 + 2 * NS;   // face and rhs window: rows k-NF+1 .. k
     constexpr int LAG = 2 * NS;      // residual and output row k - LAG
-    constexpr int SW_PF = SwPf<T>::value;
+    constexpr int SW_PF = SwPf<T, FINE0>::value;
     Lane L;
     Chunk K;
     if (!wave_setup<NS>(A, wid, L, K)) return;
@@ -1430,7 +1436,7 @@ __device__ __forceinline__ void mg_up_body(const FusedArgsT<T> &A, const int wid
     constexpr int NX = 2 + 2 * NS;
     constexpr int NF = 1 + 2 * NS;
     constexpr int LAG = 2 * NS;
-    constexpr int SW_PF = SwPf<T>::value;
+    constexpr int SW_PF = SwPf<T, FINE0>::value;
     Lane L;
     Chunk K;
     if (!wave_setup<NS>(A, wid, L, K)) return;
