@@ -214,6 +214,13 @@ enum { CN_S = 1, CN_P0 = 2, CN_R = 3, CN_RX = 4 };
 #ifndef IMPEC_CNRX_PF
 #define IMPEC_CNRX_PF 2
 #endif
+// cache hints of the p-launch: 1 reads and writes x (used again only by the
+// next p-launch) non-temporally, 2 also writes p'.  Both make the next
+// s-launch faster (170-173 against 181 us at 4096^2) and the p-launch slower
+// (316-317 against 303): off
+#ifndef IMPEC_CNP_NT
+#define IMPEC_CNP_NT 0
+#endif
 #ifndef IMPEC_CNP_PF
 #define IMPEC_CNP_PF 3
 #endif
@@ -515,7 +522,14 @@ __global__ void __launch_bounds__(64 * CN_WAVES, IMPEC_CNP_MIN_WAVES) k_cn_pstep
         const int mx = m - 1 < i0 ? i0 : (m - 1 > i1 - 1 ? i1 - 1 : m - 1);
         const int m0 = m - 2 < i0 ? i0 : (m - 2 > i1 - 1 ? i1 - 1 : m - 2);
         Pp[u] = cell(A.pin, mp);
-        if (!A.xzero) Px[u] = cellx(A.x, mx);
+        if (!A.xzero) {
+            if (IMPEC_CNP_NT >= 1) {
+                const int mc = mx < 0 ? 0 : (mx > N1 - 1 ? N1 - 1 : mx);
+                Px[u] = __builtin_nontemporal_load(A.x + (long)mc * A.ldx + jc);
+            } else {
+                Px[u] = cellx(A.x, mx);
+            }
+        }
         P0[u] = (double)cellf(A.r0, m0);
     };
     const int mstart = i0 - 2, mend = i1 + 2;
@@ -549,10 +563,13 @@ __global__ void __launch_bounds__(64 * CN_WAVES, IMPEC_CNP_MIN_WAVES) k_cn_pstep
             const double pq = rq + beta * (Pp[u] - om * vprev);
             if (qin && q >= i0 && q < i1 && outl) {
                 const long o = (long)q * N2 + j;
-                A.pout[o] = pq;
+                if (IMPEC_CNP_NT >= 2) __builtin_nontemporal_store(pq, A.pout + o);
+                else A.pout[o] = pq;
                 A.rout[o] = rq;
                 const double dx_ = alpha * (Pp[u] * dprev) + om * (sprev * dprev);
-                A.x[(long)q * A.ldx + j] = A.xzero ? dx_ : Px[u] + dx_;
+                if (IMPEC_CNP_NT >= 1)
+                    __builtin_nontemporal_store(A.xzero ? dx_ : Px[u] + dx_, A.x + (long)q * A.ldx + j);
+                else A.x[(long)q * A.ldx + j] = A.xzero ? dx_ : Px[u] + dx_;
                 s1 += rq * rq;
             }
             YP0 = YP1; YP1 = YP2; YP2 = (qin && jv) ? pq * dprev : 0.;

@@ -2204,7 +2204,14 @@ __global__ void k_cg_fold32(CgArgs c, double *__restrict__ p, const float *__res
 }
 
 
-// x += alpha d, r -= alpha q; partials (r.r, sum r)
+// x += alpha d, r -= alpha q; partials (r.r, sum r).  IMPEC_CGU_NT: cache
+// hints; 2 reads d, q and p and writes p non-temporally (none of them is read
+// again before the next direction pass), which leaves the Infinity Cache to r,
+// the next V-cycle's input: its first leg 116 against 124 us at 4096^2, the
+// step 0.2% faster; 1 leaves d cached; 0 none
+#ifndef IMPEC_CGU_NT
+#define IMPEC_CGU_NT 2
+#endif
 template <class G>
 __device__ __forceinline__ void cg_update2_body(const G &gr_, CgArgs c, double *__restrict__ p, const double *__restrict__ d, double *__restrict__ r,
                              const double *__restrict__ qv, const double *__restrict__ S, double *__restrict__ part) {
@@ -2217,8 +2224,14 @@ __device__ __forceinline__ void cg_update2_body(const G &gr_, CgArgs c, double *
         double *xp = p + (long)(i + c.B) * c.ldp + (j + c.B);
         double rv = r[o];
         if (alpha != 0.) {   // no step: a broken-down direction may hold non-finite values
+#if IMPEC_CGU_NT
+            const double dv = IMPEC_CGU_NT > 1 ? __builtin_nontemporal_load(d + o) : d[o];
+            __builtin_nontemporal_store(__builtin_nontemporal_load(xp) + alpha * dv, xp);
+            rv = rv - alpha * __builtin_nontemporal_load(qv + o);
+#else
             *xp = *xp + alpha * d[o];
             rv = rv - alpha * qv[o];
+#endif
             r[o] = rv;
         }
         s[0] += rv * rv;
