@@ -256,6 +256,24 @@ int ctx_create(const impec_params *P, hipStream_t ext, impec_ctx **out);
 void prof_begin(impec_ctx *ctx);
 void prof_end(impec_ctx *ctx, const char *name);
 
+// Rows per chunk of a streaming launch over n rows in nstrip strips (one wave
+// per strip and chunk), aimed at `target` waves.  R = ceil(n nstrip / target)
+// leaves ceil(n / R) chunks, which can overshoot the target by up to a row of
+// chunks; when the target is exactly the waves the GPU holds at once, the
+// overshoot waits for free slots and runs as a short round of its own after
+// the rest (the dispersion s-launch at 4096^2: 520 workgroups for 512 slots).
+// `capped` chooses R so that nstrip ceil(n / R) <= target instead.
+#ifndef IMPEC_WAVE_CAP
+#define IMPEC_WAVE_CAP 1
+#endif
+inline long chunk_rows(long n, long nstrip, long target, bool capped) {
+    if (capped && nstrip <= target) {
+        const long per = target / nstrip;   // chunks per strip
+        return (n + per - 1) / per;
+    }
+    return (n * nstrip + target - 1) / target;
+}
+
 inline size_t sidx(const impec_ctx *ctx, int i, int j) {
     return (size_t)(i + ctx->B) * ctx->lds + (j + ctx->B);
 }

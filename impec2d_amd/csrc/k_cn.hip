@@ -938,8 +938,9 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     a.part = ctx->part;
     a.nstrip = (N2 + CN_OUT - 1) / CN_OUT;
     {
-        // about IMPEC_CN_TARGET_WAVES wavefronts; chunks of at least 8 rows
-        long R = ((long)N1 * a.nstrip + IMPEC_CN_TARGET_WAVES - 1) / IMPEC_CN_TARGET_WAVES;
+        // at most IMPEC_CN_TARGET_WAVES wavefronts, the waves the GPU holds at
+        // once at this kernel's 4 waves per SIMD; chunks of at least 8 rows
+        long R = chunk_rows(N1, a.nstrip, IMPEC_CN_TARGET_WAVES, IMPEC_WAVE_CAP);
         if (R < 8) R = 8;
         if (R > N1) R = N1;
         a.R = (int)R;
@@ -949,7 +950,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
     CnArgs a2 = a;
     a2.nstrip = (N2 + CNP_OUT - 1) / CNP_OUT;
     {
-        long R = ((long)N1 * a2.nstrip + IMPEC_CNP_TARGET_WAVES - 1) / IMPEC_CNP_TARGET_WAVES;
+        long R = chunk_rows(N1, a2.nstrip, IMPEC_CNP_TARGET_WAVES, IMPEC_WAVE_CAP);
         if (R < 8) R = 8;
         if (R > N1) R = N1;
         a2.R = (int)R;
@@ -972,7 +973,7 @@ int cn_solve(impec_ctx *ctx, double dt, int *its_out) {
         CnArgs az = a2;
         az.out = vb[1];
         {
-            long R = ((long)N1 * az.nstrip + IMPEC_CNS_TARGET_WAVES - 1) / IMPEC_CNS_TARGET_WAVES;
+            long R = chunk_rows(N1, az.nstrip, IMPEC_CNS_TARGET_WAVES, IMPEC_WAVE_CAP);
             if (R < 8) R = 8;
             if (R > N1) R = N1;
             az.R = (int)R;

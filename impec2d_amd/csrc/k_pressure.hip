@@ -2294,7 +2294,9 @@ static int stream_rows(const impec_ctx *ctx, int l, int n1, int nstrip, bool f32
     const long target = f32 ? (l == 0 ? IMPEC_MG_WAVES0_F32 : IMPEC_MG_WAVES_COARSE_F32)
                             : (l == 0 ? IMPEC_MG_WAVES
                                       : (l == 1 && IMPEC_MG_WAVES_L1 > 0 ? IMPEC_MG_WAVES_L1 : IMPEC_MG_WAVES_COARSE));
-    long r = ((long)n1 * nstrip + target - 1) / target;
+    // the FP64 finest level's target is the waves the GPU holds at once (3 per
+    // SIMD): no more chunks than that, or the excess runs as a round of its own
+    long r = chunk_rows(n1, nstrip, target, IMPEC_WAVE_CAP && l == 0 && !f32);
     r = (r + 1) & ~1L;
     const long minr = n1 <= IMPEC_MG_SMALL_ROWS ? IMPEC_MG_MINROWS : (IMPEC_MG_MINROWS > 4 ? IMPEC_MG_MINROWS : 4);
     if (r < minr) r = minr;
