@@ -261,7 +261,9 @@ void prof_end(impec_ctx *ctx, const char *name);
 // leaves ceil(n / R) chunks, which can overshoot the target by up to a row of
 // chunks; when the target is exactly the waves the GPU holds at once, the
 // overshoot waits for free slots and runs as a short round of its own after
-// the rest (the dispersion s-launch at 4096^2: 520 workgroups for 512 slots).
+// the rest (the dispersion s-launch at 4096^2: 520 workgroups for 512 slots);
+// when the target is a whole number of workgroups per CU, the overshoot gives
+// some CUs one more workgroup than the others, and they finish last.
 // `capped` chooses R so that nstrip ceil(n / R) <= target instead.
 #ifndef IMPEC_WAVE_CAP
 #define IMPEC_WAVE_CAP 1

@@ -2619,7 +2619,9 @@ static DirMvArgs<T> dir_mv_args(impec_ctx *ctx, const LevArgs &a0, const CgArgs 
     A.S = ctx->scal;
     A.part = ctx->part;
     A.nstrip = (c.n2 + DM_OUT - 1) / DM_OUT;
-    long R = ((long)c.n1 * A.nstrip + waves - 1) / waves;
+    // no more chunks than the target: 4096 waves are 1024 workgroups, four
+    // per CU; a few more put a fifth on some CUs, which then finish last
+    long R = chunk_rows(c.n1, A.nstrip, waves, IMPEC_WAVE_CAP);
     if (R < 8) R = 8;
     A.R = (int)R;
     A.nwave = A.nstrip * ((c.n1 + A.R - 1) / A.R);
