@@ -413,7 +413,9 @@ __device__ __forceinline__ void cfl_body(const G &gr_, CflConst k, const double 
     double mx[1] = {0.0}, mn[2] = {DBL_MAX, 1.e10};
     for (long long id = (long long)gr_.bx() * blockDim.x + threadIdx.x; id < n;
          id += (long long)gr_.gx() * blockDim.x) {
-        const int i = (int)(id / k.N2), j = (int)(id - (long long)i * k.N2);
+        // fewer than 2^31 cells (checked at creation): 32-bit index arithmetic
+        // instead of a 64-bit division per cell
+        const int id32 = (int)id, i = id32 / k.N2, j = id32 - i * k.N2;
         const size_t o = (size_t)(i + k.B) * k.ld + (j + k.B);
         const double u0 = u[o], u1 = u[o + k.ld], v0 = v[o], v1 = v[o + 1];
         double uc = fabs(0.5 * (u0 + u1));
@@ -590,7 +592,7 @@ __global__ void k_diag(const double *__restrict__ c, const double *__restrict__ 
     double s[3] = {0., 0., 0.};
     for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n;
          id += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(id / N2), j = (int)(id - (long long)i * N2);
+        const int id32 = (int)id, i = id32 / N2, j = id32 - i * N2;   // fewer than 2^31 cells
         s[0] += c[(size_t)(i + B) * lds + (j + B)];
         const size_t o = (size_t)(i + B) * ldv + (j + B);
         const double div = (v[o + 1] - v[o]) / dy + (u[o + ldv] - u[o]) / dx;
